@@ -1,0 +1,192 @@
+"""Benchmark: particle-steps/s of the electrostatic PIC step, MG Poisson solve
+included, mixed precision vs FP64, on one MI355X.
+
+Workload (default): 2D two-stream instability, 1024 x 1024 periodic nodes,
+100 electrons per cell (104,857,600 particles), beams +-5 v_th, dt = 0.05,
+FP32 particle state with FP32 V-cycles inside FP64 residual refinement to
+||r||/||b|| <= 1e-8.  The same seeded inputs (cast to FP64) run through the
+plain FP64 path for the mixed/FP64 ratio.
+
+One step = deposit (fused into the previous push) + neutralisation + MG
+solve + field energy + gather/push/boundaries + diagnostics, all on the
+device; `value` = N_p * K / (wall time of K steps) for the mixed path.
+
+Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this
+engine runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s incl. MG Poisson, mixed vs FP64 on 1 GPU; adjoint/fwd time"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every):
+    import torch
+    sp = wl["species"]
+    t0 = time.time()
+    st = E.Stepper(wl["grid"], sp, dtype=dtype, mixed_mg=mixed, depth=depth, pre=2, post=1,
+                   max_cycles=40, tol=tol, dt=wl["dt"], rho_bg=wl["rho_bg"],
+                   sort_every=sort_every)
+    log(f"[bench] {dtype}{'+mixed MG' if mixed else ''}: setup {time.time() - t0:.1f}s")
+    if warmup:
+        st.run(warmup, history=False)
+    torch.cuda.synchronize()
+    cyc0 = st.stats()["cycles"]
+    st.set_timing(True)
+    t0 = time.perf_counter()
+    hist = st.run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    push_ms, launches, pushed = st.push_timing()
+    stats = st.stats()
+    n = sum(len(s["pos"][0]) for s in sp)
+    res = dict(wall_s=wall, ms_per_step=1e3 * wall / steps, particle_steps_per_s=n * steps / wall,
+               push_ms_avg=push_ms / max(1, launches), push_launches=launches,
+               particles_per_launch=pushed / max(1, launches),
+               cycles_per_step=(stats["cycles"] - cyc0) / steps,
+               last_rel_res=stats["last_rel_res"], tile_fallback_blocks=stats["tile_fallback_blocks"],
+               hist=hist)
+    st.close()
+    log(f"[bench] {dtype}: {res['ms_per_step']:.3f} ms/step, "
+        f"{res['particle_steps_per_s']:.3e} particle-steps/s, push {res['push_ms_avg']:.3f} ms, "
+        f"{res['cycles_per_step']:.2f} V-cycles/step")
+    return res
+
+
+def cpu_baseline(wl, steps):
+    """The CPU oracle (FP64 restatement of the reference step, OpenMP) on the
+    first `steps` steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    sp = wl["species"][0]
+    g = wl["grid"]
+    og = O.Grid()
+    for f, _ in O.Grid._fields_:
+        v = getattr(g, f)
+        if hasattr(v, "__len__"):
+            getattr(og, f)[:] = list(v)
+        else:
+            setattr(og, f, v)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pos = [a.astype(np.float64) for a in sp["pos"]]
+    vel = [a.astype(np.float64) for a in sp["vel"]]
+    t0 = time.perf_counter()
+    O.pic_run(og, [a.copy() for a in pos[:1]] + [a.copy() for a in pos[1:]], [a.copy() for a in vel],
+              sp["q"], sp["m"], sp["w"], wl["rho_bg"], 1.0, wl["dt"], 0, depth=5, tol=1e-8,
+              max_cycles=40)
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.pic_run(og, pos, vel, sp["q"], sp["m"], sp["w"], wl["rho_bg"], 1.0, wl["dt"], steps, depth=5,
+              tol=1e-8, max_cycles=40)
+    total = time.perf_counter() - t0
+    t = max(1e-9, total - setup)
+    n = len(sp["pos"][0])
+    log(f"[bench] cpu oracle: {steps} steps in {t:.1f}s (+{setup:.1f}s setup), {cores} threads")
+    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"first {steps} steps of the same workload ({n} particles, FP64 oracle "
+                      f"restatement of the reference step, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--ppc", type=int, default=100)
+    ap.add_argument("--depth", type=int, default=6)
+    ap.add_argument("--sort-every", type=int, default=50)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from psum_amd import engine as E
+    from psum_amd.workloads import two_stream_2d
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    t0 = time.time()
+    wl = two_stream_2d(n=args.grid, ppc=args.ppc, dtype=np.float32)
+    n = len(wl["species"][0]["pos"][0])
+    log(f"[bench] workload {wl['name']}: {n} particles, generated in {time.time() - t0:.1f}s")
+
+    mixed = run_engine(E, wl, "f32", True, args.steps, args.warmup, 1e-8, args.depth,
+                       args.sort_every)
+    fp64 = None
+    if not args.no_fp64:
+        wl64 = dict(wl)
+        wl64["species"] = [dict(s, pos=[a.astype(np.float64) for a in s["pos"]],
+                                vel=[a.astype(np.float64) for a in s["vel"]]) for s in wl["species"]]
+        fp64 = run_engine(E, wl64, "f64", False, args.fp64_steps or args.steps, args.warmup, 1e-8,
+                          args.depth, args.sort_every)
+        del wl64
+
+    # dominant kernel: fused gather + push + deposit; algorithmic bytes per
+    # launch = N_p * 2 * (2D) * 4 B (read + write FP32 position and velocity)
+    dim = wl["grid"].dim
+    bytes_per_launch = mixed["particles_per_launch"] * 2 * (2 * dim) * 4
+    achieved = bytes_per_launch / (mixed["push_ms_avg"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_push_deposit<float,2,true,true>",
+                "bytes_per_launch": bytes_per_launch, "avg_launch_ms": mixed["push_ms_avg"]}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(wl, args.cpu_steps)
+
+    fe_err = None
+    if fp64 is not None:
+        h32, h64 = mixed["hist"], fp64["hist"]
+        k = min(len(h32), len(h64), 100)
+        fe_err = float(np.max(np.abs(h32[:k, 0] - h64[:k, 0]) / np.abs(h64[:k, 0])))
+    out = {
+        "metric": METRIC,
+        "value": mixed["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D two-stream, {args.grid}x{args.grid} periodic nodes, "
+                               f"{args.ppc} e-/cell ({n} particles), dt 0.05, FP32 state, "
+                               f"FP32 V-cycles in FP64 refinement to 1e-8",
+                   "grid": [args.grid, args.grid], "particles": n, "mg_depth": args.depth,
+                   "sort_every": args.sort_every, "parallelism": "single GPU"},
+        "fp64": None if fp64 is None else {
+            "value": fp64["particle_steps_per_s"], "ms_per_step": fp64["ms_per_step"],
+            "cycles_per_step": fp64["cycles_per_step"], "push_ms_avg": fp64["push_ms_avg"]},
+        "mixed_over_fp64": None if fp64 is None else
+            mixed["particle_steps_per_s"] / fp64["particle_steps_per_s"],
+        "field_energy_rel_err_first100": fe_err,
+        "mixed_cycles_per_step": mixed["cycles_per_step"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

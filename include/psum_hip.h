@@ -1,0 +1,193 @@
+/*
+ * psum_hip.h -- C ABI of the MI355X (gfx950) electrostatic PIC step engine.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory;
+ * `stream` is a hipStream_t (NULL = the legacy default stream).  Every entry
+ * point returns a psum_status; nothing throws across this boundary.
+ *
+ * Each entry point replaces a reference (PSuM) interface; the citation is
+ * given beside it.  The field-solver plugin ABI that PSuM's Poisson
+ * frontends bind (register_solver / get_solver_funcs and the func_table) is
+ * declared in psum_field_solver.h and implemented by the same library.
+ */
+#ifndef PSUM_HIP_H
+#define PSUM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int psum_status;
+#define PSUM_OK 0
+#define PSUM_ERR_ARG 1          /* bad argument (null pointer, size, dtype) */
+#define PSUM_ERR_HIP 2          /* a HIP runtime call failed */
+#define PSUM_ERR_UNSUPPORTED 3  /* input outside what the engine implements */
+#define PSUM_ERR_NOMEM 4
+#define PSUM_ERR_STATE 5        /* call order violated (e.g. solve before set_matrix) */
+
+#define PSUM_F64 0
+#define PSUM_F32 1
+
+#define PSUM_KEY_INVALID 0xFFFFFFFFu
+
+/* Node grid, reference semantics of simple_grid<D> (src/field/simple_grid.hpp:16-307):
+ * cells[d] cells and cells[d]+1 nodes per dimension, row-major node index
+ * with the last dimension fastest (n2i, :107-112), delta = (ub-lb)/cells and
+ * delta_r = 1/delta computed once (:46-47).  periodic[d] picks the particle
+ * boundary: 1 wraps into [plo, plo+plen) (TSI/main.cpp:71-72), 0 absorbs a
+ * particle leaving [lb, ub) (inGrid, :168-174; validator.hpp:20-22). */
+typedef struct psum_grid {
+    int32_t dim;
+    int32_t cells[3];
+    double lb[3];
+    double ub[3];
+    double delta[3];
+    double delta_r[3];
+    int32_t periodic[3];
+    int32_t pad_;
+    double plo[3];
+    double plen[3];
+} psum_grid;
+
+/* Last error message of the calling thread (static storage). */
+const char* psum_hip_last_error(void);
+
+/* ---- particle container ------------------------------------------------
+ * Particles are SoA: pos[d][i], vel[d][i] for d < dim, all of one dtype.
+ * Invalid particles have non-finite pos[0][i] (pos_x_nan_is_invalid,
+ * src/particle_container/validator.hpp:18-26). */
+
+/* Cell binning: keys[i] = n2i of the wrapped base node cN(x_i)
+ * (simple_grid.hpp:89-95,107-112,338-343), PSUM_KEY_INVALID for invalid
+ * particles.  The index arithmetic is FP64 for both dtypes.
+ * No reference counterpart (the reference never bins); it feeds the sort. */
+psum_status psum_hip_bin(const psum_grid* g, int dtype, void* const* pos, int64_t n,
+                         uint32_t* keys, void* stream);
+
+/* Bytes of device workspace psum_hip_sort needs for n keys. */
+size_t psum_hip_sort_workspace(int64_t n);
+
+/* Stable LSD radix sort of keys in [0, nkeys) plus PSUM_KEY_INVALID (last).
+ * perm[r] = index of the r-th particle; keys_out = sorted keys.
+ * No reference counterpart; defines the sort order the engine uses. */
+psum_status psum_hip_sort(const uint32_t* keys, int64_t n, uint32_t nkeys, uint32_t* keys_out,
+                          uint32_t* perm, void* work, size_t work_bytes, void* stream);
+
+/* dst[c][i] = src[c][perm[i]] for c < ncomp arrays of `dtype`. */
+psum_status psum_hip_permute(int dtype, int ncomp, void* const* src, void* const* dst,
+                             const uint32_t* perm, int64_t n, void* stream);
+
+/* Stable stream compaction of valid particles into out_pos/out_vel;
+ * *count_host receives the survivor count.  Replaces
+ * particle_group::compress (particle_container/particle_group.hpp:223-238),
+ * whose atomic push_back order is nondeterministic; here the order is the
+ * original index order. */
+size_t psum_hip_compact_workspace(int64_t n);
+psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
+                             void* const* out_pos, void* const* out_vel, int64_t* count_host,
+                             void* work, size_t work_bytes, void* stream);
+
+/* ---- field: CIC deposit and gather ------------------------------------
+ * Deposit (add_back, src/field/device_field.hpp:295-310 ->
+ * field_interp.hpp:63-75 -> atomic_add.hpp:34-48): for every valid
+ * particle and every support node with |coef| > 1e-10,
+ *     acc[node] += llrint((w * coef) * scale)          (int64, exact)
+ * scale must be a power of two.  Integer accumulation makes the result
+ * independent of particle order and thread schedule; no float atomics.
+ * acc is accumulated into (zero it first).  counters (device, may be NULL):
+ * [0] particles whose support was out of range, [1] blocks that fell back
+ * from the LDS tile to global integer atomics. */
+psum_status psum_hip_deposit_cic(const psum_grid* g, int dtype, void* const* pos, int64_t n,
+                                 double w, double scale, int64_t* acc, int64_t* counters,
+                                 void* stream);
+
+/* rho[i] = (double)acc[i] * inv_scale + offset */
+psum_status psum_hip_fixed_to_double(const int64_t* acc, int64_t nn, double inv_scale,
+                                     double offset, double* rho, void* stream);
+
+/* Gather + push (the push lambda of TSI/main.cpp:59-75 and
+ * big_bang/stage_6.cpp:48-65 with interp_diff, device_field.hpp:270-284):
+ *     E = -grad(phi)(x)   (gradient of the multilinear interpolant, terms
+ *                          with |coef| <= 1e-10 skipped, field_interp.hpp:108)
+ *     v = v + (E*qm)*dt ;  x = x + v*dt
+ * then periodic wrap / absorbing wall per psum_grid.  phi has the particle
+ * dtype.  When acc != NULL the new positions are also deposited as in
+ * psum_hip_deposit_cic with (w_dep, scale) -- the next step's charge.
+ * diag (device, may be NULL) receives per-block partial sums of
+ * 1/2 m|v|^2 and m v_d over the velocities entering the step; reduce them
+ * with psum_hip_reduce_diag.  counters (device, may be NULL): [0] support
+ * out of range, [1] tile fallbacks, [2] particles absorbed this call. */
+int64_t psum_hip_push_blocks(int64_t n);
+psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos,
+                                 void* const* vel, int64_t n, const void* phi, double qm,
+                                 double dt, double mass, double w_dep, double scale, int64_t* acc,
+                                 double* diag, int64_t* counters, void* stream);
+
+/* out[k] = sum over blocks of diag[4*b + k], k < 4, fixed order (deterministic). */
+psum_status psum_hip_reduce_diag(const double* diag, int64_t nblocks, double* out, void* stream);
+
+/* Field energy 1/2 eps sum_d sum_edges ((phi[n+e_d]-phi[n]) * delta_r_d)^2 * dV
+ * (TSI/main.cpp:138-141 in D dimensions; periodic dims include the wrap
+ * edge).  out: 4 doubles on the device, out[0] = energy (out[1..3] scratch);
+ * work: device workspace of psum_hip_field_energy_workspace(g) bytes. */
+size_t psum_hip_field_energy_workspace(const psum_grid* g);
+psum_status psum_hip_field_energy(const psum_grid* g, const double* phi, double eps, double* out,
+                                  void* work, size_t work_bytes, void* stream);
+
+/* Element-wise precision conversions on the device. */
+psum_status psum_hip_f64_to_f32(const double* in, float* out, int64_t n, void* stream);
+
+/* ---- step engine -------------------------------------------------------
+ * One step of the reference user loop (TSI/main.cpp:121-146): deposit ->
+ * Poisson_solver::solve through the multigrid backend -> gather + push ->
+ * boundaries, with sort every `sort_every` steps.  Particle state is
+ * resident on the device; nothing crosses PCIe inside psum_stepper_run. */
+typedef struct psum_species {
+    double q, m, w;          /* charge, mass, macro-particle weight */
+    int64_t n;               /* particle count */
+    int64_t capacity;        /* allocated slots (>= n) */
+} psum_species;
+
+typedef struct psum_step_config {
+    psum_grid grid;
+    int32_t dtype;           /* particle state: PSUM_F64 or PSUM_F32 */
+    int32_t mixed_mg;        /* 0: FP64 V-cycles; 1: FP32 V-cycles inside FP64 refinement */
+    int32_t nspecies;        /* 1..4 */
+    int32_t depth, pre, post, max_cycles;
+    int32_t sort_every;      /* 0 = never re-sort */
+    double tol;              /* relative residual target ||b-A phi|| / ||b|| */
+    double eps, dt, rho_bg;
+    psum_species species[4];
+    const uint8_t* fixed;    /* host mask of Dirichlet nodes (NULL = none); all-periodic
+                                grids pin node 0 */
+    const double* fixed_val; /* host values for fixed nodes (NULL = 0) */
+} psum_step_config;
+
+void* psum_stepper_create(const psum_step_config* cfg);
+/* copy species s from host (d < dim arrays of n particles, dtype of cfg) */
+psum_status psum_stepper_load(void* h, int s, const void* const* pos, const void* const* vel,
+                              int64_t n);
+/* device pointers of species s state (for zero-copy wrapping) */
+psum_status psum_stepper_state(void* h, int s, void** pos, void** vel, int64_t* n);
+/* run `steps` steps; hist (host, may be NULL) gets 5 doubles per step:
+ * field energy, kinetic energy, momentum x, y, z, as in orc_pic_run_f64 */
+psum_status psum_stepper_run(void* h, int steps, double* hist);
+/* copy phi (FP64, host) */
+psum_status psum_stepper_phi(void* h, double* phi_host);
+/* total V-cycles so far, last relative residual */
+psum_status psum_stepper_stats(void* h, int64_t* cycles, double* last_rel_res, int64_t* counters);
+psum_status psum_stepper_sync(void* h);
+/* HIP-event timing of the fused gather/push/deposit launches on the engine's
+ * stream (reset by set_timing): total ms, launches, particles pushed. */
+psum_status psum_stepper_set_timing(void* h, int on);
+psum_status psum_stepper_push_timing(void* h, double* ms, int64_t* launches, int64_t* particles);
+void psum_stepper_destroy(void* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

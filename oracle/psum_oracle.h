@@ -1,0 +1,160 @@
+/*
+ * psum_oracle.h -- CPU restatement of PSuM's electrostatic PIC step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product in psum_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in the product path
+ * calls it.
+ *
+ * Every function restates a reference algorithm; the citation beside each
+ * declaration names the reference file:line it follows (paths relative to
+ * the PSuM source tree).  The arithmetic order is spelled out in the .c
+ * files because several checks are bit-exact.
+ *
+ * Parity pins (how each function is checked, see tests/test_oracle_*.py):
+ *   orc_cell_keys_*        closed form floor((x-lb)*delta_r) on edge cases,
+ *                          brute force vs numpy on random points
+ *   orc_stable_sort        numpy argsort(kind="stable")
+ *   orc_cic_weights        reference KATs: partition of unity, range [0,1],
+ *                          one-hot at nodes, linearity inside a cell
+ *                          (test/field/test_simpleinterp.cpp:13-230)
+ *   orc_cic_grad_weights   central FD of the interpolant (h=1e-7, mean err
+ *                          < 1e-6; test/field/test_field_interpDiff.cpp:15-148)
+ *   orc_deposit_*          charge conservation; transpose identity
+ *                          <deposit(w), phi> = sum_p w interp(phi, x_p);
+ *                          fixed-point result equals the FP64 sum to 2^-S
+ *   orc_gather_push_*      single particle in a uniform field (closed form);
+ *                          Langmuir oscillation frequency
+ *   orc_compact_*          numpy boolean mask
+ *   orc_poisson_coo        closed-form 5/7-point coefficients; DFT
+ *                          eigenvalues of the periodic Laplacian
+ *   orc_mg_*               exact discrete periodic solution via FFT,
+ *                          manufactured sin*sin solution, >=1e5 residual
+ *                          reduction in 10 V(2,1) cycles on 256^2
+ *                          (test/field_solver/test_multigrid_core.cpp:142-209)
+ *   orc_field_energy etc.  sum-by-parts identity 1/2 sum phi*rho = 1/2 eps |grad phi|^2
+ */
+#ifndef PSUM_ORACLE_H
+#define PSUM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Node grid with reference semantics (src/field/simple_grid.hpp:38-51):
+ * cells[d] cells, cells[d]+1 nodes per dimension, delta = (ub-lb)/cells,
+ * delta_r = 1/delta.  periodic[d] selects the particle boundary in that
+ * dimension: 1 = wrap into [plo, plo+plen), 0 = absorb outside [lb, ub).
+ * Layout is identical to psum_grid in include/psum_hip.h. */
+typedef struct {
+    int32_t dim;
+    int32_t cells[3];
+    double lb[3];
+    double ub[3];
+    double delta[3];
+    double delta_r[3];
+    int32_t periodic[3];
+    int32_t pad_;
+    double plo[3];
+    double plen[3];
+} orc_grid;
+
+#define ORC_KEY_INVALID 0xFFFFFFFFu
+
+/* a1: cell key = n2i(wrapped cN(x)) (simple_grid.hpp:89-95,107-112,338-343).
+ * Invalid (non-finite x[0]) or out-of-range particles get ORC_KEY_INVALID. */
+void orc_cell_keys_f64(const orc_grid* g, const double* const* pos, int64_t n, uint32_t* keys);
+void orc_cell_keys_f32(const orc_grid* g, const float* const* pos, int64_t n, uint32_t* keys);
+
+/* Stable counting sort of keys: perm[r] = index of the r-th particle.
+ * ORC_KEY_INVALID sorts last.  Keys must be < nkeys otherwise. */
+void orc_stable_sort(const uint32_t* keys, int64_t n, uint32_t nkeys, uint32_t* perm);
+
+/* a2/a3: CIC weights for one point (simple_grid.hpp:316-349,351-385;
+ * simple_interpolation.hpp:33-98).  idx[2^D], coef[2^D]; gcoef[d*8+i].
+ * Returns 0 when the support is out of range. */
+int orc_cic_weights(const orc_grid* g, const double* x, int64_t* idx, double* coef);
+int orc_cic_grad_weights(const orc_grid* g, const double* x, int64_t* idx, double* gcoef);
+
+/* total node count of the grid */
+int64_t orc_num_nodes(const orc_grid* g);
+
+/* a4: deposit.  Fixed-point form (the build's deterministic deposit):
+ * acc[node] += llrint((w*coef) * scale) for |coef| > 1e-10
+ * (field_interp.hpp:63-75, atomic_add.hpp:34-48 with the FP64 atomic
+ * replaced by exact integer accumulation).  scale must be a power of two. */
+int64_t orc_deposit_fx_f64(const orc_grid* g, const double* const* pos, int64_t n,
+                           double w, double scale, int64_t* acc);
+int64_t orc_deposit_fx_f32(const orc_grid* g, const float* const* pos, int64_t n,
+                           double w, double scale, int64_t* acc);
+/* Reference arithmetic: rho[node] += w*coef in FP64, particle order. */
+int64_t orc_deposit_ref_f64(const orc_grid* g, const double* const* pos, int64_t n,
+                            double w, double* rho);
+
+/* a5/a6: gather grad(phi) at x, kick then drift, boundary.
+ * (field_interp.hpp:93-113; TSI/main.cpp:59-75; big_bang/stage_6.cpp:48-65;
+ * validator.hpp:18-26).  diag[0] = sum 1/2 m |v|^2 over valid particles,
+ * diag[1..3] = sum m v_d (velocities read before the kick).
+ * Returns the number of particles invalidated by an absorbing wall. */
+int64_t orc_gather_push_f64(const orc_grid* g, double** pos, double** vel, int64_t n,
+                            const double* phi, double qm, double dt, double mass, double* diag);
+int64_t orc_gather_push_f32(const orc_grid* g, float** pos, float** vel, int64_t n,
+                            const float* phi, float qm, float dt, double mass, double* diag);
+
+/* a7: stable compaction of valid particles (finite pos[0]) in index order. */
+int64_t orc_compact_f64(int dim, double* const* pos, double* const* vel, int64_t n,
+                        double** out_pos, double** out_vel);
+int64_t orc_compact_f32(int dim, float* const* pos, float* const* vel, int64_t n,
+                        float** out_pos, float** out_vel);
+
+/* a8: Cartesian FD Poisson assembly A = -div(eps grad) with identity rows at
+ * fixed nodes (Poisson_solver_2d.hpp:182-410, Poisson_solver_3d.hpp:178-431).
+ * Output in the column-major walk the frontend hands to set_matrix
+ * (Poisson_solver_2d.hpp:387-404).  Call with rows==NULL to get nnz. */
+int64_t orc_poisson_coo(const orc_grid* g, double eps, const uint8_t* fixed,
+                        uint64_t* rows, uint64_t* cols, double* vals);
+
+/* a9-a16: geometric-transfer Galerkin multigrid restated with CSR matrices
+ * (multigrid_utilities.hpp:55-104, multigrid_core.hpp:49-272,
+ *  sycl_multigrid_gpu.cpp:93-132, sycl_multigrid32f_gpu.cpp:106-160). */
+void* orc_mg_create(int64_t n, int64_t nnz, const uint64_t* rows, const uint64_t* cols,
+                    const double* vals, int I, int J, int K, int depth, int pre, int post);
+int orc_mg_levels(void* h, int64_t* sizes);
+/* x is the warm start and the result; runs exactly `cycles` V-cycles. */
+void orc_mg_vcycles(void* h, const double* b, double* x, int cycles);
+/* 32f backend: FP64 residual, FP32 V-cycles in batches of corr_interval,
+ * FP64 accumulation; FP32 coarse inverse as in sycl_inner_solver<float>. */
+void orc_mg_solve_mixed(void* h, const double* b, double* x, int max_cycles, int corr_interval);
+/* ||b - A x||_2 with the unscaled fine matrix. */
+double orc_mg_residual_norm(void* h, const double* b, const double* x);
+void orc_mg_free(void* h);
+
+/* Field energy 1/2 eps sum_edges (dphi/dx_d)^2 * cell volume over the node
+ * grid edges (TSI/main.cpp:138-141 generalised to D dimensions); periodic
+ * dimensions include the wrap edge. */
+double orc_field_energy(const orc_grid* g, const double* phi, double eps);
+
+/* Whole-step driver used for history parity and for the CPU baseline
+ * (the user loop of TSI/main.cpp:121-146 and big_bang/stage_6.cpp:125-147):
+ * per step
+ *   rho  = deposit(x, q*w/dV) + rho_bg            (fixed point when dep_mode=0,
+ *                                                  FP64 particle-order sum when 1)
+ *   all-periodic grids: rho -= mean(rho) (FP64) and node 0 is pinned to 0
+ *   phi  = MG V-cycles, warm started, until ||b-A phi||/||b|| <= tol
+ *          (checked after every cycle, at most max_cycles)
+ *   hist[5*s + {0..4}] = field energy, kinetic energy, momentum x, y, z
+ *   gather + kick + drift + boundary.
+ * fixed (nodes mask) may be NULL for all-periodic grids.  Returns the
+ * total number of V-cycles run. */
+int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n,
+                        double q, double m, double w, double rho_bg, double eps, double dt,
+                        int steps, int depth, int pre, int post, double tol, int max_cycles,
+                        int dep_mode, const uint8_t* fixed, double* hist, double* phi_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1357 @@
+// Geometric-transfer Galerkin multigrid for the FD Poisson problem on
+// MI355X, served through PSuM's field-solver plugin ABI.
+//
+// Algorithm (the reference's, restated; PSuM paths):
+//   transfers  P = tensor product of 1-D linear interpolation on index axes
+//              that keep every other node and the last one; R = P^T scaled
+//              by fine/coarse lumped volumes
+//              (field_solver/implements/multigrid/multigrid_utilities.hpp:28-104)
+//   setup      A0^ = D0^-1 A, A(l+1)^ = D(l+1)^-1 (R A(l)^ P), relax = I - 2/3 A^,
+//              dense inverse of the coarsest A^      (multigrid_core.hpp:49-150)
+//   V-cycle    weighted Jacobi pre/post smoothing, residual, restrict + D^-1,
+//              prolong-add, coarse dense solve        (multigrid_core.hpp:163-272)
+//   backends   sycl_multigrid_gpu.cpp:22-230 (FP64) and
+//              sycl_multigrid32f_gpu.cpp:23-266 (FP32 V-cycles inside an FP64
+//              residual refinement loop; here the coarse solve stays FP64)
+//
+// MI355X form: the matrices never exist on the device.  Every level is a
+// structured node grid (n0 x n1 [x n2], last dimension fastest) whose rows
+// are 3^D-point stencils; rows with identical coefficients share a class,
+// so a level is a uint16 class id per node plus a small coefficient table.
+// Level 0 of a 5/7-point matrix uses the compact (axis-only) stencil kernel.
+// Transfers are per-axis weight tables (<= 3 fine nodes per coarse node and
+// <= 2 coarse nodes per fine node along each axis).
+
+#include <algorithm>
+#include <array>
+#include <type_traits>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+#include "psum_field_solver.h"
+
+using namespace psum_hip;
+
+namespace {
+
+constexpr int kMgThreads = 256;
+
+#define MG_HIP(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw std::runtime_error(std::string("hip_multigrid: ") + #expr + ": " +       \
+                                     hipGetErrorString(e_));                                \
+    } while (0)
+
+template <typename F>
+void parallel_for(int64_t n, F&& f)
+{
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const int64_t b = (int64_t)t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([b, e, &f]() {
+            for (int64_t i = b; i < e; ++i) f(i);
+        });
+    }
+    for (auto& t : th) t.join();
+}
+
+// ------------------------------------------------------------ device data
+
+struct DevLevel {
+    int n[3];
+    int wrap[3];
+    int64_t N;
+    const uint16_t* cls;
+};
+
+template <typename T>
+struct DevTable {
+    const T* coef;    // ncls * KS
+    const T* dinv;    // ncls
+};
+
+template <typename T>
+struct DevTransfer {
+    const int* pc[3];     // fine f -> lower coarse index
+    const T* pw[3];       // fine f -> weights (2 per f)
+    const int* rf[3];     // coarse c -> first fine index
+    const int* rn[3];     // coarse c -> number of fine nodes (1..3)
+    const T* rw[3];       // coarse c -> weights (3 per c)
+};
+
+template <int D>
+__device__ __forceinline__ void node_coords(const DevLevel& L, int64_t i, int* c)
+{
+    if (D == 3) {
+        c[2] = (int)(i % L.n[2]);
+        i /= L.n[2];
+    }
+    c[1] = (int)(i % L.n[1]);
+    c[0] = (int)(i / L.n[1]);
+}
+
+__device__ __forceinline__ int nb_index(int c, int o, int n, int wrap)
+{
+    int q = c + o;
+    if (q < 0) q = wrap ? q + n : 0;
+    else if (q >= n) q = wrap ? q - n : n - 1;
+    return q;
+}
+
+// sum_k coef[k] * x[nb_k], k ascending over the 3^D offsets (o0 slowest),
+// COMPACT: only the 2D+1 axis offsets (all other coefficients are zero).
+template <typename T, int D, bool COMPACT>
+__device__ __forceinline__ T stencil_apply(const DevLevel& L, const T* __restrict__ coef,
+                                          const T* __restrict__ x, const int* c)
+{
+    T s = (T)0;
+    if (D == 2) {
+        const int n0 = L.n[0], n1 = L.n[1];
+#pragma unroll
+        for (int a = -1; a <= 1; ++a)
+#pragma unroll
+            for (int b = -1; b <= 1; ++b) {
+                if (COMPACT && a != 0 && b != 0) continue;
+                const int k = (a + 1) * 3 + (b + 1);
+                const int q0 = nb_index(c[0], a, n0, L.wrap[0]);
+                const int q1 = nb_index(c[1], b, n1, L.wrap[1]);
+                s = s + coef[k] * x[(int64_t)q0 * n1 + q1];
+            }
+    } else {
+        const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+#pragma unroll
+        for (int a = -1; a <= 1; ++a)
+#pragma unroll
+            for (int b = -1; b <= 1; ++b)
+#pragma unroll
+                for (int e = -1; e <= 1; ++e) {
+                    if (COMPACT && ((a != 0) + (b != 0) + (e != 0)) > 1) continue;
+                    const int k = (a + 1) * 9 + (b + 1) * 3 + (e + 1);
+                    const int q0 = nb_index(c[0], a, n0, L.wrap[0]);
+                    const int q1 = nb_index(c[1], b, n1, L.wrap[1]);
+                    const int q2 = nb_index(c[2], e, n2, L.wrap[2]);
+                    s = s + coef[k] * x[((int64_t)q0 * n1 + q1) * n2 + q2];
+                }
+    }
+    return s;
+}
+
+// x_out = relax x_in + b * (2/3)   (multiply_add(relax, x, b, buf, 2/3, 1))
+template <typename T, int D, bool COMPACT>
+__global__ __launch_bounds__(kMgThreads) void k_smooth(DevLevel L, DevTable<T> relax,
+                                                       const T* __restrict__ xin,
+                                                       const T* __restrict__ b,
+                                                       T* __restrict__ xout, T two3)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i >= L.N) return;
+    int c[3];
+    node_coords<D>(L, i, c);
+    const T* coef = relax.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
+    const T t = stencil_apply<T, D, COMPACT>(L, coef, xin, c);
+    xout[i] = t * (T)1 + b[i] * (two3 * (T)1);
+}
+
+// r = (A^ x)*(-1) + b*((-1)*(-1))   (multiply_add(A, x, b, buf, -1, -1))
+template <typename T, int D, bool COMPACT>
+__global__ __launch_bounds__(kMgThreads) void k_residual(DevLevel L, DevTable<T> A,
+                                                         const T* __restrict__ x,
+                                                         const T* __restrict__ b,
+                                                         T* __restrict__ r)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i >= L.N) return;
+    int c[3];
+    node_coords<D>(L, i, c);
+    const T* coef = A.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
+    const T t = stencil_apply<T, D, COMPACT>(L, coef, x, c);
+    r[i] = t * (T)-1 + b[i] * (T)1;
+}
+
+// b_c = (R r) o D_c^-1 ; x_c = 0
+template <typename T, int D>
+__global__ __launch_bounds__(kMgThreads) void k_restrict(DevLevel F, DevLevel Cl,
+                                                         DevTransfer<T> tr, DevTable<T> Ac,
+                                                         const T* __restrict__ r,
+                                                         T* __restrict__ bc, T* __restrict__ xc)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i >= Cl.N) return;
+    int c[3];
+    node_coords<D>(Cl, i, c);
+    T s = (T)0;
+    const int f0 = tr.rf[0][c[0]], m0 = tr.rn[0][c[0]];
+    const int f1 = tr.rf[1][c[1]], m1 = tr.rn[1][c[1]];
+    if (D == 2) {
+        for (int a = 0; a < m0; ++a) {
+            const T w0 = tr.rw[0][3 * c[0] + a];
+            for (int b = 0; b < m1; ++b) {
+                const T w = w0 * tr.rw[1][3 * c[1] + b];
+                s = s + w * r[(int64_t)(f0 + a) * F.n[1] + (f1 + b)];
+            }
+        }
+    } else {
+        const int f2 = tr.rf[2][c[2]], m2 = tr.rn[2][c[2]];
+        for (int a = 0; a < m0; ++a) {
+            const T w0 = tr.rw[0][3 * c[0] + a];
+            for (int b = 0; b < m1; ++b) {
+                const T w01 = w0 * tr.rw[1][3 * c[1] + b];
+                const int64_t row = ((int64_t)(f0 + a) * F.n[1] + (f1 + b)) * F.n[2];
+                for (int e = 0; e < m2; ++e) {
+                    const T w = w01 * tr.rw[2][3 * c[2] + e];
+                    s = s + w * r[row + f2 + e];
+                }
+            }
+        }
+    }
+    bc[i] = s * Ac.dinv[Cl.cls[i]];
+    xc[i] = (T)0;
+}
+
+// x_f = (P x_c)*1 + x_f*(1*1)
+template <typename T, int D>
+__global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel Cl,
+                                                            DevTransfer<T> tr,
+                                                            const T* __restrict__ xc,
+                                                            T* __restrict__ xf)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i >= F.N) return;
+    int c[3];
+    node_coords<D>(F, i, c);
+    T s = (T)0;
+    const int p0 = tr.pc[0][c[0]], p1 = tr.pc[1][c[1]];
+    const T* w0 = tr.pw[0] + 2 * c[0];
+    const T* w1 = tr.pw[1] + 2 * c[1];
+    if (D == 2) {
+        for (int a = 0; a < 2; ++a) {
+            if (w0[a] == (T)0) continue;
+            for (int b = 0; b < 2; ++b) {
+                if (w1[b] == (T)0) continue;
+                s = s + (w0[a] * w1[b]) * xc[(int64_t)(p0 + a) * Cl.n[1] + (p1 + b)];
+            }
+        }
+    } else {
+        const int p2 = tr.pc[2][c[2]];
+        const T* w2 = tr.pw[2] + 2 * c[2];
+        for (int a = 0; a < 2; ++a) {
+            if (w0[a] == (T)0) continue;
+            for (int b = 0; b < 2; ++b) {
+                if (w1[b] == (T)0) continue;
+                const T w01 = w0[a] * w1[b];
+                for (int e = 0; e < 2; ++e) {
+                    if (w2[e] == (T)0) continue;
+                    s = s + (w01 * w2[e]) *
+                                xc[((int64_t)(p0 + a) * Cl.n[1] + (p1 + b)) * Cl.n[2] + (p2 + e)];
+                }
+            }
+        }
+    }
+    xf[i] = s * (T)1 + xf[i] * (T)1;
+}
+
+// b0 = b o D0^-1  (hadamard_to)
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_hadamard(DevLevel L, DevTable<T> A,
+                                                         const T* __restrict__ b,
+                                                         T* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i < L.N) out[i] = b[i] * A.dinv[L.cls[i]];
+}
+
+// x = Ainv b, one wave per row, fixed-order lane partials + butterfly.
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_coarse_solve(const double* __restrict__ inv, int n,
+                                                             const T* __restrict__ b,
+                                                             T* __restrict__ x)
+{
+    const int row = (int)(((int64_t)blockIdx.x * kMgThreads + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += inv[(int64_t)row * n + j] * (double)b[j];
+    s = wave_sum(s);
+    if (lane == 0) x[row] = (T)s;
+}
+
+// FP64 residual of the unscaled operator: r = b - A x, partial sums of r^2
+// and of b^2 per block; when OUTF32 also emit (float)r o D0^-1 (the mixed
+// backend's residual_to + hadamard_to).
+template <int D, bool COMPACT, bool OUTF32>
+__global__ __launch_bounds__(kMgThreads) void k_outer_residual(
+    DevLevel L, DevTable<double> K, const double* __restrict__ x, const double* __restrict__ b,
+    const float* __restrict__ dinv32, float* __restrict__ b0, double* __restrict__ part)
+{
+    __shared__ double ws[kMgThreads / 64][2];
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    double rr = 0.0, bb = 0.0;
+    if (i < L.N) {
+        int c[3];
+        node_coords<D>(L, i, c);
+        const double* coef = K.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
+        const double ax = stencil_apply<double, D, COMPACT>(L, coef, x, c);
+        const double r = b[i] - ax;
+        rr = r * r;
+        bb = b[i] * b[i];
+        if (OUTF32) b0[i] = (float)r * dinv32[L.cls[i]];
+    }
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if ((threadIdx.x & 63) == 0) {
+        ws[threadIdx.x >> 6][0] = rr;
+        ws[threadIdx.x >> 6][1] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double s = 0.0;
+        for (int w = 0; w < kMgThreads / 64; ++w) s += ws[w][threadIdx.x];
+        part[2 * (int64_t)blockIdx.x + threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_reduce2(const double* __restrict__ part, int64_t nb,
+                                                        double* __restrict__ out)
+{
+    __shared__ double sh[2][kMgThreads];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += kMgThreads) {
+        s0 += part[2 * b];
+        s1 += part[2 * b + 1];
+    }
+    sh[0][threadIdx.x] = s0;
+    sh[1][threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = kMgThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_zero_f32(float* __restrict__ x, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i < n) x[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_add_correction(double* __restrict__ x,
+                                                               const float* __restrict__ e,
+                                                               int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i < n) x[i] += (double)e[i];
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_addback(double* __restrict__ b,
+                                                        const unsigned long long* __restrict__ idx,
+                                                        const double* __restrict__ v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i < n) b[idx[i]] += v[i];
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_replace(double* __restrict__ b,
+                                                        const unsigned long long* __restrict__ idx,
+                                                        const double* __restrict__ v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i < n) b[idx[i]] = v[i];
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kMgThreads - 1) / kMgThreads); }
+
+// ------------------------------------------------------------ host levels
+
+struct Axis {
+    std::vector<double> x;
+};
+
+Axis coarsen(const Axis& f)
+{
+    Axis c;
+    for (size_t i = 0; i < f.x.size(); i += 2) c.x.push_back(f.x[i]);
+    if (c.x.back() != f.x.back()) c.x.push_back(f.x.back());
+    return c;
+}
+
+std::vector<double> volumes(const Axis& a)
+{
+    std::vector<double> w(a.x.size(), 1.0);
+    if (a.x.size() == 1) return w;
+    for (size_t i = 0; i < a.x.size(); ++i)
+        w[i] = ((i ? a.x[i] - a.x[i - 1] : 0.0) + (i + 1 < a.x.size() ? a.x[i + 1] - a.x[i] : 0.0)) * 0.5;
+    return w;
+}
+
+struct AxisTransfer {
+    std::vector<int> pc;        // per fine f
+    std::vector<double> pw;     // 2 per fine f (w_lo, w_hi); exact node: (1, 0)
+    std::vector<int> rf, rn;    // per coarse c
+    std::vector<double> rw;     // 3 per coarse c
+};
+
+AxisTransfer axis_transfer(const Axis& f, const Axis& c)
+{
+    AxisTransfer t;
+    const int nf = (int)f.x.size(), nc = (int)c.x.size();
+    t.pc.resize(nf);
+    t.pw.resize(2 * nf);
+    for (int i = 0; i < nf; ++i) {
+        const double x = f.x[i];
+        int hi = (int)(std::lower_bound(c.x.begin(), c.x.end(), x) - c.x.begin());
+        if (hi >= nc) throw std::runtime_error("hip_multigrid: point outside coarse grid");
+        if (c.x[hi] == x) {
+            t.pc[i] = hi;
+            t.pw[2 * i] = 1.0;
+            t.pw[2 * i + 1] = 0.0;
+        } else {
+            if (hi == 0) throw std::runtime_error("hip_multigrid: point outside coarse grid");
+            const double tt = (x - c.x[hi - 1]) / (c.x[hi] - c.x[hi - 1]);
+            t.pc[i] = hi - 1;
+            t.pw[2 * i] = 1.0 - tt;
+            t.pw[2 * i + 1] = tt;
+        }
+    }
+    const std::vector<double> vol = volumes(f);
+    std::vector<double> cvol(nc, 0.0);
+    for (int i = 0; i < nf; ++i) {
+        cvol[t.pc[i]] += t.pw[2 * i] * vol[i];
+        if (t.pw[2 * i + 1] != 0.0) cvol[t.pc[i] + 1] += t.pw[2 * i + 1] * vol[i];
+    }
+    t.rf.assign(nc, -1);
+    t.rn.assign(nc, 0);
+    t.rw.assign(3 * nc, 0.0);
+    for (int i = 0; i < nf; ++i)
+        for (int h = 0; h < 2; ++h) {
+            const double p = t.pw[2 * i + h];
+            if (p == 0.0) continue;
+            const int cc = t.pc[i] + h;
+            if (t.rf[cc] < 0) t.rf[cc] = i;
+            const int k = i - t.rf[cc];
+            if (k < 0 || k > 2) throw std::runtime_error("hip_multigrid: restriction support > 3");
+            t.rw[3 * cc + k] = p * (vol[i] / cvol[cc]);
+            t.rn[cc] = std::max(t.rn[cc], k + 1);
+        }
+    return t;
+}
+
+struct RowKey {
+    const double* v;
+    int ks;
+};
+
+struct HostLevel {
+    int D = 2;
+    int n[3] = {1, 1, 1};
+    int wrap[3] = {0, 0, 0};
+    int64_t N = 0;
+    int KS = 9;
+    bool compact = false;
+    std::vector<uint16_t> cls;
+    std::vector<double> K;     // unscaled class rows
+    std::vector<double> dinv;  // per class
+    std::vector<double> Ah;    // scaled rows
+    std::vector<double> Rx;    // relax rows
+};
+
+// Dedupe rows (bitwise) into classes.  rows: N * KS doubles.
+void classify(HostLevel& L, const std::vector<double>& rows)
+{
+    const int KS = L.KS;
+    std::unordered_map<std::string, int> map;
+    L.cls.resize(L.N);
+    L.K.clear();
+    for (int64_t i = 0; i < L.N; ++i) {
+        std::string key(reinterpret_cast<const char*>(&rows[i * KS]), sizeof(double) * KS);
+        auto it = map.find(key);
+        int id;
+        if (it == map.end()) {
+            id = (int)map.size();
+            if (id >= 65535) throw std::runtime_error("hip_multigrid: more than 65535 distinct stencil rows");
+            map.emplace(std::move(key), id);
+            L.K.insert(L.K.end(), rows.begin() + i * KS, rows.begin() + (i + 1) * KS);
+        } else {
+            id = it->second;
+        }
+        L.cls[i] = (uint16_t)id;
+    }
+    const int ncls = (int)map.size();
+    const int center = (KS - 1) / 2;
+    L.dinv.resize(ncls);
+    L.Ah.resize((size_t)ncls * KS);
+    L.Rx.resize((size_t)ncls * KS);
+    L.compact = true;
+    for (int c = 0; c < ncls; ++c) {
+        const double diag = L.K[(size_t)c * KS + center];
+        L.dinv[c] = 1 / diag;
+        for (int k = 0; k < KS; ++k) {
+            const double a = L.K[(size_t)c * KS + k] * L.dinv[c];
+            L.Ah[(size_t)c * KS + k] = a;
+            const double s = a * 2.0 / 3;
+            L.Rx[(size_t)c * KS + k] = (k == center) ? 1.0 - s : 0.0 - s;
+            if (a != 0.0) {
+                int nz = 0;
+                int kk = k;
+                for (int d = 0; d < L.D; ++d) {
+                    if (kk % 3 != 1) ++nz;
+                    kk /= 3;
+                }
+                if (nz > 1) L.compact = false;
+            }
+        }
+    }
+}
+
+// offset of column (cc) from row (rc) along an axis of n nodes; wrapped
+// offsets flag the axis periodic.  Returns false if not a 3-point neighbour.
+bool axis_offset(int rc, int cc, int n, int& off, int& wrapped)
+{
+    const int d = cc - rc;
+    if (d >= -1 && d <= 1) {
+        off = d;
+        return true;
+    }
+    if (d == n - 1) {
+        off = -1;
+        wrapped = 1;
+        return true;
+    }
+    if (d == -(n - 1)) {
+        off = 1;
+        wrapped = 1;
+        return true;
+    }
+    return false;
+}
+
+HostLevel level_from_coo(int I, int J, int K, unsigned long long n_row, unsigned long long nnz,
+                         const unsigned long long* rows, const unsigned long long* cols,
+                         const double* vals)
+{
+    HostLevel L;
+    L.D = K > 1 ? 3 : 2;
+    L.n[0] = I;
+    L.n[1] = J;
+    L.n[2] = K;
+    L.N = (int64_t)I * J * K;
+    L.KS = L.D == 2 ? 9 : 27;
+    if ((unsigned long long)L.N != n_row)
+        throw std::runtime_error("hip_multigrid: I*J*K does not match the matrix size");
+    std::vector<double> R((size_t)L.N * L.KS, 0.0);
+    int wrapped[3] = {0, 0, 0};
+    for (unsigned long long q = 0; q < nnz; ++q) {
+        const unsigned long long r = rows[q], c = cols[q];
+        if (r >= n_row || c >= n_row) throw std::runtime_error("hip_multigrid: index out of range");
+        int rc[3], cc[3];
+        unsigned long long a = r, b = c;
+        rc[2] = (int)(a % K); a /= K; rc[1] = (int)(a % J); rc[0] = (int)(a / J);
+        cc[2] = (int)(b % K); b /= K; cc[1] = (int)(b % J); cc[0] = (int)(b / J);
+        int off[3] = {0, 0, 0};
+        const int dims[3] = {I, J, K};
+        for (int d = 0; d < 3; ++d)
+            if (!axis_offset(rc[d], cc[d], dims[d], off[d], wrapped[d]))
+                throw std::runtime_error("hip_multigrid: matrix is not a 3^D-point stencil on the I x J x K node grid");
+        int k;
+        if (L.D == 2) k = (off[0] + 1) * 3 + (off[1] + 1);
+        else k = (off[0] + 1) * 9 + (off[1] + 1) * 3 + (off[2] + 1);
+        R[(size_t)r * L.KS + k] += vals[q];
+    }
+    for (int d = 0; d < 3; ++d) L.wrap[d] = wrapped[d];
+    classify(L, R);
+    return L;
+}
+
+// Galerkin coarse level: rows of R A^ P with A^ the scaled fine operator.
+HostLevel galerkin(const HostLevel& F, const AxisTransfer* tr, const int* nc)
+{
+    HostLevel C;
+    C.D = F.D;
+    C.KS = F.KS;
+    for (int d = 0; d < 3; ++d) {
+        C.n[d] = nc[d];
+        C.wrap[d] = F.wrap[d];
+    }
+    C.N = (int64_t)nc[0] * nc[1] * nc[2];
+    const int D = F.D, KS = F.KS;
+    std::vector<double> rows((size_t)C.N * KS, 0.0);
+    std::atomic<int> bad{0};
+    parallel_for(C.N, [&](int64_t ci) {
+        int c[3];
+        int64_t t = ci;
+        c[2] = (int)(t % nc[2]); t /= nc[2]; c[1] = (int)(t % nc[1]); c[0] = (int)(t / nc[1]);
+        double* out = &rows[(size_t)ci * KS];
+        const int m0 = tr[0].rn[c[0]], m1 = tr[1].rn[c[1]], m2 = D == 3 ? tr[2].rn[c[2]] : 1;
+        for (int a = 0; a < m0; ++a)
+            for (int b = 0; b < m1; ++b)
+                for (int e = 0; e < m2; ++e) {
+                    const int f[3] = {tr[0].rf[c[0]] + a, tr[1].rf[c[1]] + b, D == 3 ? tr[2].rf[c[2]] + e : 0};
+                    double rw = tr[0].rw[3 * c[0] + a] * tr[1].rw[3 * c[1] + b];
+                    if (D == 3) rw = rw * tr[2].rw[3 * c[2] + e];
+                    const int64_t fi = ((int64_t)f[0] * F.n[1] + f[1]) * F.n[2] + f[2];
+                    const double* arow = &F.Ah[(size_t)F.cls[fi] * KS];
+                    for (int k = 0; k < KS; ++k) {
+                        const double av = arow[k];
+                        if (av == 0.0) continue;
+                        int o[3] = {0, 0, 0};
+                        if (D == 2) { o[0] = k / 3 - 1; o[1] = k % 3 - 1; }
+                        else { o[0] = k / 9 - 1; o[1] = (k / 3) % 3 - 1; o[2] = k % 3 - 1; }
+                        int g[3];
+                        for (int d = 0; d < 3; ++d) {
+                            g[d] = f[d] + o[d];
+                            if (g[d] < 0) g[d] += F.n[d];
+                            else if (g[d] >= F.n[d]) g[d] -= F.n[d];
+                        }
+                        const double ra = rw * av;
+                        const int hm2 = D == 3 ? 2 : 1;
+                        for (int h0 = 0; h0 < 2; ++h0) {
+                            const double p0 = tr[0].pw[2 * g[0] + h0];
+                            if (p0 == 0.0) continue;
+                            for (int h1 = 0; h1 < 2; ++h1) {
+                                const double p1 = tr[1].pw[2 * g[1] + h1];
+                                if (p1 == 0.0) continue;
+                                for (int h2 = 0; h2 < hm2; ++h2) {
+                                    double p = p0 * p1;
+                                    int cc[3] = {tr[0].pc[g[0]] + h0, tr[1].pc[g[1]] + h1, 0};
+                                    if (D == 3) {
+                                        const double p2 = tr[2].pw[2 * g[2] + h2];
+                                        if (p2 == 0.0) continue;
+                                        p = p * p2;
+                                        cc[2] = tr[2].pc[g[2]] + h2;
+                                    }
+                                    int oo[3] = {0, 0, 0};
+                                    for (int d = 0; d < D; ++d) {
+                                        int dd = cc[d] - c[d];
+                                        if (C.wrap[d]) {
+                                            if (dd > 1) dd -= nc[d];
+                                            else if (dd < -1) dd += nc[d];
+                                        }
+                                        if (dd < -1 || dd > 1) { bad = 1; dd = 0; }
+                                        oo[d] = dd;
+                                    }
+                                    const int kk = D == 2 ? (oo[0] + 1) * 3 + (oo[1] + 1)
+                                                          : (oo[0] + 1) * 9 + (oo[1] + 1) * 3 + (oo[2] + 1);
+                                    out[kk] += ra * p;
+                                }
+                            }
+                        }
+                    }
+                }
+    });
+    if (bad) throw std::runtime_error("hip_multigrid: Galerkin operator wider than 3 nodes");
+    classify(C, rows);
+    return C;
+}
+
+void dense_inverse(const HostLevel& L, std::vector<double>& inv)
+{
+    const int64_t n = L.N;
+    std::vector<double> m((size_t)(n * n), 0.0);
+    inv.assign((size_t)(n * n), 0.0);
+    const int D = L.D, KS = L.KS;
+    for (int64_t i = 0; i < n; ++i) {
+        int c[3];
+        int64_t t = i;
+        c[2] = (int)(t % L.n[2]); t /= L.n[2]; c[1] = (int)(t % L.n[1]); c[0] = (int)(t / L.n[1]);
+        const double* row = &L.Ah[(size_t)L.cls[i] * KS];
+        for (int k = 0; k < KS; ++k) {
+            if (row[k] == 0.0) continue;
+            int o[3] = {0, 0, 0};
+            if (D == 2) { o[0] = k / 3 - 1; o[1] = k % 3 - 1; }
+            else { o[0] = k / 9 - 1; o[1] = (k / 3) % 3 - 1; o[2] = k % 3 - 1; }
+            int g[3];
+            for (int d = 0; d < 3; ++d) {
+                g[d] = c[d] + o[d];
+                if (g[d] < 0) g[d] += L.n[d];
+                else if (g[d] >= L.n[d]) g[d] -= L.n[d];
+            }
+            const int64_t j = ((int64_t)g[0] * L.n[1] + g[1]) * L.n[2] + g[2];
+            m[(size_t)(i * n + j)] += row[k];
+        }
+        inv[(size_t)(i * n + i)] = 1.0;
+    }
+    for (int64_t col = 0; col < n; ++col) {
+        int64_t p = col;
+        for (int64_t r = col + 1; r < n; ++r)
+            if (std::fabs(m[(size_t)(r * n + col)]) > std::fabs(m[(size_t)(p * n + col)])) p = r;
+        if (m[(size_t)(p * n + col)] == 0.0) throw std::runtime_error("hip_multigrid: singular coarsest operator");
+        if (p != col)
+            for (int64_t j = 0; j < n; ++j) {
+                std::swap(m[(size_t)(col * n + j)], m[(size_t)(p * n + j)]);
+                std::swap(inv[(size_t)(col * n + j)], inv[(size_t)(p * n + j)]);
+            }
+        const double d = m[(size_t)(col * n + col)];
+        for (int64_t j = 0; j < n; ++j) {
+            m[(size_t)(col * n + j)] /= d;
+            inv[(size_t)(col * n + j)] /= d;
+        }
+        parallel_for(n, [&](int64_t r) {
+            if (r == col) return;
+            const double f = m[(size_t)(r * n + col)];
+            if (f == 0.0) return;
+            for (int64_t j = 0; j < n; ++j) {
+                m[(size_t)(r * n + j)] -= f * m[(size_t)(col * n + j)];
+                inv[(size_t)(r * n + j)] -= f * inv[(size_t)(col * n + j)];
+            }
+        });
+    }
+}
+
+// ------------------------------------------------------------ device buffers
+
+template <typename T>
+T* dev_copy(const std::vector<T>& v)
+{
+    T* p = nullptr;
+    MG_HIP(hipMalloc(&p, std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) MG_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+template <typename T>
+std::vector<T> cast_vec(const std::vector<double>& v)
+{
+    return std::vector<T>(v.begin(), v.end());
+}
+
+struct DevLevelBuf {
+    uint16_t* cls = nullptr;
+    double *K = nullptr, *Ah = nullptr, *Rx = nullptr, *dinv = nullptr;
+    float *Ah32 = nullptr, *Rx32 = nullptr, *dinv32 = nullptr;
+    // vectors: FP64 x/b/t and FP32 x/b/t
+    double *x = nullptr, *b = nullptr, *t = nullptr;
+    float *xf = nullptr, *bf = nullptr, *tf = nullptr;
+};
+
+struct DevTransferBuf {
+    int *pc[3] = {nullptr, nullptr, nullptr}, *rf[3] = {nullptr, nullptr, nullptr},
+        *rn[3] = {nullptr, nullptr, nullptr};
+    double *pw[3] = {nullptr, nullptr, nullptr}, *rw[3] = {nullptr, nullptr, nullptr};
+    float *pw32[3] = {nullptr, nullptr, nullptr}, *rw32[3] = {nullptr, nullptr, nullptr};
+};
+
+}  // namespace
+
+// ================================================================= solver
+
+namespace psum_hip {
+
+class HipMultigrid {
+public:
+    explicit HipMultigrid(bool mixed) : mixed_(mixed) {}
+    ~HipMultigrid() { release(); }
+
+    // options (sycl_multigrid_gpu.cpp:134-162, sycl_multigrid32f_gpu.cpp:162-196)
+    void set_options(const char* options)
+    {
+        std::istringstream iss(options ? options : "");
+        std::string key;
+        bool k_provided = false;
+        while (iss >> key) {
+            if (key == "I") iss >> I_;
+            else if (key == "J") iss >> J_;
+            else if (key == "K") { iss >> K_; k_provided = true; }
+            else if (key == "depth") iss >> depth_;
+            else if (key == "pre_relax") iss >> pre_;
+            else if (key == "post_relax") iss >> post_;
+            else if (key == "max_cycles") iss >> max_cycles_;
+            else if (key == "correction_interval") {
+                int v = 0;
+                if (!(iss >> v) || v < 1) throw std::invalid_argument("correction_interval must be positive");
+                corr_ = v;
+            } else if (key == "tol") iss >> tol_;
+            else if (key == "num_conv_mask" || key == "num_conv_mask_relax" ||
+                     key == "num_conv_mask_restrict" || key == "num_conv_mask_interpolate" ||
+                     key == "num_conv_mask_a") {
+                int ignored;
+                iss >> ignored;
+            }
+        }
+        if (!k_provided) K_ = 1;
+    }
+
+    void set_stream(hipStream_t s) { stream_ = s; }
+
+    void set_matrix(unsigned long long n_row, unsigned long long n_col, unsigned long long nnz,
+                    const unsigned long long* rows, const unsigned long long* cols, const double* vals)
+    {
+        if (n_row != n_col) throw std::runtime_error("hip_multigrid: matrix must be square");
+        if (I_ < 1 || J_ < 1 || K_ < 1 || depth_ < 1)
+            throw std::invalid_argument("multigrid: positive grid dimensions and depth required");
+        release();
+        std::vector<HostLevel> L;
+        L.push_back(level_from_coo(I_, J_, K_, n_row, nnz, rows, cols, vals));
+        std::vector<std::array<AxisTransfer, 3>> T;
+        Axis ax[3];
+        const int sz[3] = {I_, J_, K_};
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < sz[d]; ++i) ax[d].x.push_back(i);
+        for (int level = 0; level < depth_; ++level) {
+            Axis co[3];
+            int nc[3];
+            bool same = true;
+            for (int d = 0; d < 3; ++d) {
+                co[d] = coarsen(ax[d]);
+                nc[d] = (int)co[d].x.size();
+                if (co[d].x.size() != ax[d].x.size()) same = false;
+            }
+            if (same) break;   // endpoints cannot be coarsened further
+            std::array<AxisTransfer, 3> tr;
+            for (int d = 0; d < 3; ++d) tr[d] = axis_transfer(ax[d], co[d]);
+            L.push_back(galerkin(L.back(), tr.data(), nc));
+            T.push_back(tr);
+            for (int d = 0; d < 3; ++d) ax[d] = co[d];
+        }
+        if (T.empty()) throw std::invalid_argument("multigrid: grid cannot be coarsened while retaining endpoints");
+        if (L.back().N > 8192)
+            throw std::runtime_error("hip_multigrid: coarse matrix too large (" + std::to_string(L.back().N) +
+                                     " > 8192); increase depth");
+        dense_inverse(L.back(), cinv_host_);
+        host_ = std::move(L);
+        tr_host_ = std::move(T);
+        upload();
+    }
+
+    void set_source_replace(unsigned long long n, const unsigned long long* idx, const double* v)
+    {
+        // unique indices; the last value of a repeated index wins
+        std::map<unsigned long long, double> m;
+        for (unsigned long long i = 0; i < n; ++i) m[idx[i]] = v[i];
+        upload_pairs(m, rep_idx_, rep_val_, rep_n_);
+    }
+
+    void set_source_addback(unsigned long long n, const unsigned long long* idx, const double* v)
+    {
+        // repeated indices accumulate (in argument order) before the device add
+        std::map<unsigned long long, double> m;
+        for (unsigned long long i = 0; i < n; ++i) m[idx[i]] += v[i];
+        upload_pairs(m, add_idx_, add_val_, add_n_);
+    }
+
+    // solve(b, x): device pointers, b modified in place (addback, replace)
+    void solve(double* b, double* x, bool sync = true)
+    {
+        if (host_.empty()) throw std::runtime_error("hip_multigrid: solve before set_matrix");
+        hipStream_t st = stream_;
+        if (add_n_) hipLaunchKernelGGL(k_addback, dim3(nblk(add_n_)), dim3(kMgThreads), 0, st, b, add_idx_, add_val_, add_n_);
+        if (rep_n_) hipLaunchKernelGGL(k_replace, dim3(nblk(rep_n_)), dim3(kMgThreads), 0, st, b, rep_idx_, rep_val_, rep_n_);
+        MG_HIP(hipGetLastError());
+        last_cycles_ = 0;
+        last_rel_ = NAN;
+        const bool use_tol = tol_ > 0.0;
+        if (!mixed_) {
+            if (!use_tol) {
+                vcycles_f64(b, x, max_cycles_);
+                last_cycles_ = max_cycles_;
+            } else {
+                double rr, bb;
+                for (int c = 0; c < max_cycles_; ++c) {
+                    vcycles_f64(b, x, 1);
+                    ++last_cycles_;
+                    outer_residual(b, x, false, &rr, &bb);
+                    last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+                    if (last_rel_ <= tol_) break;
+                }
+            }
+        } else {
+            for (int cycle = 0; cycle < max_cycles_;) {
+                const int batch = use_tol ? 1 : std::min(corr_, max_cycles_ - cycle);
+                double rr, bb;
+                outer_residual(b, x, true, &rr, &bb);   // b0 (float) = (float)(b - A x) o D^-1
+                if (use_tol) {
+                    last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+                    if (last_rel_ <= tol_) break;
+                }
+                vcycles_f32(batch);
+                hipLaunchKernelGGL(k_add_correction, dim3(nblk(N0())), dim3(kMgThreads), 0, st, x, dev_[0].xf, N0());
+                MG_HIP(hipGetLastError());
+                cycle += batch;
+                last_cycles_ += batch;
+            }
+            if (use_tol && !(last_rel_ <= tol_)) {
+                double rr, bb;
+                outer_residual(b, x, false, &rr, &bb);
+                last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+            }
+        }
+        if (sync) MG_HIP(hipStreamSynchronize(st));
+    }
+
+    int last_cycles() const { return last_cycles_; }
+    double last_rel() const { return last_rel_; }
+    int64_t N0() const { return host_.empty() ? 0 : host_[0].N; }
+    int levels() const { return (int)host_.size(); }
+
+private:
+    bool mixed_;
+    int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
+    double tol_ = 0.0;
+    hipStream_t stream_ = nullptr;
+    std::vector<HostLevel> host_;
+    std::vector<std::array<AxisTransfer, 3>> tr_host_;
+    std::vector<double> cinv_host_;
+    std::vector<DevLevelBuf> dev_;
+    std::vector<DevTransferBuf> dtr_;
+    double* cinv_ = nullptr;
+    double* part_ = nullptr;
+    double* red_ = nullptr;
+    double* red_host_ = nullptr;
+    unsigned long long *rep_idx_ = nullptr, *add_idx_ = nullptr;
+    double *rep_val_ = nullptr, *add_val_ = nullptr;
+    int64_t rep_n_ = 0, add_n_ = 0, rep_cap_ = 0, add_cap_ = 0;
+    int last_cycles_ = 0;
+    double last_rel_ = NAN;
+
+    void upload_pairs(const std::map<unsigned long long, double>& m, unsigned long long*& di,
+                      double*& dv, int64_t& n)
+    {
+        std::vector<unsigned long long> hi;
+        std::vector<double> hv;
+        for (auto& kv : m) {
+            if (!host_.empty() && kv.first >= (unsigned long long)host_[0].N)
+                throw std::runtime_error("hip_multigrid: source index out of range");
+            hi.push_back(kv.first);
+            hv.push_back(kv.second);
+        }
+        if (di) (void)hipFree(di);
+        if (dv) (void)hipFree(dv);
+        di = nullptr;
+        dv = nullptr;
+        n = (int64_t)hi.size();
+        if (n) {
+            di = dev_copy(hi);
+            dv = dev_copy(hv);
+        }
+    }
+
+    void release()
+    {
+        for (auto& d : dev_) {
+            for (void* p : {(void*)d.cls, (void*)d.K, (void*)d.Ah, (void*)d.Rx, (void*)d.dinv,
+                            (void*)d.Ah32, (void*)d.Rx32, (void*)d.dinv32, (void*)d.t, (void*)d.bf,
+                            (void*)d.xf, (void*)d.tf, (void*)d.b, (void*)d.x})
+                if (p) (void)hipFree(p);
+        }
+        dev_.clear();
+        for (auto& t : dtr_)
+            for (int d = 0; d < 3; ++d)
+                for (void* p : {(void*)t.pc[d], (void*)t.rf[d], (void*)t.rn[d], (void*)t.pw[d],
+                                (void*)t.rw[d], (void*)t.pw32[d], (void*)t.rw32[d]})
+                    if (p) (void)hipFree(p);
+        dtr_.clear();
+        if (cinv_) (void)hipFree(cinv_);
+        if (part_) (void)hipFree(part_);
+        if (red_) (void)hipFree(red_);
+        if (red_host_) (void)hipHostFree(red_host_);
+        cinv_ = nullptr;
+        part_ = nullptr;
+        red_ = nullptr;
+        red_host_ = nullptr;
+    }
+
+    void upload()
+    {
+        const int nl = (int)host_.size();
+        dev_.resize(nl);
+        for (int l = 0; l < nl; ++l) {
+            const HostLevel& h = host_[l];
+            DevLevelBuf& d = dev_[l];
+            d.cls = dev_copy(h.cls);
+            d.K = dev_copy(h.K);
+            d.Ah = dev_copy(h.Ah);
+            d.Rx = dev_copy(h.Rx);
+            d.dinv = dev_copy(h.dinv);
+            d.Ah32 = dev_copy(cast_vec<float>(h.Ah));
+            d.Rx32 = dev_copy(cast_vec<float>(h.Rx));
+            d.dinv32 = dev_copy(cast_vec<float>(h.dinv));
+            const size_t N = (size_t)h.N;
+            if (!mixed_) {
+                if (l > 0) MG_HIP(hipMalloc(&d.x, N * sizeof(double)));
+                if (l > 0) MG_HIP(hipMalloc(&d.b, N * sizeof(double)));
+                else MG_HIP(hipMalloc(&d.b, N * sizeof(double)));
+                MG_HIP(hipMalloc(&d.t, N * sizeof(double)));
+            } else {
+                MG_HIP(hipMalloc(&d.xf, N * sizeof(float)));
+                MG_HIP(hipMalloc(&d.bf, N * sizeof(float)));
+                MG_HIP(hipMalloc(&d.tf, N * sizeof(float)));
+            }
+        }
+        dtr_.resize(tr_host_.size());
+        for (size_t l = 0; l < tr_host_.size(); ++l)
+            for (int d = 0; d < 3; ++d) {
+                const AxisTransfer& t = tr_host_[l][d];
+                dtr_[l].pc[d] = dev_copy(t.pc);
+                dtr_[l].rf[d] = dev_copy(t.rf);
+                dtr_[l].rn[d] = dev_copy(t.rn);
+                dtr_[l].pw[d] = dev_copy(t.pw);
+                dtr_[l].rw[d] = dev_copy(t.rw);
+                dtr_[l].pw32[d] = dev_copy(cast_vec<float>(t.pw));
+                dtr_[l].rw32[d] = dev_copy(cast_vec<float>(t.rw));
+            }
+        cinv_ = dev_copy(cinv_host_);
+        const int64_t nb = (int64_t)nblk(host_[0].N);
+        MG_HIP(hipMalloc(&part_, (size_t)nb * 2 * sizeof(double)));
+        MG_HIP(hipMalloc(&red_, 2 * sizeof(double)));
+        MG_HIP(hipHostMalloc(&red_host_, 2 * sizeof(double), hipHostMallocDefault));
+    }
+
+    DevLevel dl(int l) const
+    {
+        DevLevel d;
+        for (int q = 0; q < 3; ++q) {
+            d.n[q] = host_[l].n[q];
+            d.wrap[q] = host_[l].wrap[q];
+        }
+        d.N = host_[l].N;
+        d.cls = dev_[l].cls;
+        return d;
+    }
+
+    template <typename T>
+    DevTable<T> table(int l, bool relax) const
+    {
+        DevTable<T> t;
+        if constexpr (std::is_same<T, double>::value) {
+            t.coef = relax ? dev_[l].Rx : dev_[l].Ah;
+            t.dinv = dev_[l].dinv;
+        } else {
+            t.coef = relax ? dev_[l].Rx32 : dev_[l].Ah32;
+            t.dinv = dev_[l].dinv32;
+        }
+        return t;
+    }
+
+    template <typename T>
+    DevTransfer<T> transfer(int l) const
+    {
+        DevTransfer<T> t;
+        for (int d = 0; d < 3; ++d) {
+            t.pc[d] = dtr_[l].pc[d];
+            t.rf[d] = dtr_[l].rf[d];
+            t.rn[d] = dtr_[l].rn[d];
+            if constexpr (std::is_same<T, double>::value) {
+                t.pw[d] = dtr_[l].pw[d];
+                t.rw[d] = dtr_[l].rw[d];
+            } else {
+                t.pw[d] = dtr_[l].pw32[d];
+                t.rw[d] = dtr_[l].rw32[d];
+            }
+        }
+        return t;
+    }
+
+    template <typename T>
+    void smooth(int l, T*& x, T*& buf, const T* b)
+    {
+        const DevLevel L = dl(l);
+        const T two3 = (T)(2.0 / 3);
+        const unsigned nb = nblk(L.N);
+        if (host_[l].D == 2) {
+            if (host_[l].compact) hipLaunchKernelGGL((k_smooth<T, 2, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
+            else hipLaunchKernelGGL((k_smooth<T, 2, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
+        } else {
+            if (host_[l].compact) hipLaunchKernelGGL((k_smooth<T, 3, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
+            else hipLaunchKernelGGL((k_smooth<T, 3, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
+        }
+        std::swap(x, buf);
+    }
+
+    template <typename T>
+    void residual(int l, const T* x, const T* b, T* r)
+    {
+        const DevLevel L = dl(l);
+        const unsigned nb = nblk(L.N);
+        if (host_[l].D == 2) {
+            if (host_[l].compact) hipLaunchKernelGGL((k_residual<T, 2, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, false), x, b, r);
+            else hipLaunchKernelGGL((k_residual<T, 2, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, false), x, b, r);
+        } else {
+            if (host_[l].compact) hipLaunchKernelGGL((k_residual<T, 3, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, false), x, b, r);
+            else hipLaunchKernelGGL((k_residual<T, 3, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, false), x, b, r);
+        }
+    }
+
+    // One or more V-cycles on level-0 arrays (b0 scaled, x0 warm start).
+    // xs[l] / ts[l] are the ping-pong pair of level l, bs[l] its rhs.
+    template <typename T>
+    void vcycles(std::vector<T*>& xs, std::vector<T*>& ts, std::vector<T*>& bs, int cycles)
+    {
+        const int nl = (int)host_.size();
+        const int Lc = nl - 1;
+        for (int cyc = 0; cyc < cycles; ++cyc) {
+            for (int l = 0; l < Lc; ++l) {
+                for (int s = 0; s < pre_; ++s) smooth<T>(l, xs[l], ts[l], bs[l]);
+                residual<T>(l, xs[l], bs[l], ts[l]);
+                const DevLevel F = dl(l), C = dl(l + 1);
+                if (host_[l].D == 2)
+                    hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1]);
+                else
+                    hipLaunchKernelGGL((k_restrict<T, 3>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1]);
+            }
+            {
+                const int n = (int)host_[Lc].N;
+                const unsigned nb = (unsigned)(((int64_t)n * 64 + kMgThreads - 1) / kMgThreads);
+                hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nb), dim3(kMgThreads), 0, stream_, cinv_, n, bs[Lc], xs[Lc]);
+            }
+            for (int l = Lc - 1; l >= 0; --l) {
+                const DevLevel F = dl(l), C = dl(l + 1);
+                if (host_[l].D == 2)
+                    hipLaunchKernelGGL((k_prolong_add<T, 2>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xs[l + 1], xs[l]);
+                else
+                    hipLaunchKernelGGL((k_prolong_add<T, 3>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xs[l + 1], xs[l]);
+                for (int s = 0; s < post_; ++s) smooth<T>(l, xs[l], ts[l], bs[l]);
+            }
+        }
+        MG_HIP(hipGetLastError());
+    }
+
+    void vcycles_f64(const double* b, double* x, int cycles)
+    {
+        const int nl = (int)host_.size();
+        std::vector<double*> xs(nl), ts(nl), bs(nl);
+        for (int l = 0; l < nl; ++l) {
+            xs[l] = l == 0 ? x : dev_[l].x;
+            ts[l] = dev_[l].t;
+            bs[l] = dev_[l].b;
+        }
+        hipLaunchKernelGGL(k_hadamard<double>, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, dl(0), table<double>(0, false), b, bs[0]);
+        vcycles<double>(xs, ts, bs, cycles);
+        if (xs[0] != x) MG_HIP(hipMemcpyAsync(x, xs[0], (size_t)N0() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    }
+
+    // FP32 V-cycles on dev_[0].bf (already scaled), correction in dev_[0].xf
+    void vcycles_f32(int cycles)
+    {
+        const int nl = (int)host_.size();
+        std::vector<float*> xs(nl), ts(nl), bs(nl);
+        for (int l = 0; l < nl; ++l) {
+            xs[l] = dev_[l].xf;
+            ts[l] = dev_[l].tf;
+            bs[l] = dev_[l].bf;
+        }
+        // e0 = 0
+        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, xs[0], N0());
+        vcycles<float>(xs, ts, bs, cycles);
+        if (xs[0] != dev_[0].xf) {
+            // keep the result in dev_[0].xf for k_add_correction
+            std::swap(dev_[0].xf, dev_[0].tf);
+        }
+    }
+
+    void outer_residual(const double* b, const double* x, bool emit_f32, double* rr, double* bb)
+    {
+        const DevLevel L = dl(0);
+        const unsigned nb = nblk(L.N);
+        DevTable<double> K;
+        K.coef = dev_[0].K;
+        K.dinv = dev_[0].dinv;
+        const bool c = host_[0].compact;
+        float* b0 = emit_f32 ? dev_[0].bf : nullptr;
+        const float* d32 = dev_[0].dinv32;
+#define OUTER(DIM, CMP, F32) hipLaunchKernelGGL((k_outer_residual<DIM, CMP, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, L, K, x, b, d32, b0, part_)
+        if (host_[0].D == 2) {
+            if (c) { if (emit_f32) OUTER(2, true, true); else OUTER(2, true, false); }
+            else { if (emit_f32) OUTER(2, false, true); else OUTER(2, false, false); }
+        } else {
+            if (c) { if (emit_f32) OUTER(3, true, true); else OUTER(3, true, false); }
+            else { if (emit_f32) OUTER(3, false, true); else OUTER(3, false, false); }
+        }
+#undef OUTER
+        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kMgThreads), 0, stream_, part_, (int64_t)nb, red_);
+        MG_HIP(hipGetLastError());
+        MG_HIP(hipMemcpyAsync(red_host_, red_, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        MG_HIP(hipStreamSynchronize(stream_));
+        *rr = red_host_[0];
+        *bb = red_host_[1];
+    }
+};
+
+}  // namespace psum_hip
+
+// ============================================================ plugin ABI
+
+namespace {
+
+std::map<std::string, func_table>& registry()
+{
+    static std::map<std::string, func_table> r;
+    return r;
+}
+
+template <bool MIXED>
+psum_field_solver_handle mg_init()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        throw std::runtime_error(MIXED ? "hip_multigrid32f_gpu: no GPU device available"
+                                       : "hip_multigrid_gpu: no GPU device available");
+    return new HipMultigrid(MIXED);
+}
+
+void mg_set_matrix(psum_field_solver_handle h, unsigned long long n_row, unsigned long long n_col,
+                   unsigned long long nnz, unsigned long long* rows, unsigned long long* cols,
+                   double* vals)
+{
+    static_cast<HipMultigrid*>(h)->set_matrix(n_row, n_col, nnz, rows, cols, vals);
+}
+
+void mg_solve(psum_field_solver_handle h, double* b, double* x)
+{
+    static_cast<HipMultigrid*>(h)->solve(b, x, true);
+}
+
+void mg_set_options(psum_field_solver_handle h, const char* o)
+{
+    static_cast<HipMultigrid*>(h)->set_options(o);
+}
+
+void mg_set_replace(psum_field_solver_handle h, unsigned long long n, unsigned long long* i, double* v)
+{
+    static_cast<HipMultigrid*>(h)->set_source_replace(n, i, v);
+}
+
+void mg_set_addback(psum_field_solver_handle h, unsigned long long n, unsigned long long* i, double* v)
+{
+    static_cast<HipMultigrid*>(h)->set_source_addback(n, i, v);
+}
+
+void mg_free(psum_field_solver_handle h) { delete static_cast<HipMultigrid*>(h); }
+
+const int registered = []() {
+    func_table t64{mg_init<false>, mg_set_matrix, mg_solve, mg_set_options, mg_set_replace, mg_set_addback, mg_free};
+    func_table t32{mg_init<true>, mg_set_matrix, mg_solve, mg_set_options, mg_set_replace, mg_set_addback, mg_free};
+    registry()["hip_multigrid_gpu"] = t64;
+    registry()["hip_multigrid32f_gpu"] = t32;
+    return 0;
+}();
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return PSUM_OK;
+    } catch (const std::invalid_argument& e) {
+        set_error(e.what());
+        return PSUM_ERR_ARG;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return PSUM_ERR_UNSUPPORTED;
+    }
+}
+
+}  // namespace
+
+namespace psum_hip {
+
+// in-library access for the step engine
+HipMultigrid* mg_create(bool mixed) { return new HipMultigrid(mixed); }
+void mg_destroy(HipMultigrid* h) { delete h; }
+void mg_set_options(HipMultigrid* h, const char* o) { h->set_options(o); }
+void mg_set_matrix(HipMultigrid* h, unsigned long long n, unsigned long long nnz,
+                   const unsigned long long* r, const unsigned long long* c, const double* v)
+{
+    h->set_matrix(n, n, nnz, r, c, v);
+}
+void mg_set_replace(HipMultigrid* h, unsigned long long n, const unsigned long long* i, const double* v)
+{
+    h->set_source_replace(n, i, v);
+}
+void mg_set_stream(HipMultigrid* h, hipStream_t s) { h->set_stream(s); }
+void mg_solve(HipMultigrid* h, double* b, double* x, bool sync) { h->solve(b, x, sync); }
+int mg_last_cycles(HipMultigrid* h) { return h->last_cycles(); }
+double mg_last_rel(HipMultigrid* h) { return h->last_rel(); }
+
+}  // namespace psum_hip
+
+extern "C" {
+
+void register_solver(const char* name, func_table table) { registry()[name] = table; }
+
+func_table get_solver_funcs(const char* name)
+{
+    func_table ans{};
+    ans.init = nullptr;
+    auto& reg = registry();
+    auto it = reg.find(name ? name : "");
+    if (it != reg.end()) return it->second;
+    return ans;
+}
+
+void* psum_solver_create(const char* kind)
+{
+    func_table t = get_solver_funcs(kind);
+    if (!t.init) {
+        set_error(std::string("psum_solver_create: unknown backend ") + (kind ? kind : "(null)"));
+        return nullptr;
+    }
+    void* h = nullptr;
+    if (guarded([&]() { h = t.init(); }) != PSUM_OK) return nullptr;
+    return h;
+}
+
+int psum_solver_set_options(void* h, const char* options)
+{
+    if (!h) { set_error("psum_solver_set_options: null handle"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<HipMultigrid*>(h)->set_options(options); });
+}
+
+int psum_solver_set_matrix(void* h, unsigned long long n_row, unsigned long long n_col,
+                           unsigned long long nnz, unsigned long long* rows,
+                           unsigned long long* cols, double* vals)
+{
+    if (!h) { set_error("psum_solver_set_matrix: null handle"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<HipMultigrid*>(h)->set_matrix(n_row, n_col, nnz, rows, cols, vals); });
+}
+
+int psum_solver_set_source_replace(void* h, unsigned long long n, unsigned long long* idxs, double* vals)
+{
+    if (!h) { set_error("null handle"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<HipMultigrid*>(h)->set_source_replace(n, idxs, vals); });
+}
+
+int psum_solver_set_source_addback(void* h, unsigned long long n, unsigned long long* idxs, double* vals)
+{
+    if (!h) { set_error("null handle"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<HipMultigrid*>(h)->set_source_addback(n, idxs, vals); });
+}
+
+int psum_solver_solve(void* h, double* b_dev, double* x_dev)
+{
+    if (!h) { set_error("null handle"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<HipMultigrid*>(h)->solve(b_dev, x_dev, true); });
+}
+
+int psum_solver_stats(void* h, int* cycles, double* rel_res)
+{
+    if (!h) { set_error("null handle"); return PSUM_ERR_ARG; }
+    auto* m = static_cast<HipMultigrid*>(h);
+    if (cycles) *cycles = m->last_cycles();
+    if (rel_res) *rel_res = m->last_rel();
+    return PSUM_OK;
+}
+
+void psum_solver_free(void* h) { delete static_cast<HipMultigrid*>(h); }
+
+}  // extern "C"

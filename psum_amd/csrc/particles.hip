@@ -1,0 +1,957 @@
+// Particle-side kernels of the MI355X PIC step engine (gfx950):
+//   cell binning, stable LSD radix sort, permutation, stable compaction,
+//   deterministic CIC deposit (LDS int64 tile), fused gather + push (+ next
+//   deposit), diagnostics reductions.
+//
+// Reference behaviour restated here (PSuM paths):
+//   cN / n2i / periodic support wrap  src/field/simple_grid.hpp:89-128,316-385
+//   CIC weights and gradient weights  src/field/simple_interpolation.hpp:33-98
+//   add_back / interp_diff            src/field/field_interp.hpp:63-113,
+//                                     src/field/device_field.hpp:270-310
+//   push, wrap, absorb                application/benchmark/TSI/main.cpp:59-75,
+//                                     example/big_bang/stage_6.cpp:48-65,
+//                                     src/particle_container/validator.hpp:18-26
+//   compress                          src/particle_container/particle_group.hpp:223-238
+//
+// Floating point: built with -ffp-contract=off, so every a*b+c below is two
+// rounded operations in the order written (the CPU oracle does the same).
+
+#include <climits>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace psum_hip {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace psum_hip
+
+using namespace psum_hip;
+
+extern "C" const char* psum_hip_last_error(void) { return g_last_error.c_str(); }
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned grid_blocks(int64_t n, int64_t per_block)
+{
+    return (unsigned)((n + per_block - 1) / per_block);
+}
+
+// ------------------------------------------------------------------ binning
+
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void k_bin(DevGrid g, const T* __restrict__ x0,
+                                                  const T* __restrict__ x1,
+                                                  const T* __restrict__ x2, int64_t n,
+                                                  uint32_t* __restrict__ keys)
+{
+    const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= n) return;
+    double x[3];
+    x[0] = (double)x0[p];
+    if (D > 1) x[1] = (double)x1[p];
+    if (D > 2) x[2] = (double)x2[p];
+    uint32_t key = PSUM_KEY_INVALID;
+    int cw[3];
+    double s[3];
+    if (isfinite(x[0]) && base_cell<D>(g, x, cw, s)) {
+        uint32_t lin = (uint32_t)cw[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) lin = lin * (uint32_t)g.w[d] + (uint32_t)cw[d];
+        key = lin;
+    }
+    keys[p] = key;
+}
+
+// ------------------------------------------------------------------- scan
+// Exclusive scan of uint32, 4096 items per block (256 threads x 16 items),
+// block sums scanned recursively.
+
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
+{
+    __shared__ uint32_t wsum[kThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0, tot = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+        if (w < wave) wave_off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return wave_off + inc - v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_tiles(const uint32_t* __restrict__ in,
+                                                         uint32_t* __restrict__ out,
+                                                         uint32_t* __restrict__ bsum, int64_t n)
+{
+    __shared__ uint32_t buf[kScanTile];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    for (int i = threadIdx.x; i < kScanTile; i += kThreads) {
+        const int64_t p = base + i;
+        buf[i] = p < n ? in[p] : 0u;
+    }
+    __syncthreads();
+    uint32_t loc[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        loc[j] = buf[threadIdx.x * kScanItems + j];
+        sum += loc[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, &total);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        buf[threadIdx.x * kScanItems + j] = run;
+        run += loc[j];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kScanTile; i += kThreads) {
+        const int64_t p = base + i;
+        if (p < n) out[p] = buf[i];
+    }
+    if (threadIdx.x == 0 && bsum) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t* __restrict__ out,
+                                                       const uint32_t* __restrict__ boff, int64_t n)
+{
+    const uint32_t add = boff[blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    for (int i = threadIdx.x; i < kScanTile; i += kThreads) {
+        const int64_t p = base + i;
+        if (p < n) out[p] += add;
+    }
+}
+
+// ------------------------------------------------------------- radix sort
+// Stable LSD radix sort, 8-bit digits.  A block owns a tile of 4096 keys;
+// wave w owns the contiguous quarter [w*1024, (w+1)*1024) and visits it in
+// 16 rows of 64 keys, so lane order within a row and row order within the
+// wave are index order.  Within-wave stable ranks come from 8 ballots per
+// key (lanes with equal digit), across waves from per-wave digit counts.
+
+constexpr int kSortWaveItems = 16;
+constexpr int kSortTile = kThreads * kSortWaveItems;
+
+__device__ __forceinline__ uint32_t sort_digit(uint32_t key, uint32_t nkeys, int shift)
+{
+    const uint32_t k = key == PSUM_KEY_INVALID ? nkeys : key;
+    return (k >> shift) & 255u;
+}
+
+__global__ __launch_bounds__(kThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
+                                                        int64_t n, uint32_t nkeys, int shift,
+                                                        uint32_t* __restrict__ ghist, int nb)
+{
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kSortTile + wave * (64 * kSortWaveItems);
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+        const int64_t p = base + j * 64 + lane;
+        if (p < n) atomicAdd(&hist[sort_digit(keys[p], nkeys, shift)], 1u);
+    }
+    __syncthreads();
+    ghist[(int64_t)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void k_sort_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n,
+    uint32_t nkeys, int shift, const uint32_t* __restrict__ goff, int nb,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout)
+{
+    __shared__ uint32_t off[256];
+    __shared__ uint32_t whist[kThreads / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    off[threadIdx.x] = goff[(int64_t)threadIdx.x * nb + blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) whist[wave][lane + 64 * q] = 0;
+    __syncthreads();
+
+    const int64_t base = (int64_t)blockIdx.x * kSortTile + wave * (64 * kSortWaveItems);
+    const uint64_t lt = __lanemask_lt();
+    uint32_t key[kSortWaveItems], val[kSortWaveItems], rank[kSortWaveItems];
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+        const int64_t p = base + j * 64 + lane;
+        const bool act = p < n;
+        key[j] = act ? kin[p] : 0u;
+        val[j] = act ? (vin ? vin[p] : (uint32_t)p) : 0u;
+        const uint32_t d = sort_digit(key[j], nkeys, shift);
+        uint64_t peers = __ballot(act);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        if (act) {
+            const uint32_t before = whist[wave][d];
+            rank[j] = before + (uint32_t)__popcll(peers & lt);
+            if ((peers & lt) == 0) whist[wave][d] = before + (uint32_t)__popcll(peers);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+        const int64_t p = base + j * 64 + lane;
+        if (p < n) {
+            const uint32_t d = sort_digit(key[j], nkeys, shift);
+            uint32_t dst = off[d] + rank[j];
+            for (int w = 0; w < wave; ++w) dst += whist[w][d];
+            kout[dst] = key[j];
+            vout[dst] = val[j];
+        }
+    }
+}
+
+// -------------------------------------------------------------- permute
+
+struct PtrPack {
+    const void* src[8];
+    void* dst[8];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_permute(PtrPack pk, int ncomp,
+                                                      const uint32_t* __restrict__ perm, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    for (int c = 0; c < ncomp; ++c)
+        static_cast<T*>(pk.dst[c])[i] = static_cast<const T*>(pk.src[c])[s];
+}
+
+// ------------------------------------------------------------ compaction
+// Block b counts the valid particles of [b*1024, (b+1)*1024); the counts are
+// scanned; the block then writes its survivors in index order.
+
+constexpr int kCompactRows = 4;
+constexpr int kCompactTile = kThreads * kCompactRows;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_compact_count(const T* __restrict__ x0, int64_t n,
+                                                            uint32_t* __restrict__ bcount)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) bcount[gridDim.x] = 0;   // becomes the total
+    const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < kCompactRows; ++r) {
+        const int64_t p = base + r * kThreads + threadIdx.x;
+        if (p < n && isfinite((double)x0[p])) ++c;
+    }
+    uint32_t total;
+    block_exclusive_scan(c, &total);
+    if (threadIdx.x == 0) bcount[blockIdx.x] = total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_compact_scatter(PtrPack pk, int ncomp, int64_t n,
+                                                              const uint32_t* __restrict__ boff)
+{
+    const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+    uint32_t run = boff[blockIdx.x];
+    const T* x0 = static_cast<const T*>(pk.src[0]);
+    for (int r = 0; r < kCompactRows; ++r) {
+        const int64_t p = base + r * kThreads + threadIdx.x;
+        const bool ok = p < n && isfinite((double)x0[p]);
+        uint32_t total;
+        const uint32_t pre = block_exclusive_scan(ok ? 1u : 0u, &total);
+        if (ok) {
+            const uint32_t dst = run + pre;
+            for (int c = 0; c < ncomp; ++c)
+                static_cast<T*>(pk.dst[c])[dst] = static_cast<const T*>(pk.src[c])[p];
+        }
+        run += total;
+    }
+}
+
+// --------------------------------------------- gather + push + deposit
+// One block = 256 threads x 4 particles.  With DEPOSIT the block deposits
+// the (new) positions of its particles into an LDS int64 tile spanning the
+// bounding box of their support nodes, then flushes non-zero tile entries
+// to the global int64 accumulator; a block whose box exceeds the tile uses
+// global integer atomics directly.  Integer sums make both paths exact.
+
+constexpr int kPPT = 4;
+constexpr int kPushTile = kThreads * kPPT;
+constexpr int kTileNodes = 4096;
+
+template <typename T, int D, bool PUSH, bool DEPOSIT>
+__global__ __launch_bounds__(kThreads) void k_push_deposit(
+    DevGrid g, T* __restrict__ x0, T* __restrict__ x1, T* __restrict__ x2, T* __restrict__ v0,
+    T* __restrict__ v1, T* __restrict__ v2, int64_t n, const T* __restrict__ phi, T qm, T dt,
+    double mass, double w_dep, double scale, unsigned long long* __restrict__ acc,
+    double* __restrict__ diag, unsigned long long* __restrict__ counters)
+{
+    __shared__ unsigned long long tile[DEPOSIT ? kTileNodes : 1];
+    __shared__ int bmin[3], bmax[3];
+    __shared__ double wdiag[kThreads / 64][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (DEPOSIT && threadIdx.x < 3) {
+        bmin[threadIdx.x] = INT_MAX;
+        bmax[threadIdx.x] = INT_MIN;
+    }
+    T* const xs[3] = {x0, x1, x2};
+    T* const vs[3] = {v0, v1, v2};
+    const int64_t base = (int64_t)blockIdx.x * kPushTile;
+
+    bool ok[kPPT];
+    int cw[kPPT][D];
+    T sw[kPPT][D];
+    double ke = 0.0, pm[3] = {0.0, 0.0, 0.0};
+    unsigned oor = 0, absorbed = 0;
+
+#pragma unroll
+    for (int k = 0; k < kPPT; ++k) {
+        ok[k] = false;
+        const int64_t p = base + k * kThreads + threadIdx.x;
+        if (p >= n) continue;
+        T x[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) x[d] = xs[d][p];
+        if (!isfinite((double)x[0])) continue;
+        bool alive = true;
+        if (PUSH) {
+            T v[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) v[d] = vs[d][p];
+            double v2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) v2 += (double)v[d] * (double)v[d];
+            ke += 0.5 * mass * v2;
+#pragma unroll
+            for (int d = 0; d < D; ++d) pm[d] += mass * (double)v[d];
+
+            // gather grad(phi) at x (field_interp.hpp:96-111)
+            T grad[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) grad[d] = (T)0;
+            double xd[D], s[D];
+            int c[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
+            if (base_cell<D>(g, xd, c, s)) {
+                T W[D][2], G[D][2];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const T sd = (T)s[d];
+                    W[d][0] = (T)1 - sd;
+                    W[d][1] = sd;
+                    const T dr = (T)g.delta_r[d];
+                    G[d][0] = (T)-1 * dr;
+                    G[d][1] = (T)1 * dr;
+                }
+                T ph[1 << D];
+#pragma unroll
+                for (int i = 0; i < (1 << D); ++i) ph[i] = phi[support_index<D>(g, c, i)];
+#pragma unroll
+                for (int dd = 0; dd < D; ++dd) {
+                    T r = (T)0;
+#pragma unroll
+                    for (int i = 0; i < (1 << D); ++i) {
+                        const T gc = cic_grad_coef<T, D>(W, G, i, dd);
+                        if (fabs((double)gc) > 1e-10) r = r + ph[i] * gc;
+                    }
+                    grad[dd] = r;
+                }
+            }
+            // kick then drift (TSI/main.cpp:65-68)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const T e = -grad[d];
+                v[d] = v[d] + (e * qm) * dt;
+                x[d] = x[d] + v[d] * dt;
+            }
+            // boundaries: bounded periodic wrap, then absorbing walls
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (g.periodic[d]) {
+                    const T lo = (T)g.plo[d], hi = (T)g.phi[d], L = (T)g.plen[d];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        if (x[d] < lo) x[d] = x[d] + L;
+                        else if (x[d] >= hi) x[d] = x[d] - L;
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (!g.periodic[d]) {
+                    const double xx = (double)x[d];
+                    if (!(xx >= g.lb[d] && xx < g.ub[d])) alive = false;
+                }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                xs[d][p] = x[d];
+                vs[d][p] = v[d];
+            }
+            if (!alive) {
+                xs[0][p] = (T)NAN;
+                ++absorbed;
+            }
+        }
+        if (DEPOSIT && alive) {
+            double xd[D], s[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
+            if (base_cell<D>(g, xd, cw[k], s)) {
+                ok[k] = true;
+#pragma unroll
+                for (int d = 0; d < D; ++d) sw[k][d] = (T)s[d];
+            } else {
+                ++oor;
+            }
+        }
+    }
+
+    if (PUSH && diag) {
+        double vals[4] = {ke, pm[0], pm[1], pm[2]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double r = wave_sum(vals[q]);
+            if (lane == 0) wdiag[wave][q] = r;
+        }
+    }
+    if (counters) {
+        const int ab = wave_isum((int)absorbed), oo = wave_isum((int)oor);
+        if (lane == 0) {
+            if (ab) atomicAdd(&counters[2], (unsigned long long)ab);
+            if (oo) atomicAdd(&counters[0], (unsigned long long)oo);
+        }
+    }
+
+    if (DEPOSIT) {
+        int lmin[3] = {INT_MAX, INT_MAX, INT_MAX}, lmax[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+        for (int k = 0; k < kPPT; ++k)
+            if (ok[k]) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    lmin[d] = min(lmin[d], cw[k][d]);
+                    lmax[d] = max(lmax[d], cw[k][d]);
+                }
+            }
+        __syncthreads();   // bmin/bmax initialised
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int mn = wave_min(lmin[d]), mx = wave_max(lmax[d]);
+            if (lane == 0 && mn != INT_MAX) {
+                atomicMin(&bmin[d], mn);
+                atomicMax(&bmax[d], mx);
+            }
+        }
+        __syncthreads();
+        int e[3] = {1, 1, 1}, lo[3] = {0, 0, 0};
+        int64_t tsize = 1;
+        const bool any = bmin[0] != INT_MAX;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            lo[d] = bmin[d];
+            e[d] = bmax[d] - bmin[d] + 2;   // support adds one node
+            tsize *= any ? (int64_t)e[d] : 0;
+        }
+        const bool use_lds = any && tsize <= kTileNodes;
+        if (use_lds)
+            for (int i = threadIdx.x; i < (int)tsize; i += kThreads) tile[i] = 0ull;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPPT; ++k) {
+            if (!ok[k]) continue;
+            T W[D][2];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                W[d][0] = (T)1 - sw[k][d];
+                W[d][1] = sw[k][d];
+            }
+#pragma unroll
+            for (int i = 0; i < (1 << D); ++i) {
+                const T cf = cic_coef<T, D>(W, i);
+                if (!(fabs((double)cf) > 1e-10)) continue;
+                const double v = w_dep * (double)cf;
+                const long long term = __double2ll_rn(v * scale);
+                if (use_lds) {
+                    int li = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+                        li = li * e[d] + (cw[k][d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
+                    atomicAdd(&tile[li], (unsigned long long)term);
+                } else {
+                    atomicAdd(&acc[support_index<D>(g, cw[k], i)], (unsigned long long)term);
+                }
+            }
+        }
+        if (use_lds) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < (int)tsize; i += kThreads) {
+                const unsigned long long v = tile[i];
+                if (v == 0ull) continue;
+                int rem = i;
+                int nd[3];
+#pragma unroll
+                for (int d = D - 1; d >= 0; --d) {
+                    nd[d] = lo[d] + rem % e[d];
+                    rem /= e[d];
+                    if (nd[d] == g.w[d]) nd[d] = 0;
+                }
+                uint32_t lin = (uint32_t)nd[0];
+#pragma unroll
+                for (int d = 1; d < D; ++d) lin = lin * (uint32_t)g.w[d] + (uint32_t)nd[d];
+                atomicAdd(&acc[lin], v);
+            }
+        } else if (any && counters && threadIdx.x == 0) {
+            atomicAdd(&counters[1], 1ull);
+        }
+    }
+
+    if (PUSH && diag) {
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double s = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) s += wdiag[w][threadIdx.x];
+            diag[(int64_t)blockIdx.x * 4 + threadIdx.x] = s;
+        }
+    }
+}
+
+// -------------------------------------------------------- small kernels
+
+__global__ __launch_bounds__(kThreads) void k_fixed_to_double(const long long* __restrict__ acc,
+                                                              int64_t n, double inv_scale,
+                                                              double offset,
+                                                              double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) out[i] = (double)acc[i] * inv_scale + offset;
+}
+
+__global__ __launch_bounds__(kThreads) void k_f64_to_f32(const double* __restrict__ in,
+                                                         float* __restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+// Fixed-order reduction of nb x 4 partials by one block.
+__global__ __launch_bounds__(kThreads) void k_reduce4(const double* __restrict__ part, int64_t nb,
+                                                      double* __restrict__ out)
+{
+    __shared__ double sh[4][kThreads];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t b = threadIdx.x; b < nb; b += kThreads)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] += part[b * 4 + q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Field energy partials: one value per block (written to part[b*4]).
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_field_energy(DevGrid g, const double* __restrict__ phi,
+                                                           int64_t nn, double* __restrict__ part)
+{
+    __shared__ double wsum[kThreads / 64];
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    double acc = 0.0;
+    if (i < nn) {
+        int64_t idx[3] = {0, 0, 0};
+        int64_t rem = i;
+        for (int d = D - 1; d >= 0; --d) {
+            idx[d] = rem % g.w[d];
+            rem /= g.w[d];
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            int64_t nb = idx[d] + 1;
+            if (nb == g.w[d]) {
+                if (!g.periodic[d]) continue;
+                nb = 0;
+            }
+            int64_t lin = 0;
+#pragma unroll
+            for (int q = 0; q < D; ++q) lin = lin * g.w[q] + (q == d ? nb : idx[q]);
+            const double gd = (phi[lin] - phi[i]) * g.delta_r[d];
+            acc += gd * gd;
+        }
+    }
+    const double r = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) s += wsum[w];
+        part[blockIdx.x * 4 + 0] = s;
+        part[blockIdx.x * 4 + 1] = 0.0;
+        part[blockIdx.x * 4 + 2] = 0.0;
+        part[blockIdx.x * 4 + 3] = 0.0;
+    }
+}
+
+// v[0] = (f * v[0]) * vol, the order of orc_field_energy
+__global__ void k_scale_first(double* v, double f, double vol)
+{
+    if (threadIdx.x == 0) v[0] = (f * v[0]) * vol;
+}
+
+// ------------------------------------------------------ launch helpers
+
+template <typename T>
+psum_status launch_push(const psum_grid* g, void* const* pos, void* const* vel, int64_t n,
+                        const void* phi, double qm, double dt, double mass, double w_dep,
+                        double scale, int64_t* acc, double* diag, int64_t* counters, bool push,
+                        hipStream_t st)
+{
+    const DevGrid dg = to_dev(*g);
+    const unsigned nb = grid_blocks(n, kPushTile);
+    if (nb == 0) return PSUM_OK;
+    T* p[3] = {nullptr, nullptr, nullptr};
+    T* v[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < g->dim; ++d) {
+        p[d] = static_cast<T*>(pos[d]);
+        if (push) v[d] = static_cast<T*>(vel[d]);
+    }
+    auto* a = reinterpret_cast<unsigned long long*>(acc);
+    auto* c = reinterpret_cast<unsigned long long*>(counters);
+    const T* ph = static_cast<const T*>(phi);
+#define PSUM_LAUNCH(DIM, PU, DE)                                                              \
+    hipLaunchKernelGGL((k_push_deposit<T, DIM, PU, DE>), dim3(nb), dim3(kThreads), 0, st, dg,  \
+                       p[0], p[1], p[2], v[0], v[1], v[2], n, ph, (T)qm, (T)dt, mass, w_dep,   \
+                       scale, a, diag, c)
+#define PSUM_DISPATCH(DIM)                                                                    \
+    if (push && acc) PSUM_LAUNCH(DIM, true, true);                                            \
+    else if (push) PSUM_LAUNCH(DIM, true, false);                                             \
+    else PSUM_LAUNCH(DIM, false, true);
+    switch (g->dim) {
+    case 1: PSUM_DISPATCH(1); break;
+    case 2: PSUM_DISPATCH(2); break;
+    default: PSUM_DISPATCH(3); break;
+    }
+#undef PSUM_DISPATCH
+#undef PSUM_LAUNCH
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+namespace psum_hip {
+
+size_t scan_workspace(int64_t n)
+{
+    size_t bytes = 0;
+    while (n > kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        bytes += ((size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
+    }
+    return bytes + 256;
+}
+
+psum_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* work,
+                               hipStream_t st)
+{
+    if (n <= 0) return PSUM_OK;
+    const unsigned nb = grid_blocks(n, kScanTile);
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, st, in, out, nullptr, n);
+        PSUM_HIP_TRY(hipGetLastError());
+        return PSUM_OK;
+    }
+    uint32_t* bsum = static_cast<uint32_t*>(work);
+    void* rest = static_cast<char*>(work) + (((size_t)nb * sizeof(uint32_t) + 255) & ~(size_t)255);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(kThreads), 0, st, in, out, bsum, n);
+    PSUM_HIP_TRY(hipGetLastError());
+    psum_status s = exclusive_scan_u32(bsum, bsum, nb, rest, st);
+    if (s != PSUM_OK) return s;
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kThreads), 0, st, out, bsum, n);
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+}  // namespace psum_hip
+
+extern "C" {
+
+psum_status psum_hip_bin(const psum_grid* g, int dtype, void* const* pos, int64_t n,
+                         uint32_t* keys, void* stream)
+{
+    PSUM_ARG_CHECK(grid_ok(g), "psum_hip_bin: invalid grid");
+    PSUM_ARG_CHECK(n >= 0 && (n == 0 || (pos && keys)), "psum_hip_bin: null pointer");
+    if (n == 0) return PSUM_OK;
+    const DevGrid dg = to_dev(*g);
+    const unsigned nb = grid_blocks(n, kThreads);
+#define PSUM_BIN(T, DIM)                                                                       \
+    hipLaunchKernelGGL((k_bin<T, DIM>), dim3(nb), dim3(kThreads), 0, S(stream), dg,             \
+                       static_cast<const T*>(pos[0]),                                           \
+                       DIM > 1 ? static_cast<const T*>(pos[1]) : nullptr,                       \
+                       DIM > 2 ? static_cast<const T*>(pos[2]) : nullptr, n, keys)
+    if (dtype == PSUM_F64) {
+        if (g->dim == 1) PSUM_BIN(double, 1); else if (g->dim == 2) PSUM_BIN(double, 2); else PSUM_BIN(double, 3);
+    } else if (dtype == PSUM_F32) {
+        if (g->dim == 1) PSUM_BIN(float, 1); else if (g->dim == 2) PSUM_BIN(float, 2); else PSUM_BIN(float, 3);
+    } else {
+        set_error("psum_hip_bin: bad dtype");
+        return PSUM_ERR_ARG;
+    }
+#undef PSUM_BIN
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t psum_hip_sort_workspace(int64_t n)
+{
+    if (n <= 0) return 256;
+    const int64_t nb = (n + kSortTile - 1) / kSortTile;
+    const int64_t nh = nb * 256;
+    return 2 * align256((size_t)n * 4) + align256((size_t)nh * 4) + scan_workspace(nh);
+}
+
+psum_status psum_hip_sort(const uint32_t* keys, int64_t n, uint32_t nkeys, uint32_t* keys_out,
+                          uint32_t* perm, void* work, size_t work_bytes, void* stream)
+{
+    PSUM_ARG_CHECK(n >= 0 && n < (int64_t)0xFFFFFFFF, "psum_hip_sort: n out of range");
+    if (n == 0) return PSUM_OK;
+    PSUM_ARG_CHECK(keys && keys_out && perm && work, "psum_hip_sort: null pointer");
+    PSUM_ARG_CHECK(work_bytes >= psum_hip_sort_workspace(n), "psum_hip_sort: workspace too small");
+    PSUM_ARG_CHECK(nkeys >= 1 && nkeys < PSUM_KEY_INVALID, "psum_hip_sort: bad nkeys");
+    hipStream_t st = S(stream);
+    const int nb = (int)((n + kSortTile - 1) / kSortTile);
+    char* w = static_cast<char*>(work);
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(w);
+    w += align256((size_t)n * 4);
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(w);
+    w += align256((size_t)n * 4);
+    uint32_t* gh = reinterpret_cast<uint32_t*>(w);
+    w += align256((size_t)nb * 256 * 4);
+    void* scan_work = w;
+
+    int bits = 32 - __builtin_clz(nkeys);   // INVALID maps to nkeys
+    const int passes = (bits + 7) / 8;
+    const uint32_t* kin = keys;
+    const uint32_t* vin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        // the last pass lands in (keys_out, perm)
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t* kout = to_out ? keys_out : k1;
+        uint32_t* vout = to_out ? perm : v1;
+        hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(kThreads), 0, st, kin, n, nkeys, 8 * p, gh,
+                           nb);
+        PSUM_HIP_TRY(hipGetLastError());
+        psum_status s = exclusive_scan_u32(gh, gh, (int64_t)nb * 256, scan_work, st);
+        if (s != PSUM_OK) return s;
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(kThreads), 0, st, kin, vin, n, nkeys,
+                           8 * p, gh, nb, kout, vout);
+        PSUM_HIP_TRY(hipGetLastError());
+        kin = kout;
+        vin = vout;
+    }
+    return PSUM_OK;
+}
+
+psum_status psum_hip_permute(int dtype, int ncomp, void* const* src, void* const* dst,
+                             const uint32_t* perm, int64_t n, void* stream)
+{
+    PSUM_ARG_CHECK(ncomp >= 1 && ncomp <= 8, "psum_hip_permute: ncomp must be 1..8");
+    if (n == 0) return PSUM_OK;
+    PtrPack pk{};
+    for (int c = 0; c < ncomp; ++c) {
+        PSUM_ARG_CHECK(src[c] && dst[c] && src[c] != dst[c], "psum_hip_permute: bad arrays");
+        pk.src[c] = src[c];
+        pk.dst[c] = dst[c];
+    }
+    const unsigned nb = grid_blocks(n, kThreads);
+    if (dtype == PSUM_F64)
+        hipLaunchKernelGGL(k_permute<double>, dim3(nb), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
+    else if (dtype == PSUM_F32)
+        hipLaunchKernelGGL(k_permute<float>, dim3(nb), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
+    else {
+        set_error("psum_hip_permute: bad dtype");
+        return PSUM_ERR_ARG;
+    }
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+size_t psum_hip_compact_workspace(int64_t n)
+{
+    const int64_t nb = (n + kCompactTile - 1) / kCompactTile;
+    return align256((size_t)(nb + 1) * 4) + scan_workspace(nb) + 256;
+}
+
+psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
+                             void* const* out_pos, void* const* out_vel, int64_t* count_host,
+                             void* work, size_t work_bytes, void* stream)
+{
+    PSUM_ARG_CHECK(dim >= 1 && dim <= 3, "psum_hip_compact: bad dim");
+    PSUM_ARG_CHECK(work_bytes >= psum_hip_compact_workspace(n), "psum_hip_compact: workspace too small");
+    hipStream_t st = S(stream);
+    if (n == 0) {
+        if (count_host) *count_host = 0;
+        return PSUM_OK;
+    }
+    const unsigned nb = grid_blocks(n, kCompactTile);
+    uint32_t* bc = static_cast<uint32_t*>(work);
+    void* scan_work = static_cast<char*>(work) + align256((size_t)(nb + 1) * 4);
+    PtrPack pk{};
+    int nc = 0;
+    for (int d = 0; d < dim; ++d) { pk.src[nc] = pos[d]; pk.dst[nc] = out_pos[d]; ++nc; }
+    for (int d = 0; d < dim; ++d) { pk.src[nc] = vel[d]; pk.dst[nc] = out_vel[d]; ++nc; }
+    for (int c = 0; c < nc; ++c) PSUM_ARG_CHECK(pk.src[c] && pk.dst[c], "psum_hip_compact: null array");
+    if (dtype == PSUM_F64) {
+        hipLaunchKernelGGL(k_compact_count<double>, dim3(nb), dim3(kThreads), 0, st,
+                           static_cast<const double*>(pos[0]), n, bc);
+    } else if (dtype == PSUM_F32) {
+        hipLaunchKernelGGL(k_compact_count<float>, dim3(nb), dim3(kThreads), 0, st,
+                           static_cast<const float*>(pos[0]), n, bc);
+    } else {
+        set_error("psum_hip_compact: bad dtype");
+        return PSUM_ERR_ARG;
+    }
+    PSUM_HIP_TRY(hipGetLastError());
+    // scan nb+1 entries: entry nb becomes the total
+    psum_status s = exclusive_scan_u32(bc, bc, (int64_t)nb + 1, scan_work, st);
+    if (s != PSUM_OK) return s;
+    if (dtype == PSUM_F64)
+        hipLaunchKernelGGL(k_compact_scatter<double>, dim3(nb), dim3(kThreads), 0, st, pk, nc, n, bc);
+    else
+        hipLaunchKernelGGL(k_compact_scatter<float>, dim3(nb), dim3(kThreads), 0, st, pk, nc, n, bc);
+    PSUM_HIP_TRY(hipGetLastError());
+    if (count_host) {
+        uint32_t total = 0;
+        PSUM_HIP_TRY(hipMemcpyAsync(&total, bc + nb, 4, hipMemcpyDeviceToHost, st));
+        PSUM_HIP_TRY(hipStreamSynchronize(st));
+        *count_host = total;
+    }
+    return PSUM_OK;
+}
+
+psum_status psum_hip_deposit_cic(const psum_grid* g, int dtype, void* const* pos, int64_t n,
+                                 double w, double scale, int64_t* acc, int64_t* counters,
+                                 void* stream)
+{
+    PSUM_ARG_CHECK(grid_ok(g), "psum_hip_deposit_cic: invalid grid");
+    PSUM_ARG_CHECK(acc && (n == 0 || pos), "psum_hip_deposit_cic: null pointer");
+    if (dtype == PSUM_F64)
+        return launch_push<double>(g, pos, nullptr, n, nullptr, 0, 0, 0, w, scale, acc, nullptr,
+                                   counters, false, S(stream));
+    if (dtype == PSUM_F32)
+        return launch_push<float>(g, pos, nullptr, n, nullptr, 0, 0, 0, w, scale, acc, nullptr,
+                                  counters, false, S(stream));
+    set_error("psum_hip_deposit_cic: bad dtype");
+    return PSUM_ERR_ARG;
+}
+
+psum_status psum_hip_fixed_to_double(const int64_t* acc, int64_t nn, double inv_scale,
+                                     double offset, double* rho, void* stream)
+{
+    if (nn == 0) return PSUM_OK;
+    hipLaunchKernelGGL(k_fixed_to_double, dim3(grid_blocks(nn, kThreads)), dim3(kThreads), 0,
+                       S(stream), reinterpret_cast<const long long*>(acc), nn, inv_scale, offset,
+                       rho);
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+int64_t psum_hip_push_blocks(int64_t n) { return (n + kPushTile - 1) / kPushTile; }
+
+psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos,
+                                 void* const* vel, int64_t n, const void* phi, double qm,
+                                 double dt, double mass, double w_dep, double scale, int64_t* acc,
+                                 double* diag, int64_t* counters, void* stream)
+{
+    PSUM_ARG_CHECK(grid_ok(g), "psum_hip_gather_push: invalid grid");
+    PSUM_ARG_CHECK(n == 0 || (pos && vel && phi), "psum_hip_gather_push: null pointer");
+    if (dtype == PSUM_F64)
+        return launch_push<double>(g, pos, vel, n, phi, qm, dt, mass, w_dep, scale, acc, diag,
+                                   counters, true, S(stream));
+    if (dtype == PSUM_F32)
+        return launch_push<float>(g, pos, vel, n, phi, qm, dt, mass, w_dep, scale, acc, diag,
+                                  counters, true, S(stream));
+    set_error("psum_hip_gather_push: bad dtype");
+    return PSUM_ERR_ARG;
+}
+
+psum_status psum_hip_reduce_diag(const double* diag, int64_t nblocks, double* out, void* stream)
+{
+    hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(kThreads), 0, S(stream), diag, nblocks, out);
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+size_t psum_hip_field_energy_workspace(const psum_grid* g)
+{
+    if (!grid_ok(g)) return 0;
+    const int64_t nb = (grid_nodes(*g) + kThreads - 1) / kThreads;
+    return (size_t)nb * 4 * sizeof(double);
+}
+
+psum_status psum_hip_field_energy(const psum_grid* g, const double* phi, double eps, double* out,
+                                  void* work, size_t work_bytes, void* stream)
+{
+    PSUM_ARG_CHECK(grid_ok(g) && phi && out && work, "psum_hip_field_energy: bad argument");
+    PSUM_ARG_CHECK(work_bytes >= psum_hip_field_energy_workspace(g),
+                   "psum_hip_field_energy: workspace too small");
+    const int64_t nn = grid_nodes(*g);
+    const int64_t nb = (nn + kThreads - 1) / kThreads;
+    double* part = static_cast<double*>(work);
+    const DevGrid dg = to_dev(*g);
+    hipStream_t st = S(stream);
+    if (g->dim == 1)
+        hipLaunchKernelGGL(k_field_energy<1>, dim3(nb), dim3(kThreads), 0, st, dg, phi, nn, part);
+    else if (g->dim == 2)
+        hipLaunchKernelGGL(k_field_energy<2>, dim3(nb), dim3(kThreads), 0, st, dg, phi, nn, part);
+    else
+        hipLaunchKernelGGL(k_field_energy<3>, dim3(nb), dim3(kThreads), 0, st, dg, phi, nn, part);
+    PSUM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(kThreads), 0, st, part, nb, out);
+    PSUM_HIP_TRY(hipGetLastError());
+    double vol = 1.0;
+    for (int d = 0; d < g->dim; ++d) vol *= g->delta[d];
+    hipLaunchKernelGGL(k_scale_first, dim3(1), dim3(64), 0, st, out, 0.5 * eps, vol);
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+psum_status psum_hip_f64_to_f32(const double* in, float* out, int64_t n, void* stream)
+{
+    if (n == 0) return PSUM_OK;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_blocks(n, kThreads)), dim3(kThreads), 0, S(stream),
+                       in, out, n);
+    PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+}  // extern "C"

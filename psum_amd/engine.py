@@ -1,0 +1,406 @@
+"""Python host layer over the HIP step engine's C ABI.
+
+Mirrors the reference interfaces the hot path exposes:
+  * simple_grid construction (src/field/simple_grid.hpp:38-51)
+  * particle SoA operations: cell binning, stable sort, permute, compaction
+    (particle_group::compress, src/particle_container/particle_group.hpp:223-238)
+  * add_back / interp_diff + push (src/field/device_field.hpp:270-310)
+  * solver_backend (src/field_solver/backend.hpp:11-107) over the registered
+    "hip_multigrid_gpu" / "hip_multigrid32f_gpu" backends
+  * a native Stepper running the whole reference step loop on the device.
+
+Tensors are torch CUDA tensors (HIP device memory); torch is used only to
+own device memory and provide the stream.  Every call goes through
+libpsum_hip.so; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import (KEY_INVALID, PSUM_F32, PSUM_F64, FuncTable, PsumError, PsumGrid,
+                   PsumSpecies, PsumStepConfig, check, lib)
+
+__all__ = ["make_grid", "periodic_grid", "wall_grid", "num_nodes", "fx_shift", "cell_keys",
+           "stable_sort", "permute", "compact", "deposit", "fixed_to_double", "gather_push",
+           "field_energy", "SolverBackend", "Stepper", "KEY_INVALID", "PsumError",
+           "get_solver_funcs"]
+
+
+# ---------------------------------------------------------------- grids
+
+def make_grid(lb, ub, cells, periodic=None, plo=None, plen=None):
+    """simple_grid(lb, ub, cells) (simple_grid.hpp:38-51): delta = (ub-lb)/cells,
+    delta_r = 1/delta.  periodic[d] = 1 wraps particles into [plo, plo+plen)."""
+    d = len(cells)
+    g = PsumGrid()
+    g.dim = d
+    for i in range(d):
+        g.cells[i] = int(cells[i])
+        g.lb[i] = float(lb[i])
+        g.ub[i] = float(ub[i])
+        delta = (float(ub[i]) - float(lb[i])) / int(cells[i])
+        g.delta[i] = delta
+        g.delta_r[i] = 1.0 / delta
+        g.periodic[i] = int(periodic[i]) if periodic else 0
+        g.plo[i] = float(plo[i]) if plo else 0.0
+        g.plen[i] = float(plen[i]) if plen else 0.0
+    return g
+
+
+def periodic_grid(n, dx=1.0):
+    """n[d] nodes per dimension at dx/2 + i*dx with period n*dx, the reference's
+    periodic node convention (TSI/main.cpp:94: grid({dx/2}, {L - dx/2}, {N - 1}))."""
+    d = len(n)
+    return make_grid([dx / 2] * d, [ni * dx - dx / 2 for ni in n], [ni - 1 for ni in n],
+                     periodic=[1] * d, plo=[0.0] * d, plen=[ni * dx for ni in n])
+
+
+def wall_grid(cells, dx=1.0):
+    """cells[d] cells on [0, cells*dx), absorbing particle walls."""
+    d = len(cells)
+    return make_grid([0.0] * d, [c * dx for c in cells], cells, periodic=[0] * d)
+
+
+def num_nodes(g):
+    n = 1
+    for i in range(g.dim):
+        n *= g.cells[i] + 1
+    return n
+
+
+def node_shape(g):
+    return tuple(g.cells[i] + 1 for i in range(g.dim))
+
+
+def fx_shift(w):
+    """Deposit fixed-point exponent S = 40 - e, 2^(e-1) <= |w| < 2^e."""
+    _, e = np.frexp(abs(w))
+    return 40 - int(e)
+
+
+# ---------------------------------------------------------------- helpers
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dtype_code(t):
+    if t.dtype == torch.float64:
+        return PSUM_F64
+    if t.dtype == torch.float32:
+        return PSUM_F32
+    raise PsumError(f"unsupported particle dtype {t.dtype}")
+
+
+def _ptrs(tensors):
+    arr = (C.c_void_p * max(1, len(tensors)))()
+    for i, t in enumerate(tensors):
+        assert t.is_cuda and t.is_contiguous()
+        arr[i] = t.data_ptr()
+    return arr
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _check_soa(g, arrs):
+    if len(arrs) != g.dim:
+        raise PsumError(f"expected {g.dim} component arrays, got {len(arrs)}")
+    n = arrs[0].numel()
+    for a in arrs:
+        if a.numel() != n or a.dtype != arrs[0].dtype or not a.is_cuda or not a.is_contiguous():
+            raise PsumError("SoA arrays must be contiguous CUDA tensors of equal size and dtype")
+    return n
+
+
+# --------------------------------------------------------- particle ops
+
+def cell_keys(g, pos):
+    """keys[i] = n2i(wrapped cN(x_i)); KEY_INVALID for invalid particles.
+    Returns an int32 tensor holding the uint32 bit patterns."""
+    n = _check_soa(g, pos)
+    keys = torch.empty(n, dtype=torch.int32, device=pos[0].device)
+    check(lib().psum_hip_bin(C.byref(g), _dtype_code(pos[0]), _ptrs(pos), n, _p(keys), _stream()),
+          "psum_hip_bin")
+    return keys
+
+
+def stable_sort(keys, nkeys):
+    """Stable sort of uint32 keys (int32 tensor bit patterns).  Returns
+    (sorted_keys, perm)."""
+    n = keys.numel()
+    out = torch.empty_like(keys)
+    perm = torch.empty_like(keys)
+    wb = lib().psum_hip_sort_workspace(n)
+    work = torch.empty(max(1, wb), dtype=torch.uint8, device=keys.device)
+    check(lib().psum_hip_sort(_p(keys), n, nkeys, _p(out), _p(perm), _p(work), wb, _stream()),
+          "psum_hip_sort")
+    return out, perm
+
+
+def permute(arrays, perm):
+    """[a[perm] for a in arrays] on the device."""
+    dst = [torch.empty_like(a) for a in arrays]
+    check(lib().psum_hip_permute(_dtype_code(arrays[0]), len(arrays), _ptrs(arrays), _ptrs(dst),
+                                 _p(perm), arrays[0].numel(), _stream()), "psum_hip_permute")
+    return dst
+
+
+def compact(pos, vel):
+    """Stable compaction of valid particles (finite pos[0])."""
+    dim = len(pos)
+    n = pos[0].numel()
+    op = [torch.empty_like(a) for a in pos]
+    ov = [torch.empty_like(a) for a in vel]
+    wb = lib().psum_hip_compact_workspace(n)
+    work = torch.empty(max(1, wb), dtype=torch.uint8, device=pos[0].device)
+    cnt = C.c_int64(0)
+    check(lib().psum_hip_compact(_dtype_code(pos[0]), dim, _ptrs(pos), _ptrs(vel), n, _ptrs(op),
+                                 _ptrs(ov), C.byref(cnt), _p(work), wb, _stream()),
+          "psum_hip_compact")
+    m = cnt.value
+    return [a[:m] for a in op], [a[:m] for a in ov]
+
+
+def deposit(g, pos, w, scale=None, acc=None, counters=None):
+    """Fixed-point CIC deposit; returns (acc int64 tensor, scale)."""
+    n = _check_soa(g, pos)
+    if scale is None:
+        scale = float(np.ldexp(1.0, fx_shift(w)))
+    if acc is None:
+        acc = torch.zeros(num_nodes(g), dtype=torch.int64, device=pos[0].device)
+    check(lib().psum_hip_deposit_cic(C.byref(g), _dtype_code(pos[0]), _ptrs(pos), n, w, scale,
+                                     _p(acc), _p(counters), _stream()), "psum_hip_deposit_cic")
+    return acc, scale
+
+
+def fixed_to_double(acc, scale, offset=0.0):
+    out = torch.empty(acc.numel(), dtype=torch.float64, device=acc.device)
+    check(lib().psum_hip_fixed_to_double(_p(acc), acc.numel(), 1.0 / scale, offset, _p(out),
+                                         _stream()), "psum_hip_fixed_to_double")
+    return out
+
+
+def gather_push(g, pos, vel, phi, qm, dt, mass=1.0, w_dep=None, scale=None, counters=None):
+    """Gather grad(phi), kick, drift, boundaries, in place on pos/vel.
+    With w_dep the new positions are deposited.  Returns (acc or None, diag)
+    where diag = [sum 1/2 m v^2, sum m vx, sum m vy, sum m vz] of the entering
+    velocities."""
+    n = _check_soa(g, pos)
+    _check_soa(g, vel)
+    dev = pos[0].device
+    if phi.dtype != pos[0].dtype:
+        raise PsumError("phi must have the particle dtype")
+    nb = lib().psum_hip_push_blocks(n)
+    diag = torch.zeros(max(1, nb) * 4, dtype=torch.float64, device=dev)
+    acc = None
+    if w_dep is not None:
+        if scale is None:
+            scale = float(np.ldexp(1.0, fx_shift(w_dep)))
+        acc = torch.zeros(num_nodes(g), dtype=torch.int64, device=dev)
+    check(lib().psum_hip_gather_push(C.byref(g), _dtype_code(pos[0]), _ptrs(pos), _ptrs(vel), n,
+                                     _p(phi), qm, dt, mass, w_dep or 0.0, scale or 1.0, _p(acc),
+                                     _p(diag), _p(counters), _stream()), "psum_hip_gather_push")
+    out = torch.zeros(4, dtype=torch.float64, device=dev)
+    if nb > 0:
+        check(lib().psum_hip_reduce_diag(_p(diag), nb, _p(out), _stream()), "psum_hip_reduce_diag")
+    return acc, out
+
+
+def field_energy(g, phi, eps=1.0):
+    out = torch.zeros(4, dtype=torch.float64, device=phi.device)
+    wb = lib().psum_hip_field_energy_workspace(C.byref(g))
+    work = torch.empty(max(1, wb), dtype=torch.uint8, device=phi.device)
+    check(lib().psum_hip_field_energy(C.byref(g), _p(phi), eps, _p(out), _p(work), wb, _stream()),
+          "psum_hip_field_energy")
+    return out[0]
+
+
+# ---------------------------------------------------------------- solver
+
+def get_solver_funcs(name):
+    """The registry lookup of the reference ABI (registry.cpp:16-22)."""
+    return lib().get_solver_funcs(name.encode())
+
+
+class SolverBackend:
+    """solver_backend (src/field_solver/backend.hpp:11-107) over the HIP
+    multigrid backends.  Errors raise PsumError (the C++ backends throw)."""
+
+    def __init__(self, kind="hip_multigrid_gpu"):
+        t = get_solver_funcs(kind)
+        if not t.init:
+            raise PsumError(f"Error: null pointer (no backend named {kind!r})")
+        self.h = lib().psum_solver_create(kind.encode())
+        if not self.h:
+            raise PsumError("Error: failed to create solver. " +
+                            lib().psum_hip_last_error().decode(errors="replace"))
+
+    def set_options(self, options):
+        check(lib().psum_solver_set_options(self.h, options.encode()), "set_options")
+
+    def set_matrix(self, n_row, n_col, rows, cols, vals):
+        rows = np.ascontiguousarray(rows, np.uint64)
+        cols = np.ascontiguousarray(cols, np.uint64)
+        vals = np.ascontiguousarray(vals, np.float64)
+        check(lib().psum_solver_set_matrix(self.h, n_row, n_col, vals.shape[0],
+                                           rows.ctypes.data_as(C.c_void_p),
+                                           cols.ctypes.data_as(C.c_void_p),
+                                           vals.ctypes.data_as(C.c_void_p)), "set_matrix")
+
+    def set_source_replace(self, idxs, vals):
+        idxs = np.ascontiguousarray(idxs, np.uint64)
+        vals = np.ascontiguousarray(vals, np.float64)
+        check(lib().psum_solver_set_source_replace(self.h, idxs.shape[0],
+                                                   idxs.ctypes.data_as(C.c_void_p),
+                                                   vals.ctypes.data_as(C.c_void_p)), "replace")
+
+    def set_source_addback(self, idxs, vals):
+        idxs = np.ascontiguousarray(idxs, np.uint64)
+        vals = np.ascontiguousarray(vals, np.float64)
+        check(lib().psum_solver_set_source_addback(self.h, idxs.shape[0],
+                                                   idxs.ctypes.data_as(C.c_void_p),
+                                                   vals.ctypes.data_as(C.c_void_p)), "addback")
+
+    def solve(self, b, x):
+        """b, x: float64 CUDA tensors; b is modified in place, x is the warm
+        start and the result."""
+        assert b.dtype == torch.float64 and x.dtype == torch.float64
+        torch.cuda.synchronize()
+        check(lib().psum_solver_solve(self.h, _p(b), _p(x)), "solve")
+
+    def stats(self):
+        cyc = C.c_int(0)
+        rel = C.c_double(0)
+        check(lib().psum_solver_stats(self.h, C.byref(cyc), C.byref(rel)), "stats")
+        return cyc.value, rel.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psum_solver_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ---------------------------------------------------------------- stepper
+
+class Stepper:
+    """The reference step loop (deposit -> solve -> gather/push) run natively.
+
+    species: list of dicts with q, m, w, pos (list of numpy arrays), vel.
+    """
+
+    def __init__(self, g, species, dtype="f64", mixed_mg=False, depth=4, pre=2, post=1,
+                 max_cycles=20, tol=1e-10, eps=1.0, dt=0.1, rho_bg=0.0, sort_every=0,
+                 fixed=None, fixed_val=None, capacity=None):
+        cfg = PsumStepConfig()
+        cfg.grid = g
+        cfg.dtype = PSUM_F64 if dtype == "f64" else PSUM_F32
+        cfg.mixed_mg = int(bool(mixed_mg))
+        cfg.nspecies = len(species)
+        cfg.depth, cfg.pre, cfg.post, cfg.max_cycles = depth, pre, post, max_cycles
+        cfg.sort_every = sort_every
+        cfg.tol, cfg.eps, cfg.dt, cfg.rho_bg = tol, eps, dt, rho_bg
+        self._keep = []
+        if fixed is not None:
+            fm = np.ascontiguousarray(fixed, np.uint8)
+            self._keep.append(fm)
+            cfg.fixed = fm.ctypes.data
+        if fixed_val is not None:
+            fv = np.ascontiguousarray(fixed_val, np.float64)
+            self._keep.append(fv)
+            cfg.fixed_val = fv.ctypes.data
+        np_dt = np.float64 if dtype == "f64" else np.float32
+        for i, sp in enumerate(species):
+            n = len(sp["pos"][0])
+            cfg.species[i] = PsumSpecies(sp["q"], sp["m"], sp["w"], n,
+                                         capacity if capacity else n)
+        self.g = g
+        self.dim = g.dim
+        self.np_dtype = np_dt
+        self.h = lib().psum_stepper_create(C.byref(cfg))
+        if not self.h:
+            raise PsumError("psum_stepper_create: " +
+                            lib().psum_hip_last_error().decode(errors="replace"))
+        for i, sp in enumerate(species):
+            pos = [np.ascontiguousarray(a, np_dt) for a in sp["pos"]]
+            vel = [np.ascontiguousarray(a, np_dt) for a in sp["vel"]]
+            pa = (C.c_void_p * 3)(*[a.ctypes.data for a in pos])
+            va = (C.c_void_p * 3)(*[a.ctypes.data for a in vel])
+            check(lib().psum_stepper_load(self.h, i, pa, va, len(pos[0])), "psum_stepper_load")
+
+    def run(self, steps, history=True):
+        hist = np.zeros((steps, 5), np.float64) if history else None
+        check(lib().psum_stepper_run(self.h, steps,
+                                     hist.ctypes.data_as(C.c_void_p) if history else None),
+              "psum_stepper_run")
+        return hist
+
+    def phi(self):
+        out = np.zeros(num_nodes(self.g), np.float64)
+        check(lib().psum_stepper_phi(self.h, out.ctypes.data_as(C.c_void_p)), "psum_stepper_phi")
+        return out
+
+    def state(self, s=0):
+        """(pos, vel) of species s copied to host numpy arrays."""
+        pos = (C.c_void_p * 3)()
+        vel = (C.c_void_p * 3)()
+        n = C.c_int64(0)
+        check(lib().psum_stepper_state(self.h, s, pos, vel, C.byref(n)), "psum_stepper_state")
+        esz = np.dtype(self.np_dtype).itemsize
+        out_p, out_v = [], []
+        check(lib().psum_stepper_sync(self.h), "psum_stepper_sync")
+        for d in range(self.dim):
+            for src, out in ((pos[d], out_p), (vel[d], out_v)):
+                a = np.empty(n.value, self.np_dtype)
+                _hip_memcpy_d2h(a, src, n.value * esz)
+                out.append(a)
+        return out_p, out_v
+
+    def stats(self):
+        cyc = C.c_int64(0)
+        rel = C.c_double(0)
+        cnt = (C.c_int64 * 4)()
+        check(lib().psum_stepper_stats(self.h, C.byref(cyc), C.byref(rel), cnt), "stats")
+        return {"cycles": cyc.value, "last_rel_res": rel.value, "oob": cnt[0],
+                "tile_fallback_blocks": cnt[1], "absorbed": cnt[2]}
+
+    def set_timing(self, on=True):
+        check(lib().psum_stepper_set_timing(self.h, int(on)), "set_timing")
+
+    def push_timing(self):
+        """(total ms, launches, particles) of the fused push kernel, HIP events."""
+        ms = C.c_double(0)
+        nl = C.c_int64(0)
+        npart = C.c_int64(0)
+        check(lib().psum_stepper_push_timing(self.h, C.byref(ms), C.byref(nl), C.byref(npart)),
+              "push_timing")
+        return ms.value, nl.value, npart.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psum_stepper_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+_hip = None
+
+
+def _hip_memcpy_d2h(dst_np, src_ptr, nbytes):
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    if nbytes == 0:
+        return
+    rc = _hip.hipMemcpy(dst_np.ctypes.data_as(C.c_void_p), src_ptr, nbytes, 2)  # DeviceToHost
+    if rc != 0:
+        raise PsumError(f"hipMemcpy failed: {rc}")

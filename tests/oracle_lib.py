@@ -1,0 +1,302 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.  The product (psum_amd) never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+
+KEY_INVALID = 0xFFFFFFFF
+
+
+class Grid(C.Structure):
+    """Same layout as orc_grid / psum_grid."""
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("cells", C.c_int32 * 3),
+        ("lb", C.c_double * 3),
+        ("ub", C.c_double * 3),
+        ("delta", C.c_double * 3),
+        ("delta_r", C.c_double * 3),
+        ("periodic", C.c_int32 * 3),
+        ("pad_", C.c_int32),
+        ("plo", C.c_double * 3),
+        ("plen", C.c_double * 3),
+    ]
+
+
+def make_grid(lb, ub, cells, periodic=None, plo=None, plen=None):
+    """simple_grid(lb, ub, cells) semantics (simple_grid.hpp:38-51)."""
+    d = len(cells)
+    g = Grid()
+    g.dim = d
+    for i in range(d):
+        g.cells[i] = int(cells[i])
+        g.lb[i] = float(lb[i])
+        g.ub[i] = float(ub[i])
+        delta = (float(ub[i]) - float(lb[i])) / int(cells[i])
+        g.delta[i] = delta
+        g.delta_r[i] = 1.0 / delta
+        g.periodic[i] = int(periodic[i]) if periodic else 0
+        g.plo[i] = float(plo[i]) if plo else 0.0
+        g.plen[i] = float(plen[i]) if plen else 0.0
+    return g
+
+
+def periodic_grid(n, dx=1.0):
+    """Reference periodic node convention (TSI/main.cpp:94): nodes at
+    dx/2 + i*dx, i < n per dimension, particle domain [0, n*dx)."""
+    d = len(n)
+    return make_grid([dx / 2] * d, [ni * dx - dx / 2 for ni in n], [ni - 1 for ni in n],
+                     periodic=[1] * d, plo=[0.0] * d, plen=[ni * dx for ni in n])
+
+
+def wall_grid(cells, dx=1.0):
+    d = len(cells)
+    return make_grid([0.0] * d, [c * dx for c in cells], cells, periodic=[0] * d)
+
+
+def num_nodes(g):
+    n = 1
+    for i in range(g.dim):
+        n *= g.cells[i] + 1
+    return n
+
+
+def node_shape(g):
+    return tuple(g.cells[i] + 1 for i in range(g.dim))
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        i64 = C.c_int64
+        L.orc_cell_keys_f64.argtypes = [C.POINTER(Grid), P, i64, P]
+        L.orc_cell_keys_f32.argtypes = [C.POINTER(Grid), P, i64, P]
+        L.orc_stable_sort.argtypes = [P, i64, C.c_uint32, P]
+        L.orc_cic_weights.argtypes = [C.POINTER(Grid), P, P, P]
+        L.orc_cic_grad_weights.argtypes = [C.POINTER(Grid), P, P, P]
+        L.orc_num_nodes.argtypes = [C.POINTER(Grid)]
+        L.orc_num_nodes.restype = i64
+        for nm in ("orc_deposit_fx_f64", "orc_deposit_fx_f32"):
+            getattr(L, nm).argtypes = [C.POINTER(Grid), P, i64, C.c_double, C.c_double, P]
+            getattr(L, nm).restype = i64
+        L.orc_deposit_ref_f64.argtypes = [C.POINTER(Grid), P, i64, C.c_double, P]
+        L.orc_deposit_ref_f64.restype = i64
+        L.orc_gather_push_f64.argtypes = [C.POINTER(Grid), P, P, i64, P, C.c_double, C.c_double,
+                                          C.c_double, P]
+        L.orc_gather_push_f64.restype = i64
+        L.orc_gather_push_f32.argtypes = [C.POINTER(Grid), P, P, i64, P, C.c_float, C.c_float,
+                                          C.c_double, P]
+        L.orc_gather_push_f32.restype = i64
+        for nm in ("orc_compact_f64", "orc_compact_f32"):
+            getattr(L, nm).argtypes = [C.c_int, P, P, i64, P, P]
+            getattr(L, nm).restype = i64
+        L.orc_poisson_coo.argtypes = [C.POINTER(Grid), C.c_double, P, P, P, P]
+        L.orc_poisson_coo.restype = i64
+        L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int]
+        L.orc_mg_create.restype = P
+        L.orc_mg_levels.argtypes = [P, P]
+        L.orc_mg_levels.restype = C.c_int
+        L.orc_mg_vcycles.argtypes = [P, P, P, C.c_int]
+        L.orc_mg_solve_mixed.argtypes = [P, P, P, C.c_int, C.c_int]
+        L.orc_mg_residual_norm.argtypes = [P, P, P]
+        L.orc_mg_residual_norm.restype = C.c_double
+        L.orc_mg_free.argtypes = [P]
+        L.orc_field_energy.argtypes = [C.POINTER(Grid), P, C.c_double]
+        L.orc_field_energy.restype = C.c_double
+        L.orc_pic_run_f64.argtypes = [C.POINTER(Grid), P, P, i64, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                      P, P, P]
+        L.orc_pic_run_f64.restype = i64
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ptr_array(arrs):
+    """C array of pointers to the given numpy arrays (kept alive by caller)."""
+    arr = (C.c_void_p * 3)()
+    for i, a in enumerate(arrs):
+        arr[i] = a.ctypes.data
+    return arr
+
+
+def cell_keys(g, pos):
+    pos = [np.ascontiguousarray(p) for p in pos]
+    n = pos[0].shape[0]
+    keys = np.empty(n, np.uint32)
+    f = lib().orc_cell_keys_f32 if pos[0].dtype == np.float32 else lib().orc_cell_keys_f64
+    f(C.byref(g), C.cast(_ptr_array(pos), C.c_void_p), n, _ptr(keys))
+    return keys
+
+
+def stable_sort(keys, nkeys):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    perm = np.empty(keys.shape[0], np.uint32)
+    lib().orc_stable_sort(_ptr(keys), keys.shape[0], nkeys, _ptr(perm))
+    return perm
+
+
+def cic_weights(g, x):
+    x = np.ascontiguousarray(x, np.float64)
+    idx = np.zeros(8, np.int64)
+    coef = np.zeros(8, np.float64)
+    ok = lib().orc_cic_weights(C.byref(g), _ptr(x), _ptr(idx), _ptr(coef))
+    s = 1 << g.dim
+    return bool(ok), idx[:s], coef[:s]
+
+
+def cic_grad_weights(g, x):
+    x = np.ascontiguousarray(x, np.float64)
+    idx = np.zeros(8, np.int64)
+    gc = np.zeros(24, np.float64)
+    ok = lib().orc_cic_grad_weights(C.byref(g), _ptr(x), _ptr(idx), _ptr(gc))
+    s = 1 << g.dim
+    return bool(ok), idx[:s], gc.reshape(3, 8)[:g.dim, :s]
+
+
+def fx_shift(w):
+    """Deposit fixed-point exponent: S = 40 - e with 2^(e-1) <= |w| < 2^e."""
+    _, e = np.frexp(abs(w))
+    return 40 - int(e)
+
+
+def deposit_fx(g, pos, w, scale=None):
+    pos = [np.ascontiguousarray(p) for p in pos]
+    if scale is None:
+        scale = float(np.ldexp(1.0, fx_shift(w)))
+    acc = np.zeros(num_nodes(g), np.int64)
+    f = lib().orc_deposit_fx_f32 if pos[0].dtype == np.float32 else lib().orc_deposit_fx_f64
+    skipped = f(C.byref(g), C.cast(_ptr_array(pos), C.c_void_p), pos[0].shape[0], w, scale,
+                _ptr(acc))
+    return acc, scale, skipped
+
+
+def deposit_ref(g, pos, w):
+    pos = [np.ascontiguousarray(p, np.float64) for p in pos]
+    rho = np.zeros(num_nodes(g), np.float64)
+    lib().orc_deposit_ref_f64(C.byref(g), C.cast(_ptr_array(pos), C.c_void_p), pos[0].shape[0],
+                              w, _ptr(rho))
+    return rho
+
+
+def gather_push(g, pos, vel, phi, qm, dt, mass=1.0):
+    """In place on pos/vel (lists of contiguous arrays).  Returns (absorbed, diag)."""
+    diag = np.zeros(4, np.float64)
+    pa, va = _ptr_array(pos), _ptr_array(vel)
+    phi = np.ascontiguousarray(phi)
+    if pos[0].dtype == np.float32:
+        absorbed = lib().orc_gather_push_f32(C.byref(g), C.cast(pa, C.c_void_p),
+                                             C.cast(va, C.c_void_p), pos[0].shape[0],
+                                             _ptr(phi), qm, dt, mass, _ptr(diag))
+    else:
+        absorbed = lib().orc_gather_push_f64(C.byref(g), C.cast(pa, C.c_void_p),
+                                             C.cast(va, C.c_void_p), pos[0].shape[0],
+                                             _ptr(phi), qm, dt, mass, _ptr(diag))
+    return absorbed, diag
+
+
+def compact(pos, vel):
+    d = len(pos)
+    dt = pos[0].dtype
+    n = pos[0].shape[0]
+    op = [np.empty(n, dt) for _ in range(d)]
+    ov = [np.empty(n, dt) for _ in range(d)]
+    f = lib().orc_compact_f32 if dt == np.float32 else lib().orc_compact_f64
+    m = f(d, C.cast(_ptr_array(pos), C.c_void_p), C.cast(_ptr_array(vel), C.c_void_p), n,
+          C.cast(_ptr_array(op), C.c_void_p), C.cast(_ptr_array(ov), C.c_void_p))
+    return [a[:m].copy() for a in op], [a[:m].copy() for a in ov]
+
+
+def poisson_coo(g, eps, fixed):
+    fixed = np.ascontiguousarray(fixed, np.uint8)
+    nnz = lib().orc_poisson_coo(C.byref(g), eps, _ptr(fixed), None, None, None)
+    r = np.empty(nnz, np.uint64)
+    c = np.empty(nnz, np.uint64)
+    v = np.empty(nnz, np.float64)
+    lib().orc_poisson_coo(C.byref(g), eps, _ptr(fixed), _ptr(r), _ptr(c), _ptr(v))
+    return r, c, v
+
+
+class MG:
+    """Reference-restated Galerkin multigrid (FP64 and the 32f mixed form)."""
+
+    def __init__(self, n, rows, cols, vals, I, J, K=1, depth=3, pre=2, post=1):
+        self.rows = np.ascontiguousarray(rows, np.uint64)
+        self.cols = np.ascontiguousarray(cols, np.uint64)
+        self.vals = np.ascontiguousarray(vals, np.float64)
+        self.n = n
+        self.h = lib().orc_mg_create(n, self.vals.shape[0], _ptr(self.rows), _ptr(self.cols),
+                                     _ptr(self.vals), I, J, K, depth, pre, post)
+        if not self.h:
+            raise ValueError("orc_mg_create failed")
+
+    def levels(self):
+        sizes = np.zeros(64, np.int64)
+        nl = lib().orc_mg_levels(self.h, _ptr(sizes))
+        return list(sizes[:nl])
+
+    def vcycles(self, b, x, cycles):
+        b = np.ascontiguousarray(b, np.float64)
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        lib().orc_mg_vcycles(self.h, _ptr(b), _ptr(x), cycles)
+        return x
+
+    def solve_mixed(self, b, x, max_cycles, corr_interval=4):
+        b = np.ascontiguousarray(b, np.float64)
+        lib().orc_mg_solve_mixed(self.h, _ptr(b), _ptr(x), max_cycles, corr_interval)
+        return x
+
+    def residual_norm(self, b, x):
+        return lib().orc_mg_residual_norm(self.h, _ptr(np.ascontiguousarray(b, np.float64)),
+                                          _ptr(np.ascontiguousarray(x, np.float64)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mg_free(self.h)
+            self.h = None
+
+
+def field_energy(g, phi, eps=1.0):
+    phi = np.ascontiguousarray(phi, np.float64)
+    return lib().orc_field_energy(C.byref(g), _ptr(phi), eps)
+
+
+def pic_run(g, pos, vel, q, m, w, rho_bg, eps, dt, steps, depth, pre=2, post=1, tol=1e-10,
+            max_cycles=20, dep_mode=0, fixed=None):
+    """In place on pos/vel (float64).  Returns (hist[steps,5], phi, total_cycles)."""
+    hist = np.zeros((steps, 5), np.float64)
+    phi = np.zeros(num_nodes(g), np.float64)
+    pa, va = _ptr_array(pos), _ptr_array(vel)
+    fx = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+    cyc = lib().orc_pic_run_f64(C.byref(g), C.cast(pa, C.c_void_p), C.cast(va, C.c_void_p),
+                                pos[0].shape[0], q, m, w, rho_bg, eps, dt, steps, depth, pre, post,
+                                tol, max_cycles, dep_mode, None if fx is None else _ptr(fx),
+                                _ptr(hist), _ptr(phi))
+    if cyc < 0:
+        raise RuntimeError("orc_pic_run_f64 failed")
+    return hist, phi, cyc
