@@ -270,6 +270,19 @@ static csr csr_mul(const csr* a, const csr* b)
     return c;
 }
 
+static csr csr_copy(const csr* a)
+{
+    csr c; c.n = a->n; c.m = a->m;
+    const int64_t nnz = a->rp[a->n];
+    c.rp = (int64_t*)malloc(sizeof(int64_t) * (size_t)(a->n + 1));
+    c.ci = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
+    c.v = (double*)malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    memcpy(c.rp, a->rp, sizeof(int64_t) * (size_t)(a->n + 1));
+    memcpy(c.ci, a->ci, sizeof(int64_t) * (size_t)nnz);
+    memcpy(c.v, a->v, sizeof(double) * (size_t)nnz);
+    return c;
+}
+
 static double csr_coeff(const csr* a, int64_t i, int64_t j)
 {
     for (int64_t q = a->rp[i]; q < a->rp[i + 1]; ++q) if (a->ci[q] == j) return a->v[q];
@@ -311,7 +324,10 @@ static int axis_interp(double x, axis c, int64_t* ix, double* w)
 typedef struct {
     int L;                      /* number of transfers; levels 0..L */
     int pre, post;
-    csr A0;                     /* unscaled fine matrix */
+    csr A0;                     /* unscaled fine matrix as given */
+    int singular;               /* pinned-periodic matrix solved in consistent singular form */
+    int64_t pin;                /* the pinned node */
+    csr As;                     /* unscaled singular fine matrix (singular mode) */
     csr* ah;                    /* scaled level matrices, L+1 */
     csr* relax;                 /* L */
     csr* R; csr* P;             /* L */
@@ -359,35 +375,133 @@ static float* to_f32(const double* x, int64_t n)
     return y;
 }
 
+/* Axis roles in periodic-aware mode: an axis is wrapped when the matrix
+ * couples its first and last node.  A wrapped axis with an even node count
+ * >= 4 coarsens periodically (even nodes kept, odd nodes interpolated
+ * between their neighbours with wrap-around, uniform volumes); a wrapped
+ * axis that cannot be halved is kept unchanged (identity transfer).  Other
+ * axes follow the reference coarsening. */
+static void detect_wrap(const csr* a, const int* sz, int* wrapped)
+{
+    wrapped[0] = wrapped[1] = wrapped[2] = 0;
+    for (int64_t r = 0; r < a->n; ++r)
+        for (int64_t q = a->rp[r]; q < a->rp[r + 1]; ++q) {
+            int64_t x = r, y = a->ci[q];
+            int rc[3], cc[3];
+            rc[2] = (int)(x % sz[2]); x /= sz[2]; rc[1] = (int)(x % sz[1]); rc[0] = (int)(x / sz[1]);
+            cc[2] = (int)(y % sz[2]); y /= sz[2]; cc[1] = (int)(y % sz[1]); cc[0] = (int)(y / sz[1]);
+            for (int d = 0; d < 3; ++d)
+                if (sz[d] >= 3 && abs(rc[d] - cc[d]) == sz[d] - 1) wrapped[d] = 1;
+        }
+}
+
+/* A pinned-periodic matrix: every axis of size > 1 wrapped, exactly one
+ * identity row (the pinned node p), and every other row summing to zero.
+ * Its pinned solution equals the solution of the consistent singular system
+ * with row p restored (by symmetry from column p) and b'_p = b_p - sum(b),
+ * shifted so that x_p = b_p.  Returns p or -1. */
+static int64_t detect_pinned_periodic(const csr* a, const int* sz, const int* wrapped)
+{
+    for (int d = 0; d < 3; ++d) if (sz[d] > 1 && !wrapped[d]) return -1;
+    int64_t pin = -1;
+    for (int64_t r = 0; r < a->n; ++r) {
+        const int64_t len = a->rp[r + 1] - a->rp[r];
+        if (len == 1 && a->ci[a->rp[r]] == r && a->v[a->rp[r]] == 1.0) {
+            if (pin >= 0) return -1;
+            pin = r;
+            continue;
+        }
+        double s = 0.0, m = 0.0;
+        for (int64_t q = a->rp[r]; q < a->rp[r + 1]; ++q) { s += a->v[q]; m += fabs(a->v[q]); }
+        if (fabs(s) > 1e-12 * m) return -1;
+    }
+    return pin;
+}
+
+static csr restore_pinned_row(const csr* a, int64_t p)
+{
+    const int64_t n = a->n, nnz = a->rp[n];
+    uint64_t* tr = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(nnz + 16));
+    uint64_t* tc = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(nnz + 16));
+    double* tv = (double*)malloc(sizeof(double) * (size_t)(nnz + 16));
+    int64_t nt = 0;
+    double off = 0.0;
+    for (int64_t r = 0; r < n; ++r) {
+        if (r == p) continue;
+        for (int64_t q = a->rp[r]; q < a->rp[r + 1]; ++q) {
+            tr[nt] = (uint64_t)r; tc[nt] = (uint64_t)a->ci[q]; tv[nt] = a->v[q]; ++nt;
+            if (a->ci[q] == p) {       /* A symmetric: A[p][r] = A[r][p] */
+                tr[nt] = (uint64_t)p; tc[nt] = (uint64_t)r; tv[nt] = a->v[q]; ++nt;
+                off += a->v[q];
+            }
+        }
+    }
+    tr[nt] = (uint64_t)p; tc[nt] = (uint64_t)p; tv[nt] = -off; ++nt;
+    csr s = csr_from_trips(n, n, nt, tr, tc, tv);
+    free(tr); free(tc); free(tv);
+    return s;
+}
+
 void* orc_mg_create(int64_t n, int64_t nnz, const uint64_t* rows, const uint64_t* cols,
-                    const double* vals, int I, int J, int K, int depth, int pre, int post)
+                    const double* vals, int I, int J, int K, int depth, int pre, int post,
+                    int mode)
 {
     if ((int64_t)I * J * K != n || depth < 1) return NULL;
     orc_mg* h = (orc_mg*)calloc(1, sizeof(orc_mg));
     h->pre = pre; h->post = post;
     h->A0 = csr_from_trips(n, n, nnz, rows, cols, vals);
+    h->pin = -1;
+    const int sz[3] = {I, J, K};
+    int wrapped[3] = {0, 0, 0};
+    if (mode) {
+        detect_wrap(&h->A0, sz, wrapped);
+        h->pin = detect_pinned_periodic(&h->A0, sz, wrapped);
+        if (h->pin >= 0) {
+            h->singular = 1;
+            h->As = restore_pinned_row(&h->A0, h->pin);
+        }
+    }
 
     /* transfers */
     csr Rs[32], Ps[32];
     int L = 0;
     axis ax[3];
-    const int sz[3] = {I, J, K};
+    int per[3];
     for (int d = 0; d < 3; ++d) {
         ax[d].n = sz[d]; ax[d].x = (double*)malloc(sizeof(double) * (size_t)sz[d]);
         for (int i = 0; i < sz[d]; ++i) ax[d].x[i] = i;
+        per[d] = mode && wrapped[d];
     }
     for (int level = 0; level < depth && level < 32; ++level) {
         axis co[3];
         double* vol[3];
         int64_t* wi[3]; double* ww[3]; int* wn[3];
         for (int d = 0; d < 3; ++d) {
-            co[d] = axis_coarsen(ax[d]);
-            vol[d] = (double*)malloc(sizeof(double) * (size_t)ax[d].n);
-            axis_volumes(ax[d], vol[d]);
-            wi[d] = (int64_t*)malloc(sizeof(int64_t) * 2 * (size_t)ax[d].n);
-            ww[d] = (double*)malloc(sizeof(double) * 2 * (size_t)ax[d].n);
-            wn[d] = (int*)malloc(sizeof(int) * (size_t)ax[d].n);
-            for (int64_t i = 0; i < ax[d].n; ++i) wn[d][i] = axis_interp(ax[d].x[i], co[d], wi[d] + 2 * i, ww[d] + 2 * i);
+            const int64_t nf = ax[d].n;
+            vol[d] = (double*)malloc(sizeof(double) * (size_t)nf);
+            wi[d] = (int64_t*)malloc(sizeof(int64_t) * 2 * (size_t)nf);
+            ww[d] = (double*)malloc(sizeof(double) * 2 * (size_t)nf);
+            wn[d] = (int*)malloc(sizeof(int) * (size_t)nf);
+            if (per[d]) {
+                const int halve = (nf % 2 == 0) && nf >= 4;
+                const int64_t nc = halve ? nf / 2 : nf;
+                co[d].n = nc;
+                co[d].x = (double*)malloc(sizeof(double) * (size_t)nc);
+                for (int64_t i = 0; i < nc; ++i) co[d].x[i] = halve ? ax[d].x[2 * i] : ax[d].x[i];
+                for (int64_t i = 0; i < nf; ++i) {
+                    vol[d][i] = 1.0;
+                    if (!halve || i % 2 == 0) { wn[d][i] = 1; wi[d][2 * i] = halve ? i / 2 : i; ww[d][2 * i] = 1.0; }
+                    else {
+                        wn[d][i] = 2;
+                        wi[d][2 * i] = (i - 1) / 2; ww[d][2 * i] = 0.5;
+                        wi[d][2 * i + 1] = ((i + 1) / 2) % nc; ww[d][2 * i + 1] = 0.5;
+                    }
+                }
+            } else {
+                co[d] = axis_coarsen(ax[d]);
+                axis_volumes(ax[d], vol[d]);
+                for (int64_t i = 0; i < nf; ++i) wn[d][i] = axis_interp(ax[d].x[i], co[d], wi[d] + 2 * i, ww[d] + 2 * i);
+            }
         }
         const int64_t ni = ax[0].n, nj = ax[1].n, nk = ax[2].n;
         const int64_t ci = co[0].n, cj = co[1].n, ck = co[2].n;
@@ -436,7 +550,11 @@ void* orc_mg_create(int64_t n, int64_t nnz, const uint64_t* rows, const uint64_t
     for (int l = 0; l <= L; ++l) {
         csr a;
         if (l == 0) {
-            a = csr_from_trips(n, n, nnz, rows, cols, vals);
+            if (h->singular) {
+                a = csr_copy(&h->As);
+            } else {
+                a = csr_from_trips(n, n, nnz, rows, cols, vals);
+            }
         } else {
             csr ra = csr_mul(&h->R[l - 1], &h->ah[l - 1]);
             a = csr_mul(&ra, &h->P[l - 1]);
@@ -450,7 +568,26 @@ void* orc_mg_create(int64_t n, int64_t nnz, const uint64_t* rows, const uint64_t
     }
     h->nc = h->ah[L].n;
     h->cinv = (double*)malloc(sizeof(double) * (size_t)(h->nc * h->nc));
-    dense_inverse(&h->ah[L], h->cinv);
+    if (h->singular) {
+        /* singular coarsest operator: pin coarse node 0 (identity row, b_0
+         * ignored), i.e. invert with row 0 replaced and drop column 0 */
+        csr c = h->ah[L];
+        csr pinned; pinned.n = c.n; pinned.m = c.m;
+        pinned.rp = (int64_t*)calloc((size_t)c.n + 1, sizeof(int64_t));
+        pinned.ci = (int64_t*)malloc(sizeof(int64_t) * (size_t)(c.rp[c.n] + 1));
+        pinned.v = (double*)malloc(sizeof(double) * (size_t)(c.rp[c.n] + 1));
+        int64_t o = 0;
+        for (int64_t i = 0; i < c.n; ++i) {
+            if (i == 0) { pinned.ci[o] = 0; pinned.v[o] = 1.0; ++o; }
+            else for (int64_t q = c.rp[i]; q < c.rp[i + 1]; ++q) { pinned.ci[o] = c.ci[q]; pinned.v[o] = c.v[q]; ++o; }
+            pinned.rp[i + 1] = o;
+        }
+        dense_inverse(&pinned, h->cinv);
+        csr_free(&pinned);
+        for (int64_t i = 0; i < h->nc; ++i) h->cinv[i * h->nc] = 0.0;
+    } else {
+        dense_inverse(&h->ah[L], h->cinv);
+    }
     h->relax = (csr*)calloc((size_t)L, sizeof(csr));
     for (int l = 0; l < L; ++l) {
         /* relax = I - A*2/3 on the union pattern (diagonal always present) */
@@ -512,9 +649,42 @@ int orc_mg_levels(void* hv, int64_t* sizes)
 #undef MG_T
 #undef MG_SUF
 
+/* singular mode: b'_p = b_p - sum(b) (sequential sum) */
+static double* singular_rhs(const orc_mg* h, const double* b)
+{
+    const int64_t n = h->A0.n;
+    double* bp = (double*)malloc(sizeof(double) * (size_t)n);
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) { bp[i] = b[i]; s += b[i]; }
+    bp[h->pin] = b[h->pin] - s;
+    return bp;
+}
+
+/* gauge: x -= x_p - b_p, so the pinned row holds exactly */
+static void gauge_fix(const orc_mg* h, const double* b, double* x)
+{
+    const double shift = x[h->pin] - b[h->pin];
+    for (int64_t i = 0; i < h->A0.n; ++i) x[i] = x[i] - shift;
+}
+
 void orc_mg_vcycles(void* hv, const double* b, double* x, int cycles)
 {
-    vcycle_f64((orc_mg*)hv, b, x, cycles);
+    orc_mg* h = (orc_mg*)hv;
+    if (!h->singular) {
+        vcycle_f64(h, b, x, cycles);
+        return;
+    }
+    double* bp = singular_rhs(h, b);
+    vcycle_f64(h, bp, x, cycles);
+    free(bp);
+    gauge_fix(h, b, x);
+}
+
+int orc_mg_is_singular(void* hv, int64_t* pin)
+{
+    orc_mg* h = (orc_mg*)hv;
+    if (pin) *pin = h->pin;
+    return h->singular;
 }
 
 double orc_mg_residual_norm(void* hv, const double* b, const double* x)
@@ -536,15 +706,19 @@ void orc_mg_solve_mixed(void* hv, const double* b, double* x, int max_cycles, in
     double* ax = (double*)malloc(sizeof(double) * (size_t)n);
     float* db = (float*)malloc(sizeof(float) * (size_t)n);
     float* dx = (float*)malloc(sizeof(float) * (size_t)n);
+    const csr* A = h->singular ? &h->As : &h->A0;
+    double* bp = h->singular ? singular_rhs(h, b) : NULL;
+    const double* bb = h->singular ? bp : b;
     for (int cycle = 0; cycle < max_cycles;) {
         const int batch = corr_interval < max_cycles - cycle ? corr_interval : max_cycles - cycle;
-        csr_spmv(&h->A0, x, ax);
-        for (int64_t i = 0; i < n; ++i) db[i] = (float)(b[i] - ax[i]);
+        csr_spmv(A, x, ax);
+        for (int64_t i = 0; i < n; ++i) db[i] = (float)(bb[i] - ax[i]);
         memset(dx, 0, sizeof(float) * (size_t)n);
         vcycle_f32(h, db, dx, batch);
         for (int64_t i = 0; i < n; ++i) x[i] += (double)dx[i];
         cycle += batch;
     }
+    if (h->singular) { gauge_fix(h, b, x); free(bp); }
     free(ax); free(db); free(dx);
 }
 
@@ -558,6 +732,7 @@ void orc_mg_free(void* hv)
         free(h->relax_v32[l]); free(h->R_v32[l]); free(h->P_v32[l]);
     }
     csr_free(&h->A0);
+    if (h->singular) csr_free(&h->As);
     free(h->ah); free(h->dinv); free(h->relax); free(h->R); free(h->P);
     free(h->ah_v32); free(h->dinv32); free(h->relax_v32); free(h->R_v32); free(h->P_v32);
     free(h->cinv); free(h->cinv32);

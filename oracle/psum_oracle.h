@@ -118,9 +118,20 @@ int64_t orc_poisson_coo(const orc_grid* g, double eps, const uint8_t* fixed,
 
 /* a9-a16: geometric-transfer Galerkin multigrid restated with CSR matrices
  * (multigrid_utilities.hpp:55-104, multigrid_core.hpp:49-272,
- *  sycl_multigrid_gpu.cpp:93-132, sycl_multigrid32f_gpu.cpp:106-160). */
+ *  sycl_multigrid_gpu.cpp:93-132, sycl_multigrid32f_gpu.cpp:106-160).
+ * mode 0: the reference algorithm exactly.
+ * mode 1: periodic-aware (the build's backends): axes the matrix wraps
+ *   coarsen periodically (halving while even and >= 4, wrap-around linear
+ *   interpolation, uniform volumes), and a pinned-periodic matrix (all axes
+ *   wrapped, one identity row p, zero row sums elsewhere) is solved as the
+ *   equivalent consistent singular system (row p restored from column p,
+ *   b'_p = b_p - sum b, coarsest level pinned at node 0) followed by the
+ *   gauge shift x -= x_p - b_p.  The reference algorithm stagnates or
+ *   diverges on such matrices; mode 1 converges to the pinned solution. */
 void* orc_mg_create(int64_t n, int64_t nnz, const uint64_t* rows, const uint64_t* cols,
-                    const double* vals, int I, int J, int K, int depth, int pre, int post);
+                    const double* vals, int I, int J, int K, int depth, int pre, int post,
+                    int mode);
+int orc_mg_is_singular(void* h, int64_t* pin);
 int orc_mg_levels(void* h, int64_t* sizes);
 /* x is the warm start and the result; runs exactly `cycles` V-cycles. */
 void orc_mg_vcycles(void* h, const double* b, double* x, int cycles);

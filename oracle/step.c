@@ -39,7 +39,7 @@ int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n
     double* v = (double*)malloc(sizeof(double) * (size_t)nnz);
     orc_poisson_coo(g, eps, fx, r, c, v);
     const int I = g->cells[0] + 1, J = D > 1 ? g->cells[1] + 1 : 1, K = D > 2 ? g->cells[2] + 1 : 1;
-    void* mg = orc_mg_create(nn, nnz, r, c, v, I, J, K, depth, pre, post);
+    void* mg = orc_mg_create(nn, nnz, r, c, v, I, J, K, depth, pre, post, 1);
     free(r); free(c); free(v);
     if (!mg) { free(fx); return -1; }
 

@@ -96,7 +96,14 @@ struct DevTransfer {
     const int* rf[3];     // coarse c -> first fine index
     const int* rn[3];     // coarse c -> number of fine nodes (1..3)
     const T* rw[3];       // coarse c -> weights (3 per c)
+    int wrap[3];          // periodic axis: indices wrap modulo the axis length
 };
+
+__device__ __forceinline__ int wrap_index(int i, int n, int wrap)
+{
+    if (!wrap) return i;
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
 
 template <int D>
 __device__ __forceinline__ void node_coords(const DevLevel& L, int64_t i, int* c)
@@ -204,21 +211,25 @@ __global__ __launch_bounds__(kMgThreads) void k_restrict(DevLevel F, DevLevel Cl
     if (D == 2) {
         for (int a = 0; a < m0; ++a) {
             const T w0 = tr.rw[0][3 * c[0] + a];
+            const int i0 = wrap_index(f0 + a, F.n[0], tr.wrap[0]);
             for (int b = 0; b < m1; ++b) {
                 const T w = w0 * tr.rw[1][3 * c[1] + b];
-                s = s + w * r[(int64_t)(f0 + a) * F.n[1] + (f1 + b)];
+                const int i1 = wrap_index(f1 + b, F.n[1], tr.wrap[1]);
+                s = s + w * r[(int64_t)i0 * F.n[1] + i1];
             }
         }
     } else {
         const int f2 = tr.rf[2][c[2]], m2 = tr.rn[2][c[2]];
         for (int a = 0; a < m0; ++a) {
             const T w0 = tr.rw[0][3 * c[0] + a];
+            const int i0 = wrap_index(f0 + a, F.n[0], tr.wrap[0]);
             for (int b = 0; b < m1; ++b) {
                 const T w01 = w0 * tr.rw[1][3 * c[1] + b];
-                const int64_t row = ((int64_t)(f0 + a) * F.n[1] + (f1 + b)) * F.n[2];
+                const int i1 = wrap_index(f1 + b, F.n[1], tr.wrap[1]);
+                const int64_t row = ((int64_t)i0 * F.n[1] + i1) * F.n[2];
                 for (int e = 0; e < m2; ++e) {
                     const T w = w01 * tr.rw[2][3 * c[2] + e];
-                    s = s + w * r[row + f2 + e];
+                    s = s + w * r[row + wrap_index(f2 + e, F.n[2], tr.wrap[2])];
                 }
             }
         }
@@ -245,9 +256,11 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
     if (D == 2) {
         for (int a = 0; a < 2; ++a) {
             if (w0[a] == (T)0) continue;
+            const int i0 = wrap_index(p0 + a, Cl.n[0], tr.wrap[0]);
             for (int b = 0; b < 2; ++b) {
                 if (w1[b] == (T)0) continue;
-                s = s + (w0[a] * w1[b]) * xc[(int64_t)(p0 + a) * Cl.n[1] + (p1 + b)];
+                const int i1 = wrap_index(p1 + b, Cl.n[1], tr.wrap[1]);
+                s = s + (w0[a] * w1[b]) * xc[(int64_t)i0 * Cl.n[1] + i1];
             }
         }
     } else {
@@ -255,13 +268,15 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
         const T* w2 = tr.pw[2] + 2 * c[2];
         for (int a = 0; a < 2; ++a) {
             if (w0[a] == (T)0) continue;
+            const int i0 = wrap_index(p0 + a, Cl.n[0], tr.wrap[0]);
             for (int b = 0; b < 2; ++b) {
                 if (w1[b] == (T)0) continue;
                 const T w01 = w0[a] * w1[b];
+                const int i1 = wrap_index(p1 + b, Cl.n[1], tr.wrap[1]);
                 for (int e = 0; e < 2; ++e) {
                     if (w2[e] == (T)0) continue;
-                    s = s + (w01 * w2[e]) *
-                                xc[((int64_t)(p0 + a) * Cl.n[1] + (p1 + b)) * Cl.n[2] + (p2 + e)];
+                    const int i2 = wrap_index(p2 + e, Cl.n[2], tr.wrap[2]);
+                    s = s + (w01 * w2[e]) * xc[((int64_t)i0 * Cl.n[1] + i1) * Cl.n[2] + i2];
                 }
             }
         }
@@ -297,10 +312,14 @@ __global__ __launch_bounds__(kMgThreads) void k_coarse_solve(const double* __res
 // FP64 residual of the unscaled operator: r = b - A x, partial sums of r^2
 // and of b^2 per block; when OUTF32 also emit (float)r o D0^-1 (the mixed
 // backend's residual_to + hadamard_to).
+// FP64 residual r = b - A x of the unscaled level-0 operator; partial sums
+// of r^2 (skipping row `pin` of a pinned-periodic system, which the gauge
+// shift satisfies exactly) and of bn^2 (the caller's right-hand side).
 template <int D, bool COMPACT, bool OUTF32>
 __global__ __launch_bounds__(kMgThreads) void k_outer_residual(
     DevLevel L, DevTable<double> K, const double* __restrict__ x, const double* __restrict__ b,
-    const float* __restrict__ dinv32, float* __restrict__ b0, double* __restrict__ part)
+    const double* __restrict__ bn, int64_t pin, const float* __restrict__ dinv32,
+    float* __restrict__ b0, double* __restrict__ part)
 {
     __shared__ double ws[kMgThreads / 64][2];
     const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
@@ -311,8 +330,8 @@ __global__ __launch_bounds__(kMgThreads) void k_outer_residual(
         const double* coef = K.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
         const double ax = stencil_apply<double, D, COMPACT>(L, coef, x, c);
         const double r = b[i] - ax;
-        rr = r * r;
-        bb = b[i] * b[i];
+        rr = i == pin ? 0.0 : r * r;
+        bb = bn[i] * bn[i];
         if (OUTF32) b0[i] = (float)r * dinv32[L.cls[i]];
     }
     rr = wave_sum(rr);
@@ -349,6 +368,58 @@ __global__ __launch_bounds__(kMgThreads) void k_reduce2(const double* __restrict
         __syncthreads();
     }
     if (threadIdx.x < 2) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// per-block sums of x (part[b]), then a fixed-order single-block total
+__global__ __launch_bounds__(kMgThreads) void k_psum(const double* __restrict__ x, int64_t n,
+                                                     double* __restrict__ part)
+{
+    __shared__ double ws[kMgThreads / 64];
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    double v = wave_sum(i < n ? x[i] : 0.0);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kMgThreads / 64; ++w) s += ws[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_fsum(const double* __restrict__ part, int64_t nb,
+                                                     double* __restrict__ out)
+{
+    __shared__ double sh[kMgThreads];
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += kMgThreads) s += part[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kMgThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// b'_p = b_p - sum(b): the consistent right-hand side of the singular system
+__global__ void k_set_pin(double* __restrict__ bp, const double* __restrict__ b,
+                          const double* __restrict__ sum, int64_t p)
+{
+    if (threadIdx.x == 0) bp[p] = b[p] - sum[0];
+}
+
+// gauge: shift = x_p - b_p ; x -= shift
+__global__ void k_gauge_read(const double* __restrict__ x, const double* __restrict__ b, int64_t p,
+                             double* __restrict__ shift)
+{
+    if (threadIdx.x == 0) shift[0] = x[p] - b[p];
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_gauge_apply(double* __restrict__ x, int64_t n,
+                                                            const double* __restrict__ shift)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i < n) x[i] = x[i] - shift[0];
 }
 
 __global__ __launch_bounds__(kMgThreads) void k_zero_f32(float* __restrict__ x, int64_t n)
@@ -411,7 +482,52 @@ struct AxisTransfer {
     std::vector<double> pw;     // 2 per fine f (w_lo, w_hi); exact node: (1, 0)
     std::vector<int> rf, rn;    // per coarse c
     std::vector<double> rw;     // 3 per coarse c
+    int wrap = 0;               // periodic axis: fine/coarse indices wrap
 };
+
+// Periodic axis of even length nf >= 4: coarse node c is fine node 2c; an odd
+// fine node interpolates its two coarse neighbours (wrapping); volumes are
+// uniform, so R = P^T / 2 = {1/4, 1/2, 1/4} around 2c.
+AxisTransfer axis_transfer_periodic(int nf)
+{
+    AxisTransfer t;
+    const int nc = nf / 2;
+    t.wrap = 1;
+    t.pc.resize(nf);
+    t.pw.resize(2 * nf);
+    for (int i = 0; i < nf; ++i) {
+        if (i % 2 == 0) { t.pc[i] = i / 2; t.pw[2 * i] = 1.0; t.pw[2 * i + 1] = 0.0; }
+        else { t.pc[i] = (i - 1) / 2; t.pw[2 * i] = 0.5; t.pw[2 * i + 1] = 0.5; }
+    }
+    t.rf.resize(nc);
+    t.rn.assign(nc, 3);
+    t.rw.resize(3 * nc);
+    for (int c = 0; c < nc; ++c) {
+        t.rf[c] = 2 * c - 1;
+        t.rw[3 * c] = 0.25;
+        t.rw[3 * c + 1] = 0.5;
+        t.rw[3 * c + 2] = 0.25;
+    }
+    return t;
+}
+
+AxisTransfer axis_transfer_identity(int n, int wrap)
+{
+    AxisTransfer t;
+    t.wrap = wrap;
+    t.pc.resize(n);
+    t.pw.assign(2 * n, 0.0);
+    t.rf.resize(n);
+    t.rn.assign(n, 1);
+    t.rw.assign(3 * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        t.pc[i] = i;
+        t.pw[2 * i] = 1.0;
+        t.rf[i] = i;
+        t.rw[3 * i] = 1.0;
+    }
+    return t;
+}
 
 AxisTransfer axis_transfer(const Axis& f, const Axis& c)
 {
@@ -547,9 +663,68 @@ bool axis_offset(int rc, int cc, int n, int& off, int& wrapped)
     return false;
 }
 
+inline int dims_n(const HostLevel& L, int d) { return L.n[d]; }
+
+// Pinned-periodic detection on the level-0 rows (N x KS): every axis of
+// size > 1 wrapped, exactly one identity row p, zero row sums elsewhere.
+// Returns p, or -1.
+int64_t detect_pinned_periodic(const HostLevel& L, const std::vector<double>& R)
+{
+    for (int d = 0; d < 3; ++d)
+        if (L.n[d] > 1 && !L.wrap[d]) return -1;
+    const int KS = L.KS, center = (KS - 1) / 2;
+    int64_t pin = -1;
+    for (int64_t i = 0; i < L.N; ++i) {
+        const double* r = &R[(size_t)i * KS];
+        bool ident = r[center] == 1.0;
+        double s = 0.0, m = 0.0;
+        for (int k = 0; k < KS; ++k) {
+            if (k != center && r[k] != 0.0) ident = false;
+            s += r[k];
+            m += std::fabs(r[k]);
+        }
+        if (ident) {
+            if (pin >= 0) return -1;
+            pin = i;
+        } else if (std::fabs(s) > 1e-12 * m) {
+            return -1;
+        }
+    }
+    return pin;
+}
+
+// Row p of the singular periodic operator from column p (the FD matrix is
+// symmetric): A[p][p+o] = A[p+o][p]; the diagonal closes the zero row sum.
+void restore_pinned_row(const HostLevel& L, std::vector<double>& R, int64_t p)
+{
+    const int KS = L.KS, center = (KS - 1) / 2, D = L.D;
+    int c[3];
+    int64_t t = p;
+    c[2] = (int)(t % L.n[2]); t /= L.n[2]; c[1] = (int)(t % L.n[1]); c[0] = (int)(t / L.n[1]);
+    double off = 0.0;
+    for (int k = 0; k < KS; ++k) {
+        if (k == center) continue;
+        int o[3] = {0, 0, 0};
+        if (D == 2) { o[0] = k / 3 - 1; o[1] = k % 3 - 1; }
+        else { o[0] = k / 9 - 1; o[1] = (k / 3) % 3 - 1; o[2] = k % 3 - 1; }
+        int g[3];
+        for (int d = 0; d < 3; ++d) {
+            g[d] = c[d] + o[d];
+            if (g[d] < 0) g[d] += L.n[d];
+            else if (g[d] >= L.n[d]) g[d] -= L.n[d];
+        }
+        const int64_t j = ((int64_t)g[0] * L.n[1] + g[1]) * L.n[2] + g[2];
+        const double v = R[(size_t)j * KS + (KS - 1 - k)];
+        R[(size_t)p * KS + k] = v;
+        off += v;
+    }
+    R[(size_t)p * KS + center] = -off;
+}
+
 HostLevel level_from_coo(int I, int J, int K, unsigned long long n_row, unsigned long long nnz,
                          const unsigned long long* rows, const unsigned long long* cols,
-                         const double* vals)
+                         const double* vals, bool periodic_aware, int64_t* pin_out,
+                         std::vector<double>* pinned_rows)
 {
     HostLevel L;
     L.D = K > 1 ? 3 : 2;
@@ -579,7 +754,15 @@ HostLevel level_from_coo(int I, int J, int K, unsigned long long n_row, unsigned
         else k = (off[0] + 1) * 9 + (off[1] + 1) * 3 + (off[2] + 1);
         R[(size_t)r * L.KS + k] += vals[q];
     }
-    for (int d = 0; d < 3; ++d) L.wrap[d] = wrapped[d];
+    for (int d = 0; d < 3; ++d) L.wrap[d] = (dims_n(L, d) >= 3) ? wrapped[d] : 0;
+    *pin_out = -1;
+    if (periodic_aware) {
+        *pin_out = detect_pinned_periodic(L, R);
+        if (*pin_out >= 0) {
+            if (pinned_rows) *pinned_rows = R;
+            restore_pinned_row(L, R, *pin_out);
+        }
+    }
     classify(L, R);
     return L;
 }
@@ -607,7 +790,9 @@ HostLevel galerkin(const HostLevel& F, const AxisTransfer* tr, const int* nc)
         for (int a = 0; a < m0; ++a)
             for (int b = 0; b < m1; ++b)
                 for (int e = 0; e < m2; ++e) {
-                    const int f[3] = {tr[0].rf[c[0]] + a, tr[1].rf[c[1]] + b, D == 3 ? tr[2].rf[c[2]] + e : 0};
+                    int f[3] = {tr[0].rf[c[0]] + a, tr[1].rf[c[1]] + b, D == 3 ? tr[2].rf[c[2]] + e : 0};
+                    for (int d = 0; d < 3; ++d)
+                        if (tr[d].wrap) f[d] = f[d] < 0 ? f[d] + F.n[d] : (f[d] >= F.n[d] ? f[d] - F.n[d] : f[d]);
                     double rw = tr[0].rw[3 * c[0] + a] * tr[1].rw[3 * c[1] + b];
                     if (D == 3) rw = rw * tr[2].rw[3 * c[2] + e];
                     const int64_t fi = ((int64_t)f[0] * F.n[1] + f[1]) * F.n[2] + f[2];
@@ -641,6 +826,8 @@ HostLevel galerkin(const HostLevel& F, const AxisTransfer* tr, const int* nc)
                                         p = p * p2;
                                         cc[2] = tr[2].pc[g[2]] + h2;
                                     }
+                                    for (int d = 0; d < 3; ++d)
+                                        if (tr[d].wrap && cc[d] == nc[d]) cc[d] = 0;
                                     int oo[3] = {0, 0, 0};
                                     for (int d = 0; d < D; ++d) {
                                         int dd = cc[d] - c[d];
@@ -665,7 +852,7 @@ HostLevel galerkin(const HostLevel& F, const AxisTransfer* tr, const int* nc)
     return C;
 }
 
-void dense_inverse(const HostLevel& L, std::vector<double>& inv)
+void dense_inverse(const HostLevel& L, std::vector<double>& inv, bool pin0)
 {
     const int64_t n = L.N;
     std::vector<double> m((size_t)(n * n), 0.0);
@@ -692,6 +879,11 @@ void dense_inverse(const HostLevel& L, std::vector<double>& inv)
         }
         inv[(size_t)(i * n + i)] = 1.0;
     }
+    if (pin0) {
+        // singular coarsest operator: pin coarse node 0 (identity row)
+        for (int64_t j = 0; j < n; ++j) m[(size_t)j] = 0.0;
+        m[0] = 1.0;
+    }
     for (int64_t col = 0; col < n; ++col) {
         int64_t p = col;
         for (int64_t r = col + 1; r < n; ++r)
@@ -717,6 +909,9 @@ void dense_inverse(const HostLevel& L, std::vector<double>& inv)
             }
         });
     }
+    // with node 0 pinned its right-hand side entry is ignored (b_0 -> 0)
+    if (pin0)
+        for (int64_t i = 0; i < n; ++i) inv[(size_t)(i * n)] = 0.0;
 }
 
 // ------------------------------------------------------------ device buffers
@@ -782,6 +977,7 @@ public:
                 if (!(iss >> v) || v < 1) throw std::invalid_argument("correction_interval must be positive");
                 corr_ = v;
             } else if (key == "tol") iss >> tol_;
+            else if (key == "periodic_mode") iss >> periodic_mode_;
             else if (key == "num_conv_mask" || key == "num_conv_mask_relax" ||
                      key == "num_conv_mask_restrict" || key == "num_conv_mask_interpolate" ||
                      key == "num_conv_mask_a") {
@@ -802,24 +998,43 @@ public:
             throw std::invalid_argument("multigrid: positive grid dimensions and depth required");
         release();
         std::vector<HostLevel> L;
-        L.push_back(level_from_coo(I_, J_, K_, n_row, nnz, rows, cols, vals));
+        int64_t pin = -1;
+        L.push_back(level_from_coo(I_, J_, K_, n_row, nnz, rows, cols, vals, periodic_mode_ != 0,
+                                   &pin, nullptr));
+        pin_ = pin;
         std::vector<std::array<AxisTransfer, 3>> T;
         Axis ax[3];
         const int sz[3] = {I_, J_, K_};
-        for (int d = 0; d < 3; ++d)
+        int per[3];
+        for (int d = 0; d < 3; ++d) {
             for (int i = 0; i < sz[d]; ++i) ax[d].x.push_back(i);
+            per[d] = periodic_mode_ != 0 && L[0].wrap[d];
+        }
         for (int level = 0; level < depth_; ++level) {
             Axis co[3];
             int nc[3];
             bool same = true;
+            std::array<AxisTransfer, 3> tr;
             for (int d = 0; d < 3; ++d) {
-                co[d] = coarsen(ax[d]);
+                const int nf = (int)ax[d].x.size();
+                if (per[d]) {
+                    // periodic axis: halve while even and >= 4, else keep
+                    if (nf % 2 == 0 && nf >= 4) {
+                        tr[d] = axis_transfer_periodic(nf);
+                        for (int i = 0; i < nf; i += 2) co[d].x.push_back(ax[d].x[i]);
+                    } else {
+                        tr[d] = axis_transfer_identity(nf, 1);
+                        co[d] = ax[d];
+                    }
+                } else {
+                    co[d] = coarsen(ax[d]);
+                }
                 nc[d] = (int)co[d].x.size();
                 if (co[d].x.size() != ax[d].x.size()) same = false;
             }
             if (same) break;   // endpoints cannot be coarsened further
-            std::array<AxisTransfer, 3> tr;
-            for (int d = 0; d < 3; ++d) tr[d] = axis_transfer(ax[d], co[d]);
+            for (int d = 0; d < 3; ++d)
+                if (!per[d]) tr[d] = axis_transfer(ax[d], co[d]);
             L.push_back(galerkin(L.back(), tr.data(), nc));
             T.push_back(tr);
             for (int d = 0; d < 3; ++d) ax[d] = co[d];
@@ -828,7 +1043,7 @@ public:
         if (L.back().N > 8192)
             throw std::runtime_error("hip_multigrid: coarse matrix too large (" + std::to_string(L.back().N) +
                                      " > 8192); increase depth");
-        dense_inverse(L.back(), cinv_host_);
+        dense_inverse(L.back(), cinv_host_, pin_ >= 0);
         host_ = std::move(L);
         tr_host_ = std::move(T);
         upload();
@@ -861,16 +1076,27 @@ public:
         last_cycles_ = 0;
         last_rel_ = NAN;
         const bool use_tol = tol_ > 0.0;
+        const int64_t N = N0();
+        const double* bs = b;   // right-hand side the V-cycles see
+        if (pin_ >= 0) {
+            // consistent singular form: b' = b, b'_p = b_p - sum(b)
+            MG_HIP(hipMemcpyAsync(bprime_, b, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_psum, dim3(nblk(N)), dim3(kMgThreads), 0, st, b, N, part_);
+            hipLaunchKernelGGL(k_fsum, dim3(1), dim3(kMgThreads), 0, st, part_, (int64_t)nblk(N), shift_);
+            hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(64), 0, st, bprime_, b, shift_, pin_);
+            MG_HIP(hipGetLastError());
+            bs = bprime_;
+        }
         if (!mixed_) {
             if (!use_tol) {
-                vcycles_f64(b, x, max_cycles_);
+                vcycles_f64(bs, x, max_cycles_);
                 last_cycles_ = max_cycles_;
             } else {
                 double rr, bb;
                 for (int c = 0; c < max_cycles_; ++c) {
-                    vcycles_f64(b, x, 1);
+                    vcycles_f64(bs, x, 1);
                     ++last_cycles_;
-                    outer_residual(b, x, false, &rr, &bb);
+                    outer_residual(bs, b, x, false, &rr, &bb);
                     last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
                     if (last_rel_ <= tol_) break;
                 }
@@ -879,22 +1105,27 @@ public:
             for (int cycle = 0; cycle < max_cycles_;) {
                 const int batch = use_tol ? 1 : std::min(corr_, max_cycles_ - cycle);
                 double rr, bb;
-                outer_residual(b, x, true, &rr, &bb);   // b0 (float) = (float)(b - A x) o D^-1
+                outer_residual(bs, b, x, true, &rr, &bb);   // b0 (float) = (float)(b - A x) o D^-1
                 if (use_tol) {
                     last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
                     if (last_rel_ <= tol_) break;
                 }
                 vcycles_f32(batch);
-                hipLaunchKernelGGL(k_add_correction, dim3(nblk(N0())), dim3(kMgThreads), 0, st, x, dev_[0].xf, N0());
+                hipLaunchKernelGGL(k_add_correction, dim3(nblk(N)), dim3(kMgThreads), 0, st, x, dev_[0].xf, N);
                 MG_HIP(hipGetLastError());
                 cycle += batch;
                 last_cycles_ += batch;
             }
             if (use_tol && !(last_rel_ <= tol_)) {
                 double rr, bb;
-                outer_residual(b, x, false, &rr, &bb);
+                outer_residual(bs, b, x, false, &rr, &bb);
                 last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
             }
+        }
+        if (pin_ >= 0) {
+            hipLaunchKernelGGL(k_gauge_read, dim3(1), dim3(64), 0, st, x, b, pin_, shift_ + 1);
+            hipLaunchKernelGGL(k_gauge_apply, dim3(nblk(N)), dim3(kMgThreads), 0, st, x, N, shift_ + 1);
+            MG_HIP(hipGetLastError());
         }
         if (sync) MG_HIP(hipStreamSynchronize(st));
     }
@@ -908,6 +1139,10 @@ private:
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     double tol_ = 0.0;
+    int periodic_mode_ = 1;
+    int64_t pin_ = -1;                 // pinned node of a pinned-periodic matrix
+    double* bprime_ = nullptr;         // consistent singular right-hand side
+    double* shift_ = nullptr;          // device scalar for the gauge shift / sums
     hipStream_t stream_ = nullptr;
     std::vector<HostLevel> host_;
     std::vector<std::array<AxisTransfer, 3>> tr_host_;
@@ -965,6 +1200,10 @@ private:
         if (part_) (void)hipFree(part_);
         if (red_) (void)hipFree(red_);
         if (red_host_) (void)hipHostFree(red_host_);
+        if (shift_) (void)hipFree(shift_);
+        if (bprime_) (void)hipFree(bprime_);
+        shift_ = nullptr;
+        bprime_ = nullptr;
         cinv_ = nullptr;
         part_ = nullptr;
         red_ = nullptr;
@@ -1014,6 +1253,8 @@ private:
         const int64_t nb = (int64_t)nblk(host_[0].N);
         MG_HIP(hipMalloc(&part_, (size_t)nb * 2 * sizeof(double)));
         MG_HIP(hipMalloc(&red_, 2 * sizeof(double)));
+        MG_HIP(hipMalloc(&shift_, 4 * sizeof(double)));
+        if (pin_ >= 0) MG_HIP(hipMalloc(&bprime_, (size_t)host_[0].N * sizeof(double)));
         MG_HIP(hipHostMalloc(&red_host_, 2 * sizeof(double), hipHostMallocDefault));
     }
 
@@ -1048,6 +1289,7 @@ private:
     {
         DevTransfer<T> t;
         for (int d = 0; d < 3; ++d) {
+            t.wrap[d] = tr_host_[l][d].wrap;
             t.pc[d] = dtr_[l].pc[d];
             t.rf[d] = dtr_[l].rf[d];
             t.rn[d] = dtr_[l].rn[d];
@@ -1159,7 +1401,8 @@ private:
         }
     }
 
-    void outer_residual(const double* b, const double* x, bool emit_f32, double* rr, double* bb)
+    void outer_residual(const double* b, const double* bn, const double* x, bool emit_f32,
+                        double* rr, double* bb)
     {
         const DevLevel L = dl(0);
         const unsigned nb = nblk(L.N);
@@ -1169,7 +1412,7 @@ private:
         const bool c = host_[0].compact;
         float* b0 = emit_f32 ? dev_[0].bf : nullptr;
         const float* d32 = dev_[0].dinv32;
-#define OUTER(DIM, CMP, F32) hipLaunchKernelGGL((k_outer_residual<DIM, CMP, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, L, K, x, b, d32, b0, part_)
+#define OUTER(DIM, CMP, F32) hipLaunchKernelGGL((k_outer_residual<DIM, CMP, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, L, K, x, b, bn, pin_, d32, b0, part_)
         if (host_[0].D == 2) {
             if (c) { if (emit_f32) OUTER(2, true, true); else OUTER(2, true, false); }
             else { if (emit_f32) OUTER(2, false, true); else OUTER(2, false, false); }

@@ -113,7 +113,9 @@ def lib():
         L.orc_poisson_coo.argtypes = [C.POINTER(Grid), C.c_double, P, P, P, P]
         L.orc_poisson_coo.restype = i64
         L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
-                                    C.c_int, C.c_int]
+                                    C.c_int, C.c_int, C.c_int]
+        L.orc_mg_is_singular.argtypes = [P, P]
+        L.orc_mg_is_singular.restype = C.c_int
         L.orc_mg_create.restype = P
         L.orc_mg_levels.argtypes = [P, P]
         L.orc_mg_levels.restype = C.c_int
@@ -245,15 +247,20 @@ def poisson_coo(g, eps, fixed):
 class MG:
     """Reference-restated Galerkin multigrid (FP64 and the 32f mixed form)."""
 
-    def __init__(self, n, rows, cols, vals, I, J, K=1, depth=3, pre=2, post=1):
+    def __init__(self, n, rows, cols, vals, I, J, K=1, depth=3, pre=2, post=1, mode=1):
         self.rows = np.ascontiguousarray(rows, np.uint64)
         self.cols = np.ascontiguousarray(cols, np.uint64)
         self.vals = np.ascontiguousarray(vals, np.float64)
         self.n = n
         self.h = lib().orc_mg_create(n, self.vals.shape[0], _ptr(self.rows), _ptr(self.cols),
-                                     _ptr(self.vals), I, J, K, depth, pre, post)
+                                     _ptr(self.vals), I, J, K, depth, pre, post, mode)
         if not self.h:
             raise ValueError("orc_mg_create failed")
+
+    def singular(self):
+        pin = np.zeros(1, np.int64)
+        s = lib().orc_mg_is_singular(self.h, _ptr(pin))
+        return bool(s), int(pin[0])
 
     def levels(self):
         sizes = np.zeros(64, np.int64)

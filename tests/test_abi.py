@@ -1,0 +1,62 @@
+"""The C-ABI library loads on the CPU and exports every function that
+include/*.h declares (no compute call needs a GPU here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in ("psum_hip.h", "psum_field_solver.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = re.sub(r"typedef[^;]*;", "", src)
+        for m in re.finditer(r"\b(psum_\w+|register_solver|get_solver_funcs)\s*\(", src):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from psum_amd import _lib
+    L = _lib.lib()
+    declared = _declared()
+    assert len(declared) > 30
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert declared <= bound | {"psum_hip_last_error"}, declared - bound
+
+
+def test_registry_is_host_only_and_reference_shaped():
+    from psum_amd import _lib
+    L = _lib.lib()
+    t = L.get_solver_funcs(b"hip_multigrid_gpu")
+    assert t.init and t.set_matrix and t.solve and t.set_options and t.set_source_replace \
+        and t.set_source_addback and t.free
+    assert L.get_solver_funcs(b"hip_multigrid32f_gpu").init
+    assert not L.get_solver_funcs(b"sycl_multigrid_gpu").init
+    assert C.sizeof(_lib.FuncTable) == 7 * C.sizeof(C.c_void_p)
+
+
+def test_argument_errors_do_not_touch_the_gpu():
+    from psum_amd import _lib
+    L = _lib.lib()
+    g = _lib.PsumGrid()
+    g.dim = 7
+    rc = L.psum_hip_bin(C.byref(g), 0, None, 10, None, None)
+    assert rc == 1
+    assert b"invalid grid" in L.psum_hip_last_error()
+    assert L.psum_hip_sort(None, -1, 5, None, None, None, 0, None) == 1
+
+
+def test_grid_struct_layout_matches_oracle():
+    from psum_amd import _lib
+    import oracle_lib as O
+    assert C.sizeof(_lib.PsumGrid) == C.sizeof(O.Grid)
+    for (a, ta), (b, tb) in zip(_lib.PsumGrid._fields_, O.Grid._fields_):
+        assert a == b and C.sizeof(ta) == C.sizeof(tb)
+        assert getattr(_lib.PsumGrid, a).offset == getattr(O.Grid, b).offset

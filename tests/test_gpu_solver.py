@@ -60,14 +60,18 @@ def test_registry_names():
     assert not E.get_solver_funcs("no_such_backend").init
 
 
-@pytest.mark.parametrize("n", [64, 128])
-def test_vcycle_matches_oracle_periodic(n):
+@pytest.mark.parametrize("n,depth,mode", [(64, 3, 1), (128, 5, 1), (64, 2, 0), (66, 3, 1)])
+def test_vcycle_matches_oracle_periodic(n, depth, mode):
+    """Per-cycle agreement with the oracle: mode 1 (periodic-aware, the
+    default) and mode 0 (the reference algorithm exactly; 64^2 at depth 2 is
+    a case where it stays bounded)."""
     E = _E()
     og, nn, r, c, v, b = _periodic_problem(n, seed=n)
-    depth = 3
-    mg = O.MG(nn, r, c, v, n, n, 1, depth=depth, pre=2, post=1)
+    mg = O.MG(nn, r, c, v, n, n, 1, depth=depth, pre=2, post=1, mode=mode)
+    assert mg.singular()[0] == (mode == 1)
     s = E.SolverBackend("hip_multigrid_gpu")
-    s.set_options(f"I {n} J {n} depth {depth} pre_relax 2 post_relax 1 max_cycles 1")
+    s.set_options(f"I {n} J {n} depth {depth} pre_relax 2 post_relax 1 max_cycles 1 "
+                  f"periodic_mode {mode}")
     s.set_matrix(nn, nn, r, c, v)
     x_ref = np.zeros(nn)
     x = torch.zeros(nn, dtype=torch.float64, device="cuda")
@@ -77,7 +81,7 @@ def test_vcycle_matches_oracle_periodic(n):
         s.solve(bd, x)
         got = x.cpu().numpy()
         err = np.max(np.abs(got - x_ref)) / np.max(np.abs(x_ref))
-        assert err < 1e-12, (cyc, err)
+        assert err < 1e-10, (cyc, err)
 
 
 def test_converges_to_exact_periodic_solution():
@@ -86,7 +90,7 @@ def test_converges_to_exact_periodic_solution():
     n = 128
     og, nn, r, c, v, b = _periodic_problem(n, seed=7)
     s = E.SolverBackend("hip_multigrid_gpu")
-    s.set_options(f"I {n} J {n} depth 4 pre_relax 2 post_relax 1 max_cycles 60 tol 1e-12")
+    s.set_options(f"I {n} J {n} depth 5 pre_relax 2 post_relax 1 max_cycles 60 tol 1e-12")
     s.set_matrix(nn, nn, r, c, v)
     s.set_source_replace([0], [0.0])
     rho = b.copy()
@@ -114,10 +118,10 @@ def test_mixed_backend_converges_and_matches_fp64():
     n = 128
     og, nn, r, c, v, b = _periodic_problem(n, seed=9)
     s64 = E.SolverBackend("hip_multigrid_gpu")
-    s64.set_options(f"I {n} J {n} depth 4 max_cycles 60 tol 1e-11")
+    s64.set_options(f"I {n} J {n} depth 5 max_cycles 60 tol 1e-11")
     s64.set_matrix(nn, nn, r, c, v)
     s32 = E.SolverBackend("hip_multigrid32f_gpu")
-    s32.set_options(f"I {n} J {n} depth 4 max_cycles 60 tol 1e-10")
+    s32.set_options(f"I {n} J {n} depth 5 max_cycles 60 tol 1e-10")
     s32.set_matrix(nn, nn, r, c, v)
     x64 = torch.zeros(nn, dtype=torch.float64, device="cuda")
     x32 = torch.zeros(nn, dtype=torch.float64, device="cuda")
@@ -128,11 +132,11 @@ def test_mixed_backend_converges_and_matches_fp64():
     a, m = x64.cpu().numpy(), x32.cpu().numpy()
     assert np.max(np.abs(a - m)) <= 1e-8 * np.max(np.abs(a))
     # reference 32f semantics (fixed count, correction_interval 4) vs the oracle
-    mg = O.MG(nn, r, c, v, n, n, 1, depth=4, pre=2, post=1)
+    mg = O.MG(nn, r, c, v, n, n, 1, depth=5, pre=2, post=1)
     x_ref = np.zeros(nn)
     mg.solve_mixed(b, x_ref, 8, 4)
     s32b = E.SolverBackend("hip_multigrid32f_gpu")
-    s32b.set_options(f"I {n} J {n} depth 4 max_cycles 8 correction_interval 4")
+    s32b.set_options(f"I {n} J {n} depth 5 max_cycles 8 correction_interval 4")
     s32b.set_matrix(nn, nn, r, c, v)
     xb = torch.zeros(nn, dtype=torch.float64, device="cuda")
     s32b.solve(_dev(b), xb)
