@@ -33,13 +33,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every):
+def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every, pre=2, post=1,
+               graph=True, extrapolate=False):
     import torch
     sp = wl["species"]
     t0 = time.time()
-    st = E.Stepper(wl["grid"], sp, dtype=dtype, mixed_mg=mixed, depth=depth, pre=2, post=1,
+    st = E.Stepper(wl["grid"], sp, dtype=dtype, mixed_mg=mixed, depth=depth, pre=pre, post=post,
                    max_cycles=40, tol=tol, dt=wl["dt"], rho_bg=wl["rho_bg"],
-                   sort_every=sort_every)
+                   sort_every=sort_every, mg_graph=graph, extrapolate=extrapolate)
     log(f"[bench] {dtype}{'+mixed MG' if mixed else ''}: setup {time.time() - t0:.1f}s")
     if warmup:
         st.run(warmup, history=False)
@@ -62,7 +63,7 @@ def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every):
     st.close()
     log(f"[bench] {dtype}: {res['ms_per_step']:.3f} ms/step, "
         f"{res['particle_steps_per_s']:.3e} particle-steps/s, push {res['push_ms_avg']:.3f} ms, "
-        f"{res['cycles_per_step']:.2f} V-cycles/step")
+        f"{res['cycles_per_step']:.2f} V-cycles/step, tile fallbacks {res['tile_fallback_blocks']}")
     return res
 
 
@@ -107,7 +108,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--ppc", type=int, default=100)
-    ap.add_argument("--depth", type=int, default=6)
+    ap.add_argument("--depth", type=int, default=5)
+    ap.add_argument("--pre", type=int, default=3)
+    ap.add_argument("--post", type=int, default=3)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--extrapolate", action="store_true")
     ap.add_argument("--sort-every", type=int, default=50)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -129,14 +134,15 @@ def main():
     log(f"[bench] workload {wl['name']}: {n} particles, generated in {time.time() - t0:.1f}s")
 
     mixed = run_engine(E, wl, "f32", True, args.steps, args.warmup, 1e-8, args.depth,
-                       args.sort_every)
+                       args.sort_every, args.pre, args.post, not args.no_graph, args.extrapolate)
     fp64 = None
     if not args.no_fp64:
         wl64 = dict(wl)
         wl64["species"] = [dict(s, pos=[a.astype(np.float64) for a in s["pos"]],
                                 vel=[a.astype(np.float64) for a in s["vel"]]) for s in wl["species"]]
         fp64 = run_engine(E, wl64, "f64", False, args.fp64_steps or args.steps, args.warmup, 1e-8,
-                          args.depth, args.sort_every)
+                          args.depth, args.sort_every, args.pre, args.post, not args.no_graph,
+                          args.extrapolate)
         del wl64
 
     # dominant kernel: fused gather + push + deposit; algorithmic bytes per
@@ -174,6 +180,8 @@ def main():
                                f"{args.ppc} e-/cell ({n} particles), dt 0.05, FP32 state, "
                                f"FP32 V-cycles in FP64 refinement to 1e-8",
                    "grid": [args.grid, args.grid], "particles": n, "mg_depth": args.depth,
+                   "mg_pre_post": [args.pre, args.post], "mg_graph": not args.no_graph,
+                   "extrapolate": args.extrapolate,
                    "sort_every": args.sort_every, "parallelism": "single GPU"},
         "fp64": None if fp64 is None else {
             "value": fp64["particle_steps_per_s"], "ms_per_step": fp64["ms_per_step"],

@@ -126,8 +126,12 @@ psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos
                                  double dt, double mass, double w_dep, double scale, int64_t* acc,
                                  double* diag, int64_t* counters, void* stream);
 
-/* out[k] = sum over blocks of diag[4*b + k], k < 4, fixed order (deterministic). */
-psum_status psum_hip_reduce_diag(const double* diag, int64_t nblocks, double* out, void* stream);
+/* out[k] = sum over blocks of diag[4*b + k], k < 4, in a fixed order
+ * (deterministic).  work: psum_hip_reduce_workspace() bytes of device
+ * memory (may be NULL when nblocks <= 1024). */
+size_t psum_hip_reduce_workspace(void);
+psum_status psum_hip_reduce_diag(const double* diag, int64_t nblocks, double* out, void* work,
+                                 void* stream);
 
 /* Field energy 1/2 eps sum_d sum_edges ((phi[n+e_d]-phi[n]) * delta_r_d)^2 * dV
  * (TSI/main.cpp:138-141 in D dimensions; periodic dims include the wrap
@@ -158,6 +162,8 @@ typedef struct psum_step_config {
     int32_t nspecies;        /* 1..4 */
     int32_t depth, pre, post, max_cycles;
     int32_t sort_every;      /* 0 = never re-sort */
+    int32_t mg_graph;        /* 1: replay each V-cycle / refinement iteration from a hipGraph */
+    int32_t extrapolate;     /* 1: warm start phi by linear extrapolation 2 phi_n - phi_{n-1} */
     double tol;              /* relative residual target ||b-A phi|| / ||b|| */
     double eps, dt, rho_bg;
     psum_species species[4];
@@ -184,7 +190,52 @@ psum_status psum_stepper_sync(void* h);
  * stream (reset by set_timing): total ms, launches, particles pushed. */
 psum_status psum_stepper_set_timing(void* h, int on);
 psum_status psum_stepper_push_timing(void* h, double* ms, int64_t* launches, int64_t* particles);
+/* Loss L = field energy (psum_hip_field_energy) of phi solved from the
+ * positions after `steps` steps from the current state (no re-sorting). */
+psum_status psum_stepper_forward_loss(void* h, int steps, double* loss);
+/* Reverse mode: L as above from the current state and dL/dx0, dL/dv0 into
+ * grad_x / grad_v (device arrays of the particle dtype, index s*3 + d for
+ * species s, dimension d; NULL to skip).  Forward checkpoints (x, v, phi,
+ * solver state) every `ckpt` steps; each segment is recomputed in reverse.
+ * The state is restored to the initial one.  t_fwd_ms / t_bwd_ms: HIP-event
+ * times of the checkpointed forward and of the reverse sweep (recompute
+ * included).  No reference counterpart (the reference has no adjoint). */
+psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, void* const* grad_x,
+                                 void* const* grad_v, double* t_fwd_ms, double* t_bwd_ms);
 void psum_stepper_destroy(void* h);
+
+/* ---- adjoint kernels (used by psum_stepper_adjoint) --------------------
+ * Reverse of gather + kick + drift for one species, in place on (lx, lv):
+ *   lvt = lv + dt lx;  lx -= qm dt H(x)^T lvt;  lv = lvt;
+ *   acc[idx_k] += llrint((-qm dt sum_e lvt_e d_e c_k(x)) * scale_dev[0])
+ * (the transpose of the gather, deposited in int64 fixed point). */
+psum_status psum_hip_adjoint_push(const psum_grid* g, int dtype, void* const* x, void* const* lx,
+                                  void* const* lv, int64_t n, const void* phi, double qm, double dt,
+                                  const double* scale_dev, int64_t* acc, void* stream);
+int64_t psum_hip_adjoint_blocks(int64_t n);
+/* Reverse of the deposit: lx += w_dep grad(lrho)(x) (the transpose of the
+ * deposit); bmax[b] = per-block max of sum_e |lv_e + dt lx_e|. */
+psum_status psum_hip_adjoint_gather(const psum_grid* g, int dtype, void* const* x, void* const* lx,
+                                    void* const* lv, int64_t n, const void* lrho, double w_dep,
+                                    double dt, double* bmax, void* stream);
+/* scale_dev[0..2] = 2^(40-e), 2^(e-40), bound with 2^(e-1) <= bound < 2^e,
+ * bound = factor * max(bmax) (max with the previous bound when accumulate). */
+psum_status psum_hip_adjoint_scale(const double* bmax, int64_t nb, double factor, double* scale_dev,
+                                   int accumulate, void* stream);
+psum_status psum_hip_fixed_to_double_dev(const int64_t* acc, int64_t nn, const double* scale_dev,
+                                         double* out, void* stream);
+/* dW/dphi of the field energy: eps dV sum over existing edges of
+ * delta_r_d^2 (phi_i - phi_j). */
+psum_status psum_hip_field_energy_grad(const psum_grid* g, const double* phi, double eps, double* out,
+                                       void* stream);
+/* x_p = xi_p + sum_{k=1..nm} a_k L/(2 pi k) sin(2 pi k (xi_p - lo)/L), wrapped
+ * into [lo, lo+L) (a_dev: nm doubles on the device), and its transpose
+ * out_k = sum_p lx_p L/(2 pi k) sin(2 pi k (xi_p - lo)/L). */
+psum_status psum_hip_mode_displace(int dtype, const void* xi, void* x, int64_t n, const double* a_dev,
+                                   int nm, double lo, double L, void* stream);
+size_t psum_hip_mode_project_workspace(int64_t n, int nm);
+psum_status psum_hip_mode_project(int dtype, const void* xi, const void* lx, int64_t n, int nm,
+                                  double lo, double L, double* out_dev, void* work, void* stream);
 
 #ifdef __cplusplus
 }

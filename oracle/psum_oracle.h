@@ -164,6 +164,16 @@ int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n
                         int steps, int depth, int pre, int post, double tol, int max_cycles,
                         int dep_mode, const uint8_t* fixed, double* hist, double* phi_out);
 
+/* Several species (charge deposits summed into one fixed-point accumulator
+ * with the common scale of the largest |q w / dV|); fixed_val gives the
+ * Dirichlet values of fixed nodes (NULL = 0). */
+int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double*** vel,
+                          const int64_t* n, const double* q, const double* m, const double* w,
+                          double rho_bg, double eps, double dt, int steps, int depth, int pre,
+                          int post, double tol, int max_cycles, int dep_mode,
+                          const uint8_t* fixed, const double* fixed_val, double* hist,
+                          double* phi_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,10 +20,12 @@ static int fx_shift(double w)
     return 40 - e;
 }
 
-int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n,
-                        double q, double m, double w, double rho_bg, double eps, double dt,
-                        int steps, int depth, int pre, int post, double tol, int max_cycles,
-                        int dep_mode, const uint8_t* fixed, double* hist, double* phi_out)
+int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double*** vel,
+                          const int64_t* n, const double* q, const double* m, const double* w,
+                          double rho_bg, double eps, double dt, int steps, int depth, int pre,
+                          int post, double tol, int max_cycles, int dep_mode,
+                          const uint8_t* fixed, const double* fixed_val, double* hist,
+                          double* phi_out)
 {
     const int D = g->dim;
     const int64_t nn = orc_num_nodes(g);
@@ -45,22 +47,27 @@ int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n
 
     double dV = 1.0;
     for (int d = 0; d < D; ++d) dV *= g->delta[d];
-    const double wdep = q * w / dV;
-    const double scale = ldexp(1.0, fx_shift(wdep));
-    const double qm = q / m;
+    double wdep[8], wmax = 0.0;
+    for (int s = 0; s < nspecies; ++s) {
+        wdep[s] = q[s] * w[s] / dV;
+        if (fabs(wdep[s]) > wmax) wmax = fabs(wdep[s]);
+    }
+    const double scale = ldexp(1.0, fx_shift(wmax > 0 ? wmax : 1.0));
 
     int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * (size_t)nn);
     double* b = (double*)malloc(sizeof(double) * (size_t)nn);
     double* phi = (double*)calloc((size_t)nn, sizeof(double));
     int64_t total_cycles = 0;
-    for (int s = 0; s < steps; ++s) {
+    for (int st = 0; st < steps; ++st) {
         if (dep_mode == 0) {
             memset(acc, 0, sizeof(int64_t) * (size_t)nn);
-            orc_deposit_fx_f64(g, (const double* const*)pos, n, wdep, scale, acc);
+            for (int s = 0; s < nspecies; ++s)
+                orc_deposit_fx_f64(g, (const double* const*)pos[s], n[s], wdep[s], scale, acc);
             for (int64_t i = 0; i < nn; ++i) b[i] = (double)acc[i] * (1.0 / scale) + rho_bg;
         } else {
             memset(b, 0, sizeof(double) * (size_t)nn);
-            orc_deposit_ref_f64(g, (const double* const*)pos, n, wdep, b);
+            for (int s = 0; s < nspecies; ++s)
+                orc_deposit_ref_f64(g, (const double* const*)pos[s], n[s], wdep[s], b);
             for (int64_t i = 0; i < nn; ++i) b[i] = b[i] + rho_bg;
         }
         if (all_periodic) {
@@ -69,7 +76,8 @@ int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n
             const double mean = sum / (double)nn;
             for (int64_t i = 0; i < nn; ++i) b[i] = b[i] - mean;
         }
-        for (int64_t i = 0; i < nn; ++i) if (fx[i]) b[i] = 0.0;      /* set_source_replace */
+        for (int64_t i = 0; i < nn; ++i)                               /* set_source_replace */
+            if (fx[i]) b[i] = (fixed_val && !(all_periodic && i == 0)) ? fixed_val[i] : 0.0;
         double bn = 0.0;
         for (int64_t i = 0; i < nn; ++i) bn += b[i] * b[i];
         bn = sqrt(bn);
@@ -78,19 +86,31 @@ int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n
             ++total_cycles;
             if (orc_mg_residual_norm(mg, b, phi) <= tol * bn) break;
         }
-        double diag[4];
         const double fe = orc_field_energy(g, phi, eps);
-        orc_gather_push_f64(g, pos, vel, n, phi, qm, dt, m * w, diag);
+        double tot[4] = {0, 0, 0, 0};
+        for (int s = 0; s < nspecies; ++s) {
+            double diag[4];
+            orc_gather_push_f64(g, pos[s], vel[s], n[s], phi, q[s] / m[s], dt, m[s] * w[s], diag);
+            for (int k = 0; k < 4; ++k) tot[k] += diag[k];
+        }
         if (hist) {
-            hist[5 * s + 0] = fe;
-            hist[5 * s + 1] = diag[0];
-            hist[5 * s + 2] = diag[1];
-            hist[5 * s + 3] = diag[2];
-            hist[5 * s + 4] = diag[3];
+            hist[5 * st + 0] = fe;
+            for (int k = 0; k < 4; ++k) hist[5 * st + 1 + k] = tot[k];
         }
     }
     if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)nn);
     orc_mg_free(mg);
     free(acc); free(b); free(phi); free(fx);
     return total_cycles;
+}
+
+int64_t orc_pic_run_f64(const orc_grid* g, double** pos, double** vel, int64_t n,
+                        double q, double m, double w, double rho_bg, double eps, double dt,
+                        int steps, int depth, int pre, int post, double tol, int max_cycles,
+                        int dep_mode, const uint8_t* fixed, double* hist, double* phi_out)
+{
+    double** P[1] = {pos};
+    double** V[1] = {vel};
+    return orc_pic_run_multi(g, 1, P, V, &n, &q, &m, &w, rho_bg, eps, dt, steps, depth, pre, post,
+                             tol, max_cycles, dep_mode, fixed, NULL, hist, phi_out);
 }

@@ -50,6 +50,8 @@ class PsumStepConfig(C.Structure):
         ("post", C.c_int32),
         ("max_cycles", C.c_int32),
         ("sort_every", C.c_int32),
+        ("mg_graph", C.c_int32),
+        ("extrapolate", C.c_int32),
         ("tol", C.c_double),
         ("eps", C.c_double),
         ("dt", C.c_double),
@@ -91,7 +93,8 @@ SIGNATURES = [
     ("psum_hip_push_blocks", I64, [I64]),
     ("psum_hip_gather_push", ST, [_GP, C.c_int, P, P, I64, P, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_double, P, P, P, P]),
-    ("psum_hip_reduce_diag", ST, [P, I64, P, P]),
+    ("psum_hip_reduce_workspace", C.c_size_t, []),
+    ("psum_hip_reduce_diag", ST, [P, I64, P, P, P]),
     ("psum_hip_field_energy_workspace", C.c_size_t, [_GP]),
     ("psum_hip_field_energy", ST, [_GP, P, C.c_double, P, P, C.c_size_t, P]),
     ("psum_hip_f64_to_f32", ST, [P, P, I64, P]),
@@ -104,7 +107,19 @@ SIGNATURES = [
     ("psum_stepper_sync", ST, [P]),
     ("psum_stepper_set_timing", ST, [P, C.c_int]),
     ("psum_stepper_push_timing", ST, [P, C.POINTER(C.c_double), C.POINTER(I64), C.POINTER(I64)]),
+    ("psum_stepper_forward_loss", ST, [P, C.c_int, C.POINTER(C.c_double)]),
+    ("psum_stepper_adjoint", ST, [P, C.c_int, C.c_int, C.POINTER(C.c_double), P, P,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psum_stepper_destroy", None, [P]),
+    ("psum_hip_adjoint_push", ST, [_GP, C.c_int, P, P, P, I64, P, C.c_double, C.c_double, P, P, P]),
+    ("psum_hip_adjoint_blocks", I64, [I64]),
+    ("psum_hip_adjoint_gather", ST, [_GP, C.c_int, P, P, P, I64, P, C.c_double, C.c_double, P, P]),
+    ("psum_hip_adjoint_scale", ST, [P, I64, C.c_double, P, C.c_int, P]),
+    ("psum_hip_fixed_to_double_dev", ST, [P, I64, P, P, P]),
+    ("psum_hip_field_energy_grad", ST, [_GP, P, C.c_double, P, P]),
+    ("psum_hip_mode_displace", ST, [C.c_int, P, P, I64, P, C.c_int, C.c_double, C.c_double, P]),
+    ("psum_hip_mode_project_workspace", C.c_size_t, [I64, C.c_int]),
+    ("psum_hip_mode_project", ST, [C.c_int, P, P, I64, C.c_int, C.c_double, C.c_double, P, P, P]),
     ("register_solver", None, [C.c_char_p, FuncTable]),
     ("get_solver_funcs", FuncTable, [C.c_char_p]),
     ("psum_solver_create", P, [C.c_char_p]),

@@ -194,12 +194,15 @@ __global__ __launch_bounds__(kMgThreads) void k_residual(DevLevel L, DevTable<T>
     r[i] = t * (T)-1 + b[i] * (T)1;
 }
 
-// b_c = (R r) o D_c^-1 ; x_c = 0
+// b_c = (R r) o D_c^-1 ; x_c = 0, or, when the coarse level is pre-smoothed,
+// the first Jacobi sweep from x_c = 0 fused in: relax*0 + b_c*(2/3) is
+// exactly b_c * (2/3) (first_sweep = 2/3; 0 means plain x_c = 0).
 template <typename T, int D>
 __global__ __launch_bounds__(kMgThreads) void k_restrict(DevLevel F, DevLevel Cl,
                                                          DevTransfer<T> tr, DevTable<T> Ac,
                                                          const T* __restrict__ r,
-                                                         T* __restrict__ bc, T* __restrict__ xc)
+                                                         T* __restrict__ bc, T* __restrict__ xc,
+                                                         T first_sweep)
 {
     const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
     if (i >= Cl.N) return;
@@ -234,8 +237,9 @@ __global__ __launch_bounds__(kMgThreads) void k_restrict(DevLevel F, DevLevel Cl
             }
         }
     }
-    bc[i] = s * Ac.dinv[Cl.cls[i]];
-    xc[i] = (T)0;
+    const T bv = s * Ac.dinv[Cl.cls[i]];
+    bc[i] = bv;
+    xc[i] = first_sweep != (T)0 ? bv * (first_sweep * (T)1) : (T)0;
 }
 
 // x_f = (P x_c)*1 + x_f*(1*1)
@@ -446,10 +450,11 @@ __global__ __launch_bounds__(kMgThreads) void k_addback(double* __restrict__ b,
 
 __global__ __launch_bounds__(kMgThreads) void k_replace(double* __restrict__ b,
                                                         const unsigned long long* __restrict__ idx,
-                                                        const double* __restrict__ v, int64_t n)
+                                                        const double* __restrict__ v, int64_t n,
+                                                        int zero)
 {
     const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
-    if (i < n) b[idx[i]] = v[i];
+    if (i < n) b[idx[i]] = zero ? 0.0 : v[i];
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kMgThreads - 1) / kMgThreads); }
@@ -978,6 +983,7 @@ public:
                 corr_ = v;
             } else if (key == "tol") iss >> tol_;
             else if (key == "periodic_mode") iss >> periodic_mode_;
+            else if (key == "graph") iss >> use_graph_;
             else if (key == "num_conv_mask" || key == "num_conv_mask_relax" ||
                      key == "num_conv_mask_restrict" || key == "num_conv_mask_interpolate" ||
                      key == "num_conv_mask_a") {
@@ -1066,12 +1072,14 @@ public:
     }
 
     // solve(b, x): device pointers, b modified in place (addback, replace)
-    void solve(double* b, double* x, bool sync = true)
+    // homogeneous: no addback and zero replace values (the linear part of the
+    // solve, used by the adjoint)
+    void solve(double* b, double* x, bool sync = true, bool homogeneous = false)
     {
         if (host_.empty()) throw std::runtime_error("hip_multigrid: solve before set_matrix");
         hipStream_t st = stream_;
-        if (add_n_) hipLaunchKernelGGL(k_addback, dim3(nblk(add_n_)), dim3(kMgThreads), 0, st, b, add_idx_, add_val_, add_n_);
-        if (rep_n_) hipLaunchKernelGGL(k_replace, dim3(nblk(rep_n_)), dim3(kMgThreads), 0, st, b, rep_idx_, rep_val_, rep_n_);
+        if (add_n_ && !homogeneous) hipLaunchKernelGGL(k_addback, dim3(nblk(add_n_)), dim3(kMgThreads), 0, st, b, add_idx_, add_val_, add_n_);
+        if (rep_n_) hipLaunchKernelGGL(k_replace, dim3(nblk(rep_n_)), dim3(kMgThreads), 0, st, b, rep_idx_, rep_val_, rep_n_, homogeneous ? 1 : 0);
         MG_HIP(hipGetLastError());
         last_cycles_ = 0;
         last_rel_ = NAN;
@@ -1087,41 +1095,57 @@ public:
             MG_HIP(hipGetLastError());
             bs = bprime_;
         }
+        // With a tolerance, the first check comes after the number of cycles
+        // the previous solve needed minus one (warm-started solves of a
+        // time-stepping loop need similar counts); after that every cycle is
+        // checked.  A check reads ||b - A x||^2 and ||b||^2 back to the host.
+        const int first_check = std::max(1, std::min(max_cycles_, predicted_ - 1));
         if (!mixed_) {
             if (!use_tol) {
                 vcycles_f64(bs, x, max_cycles_);
                 last_cycles_ = max_cycles_;
             } else {
                 double rr, bb;
-                for (int c = 0; c < max_cycles_; ++c) {
-                    vcycles_f64(bs, x, 1);
-                    ++last_cycles_;
-                    outer_residual(bs, b, x, false, &rr, &bb);
+                for (int c = 0; c < first_check; ++c) cycle_f64(bs, x);
+                last_cycles_ = first_check;
+                for (;;) {
+                    outer_residual(bs, b, x, false, true, &rr, &bb);
                     last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
-                    if (last_rel_ <= tol_) break;
+                    if (last_rel_ <= tol_ || last_cycles_ >= max_cycles_) break;
+                    cycle_f64(bs, x);
+                    ++last_cycles_;
                 }
             }
         } else {
+            // refinement iteration: r = b - A x (FP64) -> FP32 V-cycles on A e = r -> x += e
+            double rr, bb;
+            outer_residual(bs, b, x, true, false, &rr, &bb);
             for (int cycle = 0; cycle < max_cycles_;) {
-                const int batch = use_tol ? 1 : std::min(corr_, max_cycles_ - cycle);
-                double rr, bb;
-                outer_residual(bs, b, x, true, &rr, &bb);   // b0 (float) = (float)(b - A x) o D^-1
-                if (use_tol) {
+                if (use_tol && cycle >= first_check) {
+                    read_norms(&rr, &bb);
                     last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
                     if (last_rel_ <= tol_) break;
                 }
+                if (use_tol) {
+                    iteration_f32(bs, b, x);
+                    cycle += 1;
+                    last_cycles_ += 1;
+                    continue;
+                }
+                const int batch = std::min(corr_, max_cycles_ - cycle);
                 vcycles_f32(batch);
                 hipLaunchKernelGGL(k_add_correction, dim3(nblk(N)), dim3(kMgThreads), 0, st, x, dev_[0].xf, N);
                 MG_HIP(hipGetLastError());
                 cycle += batch;
                 last_cycles_ += batch;
+                outer_residual(bs, b, x, true, false, &rr, &bb);
             }
-            if (use_tol && !(last_rel_ <= tol_)) {
-                double rr, bb;
-                outer_residual(bs, b, x, false, &rr, &bb);
+            if (use_tol) {
+                read_norms(&rr, &bb);
                 last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
             }
         }
+        if (use_tol) predicted_ = last_cycles_;
         if (pin_ >= 0) {
             hipLaunchKernelGGL(k_gauge_read, dim3(1), dim3(64), 0, st, x, b, pin_, shift_ + 1);
             hipLaunchKernelGGL(k_gauge_apply, dim3(nblk(N)), dim3(kMgThreads), 0, st, x, N, shift_ + 1);
@@ -1131,6 +1155,8 @@ public:
     }
 
     int last_cycles() const { return last_cycles_; }
+    int predicted() const { return predicted_; }
+    void set_predicted(int p) { predicted_ = p; }
     double last_rel() const { return last_rel_; }
     int64_t N0() const { return host_.empty() ? 0 : host_[0].N; }
     int levels() const { return (int)host_.size(); }
@@ -1139,6 +1165,7 @@ private:
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     double tol_ = 0.0;
+    int predicted_ = 1;                // cycles the previous solve needed
     int periodic_mode_ = 1;
     int64_t pin_ = -1;                 // pinned node of a pinned-periodic matrix
     double* bprime_ = nullptr;         // consistent singular right-hand side
@@ -1183,6 +1210,8 @@ private:
 
     void release()
     {
+        drop_graph(cap64_);
+        drop_graph(cap32_);
         for (auto& d : dev_) {
             for (void* p : {(void*)d.cls, (void*)d.K, (void*)d.Ah, (void*)d.Rx, (void*)d.dinv,
                             (void*)d.Ah32, (void*)d.Rx32, (void*)d.dinv32, (void*)d.t, (void*)d.bf,
@@ -1341,15 +1370,18 @@ private:
     {
         const int nl = (int)host_.size();
         const int Lc = nl - 1;
+        const T two3 = (T)(2.0 / 3);
         for (int cyc = 0; cyc < cycles; ++cyc) {
             for (int l = 0; l < Lc; ++l) {
-                for (int s = 0; s < pre_; ++s) smooth<T>(l, xs[l], ts[l], bs[l]);
+                // coarse levels received their first sweep from k_restrict
+                for (int s = (l > 0 && pre_ > 0) ? 1 : 0; s < pre_; ++s) smooth<T>(l, xs[l], ts[l], bs[l]);
                 residual<T>(l, xs[l], bs[l], ts[l]);
                 const DevLevel F = dl(l), C = dl(l + 1);
+                const T fs = (l + 1 < Lc && pre_ > 0) ? two3 : (T)0;
                 if (host_[l].D == 2)
-                    hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1]);
+                    hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
                 else
-                    hipLaunchKernelGGL((k_restrict<T, 3>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1]);
+                    hipLaunchKernelGGL((k_restrict<T, 3>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
             }
             {
                 const int n = (int)host_[Lc].N;
@@ -1395,14 +1427,69 @@ private:
         // e0 = 0
         hipLaunchKernelGGL(k_zero_f32, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, xs[0], N0());
         vcycles<float>(xs, ts, bs, cycles);
-        if (xs[0] != dev_[0].xf) {
-            // keep the result in dev_[0].xf for k_add_correction
-            std::swap(dev_[0].xf, dev_[0].tf);
-        }
+        // the correction ends in dev_[0].xf (a fixed buffer, so the sequence
+        // can be captured and replayed)
+        if (xs[0] != dev_[0].xf)
+            MG_HIP(hipMemcpyAsync(dev_[0].xf, xs[0], (size_t)N0() * sizeof(float), hipMemcpyDeviceToDevice, stream_));
     }
 
+    // One V-cycle (FP64) or one refinement iteration (mixed: FP32 V-cycle,
+    // x += e, next residual), replayed from a captured hipGraph when the
+    // solver runs on its own stream.  The graph is keyed on (b, x).
+    struct Captured {
+        hipGraphExec_t exec = nullptr;
+        const void* b = nullptr;
+        const void* x = nullptr;
+    };
+    Captured cap64_, cap32_;
+    int use_graph_ = 1;
+
+    void drop_graph(Captured& c)
+    {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        c = Captured{};
+    }
+
+    template <typename F>
+    void replay(Captured& c, const void* b, const void* x, F&& body)
+    {
+        if (!use_graph_ || stream_ == nullptr) {
+            body();
+            return;
+        }
+        if (!c.exec || c.b != b || c.x != x) {
+            drop_graph(c);
+            MG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            body();
+            hipGraph_t graph = nullptr;
+            MG_HIP(hipStreamEndCapture(stream_, &graph));
+            MG_HIP(hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            c.b = b;
+            c.x = x;
+        }
+        MG_HIP(hipGraphLaunch(c.exec, stream_));
+    }
+
+    void cycle_f64(const double* bs, double* x)
+    {
+        replay(cap64_, bs, x, [&]() { vcycles_f64(bs, x, 1); });
+    }
+
+    void iteration_f32(const double* bs, const double* b, double* x)
+    {
+        replay(cap32_, bs, x, [&]() {
+            vcycles_f32(1);
+            hipLaunchKernelGGL(k_add_correction, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, x, dev_[0].xf, N0());
+            double rr, bb;
+            outer_residual(bs, b, x, true, false, &rr, &bb);
+        });
+    }
+
+    // ||b - A x||^2 and ||bn||^2 into red_ (device) and, asynchronously, into
+    // pinned host memory; `wait` synchronises and returns them.
     void outer_residual(const double* b, const double* bn, const double* x, bool emit_f32,
-                        double* rr, double* bb)
+                        bool wait, double* rr, double* bb)
     {
         const DevLevel L = dl(0);
         const unsigned nb = nblk(L.N);
@@ -1424,6 +1511,11 @@ private:
         hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kMgThreads), 0, stream_, part_, (int64_t)nb, red_);
         MG_HIP(hipGetLastError());
         MG_HIP(hipMemcpyAsync(red_host_, red_, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        if (wait) read_norms(rr, bb);
+    }
+
+    void read_norms(double* rr, double* bb)
+    {
         MG_HIP(hipStreamSynchronize(stream_));
         *rr = red_host_[0];
         *bb = red_host_[1];
@@ -1523,6 +1615,9 @@ void mg_set_replace(HipMultigrid* h, unsigned long long n, const unsigned long l
 }
 void mg_set_stream(HipMultigrid* h, hipStream_t s) { h->set_stream(s); }
 void mg_solve(HipMultigrid* h, double* b, double* x, bool sync) { h->solve(b, x, sync); }
+void mg_solve_homogeneous(HipMultigrid* h, double* b, double* x) { h->solve(b, x, false, true); }
+int mg_predicted(HipMultigrid* h) { return h->predicted(); }
+void mg_set_predicted(HipMultigrid* h, int p) { h->set_predicted(p); }
 int mg_last_cycles(HipMultigrid* h) { return h->last_cycles(); }
 double mg_last_rel(HipMultigrid* h) { return h->last_rel(); }
 

@@ -35,6 +35,7 @@ extern "C" const char* psum_hip_last_error(void) { return g_last_error.c_str(); 
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int64_t kReduceBlocks = 256;
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -289,15 +290,119 @@ __global__ __launch_bounds__(kThreads) void k_compact_scatter(PtrPack pk, int nc
 }
 
 // --------------------------------------------- gather + push + deposit
-// One block = 256 threads x 4 particles.  With DEPOSIT the block deposits
-// the (new) positions of its particles into an LDS int64 tile spanning the
-// bounding box of their support nodes, then flushes non-zero tile entries
-// to the global int64 accumulator; a block whose box exceeds the tile uses
-// global integer atomics directly.  Integer sums make both paths exact.
+// One block = 256 threads x 4 particles; thread t owns the 4 consecutive
+// particles base + 4t .. base + 4t + 3 and moves them with 16-byte vector
+// loads and stores.  With DEPOSIT the block deposits the (new) positions of
+// its particles into an LDS int64 tile spanning the bounding box of their
+// support nodes, then flushes the non-zero entries to the global int64
+// accumulator; a block whose box exceeds the tile uses global integer
+// atomics directly.  Consecutive particles of cell-sorted storage share a
+// cell, so each thread first sums the terms of equal-cell runs in registers
+// (exact integer adds) and issues one atomic per run and support node, in a
+// lane-rotated node order to spread same-address conflicts.  Integer sums
+// make every path exact and order independent.
 
 constexpr int kPPT = 4;
 constexpr int kPushTile = kThreads * kPPT;
-constexpr int kTileNodes = 4096;
+constexpr int kTileNodes = 2048;
+
+template <typename T>
+__device__ __forceinline__ void load4(const T* __restrict__ p, T* v)
+{
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        const double2 b = *reinterpret_cast<const double2*>(p + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T* __restrict__ p, const T* v)
+{
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+    }
+}
+
+template <typename T, int D, bool PUSH>
+__device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__ phi, T qm, T dt,
+                                         double mass, T* x, T* v, double& ke, double* pm)
+{
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) v2 += (double)v[d] * (double)v[d];
+    ke += 0.5 * mass * v2;
+#pragma unroll
+    for (int d = 0; d < D; ++d) pm[d] += mass * (double)v[d];
+
+    // gather grad(phi) at x (field_interp.hpp:96-111)
+    T grad[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) grad[d] = (T)0;
+    double xd[D], s[D];
+    int c[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
+    if (base_cell<D>(g, xd, c, s)) {
+        T W[D][2], G[D][2];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T sd = (T)s[d];
+            W[d][0] = (T)1 - sd;
+            W[d][1] = sd;
+            const T dr = (T)g.delta_r[d];
+            G[d][0] = (T)-1 * dr;
+            G[d][1] = (T)1 * dr;
+        }
+        T ph[1 << D];
+#pragma unroll
+        for (int i = 0; i < (1 << D); ++i) ph[i] = phi[support_index<D>(g, c, i)];
+#pragma unroll
+        for (int dd = 0; dd < D; ++dd) {
+            T r = (T)0;
+#pragma unroll
+            for (int i = 0; i < (1 << D); ++i) {
+                const T gc = cic_grad_coef<T, D>(W, G, i, dd);
+                if (fabs((double)gc) > 1e-10) r = r + ph[i] * gc;
+            }
+            grad[dd] = r;
+        }
+    }
+    // kick then drift (TSI/main.cpp:65-68)
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const T e = -grad[d];
+        v[d] = v[d] + (e * qm) * dt;
+        x[d] = x[d] + v[d] * dt;
+    }
+    // boundaries: bounded periodic wrap, then absorbing walls
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if (g.periodic[d]) {
+            const T lo = (T)g.plo[d], hi = (T)g.phi[d], L = (T)g.plen[d];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                if (x[d] < lo) x[d] = x[d] + L;
+                else if (x[d] >= hi) x[d] = x[d] - L;
+            }
+        }
+    }
+    bool alive = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (!g.periodic[d]) {
+            const double xx = (double)x[d];
+            if (!(xx >= g.lb[d] && xx < g.ub[d])) alive = false;
+        }
+    if (!alive) x[0] = (T)NAN;
+    return alive;
+}
 
 template <typename T, int D, bool PUSH, bool DEPOSIT>
 __global__ __launch_bounds__(kThreads) void k_push_deposit(
@@ -316,113 +421,79 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
     }
     T* const xs[3] = {x0, x1, x2};
     T* const vs[3] = {v0, v1, v2};
-    const int64_t base = (int64_t)blockIdx.x * kPushTile;
+    const int64_t p0 = (int64_t)blockIdx.x * kPushTile + (int64_t)threadIdx.x * kPPT;
+    const bool full = p0 + kPPT <= n;
+
+    // load the thread's 4 particles (component-major registers)
+    T X[D][kPPT], V[D][kPPT];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if (full) {
+            load4(xs[d] + p0, X[d]);
+            if (PUSH) load4(vs[d] + p0, V[d]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPPT; ++j) {
+                X[d][j] = p0 + j < n ? xs[d][p0 + j] : (T)NAN;
+                if (PUSH) V[d][j] = p0 + j < n ? vs[d][p0 + j] : (T)0;
+            }
+        }
+    }
 
     bool ok[kPPT];
     int cw[kPPT][D];
     T sw[kPPT][D];
     double ke = 0.0, pm[3] = {0.0, 0.0, 0.0};
     unsigned oor = 0, absorbed = 0;
+    bool moved = false;
 
 #pragma unroll
-    for (int k = 0; k < kPPT; ++k) {
-        ok[k] = false;
-        const int64_t p = base + k * kThreads + threadIdx.x;
-        if (p >= n) continue;
-        T x[D];
+    for (int j = 0; j < kPPT; ++j) {
+        ok[j] = false;
+        T x[D], v[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = xs[d][p];
-        if (!isfinite((double)x[0])) continue;
+        for (int d = 0; d < D; ++d) {
+            x[d] = X[d][j];
+            if (PUSH) v[d] = V[d][j];
+        }
+        if (!isfinite((double)x[0])) continue;       // invalid (or beyond n)
         bool alive = true;
         if (PUSH) {
-            T v[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) v[d] = vs[d][p];
-            double v2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) v2 += (double)v[d] * (double)v[d];
-            ke += 0.5 * mass * v2;
-#pragma unroll
-            for (int d = 0; d < D; ++d) pm[d] += mass * (double)v[d];
-
-            // gather grad(phi) at x (field_interp.hpp:96-111)
-            T grad[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) grad[d] = (T)0;
-            double xd[D], s[D];
-            int c[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
-            if (base_cell<D>(g, xd, c, s)) {
-                T W[D][2], G[D][2];
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const T sd = (T)s[d];
-                    W[d][0] = (T)1 - sd;
-                    W[d][1] = sd;
-                    const T dr = (T)g.delta_r[d];
-                    G[d][0] = (T)-1 * dr;
-                    G[d][1] = (T)1 * dr;
-                }
-                T ph[1 << D];
-#pragma unroll
-                for (int i = 0; i < (1 << D); ++i) ph[i] = phi[support_index<D>(g, c, i)];
-#pragma unroll
-                for (int dd = 0; dd < D; ++dd) {
-                    T r = (T)0;
-#pragma unroll
-                    for (int i = 0; i < (1 << D); ++i) {
-                        const T gc = cic_grad_coef<T, D>(W, G, i, dd);
-                        if (fabs((double)gc) > 1e-10) r = r + ph[i] * gc;
-                    }
-                    grad[dd] = r;
-                }
-            }
-            // kick then drift (TSI/main.cpp:65-68)
+            alive = push_one<T, D, PUSH>(g, phi, qm, dt, mass, x, v, ke, pm);
+            if (!alive) ++absorbed;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const T e = -grad[d];
-                v[d] = v[d] + (e * qm) * dt;
-                x[d] = x[d] + v[d] * dt;
+                X[d][j] = x[d];
+                V[d][j] = v[d];
             }
-            // boundaries: bounded periodic wrap, then absorbing walls
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                if (g.periodic[d]) {
-                    const T lo = (T)g.plo[d], hi = (T)g.phi[d], L = (T)g.plen[d];
-#pragma unroll
-                    for (int r = 0; r < 2; ++r) {
-                        if (x[d] < lo) x[d] = x[d] + L;
-                        else if (x[d] >= hi) x[d] = x[d] - L;
-                    }
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-                if (!g.periodic[d]) {
-                    const double xx = (double)x[d];
-                    if (!(xx >= g.lb[d] && xx < g.ub[d])) alive = false;
-                }
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                xs[d][p] = x[d];
-                vs[d][p] = v[d];
-            }
-            if (!alive) {
-                xs[0][p] = (T)NAN;
-                ++absorbed;
-            }
+            moved = true;
         }
         if (DEPOSIT && alive) {
             double xd[D], s[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
-            if (base_cell<D>(g, xd, cw[k], s)) {
-                ok[k] = true;
+            if (base_cell<D>(g, xd, cw[j], s)) {
+                ok[j] = true;
 #pragma unroll
-                for (int d = 0; d < D; ++d) sw[k][d] = (T)s[d];
+                for (int d = 0; d < D; ++d) sw[j][d] = (T)s[d];
             } else {
                 ++oor;
+            }
+        }
+    }
+    if (PUSH && moved) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (full) {
+                store4(xs[d] + p0, X[d]);
+                store4(vs[d] + p0, V[d]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < kPPT; ++j)
+                    if (p0 + j < n) {
+                        xs[d][p0 + j] = X[d][j];
+                        vs[d][p0 + j] = V[d][j];
+                    }
             }
         }
     }
@@ -446,12 +517,12 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
     if (DEPOSIT) {
         int lmin[3] = {INT_MAX, INT_MAX, INT_MAX}, lmax[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
-        for (int k = 0; k < kPPT; ++k)
-            if (ok[k]) {
+        for (int j = 0; j < kPPT; ++j)
+            if (ok[j]) {
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    lmin[d] = min(lmin[d], cw[k][d]);
-                    lmax[d] = max(lmax[d], cw[k][d]);
+                    lmin[d] = min(lmin[d], cw[j][d]);
+                    lmax[d] = max(lmax[d], cw[j][d]);
                 }
             }
         __syncthreads();   // bmin/bmax initialised
@@ -477,32 +548,56 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         if (use_lds)
             for (int i = threadIdx.x; i < (int)tsize; i += kThreads) tile[i] = 0ull;
         __syncthreads();
+
+        constexpr int NS = 1 << D;
+        long long run[NS];
+        int cur[D];
+        bool have = false;
+        auto flush = [&]() {
 #pragma unroll
-        for (int k = 0; k < kPPT; ++k) {
-            if (!ok[k]) continue;
-            T W[D][2];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                W[d][0] = (T)1 - sw[k][d];
-                W[d][1] = sw[k][d];
-            }
-#pragma unroll
-            for (int i = 0; i < (1 << D); ++i) {
-                const T cf = cic_coef<T, D>(W, i);
-                if (!(fabs((double)cf) > 1e-10)) continue;
-                const double v = w_dep * (double)cf;
-                const long long term = __double2ll_rn(v * scale);
+            for (int q = 0; q < NS; ++q) {
+                const int i = (q + lane) & (NS - 1);
+                const long long t = run[i];
+                if (t == 0) continue;
                 if (use_lds) {
                     int li = 0;
 #pragma unroll
-                    for (int d = 0; d < D; ++d)
-                        li = li * e[d] + (cw[k][d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
-                    atomicAdd(&tile[li], (unsigned long long)term);
+                    for (int d = 0; d < D; ++d) li = li * e[d] + (cur[d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
+                    atomicAdd(&tile[li], (unsigned long long)t);
                 } else {
-                    atomicAdd(&acc[support_index<D>(g, cw[k], i)], (unsigned long long)term);
+                    atomicAdd(&acc[support_index<D>(g, cur, i)], (unsigned long long)t);
                 }
             }
+        };
+#pragma unroll
+        for (int j = 0; j < kPPT; ++j) {
+            if (!ok[j]) continue;
+            bool same = have;
+#pragma unroll
+            for (int d = 0; d < D; ++d) same = same && cur[d] == cw[j][d];
+            if (!same) {
+                if (have) flush();
+#pragma unroll
+                for (int d = 0; d < D; ++d) cur[d] = cw[j][d];
+#pragma unroll
+                for (int q = 0; q < NS; ++q) run[q] = 0;
+                have = true;
+            }
+            T W[D][2];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                W[d][0] = (T)1 - sw[j][d];
+                W[d][1] = sw[j][d];
+            }
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                const T cf = cic_coef<T, D>(W, i);
+                if (!(fabs((double)cf) > 1e-10)) continue;
+                const double v = w_dep * (double)cf;
+                run[i] += __double2ll_rn(v * scale);
+            }
         }
+        if (have) flush();
         if (use_lds) {
             __syncthreads();
             for (int i = threadIdx.x; i < (int)tsize; i += kThreads) {
@@ -554,13 +649,17 @@ __global__ __launch_bounds__(kThreads) void k_f64_to_f32(const double* __restric
     if (i < n) out[i] = (float)in[i];
 }
 
-// Fixed-order reduction of nb x 4 partials by one block.
+// Fixed-order reduction of nb x 4 partials: pass 1 has block b sum the
+// contiguous chunk [b*chunk, (b+1)*chunk) (lane-strided, then a tree), pass 2
+// sums the per-chunk results the same way in one block.
 __global__ __launch_bounds__(kThreads) void k_reduce4(const double* __restrict__ part, int64_t nb,
-                                                      double* __restrict__ out)
+                                                      int64_t chunk, double* __restrict__ out)
 {
     __shared__ double sh[4][kThreads];
     double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t b = threadIdx.x; b < nb; b += kThreads)
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    for (int64_t b = b0 + threadIdx.x; b < b1; b += kThreads)
 #pragma unroll
         for (int q = 0; q < 4; ++q) s[q] += part[b * 4 + q];
 #pragma unroll
@@ -572,7 +671,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce4(const double* __restrict__
             for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 4) out[(int64_t)blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // Field energy partials: one value per block (written to part[b*4]).
@@ -904,18 +1003,36 @@ psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos
     return PSUM_ERR_ARG;
 }
 
-psum_status psum_hip_reduce_diag(const double* diag, int64_t nblocks, double* out, void* stream)
+size_t psum_hip_reduce_workspace(void) { return (size_t)kReduceBlocks * 4 * sizeof(double); }
+
+static psum_status reduce4(const double* part, int64_t nb, double* out, double* work,
+                           hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(kThreads), 0, S(stream), diag, nblocks, out);
+    if (nb <= 4 * kThreads) {
+        hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(kThreads), 0, st, part, nb, nb, out);
+    } else {
+        const int64_t chunk = (nb + kReduceBlocks - 1) / kReduceBlocks;
+        const int64_t nr = (nb + chunk - 1) / chunk;
+        hipLaunchKernelGGL(k_reduce4, dim3((unsigned)nr), dim3(kThreads), 0, st, part, nb, chunk, work);
+        hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(kThreads), 0, st, work, nr, nr, out);
+    }
     PSUM_HIP_TRY(hipGetLastError());
     return PSUM_OK;
+}
+
+psum_status psum_hip_reduce_diag(const double* diag, int64_t nblocks, double* out, void* work,
+                                 void* stream)
+{
+    PSUM_ARG_CHECK(diag && out, "psum_hip_reduce_diag: null pointer");
+    PSUM_ARG_CHECK(work || nblocks <= 4 * kThreads, "psum_hip_reduce_diag: workspace required");
+    return reduce4(diag, nblocks, out, static_cast<double*>(work), S(stream));
 }
 
 size_t psum_hip_field_energy_workspace(const psum_grid* g)
 {
     if (!grid_ok(g)) return 0;
     const int64_t nb = (grid_nodes(*g) + kThreads - 1) / kThreads;
-    return (size_t)nb * 4 * sizeof(double);
+    return (size_t)nb * 4 * sizeof(double) + (size_t)kReduceBlocks * 4 * sizeof(double);
 }
 
 psum_status psum_hip_field_energy(const psum_grid* g, const double* phi, double eps, double* out,
@@ -936,8 +1053,8 @@ psum_status psum_hip_field_energy(const psum_grid* g, const double* phi, double 
     else
         hipLaunchKernelGGL(k_field_energy<3>, dim3(nb), dim3(kThreads), 0, st, dg, phi, nn, part);
     PSUM_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(kThreads), 0, st, part, nb, out);
-    PSUM_HIP_TRY(hipGetLastError());
+    psum_status rs = reduce4(part, nb, out, part + nb * 4, st);
+    if (rs != PSUM_OK) return rs;
     double vol = 1.0;
     for (int d = 0; d < g->dim; ++d) vol *= g->delta[d];
     hipLaunchKernelGGL(k_scale_first, dim3(1), dim3(64), 0, st, out, 0.5 * eps, vol);

@@ -34,7 +34,26 @@ void mg_set_stream(HipMultigrid* h, hipStream_t s);
 void mg_solve(HipMultigrid* h, double* b, double* x, bool sync);
 int mg_last_cycles(HipMultigrid* h);
 double mg_last_rel(HipMultigrid* h);
+void mg_solve_homogeneous(HipMultigrid* h, double* b, double* x);
+int mg_predicted(HipMultigrid* h);
+void mg_set_predicted(HipMultigrid* h, int p);
 }  // namespace psum_hip
+
+extern "C" {
+psum_status psum_hip_adjoint_push(const psum_grid* g, int dtype, void* const* x, void* const* lx,
+                                  void* const* lv, int64_t n, const void* phi, double qm, double dt,
+                                  const double* scale_dev, int64_t* acc, void* stream);
+int64_t psum_hip_adjoint_blocks(int64_t n);
+psum_status psum_hip_adjoint_gather(const psum_grid* g, int dtype, void* const* x, void* const* lx,
+                                    void* const* lv, int64_t n, const void* lrho, double w_dep,
+                                    double dt, double* bmax, void* stream);
+psum_status psum_hip_adjoint_scale(const double* bmax, int64_t nb, double factor, double* scale_dev,
+                                   int accumulate, void* stream);
+psum_status psum_hip_fixed_to_double_dev(const int64_t* acc, int64_t nn, const double* scale_dev,
+                                         double* out, void* stream);
+psum_status psum_hip_field_energy_grad(const psum_grid* g, const double* phi, double eps, double* out,
+                                       void* stream);
+}
 
 using namespace psum_hip;
 
@@ -82,6 +101,17 @@ __global__ __launch_bounds__(kT) void k_sum_final(const double* __restrict__ par
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// warm start by linear extrapolation in time: phi <- 2 phi - phi_prev, phi_prev <- phi
+__global__ __launch_bounds__(kT) void k_extrapolate(double* __restrict__ phi,
+                                                    double* __restrict__ prev, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const double t = phi[i];
+    phi[i] = 2.0 * t - prev[i];
+    prev[i] = t;
+}
+
 // x -= sum / n  (FP64 neutralisation of a periodic charge density)
 __global__ __launch_bounds__(kT) void k_sub_mean(double* __restrict__ x, int64_t n,
                                                  const double* __restrict__ sum)
@@ -89,6 +119,19 @@ __global__ __launch_bounds__(kT) void k_sub_mean(double* __restrict__ x, int64_t
     const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
     const double mean = sum[0] / (double)n;
     if (i < n) x[i] = x[i] - mean;
+}
+
+// lphi[0] -= sum(lphi): the consistent right-hand side of the adjoint of a
+// pinned periodic solve
+__global__ void k_sub_at0(double* __restrict__ x, const double* __restrict__ sum)
+{
+    if (threadIdx.x == 0) x[0] = x[0] - sum[0];
+}
+
+__global__ __launch_bounds__(kT) void k_zero_any(unsigned char* __restrict__ a, int64_t nbytes)
+{
+    const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i < nbytes) a[i] = 0;
 }
 
 // hist[5*s + k]: field energy, then species-summed kinetic energy and momentum
@@ -247,6 +290,7 @@ public:
         if (D == 3) opt += " K " + std::to_string(cfg_.grid.cells[2] + 1);
         opt += " depth " + std::to_string(cfg_.depth) + " pre_relax " + std::to_string(cfg_.pre) +
                " post_relax " + std::to_string(cfg_.post) + " max_cycles " + std::to_string(cfg_.max_cycles);
+        if (!cfg_.mg_graph) opt += " graph 0";
         if (cfg_.tol > 0) {
             char buf[64];
             std::snprintf(buf, sizeof buf, " tol %.17g", cfg_.tol);
@@ -283,11 +327,13 @@ public:
         S_TRY(hipMalloc(&acc_, nn_ * sizeof(int64_t)));
         S_TRY(hipMalloc(&b_, nn_ * sizeof(double)));
         S_TRY(hipMalloc(&phi_, nn_ * sizeof(double)));
+        if (cfg_.extrapolate) S_TRY(hipMalloc(&phi_prev_, nn_ * sizeof(double)));
         S_TRY(hipMemsetAsync(phi_, 0, nn_ * sizeof(double), st_));
         if (cfg_.dtype == PSUM_F32) S_TRY(hipMalloc(&phi32_, nn_ * sizeof(float)));
         const int64_t nbp = std::max<int64_t>(1, psum_hip_push_blocks(max_cap_));
         S_TRY(hipMalloc(&diag_, (size_t)nbp * 4 * sizeof(double)));
         S_TRY(hipMalloc(&red_, 16 * sizeof(double)));
+        S_TRY(hipMalloc(&redw_, psum_hip_reduce_workspace()));
         S_TRY(hipMalloc(&fe_, 4 * sizeof(double)));
         few_bytes_ = psum_hip_field_energy_workspace(&cfg_.grid);
         S_TRY(hipMalloc(&few_, few_bytes_));
@@ -313,12 +359,18 @@ public:
             for (int d = 0; d < 3; ++d)
                 for (void* p : {sp.pos[d], sp.vel[d], sp.pos2[d], sp.vel2[d]})
                     if (p) (void)hipFree(p);
-        for (void* p : {(void*)acc_, (void*)b_, (void*)phi_, (void*)phi32_, (void*)diag_, (void*)red_,
-                        (void*)fe_, few_, (void*)sum_part_, (void*)sum_, (void*)counters_,
+        for (void* p : {(void*)acc_, (void*)b_, (void*)phi_, (void*)phi_prev_, (void*)phi32_, (void*)diag_, (void*)red_,
+                        (void*)fe_, few_, redw_, (void*)sum_part_, (void*)sum_, (void*)counters_,
                         (void*)keys_, (void*)skeys_, (void*)perm_, sort_work_, (void*)nvalid_, (void*)hist_})
             if (p) (void)hipFree(p);
         for (auto& e : ev_)
             if (e) (void)hipEventDestroy(e);
+        for (void* p : {(void*)lphi_, (void*)lrho_, (void*)adj_scale_, (void*)bmax_, lrho_t_, (void*)adj_acc_})
+            if (p) (void)hipFree(p);
+        for (void* p : lx_) if (p) (void)hipFree(p);
+        for (void* p : lv_) if (p) (void)hipFree(p);
+        for (auto& v : seg_x_) for (void* p : v) if (p) (void)hipFree(p);
+        for (void* p : seg_phi_) if (p) (void)hipFree(p);
         if (st_) (void)hipStreamDestroy(st_);
     }
 
@@ -356,23 +408,10 @@ public:
         }
         const int D = cfg_.grid.dim;
         for (int s = 0; s < steps; ++s) {
-            if (!acc_valid_) deposit_all();
-            if (cfg_.sort_every > 0 && step_ % cfg_.sort_every == 0) sort_all();
-            CHK(psum_hip_fixed_to_double(acc_, nn_, 1.0 / scale_, cfg_.rho_bg, b_, st_));
-            if (all_periodic_) {
-                hipLaunchKernelGGL(k_sum_part, dim3(nblk(nn_)), dim3(kT), 0, st_, b_, nn_, sum_part_);
-                hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kT), 0, st_, sum_part_, (int64_t)nblk(nn_), sum_);
-                hipLaunchKernelGGL(k_sub_mean, dim3(nblk(nn_)), dim3(kT), 0, st_, b_, nn_, sum_);
-            }
-            mg_solve(mg_, b_, phi_, false);
-            cycles_ += mg_last_cycles(mg_);
-            last_rel_ = mg_last_rel(mg_);
+            if (cfg_.sort_every > 0 && !no_sort_ && step_ % cfg_.sort_every == 0) sort_all();
+            solve_field();
             CHK(psum_hip_field_energy(&cfg_.grid, phi_, cfg_.eps, fe_, few_, few_bytes_, st_));
-            const void* ph = phi_;
-            if (cfg_.dtype == PSUM_F32) {
-                CHK(psum_hip_f64_to_f32(phi_, phi32_, nn_, st_));
-                ph = phi32_;
-            }
+            const void* ph = cfg_.dtype == PSUM_F32 ? (const void*)phi32_ : (const void*)phi_;
             hipLaunchKernelGGL(k_zero_i64, dim3(nblk(nn_)), dim3(kT), 0, st_, (long long*)acc_, nn_);
             for (int q = 0; q < cfg_.nspecies; ++q) {
                 Species& sp = sp_[q];
@@ -385,7 +424,7 @@ public:
                     push_particles_ += sp.n;
                     if (++nev_ == kEv) flush_events();
                 }
-                CHK(psum_hip_reduce_diag(diag_, psum_hip_push_blocks(sp.n), red_ + 4 * q, st_));
+                CHK(psum_hip_reduce_diag(diag_, psum_hip_push_blocks(sp.n), red_ + 4 * q, redw_, st_));
                 if (sp.n == 0) S_TRY(hipMemsetAsync(red_ + 4 * q, 0, 4 * sizeof(double), st_));
             }
             hipLaunchKernelGGL(k_hist_record, dim3(1), dim3(64), 0, st_, fe_, red_, cfg_.nspecies, hist_, s);
@@ -419,6 +458,282 @@ public:
         if (particles) *particles = push_particles_;
     }
 
+    // rho from the exact deposit of the current positions, neutralised,
+    // then phi by the multigrid (warm start); phi32 refreshed in FP32 mode
+    void solve_field()
+    {
+        if (!acc_valid_) deposit_all();
+        CHK(psum_hip_fixed_to_double(acc_, nn_, 1.0 / scale_, cfg_.rho_bg, b_, st_));
+        if (all_periodic_) {
+            hipLaunchKernelGGL(k_sum_part, dim3(nblk(nn_)), dim3(kT), 0, st_, b_, nn_, sum_part_);
+            hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kT), 0, st_, sum_part_, (int64_t)nblk(nn_), sum_);
+            hipLaunchKernelGGL(k_sub_mean, dim3(nblk(nn_)), dim3(kT), 0, st_, b_, nn_, sum_);
+        }
+        if (cfg_.extrapolate && have_prev_)
+            hipLaunchKernelGGL(k_extrapolate, dim3(nblk(nn_)), dim3(kT), 0, st_, phi_, phi_prev_, nn_);
+        mg_solve(mg_, b_, phi_, false);
+        if (cfg_.extrapolate && !have_prev_) {
+            S_TRY(hipMemcpyAsync(phi_prev_, phi_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+            have_prev_ = true;
+        }
+        cycles_ += mg_last_cycles(mg_);
+        last_rel_ = mg_last_rel(mg_);
+        if (cfg_.dtype == PSUM_F32) CHK(psum_hip_f64_to_f32(phi_, phi32_, nn_, st_));
+    }
+
+    // Loss L = field energy of phi solved from the current positions.
+    double field_energy_now()
+    {
+        solve_field();
+        CHK(psum_hip_field_energy(&cfg_.grid, phi_, cfg_.eps, fe_, few_, few_bytes_, st_));
+        double e = 0.0;
+        S_TRY(hipMemcpyAsync(&e, fe_, sizeof(double), hipMemcpyDeviceToHost, st_));
+        S_TRY(hipStreamSynchronize(st_));
+        return e;
+    }
+
+    double forward_loss(int steps)
+    {
+        no_sort_ = true;
+        run(steps, nullptr);
+        no_sort_ = false;
+        return field_energy_now();
+    }
+
+    // ---------------------------------------------------------- adjoint
+    struct Snapshot {
+        std::vector<void*> x, v;   // per species * D
+        double* phi = nullptr;
+        double* phi_prev = nullptr;
+        bool have_prev = false;
+        int predicted = 1;
+    };
+
+    void snap_alloc(Snapshot& s)
+    {
+        const int D = cfg_.grid.dim;
+        s.x.assign(cfg_.nspecies * D, nullptr);
+        s.v.assign(cfg_.nspecies * D, nullptr);
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            for (int d = 0; d < D; ++d) {
+                const size_t b = (size_t)std::max<int64_t>(1, sp_[q].n) * esz_;
+                S_TRY(hipMalloc(&s.x[q * D + d], b));
+                S_TRY(hipMalloc(&s.v[q * D + d], b));
+            }
+        S_TRY(hipMalloc(&s.phi, nn_ * sizeof(double)));
+        if (cfg_.extrapolate) S_TRY(hipMalloc(&s.phi_prev, nn_ * sizeof(double)));
+    }
+
+    void snap_free(Snapshot& s)
+    {
+        for (void* p : s.x) if (p) (void)hipFree(p);
+        for (void* p : s.v) if (p) (void)hipFree(p);
+        if (s.phi) (void)hipFree(s.phi);
+        if (s.phi_prev) (void)hipFree(s.phi_prev);
+        s = Snapshot{};
+    }
+
+    void snap_save(Snapshot& s)
+    {
+        const int D = cfg_.grid.dim;
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            for (int d = 0; d < D; ++d) {
+                const size_t b = (size_t)sp_[q].n * esz_;
+                S_TRY(hipMemcpyAsync(s.x[q * D + d], sp_[q].pos[d], b, hipMemcpyDeviceToDevice, st_));
+                S_TRY(hipMemcpyAsync(s.v[q * D + d], sp_[q].vel[d], b, hipMemcpyDeviceToDevice, st_));
+            }
+        S_TRY(hipMemcpyAsync(s.phi, phi_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+        if (cfg_.extrapolate && have_prev_)
+            S_TRY(hipMemcpyAsync(s.phi_prev, phi_prev_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+        s.have_prev = have_prev_;
+        s.predicted = mg_predicted(mg_);
+    }
+
+    void snap_restore(const Snapshot& s)
+    {
+        const int D = cfg_.grid.dim;
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            for (int d = 0; d < D; ++d) {
+                const size_t b = (size_t)sp_[q].n * esz_;
+                S_TRY(hipMemcpyAsync(sp_[q].pos[d], s.x[q * D + d], b, hipMemcpyDeviceToDevice, st_));
+                S_TRY(hipMemcpyAsync(sp_[q].vel[d], s.v[q * D + d], b, hipMemcpyDeviceToDevice, st_));
+            }
+        S_TRY(hipMemcpyAsync(phi_, s.phi, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+        if (cfg_.extrapolate && s.have_prev)
+            S_TRY(hipMemcpyAsync(phi_prev_, s.phi_prev, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+        have_prev_ = s.have_prev;
+        mg_set_predicted(mg_, s.predicted);
+        acc_valid_ = false;
+    }
+
+    // lrho = S^T lphi (lphi in lphi_, overwritten); result in lrho_ (double) and lrho_t_
+    void adjoint_solve()
+    {
+        if (all_periodic_) {
+            hipLaunchKernelGGL(k_sum_part, dim3(nblk(nn_)), dim3(kT), 0, st_, lphi_, nn_, sum_part_);
+            hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kT), 0, st_, sum_part_, (int64_t)nblk(nn_), sum_);
+            hipLaunchKernelGGL(k_sub_at0, dim3(1), dim3(64), 0, st_, lphi_, sum_);
+        }
+        mg_solve_homogeneous(mg_, lphi_, lrho_);
+        if (all_periodic_) {
+            S_TRY(hipMemcpyAsync(lphi_, lrho_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+            hipLaunchKernelGGL(k_sum_part, dim3(nblk(nn_)), dim3(kT), 0, st_, lphi_, nn_, sum_part_);
+            hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kT), 0, st_, sum_part_, (int64_t)nblk(nn_), sum_);
+            hipLaunchKernelGGL(k_sub_mean, dim3(nblk(nn_)), dim3(kT), 0, st_, lphi_, nn_, sum_);
+        } else {
+            S_TRY(hipMemcpyAsync(lphi_, lrho_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+        }
+        // lphi_ now holds lrho (zero-mean); lrho_ keeps the warm start
+        if (cfg_.dtype == PSUM_F32) CHK(psum_hip_f64_to_f32(lphi_, (float*)lrho_t_, nn_, st_));
+    }
+
+    const void* lrho_view() const
+    {
+        return cfg_.dtype == PSUM_F32 ? (const void*)lrho_t_ : (const void*)lphi_;
+    }
+
+    void adjoint_gather_all(void* const* xs_of_species, double* bmax_scale_factor)
+    {
+        (void)bmax_scale_factor;
+        const int D = cfg_.grid.dim;
+        double maxfac = 0.0;
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            double dr = 0.0;
+            for (int d = 0; d < D; ++d) dr = std::max(dr, cfg_.grid.delta_r[d]);
+            maxfac = std::max(maxfac, std::fabs(sp_[q].sp.q / sp_[q].sp.m) * cfg_.dt * dr);
+        }
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            void* const* x = xs_of_species + q * 3;
+            CHK(psum_hip_adjoint_gather(&cfg_.grid, cfg_.dtype, x, &lx_[q * 3], &lv_[q * 3], sp_[q].n,
+                                        lrho_view(), sp_[q].wdep, cfg_.dt, bmax_, st_));
+            CHK(psum_hip_adjoint_scale(bmax_, psum_hip_adjoint_blocks(sp_[q].n), maxfac, adj_scale_,
+                                       q > 0 ? 1 : 0, st_));
+        }
+    }
+
+    void adjoint_push_all(void* const* xs_of_species, const void* phi_t)
+    {
+        hipLaunchKernelGGL(k_zero_i64, dim3(nblk(nn_)), dim3(kT), 0, st_, (long long*)adj_acc_, nn_);
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            void* const* x = xs_of_species + q * 3;
+            CHK(psum_hip_adjoint_push(&cfg_.grid, cfg_.dtype, x, &lx_[q * 3], &lv_[q * 3], sp_[q].n,
+                                      phi_t, sp_[q].sp.q / sp_[q].sp.m, cfg_.dt, adj_scale_,
+                                      adj_acc_, st_));
+        }
+        CHK(psum_hip_fixed_to_double_dev(adj_acc_, nn_, adj_scale_, lphi_, st_));
+    }
+
+    // dL/dx0, dL/dv0 of L = field energy after `steps` steps from the current
+    // state; forward checkpoints every `ckpt` steps, recompute per segment.
+    double adjoint(int steps, int ckpt, void* const* gx, void* const* gv, double* t_fwd, double* t_bwd)
+    {
+        if (steps < 1 || ckpt < 1) throw std::invalid_argument("psum_stepper_adjoint: steps, ckpt >= 1");
+        const int D = cfg_.grid.dim;
+        no_sort_ = true;
+        adj_buffers();
+        hipEvent_t e0, e1, e2;
+        S_TRY(hipEventCreate(&e0));
+        S_TRY(hipEventCreate(&e1));
+        S_TRY(hipEventCreate(&e2));
+        S_TRY(hipEventRecord(e0, st_));
+        const int nck = (steps + ckpt - 1) / ckpt;
+        std::vector<Snapshot> cks(nck);
+        for (auto& c : cks) snap_alloc(c);
+        for (int j = 0; j < nck; ++j) {
+            snap_save(cks[j]);
+            run(std::min(ckpt, steps - j * ckpt), nullptr);
+        }
+        const double loss = field_energy_now();
+        S_TRY(hipEventRecord(e1, st_));
+        // final: lphi = dW/dphi(phi^N), lrho = S^T lphi, lx = sum w grad(lrho)(x^N), lv = 0
+        CHK(psum_hip_field_energy_grad(&cfg_.grid, phi_, cfg_.eps, lphi_, st_));
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            for (int d = 0; d < D; ++d) {
+                hipLaunchKernelGGL(k_zero_any, dim3(nblk((int64_t)sp_[q].n * esz_)), dim3(kT), 0, st_, (unsigned char*)lx_[q * 3 + d], (int64_t)sp_[q].n * esz_);
+                hipLaunchKernelGGL(k_zero_any, dim3(nblk((int64_t)sp_[q].n * esz_)), dim3(kT), 0, st_, (unsigned char*)lv_[q * 3 + d], (int64_t)sp_[q].n * esz_);
+            }
+        hipLaunchKernelGGL(k_zero_any, dim3(nblk(nn_ * 8)), dim3(kT), 0, st_, (unsigned char*)lrho_, nn_ * 8);
+        adjoint_solve();
+        {
+            std::vector<void*> xn(cfg_.nspecies * 3, nullptr);
+            for (int q = 0; q < cfg_.nspecies; ++q)
+                for (int d = 0; d < D; ++d) xn[q * 3 + d] = sp_[q].pos[d];
+            adjoint_gather_all(xn.data(), nullptr);
+        }
+        // reverse sweep, one segment at a time
+        for (int j = nck - 1; j >= 0; --j) {
+            const int s0 = j * ckpt, len = std::min(ckpt, steps - s0);
+            snap_restore(cks[j]);
+            for (int k = 0; k < len; ++k) {
+                for (int q = 0; q < cfg_.nspecies; ++q)
+                    for (int d = 0; d < D; ++d)
+                        S_TRY(hipMemcpyAsync(seg_x_[k][q * 3 + d], sp_[q].pos[d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
+                run(1, nullptr);   // solves phi^(s0+k) then pushes
+                const void* ph = cfg_.dtype == PSUM_F32 ? (const void*)phi32_ : (const void*)phi_;
+                S_TRY(hipMemcpyAsync(seg_phi_[k], ph, (size_t)nn_ * esz_, hipMemcpyDeviceToDevice, st_));
+            }
+            for (int k = len - 1; k >= 0; --k) {
+                adjoint_push_all(seg_x_[k].data(), seg_phi_[k]);
+                adjoint_solve();
+                adjoint_gather_all(seg_x_[k].data(), nullptr);
+            }
+        }
+        S_TRY(hipEventRecord(e2, st_));
+        // outputs and restore
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            for (int d = 0; d < D; ++d) {
+                if (gx) S_TRY(hipMemcpyAsync(gx[q * 3 + d], lx_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
+                if (gv) S_TRY(hipMemcpyAsync(gv[q * 3 + d], lv_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
+            }
+        snap_restore(cks[0]);
+        S_TRY(hipStreamSynchronize(st_));
+        float ms1 = 0, ms2 = 0;
+        S_TRY(hipEventElapsedTime(&ms1, e0, e1));
+        S_TRY(hipEventElapsedTime(&ms2, e1, e2));
+        if (t_fwd) *t_fwd = ms1;
+        if (t_bwd) *t_bwd = ms2;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        for (auto& c : cks) snap_free(c);
+        no_sort_ = false;
+        return loss;
+    }
+
+    void adj_buffers()
+    {
+        if (lphi_) return;
+        const int D = cfg_.grid.dim;
+        S_TRY(hipMalloc(&lphi_, nn_ * sizeof(double)));
+        S_TRY(hipMalloc(&lrho_, nn_ * sizeof(double)));
+        S_TRY(hipMalloc(&lrho_t_, nn_ * sizeof(double)));
+        S_TRY(hipMalloc(&adj_acc_, nn_ * sizeof(int64_t)));
+        S_TRY(hipMalloc(&adj_scale_, 4 * sizeof(double)));
+        int64_t nbmax = 1;
+        for (int q = 0; q < cfg_.nspecies; ++q) nbmax = std::max(nbmax, psum_hip_adjoint_blocks(sp_[q].sp.capacity));
+        S_TRY(hipMalloc(&bmax_, (size_t)nbmax * sizeof(double)));
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            for (int d = 0; d < D; ++d) {
+                S_TRY(hipMalloc(&lx_[q * 3 + d], (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_));
+                S_TRY(hipMalloc(&lv_[q * 3 + d], (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_));
+            }
+    }
+
+    void set_ckpt_capacity(int ckpt)
+    {
+        const int D = cfg_.grid.dim;
+        for (auto& v : seg_x_) for (void* p : v) if (p) (void)hipFree(p);
+        for (void* p : seg_phi_) if (p) (void)hipFree(p);
+        seg_x_.assign(ckpt, std::vector<void*>(cfg_.nspecies * 3, nullptr));
+        seg_phi_.assign(ckpt, nullptr);
+        for (int k = 0; k < ckpt; ++k) {
+            for (int q = 0; q < cfg_.nspecies; ++q)
+                for (int d = 0; d < D; ++d)
+                    S_TRY(hipMalloc(&seg_x_[k][q * 3 + d], (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_));
+            S_TRY(hipMalloc(&seg_phi_[k], (size_t)nn_ * esz_));
+        }
+    }
+
     void phi(double* out)
     {
         S_TRY(hipMemcpyAsync(out, phi_, nn_ * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -438,6 +753,14 @@ public:
     void sync() { S_TRY(hipStreamSynchronize(st_)); }
 
 private:
+    bool no_sort_ = false;
+    double *lphi_ = nullptr, *lrho_ = nullptr, *adj_scale_ = nullptr, *bmax_ = nullptr;
+    void* lrho_t_ = nullptr;
+    int64_t* adj_acc_ = nullptr;
+    void* lx_[12] = {};
+    void* lv_[12] = {};
+    std::vector<std::vector<void*>> seg_x_;
+    std::vector<void*> seg_phi_;
     static constexpr int kEv = 64;
     hipEvent_t ev_[2 * kEv] = {};
     int nev_ = 0;
@@ -469,6 +792,9 @@ private:
     double scale_ = 1.0;
     int64_t* acc_ = nullptr;
     double *b_ = nullptr, *phi_ = nullptr, *diag_ = nullptr, *red_ = nullptr, *fe_ = nullptr;
+    double* phi_prev_ = nullptr;
+    bool have_prev_ = false;
+    void* redw_ = nullptr;
     float* phi32_ = nullptr;
     void* few_ = nullptr;
     size_t few_bytes_ = 0;
@@ -609,6 +935,24 @@ psum_status psum_stepper_push_timing(void* h, double* ms, int64_t* launches, int
 {
     if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
     return guarded([&]() { static_cast<Stepper*>(h)->push_timing(ms, launches, particles); });
+}
+
+psum_status psum_stepper_forward_loss(void* h, int steps, double* loss)
+{
+    if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() { *loss = static_cast<Stepper*>(h)->forward_loss(steps); });
+}
+
+psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, void* const* grad_x,
+                                 void* const* grad_v, double* t_fwd_ms, double* t_bwd_ms)
+{
+    if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() {
+        Stepper* s = static_cast<Stepper*>(h);
+        s->set_ckpt_capacity(ckpt);
+        const double l = s->adjoint(steps, ckpt, grad_x, grad_v, t_fwd_ms, t_bwd_ms);
+        if (loss) *loss = l;
+    });
 }
 
 void psum_stepper_destroy(void* h) { delete static_cast<Stepper*>(h); }

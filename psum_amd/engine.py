@@ -205,7 +205,9 @@ def gather_push(g, pos, vel, phi, qm, dt, mass=1.0, w_dep=None, scale=None, coun
                                      _p(diag), _p(counters), _stream()), "psum_hip_gather_push")
     out = torch.zeros(4, dtype=torch.float64, device=dev)
     if nb > 0:
-        check(lib().psum_hip_reduce_diag(_p(diag), nb, _p(out), _stream()), "psum_hip_reduce_diag")
+        work = torch.empty(lib().psum_hip_reduce_workspace() // 8, dtype=torch.float64, device=dev)
+        check(lib().psum_hip_reduce_diag(_p(diag), nb, _p(out), _p(work), _stream()),
+              "psum_hip_reduce_diag")
     return acc, out
 
 
@@ -296,7 +298,7 @@ class Stepper:
 
     def __init__(self, g, species, dtype="f64", mixed_mg=False, depth=4, pre=2, post=1,
                  max_cycles=20, tol=1e-10, eps=1.0, dt=0.1, rho_bg=0.0, sort_every=0,
-                 fixed=None, fixed_val=None, capacity=None):
+                 fixed=None, fixed_val=None, capacity=None, mg_graph=True, extrapolate=False):
         cfg = PsumStepConfig()
         cfg.grid = g
         cfg.dtype = PSUM_F64 if dtype == "f64" else PSUM_F32
@@ -304,6 +306,8 @@ class Stepper:
         cfg.nspecies = len(species)
         cfg.depth, cfg.pre, cfg.post, cfg.max_cycles = depth, pre, post, max_cycles
         cfg.sort_every = sort_every
+        cfg.mg_graph = int(bool(mg_graph))
+        cfg.extrapolate = int(bool(extrapolate))
         cfg.tol, cfg.eps, cfg.dt, cfg.rho_bg = tol, eps, dt, rho_bg
         self._keep = []
         if fixed is not None:

@@ -131,6 +131,10 @@ def lib():
                                       C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                       P, P, P]
         L.orc_pic_run_f64.restype = i64
+        L.orc_pic_run_multi.argtypes = [C.POINTER(Grid), C.c_int, P, P, P, P, P, P, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_double, C.c_int, C.c_int, P, P, P, P]
+        L.orc_pic_run_multi.restype = i64
         _lib = L
     return _lib
 
@@ -306,4 +310,35 @@ def pic_run(g, pos, vel, q, m, w, rho_bg, eps, dt, steps, depth, pre=2, post=1, 
                                 _ptr(hist), _ptr(phi))
     if cyc < 0:
         raise RuntimeError("orc_pic_run_f64 failed")
+    return hist, phi, cyc
+
+
+def pic_run_multi(g, species, rho_bg, eps, dt, steps, depth, pre=2, post=1, tol=1e-10,
+                  max_cycles=20, dep_mode=0, fixed=None, fixed_val=None):
+    """species: list of dicts (q, m, w, pos, vel) with float64 arrays, updated
+    in place.  Returns (hist[steps,5], phi, total_cycles)."""
+    ns = len(species)
+    keep = []
+    parr = (C.c_void_p * ns)()
+    varr = (C.c_void_p * ns)()
+    for i, sp in enumerate(species):
+        pa, va = _ptr_array(sp["pos"]), _ptr_array(sp["vel"])
+        keep += [pa, va]
+        parr[i] = C.cast(pa, C.c_void_p).value
+        varr[i] = C.cast(va, C.c_void_p).value
+    n = np.array([len(sp["pos"][0]) for sp in species], np.int64)
+    q = np.array([sp["q"] for sp in species], np.float64)
+    m = np.array([sp["m"] for sp in species], np.float64)
+    w = np.array([sp["w"] for sp in species], np.float64)
+    hist = np.zeros((steps, 5), np.float64)
+    phi = np.zeros(num_nodes(g), np.float64)
+    fx = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+    fv = None if fixed_val is None else np.ascontiguousarray(fixed_val, np.float64)
+    cyc = lib().orc_pic_run_multi(C.byref(g), ns, C.cast(parr, C.c_void_p),
+                                  C.cast(varr, C.c_void_p), _ptr(n), _ptr(q), _ptr(m), _ptr(w),
+                                  rho_bg, eps, dt, steps, depth, pre, post, tol, max_cycles,
+                                  dep_mode, None if fx is None else _ptr(fx),
+                                  None if fv is None else _ptr(fv), _ptr(hist), _ptr(phi))
+    if cyc < 0:
+        raise RuntimeError("orc_pic_run_multi failed")
     return hist, phi, cyc

@@ -78,3 +78,46 @@ def test_sorting_does_not_change_history():
     # particle order changes only the diagnostic sums' rounding
     assert np.array_equal(a[:, 0], b[:, 0])
     assert np.max(np.abs(a[:, 1] - b[:, 1]) / np.abs(a[:, 1])) < 1e-12
+
+
+def test_bounded_3d_two_species_matches_oracle():
+    """3D box, phi = 0 on all faces, absorbing walls, electrons + heavy ions."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import bounded_3d
+    wl = bounded_3d(cells=16, ppc=4, seed=8, dtype=np.float64)
+    steps = 8
+    og = _oracle_grid(wl["grid"])
+    ref_sp = [dict(s, pos=[a.copy() for a in s["pos"]], vel=[a.copy() for a in s["vel"]])
+              for s in wl["species"]]
+    h_ref, phi_ref, _ = O.pic_run_multi(og, ref_sp, 0.0, 1.0, wl["dt"], steps, depth=3, tol=1e-10,
+                                        max_cycles=40, fixed=wl["fixed"])
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f64", depth=3, tol=1e-10, max_cycles=40,
+                   dt=wl["dt"], rho_bg=0.0, fixed=wl["fixed"])
+    h = st.run(steps)
+    assert np.max(np.abs(h[:, 0] - h_ref[:, 0]) / np.abs(h_ref[:, 0])) < 1e-8
+    assert np.max(np.abs(h[:, 1] - h_ref[:, 1]) / np.abs(h_ref[:, 1])) < 1e-10
+    for s in range(2):
+        gp, gv = st.state(s)
+        for a, b in zip(ref_sp[s]["pos"] + ref_sp[s]["vel"], gp + gv):
+            assert np.array_equal(np.isnan(a), np.isnan(b))
+            ok = ~np.isnan(a)
+            assert np.max(np.abs(a[ok] - b[ok])) < 1e-9
+    assert st.stats()["absorbed"] > 0
+
+
+def test_bounded_3d_sort_compacts_absorbed():
+    from psum_amd import engine as E
+    from psum_amd.workloads import bounded_3d
+    wl = bounded_3d(cells=12, ppc=4, seed=9, dtype=np.float32)
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f32", mixed_mg=True, depth=3, tol=1e-8,
+                   max_cycles=40, dt=wl["dt"], rho_bg=0.0, fixed=wl["fixed"], sort_every=3)
+    st.run(10)
+    absorbed = st.stats()["absorbed"]
+    gp, _ = st.state(0)
+    gi, _ = st.state(1)
+    n0 = sum(len(s["pos"][0]) for s in wl["species"])
+    alive = len(gp[0]) + len(gi[0])
+    # particles absorbed in the last (unsorted) steps are still in the arrays as NaN
+    nan_left = int(np.isnan(gp[0]).sum() + np.isnan(gi[0]).sum())
+    assert alive - nan_left == n0 - absorbed
+    assert absorbed > 0 and alive < n0
