@@ -67,6 +67,24 @@ def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every, pre=2
     return res
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "push_deposit_two_stream_1024_hbm_traffic.md")
+
+
+def profiled_traffic(kernel, grid, ppc):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), for the workload it was measured on;
+    None for other workloads or when the summary is absent."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            f = line.split()
+            if len(f) == 5 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel \
+                    and int(f[2]) == grid and int(f[3]) == ppc:
+                return float(f[4])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(wl, steps):
     """The CPU oracle (FP64 restatement of the reference step, OpenMP) on the
     first `steps` steps of the same workload."""
@@ -115,9 +133,13 @@ def main():
     ap.add_argument("--extrapolate", action="store_true")
     ap.add_argument("--sort-every", type=int, default=50)
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the runs, copy one buffer of the push kernel's algorithmic size "
+                         "(FP32 x, y, vx, vy of every particle) so that HBM counters can be "
+                         "scaled by a plain copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported",
@@ -151,9 +173,18 @@ def main():
     bytes_per_launch = mixed["particles_per_launch"] * 2 * (2 * dim) * 4
     achieved = bytes_per_launch / (mixed["push_ms_avg"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": profiled_traffic("k_push_deposit<float,2,true,true>", args.grid, args.ppc),
                 "kernel": "k_push_deposit<float,2,true,true>",
                 "bytes_per_launch": bytes_per_launch, "avg_launch_ms": mixed["push_ms_avg"]}
+    if args.calib_copy:
+        src = torch.ones(int(mixed["particles_per_launch"]) * 2 * dim, dtype=torch.float32,
+                         device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} written")
+        del src, dst
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_steps)

@@ -33,6 +33,11 @@
  *                          reduction in 10 V(2,1) cycles on 256^2
  *                          (test/field_solver/test_multigrid_core.cpp:142-209)
  *   orc_field_energy etc.  sum-by-parts identity 1/2 sum phi*rho = 1/2 eps |grad phi|^2
+ *   orc_pic_adjoint        central finite differences of orc_pic_loss in
+ *                          directions where no particle changes cell
+ *                          (key_hash equal at +h and -h); dot-product test
+ *                          against orc_pic_tangent, an independent
+ *                          forward-mode derivation
  */
 #ifndef PSUM_ORACLE_H
 #define PSUM_ORACLE_H
@@ -173,6 +178,34 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
                           int post, double tol, int max_cycles, int dep_mode,
                           const uint8_t* fixed, const double* fixed_val, double* hist,
                           double* phi_out);
+
+/* ---- derivatives (no reference counterpart: the reference has no adjoint) */
+#define ORC_MAX_SPECIES 4
+
+typedef struct {
+    const orc_grid* g;
+    int nspecies;
+    int64_t n[ORC_MAX_SPECIES];
+    double q[ORC_MAX_SPECIES], m[ORC_MAX_SPECIES], w[ORC_MAX_SPECIES];
+    double rho_bg, eps, dt;
+    int steps, depth, pre, post, max_cycles;
+    double tol;
+    const uint8_t* fixed;       /* may be NULL on all-periodic grids */
+    const double* fixed_val;    /* may be NULL (= 0) */
+} orc_pic_problem;
+
+/* L = orc_field_energy of the potential solved from the positions after
+ * P->steps steps of the orc_pic_run_multi step (fixed-point deposit, solves
+ * to P->tol from the previous potential).  pos/vel[s][d] are not modified.
+ * key_hash (may be NULL): FNV-1a hash of every particle's cell key at every
+ * step, for telling whether two runs stayed in the same cells. */
+double orc_pic_loss(const orc_pic_problem* P, double*** pos, double*** vel, uint64_t* key_hash);
+/* Reverse mode: returns L and writes dL/dx0, dL/dv0 into gx/gv[s][d]. */
+double orc_pic_adjoint(const orc_pic_problem* P, double*** pos, double*** vel, double*** gx,
+                       double*** gv);
+/* Forward mode: returns dL = <grad L, (dpos, dvel)>; *loss = L. */
+double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, double*** dpos,
+                       double*** dvel, double* loss);
 
 #ifdef __cplusplus
 }

@@ -135,12 +135,97 @@ def lib():
                                         C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_double, C.c_int, C.c_int, P, P, P, P]
         L.orc_pic_run_multi.restype = i64
+        L.orc_pic_loss.argtypes = [C.POINTER(PicProblem), P, P, P]
+        L.orc_pic_loss.restype = C.c_double
+        L.orc_pic_adjoint.argtypes = [C.POINTER(PicProblem), P, P, P, P]
+        L.orc_pic_adjoint.restype = C.c_double
+        L.orc_pic_tangent.argtypes = [C.POINTER(PicProblem), P, P, P, P, P]
+        L.orc_pic_tangent.restype = C.c_double
         _lib = L
     return _lib
 
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+MAX_SPECIES = 4
+
+
+class PicProblem(C.Structure):
+    _fields_ = [("g", C.POINTER(Grid)), ("nspecies", C.c_int),
+                ("n", C.c_int64 * MAX_SPECIES), ("q", C.c_double * MAX_SPECIES),
+                ("m", C.c_double * MAX_SPECIES), ("w", C.c_double * MAX_SPECIES),
+                ("rho_bg", C.c_double), ("eps", C.c_double), ("dt", C.c_double),
+                ("steps", C.c_int), ("depth", C.c_int), ("pre", C.c_int), ("post", C.c_int),
+                ("max_cycles", C.c_int), ("tol", C.c_double), ("fixed", C.c_void_p),
+                ("fixed_val", C.c_void_p)]
+
+
+class Pic:
+    """Derivatives of L = field energy after `steps` steps (orc_pic_loss /
+    orc_pic_adjoint / orc_pic_tangent).  species: dicts with q, m, w and the
+    FP64 pos / vel arrays of the initial state (not modified)."""
+
+    def __init__(self, g, species, rho_bg, dt, steps, eps=1.0, depth=3, pre=2, post=1,
+                 tol=1e-13, max_cycles=80, fixed=None, fixed_val=None):
+        self.g = g
+        self.species = species
+        self.fixed = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+        self.fixed_val = None if fixed_val is None else np.ascontiguousarray(fixed_val, np.float64)
+        P = PicProblem()
+        P.g = C.pointer(g)
+        P.nspecies = len(species)
+        for i, sp in enumerate(species):
+            P.n[i] = len(sp["pos"][0])
+            P.q[i], P.m[i], P.w[i] = sp["q"], sp["m"], sp["w"]
+        P.rho_bg, P.eps, P.dt = rho_bg, eps, dt
+        P.steps, P.depth, P.pre, P.post, P.max_cycles, P.tol = steps, depth, pre, post, max_cycles, tol
+        P.fixed = None if self.fixed is None else self.fixed.ctypes.data
+        P.fixed_val = None if self.fixed_val is None else self.fixed_val.ctypes.data
+        self.P = P
+
+    def _arrs(self, per_species):
+        keep = []
+        top = (C.c_void_p * len(per_species))()
+        for i, arrs in enumerate(per_species):
+            arrs = [np.ascontiguousarray(a, np.float64) for a in arrs]
+            pa = _ptr_array(arrs)
+            keep += [pa, arrs]
+            top[i] = C.cast(pa, C.c_void_p).value
+        keep.append(top)
+        return C.cast(top, C.c_void_p), keep
+
+    def loss(self, pos=None, vel=None):
+        """(L, key_hash); pos / vel default to the species' arrays."""
+        pos = pos or [sp["pos"] for sp in self.species]
+        vel = vel or [sp["vel"] for sp in self.species]
+        pp, k1 = self._arrs(pos)
+        vp, k2 = self._arrs(vel)
+        h = C.c_uint64(0)
+        L = lib().orc_pic_loss(C.byref(self.P), pp, vp, C.cast(C.pointer(h), C.c_void_p))
+        return L, h.value
+
+    def adjoint(self):
+        """(L, grad_x[s][d], grad_v[s][d])"""
+        pp, k1 = self._arrs([sp["pos"] for sp in self.species])
+        vp, k2 = self._arrs([sp["vel"] for sp in self.species])
+        gx = [[np.zeros_like(a, dtype=np.float64) for a in sp["pos"]] for sp in self.species]
+        gv = [[np.zeros_like(a, dtype=np.float64) for a in sp["vel"]] for sp in self.species]
+        gxp, k3 = self._arrs(gx)
+        gvp, k4 = self._arrs(gv)
+        L = lib().orc_pic_adjoint(C.byref(self.P), pp, vp, gxp, gvp)
+        return L, gx, gv
+
+    def tangent(self, dpos, dvel):
+        """(dL, L) for the direction (dpos[s][d], dvel[s][d])"""
+        pp, k1 = self._arrs([sp["pos"] for sp in self.species])
+        vp, k2 = self._arrs([sp["vel"] for sp in self.species])
+        dp, k3 = self._arrs(dpos)
+        dv, k4 = self._arrs(dvel)
+        L = C.c_double(0)
+        dL = lib().orc_pic_tangent(C.byref(self.P), pp, vp, dp, dv, C.cast(C.pointer(L), C.c_void_p))
+        return dL, L.value
 
 
 def _ptr_array(arrs):
