@@ -119,6 +119,78 @@ def cpu_baseline(wl, steps):
                       f"restatement of the reference step, OpenMP)"}
 
 
+def run_adjoint(E, grid, ppc, steps, ckpt, depth, pre, post):
+    """Differentiable two-stream: L = field energy after `steps` steps of the
+    FP32 mixed-precision step, differentiated with respect to 32 initial
+    Fourier amplitudes and every initial velocity (ModeDisplace +
+    PicFieldEnergy through torch.autograd), checkpoint every `ckpt` steps.
+    The FP64 path gives the reference gradient of the same set-up."""
+    import torch
+    from psum_amd.pypsum import ModeDisplace, PicFieldEnergy
+    from psum_amd.workloads import two_stream_modes
+    wl = two_stream_modes(n=grid, ppc=ppc, nm=32, amp=1e-3, dtype=np.float32)
+    sp = wl["species"][0]
+    Lb = float(grid)
+    n = len(sp["pos"][0])
+    out = {"workload": f"2D two-stream {grid}x{grid} periodic nodes, {ppc} e-/cell ({n} particles), "
+                       f"{steps} steps, dt 0.05, loss = final field energy, d/d(32 Fourier "
+                       f"amplitudes, a_k = 1e-3) and d/d(initial velocities), checkpoint every "
+                       f"{ckpt} steps", "particles": n, "steps": steps, "ckpt": ckpt}
+    grads = {}
+    for dtype in ("f32", "f64"):
+        np_dt = np.float32 if dtype == "f32" else np.float64
+        tdt = torch.float32 if dtype == "f32" else torch.float64
+        spc = [dict(sp, pos=[a.astype(np_dt) for a in sp["pos"]],
+                    vel=[a.astype(np_dt) for a in sp["vel"]])]
+        st = E.Stepper(wl["grid"], spc, dtype=dtype, mixed_mg=(dtype == "f32"), depth=depth,
+                       pre=pre, post=post, max_cycles=40, tol=1e-8, dt=wl["dt"],
+                       rho_bg=wl["rho_bg"], sort_every=0)
+        xi = torch.from_numpy(spc[0]["pos"][0]).cuda()
+        y = torch.from_numpy(spc[0]["pos"][1]).cuda()
+        vx = torch.from_numpy(spc[0]["vel"][0]).cuda().requires_grad_(True)
+        vy = torch.from_numpy(spc[0]["vel"][1]).cuda().requires_grad_(True)
+        amps = torch.from_numpy(wl["amps"]).cuda().requires_grad_(True)
+        x = ModeDisplace.apply(xi, amps, 0.0, Lb)
+        # plain forward of the same run (diagnostics on), timed by wall clock
+        st.load(0, [x.detach(), y], [vx.detach(), vy.detach()])
+        st.run(2, history=False)
+        st.load(0, [x.detach(), y], [vx.detach(), vy.detach()])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st.run(steps, history=False)
+        torch.cuda.synchronize()
+        fwd_ms = 1e3 * (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        loss = PicFieldEnergy.apply(st, steps, ckpt, x, y, vx, vy)
+        torch.cuda.synchronize()
+        rec_ms = 1e3 * (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        ga, gvx, gvy = torch.autograd.grad(loss, [amps, vx, vy])
+        torch.cuda.synchronize()
+        rev_ms = 1e3 * (time.perf_counter() - t0)
+        grads[dtype] = (ga.cpu().numpy(), gvx.double().cpu().numpy(), gvy.double().cpu().numpy())
+        out[dtype] = {"loss": loss.item(), "forward_ms_per_step": fwd_ms / steps,
+                      "record_ms_per_step": rec_ms / steps, "reverse_ms_per_step": rev_ms / steps,
+                      "reverse_over_forward": rev_ms / fwd_ms,
+                      "record_plus_reverse_over_forward": (rec_ms + rev_ms) / fwd_ms}
+        log(f"[bench] adjoint {dtype}: forward {fwd_ms / steps:.3f} ms/step, record "
+            f"{rec_ms / steps:.3f}, reverse {rev_ms / steps:.3f} ms/step "
+            f"(reverse/forward {rev_ms / fwd_ms:.2f}), L = {loss.item():.6e}")
+        st.close()
+        del x, xi, y, vx, vy
+    g32, g64 = grads["f32"], grads["f64"]
+    dv = np.concatenate([a - b for a, b in zip(g32[1:], g64[1:])])
+    v64 = np.concatenate(g64[1:])
+    out["fp32_vs_fp64"] = {
+        "grad_amps_rel_err": float(np.abs(g32[0] - g64[0]).max() / np.abs(g64[0]).max()),
+        "grad_v_rel_l2_err": float(np.linalg.norm(dv) / np.linalg.norm(v64)),
+        "grad_v_p99_abs_err_over_max": float(np.quantile(np.abs(dv), 0.99) / np.abs(v64).max()),
+        "grad_v_max_abs_err_over_max": float(np.abs(dv).max() / np.abs(v64).max()),
+        "loss_rel_err": abs(out["f32"]["loss"] - out["f64"]["loss"]) / abs(out["f64"]["loss"])}
+    out["adjoint_over_forward"] = out["f32"]["reverse_over_forward"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +208,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-grid", type=int, default=256)
+    ap.add_argument("--adjoint-ppc", type=int, default=64)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the runs, copy one buffer of the push kernel's algorithmic size "
                          "(FP32 x, y, vx, vy of every particle) so that HBM counters can be "
@@ -185,6 +261,10 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} written")
         del src, dst
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = run_adjoint(E, args.adjoint_grid, args.adjoint_ppc, args.adjoint_steps, 20,
+                              args.depth, args.pre, args.post)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_steps)
@@ -221,6 +301,7 @@ def main():
             mixed["particle_steps_per_s"] / fp64["particle_steps_per_s"],
         "field_energy_rel_err_first100": fe_err,
         "mixed_cycles_per_step": mixed["cycles_per_step"],
+        "adjoint": adjoint,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -173,7 +173,8 @@ typedef struct psum_step_config {
 } psum_step_config;
 
 void* psum_stepper_create(const psum_step_config* cfg);
-/* copy species s from host (d < dim arrays of n particles, dtype of cfg) */
+/* copy species s in (d < dim arrays of n particles, dtype of cfg; host or
+ * device memory) */
 psum_status psum_stepper_load(void* h, int s, const void* const* pos, const void* const* vel,
                               int64_t n);
 /* device pointers of species s state (for zero-copy wrapping) */
@@ -193,13 +194,22 @@ psum_status psum_stepper_push_timing(void* h, double* ms, int64_t* launches, int
 /* Loss L = field energy (psum_hip_field_energy) of phi solved from the
  * positions after `steps` steps from the current state (no re-sorting). */
 psum_status psum_stepper_forward_loss(void* h, int steps, double* loss);
-/* Reverse mode: L as above from the current state and dL/dx0, dL/dv0 into
- * grad_x / grad_v (device arrays of the particle dtype, index s*3 + d for
- * species s, dimension d; NULL to skip).  Forward checkpoints (x, v, phi,
- * solver state) every `ckpt` steps; each segment is recomputed in reverse.
- * The state is restored to the initial one.  t_fwd_ms / t_bwd_ms: HIP-event
- * times of the checkpointed forward and of the reverse sweep (recompute
- * included).  No reference counterpart (the reference has no adjoint). */
+/* Reverse mode over the step (no reference counterpart: the reference has
+ * no adjoint), for L = field energy after `steps` steps from the current
+ * state.
+ * psum_stepper_record: forward with a checkpoint (x, v, phi, solver state)
+ *   every `ckpt` steps; *loss = L; *t_ms = HIP-event time.  Sorting is
+ *   suspended until psum_stepper_reverse returns.
+ * psum_stepper_reverse: seed * dL/dx0 and seed * dL/dv0 into grad_x / grad_v
+ *   (device arrays of the particle dtype, index s*3 + d for species s,
+ *   dimension d; NULL entries are skipped), recomputing each segment from its
+ *   checkpoint.  Restores the state of the psum_stepper_record call.
+ *   *t_ms = HIP-event time of the sweep (recompute included);
+ *   *solve_cycles = V-cycles of its adjoint solves.
+ * psum_stepper_adjoint: record + reverse with seed 1. */
+psum_status psum_stepper_record(void* h, int steps, int ckpt, double* loss, double* t_ms);
+psum_status psum_stepper_reverse(void* h, double seed, void* const* grad_x, void* const* grad_v,
+                                 double* t_ms, int64_t* solve_cycles);
 psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, void* const* grad_x,
                                  void* const* grad_v, double* t_fwd_ms, double* t_bwd_ms);
 void psum_stepper_destroy(void* h);

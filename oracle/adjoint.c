@@ -22,7 +22,10 @@
  *                              (J = mean, E zeroes the pinned node 0),
  *                              solve(A', drho with fixed nodes zeroed) otherwise
  *   S^T                      : periodic:  u = solve(A', mu), mu = lphi with
- *                              mu_0 -= sum(lphi), lrho = u - mean(u);
+ *                              mu_0 -= sum(lphi), lrho = u - mean(u); row 0 of
+ *                              A' is the identity with right-hand side 0, so
+ *                              mu_0 is never read and the solve takes lphi
+ *                              with entry 0 zeroed;
  *                              walls: lrho = solve(A', lphi, fixed nodes zeroed)
  *                              (A' is A with identity rows at pinned / fixed
  *                              nodes; its interior block is symmetric)
@@ -35,6 +38,10 @@
 
 int orc_base(const orc_grid* g, double x0, double x1, double x2, int* c, double* s);
 uint32_t orc_key(const orc_grid* g, double x0, double x1, double x2);
+
+/* planted mistakes for the tests that show the checks are sharp (0 = none) */
+static int g_plant = 0;
+void orc_adjoint_plant(int which) { g_plant = which; }
 
 typedef struct {
     const orc_pic_problem* P;
@@ -154,7 +161,6 @@ static void solve_linear(ctx_t* c, double* drho, double* dphi)
 /* S_lin^T: lrho = S_lin^T lphi (lphi overwritten) */
 static void solve_adjoint(ctx_t* c, double* lphi, double* lrho)
 {
-    if (c->all_periodic) lphi[0] = lphi[0] - sum_of(lphi, c->nn);
     for (int64_t i = 0; i < c->nn; ++i)
         if (c->fx[i]) lphi[i] = 0.0;
     memset(lrho, 0, sizeof(double) * (size_t)c->nn);
@@ -383,15 +389,15 @@ double orc_pic_adjoint(const orc_pic_problem* P, double*** pos, double*** vel, d
                             if (e == f) continue;
                             double h = 0.0;
                             for (int i = 0; i < (1 << D); ++i)
-                                h += PHI[n0][idx[i]] * hess_c(D, W, G, i, e, f);
+                                h += PHI[g_plant == 3 ? n0 + 1 : n0][idx[i]] * hess_c(D, W, G, i, e, f);
                             a += lvt[e] * h;
                         }
-                        lx[s][f][p] -= qmdt * a;
+                        if (g_plant != 1) lx[s][f][p] -= qmdt * a;
                     }
                     for (int i = 0; i < (1 << D); ++i) {
                         double t = 0.0;
                         for (int e = 0; e < D; ++e) t += lvt[e] * grad_c(D, W, G, i, e);
-                        lphi[idx[i]] += -qmdt * t;
+                        lphi[idx[i]] += (g_plant == 2 ? qmdt : -qmdt) * t;
                     }
                 }
                 for (int d = 0; d < D; ++d) lv[s][d][p] = lvt[d];

@@ -203,6 +203,10 @@ double orc_pic_loss(const orc_pic_problem* P, double*** pos, double*** vel, uint
 /* Reverse mode: returns L and writes dL/dx0, dL/dv0 into gx/gv[s][d]. */
 double orc_pic_adjoint(const orc_pic_problem* P, double*** pos, double*** vel, double*** gx,
                        double*** gv);
+/* Test hook: plant a known mistake in orc_pic_adjoint (1 = drop the Hessian
+ * term, 2 = flip the sign of the gather transpose, 3 = take the Hessian from
+ * phi^{n+1} instead of phi^n; 0 = none) to show the checks catch it. */
+void orc_adjoint_plant(int which);
 /* Forward mode: returns dL = <grad L, (dpos, dvel)>; *loss = L. */
 double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, double*** dpos,
                        double*** dvel, double* loss);

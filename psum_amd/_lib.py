@@ -108,6 +108,9 @@ SIGNATURES = [
     ("psum_stepper_set_timing", ST, [P, C.c_int]),
     ("psum_stepper_push_timing", ST, [P, C.POINTER(C.c_double), C.POINTER(I64), C.POINTER(I64)]),
     ("psum_stepper_forward_loss", ST, [P, C.c_int, C.POINTER(C.c_double)]),
+    ("psum_stepper_record", ST, [P, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
+    ("psum_stepper_reverse", ST, [P, C.c_double, P, P, C.POINTER(C.c_double), C.POINTER(I64)]),
     ("psum_stepper_adjoint", ST, [P, C.c_int, C.c_int, C.POINTER(C.c_double), P, P,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psum_stepper_destroy", None, [P]),

@@ -288,6 +288,150 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
     xf[i] = s * (T)1 + xf[i] * (T)1;
 }
 
+// ---- fused 2D kernels: several Jacobi sweeps of one tile in LDS ----------
+// The tile (kTileE^2 outputs) is loaded with a halo of `sweeps` nodes; sweep
+// k updates the region shrunk by k, so after `sweeps` sweeps the inner tile
+// holds exactly what `sweeps` launches of k_smooth produce (same operations
+// in the same order).  Halo nodes beyond a non-wrapped edge are clamped
+// copies: the stencils of real nodes give them zero coefficients, as in
+// k_smooth's clamped neighbour.  PROLONG adds P x_c to every loaded node
+// first (k_prolong_add's arithmetic), fusing the coarse-grid correction with
+// the post-sweeps; ACC adds the result into an FP64 iterate (the mixed
+// refinement's x += e, fused into the last level-0 sweep).
+constexpr int kTileE = 32;
+constexpr int kMaxFuse = 4;
+constexpr int kTileW = kTileE + 2 * kMaxFuse;
+
+__device__ __forceinline__ int tile_coord(int g, int n, int wrap)
+{
+    if (wrap) {
+        g %= n;
+        return g < 0 ? g + n : g;
+    }
+    return g < 0 ? 0 : (g >= n ? n - 1 : g);
+}
+
+template <typename T>
+__device__ __forceinline__ T prolong_at(const DevLevel& Cl, const DevTransfer<T>& tr, const T* xc,
+                                        int c0, int c1)
+{
+    T s = (T)0;
+    const int p0 = tr.pc[0][c0], p1 = tr.pc[1][c1];
+    const T* w0 = tr.pw[0] + 2 * c0;
+    const T* w1 = tr.pw[1] + 2 * c1;
+    for (int a = 0; a < 2; ++a) {
+        if (w0[a] == (T)0) continue;
+        const int i0 = wrap_index(p0 + a, Cl.n[0], tr.wrap[0]);
+        for (int b = 0; b < 2; ++b) {
+            if (w1[b] == (T)0) continue;
+            const int i1 = wrap_index(p1 + b, Cl.n[1], tr.wrap[1]);
+            s = s + (w0[a] * w1[b]) * xc[(int64_t)i0 * Cl.n[1] + i1];
+        }
+    }
+    return s;
+}
+
+template <typename T, bool COMPACT, bool PROLONG, bool ACC>
+__global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable<T> relax,
+                                                            const T* __restrict__ xin,
+                                                            const T* __restrict__ b,
+                                                            T* __restrict__ xout, T two3, int sweeps,
+                                                            DevLevel Cl, DevTransfer<T> tr,
+                                                            const T* __restrict__ xc,
+                                                            double* __restrict__ xacc)
+{
+    __shared__ T sx[2][kTileW * kTileW];
+    __shared__ T sb[kTileW * kTileW];
+    __shared__ uint16_t scl[kTileW * kTileW];
+    const int n0 = L.n[0], n1 = L.n[1];
+    const int nt1 = (n1 + kTileE - 1) / kTileE;
+    const int t0 = (int)(blockIdx.x / nt1) * kTileE, t1 = (int)(blockIdx.x % nt1) * kTileE;
+    const int h = sweeps;
+    const int W = kTileE + 2 * h;
+    for (int e = threadIdx.x; e < W * W; e += kMgThreads) {
+        const int a = e / W, c = e - a * W;
+        const int g0 = tile_coord(t0 - h + a, n0, L.wrap[0]);
+        const int g1 = tile_coord(t1 - h + c, n1, L.wrap[1]);
+        const int64_t gi = (int64_t)g0 * n1 + g1;
+        T xv = xin[gi];
+        if (PROLONG) xv = prolong_at<T>(Cl, tr, xc, g0, g1) * (T)1 + xv * (T)1;
+        const int li = a * kTileW + c;
+        sx[0][li] = xv;
+        sb[li] = b[gi];
+        scl[li] = L.cls[gi];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int k = 1; k <= h; ++k) {
+        const int span = W - 2 * k;
+        for (int e = threadIdx.x; e < span * span; e += kMgThreads) {
+            const int a = k + e / span, c = k + e % span;
+            const int li = a * kTileW + c;
+            const T* coef = relax.coef + (int64_t)scl[li] * 9;
+            const T* xs = sx[cur];
+            T t = (T)0;
+#pragma unroll
+            for (int da = -1; da <= 1; ++da)
+#pragma unroll
+                for (int db = -1; db <= 1; ++db) {
+                    if (COMPACT && da != 0 && db != 0) continue;
+                    t = t + coef[(da + 1) * 3 + (db + 1)] * xs[li + da * kTileW + db];
+                }
+            sx[cur ^ 1][li] = t * (T)1 + sb[li] * (two3 * (T)1);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int e = threadIdx.x; e < kTileE * kTileE; e += kMgThreads) {
+        const int a = e / kTileE, c = e - a * kTileE;
+        const int g0 = t0 + a, g1 = t1 + c;
+        if (g0 < n0 && g1 < n1) {
+            const int64_t gi = (int64_t)g0 * n1 + g1;
+            const T v = sx[cur][(a + h) * kTileW + (c + h)];
+            xout[gi] = v;
+            // the mixed refinement's x += e (k_add_correction's arithmetic)
+            if (ACC) xacc[gi] += (double)v;
+        }
+    }
+}
+
+// k_residual fused into k_restrict (2D): the restriction evaluates the fine
+// residual b - A^ x at each of its fine support nodes with k_residual's
+// arithmetic instead of reading a stored residual.
+template <typename T, bool COMPACT>
+__global__ __launch_bounds__(kMgThreads) void k_resid_restrict(DevLevel F, DevLevel Cl,
+                                                               DevTransfer<T> tr, DevTable<T> Af,
+                                                               DevTable<T> Ac,
+                                                               const T* __restrict__ x,
+                                                               const T* __restrict__ bf,
+                                                               T* __restrict__ bc, T* __restrict__ xc,
+                                                               T first_sweep)
+{
+    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    if (i >= Cl.N) return;
+    int c[3];
+    node_coords<2>(Cl, i, c);
+    T s = (T)0;
+    const int f0 = tr.rf[0][c[0]], m0 = tr.rn[0][c[0]];
+    const int f1 = tr.rf[1][c[1]], m1 = tr.rn[1][c[1]];
+    for (int a = 0; a < m0; ++a) {
+        const T w0 = tr.rw[0][3 * c[0] + a];
+        const int i0 = wrap_index(f0 + a, F.n[0], tr.wrap[0]);
+        for (int b = 0; b < m1; ++b) {
+            const T w = w0 * tr.rw[1][3 * c[1] + b];
+            const int i1 = wrap_index(f1 + b, F.n[1], tr.wrap[1]);
+            const int64_t fi = (int64_t)i0 * F.n[1] + i1;
+            const int fc[3] = {i0, i1, 0};
+            const T t = stencil_apply<T, 2, COMPACT>(F, Af.coef + (int64_t)F.cls[fi] * 9, x, fc);
+            const T r = t * (T)-1 + bf[fi] * (T)1;
+            s = s + w * r;
+        }
+    }
+    const T bv = s * Ac.dinv[Cl.cls[i]];
+    bc[i] = bv;
+    xc[i] = first_sweep != (T)0 ? bv * (first_sweep * (T)1) : (T)0;
+}
+
 // b0 = b o D0^-1  (hadamard_to)
 template <typename T>
 __global__ __launch_bounds__(kMgThreads) void k_hadamard(DevLevel L, DevTable<T> A,
@@ -323,7 +467,7 @@ template <int D, bool COMPACT, bool OUTF32>
 __global__ __launch_bounds__(kMgThreads) void k_outer_residual(
     DevLevel L, DevTable<double> K, const double* __restrict__ x, const double* __restrict__ b,
     const double* __restrict__ bn, int64_t pin, const float* __restrict__ dinv32,
-    float* __restrict__ b0, double* __restrict__ part)
+    float* __restrict__ b0, double* __restrict__ part, float* __restrict__ e0)
 {
     __shared__ double ws[kMgThreads / 64][2];
     const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
@@ -336,7 +480,12 @@ __global__ __launch_bounds__(kMgThreads) void k_outer_residual(
         const double r = b[i] - ax;
         rr = i == pin ? 0.0 : r * r;
         bb = bn[i] * bn[i];
-        if (OUTF32) b0[i] = (float)r * dinv32[L.cls[i]];
+        if (OUTF32) {
+            const float bv = (float)r * dinv32[L.cls[i]];
+            b0[i] = bv;
+            // first Jacobi sweep of the correction from e = 0: relax*0 + b*(2/3)
+            if (e0) e0[i] = bv * (2.0f / 3.0f * 1.0f);
+        }
     }
     rr = wave_sum(rr);
     bb = wave_sum(bb);
@@ -983,6 +1132,7 @@ public:
                 corr_ = v;
             } else if (key == "tol") iss >> tol_;
             else if (key == "periodic_mode") iss >> periodic_mode_;
+            else if (key == "fuse") iss >> fuse_;
             else if (key == "graph") iss >> use_graph_;
             else if (key == "num_conv_mask" || key == "num_conv_mask_relax" ||
                      key == "num_conv_mask_restrict" || key == "num_conv_mask_interpolate" ||
@@ -1101,15 +1251,16 @@ public:
         // checked.  A check reads ||b - A x||^2 and ||b||^2 back to the host.
         const int first_check = std::max(1, std::min(max_cycles_, predicted_ - 1));
         if (!mixed_) {
+            scale_rhs_f64(bs);
             if (!use_tol) {
-                vcycles_f64(bs, x, max_cycles_);
+                vcycles_f64(x, max_cycles_);
                 last_cycles_ = max_cycles_;
             } else {
                 double rr, bb;
                 for (int c = 0; c < first_check; ++c) cycle_f64(bs, x);
                 last_cycles_ = first_check;
                 for (;;) {
-                    outer_residual(bs, b, x, false, true, &rr, &bb);
+                    outer_residual(bs, b, x, false, false, true, true, &rr, &bb);
                     last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
                     if (last_rel_ <= tol_ || last_cycles_ >= max_cycles_) break;
                     cycle_f64(bs, x);
@@ -1119,7 +1270,8 @@ public:
         } else {
             // refinement iteration: r = b - A x (FP64) -> FP32 V-cycles on A e = r -> x += e
             double rr, bb;
-            outer_residual(bs, b, x, true, false, &rr, &bb);
+            const bool fm = fused_mixed();
+            outer_residual(bs, b, x, true, fm, false, false, &rr, &bb);
             for (int cycle = 0; cycle < max_cycles_;) {
                 if (use_tol && cycle >= first_check) {
                     read_norms(&rr, &bb);
@@ -1127,20 +1279,23 @@ public:
                     if (last_rel_ <= tol_) break;
                 }
                 if (use_tol) {
-                    iteration_f32(bs, b, x);
+                    // the check before iteration cycle+1 reads this iteration's norms
+                    iteration_f32(bs, b, x, cycle + 1 >= first_check);
                     cycle += 1;
                     last_cycles_ += 1;
                     continue;
                 }
+                // fixed count (the reference 32f backend): batches of
+                // correction_interval V-cycles between FP64 residuals, no
+                // host check (so the whole solve can be captured in a graph)
                 const int batch = std::min(corr_, max_cycles_ - cycle);
-                vcycles_f32(batch);
-                hipLaunchKernelGGL(k_add_correction, dim3(nblk(N)), dim3(kMgThreads), 0, st, x, dev_[0].xf, N);
-                MG_HIP(hipGetLastError());
+                vcycles_f32(batch, fm, x);
                 cycle += batch;
                 last_cycles_ += batch;
-                outer_residual(bs, b, x, true, false, &rr, &bb);
+                if (cycle < max_cycles_) outer_residual(bs, b, x, true, fm, false, false, &rr, &bb);
             }
             if (use_tol) {
+                if (last_cycles_ == 0) outer_residual(bs, b, x, false, false, true, false, &rr, &bb);
                 read_norms(&rr, &bb);
                 last_rel_ = bb > 0 ? std::sqrt(rr / bb) : 0.0;
             }
@@ -1154,6 +1309,19 @@ public:
         if (sync) MG_HIP(hipStreamSynchronize(st));
     }
 
+    // fixed V-cycle count, one per refinement iteration, no host checks and
+    // no graphs of its own: the solve can be captured into a caller's graph
+    void set_fixed_cycles(int cycles)
+    {
+        if (cycles < 1) throw std::invalid_argument("hip_multigrid: fixed cycle count must be >= 1");
+        tol_ = 0.0;
+        max_cycles_ = cycles;
+        corr_ = 1;
+        use_graph_ = 0;
+        for (auto& c : cap64_.e) drop_graph(c);
+        for (auto& c : cap32_.e) drop_graph(c);
+    }
+
     int last_cycles() const { return last_cycles_; }
     int predicted() const { return predicted_; }
     void set_predicted(int p) { predicted_ = p; }
@@ -1164,6 +1332,7 @@ public:
 private:
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
+    int fuse_ = 1;   // 2D: fused multi-sweep tile smoothing and residual+restriction
     double tol_ = 0.0;
     int predicted_ = 1;                // cycles the previous solve needed
     int periodic_mode_ = 1;
@@ -1210,8 +1379,8 @@ private:
 
     void release()
     {
-        drop_graph(cap64_);
-        drop_graph(cap32_);
+        for (auto& c : cap64_.e) drop_graph(c);
+        for (auto& c : cap32_.e) drop_graph(c);
         for (auto& d : dev_) {
             for (void* p : {(void*)d.cls, (void*)d.K, (void*)d.Ah, (void*)d.Rx, (void*)d.dinv,
                             (void*)d.Ah32, (void*)d.Rx32, (void*)d.dinv32, (void*)d.t, (void*)d.bf,
@@ -1363,21 +1532,84 @@ private:
         }
     }
 
+    // `count` sweeps on level l: 2D levels in fused tile launches of up to
+    // kMaxFuse sweeps (PROLONG: the first launch also applies x += P x_c)
+    // acc (FP32 levels only): the last launch also adds its result into the
+    // FP64 array acc; returns whether it did (only the fused 2D path can)
+    template <typename T>
+    bool smooth_n(int l, T*& x, T*& buf, const T* b, int count, const T* xc = nullptr,
+                  double* acc = nullptr)
+    {
+        if (host_[l].D != 2 || !fuse_ || count == 0) {
+            if (xc) prolong(l, xc, x);
+            for (int s = 0; s < count; ++s) smooth<T>(l, x, buf, b);
+            return false;
+        }
+        const DevLevel L = dl(l);
+        const T two3 = (T)(2.0 / 3);
+        const unsigned nb = (unsigned)(((L.n[0] + kTileE - 1) / kTileE) * ((L.n[1] + kTileE - 1) / kTileE));
+        const DevLevel C = xc ? dl(l + 1) : L;
+        const DevTransfer<T> tr = xc ? transfer<T>(l) : DevTransfer<T>{};
+        const bool cmp = host_[l].compact;
+        bool accd = false;
+        while (count > 0) {
+            const int k = std::min(count, kMaxFuse);
+            double* a = (k == count) ? acc : nullptr;
+#define TILE(CMP, PRO, ACC) hipLaunchKernelGGL((k_smooth_tile<T, CMP, PRO, ACC>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3, k, C, tr, xc, a)
+            if (a) {
+                if (xc) { if (cmp) TILE(true, true, true); else TILE(false, true, true); }
+                else { if (cmp) TILE(true, false, true); else TILE(false, false, true); }
+                accd = true;
+            } else {
+                if (xc) { if (cmp) TILE(true, true, false); else TILE(false, true, false); }
+                else { if (cmp) TILE(true, false, false); else TILE(false, false, false); }
+            }
+#undef TILE
+            std::swap(x, buf);
+            xc = nullptr;
+            count -= k;
+        }
+        return accd;
+    }
+
+    template <typename T>
+    void prolong(int l, const T* xc, T* x)
+    {
+        const DevLevel F = dl(l), C = dl(l + 1);
+        if (host_[l].D == 2)
+            hipLaunchKernelGGL((k_prolong_add<T, 2>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xc, x);
+        else
+            hipLaunchKernelGGL((k_prolong_add<T, 3>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xc, x);
+    }
+
     // One or more V-cycles on level-0 arrays (b0 scaled, x0 warm start).
     // xs[l] / ts[l] are the ping-pong pair of level l, bs[l] its rhs.
+    // first_swept: xs[0] already holds the first level-0 sweep from zero.
+    // acc: the last cycle's final level-0 sweep adds its result into acc
+    // (returns whether that happened).
     template <typename T>
-    void vcycles(std::vector<T*>& xs, std::vector<T*>& ts, std::vector<T*>& bs, int cycles)
+    bool vcycles(std::vector<T*>& xs, std::vector<T*>& ts, std::vector<T*>& bs, int cycles,
+                 bool first_swept = false, double* acc = nullptr)
     {
         const int nl = (int)host_.size();
         const int Lc = nl - 1;
         const T two3 = (T)(2.0 / 3);
+        bool accd = false;
         for (int cyc = 0; cyc < cycles; ++cyc) {
             for (int l = 0; l < Lc; ++l) {
-                // coarse levels received their first sweep from k_restrict
-                for (int s = (l > 0 && pre_ > 0) ? 1 : 0; s < pre_; ++s) smooth<T>(l, xs[l], ts[l], bs[l]);
-                residual<T>(l, xs[l], bs[l], ts[l]);
+                // coarse levels received their first sweep from the restriction
+                const bool swept = (l > 0 || (cyc == 0 && first_swept)) && pre_ > 0;
+                smooth_n<T>(l, xs[l], ts[l], bs[l], swept ? pre_ - 1 : pre_);
                 const DevLevel F = dl(l), C = dl(l + 1);
                 const T fs = (l + 1 < Lc && pre_ > 0) ? two3 : (T)0;
+                if (host_[l].D == 2 && fuse_) {
+                    if (host_[l].compact)
+                        hipLaunchKernelGGL((k_resid_restrict<T, true>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
+                    else
+                        hipLaunchKernelGGL((k_resid_restrict<T, false>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
+                    continue;
+                }
+                residual<T>(l, xs[l], bs[l], ts[l]);
                 if (host_[l].D == 2)
                     hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
                 else
@@ -1389,18 +1621,26 @@ private:
                 hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nb), dim3(kMgThreads), 0, stream_, cinv_, n, bs[Lc], xs[Lc]);
             }
             for (int l = Lc - 1; l >= 0; --l) {
-                const DevLevel F = dl(l), C = dl(l + 1);
-                if (host_[l].D == 2)
-                    hipLaunchKernelGGL((k_prolong_add<T, 2>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xs[l + 1], xs[l]);
-                else
-                    hipLaunchKernelGGL((k_prolong_add<T, 3>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xs[l + 1], xs[l]);
-                for (int s = 0; s < post_; ++s) smooth<T>(l, xs[l], ts[l], bs[l]);
+                if (post_ > 0) {
+                    double* a = (l == 0 && cyc == cycles - 1) ? acc : nullptr;
+                    accd = smooth_n<T>(l, xs[l], ts[l], bs[l], post_, xs[l + 1], a) || accd;
+                } else {
+                    prolong<T>(l, xs[l + 1], xs[l]);
+                }
             }
         }
         MG_HIP(hipGetLastError());
+        return accd;
     }
 
-    void vcycles_f64(const double* b, double* x, int cycles)
+    // b0 = b o D0^-1 into dev_[0].b (once per solve: the V-cycles read it)
+    void scale_rhs_f64(const double* b)
+    {
+        hipLaunchKernelGGL(k_hadamard<double>, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, dl(0), table<double>(0, false), b, dev_[0].b);
+    }
+
+    // FP64 V-cycles on x with the scaled right-hand side in dev_[0].b
+    void vcycles_f64(double* x, int cycles)
     {
         const int nl = (int)host_.size();
         std::vector<double*> xs(nl), ts(nl), bs(nl);
@@ -1409,13 +1649,21 @@ private:
             ts[l] = dev_[l].t;
             bs[l] = dev_[l].b;
         }
-        hipLaunchKernelGGL(k_hadamard<double>, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, dl(0), table<double>(0, false), b, bs[0]);
         vcycles<double>(xs, ts, bs, cycles);
         if (xs[0] != x) MG_HIP(hipMemcpyAsync(x, xs[0], (size_t)N0() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     }
 
-    // FP32 V-cycles on dev_[0].bf (already scaled), correction in dev_[0].xf
-    void vcycles_f32(int cycles)
+    // The mixed iteration fuses work into its neighbours when the level-0
+    // smoothing runs through the 2D tile kernels: the outer residual also
+    // writes the first sweep of e from zero, and the last post-sweep adds e
+    // into x.
+    bool fused_mixed() const { return fuse_ && host_[0].D == 2 && pre_ > 0 && post_ > 0; }
+
+    // FP32 V-cycles on dev_[0].bf (already scaled) for the correction e from
+    // e = 0.  first_swept: dev_[0].xf already holds the first level-0 sweep.
+    // x != nullptr: x += e is applied (fused when possible).  Otherwise e
+    // ends in dev_[0].xf (a fixed buffer, so the sequence can be captured).
+    void vcycles_f32(int cycles, bool first_swept, double* x)
     {
         const int nl = (int)host_.size();
         std::vector<float*> xs(nl), ts(nl), bs(nl);
@@ -1424,24 +1672,32 @@ private:
             ts[l] = dev_[l].tf;
             bs[l] = dev_[l].bf;
         }
-        // e0 = 0
-        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, xs[0], N0());
-        vcycles<float>(xs, ts, bs, cycles);
-        // the correction ends in dev_[0].xf (a fixed buffer, so the sequence
-        // can be captured and replayed)
-        if (xs[0] != dev_[0].xf)
+        if (!first_swept)
+            hipLaunchKernelGGL(k_zero_f32, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, xs[0], N0());
+        const bool accd = vcycles<float>(xs, ts, bs, cycles, first_swept, x);
+        if (x && !accd) {
+            hipLaunchKernelGGL(k_add_correction, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, x, xs[0], N0());
+        } else if (!x && xs[0] != dev_[0].xf) {
             MG_HIP(hipMemcpyAsync(dev_[0].xf, xs[0], (size_t)N0() * sizeof(float), hipMemcpyDeviceToDevice, stream_));
+        }
     }
 
     // One V-cycle (FP64) or one refinement iteration (mixed: FP32 V-cycle,
     // x += e, next residual), replayed from a captured hipGraph when the
-    // solver runs on its own stream.  The graph is keyed on (b, x).
+    // solver runs on its own stream.  Graphs are keyed on (b, x); a few are
+    // kept so that callers alternating between solves on different buffers
+    // (a forward and an adjoint solve) do not re-capture each time.
     struct Captured {
         hipGraphExec_t exec = nullptr;
         const void* b = nullptr;
         const void* x = nullptr;
+        int flag = 0;
     };
-    Captured cap64_, cap32_;
+    struct GraphCache {
+        Captured e[4];
+        int next = 0;
+    };
+    GraphCache cap64_, cap32_;
     int use_graph_ = 1;
 
     void drop_graph(Captured& c)
@@ -1451,45 +1707,56 @@ private:
     }
 
     template <typename F>
-    void replay(Captured& c, const void* b, const void* x, F&& body)
+    void replay(GraphCache& gc, const void* b, const void* x, int flag, F&& body)
     {
         if (!use_graph_ || stream_ == nullptr) {
             body();
             return;
         }
-        if (!c.exec || c.b != b || c.x != x) {
-            drop_graph(c);
+        Captured* c = nullptr;
+        for (auto& e : gc.e)
+            if (e.exec && e.b == b && e.x == x && e.flag == flag) c = &e;
+        if (!c) {
+            c = &gc.e[gc.next];
+            gc.next = (gc.next + 1) % 4;
+            drop_graph(*c);
             MG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             body();
             hipGraph_t graph = nullptr;
             MG_HIP(hipStreamEndCapture(stream_, &graph));
-            MG_HIP(hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0));
+            MG_HIP(hipGraphInstantiate(&c->exec, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
-            c.b = b;
-            c.x = x;
+            c->b = b;
+            c->x = x;
+            c->flag = flag;
         }
-        MG_HIP(hipGraphLaunch(c.exec, stream_));
+        MG_HIP(hipGraphLaunch(c->exec, stream_));
     }
 
     void cycle_f64(const double* bs, double* x)
     {
-        replay(cap64_, bs, x, [&]() { vcycles_f64(bs, x, 1); });
+        replay(cap64_, bs, x, 0, [&]() { vcycles_f64(x, 1); });
     }
 
-    void iteration_f32(const double* bs, const double* b, double* x)
+    // one refinement iteration: FP32 V-cycle on A e = r, x += e, next
+    // residual r = b - A x (FP64); norms: also reduce ||r||^2, ||b||^2 for a
+    // host check
+    void iteration_f32(const double* bs, const double* b, double* x, bool norms)
     {
-        replay(cap32_, bs, x, [&]() {
-            vcycles_f32(1);
-            hipLaunchKernelGGL(k_add_correction, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, x, dev_[0].xf, N0());
-            double rr, bb;
-            outer_residual(bs, b, x, true, false, &rr, &bb);
+        replay(cap32_, bs, x, norms ? 1 : 0, [&]() {
+            const bool fm = fused_mixed();
+            vcycles_f32(1, fm, x);
+            outer_residual(bs, b, x, true, fm, norms, false, nullptr, nullptr);
         });
     }
 
     // ||b - A x||^2 and ||bn||^2 into red_ (device) and, asynchronously, into
     // pinned host memory; `wait` synchronises and returns them.
+    // emit_f32: also the scaled FP32 residual into dev_[0].bf; first_sweep:
+    // and the first sweep of e from zero into dev_[0].xf; norms: reduce the
+    // norms and start their copy to the host.
     void outer_residual(const double* b, const double* bn, const double* x, bool emit_f32,
-                        bool wait, double* rr, double* bb)
+                        bool first_sweep, bool norms, bool wait, double* rr, double* bb)
     {
         const DevLevel L = dl(0);
         const unsigned nb = nblk(L.N);
@@ -1499,7 +1766,8 @@ private:
         const bool c = host_[0].compact;
         float* b0 = emit_f32 ? dev_[0].bf : nullptr;
         const float* d32 = dev_[0].dinv32;
-#define OUTER(DIM, CMP, F32) hipLaunchKernelGGL((k_outer_residual<DIM, CMP, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, L, K, x, b, bn, pin_, d32, b0, part_)
+        float* e0 = (emit_f32 && first_sweep) ? dev_[0].xf : nullptr;
+#define OUTER(DIM, CMP, F32) hipLaunchKernelGGL((k_outer_residual<DIM, CMP, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, L, K, x, b, bn, pin_, d32, b0, part_, e0)
         if (host_[0].D == 2) {
             if (c) { if (emit_f32) OUTER(2, true, true); else OUTER(2, true, false); }
             else { if (emit_f32) OUTER(2, false, true); else OUTER(2, false, false); }
@@ -1508,6 +1776,8 @@ private:
             else { if (emit_f32) OUTER(3, false, true); else OUTER(3, false, false); }
         }
 #undef OUTER
+        MG_HIP(hipGetLastError());
+        if (!norms) return;
         hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kMgThreads), 0, stream_, part_, (int64_t)nb, red_);
         MG_HIP(hipGetLastError());
         MG_HIP(hipMemcpyAsync(red_host_, red_, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -1617,6 +1887,7 @@ void mg_set_stream(HipMultigrid* h, hipStream_t s) { h->set_stream(s); }
 void mg_solve(HipMultigrid* h, double* b, double* x, bool sync) { h->solve(b, x, sync); }
 void mg_solve_homogeneous(HipMultigrid* h, double* b, double* x) { h->solve(b, x, false, true); }
 int mg_predicted(HipMultigrid* h) { return h->predicted(); }
+void mg_set_fixed_cycles(HipMultigrid* h, int cycles) { h->set_fixed_cycles(cycles); }
 void mg_set_predicted(HipMultigrid* h, int p) { h->set_predicted(p); }
 int mg_last_cycles(HipMultigrid* h) { return h->last_cycles(); }
 double mg_last_rel(HipMultigrid* h) { return h->last_rel(); }

@@ -121,30 +121,34 @@ __global__ __launch_bounds__(kT) void k_sub_mean(double* __restrict__ x, int64_t
     if (i < n) x[i] = x[i] - mean;
 }
 
-// lphi[0] -= sum(lphi): the consistent right-hand side of the adjoint of a
-// pinned periodic solve
-__global__ void k_sub_at0(double* __restrict__ x, const double* __restrict__ sum)
-{
-    if (threadIdx.x == 0) x[0] = x[0] - sum[0];
-}
-
 __global__ __launch_bounds__(kT) void k_zero_any(unsigned char* __restrict__ a, int64_t nbytes)
 {
     const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (i < nbytes) a[i] = 0;
 }
 
-// hist[5*s + k]: field energy, then species-summed kinetic energy and momentum
+// hist[5*s + k], s = *count (then incremented; rows beyond cap are dropped):
+// field energy, then species-summed kinetic energy and momentum.  The row
+// index lives on the device so that a replayed graph appends.
 __global__ void k_hist_record(const double* __restrict__ fe, const double* __restrict__ red,
-                              int nspecies, double* __restrict__ hist, int s)
+                              int nspecies, double* __restrict__ hist, long long* __restrict__ count,
+                              long long cap)
 {
     if (threadIdx.x != 0) return;
+    const long long s = count[0];
+    count[0] = s + 1;
+    if (s >= cap) return;
     hist[5 * s + 0] = fe[0];
     for (int k = 0; k < 4; ++k) {
         double t = 0.0;
         for (int q = 0; q < nspecies; ++q) t += red[4 * q + k];
         hist[5 * s + 1 + k] = t;
     }
+}
+
+__global__ void k_zero_f64(double* __restrict__ a, int n)
+{
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0.0;
 }
 
 __global__ void k_count_valid(const uint32_t* __restrict__ sorted_keys, int64_t n,
@@ -335,6 +339,7 @@ public:
         S_TRY(hipMalloc(&red_, 16 * sizeof(double)));
         S_TRY(hipMalloc(&redw_, psum_hip_reduce_workspace()));
         S_TRY(hipMalloc(&fe_, 4 * sizeof(double)));
+        S_TRY(hipMalloc(&hist_count_, sizeof(long long)));
         few_bytes_ = psum_hip_field_energy_workspace(&cfg_.grid);
         S_TRY(hipMalloc(&few_, few_bytes_));
         S_TRY(hipMalloc(&sum_part_, (size_t)nblk(nn_) * sizeof(double)));
@@ -360,7 +365,7 @@ public:
                 for (void* p : {sp.pos[d], sp.vel[d], sp.pos2[d], sp.vel2[d]})
                     if (p) (void)hipFree(p);
         for (void* p : {(void*)acc_, (void*)b_, (void*)phi_, (void*)phi_prev_, (void*)phi32_, (void*)diag_, (void*)red_,
-                        (void*)fe_, few_, redw_, (void*)sum_part_, (void*)sum_, (void*)counters_,
+                        (void*)fe_, (void*)hist_count_, few_, redw_, (void*)sum_part_, (void*)sum_, (void*)counters_,
                         (void*)keys_, (void*)skeys_, (void*)perm_, sort_work_, (void*)nvalid_, (void*)hist_})
             if (p) (void)hipFree(p);
         for (auto& e : ev_)
@@ -371,6 +376,8 @@ public:
         for (void* p : lv_) if (p) (void)hipFree(p);
         for (auto& v : seg_x_) for (void* p : v) if (p) (void)hipFree(p);
         for (void* p : seg_phi_) if (p) (void)hipFree(p);
+        for (auto& c : cks_) snap_free(c);
+        for (auto& e : aev_) if (e) (void)hipEventDestroy(e);
         if (st_) (void)hipStreamDestroy(st_);
     }
 
@@ -380,8 +387,8 @@ public:
         Species& sp = sp_[s];
         if (n > sp.sp.capacity) throw std::invalid_argument("psum_stepper_load: n exceeds capacity");
         for (int d = 0; d < cfg_.grid.dim; ++d) {
-            S_TRY(hipMemcpyAsync(sp.pos[d], pos[d], (size_t)n * esz_, hipMemcpyHostToDevice, st_));
-            S_TRY(hipMemcpyAsync(sp.vel[d], vel[d], (size_t)n * esz_, hipMemcpyHostToDevice, st_));
+            S_TRY(hipMemcpyAsync(sp.pos[d], pos[d], (size_t)n * esz_, hipMemcpyDefault, st_));
+            S_TRY(hipMemcpyAsync(sp.vel[d], vel[d], (size_t)n * esz_, hipMemcpyDefault, st_));
         }
         sp.n = n;
         acc_valid_ = false;
@@ -398,19 +405,43 @@ public:
         if (n) *n = sp_[s].n;
     }
 
-    void run(int steps, double* hist_host)
+    void run(int steps, double* hist_host) { run_steps(steps, hist_host, true, true); }
+
+    // diagnostics = false skips the field energy and the reductions of the
+    // history (recompute and recording passes of the adjoint)
+    void run_steps(int steps, double* hist_host, bool sync, bool diagnostics)
     {
         if (steps <= 0) return;
-        if (hist_cap_ < steps) {
-            if (hist_) (void)hipFree(hist_);
-            S_TRY(hipMalloc(&hist_, (size_t)steps * 5 * sizeof(double)));
-            hist_cap_ = steps;
-        }
-        const int D = cfg_.grid.dim;
+        reserve_hist(steps);
+        hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kT), 0, st_, (long long*)hist_count_, (int64_t)1);
         for (int s = 0; s < steps; ++s) {
             if (cfg_.sort_every > 0 && !no_sort_ && step_ % cfg_.sort_every == 0) sort_all();
+            step_body(diagnostics);
+        }
+        if (hist_host)
+            S_TRY(hipMemcpyAsync(hist_host, hist_, (size_t)steps * 5 * sizeof(double), hipMemcpyDeviceToHost, st_));
+        if (!sync) return;
+        S_TRY(hipStreamSynchronize(st_));
+        if (timing_) flush_events();
+    }
+
+    void reserve_hist(int64_t rows)
+    {
+        if (hist_cap_ >= rows) return;
+        if (hist_) (void)hipFree(hist_);
+        S_TRY(hipMalloc(&hist_, (size_t)rows * 5 * sizeof(double)));
+        hist_cap_ = rows;
+    }
+
+    // One step on the engine's stream with nothing the host waits for when
+    // the solver runs a fixed cycle count: graph-capturable.  History rows
+    // go to the device-side row counter.
+    void step_body(bool diagnostics)
+    {
+        const int D = cfg_.grid.dim;
+        {
             solve_field();
-            CHK(psum_hip_field_energy(&cfg_.grid, phi_, cfg_.eps, fe_, few_, few_bytes_, st_));
+            if (diagnostics) CHK(psum_hip_field_energy(&cfg_.grid, phi_, cfg_.eps, fe_, few_, few_bytes_, st_));
             const void* ph = cfg_.dtype == PSUM_F32 ? (const void*)phi32_ : (const void*)phi_;
             hipLaunchKernelGGL(k_zero_i64, dim3(nblk(nn_)), dim3(kT), 0, st_, (long long*)acc_, nn_);
             for (int q = 0; q < cfg_.nspecies; ++q) {
@@ -424,18 +455,18 @@ public:
                     push_particles_ += sp.n;
                     if (++nev_ == kEv) flush_events();
                 }
-                CHK(psum_hip_reduce_diag(diag_, psum_hip_push_blocks(sp.n), red_ + 4 * q, redw_, st_));
-                if (sp.n == 0) S_TRY(hipMemsetAsync(red_ + 4 * q, 0, 4 * sizeof(double), st_));
+                if (diagnostics) {
+                    if (sp.n > 0) CHK(psum_hip_reduce_diag(diag_, psum_hip_push_blocks(sp.n), red_ + 4 * q, redw_, st_));
+                    else hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, red_ + 4 * q, 4);
+                }
             }
-            hipLaunchKernelGGL(k_hist_record, dim3(1), dim3(64), 0, st_, fe_, red_, cfg_.nspecies, hist_, s);
+            if (diagnostics)
+                hipLaunchKernelGGL(k_hist_record, dim3(1), dim3(64), 0, st_, fe_, red_, cfg_.nspecies, hist_,
+                                   (long long*)hist_count_, (long long)hist_cap_);
             S_TRY(hipGetLastError());
             ++step_;
             (void)D;
         }
-        if (hist_host)
-            S_TRY(hipMemcpyAsync(hist_host, hist_, (size_t)steps * 5 * sizeof(double), hipMemcpyDeviceToHost, st_));
-        S_TRY(hipStreamSynchronize(st_));
-        if (timing_) flush_events();
     }
 
     // HIP-event timing of every fused gather/push/deposit launch on the
@@ -495,30 +526,40 @@ public:
     double forward_loss(int steps)
     {
         no_sort_ = true;
-        run(steps, nullptr);
+        if (steps > 0) run_steps(steps, nullptr, true, false);
         no_sort_ = false;
         return field_energy_now();
     }
 
     // ---------------------------------------------------------- adjoint
+    // Reverse mode over the step, for L = field energy after N steps.
+    // record(): forward from the current state with a checkpoint (x, v, phi,
+    // extrapolation history, MG cycle predictor) every `ckpt` steps.
+    // reverse(): seeds lphi = dL/dphi^N and sweeps the segments backwards;
+    // each segment is recomputed from its checkpoint (bit-identical: the
+    // deposit is an exact integer sum and nothing else reorders), keeping
+    // x^n and phi^n, then the reverse kernels run step by step:
+    //   adjoint push (lx, lv, lphi) -> lrho = S^T lphi -> adjoint gather (lx).
+    // Sorting is suspended between record() and reverse().
     struct Snapshot {
-        std::vector<void*> x, v;   // per species * D
+        std::vector<void*> x, v;   // index s*3 + d
         double* phi = nullptr;
         double* phi_prev = nullptr;
         bool have_prev = false;
         int predicted = 1;
+        int64_t n[4] = {0, 0, 0, 0};
     };
 
     void snap_alloc(Snapshot& s)
     {
         const int D = cfg_.grid.dim;
-        s.x.assign(cfg_.nspecies * D, nullptr);
-        s.v.assign(cfg_.nspecies * D, nullptr);
+        s.x.assign(cfg_.nspecies * 3, nullptr);
+        s.v.assign(cfg_.nspecies * 3, nullptr);
         for (int q = 0; q < cfg_.nspecies; ++q)
             for (int d = 0; d < D; ++d) {
                 const size_t b = (size_t)std::max<int64_t>(1, sp_[q].n) * esz_;
-                S_TRY(hipMalloc(&s.x[q * D + d], b));
-                S_TRY(hipMalloc(&s.v[q * D + d], b));
+                S_TRY(hipMalloc(&s.x[q * 3 + d], b));
+                S_TRY(hipMalloc(&s.v[q * 3 + d], b));
             }
         S_TRY(hipMalloc(&s.phi, nn_ * sizeof(double)));
         if (cfg_.extrapolate) S_TRY(hipMalloc(&s.phi_prev, nn_ * sizeof(double)));
@@ -536,12 +577,14 @@ public:
     void snap_save(Snapshot& s)
     {
         const int D = cfg_.grid.dim;
-        for (int q = 0; q < cfg_.nspecies; ++q)
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            s.n[q] = sp_[q].n;
             for (int d = 0; d < D; ++d) {
                 const size_t b = (size_t)sp_[q].n * esz_;
-                S_TRY(hipMemcpyAsync(s.x[q * D + d], sp_[q].pos[d], b, hipMemcpyDeviceToDevice, st_));
-                S_TRY(hipMemcpyAsync(s.v[q * D + d], sp_[q].vel[d], b, hipMemcpyDeviceToDevice, st_));
+                S_TRY(hipMemcpyAsync(s.x[q * 3 + d], sp_[q].pos[d], b, hipMemcpyDeviceToDevice, st_));
+                S_TRY(hipMemcpyAsync(s.v[q * 3 + d], sp_[q].vel[d], b, hipMemcpyDeviceToDevice, st_));
             }
+        }
         S_TRY(hipMemcpyAsync(s.phi, phi_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
         if (cfg_.extrapolate && have_prev_)
             S_TRY(hipMemcpyAsync(s.phi_prev, phi_prev_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
@@ -552,12 +595,14 @@ public:
     void snap_restore(const Snapshot& s)
     {
         const int D = cfg_.grid.dim;
-        for (int q = 0; q < cfg_.nspecies; ++q)
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            sp_[q].n = s.n[q];
             for (int d = 0; d < D; ++d) {
                 const size_t b = (size_t)sp_[q].n * esz_;
-                S_TRY(hipMemcpyAsync(sp_[q].pos[d], s.x[q * D + d], b, hipMemcpyDeviceToDevice, st_));
-                S_TRY(hipMemcpyAsync(sp_[q].vel[d], s.v[q * D + d], b, hipMemcpyDeviceToDevice, st_));
+                S_TRY(hipMemcpyAsync(sp_[q].pos[d], s.x[q * 3 + d], b, hipMemcpyDeviceToDevice, st_));
+                S_TRY(hipMemcpyAsync(sp_[q].vel[d], s.v[q * 3 + d], b, hipMemcpyDeviceToDevice, st_));
             }
+        }
         S_TRY(hipMemcpyAsync(phi_, s.phi, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
         if (cfg_.extrapolate && s.have_prev)
             S_TRY(hipMemcpyAsync(phi_prev_, s.phi_prev, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
@@ -566,138 +611,157 @@ public:
         acc_valid_ = false;
     }
 
-    // lrho = S^T lphi (lphi in lphi_, overwritten); result in lrho_ (double) and lrho_t_
+    // lrho = S^T lphi (lphi_ is overwritten).  S^T is the homogeneous solve
+    // of the same pinned / Dirichlet system (rows of pinned and fixed nodes
+    // take 0; no addback), warm started from the previous lrho.  Its result
+    // may carry a constant on periodic grids, which the gradient gather
+    // annihilates (the CIC gradient weights sum to zero).
     void adjoint_solve()
     {
-        if (all_periodic_) {
-            hipLaunchKernelGGL(k_sum_part, dim3(nblk(nn_)), dim3(kT), 0, st_, lphi_, nn_, sum_part_);
-            hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kT), 0, st_, sum_part_, (int64_t)nblk(nn_), sum_);
-            hipLaunchKernelGGL(k_sub_at0, dim3(1), dim3(64), 0, st_, lphi_, sum_);
-        }
+        // the adjoint solves keep their own cycle predictor, so that their
+        // sequence does not depend on how the recompute is segmented
+        const int fwd_pred = mg_predicted(mg_);
+        mg_set_predicted(mg_, adj_predicted_);
         mg_solve_homogeneous(mg_, lphi_, lrho_);
-        if (all_periodic_) {
-            S_TRY(hipMemcpyAsync(lphi_, lrho_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
-            hipLaunchKernelGGL(k_sum_part, dim3(nblk(nn_)), dim3(kT), 0, st_, lphi_, nn_, sum_part_);
-            hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kT), 0, st_, sum_part_, (int64_t)nblk(nn_), sum_);
-            hipLaunchKernelGGL(k_sub_mean, dim3(nblk(nn_)), dim3(kT), 0, st_, lphi_, nn_, sum_);
-        } else {
-            S_TRY(hipMemcpyAsync(lphi_, lrho_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
-        }
-        // lphi_ now holds lrho (zero-mean); lrho_ keeps the warm start
-        if (cfg_.dtype == PSUM_F32) CHK(psum_hip_f64_to_f32(lphi_, (float*)lrho_t_, nn_, st_));
+        adj_predicted_ = mg_predicted(mg_);
+        mg_set_predicted(mg_, fwd_pred);
+        adj_cycles_ += mg_last_cycles(mg_);
+        if (cfg_.dtype == PSUM_F32) CHK(psum_hip_f64_to_f32(lrho_, (float*)lrho_t_, nn_, st_));
     }
 
     const void* lrho_view() const
     {
-        return cfg_.dtype == PSUM_F32 ? (const void*)lrho_t_ : (const void*)lphi_;
+        return cfg_.dtype == PSUM_F32 ? (const void*)lrho_t_ : (const void*)lrho_;
     }
 
-    void adjoint_gather_all(void* const* xs_of_species, double* bmax_scale_factor)
+    // lx += w_dep grad(lrho)(x) for every species, and the fixed-point scale
+    // of the next adjoint push from the bound sum_e |lv_e + dt lx_e|
+    void adjoint_gather_all(void* const* xs)
     {
-        (void)bmax_scale_factor;
         const int D = cfg_.grid.dim;
+        double dr = 0.0;
+        for (int d = 0; d < D; ++d) dr = std::max(dr, cfg_.grid.delta_r[d]);
         double maxfac = 0.0;
-        for (int q = 0; q < cfg_.nspecies; ++q) {
-            double dr = 0.0;
-            for (int d = 0; d < D; ++d) dr = std::max(dr, cfg_.grid.delta_r[d]);
+        for (int q = 0; q < cfg_.nspecies; ++q)
             maxfac = std::max(maxfac, std::fabs(sp_[q].sp.q / sp_[q].sp.m) * cfg_.dt * dr);
-        }
         for (int q = 0; q < cfg_.nspecies; ++q) {
-            void* const* x = xs_of_species + q * 3;
-            CHK(psum_hip_adjoint_gather(&cfg_.grid, cfg_.dtype, x, &lx_[q * 3], &lv_[q * 3], sp_[q].n,
-                                        lrho_view(), sp_[q].wdep, cfg_.dt, bmax_, st_));
+            CHK(psum_hip_adjoint_gather(&cfg_.grid, cfg_.dtype, xs + q * 3, &lx_[q * 3], &lv_[q * 3],
+                                        sp_[q].n, lrho_view(), sp_[q].wdep, cfg_.dt, bmax_, st_));
             CHK(psum_hip_adjoint_scale(bmax_, psum_hip_adjoint_blocks(sp_[q].n), maxfac, adj_scale_,
                                        q > 0 ? 1 : 0, st_));
         }
     }
 
-    void adjoint_push_all(void* const* xs_of_species, const void* phi_t)
+    void adjoint_push_all(void* const* xs, const void* phi_t)
     {
         hipLaunchKernelGGL(k_zero_i64, dim3(nblk(nn_)), dim3(kT), 0, st_, (long long*)adj_acc_, nn_);
-        for (int q = 0; q < cfg_.nspecies; ++q) {
-            void* const* x = xs_of_species + q * 3;
-            CHK(psum_hip_adjoint_push(&cfg_.grid, cfg_.dtype, x, &lx_[q * 3], &lv_[q * 3], sp_[q].n,
-                                      phi_t, sp_[q].sp.q / sp_[q].sp.m, cfg_.dt, adj_scale_,
-                                      adj_acc_, st_));
-        }
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            CHK(psum_hip_adjoint_push(&cfg_.grid, cfg_.dtype, xs + q * 3, &lx_[q * 3], &lv_[q * 3],
+                                      sp_[q].n, phi_t, sp_[q].sp.q / sp_[q].sp.m, cfg_.dt,
+                                      adj_scale_, adj_acc_, st_));
         CHK(psum_hip_fixed_to_double_dev(adj_acc_, nn_, adj_scale_, lphi_, st_));
     }
 
-    // dL/dx0, dL/dv0 of L = field energy after `steps` steps from the current
-    // state; forward checkpoints every `ckpt` steps, recompute per segment.
-    double adjoint(int steps, int ckpt, void* const* gx, void* const* gv, double* t_fwd, double* t_bwd)
+    void zero_dev(void* p, int64_t bytes)
     {
-        if (steps < 1 || ckpt < 1) throw std::invalid_argument("psum_stepper_adjoint: steps, ckpt >= 1");
-        const int D = cfg_.grid.dim;
-        no_sort_ = true;
+        if (bytes > 0)
+            hipLaunchKernelGGL(k_zero_any, dim3(nblk(bytes)), dim3(kT), 0, st_, (unsigned char*)p, bytes);
+    }
+
+    double record(int steps, int ckpt, double* t_ms)
+    {
+        if (steps < 0 || ckpt < 1) throw std::invalid_argument("psum_stepper_record: steps >= 0, ckpt >= 1");
+        release_record();
         adj_buffers();
-        hipEvent_t e0, e1, e2;
-        S_TRY(hipEventCreate(&e0));
-        S_TRY(hipEventCreate(&e1));
-        S_TRY(hipEventCreate(&e2));
-        S_TRY(hipEventRecord(e0, st_));
-        const int nck = (steps + ckpt - 1) / ckpt;
-        std::vector<Snapshot> cks(nck);
-        for (auto& c : cks) snap_alloc(c);
+        set_ckpt_capacity(std::min(ckpt, std::max(steps, 1)));
+        no_sort_ = true;
+        rec_steps_ = steps;
+        rec_ckpt_ = ckpt;
+        S_TRY(hipEventRecord(aev_[0], st_));
+        const int nck = std::max(1, (steps + ckpt - 1) / ckpt);
+        cks_.assign(nck, Snapshot{});
+        for (auto& c : cks_) snap_alloc(c);
         for (int j = 0; j < nck; ++j) {
-            snap_save(cks[j]);
-            run(std::min(ckpt, steps - j * ckpt), nullptr);
+            snap_save(cks_[j]);
+            const int len = std::min(ckpt, steps - j * ckpt);
+            if (len > 0) run_steps(len, nullptr, false, false);
         }
         const double loss = field_energy_now();
-        S_TRY(hipEventRecord(e1, st_));
-        // final: lphi = dW/dphi(phi^N), lrho = S^T lphi, lx = sum w grad(lrho)(x^N), lv = 0
-        CHK(psum_hip_field_energy_grad(&cfg_.grid, phi_, cfg_.eps, lphi_, st_));
+        S_TRY(hipEventRecord(aev_[1], st_));
+        S_TRY(hipEventSynchronize(aev_[1]));
+        float ms = 0;
+        S_TRY(hipEventElapsedTime(&ms, aev_[0], aev_[1]));
+        if (t_ms) *t_ms = ms;
+        recorded_ = true;
+        return loss;
+    }
+
+    // seed: dL_total/dL (the upstream gradient); gx/gv[s*3+d]: device outputs
+    void reverse(double seed, void* const* gx, void* const* gv, double* t_ms)
+    {
+        if (!recorded_) throw std::invalid_argument("psum_stepper_reverse: no recorded forward");
+        const int D = cfg_.grid.dim;
+        const int steps = rec_steps_, ckpt = rec_ckpt_;
+        S_TRY(hipEventRecord(aev_[0], st_));
+        adj_cycles_ = 0;
+        adj_predicted_ = 1;
+        // final: lphi = seed dW/dphi(phi^N) (phi_ holds phi^N after record)
+        CHK(psum_hip_field_energy_grad(&cfg_.grid, phi_, cfg_.eps * seed, lphi_, st_));
         for (int q = 0; q < cfg_.nspecies; ++q)
             for (int d = 0; d < D; ++d) {
-                hipLaunchKernelGGL(k_zero_any, dim3(nblk((int64_t)sp_[q].n * esz_)), dim3(kT), 0, st_, (unsigned char*)lx_[q * 3 + d], (int64_t)sp_[q].n * esz_);
-                hipLaunchKernelGGL(k_zero_any, dim3(nblk((int64_t)sp_[q].n * esz_)), dim3(kT), 0, st_, (unsigned char*)lv_[q * 3 + d], (int64_t)sp_[q].n * esz_);
+                zero_dev(lx_[q * 3 + d], sp_[q].n * esz_);
+                zero_dev(lv_[q * 3 + d], sp_[q].n * esz_);
             }
-        hipLaunchKernelGGL(k_zero_any, dim3(nblk(nn_ * 8)), dim3(kT), 0, st_, (unsigned char*)lrho_, nn_ * 8);
+        zero_dev(lrho_, nn_ * (int64_t)sizeof(double));
         adjoint_solve();
         {
             std::vector<void*> xn(cfg_.nspecies * 3, nullptr);
             for (int q = 0; q < cfg_.nspecies; ++q)
                 for (int d = 0; d < D; ++d) xn[q * 3 + d] = sp_[q].pos[d];
-            adjoint_gather_all(xn.data(), nullptr);
+            adjoint_gather_all(xn.data());
         }
-        // reverse sweep, one segment at a time
+        const int nck = (int)cks_.size();
         for (int j = nck - 1; j >= 0; --j) {
             const int s0 = j * ckpt, len = std::min(ckpt, steps - s0);
-            snap_restore(cks[j]);
+            if (len <= 0) continue;
+            snap_restore(cks_[j]);
             for (int k = 0; k < len; ++k) {
                 for (int q = 0; q < cfg_.nspecies; ++q)
                     for (int d = 0; d < D; ++d)
-                        S_TRY(hipMemcpyAsync(seg_x_[k][q * 3 + d], sp_[q].pos[d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
-                run(1, nullptr);   // solves phi^(s0+k) then pushes
+                        S_TRY(hipMemcpyAsync(seg_x_[k][q * 3 + d], sp_[q].pos[d], (size_t)sp_[q].n * esz_,
+                                             hipMemcpyDeviceToDevice, st_));
+                run_steps(1, nullptr, false, false);   // phi^(s0+k), then the push
                 const void* ph = cfg_.dtype == PSUM_F32 ? (const void*)phi32_ : (const void*)phi_;
                 S_TRY(hipMemcpyAsync(seg_phi_[k], ph, (size_t)nn_ * esz_, hipMemcpyDeviceToDevice, st_));
             }
             for (int k = len - 1; k >= 0; --k) {
                 adjoint_push_all(seg_x_[k].data(), seg_phi_[k]);
                 adjoint_solve();
-                adjoint_gather_all(seg_x_[k].data(), nullptr);
+                adjoint_gather_all(seg_x_[k].data());
             }
         }
-        S_TRY(hipEventRecord(e2, st_));
-        // outputs and restore
         for (int q = 0; q < cfg_.nspecies; ++q)
             for (int d = 0; d < D; ++d) {
-                if (gx) S_TRY(hipMemcpyAsync(gx[q * 3 + d], lx_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
-                if (gv) S_TRY(hipMemcpyAsync(gv[q * 3 + d], lv_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
+                if (gx && gx[q * 3 + d])
+                    S_TRY(hipMemcpyAsync(gx[q * 3 + d], lx_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
+                if (gv && gv[q * 3 + d])
+                    S_TRY(hipMemcpyAsync(gv[q * 3 + d], lv_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
             }
-        snap_restore(cks[0]);
+        S_TRY(hipEventRecord(aev_[1], st_));
+        snap_restore(cks_[0]);
         S_TRY(hipStreamSynchronize(st_));
-        float ms1 = 0, ms2 = 0;
-        S_TRY(hipEventElapsedTime(&ms1, e0, e1));
-        S_TRY(hipEventElapsedTime(&ms2, e1, e2));
-        if (t_fwd) *t_fwd = ms1;
-        if (t_bwd) *t_bwd = ms2;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipEventDestroy(e2);
-        for (auto& c : cks) snap_free(c);
+        float ms = 0;
+        S_TRY(hipEventElapsedTime(&ms, aev_[0], aev_[1]));
+        if (t_ms) *t_ms = ms;
+        release_record();
+    }
+
+    void release_record()
+    {
+        for (auto& c : cks_) snap_free(c);
+        cks_.clear();
+        recorded_ = false;
         no_sort_ = false;
-        return loss;
     }
 
     void adj_buffers()
@@ -710,29 +774,34 @@ public:
         S_TRY(hipMalloc(&adj_acc_, nn_ * sizeof(int64_t)));
         S_TRY(hipMalloc(&adj_scale_, 4 * sizeof(double)));
         int64_t nbmax = 1;
-        for (int q = 0; q < cfg_.nspecies; ++q) nbmax = std::max(nbmax, psum_hip_adjoint_blocks(sp_[q].sp.capacity));
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            nbmax = std::max(nbmax, psum_hip_adjoint_blocks(sp_[q].sp.capacity));
         S_TRY(hipMalloc(&bmax_, (size_t)nbmax * sizeof(double)));
         for (int q = 0; q < cfg_.nspecies; ++q)
             for (int d = 0; d < D; ++d) {
-                S_TRY(hipMalloc(&lx_[q * 3 + d], (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_));
-                S_TRY(hipMalloc(&lv_[q * 3 + d], (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_));
+                const size_t b = (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_;
+                S_TRY(hipMalloc(&lx_[q * 3 + d], b));
+                S_TRY(hipMalloc(&lv_[q * 3 + d], b));
             }
+        S_TRY(hipEventCreate(&aev_[0]));
+        S_TRY(hipEventCreate(&aev_[1]));
     }
 
     void set_ckpt_capacity(int ckpt)
     {
+        if ((int)seg_x_.size() >= ckpt) return;
         const int D = cfg_.grid.dim;
-        for (auto& v : seg_x_) for (void* p : v) if (p) (void)hipFree(p);
-        for (void* p : seg_phi_) if (p) (void)hipFree(p);
-        seg_x_.assign(ckpt, std::vector<void*>(cfg_.nspecies * 3, nullptr));
-        seg_phi_.assign(ckpt, nullptr);
-        for (int k = 0; k < ckpt; ++k) {
+        for (int k = (int)seg_x_.size(); k < ckpt; ++k) {
+            seg_x_.emplace_back(cfg_.nspecies * 3, nullptr);
+            seg_phi_.push_back(nullptr);
             for (int q = 0; q < cfg_.nspecies; ++q)
                 for (int d = 0; d < D; ++d)
                     S_TRY(hipMalloc(&seg_x_[k][q * 3 + d], (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_));
             S_TRY(hipMalloc(&seg_phi_[k], (size_t)nn_ * esz_));
         }
     }
+
+    int64_t adjoint_cycles() const { return adj_cycles_; }
 
     void phi(double* out)
     {
@@ -761,6 +830,12 @@ private:
     void* lv_[12] = {};
     std::vector<std::vector<void*>> seg_x_;
     std::vector<void*> seg_phi_;
+    std::vector<Snapshot> cks_;
+    bool recorded_ = false;
+    int rec_steps_ = 0, rec_ckpt_ = 1;
+    int64_t adj_cycles_ = 0;
+    int adj_predicted_ = 1;
+    hipEvent_t aev_[2] = {nullptr, nullptr};
     static constexpr int kEv = 64;
     hipEvent_t ev_[2 * kEv] = {};
     int nev_ = 0;
@@ -792,6 +867,7 @@ private:
     double scale_ = 1.0;
     int64_t* acc_ = nullptr;
     double *b_ = nullptr, *phi_ = nullptr, *diag_ = nullptr, *red_ = nullptr, *fe_ = nullptr;
+    void* hist_count_ = nullptr;      // device row counter of hist_
     double* phi_prev_ = nullptr;
     bool have_prev_ = false;
     void* redw_ = nullptr;
@@ -943,14 +1019,34 @@ psum_status psum_stepper_forward_loss(void* h, int steps, double* loss)
     return guarded([&]() { *loss = static_cast<Stepper*>(h)->forward_loss(steps); });
 }
 
+psum_status psum_stepper_record(void* h, int steps, int ckpt, double* loss, double* t_ms)
+{
+    if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() {
+        const double l = static_cast<Stepper*>(h)->record(steps, ckpt, t_ms);
+        if (loss) *loss = l;
+    });
+}
+
+psum_status psum_stepper_reverse(void* h, double seed, void* const* grad_x, void* const* grad_v,
+                                 double* t_ms, int64_t* solve_cycles)
+{
+    if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() {
+        Stepper* s = static_cast<Stepper*>(h);
+        s->reverse(seed, grad_x, grad_v, t_ms);
+        if (solve_cycles) *solve_cycles = s->adjoint_cycles();
+    });
+}
+
 psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, void* const* grad_x,
                                  void* const* grad_v, double* t_fwd_ms, double* t_bwd_ms)
 {
     if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
     return guarded([&]() {
         Stepper* s = static_cast<Stepper*>(h);
-        s->set_ckpt_capacity(ckpt);
-        const double l = s->adjoint(steps, ckpt, grad_x, grad_v, t_fwd_ms, t_bwd_ms);
+        const double l = s->record(steps, ckpt, t_fwd_ms);
+        s->reverse(1.0, grad_x, grad_v, t_bwd_ms);
         if (loss) *loss = l;
     });
 }

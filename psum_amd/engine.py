@@ -330,12 +330,69 @@ class Stepper:
         if not self.h:
             raise PsumError("psum_stepper_create: " +
                             lib().psum_hip_last_error().decode(errors="replace"))
+        self.n = [len(sp["pos"][0]) for sp in species]
         for i, sp in enumerate(species):
-            pos = [np.ascontiguousarray(a, np_dt) for a in sp["pos"]]
-            vel = [np.ascontiguousarray(a, np_dt) for a in sp["vel"]]
-            pa = (C.c_void_p * 3)(*[a.ctypes.data for a in pos])
-            va = (C.c_void_p * 3)(*[a.ctypes.data for a in vel])
-            check(lib().psum_stepper_load(self.h, i, pa, va, len(pos[0])), "psum_stepper_load")
+            self.load(i, sp["pos"], sp["vel"])
+
+    def load(self, s, pos, vel):
+        """Replace species s with pos / vel: numpy arrays, or device tensors of
+        the stepper's dtype (torch CUDA tensors, copied device to device)."""
+        def addr(a):
+            if hasattr(a, "data_ptr"):
+                if a.dtype != _torch_dtype(self.np_dtype) or not a.is_contiguous():
+                    raise PsumError("load: tensors must be contiguous and of the stepper dtype")
+                return a.data_ptr(), a
+            a = np.ascontiguousarray(a, self.np_dtype)
+            return a.ctypes.data, a
+        keep = [addr(a) for a in list(pos) + list(vel)]
+        pa = (C.c_void_p * 3)(*[k[0] for k in keep[:self.dim]])
+        va = (C.c_void_p * 3)(*[k[0] for k in keep[self.dim:]])
+        check(lib().psum_stepper_load(self.h, s, pa, va, len(pos[0])), "psum_stepper_load")
+
+    def forward_loss(self, steps):
+        """L = field energy of phi solved from the positions after `steps`
+        steps (no re-sorting); the state advances by `steps`."""
+        out = C.c_double(0)
+        check(lib().psum_stepper_forward_loss(self.h, steps, C.byref(out)), "forward_loss")
+        return out.value
+
+    def record(self, steps, ckpt=20):
+        """Checkpointed forward for the adjoint: (L, HIP-event ms)."""
+        loss, ms = C.c_double(0), C.c_double(0)
+        check(lib().psum_stepper_record(self.h, steps, ckpt, C.byref(loss), C.byref(ms)), "record")
+        return loss.value, ms.value
+
+    def reverse(self, seed=1.0):
+        """Reverse sweep after record(): (grad_x, grad_v, ms, adjoint V-cycles),
+        grad_x[s][d] / grad_v[s][d] torch CUDA tensors of the particle dtype
+        holding seed * dL/dx0, seed * dL/dv0.  Restores the recorded state."""
+        import torch
+        tdt = _torch_dtype(self.np_dtype)
+        ns = len(self.n)
+        n = [self.count(s) for s in range(ns)]
+        gx = [[torch.empty(n[s], dtype=tdt, device="cuda") for _ in range(self.dim)]
+              for s in range(ns)]
+        gv = [[torch.empty(n[s], dtype=tdt, device="cuda") for _ in range(self.dim)]
+              for s in range(ns)]
+        px = (C.c_void_p * (3 * ns))()
+        pv = (C.c_void_p * (3 * ns))()
+        for s in range(ns):
+            for d in range(self.dim):
+                px[3 * s + d] = gx[s][d].data_ptr()
+                pv[3 * s + d] = gv[s][d].data_ptr()
+        torch.cuda.synchronize()
+        ms = C.c_double(0)
+        cyc = C.c_int64(0)
+        check(lib().psum_stepper_reverse(self.h, float(seed), px, pv, C.byref(ms), C.byref(cyc)),
+              "reverse")
+        return gx, gv, ms.value, cyc.value
+
+    def adjoint(self, steps, ckpt=20):
+        """(L, grad_x, grad_v, t_record_ms, t_reverse_ms) with numpy gradients."""
+        loss, t_f = self.record(steps, ckpt)
+        gx, gv, t_b, _ = self.reverse(1.0)
+        return (loss, [[a.cpu().numpy() for a in s] for s in gx],
+                [[a.cpu().numpy() for a in s] for s in gv], t_f, t_b)
 
     def run(self, steps, history=True):
         hist = np.zeros((steps, 5), np.float64) if history else None
@@ -348,6 +405,14 @@ class Stepper:
         out = np.zeros(num_nodes(self.g), np.float64)
         check(lib().psum_stepper_phi(self.h, out.ctypes.data_as(C.c_void_p)), "psum_stepper_phi")
         return out
+
+    def count(self, s=0):
+        """current particle count of species s"""
+        pos = (C.c_void_p * 3)()
+        vel = (C.c_void_p * 3)()
+        n = C.c_int64(0)
+        check(lib().psum_stepper_state(self.h, s, pos, vel, C.byref(n)), "psum_stepper_state")
+        return n.value
 
     def state(self, s=0):
         """(pos, vel) of species s copied to host numpy arrays."""
@@ -392,6 +457,11 @@ class Stepper:
 
     def __del__(self):
         self.close()
+
+
+def _torch_dtype(np_dtype):
+    import torch
+    return torch.float64 if np_dtype == np.float64 else torch.float32
 
 
 _hip = None

@@ -88,3 +88,24 @@ def bounded_3d(cells=256, ppc=32, mass_ratio=1836.0, te_ti=10.0, seed=3, dtype=n
     fixed[:, :, 0], fixed[:, :, -1] = 1, 1
     return dict(grid=g, species=species, dt=0.1, rho_bg=0.0, eps=1.0, fixed=fixed.ravel(),
                 name=f"bounded3d_{cells}^3_ppc{ppc}x2")
+
+
+def mode_basis(xi, nm, L):
+    """B[k-1, p] = L/(2 pi k) sin(2 pi k xi_p / L): the displacement field of
+    Fourier perturbation mode k (x = xi + sum_k a_k B[k-1])."""
+    k = np.arange(1, nm + 1, dtype=np.float64)[:, None]
+    return L / (2 * np.pi * k) * np.sin(2 * np.pi * k * np.asarray(xi, np.float64)[None, :] / L)
+
+
+def two_stream_modes(n=256, ppc=64, nm=32, amp=1e-3, seed=5, dtype=np.float32):
+    """Differentiable two-stream set-up: the two_stream_2d beams on n x n
+    periodic nodes, with x = xi + sum_{k<=nm} a_k L/(2 pi k) sin(2 pi k xi/L)
+    (a_k = amp for every k) applied on the device, so that the loss can be
+    differentiated with respect to the nm amplitudes and the initial
+    velocities.  Returns the two_stream_2d dict plus xi, amps, nm."""
+    wl = two_stream_2d(n=n, ppc=ppc, seed=seed, dtype=dtype)
+    wl["xi"] = wl["species"][0]["pos"][0]
+    wl["amps"] = np.full(nm, amp, np.float64)
+    wl["nm"] = nm
+    wl["name"] = f"two_stream2d_modes_{n}x{n}_ppc{ppc}_nm{nm}"
+    return wl

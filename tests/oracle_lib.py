@@ -141,6 +141,7 @@ def lib():
         L.orc_pic_adjoint.restype = C.c_double
         L.orc_pic_tangent.argtypes = [C.POINTER(PicProblem), P, P, P, P, P]
         L.orc_pic_tangent.restype = C.c_double
+        L.orc_adjoint_plant.argtypes = [C.c_int]
         _lib = L
     return _lib
 

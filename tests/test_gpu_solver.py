@@ -218,3 +218,49 @@ def test_solver_errors():
         s.set_matrix(16, 16, [0], [10], [1.0])
     with pytest.raises(E.PsumError):
         E.SolverBackend("no_such_backend")
+
+
+def _wall_problem_2d(n0, n1, seed=0):
+    og = O.wall_grid([n0 - 1, n1 - 1])
+    shp = O.node_shape(og)
+    nn = O.num_nodes(og)
+    fixed = np.zeros(shp, np.uint8)
+    fixed[0, :] = fixed[-1, :] = fixed[:, 0] = fixed[:, -1] = 1
+    fixed = fixed.ravel()
+    r, c, v = O.poisson_coo(og, 1.0, fixed)
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal(nn)
+    b[fixed == 1] = 0.0
+    return og, nn, r, c, v, b
+
+
+@pytest.mark.parametrize("backend", ["hip_multigrid_gpu", "hip_multigrid32f_gpu"])
+@pytest.mark.parametrize("case", ["periodic128", "periodic96", "walls_70x45"])
+@pytest.mark.parametrize("prepost", [(3, 3), (2, 1), (5, 0), (1, 4)])
+@pytest.mark.parametrize("stop", ["max_cycles 3", "max_cycles 30 tol 1e-9"])
+def test_fused_2d_kernels_are_bit_identical(backend, case, prepost, stop):
+    """The fused 2D tile smoother (several sweeps per launch, coarse-grid
+    correction folded in) and the fused residual + restriction perform the
+    unfused kernels' operations in the same order: equal bits."""
+    E = _E()
+    if case.startswith("periodic"):
+        n = int(case[8:])
+        og, nn, r, c, v, b = _periodic_problem(n, seed=3)
+        I, J = n, n
+    else:
+        I, J = 70, 45
+        og, nn, r, c, v, b = _wall_problem_2d(I, J, seed=4)
+    pre, post = prepost
+    out = []
+    for fuse in (0, 1):
+        s = E.SolverBackend(backend)
+        s.set_options(f"I {I} J {J} depth 4 pre_relax {pre} post_relax {post} {stop} "
+                      f"fuse {fuse}")
+        s.set_matrix(nn, nn, r, c, v)
+        x = torch.zeros(nn, dtype=torch.float64, device="cuda")
+        s.solve(_dev(b), x)
+        s.solve(_dev(b * 1.01), x)
+        out.append((x.cpu().numpy(), s.stats()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1][0] == out[1][1][0]
+    assert np.array_equal(np.float64(out[0][1][1]), np.float64(out[1][1][1]), equal_nan=True)
