@@ -191,6 +191,57 @@ def run_adjoint(E, grid, ppc, steps, ckpt, depth, pre, post):
     return out
 
 
+def run_batch(E, n_inst, steps, seed=11):
+    """Batched ensemble: n_inst independent 2D periodic instances (grids
+    from {64^2, 128^2, 256^2, 512^2}, 16-128 e-/cell, perturbation mode
+    1..8, amplitude 1e-3..5e-2), FP32 with FP64 MG refinement, autotuned
+    (cycle count, sort bin, sort interval), one step of all instances
+    replayed as one hipGraph.  Also times the same instances stepped one
+    after another without the graph."""
+    from psum_amd.workloads import ensemble
+    t0 = time.time()
+    specs = ensemble(n_inst, seed, device="cuda")
+    meta = [s.pop("meta") for s in specs]
+    total = sum(m["particles"] for m in meta)
+    b = E.Batch(specs)
+    for s in specs:
+        s["species"][0]["pos"] = s["species"][0]["vel"] = None
+    del specs
+    import torch
+    torch.cuda.empty_cache()
+    log(f"[bench] batch: {n_inst} instances, {total} particles, set up in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    tun = b.tune(warmup=20, probe=10)
+    log(f"[bench] batch: tuned in {time.time() - t0:.1f}s")
+    b.reset_history(steps + 2)
+    b.run(2)                      # capture + first replay
+    ms = b.run(steps)
+    # the same instances, same schedules, stepped one after another without the graph
+    seq_steps = max(1, min(steps, 20))
+    t0 = time.perf_counter()
+    for st in b.instances:
+        st.run(seq_steps, history=False)
+    seq_ms = 1e3 * (time.perf_counter() - t0) / seq_steps
+    hist_ok = all(np.all(np.isfinite(b.history(i)[:, 0])) for i in range(len(b)))
+    out = {
+        "workload": f"{n_inst} independent 2D periodic instances, grids from 64^2..512^2 nodes, "
+                    f"16-128 e-/cell, mode 1..8, amplitude 1e-3..5e-2, FP32 + FP64 MG refinement "
+                    f"(1e-8 calibration), {steps} steps, one hipGraph per step",
+        "instances": n_inst, "particles": total, "steps": steps,
+        "ms_per_step": ms / steps, "particle_steps_per_s": total * steps / (ms * 1e-3),
+        "sequential_ms_per_step": seq_ms, "graph_speedup": seq_ms / (ms / steps),
+        "histories_finite": bool(hist_ok),
+        "tuning": {
+            "fixed_cycles": [t["fixed_cycles"] for t in tun],
+            "sort_every": [t["sort_every"] for t in tun],
+            "sort_shift": [t["sort_shift"] for t in tun]},
+    }
+    log(f"[bench] batch: {ms / steps:.3f} ms/step ({out['particle_steps_per_s']:.3e} particle-steps/s), "
+        f"sequential {seq_ms:.3f} ms/step, graph speedup {out['graph_speedup']:.2f}")
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +263,9 @@ def main():
     ap.add_argument("--adjoint-grid", type=int, default=256)
     ap.add_argument("--adjoint-ppc", type=int, default=64)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the runs, copy one buffer of the push kernel's algorithmic size "
                          "(FP32 x, y, vx, vy of every particle) so that HBM counters can be "
@@ -265,6 +319,9 @@ def main():
     if not args.no_adjoint:
         adjoint = run_adjoint(E, args.adjoint_grid, args.adjoint_ppc, args.adjoint_steps, 20,
                               args.depth, args.pre, args.post)
+    batch = None
+    if not args.no_batch:
+        batch = run_batch(E, args.batch_instances, args.batch_steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_steps)
@@ -302,6 +359,7 @@ def main():
         "field_energy_rel_err_first100": fe_err,
         "mixed_cycles_per_step": mixed["cycles_per_step"],
         "adjoint": adjoint,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -214,6 +214,48 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
                                  void* const* grad_v, double* t_fwd_ms, double* t_bwd_ms);
 void psum_stepper_destroy(void* h);
 
+/* Solver and sort schedule of a stepper: fixed_cycles > 0 switches its
+ * multigrid to that many V-cycles per solve (one per refinement iteration,
+ * no tolerance check); sorts every sort_every steps (0 = never) on
+ * 2^sort_shift-cell bins.  No reference counterpart (tuning knobs). */
+psum_status psum_stepper_set_schedule(void* h, int fixed_cycles, int sort_every, int sort_shift);
+
+/* ---- batched ensemble (no reference counterpart: the reference steps one
+ * problem per process).  Independent instances, each configured like a
+ * stepper, advance together: one step of all of them is captured into one
+ * hipGraph and replayed.  Inside the graph each solver runs a fixed V-cycle
+ * count; sorts run between replays at per-instance intervals, on 2^shift-cell
+ * bins.  psum_batch_tune is the autotuner: per instance, `warmup`
+ * tolerance-controlled steps calibrate the cycle count (most needed + 1),
+ * then each bin shift 0..2 gets a timed sort and `probe` timed steps, and the
+ * (shift, interval) minimising the modelled sort + push cost is chosen.
+ * psum_batch_configure sets the same three values by hand.
+ * psum_batch_instance returns the instance's stepper handle for
+ * psum_stepper_state / _phi / _stats (do not destroy it). */
+typedef struct {
+    int fixed_cycles;
+    int sort_every;
+    int sort_shift;
+    int pad_;
+    double sort_ms;            /* measured sort time at the chosen shift */
+    double push_ms;            /* push time of the first step after that sort */
+    double push_sort_cost_ms;  /* modelled (sort + push) cost per step */
+} psum_batch_tuning;
+
+void* psum_batch_create(const psum_step_config* cfgs, int n);
+psum_status psum_batch_load(void* b, int inst, int s, const void* const* pos, const void* const* vel,
+                            int64_t n);
+psum_status psum_batch_configure(void* b, int inst, int fixed_cycles, int sort_every, int sort_shift);
+psum_status psum_batch_tune(void* b, int warmup, int probe, psum_batch_tuning* out);
+/* history rows (5 doubles per step, as psum_stepper_run) restart at 0 and
+ * are kept for up to `rows` steps */
+psum_status psum_batch_reset_history(void* b, int64_t rows);
+/* `steps` replays; *t_ms = HIP-event time of the whole call */
+psum_status psum_batch_run(void* b, int steps, double* t_ms);
+psum_status psum_batch_history(void* b, int inst, double* out, int64_t max_rows, int64_t* rows);
+void* psum_batch_instance(void* b, int inst);
+void psum_batch_destroy(void* b);
+
 /* ---- adjoint kernels (used by psum_stepper_adjoint) --------------------
  * Reverse of gather + kick + drift for one species, in place on (lx, lv):
  *   lvt = lv + dt lx;  lx -= qm dt H(x)^T lvt;  lv = lvt;

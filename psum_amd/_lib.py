@@ -62,6 +62,12 @@ class PsumStepConfig(C.Structure):
     ]
 
 
+class PsumBatchTuning(C.Structure):
+    _fields_ = [("fixed_cycles", C.c_int32), ("sort_every", C.c_int32), ("sort_shift", C.c_int32),
+                ("pad_", C.c_int32), ("sort_ms", C.c_double), ("push_ms", C.c_double),
+                ("push_sort_cost_ms", C.c_double)]
+
+
 class FuncTable(C.Structure):
     """struct func_table (PSuM field_solver/register/interface.h:14-23)."""
     _fields_ = [
@@ -114,6 +120,16 @@ SIGNATURES = [
     ("psum_stepper_adjoint", ST, [P, C.c_int, C.c_int, C.POINTER(C.c_double), P, P,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psum_stepper_destroy", None, [P]),
+    ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
+    ("psum_batch_create", P, [C.POINTER(PsumStepConfig), C.c_int]),
+    ("psum_batch_load", ST, [P, C.c_int, C.c_int, P, P, I64]),
+    ("psum_batch_configure", ST, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("psum_batch_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
+    ("psum_batch_reset_history", ST, [P, I64]),
+    ("psum_batch_run", ST, [P, C.c_int, C.POINTER(C.c_double)]),
+    ("psum_batch_history", ST, [P, C.c_int, P, I64, C.POINTER(I64)]),
+    ("psum_batch_instance", P, [P, C.c_int]),
+    ("psum_batch_destroy", None, [P]),
     ("psum_hip_adjoint_push", ST, [_GP, C.c_int, P, P, P, I64, P, C.c_double, C.c_double, P, P, P]),
     ("psum_hip_adjoint_blocks", I64, [I64]),
     ("psum_hip_adjoint_gather", ST, [_GP, C.c_int, P, P, P, I64, P, C.c_double, C.c_double, P, P]),

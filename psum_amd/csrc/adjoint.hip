@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kA) void k_adj_push(DevGrid g, const T* __restrict_
             T t = (T)0;
 #pragma unroll
             for (int e = 0; e < D; ++e) t = t + lvt[e] * cic_grad_coef<T, D>(W, G, i, e);
-            const long long term = __double2ll_rn((mqdt * (double)t) * scale);
+            const long long term = round_to_ll((mqdt * (double)t) * scale);
             if (term) atomicAdd(&acc[support_index<D>(g, cw, i)], (unsigned long long)term);
         }
     }

@@ -12,6 +12,12 @@ namespace psum_hip {
 
 void set_error(const std::string& msg);
 
+// Sort keys of 2^shift-cell bins (shift 0: the cell key of psum_hip_bin)
+// and the number of bins.
+psum_status bin_keys(const psum_grid* g, int dtype, void* const* pos, int64_t n, uint32_t* keys,
+                     int shift, void* stream);
+uint32_t bin_count(const psum_grid& g, int shift);
+
 #define PSUM_HIP_TRY(expr)                                                              \
     do {                                                                                \
         hipError_t e_ = (expr);                                                         \
@@ -95,6 +101,17 @@ __device__ __forceinline__ bool base_cell(const DevGrid& g, const double* x, int
         cw[d] = ci < 0 ? ci + w : (ci >= w ? ci - w : ci);
     }
     return true;
+}
+
+// Round-to-nearest-even of t to int64 for |t| < 2^51 (llrint in the default
+// rounding mode): t + 1.5*2^52 lands in [2^52, 2^53), whose ulp is 1, so the
+// addition rounds t to an integer and the difference of the bit patterns is
+// that integer.
+__device__ __forceinline__ long long round_to_ll(double t)
+{
+    const double m = 6755399441055744.0;   // 1.5 * 2^52
+    const double u = t + m;
+    return (long long)(__double_as_longlong(u) - __double_as_longlong(m));
 }
 
 // Linear node index of support point i: bit j of i offsets dimension D-1-j;

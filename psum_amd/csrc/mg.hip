@@ -289,18 +289,26 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
 }
 
 // ---- fused 2D kernels: several Jacobi sweeps of one tile in LDS ----------
-// The tile (kTileE^2 outputs) is loaded with a halo of `sweeps` nodes; sweep
-// k updates the region shrunk by k, so after `sweeps` sweeps the inner tile
-// holds exactly what `sweeps` launches of k_smooth produce (same operations
-// in the same order).  Halo nodes beyond a non-wrapped edge are clamped
-// copies: the stencils of real nodes give them zero coefficients, as in
-// k_smooth's clamped neighbour.  PROLONG adds P x_c to every loaded node
-// first (k_prolong_add's arithmetic), fusing the coarse-grid correction with
-// the post-sweeps; ACC adds the result into an FP64 iterate (the mixed
-// refinement's x += e, fused into the last level-0 sweep).
-constexpr int kTileE = 32;
-constexpr int kMaxFuse = 4;
-constexpr int kTileW = kTileE + 2 * kMaxFuse;
+// A block owns a tile of kTileRows x (kTileX - 2h) output nodes and loads it
+// with a halo of h = `sweeps` nodes (kTileX = 64 columns: one wave per row,
+// conflict-free LDS rows).  Sweep k updates the region shrunk by k, so after
+// h sweeps the output nodes hold exactly what h launches of k_smooth produce
+// (same operations in the same order).  Thread (tx, ty) owns column tx and
+// rows ty, ty+4, ...; its right-hand side and class ids stay in registers,
+// the iterate ping-pongs between two LDS planes, and class tables of up to
+// kMaxClsLds classes are staged in LDS.  Halo nodes beyond a non-wrapped
+// edge are clamped copies: the stencils of real nodes give them zero
+// coefficients, as in k_smooth's clamped neighbour.  PROLONG adds P x_c to
+// every loaded node first (k_prolong_add's arithmetic), fusing the
+// coarse-grid correction with the post-sweeps; ACC adds the result into an
+// FP64 iterate (the mixed refinement's x += e, fused into the last level-0
+// sweep).
+constexpr int kTileX = 64;
+constexpr int kTileRows = 32;
+constexpr int kMaxFuse = 3;
+constexpr int kTileRmax = kTileRows + 2 * kMaxFuse;
+constexpr int kRowsPerThread = (kTileRmax + 3) / 4;
+constexpr int kMaxClsLds = 32;
 
 __device__ __forceinline__ int tile_coord(int g, int n, int wrap)
 {
@@ -332,66 +340,87 @@ __device__ __forceinline__ T prolong_at(const DevLevel& Cl, const DevTransfer<T>
 }
 
 template <typename T, bool COMPACT, bool PROLONG, bool ACC>
-__global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable<T> relax,
+__global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable<T> relax, int ncls,
                                                             const T* __restrict__ xin,
                                                             const T* __restrict__ b,
-                                                            T* __restrict__ xout, T two3, int sweeps,
+                                                            T* __restrict__ xout, T two3, int h,
                                                             DevLevel Cl, DevTransfer<T> tr,
                                                             const T* __restrict__ xc,
                                                             double* __restrict__ xacc)
 {
-    __shared__ T sx[2][kTileW * kTileW];
-    __shared__ T sb[kTileW * kTileW];
-    __shared__ uint16_t scl[kTileW * kTileW];
+    __shared__ T sx[2][kTileRmax * kTileX];
+    __shared__ T ctab[kMaxClsLds * 9];
+    const int tx = threadIdx.x & (kTileX - 1), ty = threadIdx.x / kTileX;
     const int n0 = L.n[0], n1 = L.n[1];
-    const int nt1 = (n1 + kTileE - 1) / kTileE;
-    const int t0 = (int)(blockIdx.x / nt1) * kTileE, t1 = (int)(blockIdx.x % nt1) * kTileE;
-    const int h = sweeps;
-    const int W = kTileE + 2 * h;
-    for (int e = threadIdx.x; e < W * W; e += kMgThreads) {
-        const int a = e / W, c = e - a * W;
-        const int g0 = tile_coord(t0 - h + a, n0, L.wrap[0]);
-        const int g1 = tile_coord(t1 - h + c, n1, L.wrap[1]);
-        const int64_t gi = (int64_t)g0 * n1 + g1;
-        T xv = xin[gi];
-        if (PROLONG) xv = prolong_at<T>(Cl, tr, xc, g0, g1) * (T)1 + xv * (T)1;
-        const int li = a * kTileW + c;
-        sx[0][li] = xv;
-        sb[li] = b[gi];
-        scl[li] = L.cls[gi];
+    const int ostep = kTileX - 2 * h;
+    const int ntc = (n1 + ostep - 1) / ostep;
+    const int t0 = (int)(blockIdx.x / ntc) * kTileRows, t1 = (int)(blockIdx.x % ntc) * ostep;
+    const int R = kTileRows + 2 * h;
+    const bool lds_tab = ncls <= kMaxClsLds;
+    if (lds_tab)
+        for (int i = threadIdx.x; i < ncls * 9; i += kMgThreads) ctab[i] = relax.coef[i];
+    const int g1 = tile_coord(t1 - h + tx, n1, L.wrap[1]);
+    T breg[kRowsPerThread];
+    int creg[kRowsPerThread];
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const int i = ty + 4 * k;
+        breg[k] = (T)0;
+        creg[k] = 0;
+        if (i < R) {
+            const int g0 = tile_coord(t0 - h + i, n0, L.wrap[0]);
+            const int64_t gi = (int64_t)g0 * n1 + g1;
+            T xv = xin[gi];
+            if (PROLONG) xv = prolong_at<T>(Cl, tr, xc, g0, g1) * (T)1 + xv * (T)1;
+            sx[0][i * kTileX + tx] = xv;
+            breg[k] = b[gi];
+            creg[k] = L.cls[gi];
+        }
     }
     __syncthreads();
     int cur = 0;
-    for (int k = 1; k <= h; ++k) {
-        const int span = W - 2 * k;
-        for (int e = threadIdx.x; e < span * span; e += kMgThreads) {
-            const int a = k + e / span, c = k + e % span;
-            const int li = a * kTileW + c;
-            const T* coef = relax.coef + (int64_t)scl[li] * 9;
-            const T* xs = sx[cur];
+    for (int sw = 1; sw <= h; ++sw) {
+        const T* xs = sx[cur];
+        T* xn = sx[cur ^ 1];
+        const bool colok = tx >= sw && tx < kTileX - sw;
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int i = ty + 4 * k;
+            if (!(colok && i >= sw && i < R - sw)) continue;
+            T c[9];
+            if (lds_tab) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) c[q] = ctab[creg[k] * 9 + q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) c[q] = relax.coef[(int64_t)creg[k] * 9 + q];
+            }
+            const int li = i * kTileX + tx;
             T t = (T)0;
 #pragma unroll
             for (int da = -1; da <= 1; ++da)
 #pragma unroll
                 for (int db = -1; db <= 1; ++db) {
                     if (COMPACT && da != 0 && db != 0) continue;
-                    t = t + coef[(da + 1) * 3 + (db + 1)] * xs[li + da * kTileW + db];
+                    t = t + c[(da + 1) * 3 + (db + 1)] * xs[li + da * kTileX + db];
                 }
-            sx[cur ^ 1][li] = t * (T)1 + sb[li] * (two3 * (T)1);
+            xn[li] = t * (T)1 + breg[k] * (two3 * (T)1);
         }
         __syncthreads();
         cur ^= 1;
     }
-    for (int e = threadIdx.x; e < kTileE * kTileE; e += kMgThreads) {
-        const int a = e / kTileE, c = e - a * kTileE;
-        const int g0 = t0 + a, g1 = t1 + c;
-        if (g0 < n0 && g1 < n1) {
-            const int64_t gi = (int64_t)g0 * n1 + g1;
-            const T v = sx[cur][(a + h) * kTileW + (c + h)];
-            xout[gi] = v;
-            // the mixed refinement's x += e (k_add_correction's arithmetic)
-            if (ACC) xacc[gi] += (double)v;
-        }
+    const int g1o = t1 + tx - h;
+    if (tx < h || tx >= kTileX - h || g1o >= n1) return;
+#pragma unroll
+    for (int k = 0; k < kRowsPerThread; ++k) {
+        const int i = ty + 4 * k;
+        if (i < h || i >= h + kTileRows) continue;
+        const int g0 = t0 + i - h;
+        if (g0 >= n0) continue;
+        const int64_t gi = (int64_t)g0 * n1 + g1o;
+        const T v = sx[cur][i * kTileX + tx];
+        xout[gi] = v;
+        if (ACC) xacc[gi] += (double)v;
     }
 }
 
@@ -1547,15 +1576,17 @@ private:
         }
         const DevLevel L = dl(l);
         const T two3 = (T)(2.0 / 3);
-        const unsigned nb = (unsigned)(((L.n[0] + kTileE - 1) / kTileE) * ((L.n[1] + kTileE - 1) / kTileE));
         const DevLevel C = xc ? dl(l + 1) : L;
         const DevTransfer<T> tr = xc ? transfer<T>(l) : DevTransfer<T>{};
         const bool cmp = host_[l].compact;
+        const int ncls = (int)host_[l].dinv.size();
         bool accd = false;
         while (count > 0) {
             const int k = std::min(count, kMaxFuse);
+            const int ostep = kTileX - 2 * k;
+            const unsigned nb = (unsigned)(((L.n[0] + kTileRows - 1) / kTileRows) * ((L.n[1] + ostep - 1) / ostep));
             double* a = (k == count) ? acc : nullptr;
-#define TILE(CMP, PRO, ACC) hipLaunchKernelGGL((k_smooth_tile<T, CMP, PRO, ACC>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3, k, C, tr, xc, a)
+#define TILE(CMP, PRO, ACC) hipLaunchKernelGGL((k_smooth_tile<T, CMP, PRO, ACC>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), ncls, x, b, buf, two3, k, C, tr, xc, a)
             if (a) {
                 if (xc) { if (cmp) TILE(true, true, true); else TILE(false, true, true); }
                 else { if (cmp) TILE(true, false, true); else TILE(false, false, true); }

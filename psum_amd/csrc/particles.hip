@@ -50,7 +50,7 @@ template <typename T, int D>
 __global__ __launch_bounds__(kThreads) void k_bin(DevGrid g, const T* __restrict__ x0,
                                                   const T* __restrict__ x1,
                                                   const T* __restrict__ x2, int64_t n,
-                                                  uint32_t* __restrict__ keys)
+                                                  uint32_t* __restrict__ keys, int shift)
 {
     const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (p >= n) return;
@@ -62,9 +62,12 @@ __global__ __launch_bounds__(kThreads) void k_bin(DevGrid g, const T* __restrict
     int cw[3];
     double s[3];
     if (isfinite(x[0]) && base_cell<D>(g, x, cw, s)) {
-        uint32_t lin = (uint32_t)cw[0];
+        // shift 0: the reference cell key n2i(cN); shift b: the key of the
+        // 2^b-cell bin (sort granularity of the stepper, not a cell index)
+        uint32_t lin = (uint32_t)(cw[0] >> shift);
 #pragma unroll
-        for (int d = 1; d < D; ++d) lin = lin * (uint32_t)g.w[d] + (uint32_t)cw[d];
+        for (int d = 1; d < D; ++d)
+            lin = lin * (uint32_t)((g.w[d] + (1 << shift) - 1) >> shift) + (uint32_t)(cw[d] >> shift);
         key = lin;
     }
     keys[p] = key;
@@ -550,6 +553,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         __syncthreads();
 
         constexpr int NS = 1 << D;
+        const double wsc = w_dep * scale;
         long long run[NS];
         int cur[D];
         bool have = false;
@@ -593,8 +597,9 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
             for (int i = 0; i < NS; ++i) {
                 const T cf = cic_coef<T, D>(W, i);
                 if (!(fabs((double)cf) > 1e-10)) continue;
-                const double v = w_dep * (double)cf;
-                run[i] += __double2ll_rn(v * scale);
+                // (w*cf)*2^S: scaling by a power of two is exact, so one
+                // product with w*2^S rounds identically
+                run[i] += round_to_ll((double)cf * wsc);
             }
         }
         if (have) flush();
@@ -800,12 +805,36 @@ psum_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, voi
 
 }  // namespace psum_hip
 
+static bool pow2(double s)
+{
+    int e = 0;
+    return s > 0.0 && std::isfinite(s) && std::frexp(s, &e) == 0.5;
+}
+
 extern "C" {
 
 psum_status psum_hip_bin(const psum_grid* g, int dtype, void* const* pos, int64_t n,
                          uint32_t* keys, void* stream)
 {
+    return psum_hip::bin_keys(g, dtype, pos, n, keys, 0, stream);
+}
+
+}  // extern "C"
+
+namespace psum_hip {
+
+uint32_t bin_count(const psum_grid& g, int shift)
+{
+    uint32_t n = 1;
+    for (int d = 0; d < g.dim; ++d) n *= (uint32_t)((g.cells[d] + 1 + (1 << shift) - 1) >> shift);
+    return n;
+}
+
+psum_status bin_keys(const psum_grid* g, int dtype, void* const* pos, int64_t n, uint32_t* keys,
+                     int shift, void* stream)
+{
     PSUM_ARG_CHECK(grid_ok(g), "psum_hip_bin: invalid grid");
+    PSUM_ARG_CHECK(shift >= 0 && shift <= 8, "psum_hip_bin: bin shift out of range");
     PSUM_ARG_CHECK(n >= 0 && (n == 0 || (pos && keys)), "psum_hip_bin: null pointer");
     if (n == 0) return PSUM_OK;
     const DevGrid dg = to_dev(*g);
@@ -814,7 +843,7 @@ psum_status psum_hip_bin(const psum_grid* g, int dtype, void* const* pos, int64_
     hipLaunchKernelGGL((k_bin<T, DIM>), dim3(nb), dim3(kThreads), 0, S(stream), dg,             \
                        static_cast<const T*>(pos[0]),                                           \
                        DIM > 1 ? static_cast<const T*>(pos[1]) : nullptr,                       \
-                       DIM > 2 ? static_cast<const T*>(pos[2]) : nullptr, n, keys)
+                       DIM > 2 ? static_cast<const T*>(pos[2]) : nullptr, n, keys, shift)
     if (dtype == PSUM_F64) {
         if (g->dim == 1) PSUM_BIN(double, 1); else if (g->dim == 2) PSUM_BIN(double, 2); else PSUM_BIN(double, 3);
     } else if (dtype == PSUM_F32) {
@@ -827,6 +856,10 @@ psum_status psum_hip_bin(const psum_grid* g, int dtype, void* const* pos, int64_
     PSUM_HIP_TRY(hipGetLastError());
     return PSUM_OK;
 }
+
+}  // namespace psum_hip
+
+extern "C" {
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -963,6 +996,7 @@ psum_status psum_hip_deposit_cic(const psum_grid* g, int dtype, void* const* pos
 {
     PSUM_ARG_CHECK(grid_ok(g), "psum_hip_deposit_cic: invalid grid");
     PSUM_ARG_CHECK(acc && (n == 0 || pos), "psum_hip_deposit_cic: null pointer");
+    PSUM_ARG_CHECK(pow2(scale), "psum_hip_deposit_cic: scale must be a power of two");
     if (dtype == PSUM_F64)
         return launch_push<double>(g, pos, nullptr, n, nullptr, 0, 0, 0, w, scale, acc, nullptr,
                                    counters, false, S(stream));
@@ -993,6 +1027,7 @@ psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos
 {
     PSUM_ARG_CHECK(grid_ok(g), "psum_hip_gather_push: invalid grid");
     PSUM_ARG_CHECK(n == 0 || (pos && vel && phi), "psum_hip_gather_push: null pointer");
+    PSUM_ARG_CHECK(!acc || pow2(scale), "psum_hip_gather_push: scale must be a power of two");
     if (dtype == PSUM_F64)
         return launch_push<double>(g, pos, vel, n, phi, qm, dt, mass, w_dep, scale, acc, diag,
                                    counters, true, S(stream));

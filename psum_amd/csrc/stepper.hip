@@ -18,6 +18,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
@@ -37,6 +38,7 @@ double mg_last_rel(HipMultigrid* h);
 void mg_solve_homogeneous(HipMultigrid* h, double* b, double* x);
 int mg_predicted(HipMultigrid* h);
 void mg_set_predicted(HipMultigrid* h, int p);
+void mg_set_fixed_cycles(HipMultigrid* h, int cycles);
 }  // namespace psum_hip
 
 extern "C" {
@@ -346,15 +348,19 @@ public:
         S_TRY(hipMalloc(&sum_, 8 * sizeof(double)));
         S_TRY(hipMalloc(&counters_, 8 * sizeof(int64_t)));
         S_TRY(hipMemsetAsync(counters_, 0, 8 * sizeof(int64_t), st_));
-        if (cfg_.sort_every > 0) {
-            S_TRY(hipMalloc(&keys_, (size_t)max_cap_ * 4 + 4));
-            S_TRY(hipMalloc(&skeys_, (size_t)max_cap_ * 4 + 4));
-            S_TRY(hipMalloc(&perm_, (size_t)max_cap_ * 4 + 4));
-            sort_bytes_ = psum_hip_sort_workspace(max_cap_);
-            S_TRY(hipMalloc(&sort_work_, sort_bytes_));
-            S_TRY(hipMalloc(&nvalid_, sizeof(unsigned long long)));
-        }
+        if (cfg_.sort_every > 0) sort_buffers();
         S_TRY(hipStreamSynchronize(st_));
+    }
+
+    void sort_buffers()
+    {
+        if (keys_) return;
+        S_TRY(hipMalloc(&keys_, (size_t)max_cap_ * 4 + 4));
+        S_TRY(hipMalloc(&skeys_, (size_t)max_cap_ * 4 + 4));
+        S_TRY(hipMalloc(&perm_, (size_t)max_cap_ * 4 + 4));
+        sort_bytes_ = psum_hip_sort_workspace(max_cap_);
+        S_TRY(hipMalloc(&sort_work_, sort_bytes_));
+        S_TRY(hipMalloc(&nvalid_, sizeof(unsigned long long)));
     }
 
     ~Stepper()
@@ -508,6 +514,7 @@ public:
             have_prev_ = true;
         }
         cycles_ += mg_last_cycles(mg_);
+        max_cyc_seen_ = std::max(max_cyc_seen_, mg_last_cycles(mg_));
         last_rel_ = mg_last_rel(mg_);
         if (cfg_.dtype == PSUM_F32) CHK(psum_hip_f64_to_f32(phi_, phi32_, nn_, st_));
     }
@@ -783,8 +790,10 @@ public:
                 S_TRY(hipMalloc(&lx_[q * 3 + d], b));
                 S_TRY(hipMalloc(&lv_[q * 3 + d], b));
             }
-        S_TRY(hipEventCreate(&aev_[0]));
-        S_TRY(hipEventCreate(&aev_[1]));
+        if (!aev_[0]) {
+            S_TRY(hipEventCreate(&aev_[0]));
+            S_TRY(hipEventCreate(&aev_[1]));
+        }
     }
 
     void set_ckpt_capacity(int ckpt)
@@ -802,6 +811,75 @@ public:
     }
 
     int64_t adjoint_cycles() const { return adj_cycles_; }
+
+    // ---------------------------------------------------------- batch hooks
+    void set_sort_shift(int b) { sort_shift_ = b; }
+    int sort_shift() const { return sort_shift_; }
+    void set_sort_every(int k) { cfg_.sort_every = k; }
+    int sort_every() const { return cfg_.sort_every; }
+    void set_stable_buffers(bool on) { stable_buffers_ = on; }
+    void set_fixed_cycles(int k)
+    {
+        mg_set_fixed_cycles(mg_, k);
+        fixed_cycles_ = k;
+    }
+    int fixed_cycles() const { return fixed_cycles_; }
+    int max_cycles_seen() const { return max_cyc_seen_; }
+    void reset_cycles_seen() { max_cyc_seen_ = 0; }
+    hipStream_t stream() const { return st_; }
+    int64_t steps_done() const { return step_; }
+    void count_step() { ++step_; }
+    void uncount_step() { --step_; }
+    // total push milliseconds since set_timing(true), after a stream sync
+    void flush_timing(double* ms)
+    {
+        flush_events();
+        if (ms) *ms = push_ms_;
+    }
+    bool acc_valid() const { return acc_valid_; }
+    void prepare_capture()
+    {
+        if (!acc_valid_) deposit_all();
+        if (timing_) set_timing(false);
+    }
+    void hist_reset(int64_t rows)
+    {
+        reserve_hist(rows);
+        hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kT), 0, st_, (long long*)hist_count_, (int64_t)1);
+    }
+    int64_t hist_read(double* out, int64_t max_rows)
+    {
+        long long rows = 0;
+        S_TRY(hipMemcpyAsync(&rows, hist_count_, sizeof rows, hipMemcpyDeviceToHost, st_));
+        S_TRY(hipStreamSynchronize(st_));
+        const int64_t r = std::min<int64_t>(std::min<int64_t>(rows, hist_cap_), max_rows);
+        if (out && r > 0) {
+            S_TRY(hipMemcpyAsync(out, hist_, (size_t)r * 5 * sizeof(double), hipMemcpyDeviceToHost, st_));
+            S_TRY(hipStreamSynchronize(st_));
+        }
+        return r;
+    }
+    int64_t particles() const
+    {
+        int64_t n = 0;
+        for (int q = 0; q < cfg_.nspecies; ++q) n += sp_[q].n;
+        return n;
+    }
+    // HIP-event time of one sort pass (all species), ms
+    float timed_sort()
+    {
+        if (!aev_[0]) {
+            S_TRY(hipEventCreate(&aev_[0]));
+            S_TRY(hipEventCreate(&aev_[1]));
+        }
+        S_TRY(hipEventRecord(aev_[0], st_));
+        sort_all();
+        S_TRY(hipEventRecord(aev_[1], st_));
+        S_TRY(hipEventSynchronize(aev_[1]));
+        float ms = 0;
+        S_TRY(hipEventElapsedTime(&ms, aev_[0], aev_[1]));
+        return ms;
+    }
 
     void phi(double* out)
     {
@@ -835,6 +913,10 @@ private:
     int rec_steps_ = 0, rec_ckpt_ = 1;
     int64_t adj_cycles_ = 0;
     int adj_predicted_ = 1;
+    int sort_shift_ = 0;
+    bool stable_buffers_ = false;
+    int fixed_cycles_ = 0;
+    int max_cyc_seen_ = 0;
     hipEvent_t aev_[2] = {nullptr, nullptr};
     static constexpr int kEv = 64;
     hipEvent_t ev_[2 * kEv] = {};
@@ -905,14 +987,16 @@ private:
         acc_valid_ = true;
     }
 
+public:
     void sort_all()
     {
         const int D = cfg_.grid.dim;
-        const uint32_t nkeys = (uint32_t)nn_;
+        sort_buffers();
+        const uint32_t nkeys = sort_shift_ ? bin_count(cfg_.grid, sort_shift_) : (uint32_t)nn_;
         for (int q = 0; q < cfg_.nspecies; ++q) {
             Species& sp = sp_[q];
             if (sp.n == 0) continue;
-            CHK(psum_hip_bin(&cfg_.grid, cfg_.dtype, sp.pos, sp.n, keys_, st_));
+            CHK(bin_keys(&cfg_.grid, cfg_.dtype, sp.pos, sp.n, keys_, sort_shift_, st_));
             CHK(psum_hip_sort(keys_, sp.n, nkeys, skeys_, perm_, sort_work_, sort_bytes_, st_));
             void* src[6];
             void* dst[6];
@@ -920,9 +1004,17 @@ private:
             for (int d = 0; d < D; ++d) { src[nc] = sp.pos[d]; dst[nc] = sp.pos2[d]; ++nc; }
             for (int d = 0; d < D; ++d) { src[nc] = sp.vel[d]; dst[nc] = sp.vel2[d]; ++nc; }
             CHK(psum_hip_permute(cfg_.dtype, nc, src, dst, perm_, sp.n, st_));
-            for (int d = 0; d < D; ++d) {
-                std::swap(sp.pos[d], sp.pos2[d]);
-                std::swap(sp.vel[d], sp.vel2[d]);
+            if (stable_buffers_) {
+                // keep the arrays a captured graph points at
+                for (int d = 0; d < D; ++d) {
+                    S_TRY(hipMemcpyAsync(sp.pos[d], sp.pos2[d], (size_t)sp.n * esz_, hipMemcpyDeviceToDevice, st_));
+                    S_TRY(hipMemcpyAsync(sp.vel[d], sp.vel2[d], (size_t)sp.n * esz_, hipMemcpyDeviceToDevice, st_));
+                }
+            } else {
+                for (int d = 0; d < D; ++d) {
+                    std::swap(sp.pos[d], sp.pos2[d]);
+                    std::swap(sp.vel[d], sp.vel2[d]);
+                }
             }
             // drop particles whose key is invalid (absorbed): they sort last
             unsigned long long nv = (unsigned long long)sp.n;
@@ -932,6 +1024,223 @@ private:
             S_TRY(hipStreamSynchronize(st_));
             sp.n = (int64_t)nv;
         }
+    }
+};
+
+// ------------------------------------------------------------------ batch
+// Independent instances stepped together.  One step of every instance is
+// captured into a single hipGraph (fork/join across the instances' streams,
+// so that small kernels of different instances run concurrently) and
+// replayed every step.  Inside the graph the solvers run a fixed V-cycle
+// count; sorting runs between replays at per-instance intervals.  tune()
+// is the autotuner: it calibrates each instance's cycle count (the most a
+// tolerance-controlled warm-up needed, plus one) and picks its sort bin size
+// and sort interval from measured sort and push times.
+class Batch {
+public:
+    Batch(const psum_step_config* cfgs, int n)
+    {
+        if (n < 1) throw std::invalid_argument("psum_batch_create: need at least one instance");
+        for (int i = 0; i < n; ++i) {
+            psum_step_config c = cfgs[i];
+            c.mg_graph = 0;       // the batch graph captures the solver's kernels itself
+            c.extrapolate = 0;
+            inst_.emplace_back(new Stepper(c));
+            inst_.back()->set_stable_buffers(true);
+        }
+        S_TRY(hipStreamCreate(&origin_));
+        S_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+        join_.resize(n);
+        for (auto& e : join_) S_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        S_TRY(hipEventCreate(&t0_));
+        S_TRY(hipEventCreate(&t1_));
+    }
+
+    ~Batch()
+    {
+        drop();
+        for (auto& e : join_) if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {fork_, t0_, t1_}) if (e) (void)hipEventDestroy(e);
+        if (origin_) (void)hipStreamDestroy(origin_);
+    }
+
+    int size() const { return (int)inst_.size(); }
+    Stepper& at(int i)
+    {
+        if (i < 0 || i >= size()) throw std::invalid_argument("psum_batch: instance out of range");
+        return *inst_[i];
+    }
+
+    void load(int i, int s, const void* const* pos, const void* const* vel, int64_t n)
+    {
+        at(i).load(s, pos, vel, n);
+        drop();
+    }
+
+    void configure(int i, int cycles, int sort_every, int sort_shift)
+    {
+        Stepper& st = at(i);
+        st.set_fixed_cycles(cycles);
+        st.set_sort_every(sort_every);
+        st.set_sort_shift(sort_shift);
+        drop();
+    }
+
+    // Per instance: `warmup` tolerance-controlled steps (cycle calibration),
+    // then for each bin shift a timed sort and `probe` timed steps.  The sort
+    // interval K minimises (t_sort + sum_{j<K} t_push(j)) / K over
+    // K in {5, 10, 20, 50, 100, 200}, with t_push(j) beyond the probe window
+    // extrapolated linearly from its second half.
+    void tune(int warmup, int probe, psum_batch_tuning* out)
+    {
+        if (warmup < 1 || probe < 2) throw std::invalid_argument("psum_batch_tune: warmup >= 1, probe >= 2");
+        static const int kCand[] = {5, 10, 20, 50, 100, 200};
+        for (int i = 0; i < size(); ++i) {
+            Stepper& st = at(i);
+            st.reset_cycles_seen();
+            st.run_steps(warmup, nullptr, true, false);
+            const int cycles = st.max_cycles_seen() + 1;
+            int best_shift = 0, best_k = 0;
+            double best_cost = 1e300, best_sort = 0.0, best_push = 0.0;
+            for (int shift = 0; shift <= 2; ++shift) {
+                st.set_sort_shift(shift);
+                const float t_sort = st.timed_sort();
+                std::vector<double> tp(probe);
+                for (int j = 0; j < probe; ++j) {
+                    st.set_timing(true);
+                    st.step_body(false);
+                    S_TRY(hipStreamSynchronize(st.stream()));
+                    double ms = 0.0;
+                    st.flush_timing(&ms);
+                    tp[j] = ms;
+                }
+                st.set_timing(false);
+                // linear extrapolation from the second half of the window
+                const int h0 = probe / 2;
+                double sx = 0, sy = 0, sxx = 0, sxy = 0;
+                for (int j = h0; j < probe; ++j) {
+                    sx += j; sy += tp[j]; sxx += (double)j * j; sxy += j * tp[j];
+                }
+                const double m = probe - h0;
+                const double den = m * sxx - sx * sx;
+                const double slope = den > 0 ? std::max(0.0, (m * sxy - sx * sy) / den) : 0.0;
+                const double icpt = (sy - slope * sx) / m;
+                for (int K : kCand) {
+                    double sum = t_sort;
+                    for (int j = 0; j < K; ++j) sum += j < probe ? tp[j] : icpt + slope * j;
+                    const double cost = sum / K;
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_shift = shift;
+                        best_k = K;
+                        best_sort = t_sort;
+                        best_push = tp[0];
+                    }
+                }
+            }
+            configure(i, cycles, best_k, best_shift);
+            if (out) {
+                out[i].fixed_cycles = cycles;
+                out[i].sort_every = best_k;
+                out[i].sort_shift = best_shift;
+                out[i].pad_ = 0;
+                out[i].sort_ms = best_sort;
+                out[i].push_ms = best_push;
+                out[i].push_sort_cost_ms = best_cost;
+            }
+        }
+    }
+
+    // replay the captured step `steps` times; sorts due before a step run
+    // eagerly on the instance's stream first
+    void run(int steps, double* t_ms)
+    {
+        for (auto& p : inst_)
+            if (p->fixed_cycles() < 1)
+                throw std::invalid_argument("psum_batch_run: every instance needs a fixed cycle count "
+                                            "(psum_batch_configure or psum_batch_tune)");
+        if (!exec_) capture();
+        // instances reloaded since the capture need their deposit redone
+        for (int i = 0; i < size(); ++i)
+            if (!inst_[i]->acc_valid()) {
+                inst_[i]->prepare_capture();
+                S_TRY(hipEventRecord(join_[i], inst_[i]->stream()));
+                S_TRY(hipStreamWaitEvent(origin_, join_[i], 0));
+            }
+        S_TRY(hipEventRecord(t0_, origin_));
+        for (int s = 0; s < steps; ++s) {
+            // eager sorts run on the instances' streams after the previous
+            // replay (fork) and before the next one (join)
+            bool forked = false;
+            for (int i = 0; i < size(); ++i) {
+                Stepper& p = *inst_[i];
+                const int k = p.sort_every();
+                if (k > 0 && p.steps_done() % k == 0) {
+                    if (!forked) {
+                        S_TRY(hipEventRecord(fork_, origin_));
+                        forked = true;
+                    }
+                    S_TRY(hipStreamWaitEvent(p.stream(), fork_, 0));
+                    p.sort_all();
+                    S_TRY(hipEventRecord(join_[i], p.stream()));
+                    S_TRY(hipStreamWaitEvent(origin_, join_[i], 0));
+                }
+            }
+            S_TRY(hipGraphLaunch(exec_, origin_));
+            for (auto& p : inst_) p->count_step();
+        }
+        S_TRY(hipEventRecord(t1_, origin_));
+        S_TRY(hipEventSynchronize(t1_));
+        float ms = 0;
+        S_TRY(hipEventElapsedTime(&ms, t0_, t1_));
+        if (t_ms) *t_ms = ms;
+    }
+
+    void hist_reset(int64_t rows)
+    {
+        for (auto& p : inst_) p->hist_reset(rows);
+        for (auto& p : inst_) S_TRY(hipStreamSynchronize(p->stream()));
+        drop();   // the history capacity is baked into the graph
+    }
+
+private:
+    static void S_TRY(hipError_t e)
+    {
+        if (e != hipSuccess) throw std::runtime_error(std::string("psum_batch: ") + hipGetErrorString(e));
+    }
+
+    std::vector<std::unique_ptr<Stepper>> inst_;
+    hipStream_t origin_ = nullptr;
+    hipEvent_t fork_ = nullptr, t0_ = nullptr, t1_ = nullptr;
+    std::vector<hipEvent_t> join_;
+    hipGraphExec_t exec_ = nullptr;
+
+    void drop()
+    {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        exec_ = nullptr;
+    }
+
+    void capture()
+    {
+        for (auto& p : inst_) p->prepare_capture();
+        for (auto& p : inst_) S_TRY(hipStreamSynchronize(p->stream()));
+        S_TRY(hipStreamBeginCapture(origin_, hipStreamCaptureModeThreadLocal));
+        S_TRY(hipEventRecord(fork_, origin_));
+        for (int i = 0; i < size(); ++i) {
+            Stepper& st = *inst_[i];
+            S_TRY(hipStreamWaitEvent(st.stream(), fork_, 0));
+            st.step_body(true);
+            S_TRY(hipEventRecord(join_[i], st.stream()));
+            S_TRY(hipStreamWaitEvent(origin_, join_[i], 0));
+        }
+        hipGraph_t graph = nullptr;
+        S_TRY(hipStreamEndCapture(origin_, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec_, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        S_TRY(e);
+        // step_body counted one step per instance while capturing
+        for (auto& p : inst_) p->uncount_step();
     }
 };
 
@@ -1052,5 +1361,88 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
 }
 
 void psum_stepper_destroy(void* h) { delete static_cast<Stepper*>(h); }
+
+psum_status psum_stepper_set_schedule(void* h, int fixed_cycles, int sort_every, int sort_shift)
+{
+    if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
+    if (sort_shift < 0 || sort_shift > 8 || sort_every < 0) {
+        set_error("psum_stepper_set_schedule: sort_every >= 0, sort_shift in 0..8");
+        return PSUM_ERR_ARG;
+    }
+    return guarded([&]() {
+        Stepper* s = static_cast<Stepper*>(h);
+        if (fixed_cycles > 0) s->set_fixed_cycles(fixed_cycles);
+        s->set_sort_every(sort_every);
+        s->set_sort_shift(sort_shift);
+    });
+}
+
+void* psum_batch_create(const psum_step_config* cfgs, int n)
+{
+    if (!cfgs || n < 1) {
+        set_error("psum_batch_create: need n >= 1 configs");
+        return nullptr;
+    }
+    Batch* b = nullptr;
+    if (guarded([&]() { b = new Batch(cfgs, n); }) != PSUM_OK) return nullptr;
+    return b;
+}
+
+psum_status psum_batch_load(void* h, int inst, int s, const void* const* pos, const void* const* vel,
+                            int64_t n)
+{
+    if (!h) { set_error("null batch"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Batch*>(h)->load(inst, s, pos, vel, n); });
+}
+
+psum_status psum_batch_configure(void* h, int inst, int fixed_cycles, int sort_every, int sort_shift)
+{
+    if (!h) { set_error("null batch"); return PSUM_ERR_ARG; }
+    if (sort_shift < 0 || sort_shift > 8 || sort_every < 0) {
+        set_error("psum_batch_configure: sort_every >= 0, sort_shift in 0..8");
+        return PSUM_ERR_ARG;
+    }
+    return guarded([&]() { static_cast<Batch*>(h)->configure(inst, fixed_cycles, sort_every, sort_shift); });
+}
+
+psum_status psum_batch_tune(void* h, int warmup, int probe, psum_batch_tuning* out)
+{
+    if (!h) { set_error("null batch"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Batch*>(h)->tune(warmup, probe, out); });
+}
+
+psum_status psum_batch_reset_history(void* h, int64_t rows)
+{
+    if (!h || rows < 0) { set_error("psum_batch_reset_history: bad argument"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Batch*>(h)->hist_reset(rows); });
+}
+
+psum_status psum_batch_run(void* h, int steps, double* t_ms)
+{
+    if (!h || steps < 0) { set_error("psum_batch_run: bad argument"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Batch*>(h)->run(steps, t_ms); });
+}
+
+psum_status psum_batch_history(void* h, int inst, double* out, int64_t max_rows, int64_t* rows)
+{
+    if (!h) { set_error("null batch"); return PSUM_ERR_ARG; }
+    return guarded([&]() {
+        const int64_t r = static_cast<Batch*>(h)->at(inst).hist_read(out, max_rows);
+        if (rows) *rows = r;
+    });
+}
+
+void* psum_batch_instance(void* h, int inst)
+{
+    if (!h) return nullptr;
+    Batch* b = static_cast<Batch*>(h);
+    if (inst < 0 || inst >= b->size()) {
+        set_error("psum_batch_instance: out of range");
+        return nullptr;
+    }
+    return &b->at(inst);
+}
+
+void psum_batch_destroy(void* h) { delete static_cast<Batch*>(h); }
 
 }  // extern "C"

@@ -23,7 +23,8 @@ from ._lib import (KEY_INVALID, PSUM_F32, PSUM_F64, FuncTable, PsumError, PsumGr
 
 __all__ = ["make_grid", "periodic_grid", "wall_grid", "num_nodes", "fx_shift", "cell_keys",
            "stable_sort", "permute", "compact", "deposit", "fixed_to_double", "gather_push",
-           "field_energy", "SolverBackend", "Stepper", "KEY_INVALID", "PsumError",
+           "field_energy", "SolverBackend", "Stepper", "Batch", "step_config", "KEY_INVALID",
+           "PsumError",
            "get_solver_funcs"]
 
 
@@ -290,6 +291,109 @@ class SolverBackend:
 
 # ---------------------------------------------------------------- stepper
 
+def step_config(g, species, dtype="f64", mixed_mg=False, depth=4, pre=2, post=1, max_cycles=20,
+                tol=1e-10, eps=1.0, dt=0.1, rho_bg=0.0, sort_every=0, fixed=None, fixed_val=None,
+                capacity=None, mg_graph=True, extrapolate=False):
+    """psum_step_config for the given species (dicts with q, m, w, pos) and
+    the host arrays it points at (keep them alive while it is used)."""
+    cfg = PsumStepConfig()
+    cfg.grid = g
+    cfg.dtype = PSUM_F64 if dtype == "f64" else PSUM_F32
+    cfg.mixed_mg = int(bool(mixed_mg))
+    cfg.nspecies = len(species)
+    cfg.depth, cfg.pre, cfg.post, cfg.max_cycles = depth, pre, post, max_cycles
+    cfg.sort_every = sort_every
+    cfg.mg_graph = int(bool(mg_graph))
+    cfg.extrapolate = int(bool(extrapolate))
+    cfg.tol, cfg.eps, cfg.dt, cfg.rho_bg = tol, eps, dt, rho_bg
+    keep = []
+    if fixed is not None:
+        fm = np.ascontiguousarray(fixed, np.uint8)
+        keep.append(fm)
+        cfg.fixed = fm.ctypes.data
+    if fixed_val is not None:
+        fv = np.ascontiguousarray(fixed_val, np.float64)
+        keep.append(fv)
+        cfg.fixed_val = fv.ctypes.data
+    for i, sp in enumerate(species):
+        n = len(sp["pos"][0])
+        cfg.species[i] = PsumSpecies(sp["q"], sp["m"], sp["w"], n, capacity if capacity else n)
+    return cfg, keep
+
+
+class Batch:
+    """Independent instances stepped together in one replayed hipGraph
+    (psum_batch_*).  instances: list of dicts with the Stepper arguments
+    (g, species, dtype, ...)."""
+
+    def __init__(self, instances):
+        from ._lib import PsumStepConfig as _Cfg
+        cfgs = (_Cfg * len(instances))()
+        self._keep = []
+        for i, a in enumerate(instances):
+            a = dict(a)
+            g, species = a.pop("g"), a["species"]
+            a.pop("species")
+            c, k = step_config(g, species, **a)
+            cfgs[i] = c
+            self._keep += k
+        self.h = lib().psum_batch_create(cfgs, len(instances))
+        if not self.h:
+            raise PsumError("psum_batch_create: " +
+                            lib().psum_hip_last_error().decode(errors="replace"))
+        self.instances = []
+        for i, a in enumerate(instances):
+            hi = lib().psum_batch_instance(self.h, i)
+            st = Stepper(a["g"], a["species"], dtype=a.get("dtype", "f64"), _handle=hi)
+            for s, sp in enumerate(a["species"]):
+                st.load(s, sp["pos"], sp["vel"])
+            self.instances.append(st)
+
+    def __len__(self):
+        return len(self.instances)
+
+    def configure(self, i, fixed_cycles, sort_every=0, sort_shift=0):
+        check(lib().psum_batch_configure(self.h, i, fixed_cycles, sort_every, sort_shift),
+              "psum_batch_configure")
+
+    def tune(self, warmup=20, probe=10):
+        """Autotune every instance; returns a list of dicts (fixed_cycles,
+        sort_every, sort_shift, sort_ms, push_ms, push_sort_cost_ms)."""
+        from ._lib import PsumBatchTuning
+        out = (PsumBatchTuning * len(self.instances))()
+        check(lib().psum_batch_tune(self.h, warmup, probe, out), "psum_batch_tune")
+        return [{f: getattr(t, f) for f, _ in PsumBatchTuning._fields_ if f != "pad_"} for t in out]
+
+    def reset_history(self, rows):
+        check(lib().psum_batch_reset_history(self.h, rows), "psum_batch_reset_history")
+
+    def run(self, steps):
+        """`steps` graph replays; returns the HIP-event milliseconds."""
+        ms = C.c_double(0)
+        check(lib().psum_batch_run(self.h, steps, C.byref(ms)), "psum_batch_run")
+        return ms.value
+
+    def history(self, i, max_rows=1 << 20):
+        rows = C.c_int64(0)
+        check(lib().psum_batch_history(self.h, i, None, max_rows, C.byref(rows)),
+              "psum_batch_history")
+        n = rows.value
+        out = np.zeros((n, 5), np.float64)
+        check(lib().psum_batch_history(self.h, i, out.ctypes.data_as(C.c_void_p), n, C.byref(rows)),
+              "psum_batch_history")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            for st in self.instances:
+                st.h = None
+            lib().psum_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class Stepper:
     """The reference step loop (deposit -> solve -> gather/push) run natively.
 
@@ -298,41 +402,31 @@ class Stepper:
 
     def __init__(self, g, species, dtype="f64", mixed_mg=False, depth=4, pre=2, post=1,
                  max_cycles=20, tol=1e-10, eps=1.0, dt=0.1, rho_bg=0.0, sort_every=0,
-                 fixed=None, fixed_val=None, capacity=None, mg_graph=True, extrapolate=False):
-        cfg = PsumStepConfig()
-        cfg.grid = g
-        cfg.dtype = PSUM_F64 if dtype == "f64" else PSUM_F32
-        cfg.mixed_mg = int(bool(mixed_mg))
-        cfg.nspecies = len(species)
-        cfg.depth, cfg.pre, cfg.post, cfg.max_cycles = depth, pre, post, max_cycles
-        cfg.sort_every = sort_every
-        cfg.mg_graph = int(bool(mg_graph))
-        cfg.extrapolate = int(bool(extrapolate))
-        cfg.tol, cfg.eps, cfg.dt, cfg.rho_bg = tol, eps, dt, rho_bg
-        self._keep = []
-        if fixed is not None:
-            fm = np.ascontiguousarray(fixed, np.uint8)
-            self._keep.append(fm)
-            cfg.fixed = fm.ctypes.data
-        if fixed_val is not None:
-            fv = np.ascontiguousarray(fixed_val, np.float64)
-            self._keep.append(fv)
-            cfg.fixed_val = fv.ctypes.data
-        np_dt = np.float64 if dtype == "f64" else np.float32
-        for i, sp in enumerate(species):
-            n = len(sp["pos"][0])
-            cfg.species[i] = PsumSpecies(sp["q"], sp["m"], sp["w"], n,
-                                         capacity if capacity else n)
+                 fixed=None, fixed_val=None, capacity=None, mg_graph=True, extrapolate=False,
+                 _handle=None):
         self.g = g
         self.dim = g.dim
-        self.np_dtype = np_dt
+        self.np_dtype = np.float64 if dtype == "f64" else np.float32
+        self.n = [len(sp["pos"][0]) for sp in species]
+        self._owned = _handle is None
+        if _handle is not None:          # an instance owned by a Batch
+            self.h = _handle
+            return
+        cfg, self._keep = step_config(g, species, dtype, mixed_mg, depth, pre, post, max_cycles,
+                                      tol, eps, dt, rho_bg, sort_every, fixed, fixed_val,
+                                      capacity, mg_graph, extrapolate)
         self.h = lib().psum_stepper_create(C.byref(cfg))
         if not self.h:
             raise PsumError("psum_stepper_create: " +
                             lib().psum_hip_last_error().decode(errors="replace"))
-        self.n = [len(sp["pos"][0]) for sp in species]
         for i, sp in enumerate(species):
             self.load(i, sp["pos"], sp["vel"])
+
+    def set_schedule(self, fixed_cycles=0, sort_every=0, sort_shift=0):
+        """fixed_cycles > 0: that many V-cycles per solve, no tolerance check;
+        sort every `sort_every` steps on 2^sort_shift-cell bins."""
+        check(lib().psum_stepper_set_schedule(self.h, fixed_cycles, sort_every, sort_shift),
+              "set_schedule")
 
     def load(self, s, pos, vel):
         """Replace species s with pos / vel: numpy arrays, or device tensors of
@@ -452,7 +546,8 @@ class Stepper:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().psum_stepper_destroy(self.h)
+            if getattr(self, "_owned", True):
+                lib().psum_stepper_destroy(self.h)
             self.h = None
 
     def __del__(self):

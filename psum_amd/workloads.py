@@ -109,3 +109,53 @@ def two_stream_modes(n=256, ppc=64, nm=32, amp=1e-3, seed=5, dtype=np.float32):
     wl["nm"] = nm
     wl["name"] = f"two_stream2d_modes_{n}x{n}_ppc{ppc}_nm{nm}"
     return wl
+
+
+def ensemble(n_inst=64, seed=11, dtype=np.float32, sizes=(64, 128, 256, 512), ppc_range=(16, 128),
+             k_range=(1, 8), amp_range=(1e-3, 5e-2), device=None):
+    """Batched-ensemble instances: n_inst independent 2D periodic problems,
+    grid edge drawn uniformly from `sizes` (nodes), electrons per cell from
+    ppc_range, a perturbation of mode k and amplitude amp (k, amp drawn from
+    k_range / amp_range) imposed by the displacement
+    x = xi + amp L/(2 pi k) sin(2 pi k xi / L) of uniform xi (density
+    1 - amp cos(2 pi k x / L) to first order), Maxwellian v (v_th = 1) over a
+    uniform ion background, dt = 0.1, FP32 particles with FP32 V-cycles in
+    FP64 refinement to 1e-8.  device="cuda" draws the particles on the GPU
+    (torch generator seeded per instance) and returns CUDA tensors.
+    Returns a list of Stepper keyword dicts (g, species, ...) with a 'meta'
+    dict each."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inst):
+        n = int(rng.choice(sizes))
+        ppc = int(rng.integers(ppc_range[0], ppc_range[1] + 1))
+        k = int(rng.integers(k_range[0], k_range[1] + 1))
+        amp = float(rng.uniform(*amp_range))
+        N = n * n * ppc
+        L = float(n)
+        if device is None:
+            r = np.random.default_rng(seed * 1000 + i)
+            xi = r.random(N) * L
+            x = np.mod(xi + amp * L / (2 * np.pi * k) * np.sin(2 * np.pi * k * xi / L), L).astype(dtype)
+            y = (r.random(N) * L).astype(dtype)
+            vx = r.standard_normal(N).astype(dtype)
+            vy = r.standard_normal(N).astype(dtype)
+        else:
+            import torch
+            tdt = torch.float32 if dtype == np.float32 else torch.float64
+            gen = torch.Generator(device=device).manual_seed(seed * 1000 + i)
+            xi = torch.rand(N, generator=gen, device=device, dtype=torch.float64) * L
+            x = torch.remainder(xi + amp * L / (2 * np.pi * k) * torch.sin(2 * np.pi * k * xi / L), L)
+            x = x.to(tdt)
+            del xi
+            y = (torch.rand(N, generator=gen, device=device, dtype=torch.float64) * L).to(tdt)
+            vx = torch.randn(N, generator=gen, device=device, dtype=tdt)
+            vy = torch.randn(N, generator=gen, device=device, dtype=tdt)
+        sp = dict(q=-1.0, m=1.0, w=1.0 / ppc, pos=[x, y], vel=[vx, vy])
+        depth = max(2, int(np.log2(n)) - 4)
+        out.append(dict(g=periodic_grid([n, n]), species=[sp],
+                        dtype="f32" if dtype == np.float32 else "f64",
+                        mixed_mg=dtype == np.float32, depth=depth, pre=2, post=2, max_cycles=40,
+                        tol=1e-8, dt=0.1, rho_bg=1.0, sort_every=50,
+                        meta=dict(n=n, ppc=ppc, k=k, amp=amp, particles=N)))
+    return out

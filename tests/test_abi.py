@@ -51,6 +51,15 @@ def test_argument_errors_do_not_touch_the_gpu():
     assert rc == 1
     assert b"invalid grid" in L.psum_hip_last_error()
     assert L.psum_hip_sort(None, -1, 5, None, None, None, 0, None) == 1
+    # the fixed-point deposit needs a power-of-two scale (exact rescaling)
+    g2 = _lib.PsumGrid()
+    g2.dim = 2
+    g2.cells[0] = g2.cells[1] = 4
+    for d in range(2):
+        g2.delta[d] = g2.delta_r[d] = 1.0
+    acc = (C.c_int64 * 25)()
+    assert L.psum_hip_deposit_cic(C.byref(g2), 0, None, 0, 1.0, 3.0, acc, None, None) == 1
+    assert b"power of two" in L.psum_hip_last_error()
 
 
 def test_grid_struct_layout_matches_oracle():
