@@ -242,6 +242,43 @@ def run_batch(E, n_inst, steps, seed=11):
     return out
 
 
+def run_c3(E, cells, ppc, steps, depth, pre, post):
+    """3D bounded box (phi = 0 on the faces, absorbing walls, compaction by
+    the sort), electrons + ions (mass ratio 1836, Te/Ti = 10), FP32
+    particles with FP32 V-cycles in FP64 refinement to 1e-8, 7-point MG."""
+    import torch
+    from psum_amd.workloads import bounded_3d
+    t0 = time.time()
+    wl = bounded_3d(cells=cells, ppc=ppc, device="cuda")
+    n = sum(len(s["pos"][0]) for s in wl["species"])
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f32", mixed_mg=True, depth=depth, pre=pre,
+                   post=post, max_cycles=40, tol=1e-8, dt=wl["dt"], rho_bg=0.0, sort_every=20,
+                   fixed=wl["fixed"])
+    for s in wl["species"]:
+        s["pos"] = s["vel"] = None
+    torch.cuda.empty_cache()
+    log(f"[bench] c3: {n} particles, set up in {time.time() - t0:.1f}s")
+    st.run(2, history=False)
+    torch.cuda.synchronize()
+    st.set_timing(True)
+    t0 = time.perf_counter()
+    hist = st.run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    push_ms, launches, pushed = st.push_timing()
+    stats = st.stats()
+    out = {"workload": f"3D bounded box {cells}^3 cells, {ppc} e- + {ppc} ions per cell ({n} "
+                       f"particles), grounded walls, absorbing boundary + compaction, FP32 + FP64 "
+                       f"MG refinement to 1e-8, {steps} steps",
+           "particles": n, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+           "particle_steps_per_s": n * steps / wall, "push_ms_avg": push_ms / max(1, launches),
+           "absorbed": stats["absorbed"], "field_energy_last": float(hist[-1, 0])}
+    log(f"[bench] c3: {out['ms_per_step']:.2f} ms/step, {out['particle_steps_per_s']:.3e} "
+        f"particle-steps/s, absorbed {stats['absorbed']}")
+    st.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +303,10 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--c3", action="store_true", help="also run the 3D bounded box (256^3, 1.07e9 particles)")
+    ap.add_argument("--c3-cells", type=int, default=256)
+    ap.add_argument("--c3-ppc", type=int, default=32)
+    ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the runs, copy one buffer of the push kernel's algorithmic size "
                          "(FP32 x, y, vx, vy of every particle) so that HBM counters can be "
@@ -319,6 +360,9 @@ def main():
     if not args.no_adjoint:
         adjoint = run_adjoint(E, args.adjoint_grid, args.adjoint_ppc, args.adjoint_steps, 20,
                               args.depth, args.pre, args.post)
+    c3 = None
+    if args.c3:
+        c3 = run_c3(E, args.c3_cells, args.c3_ppc, args.c3_steps, 6, args.pre, args.post)
     batch = None
     if not args.no_batch:
         batch = run_batch(E, args.batch_instances, args.batch_steps)
@@ -360,6 +404,7 @@ def main():
         "mixed_cycles_per_step": mixed["cycles_per_step"],
         "adjoint": adjoint,
         "batch": batch,
+        "c3": c3,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

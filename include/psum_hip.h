@@ -242,6 +242,12 @@ typedef struct {
     double push_sort_cost_ms;  /* modelled (sort + push) cost per step */
 } psum_batch_tuning;
 
+/* The autotuner of psum_batch_tune for one stepper: returns the calibrated
+ * cycle count and the chosen sort interval / bin shift (the bin shift is
+ * applied; apply the rest with psum_stepper_set_schedule).  The state
+ * advances by the tuning steps. */
+psum_status psum_stepper_tune(void* h, int warmup, int probe, psum_batch_tuning* out);
+
 void* psum_batch_create(const psum_step_config* cfgs, int n);
 psum_status psum_batch_load(void* b, int inst, int s, const void* const* pos, const void* const* vel,
                             int64_t n);

@@ -121,6 +121,7 @@ SIGNATURES = [
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psum_stepper_destroy", None, [P]),
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
+    ("psum_stepper_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
     ("psum_batch_create", P, [C.POINTER(PsumStepConfig), C.c_int]),
     ("psum_batch_load", ST, [P, C.c_int, C.c_int, P, P, I64]),
     ("psum_batch_configure", ST, [P, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -307,46 +307,31 @@ constexpr int kTileX = 64;
 constexpr int kTileRows = 32;
 constexpr int kMaxFuse = 3;
 constexpr int kTileRmax = kTileRows + 2 * kMaxFuse;
-constexpr int kRowsPerThread = (kTileRmax + 3) / 4;
+constexpr int kTileThreads = 1024;
+constexpr int kTileRowGroups = kTileThreads / kTileX;                       // 16
+constexpr int kRowsPerThread = (kTileRmax + kTileRowGroups - 1) / kTileRowGroups;   // 3
 constexpr int kMaxClsLds = 32;
 
+// wrapped (periodic axis) or clamped node coordinate; g is at most a few
+// nodes outside [0, n)
 __device__ __forceinline__ int tile_coord(int g, int n, int wrap)
 {
     if (wrap) {
-        g %= n;
-        return g < 0 ? g + n : g;
+        while (g < 0) g += n;
+        while (g >= n) g -= n;
+        return g;
     }
     return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
 
-template <typename T>
-__device__ __forceinline__ T prolong_at(const DevLevel& Cl, const DevTransfer<T>& tr, const T* xc,
-                                        int c0, int c1)
-{
-    T s = (T)0;
-    const int p0 = tr.pc[0][c0], p1 = tr.pc[1][c1];
-    const T* w0 = tr.pw[0] + 2 * c0;
-    const T* w1 = tr.pw[1] + 2 * c1;
-    for (int a = 0; a < 2; ++a) {
-        if (w0[a] == (T)0) continue;
-        const int i0 = wrap_index(p0 + a, Cl.n[0], tr.wrap[0]);
-        for (int b = 0; b < 2; ++b) {
-            if (w1[b] == (T)0) continue;
-            const int i1 = wrap_index(p1 + b, Cl.n[1], tr.wrap[1]);
-            s = s + (w0[a] * w1[b]) * xc[(int64_t)i0 * Cl.n[1] + i1];
-        }
-    }
-    return s;
-}
-
 template <typename T, bool COMPACT, bool PROLONG, bool ACC>
-__global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable<T> relax, int ncls,
-                                                            const T* __restrict__ xin,
-                                                            const T* __restrict__ b,
-                                                            T* __restrict__ xout, T two3, int h,
-                                                            DevLevel Cl, DevTransfer<T> tr,
-                                                            const T* __restrict__ xc,
-                                                            double* __restrict__ xacc)
+__global__ __launch_bounds__(kTileThreads) void k_smooth_tile(DevLevel L, DevTable<T> relax, int ncls,
+                                                              const T* __restrict__ xin,
+                                                              const T* __restrict__ b,
+                                                              T* __restrict__ xout, T two3, int h,
+                                                              DevLevel Cl, DevTransfer<T> tr,
+                                                              const T* __restrict__ xc,
+                                                              double* __restrict__ xacc)
 {
     __shared__ T sx[2][kTileRmax * kTileX];
     __shared__ T ctab[kMaxClsLds * 9];
@@ -358,20 +343,46 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable
     const int R = kTileRows + 2 * h;
     const bool lds_tab = ncls <= kMaxClsLds;
     if (lds_tab)
-        for (int i = threadIdx.x; i < ncls * 9; i += kMgThreads) ctab[i] = relax.coef[i];
+        for (int i = threadIdx.x; i < ncls * 9; i += kTileThreads) ctab[i] = relax.coef[i];
     const int g1 = tile_coord(t1 - h + tx, n1, L.wrap[1]);
+    // column part of the prolongation, once per thread
+    int q1[2] = {0, 0};
+    T w1[2] = {(T)0, (T)0};
+    if (PROLONG) {
+        const int p1 = tr.pc[1][g1];
+        w1[0] = tr.pw[1][2 * g1];
+        w1[1] = tr.pw[1][2 * g1 + 1];
+        q1[0] = wrap_index(p1, Cl.n[1], tr.wrap[1]);
+        q1[1] = wrap_index(p1 + 1, Cl.n[1], tr.wrap[1]);
+    }
     T breg[kRowsPerThread];
     int creg[kRowsPerThread];
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
-        const int i = ty + 4 * k;
+        const int i = ty + kTileRowGroups * k;
         breg[k] = (T)0;
         creg[k] = 0;
         if (i < R) {
             const int g0 = tile_coord(t0 - h + i, n0, L.wrap[0]);
             const int64_t gi = (int64_t)g0 * n1 + g1;
             T xv = xin[gi];
-            if (PROLONG) xv = prolong_at<T>(Cl, tr, xc, g0, g1) * (T)1 + xv * (T)1;
+            if (PROLONG) {
+                // k_prolong_add's sum: a over the row factor, b over the column factor
+                const int p0 = tr.pc[0][g0];
+                const T w0[2] = {tr.pw[0][2 * g0], tr.pw[0][2 * g0 + 1]};
+                T s = (T)0;
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    if (w0[a] == (T)0) continue;
+                    const int i0 = wrap_index(p0 + a, Cl.n[0], tr.wrap[0]);
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb) {
+                        if (w1[bb] == (T)0) continue;
+                        s = s + (w0[a] * w1[bb]) * xc[(int64_t)i0 * Cl.n[1] + q1[bb]];
+                    }
+                }
+                xv = s * (T)1 + xv * (T)1;
+            }
             sx[0][i * kTileX + tx] = xv;
             breg[k] = b[gi];
             creg[k] = L.cls[gi];
@@ -385,7 +396,7 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable
         const bool colok = tx >= sw && tx < kTileX - sw;
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; ++k) {
-            const int i = ty + 4 * k;
+            const int i = ty + kTileRowGroups * k;
             if (!(colok && i >= sw && i < R - sw)) continue;
             T c[9];
             if (lds_tab) {
@@ -413,7 +424,7 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable
     if (tx < h || tx >= kTileX - h || g1o >= n1) return;
 #pragma unroll
     for (int k = 0; k < kRowsPerThread; ++k) {
-        const int i = ty + 4 * k;
+        const int i = ty + kTileRowGroups * k;
         if (i < h || i >= h + kTileRows) continue;
         const int g0 = t0 + i - h;
         if (g0 >= n0) continue;
@@ -422,6 +433,92 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_tile(DevLevel L, DevTable
         xout[gi] = v;
         if (ACC) xacc[gi] += (double)v;
     }
+}
+
+// Tiled residual + restriction (2D): a block restricts a kRRRows x kRRCols
+// coarse tile.  It loads x on the fine support of that tile plus a one-node
+// halo into LDS, evaluates the fine residuals there (k_residual's
+// arithmetic), then sums each coarse node's support (k_restrict's order).
+// The host uses it only where every tile's fine support fits (at most
+// kRRFineRows x kRRFineCols nodes; true for the halving and identity
+// transfers).
+constexpr int kRRRows = 16;
+constexpr int kRRCols = 30;
+constexpr int kRRFineRows = 34;
+constexpr int kRRFineCols = 62;
+
+template <typename T, bool COMPACT>
+__global__ __launch_bounds__(kTileThreads) void k_resid_restrict_tile(
+    DevLevel F, DevLevel Cl, DevTransfer<T> tr, DevTable<T> Af, int ncls, DevTable<T> Ac,
+    const T* __restrict__ x, const T* __restrict__ bf, T* __restrict__ bc, T* __restrict__ xc,
+    T first_sweep)
+{
+    __shared__ T sxx[(kRRFineRows + 2) * 64];
+    __shared__ T sr[kRRFineRows * 64];
+    __shared__ T ctab[kMaxClsLds * 9];
+    const int ntc = (Cl.n[1] + kRRCols - 1) / kRRCols;
+    const int c0lo = (int)(blockIdx.x / ntc) * kRRRows, c1lo = (int)(blockIdx.x % ntc) * kRRCols;
+    const int c0hi = min(c0lo + kRRRows, Cl.n[0]), c1hi = min(c1lo + kRRCols, Cl.n[1]);
+    const int f0lo = tr.rf[0][c0lo], f0hi = tr.rf[0][c0hi - 1] + tr.rn[0][c0hi - 1];
+    const int f1lo = tr.rf[1][c1lo], f1hi = tr.rf[1][c1hi - 1] + tr.rn[1][c1hi - 1];
+    const int FR = f0hi - f0lo, FC = f1hi - f1lo;
+    const bool lds_tab = ncls <= kMaxClsLds;
+    if (lds_tab)
+        for (int i = threadIdx.x; i < ncls * 9; i += kTileThreads) ctab[i] = Af.coef[i];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int n0 = F.n[0], n1 = F.n[1];
+    if (tx < FC + 2) {
+        const int g1 = tile_coord(f1lo - 1 + tx, n1, F.wrap[1]);
+        for (int i = ty; i < FR + 2; i += kTileRowGroups) {
+            const int g0 = tile_coord(f0lo - 1 + i, n0, F.wrap[0]);
+            sxx[i * 64 + tx] = x[(int64_t)g0 * n1 + g1];
+        }
+    }
+    __syncthreads();
+    if (tx < FC) {
+        const int g1 = tile_coord(f1lo + tx, n1, F.wrap[1]);
+        for (int i = ty; i < FR; i += kTileRowGroups) {
+            const int g0 = tile_coord(f0lo + i, n0, F.wrap[0]);
+            const int64_t gi = (int64_t)g0 * n1 + g1;
+            const int cl = F.cls[gi];
+            T c[9];
+            if (lds_tab) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) c[q] = ctab[cl * 9 + q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) c[q] = Af.coef[(int64_t)cl * 9 + q];
+            }
+            const int li = (i + 1) * 64 + (tx + 1);
+            T t = (T)0;
+#pragma unroll
+            for (int da = -1; da <= 1; ++da)
+#pragma unroll
+                for (int db = -1; db <= 1; ++db) {
+                    if (COMPACT && da != 0 && db != 0) continue;
+                    t = t + c[(da + 1) * 3 + (db + 1)] * sxx[li + da * 64 + db];
+                }
+            sr[i * 64 + tx] = t * (T)-1 + bf[gi] * (T)1;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x >= kRRRows * kRRCols) return;
+    const int c0 = c0lo + (int)threadIdx.x / kRRCols, c1 = c1lo + (int)threadIdx.x % kRRCols;
+    if (c0 >= c0hi || c1 >= c1hi) return;
+    const int f0 = tr.rf[0][c0], m0 = tr.rn[0][c0];
+    const int f1 = tr.rf[1][c1], m1 = tr.rn[1][c1];
+    T s = (T)0;
+    for (int a = 0; a < m0; ++a) {
+        const T w0 = tr.rw[0][3 * c0 + a];
+        for (int b = 0; b < m1; ++b) {
+            const T w = w0 * tr.rw[1][3 * c1 + b];
+            s = s + w * sr[(f0 + a - f0lo) * 64 + (f1 + b - f1lo)];
+        }
+    }
+    const int64_t ci = (int64_t)c0 * Cl.n[1] + c1;
+    const T bv = s * Ac.dinv[Cl.cls[ci]];
+    bc[ci] = bv;
+    xc[ci] = first_sweep != (T)0 ? bv * (first_sweep * (T)1) : (T)0;
 }
 
 // k_residual fused into k_restrict (2D): the restriction evaluates the fine
@@ -1231,6 +1328,7 @@ public:
         dense_inverse(L.back(), cinv_host_, pin_ >= 0);
         host_ = std::move(L);
         tr_host_ = std::move(T);
+        rr_fit_.clear();
         upload();
     }
 
@@ -1362,6 +1460,7 @@ private:
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     int fuse_ = 1;   // 2D: fused multi-sweep tile smoothing and residual+restriction
+    std::vector<int> rr_fit_;
     double tol_ = 0.0;
     int predicted_ = 1;                // cycles the previous solve needed
     int periodic_mode_ = 1;
@@ -1586,7 +1685,7 @@ private:
             const int ostep = kTileX - 2 * k;
             const unsigned nb = (unsigned)(((L.n[0] + kTileRows - 1) / kTileRows) * ((L.n[1] + ostep - 1) / ostep));
             double* a = (k == count) ? acc : nullptr;
-#define TILE(CMP, PRO, ACC) hipLaunchKernelGGL((k_smooth_tile<T, CMP, PRO, ACC>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), ncls, x, b, buf, two3, k, C, tr, xc, a)
+#define TILE(CMP, PRO, ACC) hipLaunchKernelGGL((k_smooth_tile<T, CMP, PRO, ACC>), dim3(nb), dim3(kTileThreads), 0, stream_, L, table<T>(l, true), ncls, x, b, buf, two3, k, C, tr, xc, a)
             if (a) {
                 if (xc) { if (cmp) TILE(true, true, true); else TILE(false, true, true); }
                 else { if (cmp) TILE(true, false, true); else TILE(false, false, true); }
@@ -1601,6 +1700,28 @@ private:
             count -= k;
         }
         return accd;
+    }
+
+    // whether every coarse tile of level l+1 has its fine support within the
+    // tiled residual + restriction kernel's LDS tile (checked once per level)
+    bool rr_tile_fits(int l)
+    {
+        if ((int)rr_fit_.size() != (int)host_.size()) rr_fit_.assign(host_.size(), -1);
+        if (rr_fit_[l] >= 0) return rr_fit_[l] == 1;
+        const HostLevel& C = host_[l + 1];
+        bool ok = true;
+        const int lim[2] = {kRRFineRows, kRRFineCols}, step[2] = {kRRRows, kRRCols};
+        for (int d = 0; d < 2 && ok; ++d) {
+            const AxisTransfer& t = tr_host_[l][d];
+            for (int lo = 0; lo < C.n[d] && ok; lo += step[d]) {
+                const int hi = std::min(lo + step[d], C.n[d]) - 1;
+                for (int c = lo; c < hi; ++c)
+                    if (t.rf[c + 1] < t.rf[c] || t.rf[c + 1] + t.rn[c + 1] < t.rf[c] + t.rn[c]) ok = false;
+                if (t.rf[hi] + t.rn[hi] - t.rf[lo] > lim[d]) ok = false;
+            }
+        }
+        rr_fit_[l] = ok ? 1 : 0;
+        return ok;
     }
 
     template <typename T>
@@ -1634,7 +1755,14 @@ private:
                 const DevLevel F = dl(l), C = dl(l + 1);
                 const T fs = (l + 1 < Lc && pre_ > 0) ? two3 : (T)0;
                 if (host_[l].D == 2 && fuse_) {
-                    if (host_[l].compact)
+                    if (rr_tile_fits(l)) {
+                        const unsigned nbt = (unsigned)(((C.n[0] + kRRRows - 1) / kRRRows) * ((C.n[1] + kRRCols - 1) / kRRCols));
+                        const int ncls = (int)host_[l].dinv.size();
+                        if (host_[l].compact)
+                            hipLaunchKernelGGL((k_resid_restrict_tile<T, true>), dim3(nbt), dim3(kTileThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), ncls, table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
+                        else
+                            hipLaunchKernelGGL((k_resid_restrict_tile<T, false>), dim3(nbt), dim3(kTileThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), ncls, table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
+                    } else if (host_[l].compact)
                         hipLaunchKernelGGL((k_resid_restrict<T, true>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
                     else
                         hipLaunchKernelGGL((k_resid_restrict<T, false>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);

@@ -354,6 +354,14 @@ public:
 
     void sort_buffers()
     {
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            Species& sp = sp_[q];
+            const size_t bytes = (size_t)std::max<int64_t>(1, sp.sp.capacity) * esz_;
+            for (int d = 0; d < cfg_.grid.dim; ++d) {
+                if (!sp.pos2[d]) S_TRY(hipMalloc(&sp.pos2[d], bytes));
+                if (!sp.vel2[d]) S_TRY(hipMalloc(&sp.vel2[d], bytes));
+            }
+        }
         if (keys_) return;
         S_TRY(hipMalloc(&keys_, (size_t)max_cap_ * 4 + 4));
         S_TRY(hipMalloc(&skeys_, (size_t)max_cap_ * 4 + 4));
@@ -865,6 +873,69 @@ public:
         for (int q = 0; q < cfg_.nspecies; ++q) n += sp_[q].n;
         return n;
     }
+    // Autotuner: `warmup` tolerance-controlled steps calibrate the fixed
+    // V-cycle count (the most any solve needed, plus one); then for each sort
+    // bin shift 0..2 a timed sort and `probe` timed steps give t_sort and the
+    // push time t_push(j) j steps after a sort, and the interval K in
+    // {5, 10, 20, 50, 100, 200} minimising (t_sort + sum_{j<K} t_push(j)) / K
+    // is chosen (t_push beyond the probe window extrapolated linearly from its
+    // second half).  Returns the choice; the state has advanced.
+    void tune(int warmup, int probe, psum_batch_tuning* out)
+    {
+        if (warmup < 1 || probe < 2) throw std::invalid_argument("tune: warmup >= 1, probe >= 2");
+        static const int kCand[] = {5, 10, 20, 50, 100, 200};
+        reset_cycles_seen();
+        run_steps(warmup, nullptr, true, false);
+        const int cycles = max_cycles_seen() + 1;
+        int best_shift = 0, best_k = 0;
+        double best_cost = 1e300, best_sort = 0.0, best_push = 0.0;
+        for (int shift = 0; shift <= 2; ++shift) {
+            set_sort_shift(shift);
+            const float t_sort = timed_sort();
+            std::vector<double> tp(probe);
+            for (int j = 0; j < probe; ++j) {
+                set_timing(true);
+                step_body(false);
+                S_TRY(hipStreamSynchronize(st_));
+                double ms = 0.0;
+                flush_timing(&ms);
+                tp[j] = ms;
+            }
+            set_timing(false);
+            const int h0 = probe / 2;
+            double sx = 0, sy = 0, sxx = 0, sxy = 0;
+            for (int j = h0; j < probe; ++j) {
+                sx += j; sy += tp[j]; sxx += (double)j * j; sxy += j * tp[j];
+            }
+            const double m = probe - h0;
+            const double den = m * sxx - sx * sx;
+            const double slope = den > 0 ? std::max(0.0, (m * sxy - sx * sy) / den) : 0.0;
+            const double icpt = (sy - slope * sx) / m;
+            for (int K : kCand) {
+                double sum = t_sort;
+                for (int j = 0; j < K; ++j) sum += j < probe ? tp[j] : icpt + slope * j;
+                const double cost = sum / K;
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_shift = shift;
+                    best_k = K;
+                    best_sort = t_sort;
+                    best_push = tp[0];
+                }
+            }
+        }
+        set_sort_shift(best_shift);
+        if (out) {
+            out->fixed_cycles = cycles;
+            out->sort_every = best_k;
+            out->sort_shift = best_shift;
+            out->pad_ = 0;
+            out->sort_ms = best_sort;
+            out->push_ms = best_push;
+            out->push_sort_cost_ms = best_cost;
+        }
+    }
+
     // HIP-event time of one sort pass (all species), ms
     float timed_sort()
     {
@@ -1093,61 +1164,11 @@ public:
     // extrapolated linearly from its second half.
     void tune(int warmup, int probe, psum_batch_tuning* out)
     {
-        if (warmup < 1 || probe < 2) throw std::invalid_argument("psum_batch_tune: warmup >= 1, probe >= 2");
-        static const int kCand[] = {5, 10, 20, 50, 100, 200};
         for (int i = 0; i < size(); ++i) {
-            Stepper& st = at(i);
-            st.reset_cycles_seen();
-            st.run_steps(warmup, nullptr, true, false);
-            const int cycles = st.max_cycles_seen() + 1;
-            int best_shift = 0, best_k = 0;
-            double best_cost = 1e300, best_sort = 0.0, best_push = 0.0;
-            for (int shift = 0; shift <= 2; ++shift) {
-                st.set_sort_shift(shift);
-                const float t_sort = st.timed_sort();
-                std::vector<double> tp(probe);
-                for (int j = 0; j < probe; ++j) {
-                    st.set_timing(true);
-                    st.step_body(false);
-                    S_TRY(hipStreamSynchronize(st.stream()));
-                    double ms = 0.0;
-                    st.flush_timing(&ms);
-                    tp[j] = ms;
-                }
-                st.set_timing(false);
-                // linear extrapolation from the second half of the window
-                const int h0 = probe / 2;
-                double sx = 0, sy = 0, sxx = 0, sxy = 0;
-                for (int j = h0; j < probe; ++j) {
-                    sx += j; sy += tp[j]; sxx += (double)j * j; sxy += j * tp[j];
-                }
-                const double m = probe - h0;
-                const double den = m * sxx - sx * sx;
-                const double slope = den > 0 ? std::max(0.0, (m * sxy - sx * sy) / den) : 0.0;
-                const double icpt = (sy - slope * sx) / m;
-                for (int K : kCand) {
-                    double sum = t_sort;
-                    for (int j = 0; j < K; ++j) sum += j < probe ? tp[j] : icpt + slope * j;
-                    const double cost = sum / K;
-                    if (cost < best_cost) {
-                        best_cost = cost;
-                        best_shift = shift;
-                        best_k = K;
-                        best_sort = t_sort;
-                        best_push = tp[0];
-                    }
-                }
-            }
-            configure(i, cycles, best_k, best_shift);
-            if (out) {
-                out[i].fixed_cycles = cycles;
-                out[i].sort_every = best_k;
-                out[i].sort_shift = best_shift;
-                out[i].pad_ = 0;
-                out[i].sort_ms = best_sort;
-                out[i].push_ms = best_push;
-                out[i].push_sort_cost_ms = best_cost;
-            }
+            psum_batch_tuning t{};
+            at(i).tune(warmup, probe, &t);
+            configure(i, t.fixed_cycles, t.sort_every, t.sort_shift);
+            if (out) out[i] = t;
         }
     }
 
@@ -1361,6 +1382,12 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
 }
 
 void psum_stepper_destroy(void* h) { delete static_cast<Stepper*>(h); }
+
+psum_status psum_stepper_tune(void* h, int warmup, int probe, psum_batch_tuning* out)
+{
+    if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->tune(warmup, probe, out); });
+}
 
 psum_status psum_stepper_set_schedule(void* h, int fixed_cycles, int sort_every, int sort_shift)
 {

@@ -422,6 +422,15 @@ class Stepper:
         for i, sp in enumerate(species):
             self.load(i, sp["pos"], sp["vel"])
 
+    def tune(self, warmup=20, probe=10):
+        """Autotune (psum_stepper_tune): dict with the calibrated fixed cycle
+        count and the chosen sort interval and bin shift (the shift is
+        applied)."""
+        from ._lib import PsumBatchTuning
+        t = PsumBatchTuning()
+        check(lib().psum_stepper_tune(self.h, warmup, probe, C.byref(t)), "psum_stepper_tune")
+        return {f: getattr(t, f) for f, _ in PsumBatchTuning._fields_ if f != "pad_"}
+
     def set_schedule(self, fixed_cycles=0, sort_every=0, sort_shift=0):
         """fixed_cycles > 0: that many V-cycles per solve, no tolerance check;
         sort every `sort_every` steps on 2^sort_shift-cell bins."""

@@ -68,17 +68,27 @@ def two_stream_2d(n=1024, ppc=100, drift=5.0, noise=1e-3, seed=2, dtype=np.float
                 name=f"two_stream2d_{n}x{n}_ppc{ppc}")
 
 
-def bounded_3d(cells=256, ppc=32, mass_ratio=1836.0, te_ti=10.0, seed=3, dtype=np.float32):
+def bounded_3d(cells=256, ppc=32, mass_ratio=1836.0, te_ti=10.0, seed=3, dtype=np.float32,
+               device=None):
     """3D bounded box: cells^3 cells, phi = 0 on all faces, absorbing walls,
     ppc electrons + ppc ions per cell, uniform, Maxwellian (Te/Ti = te_ti),
-    ion mass mass_ratio, dt = 0.1."""
+    ion mass mass_ratio, dt = 0.1.  device="cuda" draws the particles on
+    the GPU (torch generator) and returns CUDA tensors."""
     rng = np.random.default_rng(seed)
     N = cells ** 3 * ppc
     L = float(cells)
     species = []
-    for q, m, vth in ((-1.0, 1.0, 1.0), (1.0, mass_ratio, np.sqrt(1.0 / te_ti / mass_ratio))):
-        pos = [(rng.random(N) * L).astype(dtype) for _ in range(3)]
-        vel = [(rng.standard_normal(N) * vth).astype(dtype) for _ in range(3)]
+    for si, (q, m, vth) in enumerate(((-1.0, 1.0, 1.0),
+                                      (1.0, mass_ratio, np.sqrt(1.0 / te_ti / mass_ratio)))):
+        if device is None:
+            pos = [(rng.random(N) * L).astype(dtype) for _ in range(3)]
+            vel = [(rng.standard_normal(N) * vth).astype(dtype) for _ in range(3)]
+        else:
+            import torch
+            tdt = torch.float32 if dtype == np.float32 else torch.float64
+            gen = torch.Generator(device=device).manual_seed(seed * 10 + si)
+            pos = [torch.rand(N, generator=gen, device=device, dtype=tdt) * L for _ in range(3)]
+            vel = [torch.randn(N, generator=gen, device=device, dtype=tdt) * vth for _ in range(3)]
         species.append(dict(q=q, m=m, w=1.0 / ppc, pos=pos, vel=vel))
     g = wall_grid([cells] * 3)
     shp = (cells + 1,) * 3
