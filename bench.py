@@ -291,7 +291,7 @@ def main():
     ap.add_argument("--post", type=int, default=3)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--extrapolate", action="store_true")
-    ap.add_argument("--sort-every", type=int, default=50)
+    ap.add_argument("--sort-every", type=int, default=25)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")

@@ -103,6 +103,21 @@ __device__ __forceinline__ bool base_cell(const DevGrid& g, const double* x, int
     return true;
 }
 
+// |c| > 1e-10 evaluated in FP64 (field_interp.hpp:70, :100), for c of type
+// T.  For float c the same test is c > t32 with t32 the largest float not
+// above 1e-10 (1e-10 is not a float, so (double)c > 1e-10 <=> c >= the next
+// float up <=> c > t32): one float compare instead of a widening.
+template <typename T>
+__device__ __forceinline__ bool coef_kept(T c)
+{
+    if constexpr (sizeof(T) == 4) {
+        constexpr float t32 = 9.999999439624929e-11f;   // largest float <= 1e-10
+        return fabsf(c) > t32;
+    } else {
+        return fabs(c) > 1e-10;
+    }
+}
+
 // Round-to-nearest-even of t to int64 for |t| < 2^51 (llrint in the default
 // rounding mode): t + 1.5*2^52 lands in [2^52, 2^53), whose ulp is 1, so the
 // addition rounds t to an integer and the difference of the bit patterns is

@@ -372,7 +372,7 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
 #pragma unroll
             for (int i = 0; i < (1 << D); ++i) {
                 const T gc = cic_grad_coef<T, D>(W, G, i, dd);
-                if (fabs((double)gc) > 1e-10) r = r + ph[i] * gc;
+                if (coef_kept<T>(gc)) r = r + ph[i] * gc;
             }
             grad[dd] = r;
         }
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
                 const T cf = cic_coef<T, D>(W, i);
-                if (!(fabs((double)cf) > 1e-10)) continue;
+                if (!coef_kept<T>(cf)) continue;
                 // (w*cf)*2^S: scaling by a power of two is exact, so one
                 // product with w*2^S rounds identically
                 run[i] += round_to_ll((double)cf * wsc);
