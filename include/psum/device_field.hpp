@@ -1,0 +1,171 @@
+// device_field<D, Loc, Scalar>: a node- or cell-centred field on a
+// simple_grid in device memory (src/field/device_field.hpp:120-240), with
+// the device accessor that interp / interp_diff / add_back take
+// (device_field.hpp:245-321, field_interp.hpp:63-113).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cmath>
+#include <vector>
+
+#include "../psum_cic.hpp"
+#include "device_vector.hpp"
+#include "grid.hpp"
+
+namespace psum {
+namespace field {
+
+template <int D, var_loc Loc, typename Scalar>
+struct device_field_acc {
+    static constexpr int Dimension = D;
+    static constexpr var_loc Location = Loc;
+    using Value = Scalar;
+    Scalar* content = nullptr;
+    psum_hip::DevGrid grid{};
+    __device__ Scalar& operator()(size_t i) const { return content[i]; }
+};
+
+template <int D, var_loc Loc, typename Scalar, int QuantitySize = 1>
+class device_field {
+    static_assert(QuantitySize == 1, "device_field: vector-valued fields are not part of this build");
+
+public:
+    using acc_type = device_field_acc<D, Loc, Scalar>;
+    using Grid = simple_grid<D>;
+
+    device_field(hipStream_t s, const Grid& g) : grid_(g), stream_(s)
+    {
+        const size_t n = g.contentSize(Loc);
+        hip_check(hipMalloc(&data_, n * sizeof(Scalar)), "device_field");
+        setZero();
+    }
+    explicit device_field(const Grid& g) : device_field(nullptr, g) {}
+    device_field(const device_field&) = delete;
+    device_field& operator=(const device_field&) = delete;
+    ~device_field()
+    {
+        if (data_) (void)hipFree(data_);
+    }
+
+    acc_type get_access(handler&) const { return acc(); }
+    acc_type acc() const
+    {
+        acc_type a;
+        a.content = data_;
+        a.grid = psum_hip::to_dev(grid_.to_psum());
+        return a;
+    }
+    size_t size() const { return grid_.contentSize(Loc); }
+    Scalar* data() { return data_; }
+    const Scalar* data() const { return data_; }
+    const Grid& getGrid() const { return grid_; }
+    var_loc getLocation() const { return Loc; }
+    hipStream_t getQueue() const { return stream_; }
+
+    void setZero()
+    {
+        hip_check(hipMemsetAsync(data_, 0, size() * sizeof(Scalar), stream_), "setZero");
+        hip_check(hipStreamSynchronize(stream_), "setZero");
+    }
+    void copy(const std::vector<Scalar>& host)
+    {
+        if (host.size() != size()) throw std::invalid_argument("device_field::copy: size mismatch");
+        hip_check(hipMemcpyAsync(data_, host.data(), size() * sizeof(Scalar), hipMemcpyHostToDevice, stream_), "copy");
+        hip_check(hipStreamSynchronize(stream_), "copy");
+    }
+    std::vector<Scalar> to_host() const
+    {
+        std::vector<Scalar> h(size());
+        hip_check(hipMemcpyAsync(h.data(), data_, size() * sizeof(Scalar), hipMemcpyDeviceToHost, stream_), "to_host");
+        hip_check(hipStreamSynchronize(stream_), "to_host");
+        return h;
+    }
+
+private:
+    Grid grid_;
+    hipStream_t stream_ = nullptr;
+    Scalar* data_ = nullptr;
+};
+
+template <typename Scalar, int QS = 1> using node_field1D = device_field<1, var_loc::nodeCentered, Scalar, QS>;
+template <typename Scalar, int QS = 1> using node_field2D = device_field<2, var_loc::nodeCentered, Scalar, QS>;
+template <typename Scalar, int QS = 1> using node_field3D = device_field<3, var_loc::nodeCentered, Scalar, QS>;
+template <typename Scalar, int QS = 1> using cell_field1D = device_field<1, var_loc::cellCentered, Scalar, QS>;
+template <typename Scalar, int QS = 1> using cell_field2D = device_field<2, var_loc::cellCentered, Scalar, QS>;
+template <typename Scalar, int QS = 1> using cell_field3D = device_field<3, var_loc::cellCentered, Scalar, QS>;
+
+namespace detail {
+// CIC support of pos on a node field: indices and per-dimension factors
+template <int D, typename Pos>
+__device__ bool support(const psum_hip::DevGrid& g, const Pos& pos, int* cw, double (*W)[2], double (*G)[2])
+{
+    double x[D], s[D];
+    for (int d = 0; d < D; ++d) x[d] = (double)pos[d];
+    if (!psum_hip::base_cell<D>(g, x, cw, s)) return false;
+    for (int d = 0; d < D; ++d) {
+        W[d][0] = 1.0 - s[d];
+        W[d][1] = s[d];
+        G[d][0] = -1.0 * g.delta_r[d];
+        G[d][1] = 1.0 * g.delta_r[d];
+    }
+    return true;
+}
+}  // namespace detail
+
+// phi(x): sum_i phi[idx_i] c_i, skipping |c_i| <= 1e-10 (field_interp.hpp:77-91)
+template <typename Pos, int D, typename Scalar>
+__device__ Scalar interp(const Pos& pos, const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
+{
+    int cw[D];
+    double W[D][2], G[D][2];
+    Scalar r = (Scalar)0;
+    if (!detail::support<D>(acc.grid, pos, cw, W, G)) return r;
+    for (int i = 0; i < (1 << D); ++i) {
+        const double c = psum_hip::cic_coef<double, D>(W, i);
+        if (fabs(c) > 1e-10) r = r + acc.content[psum_hip::support_index<D>(acc.grid, cw, i)] * c;
+    }
+    return r;
+}
+
+// grad phi(x) of the multilinear interpolant (field_interp.hpp:93-113)
+template <typename Pos, int D, typename Scalar>
+__device__ std::array<Scalar, D> interp_diff(const Pos& pos,
+                                             const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
+{
+    int cw[D];
+    double W[D][2], G[D][2];
+    std::array<Scalar, D> g{};
+    if (!detail::support<D>(acc.grid, pos, cw, W, G)) return g;
+    Scalar ph[1 << D];
+    for (int i = 0; i < (1 << D); ++i) ph[i] = acc.content[psum_hip::support_index<D>(acc.grid, cw, i)];
+    for (int d = 0; d < D; ++d) {
+        Scalar r = (Scalar)0;
+        for (int i = 0; i < (1 << D); ++i) {
+            const double c = psum_hip::cic_grad_coef<double, D>(W, G, i, d);
+            if (fabs(c) > 1e-10) r = r + ph[i] * c;
+        }
+        g[d] = r;
+    }
+    return g;
+}
+
+// rho[idx_i] += w c_i with FP64 atomics, skipping |c_i| <= 1e-10
+// (field_interp.hpp:63-75, atomic_add.hpp:34-48).  Summation order is that of
+// the atomics; psum_hip_deposit_cic is the deterministic fixed-point form.
+template <typename Pos, int D, typename Scalar>
+__device__ void add_back(const Pos& pos, const Scalar& w,
+                         const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
+{
+    int cw[D];
+    double W[D][2], G[D][2];
+    if (!detail::support<D>(acc.grid, pos, cw, W, G)) return;
+    for (int i = 0; i < (1 << D); ++i) {
+        const double c = psum_hip::cic_coef<double, D>(W, i);
+        if (fabs(c) > 1e-10) atomicAdd(&acc.content[psum_hip::support_index<D>(acc.grid, cw, i)], w * c);
+    }
+}
+
+}  // namespace field
+}  // namespace psum

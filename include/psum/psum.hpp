@@ -1,0 +1,45 @@
+// The reference's particle / field API on HIP (names and argument meaning of
+// PSuM's psum.hpp prelude), compiled by hipcc.  The user step loop of the
+// reference ports by replacing sycl::handler with psum::handler and
+// Eigen::Vector<double, D> with psum::vec<D>; the hot loops can also go
+// through the fused psum_hip_* kernels (psum_hip.h) via to_soa / from_soa.
+#pragma once
+
+#include "../psum_hip.h"
+#include "boundary.hpp"
+#include "device_field.hpp"
+#include "device_vector.hpp"
+#include "grid.hpp"
+#include "particle_group.hpp"
+#include "tag.hpp"
+
+namespace psum {
+namespace prelude {
+using psum::handler;
+using psum::vec;
+using psum::field::add_back;
+using psum::field::cell_field1D;
+using psum::field::cell_field2D;
+using psum::field::cell_field3D;
+using psum::field::device_field;
+using psum::field::grid1D;
+using psum::field::grid2D;
+using psum::field::grid3D;
+using psum::field::interp;
+using psum::field::interp_diff;
+using psum::field::node_field1D;
+using psum::field::node_field2D;
+using psum::field::node_field3D;
+using psum::field::simple_grid;
+using psum::field::var_loc;
+using psum::particle_boundary::box_absorber;
+using psum::particle_boundary::periodic_wrap;
+using psum::particle_container::device_vector;
+using psum::particle_container::particle_group;
+using psum::particle_container::pos_x_nan_is_invalid;
+using psum::tag::get;
+using psum::tag::tag_bind;
+using psum::tag::tagged_struct;
+namespace property = psum::tag::property;
+}  // namespace prelude
+}  // namespace psum

@@ -45,6 +45,26 @@ def _compile(src):
     return obj
 
 
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "api_step.cpp")]
+
+
+def build_cpp_tests(verbose=True):
+    """Compile the C++ API programs the GPU tests run (tests/cpp/*.cpp ->
+    executables beside them, linked against libpsum_hip.so)."""
+    for src in CPP_TESTS:
+        exe = src[:-4]
+        cmd = [_hipcc(), "-O2", "-std=c++20", f"--offload-arch={ARCH}", "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-I", os.path.join(ROOT, "include"), "-o", exe + ".tmp", src,
+               "-L", HERE, "-lpsum_hip", "-Wl,-rpath,$ORIGIN/../../psum_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+        if verbose:
+            print(f"built {exe}")
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
@@ -58,6 +78,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_tests(verbose)
     return LIB
 
 

@@ -1,0 +1,185 @@
+// One step of the reference user loop written against the C++ particle /
+// field API (psum/psum.hpp), compared on the GPU with the fused C ABI
+// kernels on the same particles:
+//   deposit   add_back in for_each (FP64 atomics)  vs  psum_hip_deposit_cic (fixed point)
+//   push      interp_diff + kick + drift + wrap     vs  psum_hip_gather_push (bit-exact)
+//   compress  invalidated particles dropped, survivors in order
+// Prints one line per check and "ALL OK" at the end; exit status 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "psum/psum.hpp"
+
+using namespace psum::prelude;
+using property::position;
+using property::velocity;
+using Particle = tagged_struct<tag_bind<position, vec<2>>, tag_bind<velocity, vec<2>>>;
+using Group = particle_group<Particle, pos_x_nan_is_invalid>;
+
+static int fails = 0;
+static void report(const char* what, bool ok, double v)
+{
+    std::printf("%-40s %s (%.3e)\n", what, ok ? "ok" : "FAIL", v);
+    if (!ok) ++fails;
+}
+
+int main()
+{
+    const int n = 64, ppc = 16;
+    const double L = n;
+    // the reference periodic node convention: nodes at dx/2 + i, n nodes, period n
+    grid2D grid({0.5, 0.5}, {L - 0.5, L - 0.5}, {n - 1, n - 1});
+    const psum_grid pg = grid.to_psum({1, 1});
+    const size_t N = (size_t)n * n * ppc;
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> U(0.0, L);
+    std::normal_distribution<double> G(0.0, 1.0);
+    std::vector<Particle> host(N);
+    for (auto& p : host) {
+        get<position>(p).x() = U(rng);
+        get<position>(p).y() = U(rng);
+        get<velocity>(p).x() = G(rng);
+        get<velocity>(p).y() = G(rng);
+    }
+    hipStream_t st;
+    psum::hip_check(hipStreamCreate(&st), "stream");
+    Group group(st, N);
+    group.insert(host);
+    report("insert size", group.size() == N, (double)group.size());
+
+    // ---- deposit: add_back vs the fixed-point kernel
+    node_field2D<double> rho(st, grid);
+    const double w = -1.0 / ppc;
+    group.for_each([&](psum::handler& h) {
+        auto acc = rho.get_access(h);
+        return [=](Particle& p) { add_back(get<position>(p), w, acc); };
+    });
+    std::vector<double> rho_api = rho.to_host();
+
+    double *px, *py, *vx, *vy;
+    const size_t bytes = N * sizeof(double);
+    psum::hip_check(hipMalloc(&px, bytes), "m");
+    psum::hip_check(hipMalloc(&py, bytes), "m");
+    psum::hip_check(hipMalloc(&vx, bytes), "m");
+    psum::hip_check(hipMalloc(&vy, bytes), "m");
+    double* pos[2] = {px, py};
+    double* vel[2] = {vx, vy};
+    group.to_soa<2, double>(pos, vel);
+    const size_t nn = grid.contentSize(var_loc::nodeCentered);
+    int64_t* acc = nullptr;
+    psum::hip_check(hipMalloc(&acc, nn * 8), "m");
+    psum::hip_check(hipMemset(acc, 0, nn * 8), "m");
+    int e = 0;
+    std::frexp(std::fabs(w), &e);
+    const double scale = std::ldexp(1.0, 40 - e);
+    void* vpos[3] = {px, py, nullptr};
+    void* vvel[3] = {vx, vy, nullptr};
+    if (psum_hip_deposit_cic(&pg, PSUM_F64, vpos, (int64_t)N, w, scale, acc, nullptr, st) != PSUM_OK) {
+        std::printf("deposit failed: %s\n", psum_hip_last_error());
+        return 1;
+    }
+    std::vector<int64_t> acc_h(nn);
+    psum::hip_check(hipMemcpy(acc_h.data(), acc, nn * 8, hipMemcpyDeviceToHost), "m");
+    double err = 0, mx = 0;
+    for (size_t i = 0; i < nn; ++i) {
+        const double fx = (double)acc_h[i] / scale;
+        err = std::max(err, std::fabs(fx - rho_api[i]));
+        mx = std::max(mx, std::fabs(fx));
+    }
+    report("add_back vs fixed-point deposit", err <= 1e-12 * mx, err / mx);
+
+    // ---- push: interp_diff + kick + drift + wrap (TSI/main.cpp:59-75 in 2D)
+    node_field2D<double> phi(st, grid);
+    std::vector<double> phih(nn);
+    for (auto& v : phih) v = 1e-2 * G(rng);
+    phi.copy(phih);
+    const double qm = -1.0, dt = 0.1;
+    group.for_each([&](psum::handler& h) {
+        auto pa = phi.get_access(h);
+        return [=](Particle& p) {
+            const auto g = interp_diff(get<position>(p), pa);
+            for (int d = 0; d < 2; ++d) {
+                const double ed = -g[d];
+                get<velocity>(p)[d] = get<velocity>(p)[d] + (ed * qm) * dt;
+                get<position>(p)[d] = get<position>(p)[d] + get<velocity>(p)[d] * dt;
+            }
+            for (int d = 0; d < 2; ++d) {
+                double& x = get<position>(p)[d];
+                for (int r = 0; r < 2; ++r) {
+                    if (x < 0.0) x = x + L;
+                    else if (x >= L) x = x - L;
+                }
+            }
+        };
+    });
+    if (psum_hip_gather_push(&pg, PSUM_F64, vpos, vvel, (int64_t)N, phi.data(), qm, dt, 1.0, 0.0, 1.0,
+                             nullptr, nullptr, nullptr, st) != PSUM_OK) {
+        std::printf("push failed: %s\n", psum_hip_last_error());
+        return 1;
+    }
+    psum::hip_check(hipStreamSynchronize(st), "s");
+    std::vector<Particle> after = group.get_content().to_host();
+    std::vector<double> hx(N), hy(N), hvx(N), hvy(N);
+    psum::hip_check(hipMemcpy(hx.data(), px, bytes, hipMemcpyDeviceToHost), "m");
+    psum::hip_check(hipMemcpy(hy.data(), py, bytes, hipMemcpyDeviceToHost), "m");
+    psum::hip_check(hipMemcpy(hvx.data(), vx, bytes, hipMemcpyDeviceToHost), "m");
+    psum::hip_check(hipMemcpy(hvy.data(), vy, bytes, hipMemcpyDeviceToHost), "m");
+    size_t mism = 0;
+    for (size_t i = 0; i < N; ++i) {
+        const auto& p = after[i];
+        if (get<position>(p).x() != hx[i] || get<position>(p).y() != hy[i] ||
+            get<velocity>(p).x() != hvx[i] || get<velocity>(p).y() != hvy[i])
+            ++mism;
+    }
+    report("for_each push vs psum_hip_gather_push (bitwise)", mism == 0, (double)mism);
+
+    // ---- invalidate every 7th particle, then compress (stable)
+    group.for_each([&](psum::handler&) {
+        const Particle* base = group.get_content().data();
+        return [=](Particle& p) {
+            if (((&p - base) % 7) == 3) pos_x_nan_is_invalid<Particle>::make_invalid(p);
+        };
+    });
+    const size_t expect = N - (N + 3) / 7;
+    report("invalidated count", group.size() == expect, (double)group.size());
+    group.compress();
+    std::vector<Particle> kept = group.get_content().to_host();
+    bool order = kept.size() == expect;
+    for (size_t i = 0, k = 0; order && i < N; ++i) {
+        if (i % 7 == 3) continue;
+        order = get<position>(kept[k]).x() == get<position>(after[i]).x() &&
+                get<velocity>(kept[k]).y() == get<velocity>(after[i]).y();
+        ++k;
+    }
+    report("compress keeps survivors in order", order, (double)kept.size());
+
+    // ---- interp of a linear field reproduces it (test_simpleinterp.cpp linearity)
+    std::vector<double> lin(nn);
+    for (size_t i = 0; i < nn; ++i) {
+        const auto nd = grid.i2n(i);
+        const auto x = grid.nodePosition(nd);
+        lin[i] = 2.0 * x[0] - 3.0 * x[1] + 1.0;
+    }
+    phi.copy(lin);
+    double* vals = nullptr;
+    psum::hip_check(hipMalloc(&vals, kept.size() * sizeof(double)), "m");
+    group.for_each([&](psum::handler& h) {
+        auto pa = phi.get_access(h);
+        const Particle* base = group.get_content().data();
+        return [=](Particle& p) {
+            const auto& x = get<position>(p);
+            vals[&p - base] = (x[0] < 63.0 && x[1] < 63.0 && x[0] >= 0.5 && x[1] >= 0.5)
+                                  ? interp(x, pa) - (2.0 * x[0] - 3.0 * x[1] + 1.0) : 0.0;
+        };
+    });
+    std::vector<double> hv(kept.size());
+    psum::hip_check(hipMemcpy(hv.data(), vals, hv.size() * sizeof(double), hipMemcpyDeviceToHost), "m");
+    double lerr = 0;
+    for (double v : hv) lerr = std::max(lerr, std::fabs(v));
+    report("interp reproduces a linear field", lerr < 1e-10, lerr);
+
+    std::printf(fails ? "FAILED %d\n" : "ALL OK\n", fails);
+    return fails ? 1 : 0;
+}

@@ -69,3 +69,10 @@ def test_grid_struct_layout_matches_oracle():
     for (a, ta), (b, tb) in zip(_lib.PsumGrid._fields_, O.Grid._fields_):
         assert a == b and C.sizeof(ta) == C.sizeof(tb)
         assert getattr(_lib.PsumGrid, a).offset == getattr(O.Grid, b).offset
+
+
+def test_cpp_api_program_is_built():
+    """build() compiles the C++ API test program against the headers in
+    include/psum/ and the library (it runs only on a GPU)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "api_step")
+    assert os.path.exists(exe) and os.access(exe, os.X_OK)
