@@ -7,10 +7,12 @@
 
 #include "../psum_hip.h"
 #include "boundary.hpp"
+#include "boundary_creator.hpp"
 #include "device_field.hpp"
 #include "device_vector.hpp"
 #include "grid.hpp"
 #include "particle_group.hpp"
+#include "poisson_solver.hpp"
 #include "tag.hpp"
 
 namespace psum {
@@ -40,6 +42,13 @@ using psum::particle_container::pos_x_nan_is_invalid;
 using psum::tag::get;
 using psum::tag::tag_bind;
 using psum::tag::tagged_struct;
+using psum::field::boundary_direction_2d;
+using psum::field::boundary_direction_3d;
+using psum::field_solver::fixed_boundary_2d_list;
+using psum::field_solver::fixed_boundary_3d_list;
+using psum::field_solver::Poisson_solver_2d;
+using psum::field_solver::Poisson_solver_3d;
+namespace boundary_creator = psum::field_solver::boundary_creator;
 namespace property = psum::tag::property;
 }  // namespace prelude
 }  // namespace psum

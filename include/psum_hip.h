@@ -220,6 +220,15 @@ void psum_stepper_destroy(void* h);
  * 2^sort_shift-cell bins.  No reference counterpart (tuning knobs). */
 psum_status psum_stepper_set_schedule(void* h, int fixed_cycles, int sort_every, int sort_shift);
 
+/* FD Poisson matrix A = -div(eps grad) of the reference frontends
+ * (Poisson_solver_2d.hpp:182-410, Poisson_solver_3d.hpp:178-431, Cartesian,
+ * constant eps): identity rows at nodes with fixed[i] != 0 (NULL: none),
+ * cyclic neighbours over cells+1 nodes otherwise, as host COO.  With rows ==
+ * NULL only *nnz is returned; else up to cap entries are written. */
+psum_status psum_poisson_assemble(const psum_grid* g, double eps, const uint8_t* fixed, int64_t cap,
+                                  unsigned long long* rows, unsigned long long* cols, double* vals,
+                                  int64_t* nnz);
+
 /* ---- batched ensemble (no reference counterpart: the reference steps one
  * problem per process).  Independent instances, each configured like a
  * stepper, advance together: one step of all of them is captured into one

@@ -122,6 +122,7 @@ SIGNATURES = [
     ("psum_stepper_destroy", None, [P]),
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
     ("psum_stepper_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
+    ("psum_poisson_assemble", ST, [_GP, C.c_double, P, I64, P, P, P, C.POINTER(I64)]),
     ("psum_batch_create", P, [C.POINTER(PsumStepConfig), C.c_int]),
     ("psum_batch_load", ST, [P, C.c_int, C.c_int, P, P, I64]),
     ("psum_batch_configure", ST, [P, C.c_int, C.c_int, C.c_int, C.c_int]),

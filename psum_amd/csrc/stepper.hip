@@ -1383,6 +1383,28 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
 
 void psum_stepper_destroy(void* h) { delete static_cast<Stepper*>(h); }
 
+psum_status psum_poisson_assemble(const psum_grid* g, double eps, const uint8_t* fixed, int64_t cap,
+                                  unsigned long long* rows, unsigned long long* cols, double* vals,
+                                  int64_t* nnz)
+{
+    PSUM_ARG_CHECK(grid_ok(g) && g->dim >= 2 && nnz, "psum_poisson_assemble: bad argument");
+    return guarded([&]() {
+        const int64_t nn = grid_nodes(*g);
+        std::vector<uint8_t> fx(nn, 0);
+        if (fixed) std::memcpy(fx.data(), fixed, nn);
+        std::vector<unsigned long long> R, C;
+        std::vector<double> V;
+        assemble(*g, eps, fx, R, C, V);
+        *nnz = (int64_t)R.size();
+        if (rows && cols && vals) {
+            if (cap < (int64_t)R.size()) throw std::invalid_argument("psum_poisson_assemble: capacity too small");
+            std::memcpy(rows, R.data(), R.size() * sizeof(unsigned long long));
+            std::memcpy(cols, C.data(), C.size() * sizeof(unsigned long long));
+            std::memcpy(vals, V.data(), V.size() * sizeof(double));
+        }
+    });
+}
+
 psum_status psum_stepper_tune(void* h, int warmup, int probe, psum_batch_tuning* out)
 {
     if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }

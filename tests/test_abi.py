@@ -76,3 +76,46 @@ def test_cpp_api_program_is_built():
     include/psum/ and the library (it runs only on a GPU)."""
     exe = os.path.join(ROOT, "tests", "cpp", "api_step")
     assert os.path.exists(exe) and os.access(exe, os.X_OK)
+
+
+@pytest.mark.parametrize("case", ["periodic2d", "walls2d", "walls3d"])
+def test_poisson_assembly_matches_oracle(case):
+    """psum_poisson_assemble (host side, no GPU) builds the same matrix as the
+    oracle's restatement of the reference frontends."""
+    import numpy as np
+    from psum_amd import _lib
+    import oracle_lib as O
+    if case == "periodic2d":
+        og = O.periodic_grid([12, 9])
+        fixed = np.zeros(O.num_nodes(og), np.uint8)
+        fixed[0] = 1
+    elif case == "walls2d":
+        og = O.wall_grid([10, 7])
+        shp = O.node_shape(og)
+        f = np.zeros(shp, np.uint8)
+        f[0, :] = f[-1, :] = f[:, 0] = f[:, -1] = 1
+        fixed = f.ravel()
+    else:
+        og = O.wall_grid([5, 4, 6])
+        shp = O.node_shape(og)
+        f = np.zeros(shp, np.uint8)
+        f[0], f[-1], f[:, 0], f[:, -1], f[:, :, 0], f[:, :, -1] = 1, 1, 1, 1, 1, 1
+        fixed = f.ravel()
+    r, c, v = O.poisson_coo(og, 1.5, fixed)
+    g = _lib.PsumGrid()
+    C.memmove(C.byref(g), C.byref(og), C.sizeof(g))
+    L = _lib.lib()
+    nnz = C.c_int64(0)
+    fx = np.ascontiguousarray(fixed, np.uint8)
+    assert L.psum_poisson_assemble(C.byref(g), 1.5, fx.ctypes.data, 0, None, None, None, C.byref(nnz)) == 0
+    rr = np.zeros(nnz.value, np.uint64)
+    cc = np.zeros(nnz.value, np.uint64)
+    vv = np.zeros(nnz.value, np.float64)
+    assert L.psum_poisson_assemble(C.byref(g), 1.5, fx.ctypes.data, nnz.value, rr.ctypes.data,
+                                   cc.ctypes.data, vv.ctypes.data, C.byref(nnz)) == 0
+    keep = vv != 0
+    a = sorted(zip(rr[keep].tolist(), cc[keep].tolist(), vv[keep].tolist()))
+    kr = np.asarray(v) != 0
+    b = sorted(zip(np.asarray(r)[kr].tolist(), np.asarray(c)[kr].tolist(), np.asarray(v)[kr].tolist()))
+    assert [x[:2] for x in a] == [x[:2] for x in b]
+    assert np.allclose([x[2] for x in a], [x[2] for x in b], rtol=1e-14, atol=0)
