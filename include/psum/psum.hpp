@@ -46,6 +46,10 @@ using psum::field::boundary_direction_2d;
 using psum::field::boundary_direction_3d;
 using psum::field_solver::fixed_boundary_2d_list;
 using psum::field_solver::fixed_boundary_3d_list;
+using psum::field_solver::mixed_boundary_2d;
+using psum::field_solver::mixed_boundary_2d_list;
+using psum::field_solver::mixed_boundary_3d;
+using psum::field_solver::mixed_boundary_3d_list;
 using psum::field_solver::Poisson_solver_2d;
 using psum::field_solver::Poisson_solver_3d;
 namespace boundary_creator = psum::field_solver::boundary_creator;

@@ -223,11 +223,53 @@ psum_status psum_stepper_set_schedule(void* h, int fixed_cycles, int sort_every,
 /* FD Poisson matrix A = -div(eps grad) of the reference frontends
  * (Poisson_solver_2d.hpp:182-410, Poisson_solver_3d.hpp:178-431, Cartesian,
  * constant eps): identity rows at nodes with fixed[i] != 0 (NULL: none),
- * cyclic neighbours over cells+1 nodes otherwise, as host COO.  With rows ==
- * NULL only *nnz is returned; else up to cap entries are written. */
+ * cyclic neighbours over cells+1 nodes otherwise, as host COO in row-major
+ * order (set_matrix takes any order).  With rows == NULL only *nnz is
+ * returned; else up to cap entries are written. */
 psum_status psum_poisson_assemble(const psum_grid* g, double eps, const uint8_t* fixed, int64_t cap,
                                   unsigned long long* rows, unsigned long long* cols, double* vals,
                                   int64_t* nnz);
+
+/* Mixed (Robin/Neumann) boundary node (mixed_boundary.hpp:217-221):
+ * value_coef * phi + grad_coef * dphi/dn = value at node `element` on side
+ * `direction` -- 2D: N=0 S=1 (axis 1, +/-), E=2 W=3 (axis 0, +/-); 3D:
+ * Xpos=0 Xneg=1 Ypos=2 Yneg=3 Zpos=4 Zneg=5 (foundation.hpp:43-51). */
+typedef struct psum_mixed_node {
+    unsigned long long element;
+    int32_t direction;
+    int32_t pad_;
+    double value_coef;
+    double grad_coef;
+} psum_mixed_node;
+
+/* permittivity at a face midpoint; pos holds dim coordinates */
+typedef double (*psum_eps_fn)(const double* pos, void* user);
+
+typedef struct psum_poisson_spec {
+    int32_t cylindrical;            /* 2D only: axis 0 = z, axis 1 = r */
+    int32_t pad_;
+    double eps;                     /* used when eps_fn is NULL */
+    psum_eps_fn eps_fn;
+    void* eps_user;
+    const uint8_t* fixed;           /* node mask or NULL */
+    const psum_mixed_node* mixed;   /* n_mixed nodes or NULL */
+    int64_t n_mixed;
+} psum_poisson_spec;
+
+/* General frontend assembly (Poisson_solver_2d.hpp:182-410 with the
+ * ghost_element_manager elimination, ghost_element_manager.hpp:56-117):
+ * each mixed node adds a ghost value beyond its side, defined by the
+ * central-difference boundary equation and eliminated from the system:
+ * A' = A - B D^-1 C (rows/cols/vals, row-major) and the
+ * right-hand-side map M = -B D^-1 (mrows/mcols/mvals, row-major, column =
+ * index into spec->mixed).  A solve then uses b = source, b[r] +=
+ * sum_m M(r,m) value(m) (set_source_addback), b[fixed] = value.  Mixed
+ * nodes at fixed elements are ignored (fixed covers mixed).  NULL output
+ * pointers return the counts only. */
+psum_status psum_poisson_assemble_ex(const psum_grid* g, const psum_poisson_spec* spec, int64_t cap,
+                                     unsigned long long* rows, unsigned long long* cols, double* vals,
+                                     int64_t* nnz, int64_t mcap, unsigned long long* mrows,
+                                     unsigned long long* mcols, double* mvals, int64_t* mnnz);
 
 /* ---- batched ensemble (no reference counterpart: the reference steps one
  * problem per process).  Independent instances, each configured like a

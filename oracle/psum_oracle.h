@@ -28,6 +28,13 @@
  *   orc_compact_*          numpy boolean mask
  *   orc_poisson_coo        closed-form 5/7-point coefficients; DFT
  *                          eigenvalues of the periodic Laplacian
+ *   orc_poisson_assemble_ex  equals orc_poisson_coo without mixed nodes;
+ *                          exact on quadratics (Cartesian Neumann/Robin
+ *                          ghost rows, A phi = s + M g), exact on r^2 in the
+ *                          cylindrical form; the reference's closed forms
+ *                          (test/field_solver/test_psolver_2d.cpp:17-76
+ *                          axis value of c1 r^c2 with Neumann sides,
+ *                          :147-176 charged sphere) converge at order 2
  *   orc_mg_*               exact discrete periodic solution via FFT,
  *                          manufactured sin*sin solution, >=1e5 residual
  *                          reduction in 10 V(2,1) cycles on 256^2
@@ -120,6 +127,44 @@ int64_t orc_compact_f32(int dim, float* const* pos, float* const* vel, int64_t n
  * (Poisson_solver_2d.hpp:387-404).  Call with rows==NULL to get nnz. */
 int64_t orc_poisson_coo(const orc_grid* g, double eps, const uint8_t* fixed,
                         uint64_t* rows, uint64_t* cols, double* vals);
+
+/* Mixed (Robin/Neumann) boundary node of the reference frontends
+ * (mixed_boundary.hpp:217-221): value_coef * phi + grad_coef * dphi/dn =
+ * value at node `element`, on the side `direction` (2D: N=0 S=1 E=2 W=3,
+ * N/S along axis 1, E/W along axis 0; 3D: Xpos=0 Xneg=1 Ypos=2 Yneg=3
+ * Zpos=4 Zneg=5; foundation.hpp:43-51). */
+typedef struct {
+    uint64_t element;
+    int32_t direction;
+    int32_t pad_;
+    double value_coef;
+    double grad_coef;
+} orc_mixed_node;
+
+/* permittivity at a face midpoint (pos has dim entries) */
+typedef double (*orc_eps_fn)(const double* pos, void* user);
+
+/* General FD assembly (Poisson_solver_2d.hpp:182-349, Poisson_solver_3d.hpp:
+ * 211-350, ghost_element_manager.hpp:56-117): A' = A - B D^-1 C in the
+ * column-major walk, M = -B D^-1 row-major (column = mixed node index).
+ * Mixed nodes at fixed elements are dropped; cylindrical (2D only) takes
+ * axis 1 as r.  eps_fn NULL: constant eps.  Returns 0, -1 for a bad input or
+ * a ghost system that is not permutation-like, -2 for a small capacity.
+ * Call with rows/mrows NULL for the counts. */
+int orc_poisson_assemble_ex(const orc_grid* g, int cylindrical, double eps, orc_eps_fn eps_fn,
+                            void* user, const uint8_t* fixed, const orc_mixed_node* mixed,
+                            int64_t nmixed, int64_t cap, uint64_t* rows, uint64_t* cols,
+                            double* vals, int64_t* nnz, int64_t mcap, uint64_t* mrows,
+                            uint64_t* mcols, double* mvals, int64_t* mnnz);
+
+/* planted mistakes for the pin tests (0 = none; see poisson.c) */
+void orc_poisson_plant(int which);
+
+/* right-hand side of a solve (Poisson_solver_2d.hpp:71-103): b = source;
+ * b[r] += (0 + sum_m M(r,m) value(m)) per row of M; b[fixed] = fixed value */
+void orc_poisson_rhs(int64_t n, const double* source, int64_t mnnz, const uint64_t* mrows,
+                     const uint64_t* mcols, const double* mvals, const double* mixed_values,
+                     int64_t nfix, const uint64_t* fix_idx, const double* fix_val, double* b);
 
 /* a9-a16: geometric-transfer Galerkin multigrid restated with CSR matrices
  * (multigrid_utilities.hpp:55-104, multigrid_core.hpp:49-272,

@@ -68,6 +68,20 @@ class PsumBatchTuning(C.Structure):
                 ("push_sort_cost_ms", C.c_double)]
 
 
+class PsumMixedNode(C.Structure):
+    _fields_ = [("element", C.c_uint64), ("direction", C.c_int32), ("pad_", C.c_int32),
+                ("value_coef", C.c_double), ("grad_coef", C.c_double)]
+
+
+EPS_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
+class PsumPoissonSpec(C.Structure):
+    _fields_ = [("cylindrical", C.c_int32), ("pad_", C.c_int32), ("eps", C.c_double),
+                ("eps_fn", EPS_FN), ("eps_user", C.c_void_p), ("fixed", C.c_void_p),
+                ("mixed", C.c_void_p), ("n_mixed", C.c_int64)]
+
+
 class FuncTable(C.Structure):
     """struct func_table (PSuM field_solver/register/interface.h:14-23)."""
     _fields_ = [
@@ -123,6 +137,8 @@ SIGNATURES = [
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
     ("psum_stepper_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
     ("psum_poisson_assemble", ST, [_GP, C.c_double, P, I64, P, P, P, C.POINTER(I64)]),
+    ("psum_poisson_assemble_ex", ST, [_GP, C.POINTER(PsumPoissonSpec), I64, P, P, P, C.POINTER(I64),
+                                      I64, P, P, P, C.POINTER(I64)]),
     ("psum_batch_create", P, [C.POINTER(PsumStepConfig), C.c_int]),
     ("psum_batch_load", ST, [P, C.c_int, C.c_int, P, P, I64]),
     ("psum_batch_configure", ST, [P, C.c_int, C.c_int, C.c_int, C.c_int]),

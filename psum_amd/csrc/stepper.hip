@@ -39,6 +39,11 @@ void mg_solve_homogeneous(HipMultigrid* h, double* b, double* x);
 int mg_predicted(HipMultigrid* h);
 void mg_set_predicted(HipMultigrid* h, int p);
 void mg_set_fixed_cycles(HipMultigrid* h, int cycles);
+// Cartesian constant-eps FD Poisson matrix of the reference frontends with
+// identity rows at fixed nodes (poisson.hip)
+void assemble(const psum_grid& g, double eps, const std::vector<uint8_t>& fixed,
+              std::vector<unsigned long long>& R, std::vector<unsigned long long>& C,
+              std::vector<double>& V);
 }  // namespace psum_hip
 
 extern "C" {
@@ -180,89 +185,6 @@ struct Species {
     int64_t n = 0;
     double wdep = 0.0;
 };
-
-// Cartesian FD Poisson rows A = -div(eps grad) with identity rows at fixed
-// nodes, in the value order of Poisson_solver_{2,3}d::_init_impl
-// (Poisson_solver_2d.hpp:221-349, Poisson_solver_3d.hpp:228-350) for
-// constant eps: interior row = (gE sE e - gW sW e + gN sN e - gS sS e
-// [+ gU sU e - gD sD e]) * (-1/V), cyclic neighbours.
-void assemble(const psum_grid& g, double eps, const std::vector<uint8_t>& fixed,
-              std::vector<unsigned long long>& R, std::vector<unsigned long long>& C,
-              std::vector<double>& V)
-{
-    const int D = g.dim;
-    const int I = g.cells[0], J = D > 1 ? g.cells[1] : 0, K = D > 2 ? g.cells[2] : 0;
-    const int64_t nJ = J + 1, nK = K + 1;
-    auto cyc = [](int a, int n) { return ((a % n) + n) % n; };
-    struct Term { unsigned long long i; double v; };
-    for (int i = 0; i <= I; ++i)
-        for (int j = 0; j <= J; ++j)
-            for (int k = 0; k <= K; ++k) {
-                const unsigned long long idx = (unsigned long long)(((int64_t)i * nJ + j) * nK + k);
-                if (fixed[idx]) {
-                    R.push_back(idx); C.push_back(idx); V.push_back(1.0);
-                    continue;
-                }
-                std::vector<Term> t;
-                auto acc = [&](unsigned long long q, double v, bool sub) {
-                    for (auto& e : t)
-                        if (e.i == q) {
-                            const double old = std::fabs(v) + std::fabs(e.v);
-                            e.v = sub ? e.v - v : e.v + v;
-                            if (std::fabs(e.v) < old * 1e-14) {
-                                e.v = 0.0;
-                                e.i = ~0ull;   // erased (local_equation.hpp:35-38)
-                            }
-                            return;
-                        }
-                    t.push_back({q, sub ? -v : v});
-                };
-                struct Face { unsigned long long nb; double dr, s; bool plus; };
-                std::vector<Face> faces;
-                const double dxr = g.delta_r[0];
-                if (D == 2) {
-                    const double sNS = g.delta[0], sEW = g.delta[1];
-                    const double dyr = g.delta_r[1];
-                    faces = {{(unsigned long long)cyc(i + 1, I + 1) * nJ + j, dxr, sEW, true},
-                             {(unsigned long long)cyc(i - 1, I + 1) * nJ + j, dxr, sEW, false},
-                             {(unsigned long long)i * nJ + cyc(j + 1, J + 1), dyr, sNS, true},
-                             {(unsigned long long)i * nJ + cyc(j - 1, J + 1), dyr, sNS, false}};
-                } else {
-                    const double sNS = g.delta[0] * g.delta[2], sEW = g.delta[1] * g.delta[2];
-                    const double sUD = g.delta[0] * g.delta[1];
-                    const double dyr = g.delta_r[1], dzr = g.delta_r[2];
-                    faces = {{(unsigned long long)((cyc(i + 1, I + 1) * nJ + j) * nK + k), dxr, sEW, true},
-                             {(unsigned long long)((cyc(i - 1, I + 1) * nJ + j) * nK + k), dxr, sEW, false},
-                             {(unsigned long long)(((int64_t)i * nJ + cyc(j + 1, J + 1)) * nK + k), dyr, sNS, true},
-                             {(unsigned long long)(((int64_t)i * nJ + cyc(j - 1, J + 1)) * nK + k), dyr, sNS, false},
-                             {(unsigned long long)(((int64_t)i * nJ + j) * nK + cyc(k + 1, K + 1)), dzr, sUD, true},
-                             {(unsigned long long)(((int64_t)i * nJ + j) * nK + cyc(k - 1, K + 1)), dzr, sUD, false}};
-                }
-                const double vol = D == 2 ? g.delta[0] * g.delta[1] : g.delta[0] * g.delta[1] * g.delta[2];
-                for (size_t f = 0; f < faces.size(); ++f) {
-                    const Face& F = faces[f];
-                    // grad_plus = {idx: -dr, nb: dr}, grad_minus = {nb: -dr, idx: dr}
-                    unsigned long long i0, i1;
-                    double c0, c1;
-                    if (F.plus) { i0 = idx; c0 = F.dr * -1; i1 = F.nb; c1 = F.dr; }
-                    else { i0 = F.nb; c0 = F.dr * -1; i1 = idx; c1 = F.dr; }
-                    const bool sub = (f % 2) == 1;    // E - W + N - S (+ U - D)
-                    const double a0 = (c0 * F.s) * eps, a1 = (c1 * F.s) * eps;
-                    if (f == 0) {
-                        t.push_back({i0, a0});
-                        if (i1 != i0) t.push_back({i1, a1});
-                    } else {
-                        acc(i0, a0, sub);
-                        if (i1 != i0) acc(i1, a1, sub);
-                    }
-                }
-                const double sc = -1.0 / vol;
-                for (auto& e : t) {
-                    if (e.i == ~0ull) continue;
-                    R.push_back(idx); C.push_back(e.i); V.push_back(e.v * sc);
-                }
-            }
-}
 
 class Stepper {
 public:
@@ -1382,28 +1304,6 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
 }
 
 void psum_stepper_destroy(void* h) { delete static_cast<Stepper*>(h); }
-
-psum_status psum_poisson_assemble(const psum_grid* g, double eps, const uint8_t* fixed, int64_t cap,
-                                  unsigned long long* rows, unsigned long long* cols, double* vals,
-                                  int64_t* nnz)
-{
-    PSUM_ARG_CHECK(grid_ok(g) && g->dim >= 2 && nnz, "psum_poisson_assemble: bad argument");
-    return guarded([&]() {
-        const int64_t nn = grid_nodes(*g);
-        std::vector<uint8_t> fx(nn, 0);
-        if (fixed) std::memcpy(fx.data(), fixed, nn);
-        std::vector<unsigned long long> R, C;
-        std::vector<double> V;
-        assemble(*g, eps, fx, R, C, V);
-        *nnz = (int64_t)R.size();
-        if (rows && cols && vals) {
-            if (cap < (int64_t)R.size()) throw std::invalid_argument("psum_poisson_assemble: capacity too small");
-            std::memcpy(rows, R.data(), R.size() * sizeof(unsigned long long));
-            std::memcpy(cols, C.data(), C.size() * sizeof(unsigned long long));
-            std::memcpy(vals, V.data(), V.size() * sizeof(double));
-        }
-    });
-}
 
 psum_status psum_stepper_tune(void* h, int warmup, int probe, psum_batch_tuning* out)
 {

@@ -18,6 +18,8 @@ import ctypes as C
 import numpy as np
 import torch
 
+from . import _lib
+
 from ._lib import (KEY_INVALID, PSUM_F32, PSUM_F64, FuncTable, PsumError, PsumGrid,
                    PsumSpecies, PsumStepConfig, check, lib)
 
@@ -290,6 +292,65 @@ class SolverBackend:
 
 
 # ---------------------------------------------------------------- stepper
+
+# mixed-boundary side codes (2D: N S E W; 3D: Xpos Xneg Ypos Yneg Zpos Zneg)
+DIR2 = {"N": 0, "S": 1, "E": 2, "W": 3}
+DIR3 = {"Xpos": 0, "Xneg": 1, "Ypos": 2, "Yneg": 3, "Zpos": 4, "Zneg": 5}
+
+
+def poisson_assemble(g, eps=8.854e-12, fixed=None, mixed=(), cylindrical=False, eps_fn=None):
+    """Host assembly of the reference frontends' FD Poisson system
+    (Poisson_solver_2d.hpp:182-410 with the ghost elimination of
+    ghost_element_manager.hpp:56-117), through psum_poisson_assemble_ex.
+    mixed: (element, direction, value_coef, grad_coef) tuples; eps_fn(pos)
+    gives the permittivity at a face midpoint.  Returns ((rows, cols, vals) of
+    A', row-major) and ((rows, cols, vals) of M = -B D^-1, row-major)."""
+    nodes = list(mixed)
+    arr = (_lib.PsumMixedNode * max(1, len(nodes)))()
+    for k, (e, d, v, gc) in enumerate(nodes):
+        arr[k].element, arr[k].direction, arr[k].value_coef, arr[k].grad_coef = e, d, v, gc
+    n = num_nodes(g)
+    fx = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+    if fx is not None and fx.size != n:
+        raise ValueError("poisson_assemble: fixed mask needs one entry per node")
+    spec = _lib.PsumPoissonSpec()
+    spec.cylindrical = int(bool(cylindrical))
+    spec.eps = float(eps)
+    cb = _lib.EPS_FN(lambda p, u: float(eps_fn(p))) if eps_fn is not None else _lib.EPS_FN()
+    spec.eps_fn = cb
+    spec.fixed = None if fx is None else fx.ctypes.data
+    spec.mixed = C.cast(arr, C.c_void_p) if nodes else None
+    spec.n_mixed = len(nodes)
+    nnz, mnnz = C.c_int64(0), C.c_int64(0)
+    check(lib().psum_poisson_assemble_ex(C.byref(g), C.byref(spec), 0, None, None, None, C.byref(nnz),
+                                         0, None, None, None, C.byref(mnnz)), "poisson_assemble")
+    A = [np.zeros(nnz.value, t) for t in (np.uint64, np.uint64, np.float64)]
+    M = [np.zeros(mnnz.value, t) for t in (np.uint64, np.uint64, np.float64)]
+    check(lib().psum_poisson_assemble_ex(C.byref(g), C.byref(spec), nnz.value,
+                                         *[a.ctypes.data_as(C.c_void_p) for a in A], C.byref(nnz),
+                                         mnnz.value, *[a.ctypes.data_as(C.c_void_p) for a in M],
+                                         C.byref(mnnz)), "poisson_assemble")
+    return tuple(A), tuple(M)
+
+
+def poisson_addback(M, values):
+    """Per row of M (row-major): 0 + sum_m M(r, m) * values[m] in column order
+    (Poisson_solver_2d.hpp:71-85).  Returns (rows, sums) for
+    set_source_addback."""
+    mr, mc, mv = M
+    vals = np.asarray(values, np.float64)
+    rows, sums = [], []
+    q = 0
+    while q < len(mr):
+        e, acc = q, 0.0
+        while e < len(mr) and mr[e] == mr[q]:
+            acc += float(mv[e]) * float(vals[int(mc[e])])
+            e += 1
+        rows.append(int(mr[q]))
+        sums.append(acc)
+        q = e
+    return np.asarray(rows, np.uint64), np.asarray(sums, np.float64)
+
 
 def step_config(g, species, dtype="f64", mixed_mg=False, depth=4, pre=2, post=1, max_cycles=20,
                 tol=1e-10, eps=1.0, dt=0.1, rho_bg=0.0, sort_every=0, fixed=None, fixed_val=None,

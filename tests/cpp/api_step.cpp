@@ -212,6 +212,95 @@ int main()
         report("Poisson_solver_2d: linear Dirichlet solution", perr <= 1e-9 * pmax, perr / pmax);
     }
 
+    // ---- mixed boundaries through the frontend.  (a) Cartesian, Neumann on N,
+    // Robin on S and E, Dirichlet on W at the values of a quadratic phi: the
+    // 5-point scheme and the central-difference ghost rows are exact for
+    // quadratics, so the discrete solution is phi itself.
+    {
+        grid2D qg({-1.0, -0.5}, {1.0, 1.5}, {64, 64});
+        const double a = 0.7, b = -0.4, c = 0.3, d = 0.2, e = -0.5, f0 = 1.1, eps = 2.5;
+        auto phi = [=](const std::array<double, 2>& p) {
+            return a * p[0] * p[0] + b * p[1] * p[1] + c * p[0] * p[1] + d * p[0] + e * p[1] + f0;
+        };
+        auto dphi = [=](const std::array<double, 2>& p, int ax) {
+            return ax == 0 ? 2 * a * p[0] + c * p[1] + d : 2 * b * p[1] + c * p[0] + e;
+        };
+        using Pos = std::array<double, 2>;
+        auto robin_value = [&](double vc, double gc, int ax, double sgn) {
+            return std::function<double(const Pos&)>(
+                [=](const Pos& p) { return vc * phi(p) + gc * sgn * dphi(p, ax); });
+        };
+        auto mN = boundary_creator::Neumann_line(qg, boundary_direction_2d::N);
+        mN = robin_value(0.0, 1.0, 1, 1.0);
+        auto mS = boundary_creator::Robin_Line(qg, boundary_direction_2d::S, {2.0, 0.5});
+        mS = robin_value(2.0, 0.5, 1, -1.0);
+        auto mE = boundary_creator::Robin_Line(qg, boundary_direction_2d::E,
+                                               [](double, double y) { return std::make_pair(1.0 + y * y, 3.0); });
+        mE = std::function<double(const Pos&)>(
+            [&](const Pos& p) { return (1.0 + p[1] * p[1]) * phi(p) + 3.0 * dphi(p, 0); });
+        auto dW = boundary_creator::Dirichlet_line(qg, boundary_direction_2d::W);
+        dW = std::function<double(const Pos&)>(phi);
+        Poisson_solver_2d solver;
+        solver.set_options("depth 4 pre_relax 2 post_relax 2 max_cycles 300 tol 1e-13");
+        solver.init("hip_multigrid_gpu", qg, Poisson_solver_2d::Cartesian, mixed_boundary_2d_list{mN, mS, mE},
+                    fixed_boundary_2d_list{dW}, eps);
+        const size_t nq = qg.contentSize(var_loc::nodeCentered);
+        std::vector<double> src(nq, -eps * (2 * a + 2 * b));
+        double *dphi_d, *dsrc;
+        psum::hip_check(hipMalloc(&dphi_d, nq * 8), "m");
+        psum::hip_check(hipMalloc(&dsrc, nq * 8), "m");
+        psum::hip_check(hipMemset(dphi_d, 0, nq * 8), "m");
+        psum::hip_check(hipMemcpy(dsrc, src.data(), nq * 8, hipMemcpyHostToDevice), "m");
+        solver.solve(dphi_d, dsrc);
+        std::vector<double> hp(nq);
+        psum::hip_check(hipMemcpy(hp.data(), dphi_d, nq * 8, hipMemcpyDeviceToHost), "m");
+        double err = 0, mx = 0;
+        for (size_t i = 0; i < nq; ++i) {
+            const double ref = phi(qg.nodePosition(qg.i2n(i)));
+            err = std::max(err, std::fabs(hp[i] - ref));
+            mx = std::max(mx, std::fabs(ref));
+        }
+        report("Poisson_solver_2d: Robin/Neumann quadratic", err <= 1e-9 * mx, err / mx);
+        psum::hip_check(hipFree(dphi_d), "f");
+        psum::hip_check(hipFree(dsrc), "f");
+    }
+
+    // (b) cylindrical (z, r): Neumann at z=0, z=1 and the axis, phi=0 at r=R,
+    // source c1 r^c2; the axis potential is c1 R^(c2+2) / (eps (c2+2)^2)
+    // (test_psolver_2d.cpp:17-76), second-order accurate: 1.6e-4 at 120 x 80
+    {
+        const double c1 = 0.01, c2 = 0.5, R = 1.67, eps0 = 8.854e-12;
+        grid2D cg({0.0, 0.0}, {1.0, R}, {120, 80});
+        Poisson_solver_2d solver;
+        solver.set_options("depth 4 pre_relax 2 post_relax 2 max_cycles 300 tol 1e-12");
+        auto nE = boundary_creator::Neumann_line(cg, boundary_direction_2d::E);
+        auto nW = boundary_creator::Neumann_line(cg, boundary_direction_2d::W);
+        auto nS = boundary_creator::Neumann_line(cg, boundary_direction_2d::S);
+        nE = 0.0;
+        nW = 0.0;
+        nS = 0.0;
+        auto top = boundary_creator::Dirichlet_line(cg, boundary_direction_2d::N);
+        top = 0.0;
+        solver.init("hip_multigrid_gpu", cg, Poisson_solver_2d::Cylindrical, mixed_boundary_2d_list{nE, nW, nS},
+                    fixed_boundary_2d_list{top}, eps0);
+        const size_t nc = cg.contentSize(var_loc::nodeCentered);
+        std::vector<double> src(nc);
+        for (size_t i = 0; i < nc; ++i) src[i] = c1 * std::pow(cg.nodePosition(cg.i2n(i))[1], c2);
+        double *dphi_d, *dsrc;
+        psum::hip_check(hipMalloc(&dphi_d, nc * 8), "m");
+        psum::hip_check(hipMalloc(&dsrc, nc * 8), "m");
+        psum::hip_check(hipMemset(dphi_d, 0, nc * 8), "m");
+        psum::hip_check(hipMemcpy(dsrc, src.data(), nc * 8, hipMemcpyHostToDevice), "m");
+        solver.solve(dphi_d, dsrc);
+        double at = 0;
+        psum::hip_check(hipMemcpy(&at, dphi_d + (size_t)120 * 81, 8, hipMemcpyDeviceToHost), "m");
+        const double theory = std::pow(R, c2 + 2) * c1 / (eps0 * (c2 + 2) * (c2 + 2));
+        const double rel = std::fabs(at - theory) / theory;
+        report("Poisson_solver_2d: cylindrical axis value", rel < 5e-4, rel);
+        psum::hip_check(hipFree(dphi_d), "f");
+        psum::hip_check(hipFree(dsrc), "f");
+    }
+
     std::printf(fails ? "FAILED %d\n" : "ALL OK\n", fails);
     return fails ? 1 : 0;
 }

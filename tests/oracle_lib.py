@@ -112,6 +112,11 @@ def lib():
             getattr(L, nm).restype = i64
         L.orc_poisson_coo.argtypes = [C.POINTER(Grid), C.c_double, P, P, P, P]
         L.orc_poisson_coo.restype = i64
+        L.orc_poisson_assemble_ex.argtypes = [C.POINTER(Grid), C.c_int, C.c_double, EPS_FN, P, P, P,
+                                              i64, i64, P, P, P, P, i64, P, P, P, P]
+        L.orc_poisson_assemble_ex.restype = C.c_int
+        L.orc_poisson_rhs.argtypes = [i64, P, i64, P, P, P, P, i64, P, P, P]
+        L.orc_poisson_plant.argtypes = [C.c_int]
         L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int]
         L.orc_mg_is_singular.argtypes = [P, P]
@@ -332,6 +337,69 @@ def poisson_coo(g, eps, fixed):
     v = np.empty(nnz, np.float64)
     lib().orc_poisson_coo(C.byref(g), eps, _ptr(fixed), _ptr(r), _ptr(c), _ptr(v))
     return r, c, v
+
+
+EPS_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
+class MixedNode(C.Structure):
+    _fields_ = [("element", C.c_uint64), ("direction", C.c_int32), ("pad_", C.c_int32),
+                ("value_coef", C.c_double), ("grad_coef", C.c_double)]
+
+
+# direction codes (2D: N S E W; 3D: Xpos Xneg Ypos Yneg Zpos Zneg)
+DIR2 = {"N": 0, "S": 1, "E": 2, "W": 3}
+DIR3 = {"Xpos": 0, "Xneg": 1, "Ypos": 2, "Yneg": 3, "Zpos": 4, "Zneg": 5}
+
+
+def mixed_array(nodes):
+    """nodes: iterable of (element, direction, value_coef, grad_coef)"""
+    nodes = list(nodes)
+    arr = (MixedNode * max(1, len(nodes)))()
+    for k, (e, d, v, gc) in enumerate(nodes):
+        arr[k].element, arr[k].direction, arr[k].value_coef, arr[k].grad_coef = e, d, v, gc
+    return arr, len(nodes)
+
+
+def poisson_assemble_ex(g, eps=1.0, fixed=None, mixed=(), cylindrical=False, eps_fn=None):
+    """General assembly: returns (rows, cols, vals) of A' in column-major
+    order and (mrows, mcols, mvals) of M row-major."""
+    n = num_nodes(g)
+    fx = np.zeros(n, np.uint8) if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+    arr, nm = mixed_array(mixed)
+    cb = EPS_FN(lambda p, u: eps_fn(p)) if eps_fn is not None else EPS_FN()
+    nnz, mnnz = C.c_int64(0), C.c_int64(0)
+    rc = lib().orc_poisson_assemble_ex(C.byref(g), int(cylindrical), eps, cb, None, _ptr(fx), arr, nm,
+                                       0, None, None, None, C.byref(nnz), 0, None, None, None,
+                                       C.byref(mnnz))
+    if rc != 0:
+        raise ValueError(f"orc_poisson_assemble_ex: {rc}")
+    r, c, v = (np.empty(nnz.value, np.uint64), np.empty(nnz.value, np.uint64),
+               np.empty(nnz.value, np.float64))
+    mr, mc, mv = (np.empty(mnnz.value, np.uint64), np.empty(mnnz.value, np.uint64),
+                  np.empty(mnnz.value, np.float64))
+    rc = lib().orc_poisson_assemble_ex(C.byref(g), int(cylindrical), eps, cb, None, _ptr(fx), arr, nm,
+                                       nnz.value, _ptr(r), _ptr(c), _ptr(v), C.byref(nnz),
+                                       mnnz.value, _ptr(mr), _ptr(mc), _ptr(mv), C.byref(mnnz))
+    if rc != 0:
+        raise ValueError(f"orc_poisson_assemble_ex: {rc}")
+    return (r, c, v), (mr, mc, mv)
+
+
+def poisson_rhs(source, M, mixed_values, fix_idx=(), fix_val=()):
+    mr, mc, mv = M
+    src = np.ascontiguousarray(source, np.float64)
+    mvv = np.ascontiguousarray(mixed_values, np.float64)
+    fi = np.ascontiguousarray(fix_idx, np.uint64)
+    fv = np.ascontiguousarray(fix_val, np.float64)
+    b = np.empty_like(src)
+    lib().orc_poisson_rhs(src.size, _ptr(src), mr.size, _ptr(mr), _ptr(mc), _ptr(mv), _ptr(mvv),
+                          fi.size, _ptr(fi), _ptr(fv), _ptr(b))
+    return b
+
+
+def poisson_plant(which):
+    lib().orc_poisson_plant(which)
 
 
 class MG:

@@ -119,3 +119,107 @@ def test_poisson_assembly_matches_oracle(case):
     b = sorted(zip(np.asarray(r)[kr].tolist(), np.asarray(c)[kr].tolist(), np.asarray(v)[kr].tolist()))
     assert [x[:2] for x in a] == [x[:2] for x in b]
     assert np.allclose([x[2] for x in a], [x[2] for x in b], rtol=1e-14, atol=0)
+
+
+def _product_assemble_ex(og, eps, fixed, nodes, cylindrical=False, eps_fn=None):
+    import numpy as np
+    from psum_amd import _lib
+    g = _lib.PsumGrid()
+    C.memmove(C.byref(g), C.byref(og), C.sizeof(g))
+    arr = (_lib.PsumMixedNode * max(1, len(nodes)))()
+    for k, (e, d, v, gc) in enumerate(nodes):
+        arr[k].element, arr[k].direction, arr[k].value_coef, arr[k].grad_coef = e, d, v, gc
+    fx = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+    spec = _lib.PsumPoissonSpec()
+    spec.cylindrical = int(cylindrical)
+    spec.eps = eps
+    cb = _lib.EPS_FN(lambda p, u: eps_fn(p)) if eps_fn else _lib.EPS_FN()
+    spec.eps_fn = cb
+    spec.fixed = None if fx is None else fx.ctypes.data
+    spec.mixed = C.cast(arr, C.c_void_p) if nodes else None
+    spec.n_mixed = len(nodes)
+    L = _lib.lib()
+    nnz, mnnz = C.c_int64(0), C.c_int64(0)
+    st = L.psum_poisson_assemble_ex(C.byref(g), C.byref(spec), 0, None, None, None, C.byref(nnz),
+                                    0, None, None, None, C.byref(mnnz))
+    if st != 0:
+        return st, None, None
+    out = [np.zeros(nnz.value, t) for t in (np.uint64, np.uint64, np.float64)]
+    mout = [np.zeros(mnnz.value, t) for t in (np.uint64, np.uint64, np.float64)]
+    st = L.psum_poisson_assemble_ex(C.byref(g), C.byref(spec), nnz.value, *[a.ctypes.data for a in out],
+                                    C.byref(nnz), mnnz.value, *[a.ctypes.data for a in mout],
+                                    C.byref(mnnz))
+    return st, out, mout
+
+
+def _sorted_triplets(t):
+    import numpy as np
+    r, c, v = (np.asarray(a) for a in t)
+    o = np.lexsort((c, r))
+    return r[o], c[o], v[o]
+
+
+def _edge(og, axis, plus):
+    import numpy as np
+    import oracle_lib as O
+    shp = O.node_shape(og)
+    idx = np.arange(O.num_nodes(og)).reshape(shp)
+    sl = [slice(None)] * og.dim
+    sl[axis] = shp[axis] - 1 if plus else 0
+    return idx[tuple(sl)].ravel()
+
+
+@pytest.mark.parametrize("case", ["robin2d", "cyl", "eps_fn", "robin3d"])
+def test_poisson_assembly_ex_bitexact_vs_oracle(case):
+    """psum_poisson_assemble_ex (host, no GPU): A' and M equal the oracle's
+    restatement of the reference frontend entry for entry, bit for bit."""
+    import numpy as np
+    import oracle_lib as O
+    eps_fn, cyl = None, False
+    if case == "robin2d":
+        og = O.make_grid([-1.0, -0.5], [1.0, 1.5], [9, 7])
+        sides = [(1, True, 0.0, 1.0), (1, False, 2.0, 0.5), (0, True, 1.0, 3.0), (0, False, 1.0, 1.0)]
+        fixed_sides = [(0, False)]
+    elif case == "cyl":
+        og = O.make_grid([0.0, 0.0], [1.0, 1.67], [12, 8])
+        sides = [(0, True, 0.0, 1.0), (0, False, 0.0, 1.0), (1, False, 0.0, 1.0)]
+        fixed_sides = [(1, True)]
+        cyl = True
+    elif case == "eps_fn":
+        og = O.make_grid([0.0, 0.0], [2.0, 1.0], [10, 6])
+        sides = [(1, True, 1.0, 2.0)]
+        fixed_sides = [(0, False), (0, True)]
+        eps_fn = lambda p: 1.0 + 0.5 * np.sin(p[0]) * np.cos(3 * p[1])
+    else:
+        og = O.make_grid([0.0, -1.0, 0.5], [1.0, 0.5, 1.5], [5, 6, 4])
+        sides = [(0, True, 1.0, 2.0), (1, True, 0.0, 1.0), (1, False, 0.5, 1.0), (2, True, 0.0, 1.0),
+                 (2, False, 3.0, 0.25)]
+        fixed_sides = [(0, False)]
+    n = O.num_nodes(og)
+    fixed = np.zeros(n, np.uint8)
+    for ax, pl in fixed_sides:
+        fixed[_edge(og, ax, pl)] = 1
+    nodes = []
+    for ax, pl, v, gc in sides:
+        code = ((0 if pl else 1) if ax == 1 else (2 if pl else 3)) if og.dim == 2 else 2 * ax + (0 if pl else 1)
+        nodes += [(int(e), code, v, gc) for e in _edge(og, ax, pl)]
+    A, M = O.poisson_assemble_ex(og, 8.854e-12 if cyl else 1.25, fixed, nodes, cyl, eps_fn)
+    st, A2, M2 = _product_assemble_ex(og, 8.854e-12 if cyl else 1.25, fixed, nodes, cyl, eps_fn)
+    assert st == 0
+    for x, y in zip(_sorted_triplets(A), _sorted_triplets(A2)):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(M, M2):          # both row-major
+        np.testing.assert_array_equal(x, y)
+    assert M2[0].size > 0
+
+
+def test_poisson_assembly_ex_rejects_bad_input():
+    import oracle_lib as O
+    og = O.wall_grid([4, 3])
+    st, _, _ = _product_assemble_ex(og, 1.0, None, [(4, 0, 1.0, 0.0)])   # grad_coef 0
+    assert st != 0
+    st, _, _ = _product_assemble_ex(og, 1.0, None, [(400, 0, 0.0, 1.0)])  # element out of range
+    assert st != 0
+    og3 = O.wall_grid([3, 3, 3])
+    st, _, _ = _product_assemble_ex(og3, 1.0, None, [], cylindrical=True)
+    assert st != 0
