@@ -13,6 +13,7 @@
 #include "grid.hpp"
 #include "particle_group.hpp"
 #include "poisson_solver.hpp"
+#include "serialization.hpp"
 #include "tag.hpp"
 
 namespace psum {
@@ -53,6 +54,7 @@ using psum::field_solver::mixed_boundary_3d_list;
 using psum::field_solver::Poisson_solver_2d;
 using psum::field_solver::Poisson_solver_3d;
 namespace boundary_creator = psum::field_solver::boundary_creator;
+using psum::serialization::mas_file;
 namespace property = psum::tag::property;
 }  // namespace prelude
 }  // namespace psum

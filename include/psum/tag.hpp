@@ -35,9 +35,16 @@ struct vec {
 
 namespace tag {
 namespace property {
-struct position {};
-struct velocity {};
-struct weight {};
+// tag_name: the block-name suffix of a tag in checkpoint files
+struct position {
+    static constexpr const char* tag_name = "position";
+};
+struct velocity {
+    static constexpr const char* tag_name = "velocity";
+};
+struct weight {
+    static constexpr const char* tag_name = "weight";
+};
 }  // namespace property
 
 template <typename Tag, typename Type>

@@ -126,6 +126,24 @@ psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos
                                  double dt, double mass, double w_dep, double scale, int64_t* acc,
                                  double* diag, int64_t* counters, void* stream);
 
+/* Higher-order coupling (interp_B_spline / add_back_B_spline of
+ * higher_order_interp.hpp:196-248, selected by TSI's interp_order,
+ * TSI/main.cpp:29,54,64): quadratic (order 2), cubic (3) and quartic (4)
+ * B-spline weights (simple_interpolation.hpp:105-184) on order+1 nodes per
+ * dimension around the nearest node (odd support) or the corner node (even
+ * support), wrapped once (simple_grid.hpp:316-385).  Weights are FP64 for
+ * both dtypes.  order 1 forwards to the CIC entry points.  2D and 3D.
+ * Deposit: as psum_hip_deposit_cic with these weights (exact int64 sums,
+ * scale a power of two; counters [0] out of range, [1] tile fallbacks).
+ * Gather + push: E = -grad(phi) summed in FP64 and rounded to the dtype,
+ * then the kick, drift and boundaries of psum_hip_gather_push (no fused
+ * deposit); diag has psum_hip_push_blocks(n) blocks as there. */
+psum_status psum_hip_deposit_bspline(const psum_grid* g, int dtype, int order, void* const* pos, int64_t n,
+                                     double w, double scale, int64_t* acc, int64_t* counters, void* stream);
+psum_status psum_hip_gather_push_bspline(const psum_grid* g, int dtype, int order, void* const* pos,
+                                         void* const* vel, int64_t n, const void* phi, double qm, double dt,
+                                         double mass, double* diag, int64_t* counters, void* stream);
+
 /* out[k] = sum over blocks of diag[4*b + k], k < 4, in a fixed order
  * (deterministic).  work: psum_hip_reduce_workspace() bytes of device
  * memory (may be NULL when nblocks <= 1024). */
@@ -213,6 +231,22 @@ psum_status psum_stepper_reverse(void* h, double seed, void* const* grad_x, void
 psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, void* const* grad_x,
                                  void* const* grad_v, double* t_fwd_ms, double* t_bwd_ms);
 void psum_stepper_destroy(void* h);
+
+/* Interpolation order of the step (TSI's interp_order, TSI/main.cpp:29,54,
+ * 64): 1 = CIC (default, fused push + deposit), 2..4 = B-splines through
+ * psum_hip_gather_push_bspline / psum_hip_deposit_bspline.  The adjoint
+ * entry points (record / reverse / adjoint) accept order 1 only. */
+psum_status psum_stepper_set_interp_order(void* h, int order);
+
+/* Checkpoint files in the reference's MAS format (src/serialization/mas.hpp:
+ * 49-453; particle_group / device_field save(), particle_group.hpp:272-275,
+ * device_field.hpp:327-331): species<q>.data.content.{position,velocity}
+ * (n x dim x 1), phi.{location,grid.*,content.*} and psum.* scalars (step
+ * counter, solver cycle predictors, extrapolation state).  restore() needs a
+ * stepper of the same grid, dtype and species count; the run continues
+ * bit-identically to one that was never interrupted. */
+psum_status psum_stepper_save(void* h, const char* path);
+psum_status psum_stepper_restore(void* h, const char* path);
 
 /* Solver and sort schedule of a stepper: fixed_cycles > 0 switches its
  * multigrid to that many V-cycles per solve (one per refinement iteration,

@@ -40,6 +40,12 @@
  *                          reduction in 10 V(2,1) cycles on 256^2
  *                          (test/field_solver/test_multigrid_core.cpp:142-209)
  *   orc_field_energy etc.  sum-by-parts identity 1/2 sum phi*rho = 1/2 eps |grad phi|^2
+ *   orc_bspline_*          closed-form values at nodes and half nodes, partition
+ *                          of unity, first and second moments ((order+1)/12),
+ *                          gradient weights vs central differences of the
+ *                          interpolant, continuity across the base-node switch;
+ *                          deposits conserve charge and are the transpose of the
+ *                          gather; a uniform field is gathered exactly
  *   orc_pic_adjoint        central finite differences of orc_pic_loss in
  *                          directions where no particle changes cell
  *                          (key_hash equal at +h and -h); dot-product test
@@ -114,6 +120,31 @@ int64_t orc_gather_push_f64(const orc_grid* g, double** pos, double** vel, int64
                             const double* phi, double qm, double dt, double mass, double* diag);
 int64_t orc_gather_push_f32(const orc_grid* g, float** pos, float** vel, int64_t n,
                             const float* phi, float qm, float dt, double mass, double* diag);
+
+/* Higher-order B-spline coupling, orders 2..4 (simple_interpolation.hpp:
+ * 105-184, simple_grid.hpp:316-385, higher_order_interp.hpp:79-248).
+ * orc_bspline_1d: the 1D weights c[order+1] and derivatives dc (may be NULL)
+ * at normalised offset s.  orc_bspline_weights: support of x (FP64), idx and
+ * coef of S^D points, gcoef[d*125+i] the gradient weights (may be NULL);
+ * returns S^D, 0 when out of range.  Deposit and gather+push as the CIC
+ * functions above with these weights (FP64 weights for both dtypes; the
+ * gather sums in FP64 and rounds the gradient to the particle dtype). */
+void orc_bspline_1d(int order, double s, double* c, double* dc);
+int orc_bspline_weights(const orc_grid* g, int order, const double* x, int64_t* idx, double* coef,
+                        double* gcoef);
+int64_t orc_deposit_bs_f64(const orc_grid* g, int order, const double* const* pos, int64_t n, double w,
+                           double scale, int64_t* acc);
+int64_t orc_deposit_bs_f32(const orc_grid* g, int order, const float* const* pos, int64_t n, double w,
+                           double scale, int64_t* acc);
+int64_t orc_deposit_bs_ref(const orc_grid* g, int order, const double* const* pos, int64_t n, double w,
+                           double* rho);
+int64_t orc_gather_push_bs_f64(const orc_grid* g, int order, double** pos, double** vel, int64_t n,
+                               const double* phi, double qm, double dt, double mass, double* diag);
+int64_t orc_gather_push_bs_f32(const orc_grid* g, int order, float** pos, float** vel, int64_t n,
+                               const float* phi, float qm, float dt, double mass, double* diag);
+void orc_bspline_plant(int which);
+/* interpolation order of orc_pic_run_* (1 = CIC default, 2..4 B-splines) */
+void orc_pic_set_order(int order);
 
 /* a7: stable compaction of valid particles (finite pos[0]) in index order. */
 int64_t orc_compact_f64(int dim, double* const* pos, double* const* vel, int64_t n,

@@ -13,6 +13,11 @@
 /* Fixed-point exponent for the deposit: term = llrint(w*coef * 2^S) with
  * S = 40 - e, 2^(e-1) <= |w| < 2^e, so one particle contributes at most 2^40
  * and a node may collect up to 2^22 full particles before int64 overflows. */
+/* interpolation order of the loop: 1 = CIC, 2..4 = B-splines (TSI's
+ * interp_order, TSI/main.cpp:29,54,64) */
+static int g_order = 1;
+void orc_pic_set_order(int order) { g_order = order; }
+
 static int fx_shift(double w)
 {
     int e = 0;
@@ -62,7 +67,8 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
         if (dep_mode == 0) {
             memset(acc, 0, sizeof(int64_t) * (size_t)nn);
             for (int s = 0; s < nspecies; ++s)
-                orc_deposit_fx_f64(g, (const double* const*)pos[s], n[s], wdep[s], scale, acc);
+                if (g_order > 1) orc_deposit_bs_f64(g, g_order, (const double* const*)pos[s], n[s], wdep[s], scale, acc);
+                else orc_deposit_fx_f64(g, (const double* const*)pos[s], n[s], wdep[s], scale, acc);
             for (int64_t i = 0; i < nn; ++i) b[i] = (double)acc[i] * (1.0 / scale) + rho_bg;
         } else {
             memset(b, 0, sizeof(double) * (size_t)nn);
@@ -90,7 +96,8 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
         double tot[4] = {0, 0, 0, 0};
         for (int s = 0; s < nspecies; ++s) {
             double diag[4];
-            orc_gather_push_f64(g, pos[s], vel[s], n[s], phi, q[s] / m[s], dt, m[s] * w[s], diag);
+            if (g_order > 1) orc_gather_push_bs_f64(g, g_order, pos[s], vel[s], n[s], phi, q[s] / m[s], dt, m[s] * w[s], diag);
+            else orc_gather_push_f64(g, pos[s], vel[s], n[s], phi, q[s] / m[s], dt, m[s] * w[s], diag);
             for (int k = 0; k < 4; ++k) tot[k] += diag[k];
         }
         if (hist) {

@@ -113,6 +113,9 @@ SIGNATURES = [
     ("psum_hip_push_blocks", I64, [I64]),
     ("psum_hip_gather_push", ST, [_GP, C.c_int, P, P, I64, P, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_double, P, P, P, P]),
+    ("psum_hip_deposit_bspline", ST, [_GP, C.c_int, C.c_int, P, I64, C.c_double, C.c_double, P, P, P]),
+    ("psum_hip_gather_push_bspline", ST, [_GP, C.c_int, C.c_int, P, P, I64, P, C.c_double, C.c_double,
+                                          C.c_double, P, P, P]),
     ("psum_hip_reduce_workspace", C.c_size_t, []),
     ("psum_hip_reduce_diag", ST, [P, I64, P, P, P]),
     ("psum_hip_field_energy_workspace", C.c_size_t, [_GP]),
@@ -134,6 +137,9 @@ SIGNATURES = [
     ("psum_stepper_adjoint", ST, [P, C.c_int, C.c_int, C.POINTER(C.c_double), P, P,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psum_stepper_destroy", None, [P]),
+    ("psum_stepper_set_interp_order", ST, [P, C.c_int]),
+    ("psum_stepper_save", ST, [P, C.c_char_p]),
+    ("psum_stepper_restore", ST, [P, C.c_char_p]),
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
     ("psum_stepper_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
     ("psum_poisson_assemble", ST, [_GP, C.c_double, P, I64, P, P, P, C.POINTER(I64)]),

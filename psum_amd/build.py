@@ -65,6 +65,22 @@ def build_cpp_tests(verbose=True):
             print(f"built {exe}")
 
 
+HOST_TESTS = [os.path.join(ROOT, "tests", "cpp", "mas_io.cpp")]
+
+
+def build_host_tests(verbose=True):
+    """Host-only C++ test drivers (no GPU code): g++ against include/."""
+    for src in HOST_TESTS:
+        exe = src[:-4]
+        cmd = ["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", exe + ".tmp", src]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+        if verbose:
+            print(f"built {exe}")
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
@@ -79,6 +95,7 @@ def build(verbose=True):
     if verbose:
         print(f"built {LIB}")
     build_cpp_tests(verbose)
+    build_host_tests(verbose)
     return LIB
 
 

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "psum/mas.hpp"
 
 namespace psum_hip {
 class HipMultigrid;
@@ -384,8 +385,12 @@ public:
                 Species& sp = sp_[q];
                 const double qm = sp.sp.q / sp.sp.m;
                 if (timing_) S_TRY(hipEventRecord(ev_[2 * nev_], st_));
-                CHK(psum_hip_gather_push(&cfg_.grid, cfg_.dtype, sp.pos, sp.vel, sp.n, ph, qm, cfg_.dt,
-                                         sp.sp.m * sp.sp.w, sp.wdep, scale_, acc_, diag_, counters_, st_));
+                if (order_ > 1)
+                    CHK(psum_hip_gather_push_bspline(&cfg_.grid, cfg_.dtype, order_, sp.pos, sp.vel, sp.n, ph, qm,
+                                                     cfg_.dt, sp.sp.m * sp.sp.w, diag_, counters_, st_));
+                else
+                    CHK(psum_hip_gather_push(&cfg_.grid, cfg_.dtype, sp.pos, sp.vel, sp.n, ph, qm, cfg_.dt,
+                                             sp.sp.m * sp.sp.w, sp.wdep, scale_, acc_, diag_, counters_, st_));
                 if (timing_) {
                     S_TRY(hipEventRecord(ev_[2 * nev_ + 1], st_));
                     push_particles_ += sp.n;
@@ -396,6 +401,12 @@ public:
                     else hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st_, red_ + 4 * q, 4);
                 }
             }
+            // higher orders deposit the new positions after the push (the CIC
+            // push deposits them itself)
+            if (order_ > 1)
+                for (int q = 0; q < cfg_.nspecies; ++q)
+                    CHK(psum_hip_deposit_bspline(&cfg_.grid, cfg_.dtype, order_, sp_[q].pos, sp_[q].n, sp_[q].wdep,
+                                                 scale_, acc_, counters_, st_));
             if (diagnostics)
                 hipLaunchKernelGGL(k_hist_record, dim3(1), dim3(64), 0, st_, fe_, red_, cfg_.nspecies, hist_,
                                    (long long*)hist_count_, (long long)hist_cap_);
@@ -548,6 +559,132 @@ public:
         acc_valid_ = false;
     }
 
+    // ---- checkpoint files (MAS).  Blocks follow the reference's save()
+    // naming for a particle group and a node field (particle_group.hpp:272,
+    // device_vector.hpp:294-309, tag_sl.hpp:62-98, device_field.hpp:327-331):
+    //   species<q>.data.device, species<q>.data.content.{position,velocity}
+    //   (n x D x 1, the particle dtype), phi.location, phi.grid.{lower,upper,
+    //   cells}, phi.content.{device,content}; plus psum.* scalars holding the
+    // step counter, the solver's cycle predictors and the extrapolation
+    // state, so that a restored stepper continues bit-identically.
+    template <typename T>
+    void save_species(psum::serialization::mas_file& f, int q, const std::string& dev)
+    {
+        const int D = cfg_.grid.dim;
+        const int64_t n = sp_[q].n;
+        std::vector<T> comp((size_t)std::max<int64_t>(n, 1)), inter((size_t)std::max<int64_t>(n * D, 1));
+        const std::string name = "species" + std::to_string(q);
+        f.writeData(name + ".data.device", dev);
+        for (int which = 0; which < 2; ++which) {
+            for (int d = 0; d < D; ++d) {
+                S_TRY(hipMemcpy(comp.data(), which ? sp_[q].vel[d] : sp_[q].pos[d], (size_t)n * sizeof(T),
+                                hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < n; ++i) inter[(size_t)(i * D + d)] = comp[(size_t)i];
+            }
+            f.writeData<T>(name + (which ? ".data.content.velocity" : ".data.content.position"), inter.data(),
+                           {(size_t)n, (size_t)D, 1});
+        }
+    }
+
+    template <typename T>
+    void restore_species(psum::serialization::mas_file& f, int q)
+    {
+        const int D = cfg_.grid.dim;
+        const std::string name = "species" + std::to_string(q);
+        std::vector<size_t> sp_shape, sv_shape;
+        const std::vector<T> pos = f.read<T>(name + ".data.content.position", &sp_shape);
+        const std::vector<T> vel = f.read<T>(name + ".data.content.velocity", &sv_shape);
+        if (sp_shape.size() != 3 || sp_shape[1] != (size_t)D || sp_shape[2] != 1 || sv_shape != sp_shape)
+            throw std::invalid_argument("checkpoint: " + name + " has the wrong shape");
+        const int64_t n = (int64_t)sp_shape[0];
+        if (n > sp_[q].sp.capacity) throw std::invalid_argument("checkpoint: " + name + " exceeds the capacity");
+        std::vector<T> comp((size_t)std::max<int64_t>(n, 1));
+        for (int which = 0; which < 2; ++which)
+            for (int d = 0; d < D; ++d) {
+                const std::vector<T>& src = which ? vel : pos;
+                for (int64_t i = 0; i < n; ++i) comp[(size_t)i] = src[(size_t)(i * D + d)];
+                S_TRY(hipMemcpy(which ? sp_[q].vel[d] : sp_[q].pos[d], comp.data(), (size_t)n * sizeof(T),
+                                hipMemcpyHostToDevice));
+            }
+        sp_[q].n = n;
+    }
+
+    void save_checkpoint(const char* path)
+    {
+        using psum::serialization::mas_file;
+        S_TRY(hipStreamSynchronize(st_));
+        int dev = 0;
+        S_TRY(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        S_TRY(hipGetDeviceProperties(&prop, dev));
+        const std::string devname = std::string("[#0]") + prop.name;
+        mas_file f(path, mas_file::replaceMode);
+        const int D = cfg_.grid.dim;
+        f.writeScalar<int64_t>("psum.step", step_);
+        f.writeScalar<int32_t>("psum.dtype", cfg_.dtype);
+        f.writeScalar<int32_t>("psum.nspecies", cfg_.nspecies);
+        f.writeScalar<int32_t>("psum.mg_predicted", mg_predicted(mg_));
+        f.writeScalar<int32_t>("psum.adjoint_predicted", adj_predicted_);
+        f.writeScalar<int32_t>("psum.have_prev", have_prev_ ? 1 : 0);
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            if (cfg_.dtype == PSUM_F64) save_species<double>(f, q, devname);
+            else save_species<float>(f, q, devname);
+        }
+        std::vector<double> lower(D), upper(D);
+        std::vector<int32_t> cells(D);
+        for (int d = 0; d < D; ++d) {
+            lower[d] = cfg_.grid.lb[d];
+            upper[d] = cfg_.grid.ub[d];
+            cells[d] = cfg_.grid.cells[d];
+        }
+        f.writeScalar<int32_t>("phi.location", 0);   // node centred
+        f.writeData<double>("phi.grid.lower", lower.data(), {(size_t)D, 1});
+        f.writeData<double>("phi.grid.upper", upper.data(), {(size_t)D, 1});
+        f.writeData<int32_t>("phi.grid.cells", cells.data(), {(size_t)D, 1});
+        std::vector<double> h((size_t)nn_);
+        S_TRY(hipMemcpy(h.data(), phi_, (size_t)nn_ * sizeof(double), hipMemcpyDeviceToHost));
+        f.writeData("phi.content.device", devname);
+        f.writeData<double>("phi.content.content", h.data(), {(size_t)nn_, 1});
+        if (cfg_.extrapolate && have_prev_) {
+            S_TRY(hipMemcpy(h.data(), phi_prev_, (size_t)nn_ * sizeof(double), hipMemcpyDeviceToHost));
+            f.writeData<double>("psum.phi_prev", h.data(), {(size_t)nn_, 1});
+        }
+    }
+
+    void restore_checkpoint(const char* path)
+    {
+        using psum::serialization::mas_file;
+        mas_file f(path, mas_file::autoMode);
+        auto scalar_i32 = [&](const char* k) { return f.read<int32_t>(k).at(0); };
+        if (scalar_i32("psum.dtype") != cfg_.dtype) throw std::invalid_argument("checkpoint: dtype differs");
+        if (scalar_i32("psum.nspecies") != cfg_.nspecies)
+            throw std::invalid_argument("checkpoint: species count differs");
+        const std::vector<int32_t> cells = f.read<int32_t>("phi.grid.cells");
+        if ((int)cells.size() != cfg_.grid.dim) throw std::invalid_argument("checkpoint: grid dimension differs");
+        for (int d = 0; d < cfg_.grid.dim; ++d)
+            if (cells[d] != cfg_.grid.cells[d]) throw std::invalid_argument("checkpoint: grid differs");
+        const std::vector<double> phi = f.read<double>("phi.content.content");
+        if ((int64_t)phi.size() != nn_) throw std::invalid_argument("checkpoint: phi size differs");
+        S_TRY(hipStreamSynchronize(st_));
+        for (int q = 0; q < cfg_.nspecies; ++q) {
+            if (cfg_.dtype == PSUM_F64) restore_species<double>(f, q);
+            else restore_species<float>(f, q);
+        }
+        S_TRY(hipMemcpy(phi_, phi.data(), (size_t)nn_ * sizeof(double), hipMemcpyHostToDevice));
+        have_prev_ = scalar_i32("psum.have_prev") != 0;
+        if (cfg_.extrapolate && have_prev_) {
+            const std::vector<double> prev = f.read<double>("psum.phi_prev");
+            if ((int64_t)prev.size() != nn_) throw std::invalid_argument("checkpoint: phi_prev size differs");
+            S_TRY(hipMemcpy(phi_prev_, prev.data(), (size_t)nn_ * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            have_prev_ = false;
+        }
+        mg_set_predicted(mg_, scalar_i32("psum.mg_predicted"));
+        adj_predicted_ = scalar_i32("psum.adjoint_predicted");
+        step_ = f.read<int64_t>("psum.step").at(0);
+        acc_valid_ = false;
+    }
+
     // lrho = S^T lphi (lphi_ is overwritten).  S^T is the homogeneous solve
     // of the same pinned / Dirichlet system (rows of pinned and fixed nodes
     // take 0; no addback), warm started from the previous lrho.  Its result
@@ -607,6 +744,7 @@ public:
 
     double record(int steps, int ckpt, double* t_ms)
     {
+        require_cic("psum_stepper_record");
         if (steps < 0 || ckpt < 1) throw std::invalid_argument("psum_stepper_record: steps >= 0, ckpt >= 1");
         release_record();
         adj_buffers();
@@ -636,6 +774,7 @@ public:
     // seed: dL_total/dL (the upstream gradient); gx/gv[s*3+d]: device outputs
     void reverse(double seed, void* const* gx, void* const* gv, double* t_ms)
     {
+        require_cic("psum_stepper_reverse");
         if (!recorded_) throw std::invalid_argument("psum_stepper_reverse: no recorded forward");
         const int D = cfg_.grid.dim;
         const int steps = rec_steps_, ckpt = rec_ckpt_;
@@ -746,6 +885,16 @@ public:
     void set_sort_shift(int b) { sort_shift_ = b; }
     int sort_shift() const { return sort_shift_; }
     void set_sort_every(int k) { cfg_.sort_every = k; }
+    void set_interp_order(int o)
+    {
+        if (o < 1 || o > 4) throw std::invalid_argument("psum_stepper_set_interp_order: order must be 1..4");
+        order_ = o;
+        acc_valid_ = false;
+    }
+    void require_cic(const char* what) const
+    {
+        if (order_ != 1) throw std::invalid_argument(std::string(what) + ": the adjoint covers CIC (order 1) only");
+    }
     int sort_every() const { return cfg_.sort_every; }
     void set_stable_buffers(bool on) { stable_buffers_ = on; }
     void set_fixed_cycles(int k)
@@ -958,6 +1107,7 @@ private:
     unsigned long long* nvalid_ = nullptr;
     bool acc_valid_ = false;
     int64_t step_ = 0;
+    int order_ = 1;   // interpolation order: 1 CIC, 2..4 B-splines
     int64_t cycles_ = 0;
     double last_rel_ = NAN;
 
@@ -975,7 +1125,11 @@ private:
         hipLaunchKernelGGL(k_zero_i64, dim3(nblk(nn_)), dim3(kT), 0, st_, (long long*)acc_, nn_);
         for (int q = 0; q < cfg_.nspecies; ++q) {
             Species& sp = sp_[q];
-            CHK(psum_hip_deposit_cic(&cfg_.grid, cfg_.dtype, sp.pos, sp.n, sp.wdep, scale_, acc_, counters_, st_));
+            if (order_ > 1)
+                CHK(psum_hip_deposit_bspline(&cfg_.grid, cfg_.dtype, order_, sp.pos, sp.n, sp.wdep, scale_, acc_,
+                                             counters_, st_));
+            else
+                CHK(psum_hip_deposit_cic(&cfg_.grid, cfg_.dtype, sp.pos, sp.n, sp.wdep, scale_, acc_, counters_, st_));
         }
         acc_valid_ = true;
     }
@@ -1301,6 +1455,24 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
         s->reverse(1.0, grad_x, grad_v, t_bwd_ms);
         if (loss) *loss = l;
     });
+}
+
+psum_status psum_stepper_set_interp_order(void* h, int order)
+{
+    if (!h) { set_error("psum_stepper_set_interp_order: null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->set_interp_order(order); });
+}
+
+psum_status psum_stepper_save(void* h, const char* path)
+{
+    if (!h || !path) { set_error("psum_stepper_save: null argument"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->save_checkpoint(path); });
+}
+
+psum_status psum_stepper_restore(void* h, const char* path)
+{
+    if (!h || !path) { set_error("psum_stepper_restore: null argument"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->restore_checkpoint(path); });
 }
 
 void psum_stepper_destroy(void* h) { delete static_cast<Stepper*>(h); }

@@ -14,6 +14,7 @@ own device memory and provide the stream.  Every call goes through
 libpsum_hip.so; there is no CPU path.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import torch
@@ -25,7 +26,7 @@ from ._lib import (KEY_INVALID, PSUM_F32, PSUM_F64, FuncTable, PsumError, PsumGr
 
 __all__ = ["make_grid", "periodic_grid", "wall_grid", "num_nodes", "fx_shift", "cell_keys",
            "stable_sort", "permute", "compact", "deposit", "fixed_to_double", "gather_push",
-           "field_energy", "SolverBackend", "Stepper", "Batch", "step_config", "KEY_INVALID",
+           "field_energy", "deposit_bspline", "gather_push_bspline", "SolverBackend", "Stepper", "Batch", "step_config", "KEY_INVALID",
            "PsumError",
            "get_solver_funcs"]
 
@@ -212,6 +213,41 @@ def gather_push(g, pos, vel, phi, qm, dt, mass=1.0, w_dep=None, scale=None, coun
         check(lib().psum_hip_reduce_diag(_p(diag), nb, _p(out), _p(work), _stream()),
               "psum_hip_reduce_diag")
     return acc, out
+
+
+def deposit_bspline(g, pos, w, order, scale=None, acc=None, counters=None):
+    """B-spline deposit (order 2..4; 1 = CIC) as deposit() with the
+    higher-order weights; returns (acc int64 tensor, scale)."""
+    n = _check_soa(g, pos)
+    if scale is None:
+        scale = float(np.ldexp(1.0, fx_shift(w)))
+    if acc is None:
+        acc = torch.zeros(num_nodes(g), dtype=torch.int64, device=pos[0].device)
+    check(lib().psum_hip_deposit_bspline(C.byref(g), _dtype_code(pos[0]), int(order), _ptrs(pos), n, w,
+                                         scale, _p(acc), _p(counters), _stream()),
+          "psum_hip_deposit_bspline")
+    return acc, scale
+
+
+def gather_push_bspline(g, pos, vel, phi, qm, dt, order, mass=1.0, counters=None):
+    """B-spline gather + kick + drift + boundaries in place (order 2..4; 1 =
+    CIC); returns diag as gather_push does."""
+    n = _check_soa(g, pos)
+    _check_soa(g, vel)
+    dev = pos[0].device
+    if phi.dtype != pos[0].dtype:
+        raise PsumError("phi must have the particle dtype")
+    nb = lib().psum_hip_push_blocks(n)
+    diag = torch.zeros(max(1, nb) * 4, dtype=torch.float64, device=dev)
+    check(lib().psum_hip_gather_push_bspline(C.byref(g), _dtype_code(pos[0]), int(order), _ptrs(pos),
+                                             _ptrs(vel), n, _p(phi), qm, dt, mass, _p(diag), _p(counters),
+                                             _stream()), "psum_hip_gather_push_bspline")
+    out = torch.zeros(4, dtype=torch.float64, device=dev)
+    if nb > 0:
+        work = torch.empty(lib().psum_hip_reduce_workspace() // 8, dtype=torch.float64, device=dev)
+        check(lib().psum_hip_reduce_diag(_p(diag), nb, _p(out), _p(work), _stream()),
+              "psum_hip_reduce_diag")
+    return out
 
 
 def field_energy(g, phi, eps=1.0):
@@ -464,7 +500,7 @@ class Stepper:
     def __init__(self, g, species, dtype="f64", mixed_mg=False, depth=4, pre=2, post=1,
                  max_cycles=20, tol=1e-10, eps=1.0, dt=0.1, rho_bg=0.0, sort_every=0,
                  fixed=None, fixed_val=None, capacity=None, mg_graph=True, extrapolate=False,
-                 _handle=None):
+                 interp_order=1, _handle=None):
         self.g = g
         self.dim = g.dim
         self.np_dtype = np.float64 if dtype == "f64" else np.float32
@@ -480,8 +516,15 @@ class Stepper:
         if not self.h:
             raise PsumError("psum_stepper_create: " +
                             lib().psum_hip_last_error().decode(errors="replace"))
+        if interp_order != 1:
+            self.set_interp_order(interp_order)
         for i, sp in enumerate(species):
             self.load(i, sp["pos"], sp["vel"])
+
+    def set_interp_order(self, order):
+        """1 = CIC (fused push + deposit), 2..4 = quadratic / cubic / quartic
+        B-splines (TSI's interp_order)"""
+        check(lib().psum_stepper_set_interp_order(self.h, int(order)), "set_interp_order")
 
     def tune(self, warmup=20, probe=10):
         """Autotune (psum_stepper_tune): dict with the calibrated fixed cycle
@@ -613,6 +656,14 @@ class Stepper:
         check(lib().psum_stepper_push_timing(self.h, C.byref(ms), C.byref(nl), C.byref(npart)),
               "push_timing")
         return ms.value, nl.value, npart.value
+
+    def save(self, path):
+        """write a MAS checkpoint (particles, phi, step counter and solver
+        state); restore() continues the run bit-identically"""
+        check(lib().psum_stepper_save(self.h, os.fspath(path).encode()), "save")
+
+    def restore(self, path):
+        check(lib().psum_stepper_restore(self.h, os.fspath(path).encode()), "restore")
 
     def close(self):
         if getattr(self, "h", None):

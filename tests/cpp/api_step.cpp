@@ -7,6 +7,7 @@
 // Prints one line per check and "ALL OK" at the end; exit status 0 on success.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -299,6 +300,46 @@ int main()
         report("Poisson_solver_2d: cylindrical axis value", rel < 5e-4, rel);
         psum::hip_check(hipFree(dphi_d), "f");
         psum::hip_check(hipFree(dsrc), "f");
+    }
+
+    // ---- checkpoint round trip (MAS): a particle group and a node field
+    {
+        const std::string path = "/tmp/psum_api_step_ck.mas";
+        {
+            mas_file f(path, mas_file::replaceMode);
+            psum::serialization::save(f, "electrons", group);
+            psum::serialization::save(f, "rho", rho);
+        }
+        mas_file f(path);
+        Group g2(st, N);
+        psum::serialization::load(f, "electrons", g2);
+        node_field2D<double> rho2(st, grid);
+        psum::serialization::load(f, "rho", rho2);
+        const auto a = group.get_content().to_host(), b = g2.get_content().to_host();
+        bool same_p = a.size() == b.size();
+        size_t first_diff = a.size();
+        for (size_t i = 0; same_p && i < a.size(); ++i)
+            if (std::memcmp(&a[i], &b[i], sizeof(Particle)) != 0) {
+                same_p = false;
+                first_diff = i;
+            }
+        const auto r1 = rho.to_host(), r2 = rho2.to_host();
+        const bool same_f = r1 == r2;
+        std::vector<size_t> shp;
+        const auto posblk = f.read<double>("electrons.data.content.position", &shp);
+        const bool same_b = shp.size() == 3 && shp[0] == a.size() && shp[1] == 2 && shp[2] == 1 &&
+                            posblk[2 * 5 + 1] == get<position>(a[5]).y();
+        if (!same_p && first_diff < a.size() && first_diff < b.size())
+            std::printf("  record %zu: saved (%.17g %.17g | %.17g %.17g) loaded (%.17g %.17g | %.17g %.17g)\n",
+                        first_diff, get<position>(a[first_diff]).x(), get<position>(a[first_diff]).y(),
+                        get<velocity>(a[first_diff]).x(), get<velocity>(a[first_diff]).y(),
+                        get<position>(b[first_diff]).x(), get<position>(b[first_diff]).y(),
+                        get<velocity>(b[first_diff]).x(), get<velocity>(b[first_diff]).y());
+        if (!(same_p && same_f && same_b))
+            std::printf("  particles %d (sizes %zu %zu, first difference %zu), field %d (%zu %zu), blocks %d\n",
+                        (int)same_p, a.size(), b.size(), first_diff, (int)same_f, r1.size(), r2.size(), (int)same_b);
+        report("MAS save/load of particle_group and field", same_p && same_f && same_b, (double)a.size());
+        std::remove(path.c_str());
     }
 
     std::printf(fails ? "FAILED %d\n" : "ALL OK\n", fails);

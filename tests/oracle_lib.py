@@ -117,6 +117,22 @@ def lib():
         L.orc_poisson_assemble_ex.restype = C.c_int
         L.orc_poisson_rhs.argtypes = [i64, P, i64, P, P, P, P, i64, P, P, P]
         L.orc_poisson_plant.argtypes = [C.c_int]
+        L.orc_bspline_1d.argtypes = [C.c_int, C.c_double, P, P]
+        L.orc_bspline_weights.argtypes = [C.POINTER(Grid), C.c_int, P, P, P, P]
+        L.orc_bspline_weights.restype = C.c_int
+        for nm in ("orc_deposit_bs_f64", "orc_deposit_bs_f32"):
+            getattr(L, nm).argtypes = [C.POINTER(Grid), C.c_int, P, i64, C.c_double, C.c_double, P]
+            getattr(L, nm).restype = i64
+        L.orc_deposit_bs_ref.argtypes = [C.POINTER(Grid), C.c_int, P, i64, C.c_double, P]
+        L.orc_deposit_bs_ref.restype = i64
+        L.orc_gather_push_bs_f64.argtypes = [C.POINTER(Grid), C.c_int, P, P, i64, P, C.c_double,
+                                             C.c_double, C.c_double, P]
+        L.orc_gather_push_bs_f64.restype = i64
+        L.orc_gather_push_bs_f32.argtypes = [C.POINTER(Grid), C.c_int, P, P, i64, P, C.c_float,
+                                             C.c_float, C.c_double, P]
+        L.orc_gather_push_bs_f32.restype = i64
+        L.orc_bspline_plant.argtypes = [C.c_int]
+        L.orc_pic_set_order.argtypes = [C.c_int]
         L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int]
         L.orc_mg_is_singular.argtypes = [P, P]
@@ -396,6 +412,63 @@ def poisson_rhs(source, M, mixed_values, fix_idx=(), fix_val=()):
     lib().orc_poisson_rhs(src.size, _ptr(src), mr.size, _ptr(mr), _ptr(mc), _ptr(mv), _ptr(mvv),
                           fi.size, _ptr(fi), _ptr(fv), _ptr(b))
     return b
+
+
+def bspline_1d(order, s):
+    c = np.zeros(5)
+    dc = np.zeros(5)
+    lib().orc_bspline_1d(order, s, _ptr(c), _ptr(dc))
+    return c[:order + 1], dc[:order + 1]
+
+
+def bspline_weights(g, order, x):
+    """(idx, coef, gcoef[D, S]) of the support of x, or None out of range"""
+    xx = np.zeros(3)
+    xx[:len(x)] = x
+    idx = np.zeros(125, np.int64)
+    coef = np.zeros(125)
+    gc = np.zeros(375)
+    S = lib().orc_bspline_weights(C.byref(g), order, _ptr(xx), _ptr(idx), _ptr(coef), _ptr(gc))
+    if S == 0:
+        return None
+    return idx[:S].copy(), coef[:S].copy(), gc.reshape(3, 125)[:g.dim, :S].copy()
+
+
+def deposit_bs(g, order, pos, w, scale):
+    dt = pos[0].dtype
+    pos = [np.ascontiguousarray(a) for a in pos]
+    acc = np.zeros(num_nodes(g), np.int64)
+    f = lib().orc_deposit_bs_f32 if dt == np.float32 else lib().orc_deposit_bs_f64
+    skipped = f(C.byref(g), order, C.cast(_ptr_array(pos), C.c_void_p), pos[0].size, w, scale, _ptr(acc))
+    return acc, skipped
+
+
+def deposit_bs_ref(g, order, pos, w):
+    pos = [np.ascontiguousarray(a, np.float64) for a in pos]
+    rho = np.zeros(num_nodes(g))
+    lib().orc_deposit_bs_ref(C.byref(g), order, C.cast(_ptr_array(pos), C.c_void_p), pos[0].size, w, _ptr(rho))
+    return rho
+
+
+def gather_push_bs(g, order, pos, vel, phi, qm, dt, mass=1.0):
+    """in place on copies; returns (pos, vel, absorbed, diag)"""
+    pos = [np.array(a) for a in pos]
+    vel = [np.array(a) for a in vel]
+    t = pos[0].dtype
+    diag = np.zeros(4)
+    phi = np.ascontiguousarray(phi, t)
+    f = lib().orc_gather_push_bs_f32 if t == np.float32 else lib().orc_gather_push_bs_f64
+    ab = f(C.byref(g), order, C.cast(_ptr_array(pos), C.c_void_p), C.cast(_ptr_array(vel), C.c_void_p),
+           pos[0].size, _ptr(phi), qm, dt, mass, _ptr(diag))
+    return pos, vel, ab, diag
+
+
+def pic_set_order(order):
+    lib().orc_pic_set_order(order)
+
+
+def bspline_plant(which):
+    lib().orc_bspline_plant(which)
 
 
 def poisson_plant(which):
