@@ -54,29 +54,41 @@ struct tag_bind {
 };
 
 namespace detail {
+// the last binding has no tail member, so a record is exactly its bound
+// types in order, with no empty sub-object padding it out
 template <typename... B>
 struct record;
 template <>
 struct record<> {};
-template <typename B0, typename... B>
-struct record<B0, B...> {
+template <typename B0>
+struct record<B0> {
     typename B0::type head{};
-    record<B...> tail;
+};
+template <typename B0, typename B1, typename... B>
+struct record<B0, B1, B...> {
+    typename B0::type head{};
+    record<B1, B...> tail;
 };
 
 template <typename Tag, typename R>
 struct finder;
-template <typename Tag, typename B0, typename... B>
-struct finder<Tag, record<B0, B...>> {
-    __host__ __device__ static auto& get(record<B0, B...>& r)
+template <typename Tag, typename B0>
+struct finder<Tag, record<B0>> {
+    static_assert(std::is_same_v<Tag, typename B0::tag>, "tag not bound in this record");
+    __host__ __device__ static auto& get(record<B0>& r) { return r.head; }
+    __host__ __device__ static const auto& get(const record<B0>& r) { return r.head; }
+};
+template <typename Tag, typename B0, typename B1, typename... B>
+struct finder<Tag, record<B0, B1, B...>> {
+    __host__ __device__ static auto& get(record<B0, B1, B...>& r)
     {
         if constexpr (std::is_same_v<Tag, typename B0::tag>) return r.head;
-        else return finder<Tag, record<B...>>::get(r.tail);
+        else return finder<Tag, record<B1, B...>>::get(r.tail);
     }
-    __host__ __device__ static const auto& get(const record<B0, B...>& r)
+    __host__ __device__ static const auto& get(const record<B0, B1, B...>& r)
     {
         if constexpr (std::is_same_v<Tag, typename B0::tag>) return r.head;
-        else return finder<Tag, record<B...>>::get(r.tail);
+        else return finder<Tag, record<B1, B...>>::get(r.tail);
     }
 };
 }  // namespace detail

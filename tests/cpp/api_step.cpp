@@ -318,6 +318,7 @@ int main()
         const auto a = group.get_content().to_host(), b = g2.get_content().to_host();
         bool same_p = a.size() == b.size();
         size_t first_diff = a.size();
+        static_assert(sizeof(Particle) == 4 * sizeof(double), "records carry no padding");
         for (size_t i = 0; same_p && i < a.size(); ++i)
             if (std::memcmp(&a[i], &b[i], sizeof(Particle)) != 0) {
                 same_p = false;
