@@ -232,6 +232,12 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
                                  void* const* grad_v, double* t_fwd_ms, double* t_bwd_ms);
 void psum_stepper_destroy(void* h);
 
+/* Sort every species now, as the step does every sort_every steps: stable
+ * LSD radix sort by the key bin * 2^dim + sum_d [v_d >= 0] 2^(dim-1-d), bin =
+ * the 2^sort_shift-cell bin of x (the cell key of psum_hip_bin at shift 0);
+ * absorbed particles are dropped.  No reference counterpart. */
+psum_status psum_stepper_sort(void* h);
+
 /* Interpolation order of the step (TSI's interp_order, TSI/main.cpp:29,54,
  * 64): 1 = CIC (default, fused push + deposit), 2..4 = B-splines through
  * psum_hip_gather_push_bspline / psum_hip_deposit_bspline.  The adjoint

@@ -138,6 +138,7 @@ SIGNATURES = [
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psum_stepper_destroy", None, [P]),
     ("psum_stepper_set_interp_order", ST, [P, C.c_int]),
+    ("psum_stepper_sort", ST, [P]),
     ("psum_stepper_save", ST, [P, C.c_char_p]),
     ("psum_stepper_restore", ST, [P, C.c_char_p]),
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),

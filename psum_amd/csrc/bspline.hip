@@ -145,6 +145,8 @@ __device__ __forceinline__ bool support_of(const DevGrid& g, const double* x, in
 
 // node of support offset k in dimension d, wrapped once
 __device__ __forceinline__ int wrap_node(int n, int w) { return n < 0 ? n + w : (n >= w ? n - w : n); }
+// any node coordinate of an unwrapped box back into [0, w)
+__device__ __forceinline__ int mod_node(int n, int w) { return ((n % w) + w) % w; }
 
 // ------------------------------------------------------------ gather + push
 
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
     constexpr int NS = D == 2 ? S * S : S * S * S;
     constexpr int shift = Support<ORDER>::shift;
     __shared__ unsigned long long tile[kBsTileNodes];
-    __shared__ int bmin[3], bmax[3];
+    __shared__ int bmin[3], bmax[3], bref[3];
     const T* const xs[3] = {x0, x1, x2};
     const int lane = threadIdx.x & 63;
     if (threadIdx.x < 3) {
@@ -313,17 +315,37 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
             continue;
         }
         ok[j] = true;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            lmin[d] = min(lmin[d], base[j][d]);
-            lmax[d] = max(lmax[d], base[j][d]);
-        }
     }
     if (counters) {
         const int oo = wave_isum((int)oor);
         if (lane == 0 && oo) atomicAdd(&counters[0], (unsigned long long)oo);
     }
+    // box over base nodes unwrapped around thread 0's first particle on
+    // periodic axes (a block straddling the seam keeps a small box)
+    if (threadIdx.x == 0) {
+        int r[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+        for (int j = kBsPPT - 1; j >= 0; --j)
+            if (ok[j])
+#pragma unroll
+                for (int d = 0; d < D; ++d) r[d] = base[j][d];
+#pragma unroll
+        for (int d = 0; d < D; ++d) bref[d] = r[d];
+    }
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBsPPT; ++j)
+        if (ok[j])
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (g.periodic[d] && bref[d] != INT_MIN) {
+                    const int w = g.w[d];
+                    if (base[j][d] - bref[d] > w / 2) base[j][d] -= w;
+                    else if (bref[d] - base[j][d] > w / 2) base[j][d] += w;
+                }
+                lmin[d] = min(lmin[d], base[j][d]);
+                lmax[d] = max(lmax[d], base[j][d]);
+            }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int mn = wave_min(lmin[d]), mx = wave_max(lmax[d]);
@@ -372,7 +394,7 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
                 uint32_t lin = 0;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const int nd = wrap_node(base[j][d] + k[d] + shift, g.w[d]);
+                    const int nd = mod_node(base[j][d] + k[d] + shift, g.w[d]);
                     lin = d == 0 ? (uint32_t)nd : lin * (uint32_t)g.w[d] + (uint32_t)nd;
                 }
                 atomicAdd(&acc[lin], (unsigned long long)term);
@@ -388,7 +410,7 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
             int nd[3];
 #pragma unroll
             for (int d = D - 1; d >= 0; --d) {
-                nd[d] = wrap_node(lo[d] + rem % e[d], g.w[d]);
+                nd[d] = mod_node(lo[d] + rem % e[d], g.w[d]);
                 rem /= e[d];
             }
             uint32_t lin = (uint32_t)nd[0];

@@ -13,9 +13,11 @@ namespace psum_hip {
 void set_error(const std::string& msg);
 
 // Sort keys of 2^shift-cell bins (shift 0: the cell key of psum_hip_bin)
-// and the number of bins.
+// and the number of bins.  With vel != NULL the key is refined by the signs
+// of the velocity components, key = bin * 2^D + sum_d [v_d >= 0] 2^(D-1-d),
+// so that particles of one bin moving the same way stay adjacent.
 psum_status bin_keys(const psum_grid* g, int dtype, void* const* pos, int64_t n, uint32_t* keys,
-                     int shift, void* stream);
+                     int shift, void* stream, void* const* vel = nullptr);
 uint32_t bin_count(const psum_grid& g, int shift);
 
 #define PSUM_HIP_TRY(expr)                                                              \

@@ -46,11 +46,12 @@ inline unsigned grid_blocks(int64_t n, int64_t per_block)
 
 // ------------------------------------------------------------------ binning
 
-template <typename T, int D>
+template <typename T, int D, bool VEL>
 __global__ __launch_bounds__(kThreads) void k_bin(DevGrid g, const T* __restrict__ x0,
                                                   const T* __restrict__ x1,
-                                                  const T* __restrict__ x2, int64_t n,
-                                                  uint32_t* __restrict__ keys, int shift)
+                                                  const T* __restrict__ x2, const T* __restrict__ v0,
+                                                  const T* __restrict__ v1, const T* __restrict__ v2,
+                                                  int64_t n, uint32_t* __restrict__ keys, int shift)
 {
     const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (p >= n) return;
@@ -68,6 +69,11 @@ __global__ __launch_bounds__(kThreads) void k_bin(DevGrid g, const T* __restrict
 #pragma unroll
         for (int d = 1; d < D; ++d)
             lin = lin * (uint32_t)((g.w[d] + (1 << shift) - 1) >> shift) + (uint32_t)(cw[d] >> shift);
+        if constexpr (VEL) {
+            const T* const vs[3] = {v0, v1, v2};
+#pragma unroll
+            for (int d = 0; d < D; ++d) lin = (lin << 1) | (vs[d][p] >= (T)0 ? 1u : 0u);
+        }
         key = lin;
     }
     keys[p] = key;
@@ -415,7 +421,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
     double* __restrict__ diag, unsigned long long* __restrict__ counters)
 {
     __shared__ unsigned long long tile[DEPOSIT ? kTileNodes : 1];
-    __shared__ int bmin[3], bmax[3];
+    __shared__ int bmin[3], bmax[3], bref[3];
     __shared__ double wdiag[kThreads / 64][4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (DEPOSIT && threadIdx.x < 3) {
@@ -518,17 +524,41 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
     }
 
     if (DEPOSIT) {
+        // the tile box is taken over cells unwrapped around a reference cell
+        // (thread 0's first particle) on periodic axes, so a block whose
+        // particles straddle the periodic seam still gets a small box
+        if (threadIdx.x == 0) {
+            int r[3] = {-1, -1, -1};
+#pragma unroll
+            for (int j = kPPT - 1; j >= 0; --j)
+                if (ok[j])
+#pragma unroll
+                    for (int d = 0; d < D; ++d) r[d] = cw[j][d];
+#pragma unroll
+            for (int d = 0; d < D; ++d) bref[d] = r[d];
+        }
+        __syncthreads();   // bmin/bmax initialised, bref set
+        int ref[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) ref[d] = bref[d];
+        int uw[kPPT][D];
         int lmin[3] = {INT_MAX, INT_MAX, INT_MAX}, lmax[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
         for (int j = 0; j < kPPT; ++j)
             if (ok[j]) {
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    lmin[d] = min(lmin[d], cw[j][d]);
-                    lmax[d] = max(lmax[d], cw[j][d]);
+                    int u = cw[j][d];
+                    if (g.periodic[d] && ref[d] >= 0) {
+                        const int w = g.w[d];
+                        if (u - ref[d] > w / 2) u -= w;
+                        else if (ref[d] - u > w / 2) u += w;
+                    }
+                    uw[j][d] = u;
+                    lmin[d] = min(lmin[d], u);
+                    lmax[d] = max(lmax[d], u);
                 }
             }
-        __syncthreads();   // bmin/bmax initialised
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int mn = wave_min(lmin[d]), mx = wave_max(lmax[d]);
@@ -555,7 +585,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         constexpr int NS = 1 << D;
         const double wsc = w_dep * scale;
         long long run[NS];
-        int cur[D];
+        int cur[D], curu[D];
         bool have = false;
         auto flush = [&]() {
 #pragma unroll
@@ -566,7 +596,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 if (use_lds) {
                     int li = 0;
 #pragma unroll
-                    for (int d = 0; d < D; ++d) li = li * e[d] + (cur[d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
+                    for (int d = 0; d < D; ++d) li = li * e[d] + (curu[d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
                     atomicAdd(&tile[li], (unsigned long long)t);
                 } else {
                     atomicAdd(&acc[support_index<D>(g, cur, i)], (unsigned long long)t);
@@ -582,7 +612,10 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
             if (!same) {
                 if (have) flush();
 #pragma unroll
-                for (int d = 0; d < D; ++d) cur[d] = cw[j][d];
+                for (int d = 0; d < D; ++d) {
+                    cur[d] = cw[j][d];
+                    curu[d] = uw[j][d];
+                }
 #pragma unroll
                 for (int q = 0; q < NS; ++q) run[q] = 0;
                 have = true;
@@ -614,7 +647,8 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 for (int d = D - 1; d >= 0; --d) {
                     nd[d] = lo[d] + rem % e[d];
                     rem /= e[d];
-                    if (nd[d] == g.w[d]) nd[d] = 0;
+                    if (nd[d] < 0) nd[d] += g.w[d];
+                    else if (nd[d] >= g.w[d]) nd[d] -= g.w[d];
                 }
                 uint32_t lin = (uint32_t)nd[0];
 #pragma unroll
@@ -831,19 +865,29 @@ uint32_t bin_count(const psum_grid& g, int shift)
 }
 
 psum_status bin_keys(const psum_grid* g, int dtype, void* const* pos, int64_t n, uint32_t* keys,
-                     int shift, void* stream)
+                     int shift, void* stream, void* const* vel)
 {
     PSUM_ARG_CHECK(grid_ok(g), "psum_hip_bin: invalid grid");
     PSUM_ARG_CHECK(shift >= 0 && shift <= 8, "psum_hip_bin: bin shift out of range");
     PSUM_ARG_CHECK(n >= 0 && (n == 0 || (pos && keys)), "psum_hip_bin: null pointer");
+    if (vel) PSUM_ARG_CHECK(((uint64_t)bin_count(*g, shift) << g->dim) < (uint64_t)PSUM_KEY_INVALID,
+                            "psum_hip_bin: too many bins for velocity-refined keys");
     if (n == 0) return PSUM_OK;
     const DevGrid dg = to_dev(*g);
     const unsigned nb = grid_blocks(n, kThreads);
-#define PSUM_BIN(T, DIM)                                                                       \
-    hipLaunchKernelGGL((k_bin<T, DIM>), dim3(nb), dim3(kThreads), 0, S(stream), dg,             \
-                       static_cast<const T*>(pos[0]),                                           \
-                       DIM > 1 ? static_cast<const T*>(pos[1]) : nullptr,                       \
-                       DIM > 2 ? static_cast<const T*>(pos[2]) : nullptr, n, keys, shift)
+#define PSUM_BIN(T, DIM)                                                                         \
+    do {                                                                                         \
+        const T* xp[3] = {static_cast<const T*>(pos[0]), DIM > 1 ? static_cast<const T*>(pos[1]) : nullptr, \
+                          DIM > 2 ? static_cast<const T*>(pos[2]) : nullptr};                     \
+        if (vel)                                                                                 \
+            hipLaunchKernelGGL((k_bin<T, DIM, true>), dim3(nb), dim3(kThreads), 0, S(stream), dg, xp[0], \
+                               xp[1], xp[2], static_cast<const T*>(vel[0]),                      \
+                               DIM > 1 ? static_cast<const T*>(vel[1]) : nullptr,                \
+                               DIM > 2 ? static_cast<const T*>(vel[2]) : nullptr, n, keys, shift); \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_bin<T, DIM, false>), dim3(nb), dim3(kThreads), 0, S(stream), dg, xp[0], \
+                               xp[1], xp[2], nullptr, nullptr, nullptr, n, keys, shift);        \
+    } while (0)
     if (dtype == PSUM_F64) {
         if (g->dim == 1) PSUM_BIN(double, 1); else if (g->dim == 2) PSUM_BIN(double, 2); else PSUM_BIN(double, 3);
     } else if (dtype == PSUM_F32) {

@@ -885,6 +885,7 @@ public:
     void set_sort_shift(int b) { sort_shift_ = b; }
     int sort_shift() const { return sort_shift_; }
     void set_sort_every(int k) { cfg_.sort_every = k; }
+    void invalidate_deposit() { acc_valid_ = false; }
     void set_interp_order(int o)
     {
         if (o < 1 || o > 4) throw std::invalid_argument("psum_stepper_set_interp_order: order must be 1..4");
@@ -1139,11 +1140,16 @@ public:
     {
         const int D = cfg_.grid.dim;
         sort_buffers();
-        const uint32_t nkeys = sort_shift_ ? bin_count(cfg_.grid, sort_shift_) : (uint32_t)nn_;
+        // keys: the 2^shift-cell bin refined by the velocity signs, so that
+        // co-moving particles stay adjacent between sorts (long equal-cell
+        // runs for the deposit even when beams interpenetrate)
+        const uint32_t nbins = sort_shift_ ? bin_count(cfg_.grid, sort_shift_) : (uint32_t)nn_;
+        const bool vel_keys = ((uint64_t)nbins << D) < (uint64_t)PSUM_KEY_INVALID;
+        const uint32_t nkeys = vel_keys ? nbins << D : nbins;
         for (int q = 0; q < cfg_.nspecies; ++q) {
             Species& sp = sp_[q];
             if (sp.n == 0) continue;
-            CHK(bin_keys(&cfg_.grid, cfg_.dtype, sp.pos, sp.n, keys_, sort_shift_, st_));
+            CHK(bin_keys(&cfg_.grid, cfg_.dtype, sp.pos, sp.n, keys_, sort_shift_, st_, vel_keys ? sp.vel : nullptr));
             CHK(psum_hip_sort(keys_, sp.n, nkeys, skeys_, perm_, sort_work_, sort_bytes_, st_));
             void* src[6];
             void* dst[6];
@@ -1454,6 +1460,16 @@ psum_status psum_stepper_adjoint(void* h, int steps, int ckpt, double* loss, voi
         const double l = s->record(steps, ckpt, t_fwd_ms);
         s->reverse(1.0, grad_x, grad_v, t_bwd_ms);
         if (loss) *loss = l;
+    });
+}
+
+psum_status psum_stepper_sort(void* h)
+{
+    if (!h) { set_error("psum_stepper_sort: null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() {
+        Stepper* s = static_cast<Stepper*>(h);
+        s->sort_all();
+        s->invalidate_deposit();
     });
 }
 

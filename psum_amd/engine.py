@@ -521,6 +521,10 @@ class Stepper:
         for i, sp in enumerate(species):
             self.load(i, sp["pos"], sp["vel"])
 
+    def sort(self):
+        """sort every species now (cell bin refined by velocity signs)"""
+        check(lib().psum_stepper_sort(self.h), "sort")
+
     def set_interp_order(self, order):
         """1 = CIC (fused push + deposit), 2..4 = quadratic / cubic / quartic
         B-splines (TSI's interp_order)"""

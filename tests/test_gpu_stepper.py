@@ -121,3 +121,24 @@ def test_bounded_3d_sort_compacts_absorbed():
     nan_left = int(np.isnan(gp[0]).sum() + np.isnan(gi[0]).sum())
     assert alive - nan_left == n0 - absorbed
     assert absorbed > 0 and alive < n0
+
+
+def test_sort_order_is_cell_then_velocity_signs():
+    """psum_stepper_sort: stable order by cell key * 2^D + velocity-sign bits
+    (dimension 0 the most significant), restated with the oracle's keys and
+    stable sort."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import langmuir_2d
+    wl = langmuir_2d(n=32, ppc=16, seed=8)
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f64", depth=2, dt=wl["dt"], rho_bg=wl["rho_bg"])
+    st.run(3)                                  # unsorted, moved state
+    p0, v0 = st.state(0)
+    og = _oracle_grid(wl["grid"])
+    ck = O.cell_keys(og, p0)
+    keys = (ck.astype(np.uint64) << 2) | ((v0[0] >= 0).astype(np.uint64) << 1) | (v0[1] >= 0).astype(np.uint64)
+    keys = np.where(ck == O.KEY_INVALID, np.uint64(O.KEY_INVALID), keys)
+    order = O.stable_sort(keys.astype(np.uint32), O.num_nodes(og) * 4)
+    st.sort()
+    p1, v1 = st.state(0)
+    for a, b in zip(p0 + v0, p1 + v1):
+        np.testing.assert_array_equal(a[order], b)
