@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -96,6 +97,10 @@ public:
         const bool exists = probe.good();
         probe.close();
         if (!exists && mode == quickMode) throw std::runtime_error("MAS: file not found: " + path);
+        if (!exists) {
+            const std::filesystem::path parent = std::filesystem::path(path).parent_path();
+            if (!parent.empty()) std::filesystem::create_directories(parent);
+        }
         if (!exists || mode == replaceMode) clear();
         else if (mode == autoMode) readHead();
     }
