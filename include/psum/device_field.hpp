@@ -10,6 +10,7 @@
 #include <cmath>
 #include <vector>
 
+#include "../psum_bspline.hpp"
 #include "../psum_cic.hpp"
 #include "device_vector.hpp"
 #include "grid.hpp"
@@ -164,6 +165,101 @@ __device__ void add_back(const Pos& pos, const Scalar& w,
     for (int i = 0; i < (1 << D); ++i) {
         const double c = psum_hip::cic_coef<double, D>(W, i);
         if (fabs(c) > 1e-10) atomicAdd(&acc.content[psum_hip::support_index<D>(acc.grid, cw, i)], w * c);
+    }
+}
+
+// ---- B-spline facade (higher_order_interp.hpp:196-248): Order 1 is the
+// CIC functions above; 2..4 the quadratic / cubic / quartic B-splines, with
+// FP64 weights, support order+1 nodes per dimension, and the same 1e-10 cut.
+namespace detail {
+template <int D, int O, typename F>
+__device__ void bspline_points(const psum_hip::DevGrid& g, const int* base, F&& f)
+{
+    using psum_hip::bspline::Support;
+    constexpr int S = O + 1;
+    constexpr int NS = D == 1 ? S : (D == 2 ? S * S : S * S * S);
+    for (int i = 0; i < NS; ++i) {
+        int k[3] = {0, 0, 0};
+        int t = i;
+        for (int j = 0; j < D; ++j) {
+            k[D - 1 - j] = t % S;
+            t /= S;
+        }
+        uint32_t lin = 0;
+        for (int d = 0; d < D; ++d) {
+            const int nd = psum_hip::bspline::wrap_node(base[d] + k[d] + Support<O>::shift, g.w[d]);
+            lin = d == 0 ? (uint32_t)nd : lin * (uint32_t)g.w[d] + (uint32_t)nd;
+        }
+        f(k, lin);
+    }
+}
+}  // namespace detail
+
+template <int Order, typename Pos, int D, typename Scalar>
+__device__ Scalar interp_B_spline(const Pos& pos, const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
+{
+    if constexpr (Order == 1) {
+        return interp(pos, acc);
+    } else {
+        double x[D], c[D][Order + 1];
+        int base[D];
+        for (int d = 0; d < D; ++d) x[d] = (double)pos[d];
+        double r = 0.0;
+        if (!psum_hip::bspline::support_of<D, Order, false>(acc.grid, x, base, c, nullptr)) return (Scalar)r;
+        detail::bspline_points<D, Order>(acc.grid, base, [&](const int* k, uint32_t lin) {
+            double cf = c[D - 1][k[D - 1]];
+            for (int j = 1; j < D; ++j) cf = cf * c[D - 1 - j][k[D - 1 - j]];
+            if (fabs(cf) > 1e-10) r += (double)acc.content[lin] * cf;
+        });
+        return (Scalar)r;
+    }
+}
+
+template <int Order, typename Pos, int D, typename Scalar>
+__device__ std::array<Scalar, D> interp_diff_B_spline(const Pos& pos,
+                                                      const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
+{
+    if constexpr (Order == 1) {
+        return interp_diff(pos, acc);
+    } else {
+        double x[D], c[D][Order + 1], dc[D][Order + 1], r[D] = {};
+        int base[D];
+        std::array<Scalar, D> out{};
+        for (int d = 0; d < D; ++d) x[d] = (double)pos[d];
+        if (!psum_hip::bspline::support_of<D, Order, true>(acc.grid, x, base, c, dc)) return out;
+        detail::bspline_points<D, Order>(acc.grid, base, [&](const int* k, uint32_t lin) {
+            const double ph = (double)acc.content[lin];
+            for (int dg = 0; dg < D; ++dg) {
+                double gf = dg == D - 1 ? dc[D - 1][k[D - 1]] : c[D - 1][k[D - 1]];
+                for (int j = 1; j < D; ++j) {
+                    const int dd = D - 1 - j;
+                    gf = gf * (dd == dg ? dc[dd][k[dd]] : c[dd][k[dd]]);
+                }
+                if (fabs(gf) > 1e-10) r[dg] += ph * gf;
+            }
+        });
+        for (int d = 0; d < D; ++d) out[d] = (Scalar)r[d];
+        return out;
+    }
+}
+
+// FP64 atomics like add_back; psum_hip_deposit_bspline is the fixed-point form
+template <int Order, typename Pos, int D, typename Scalar>
+__device__ void add_back_B_spline(const Pos& pos, const Scalar& w,
+                                  const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
+{
+    if constexpr (Order == 1) {
+        add_back(pos, w, acc);
+    } else {
+        double x[D], c[D][Order + 1];
+        int base[D];
+        for (int d = 0; d < D; ++d) x[d] = (double)pos[d];
+        if (!psum_hip::bspline::support_of<D, Order, false>(acc.grid, x, base, c, nullptr)) return;
+        detail::bspline_points<D, Order>(acc.grid, base, [&](const int* k, uint32_t lin) {
+            double cf = c[D - 1][k[D - 1]];
+            for (int j = 1; j < D; ++j) cf = cf * c[D - 1 - j][k[D - 1 - j]];
+            if (fabs(cf) > 1e-10) atomicAdd(&acc.content[lin], w * cf);
+        });
     }
 }
 

@@ -21,6 +21,9 @@ namespace prelude {
 using psum::handler;
 using psum::vec;
 using psum::field::add_back;
+using psum::field::add_back_B_spline;
+using psum::field::interp_B_spline;
+using psum::field::interp_diff_B_spline;
 using psum::field::cell_field1D;
 using psum::field::cell_field2D;
 using psum::field::cell_field3D;
