@@ -191,13 +191,14 @@ def run_adjoint(E, grid, ppc, steps, ckpt, depth, pre, post):
     return out
 
 
-def run_batch(E, n_inst, steps, seed=11):
+def run_batch(E, n_inst, steps, lanes=0, seed=11):
     """Batched ensemble: n_inst independent 2D periodic instances (grids
     from {64^2, 128^2, 256^2, 512^2}, 16-128 e-/cell, perturbation mode
     1..8, amplitude 1e-3..5e-2), FP32 with FP64 MG refinement, autotuned
     (cycle count, sort bin, sort interval), one step of all instances
-    replayed as one hipGraph.  Also times the same instances stepped one
-    after another without the graph."""
+    replayed as per-lane hipGraphs on concurrent streams (lanes=0: the
+    lane count in {1, 2, 4, 8} with the shortest 20-step replay).  Also
+    times the same instances stepped one after another without the graph."""
     from psum_amd.workloads import ensemble
     t0 = time.time()
     specs = ensemble(n_inst, seed, device="cuda")
@@ -213,6 +214,15 @@ def run_batch(E, n_inst, steps, seed=11):
     t0 = time.time()
     tun = b.tune(warmup=20, probe=10)
     log(f"[bench] batch: tuned in {time.time() - t0:.1f}s")
+    sweep = {}
+    if lanes == 0:
+        for L in (1, 2, 4, 8):
+            b.set_lanes(L)
+            b.run(2)              # capture + first replay
+            sweep[L] = b.run(20) / 20
+        lanes = min(sweep, key=sweep.get)
+        log("[bench] batch: lanes " + ", ".join(f"{L}: {t:.2f} ms/step" for L, t in sweep.items()))
+    b.set_lanes(lanes)
     b.reset_history(steps + 2)
     b.run(2)                      # capture + first replay
     ms = b.run(steps)
@@ -226,8 +236,9 @@ def run_batch(E, n_inst, steps, seed=11):
     out = {
         "workload": f"{n_inst} independent 2D periodic instances, grids from 64^2..512^2 nodes, "
                     f"16-128 e-/cell, mode 1..8, amplitude 1e-3..5e-2, FP32 + FP64 MG refinement "
-                    f"(1e-8 calibration), {steps} steps, one hipGraph per step",
-        "instances": n_inst, "particles": total, "steps": steps,
+                    f"(1e-8 calibration), {steps} steps, {lanes} hipGraph lanes per step",
+        "instances": n_inst, "particles": total, "steps": steps, "lanes": lanes,
+        "lane_sweep_ms_per_step": {str(k): v for k, v in sweep.items()},
         "ms_per_step": ms / steps, "particle_steps_per_s": total * steps / (ms * 1e-3),
         "sequential_ms_per_step": seq_ms, "graph_speedup": seq_ms / (ms / steps),
         "histories_finite": bool(hist_ok),
@@ -302,6 +313,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-lanes", type=int, default=1,
+                    help="concurrent graph lanes for the batch (0: pick from 1, 2, 4, 8)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--c3", action="store_true", help="also run the 3D bounded box (256^3, 1.07e9 particles)")
     ap.add_argument("--c3-cells", type=int, default=256)
@@ -365,7 +378,7 @@ def main():
         c3 = run_c3(E, args.c3_cells, args.c3_ppc, args.c3_steps, 6, args.pre, args.post)
     batch = None
     if not args.no_batch:
-        batch = run_batch(E, args.batch_instances, args.batch_steps)
+        batch = run_batch(E, args.batch_instances, args.batch_steps, args.batch_lanes)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_steps)

@@ -347,6 +347,11 @@ psum_status psum_batch_tune(void* b, int warmup, int probe, psum_batch_tuning* o
 /* history rows (5 doubles per step, as psum_stepper_run) restart at 0 and
  * are kept for up to `rows` steps */
 psum_status psum_batch_reset_history(void* b, int64_t rows);
+/* Split the instances into `lanes` (1..64) groups balanced by particle
+ * count; each group's step is its own graph, replayed on its own stream, so
+ * groups run concurrently.  Takes effect at the next psum_batch_run (the
+ * graphs are captured again). */
+psum_status psum_batch_set_lanes(void* b, int lanes);
 /* `steps` replays; *t_ms = HIP-event time of the whole call */
 psum_status psum_batch_run(void* b, int steps, double* t_ms);
 psum_status psum_batch_history(void* b, int inst, double* out, int64_t max_rows, int64_t* rows);

@@ -1211,13 +1211,19 @@ public:
 
     ~Batch()
     {
-        drop();
+        release_lanes();
         for (auto& e : join_) if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : {fork_, t0_, t1_}) if (e) (void)hipEventDestroy(e);
         if (origin_) (void)hipStreamDestroy(origin_);
     }
 
     int size() const { return (int)inst_.size(); }
+    void set_lanes(int lanes)
+    {
+        if (lanes < 1 || lanes > 64) throw std::invalid_argument("psum_batch_set_lanes: lanes must be 1..64");
+        lanes_ = lanes;
+        release_lanes();
+    }
     Stepper& at(int i)
     {
         if (i < 0 || i >= size()) throw std::invalid_argument("psum_batch: instance out of range");
@@ -1262,7 +1268,7 @@ public:
             if (p->fixed_cycles() < 1)
                 throw std::invalid_argument("psum_batch_run: every instance needs a fixed cycle count "
                                             "(psum_batch_configure or psum_batch_tune)");
-        if (!exec_) capture();
+        if (!captured_) capture();
         // instances reloaded since the capture need their deposit redone
         for (int i = 0; i < size(); ++i)
             if (!inst_[i]->acc_valid()) {
@@ -1271,26 +1277,34 @@ public:
                 S_TRY(hipStreamWaitEvent(origin_, join_[i], 0));
             }
         S_TRY(hipEventRecord(t0_, origin_));
+        S_TRY(hipEventRecord(fork_, origin_));
+        for (auto st : lane_st_) S_TRY(hipStreamWaitEvent(st, fork_, 0));
         for (int s = 0; s < steps; ++s) {
-            // eager sorts run on the instances' streams after the previous
-            // replay (fork) and before the next one (join)
-            bool forked = false;
-            for (int i = 0; i < size(); ++i) {
-                Stepper& p = *inst_[i];
-                const int k = p.sort_every();
-                if (k > 0 && p.steps_done() % k == 0) {
-                    if (!forked) {
-                        S_TRY(hipEventRecord(fork_, origin_));
-                        forked = true;
+            for (size_t l = 0; l < lane_st_.size(); ++l) {
+                // eager sorts due before this step run on the instances'
+                // streams after the lane's previous replay
+                bool forked = false;
+                for (int i : lane_inst_[l]) {
+                    Stepper& p = *inst_[i];
+                    const int k = p.sort_every();
+                    if (k > 0 && p.steps_done() % k == 0) {
+                        if (!forked) {
+                            S_TRY(hipEventRecord(lane_fork_[l], lane_st_[l]));
+                            forked = true;
+                        }
+                        S_TRY(hipStreamWaitEvent(p.stream(), lane_fork_[l], 0));
+                        p.sort_all();
+                        S_TRY(hipEventRecord(join_[i], p.stream()));
+                        S_TRY(hipStreamWaitEvent(lane_st_[l], join_[i], 0));
                     }
-                    S_TRY(hipStreamWaitEvent(p.stream(), fork_, 0));
-                    p.sort_all();
-                    S_TRY(hipEventRecord(join_[i], p.stream()));
-                    S_TRY(hipStreamWaitEvent(origin_, join_[i], 0));
                 }
+                S_TRY(hipGraphLaunch(lane_exec_[l], lane_st_[l]));
             }
-            S_TRY(hipGraphLaunch(exec_, origin_));
             for (auto& p : inst_) p->count_step();
+        }
+        for (size_t l = 0; l < lane_st_.size(); ++l) {
+            S_TRY(hipEventRecord(lane_join_[l], lane_st_[l]));
+            S_TRY(hipStreamWaitEvent(origin_, lane_join_[l], 0));
         }
         S_TRY(hipEventRecord(t1_, origin_));
         S_TRY(hipEventSynchronize(t1_));
@@ -1316,32 +1330,89 @@ private:
     hipStream_t origin_ = nullptr;
     hipEvent_t fork_ = nullptr, t0_ = nullptr, t1_ = nullptr;
     std::vector<hipEvent_t> join_;
-    hipGraphExec_t exec_ = nullptr;
+    // lanes: the instances are split into `lanes_` groups balanced by
+    // particle count; each group's step is one captured graph replayed on
+    // the group's own stream, so groups run concurrently on the hardware
+    // queues (branches inside one graph run in order)
+    int lanes_ = 1;
+    std::vector<hipStream_t> lane_st_;
+    std::vector<hipEvent_t> lane_fork_, lane_join_;
+    std::vector<std::vector<int>> lane_inst_;
+    std::vector<hipGraphExec_t> lane_exec_;
+    bool captured_ = false;
 
     void drop()
     {
-        if (exec_) (void)hipGraphExecDestroy(exec_);
-        exec_ = nullptr;
+        for (auto& e : lane_exec_)
+            if (e) (void)hipGraphExecDestroy(e);
+        lane_exec_.clear();
+        captured_ = false;
+    }
+
+    void release_lanes()
+    {
+        drop();
+        for (auto& st : lane_st_) if (st) (void)hipStreamDestroy(st);
+        for (auto& e : lane_fork_) if (e) (void)hipEventDestroy(e);
+        for (auto& e : lane_join_) if (e) (void)hipEventDestroy(e);
+        lane_st_.clear();
+        lane_fork_.clear();
+        lane_join_.clear();
+        lane_inst_.clear();
+    }
+
+    // greedy balance by particle count, largest first
+    void plan_lanes()
+    {
+        release_lanes();
+        const int L = std::max(1, std::min(lanes_, size()));
+        std::vector<int> order(size());
+        for (int i = 0; i < size(); ++i) order[i] = i;
+        std::sort(order.begin(), order.end(),
+                  [&](int a, int b) { return inst_[a]->particles() > inst_[b]->particles(); });
+        std::vector<int64_t> load(L, 0);
+        lane_inst_.assign(L, {});
+        for (int i : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            lane_inst_[l].push_back(i);
+            load[l] += std::max<int64_t>(1, inst_[i]->particles());
+        }
+        for (int l = 0; l < L; ++l) {
+            hipStream_t st = nullptr;
+            S_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            lane_st_.push_back(st);
+            hipEvent_t f = nullptr, j = nullptr;
+            S_TRY(hipEventCreateWithFlags(&f, hipEventDisableTiming));
+            S_TRY(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+            lane_fork_.push_back(f);
+            lane_join_.push_back(j);
+        }
     }
 
     void capture()
     {
+        if (lane_st_.empty()) plan_lanes();
         for (auto& p : inst_) p->prepare_capture();
         for (auto& p : inst_) S_TRY(hipStreamSynchronize(p->stream()));
-        S_TRY(hipStreamBeginCapture(origin_, hipStreamCaptureModeThreadLocal));
-        S_TRY(hipEventRecord(fork_, origin_));
-        for (int i = 0; i < size(); ++i) {
-            Stepper& st = *inst_[i];
-            S_TRY(hipStreamWaitEvent(st.stream(), fork_, 0));
-            st.step_body(true);
-            S_TRY(hipEventRecord(join_[i], st.stream()));
-            S_TRY(hipStreamWaitEvent(origin_, join_[i], 0));
+        lane_exec_.assign(lane_st_.size(), nullptr);
+        for (size_t l = 0; l < lane_st_.size(); ++l) {
+            hipStream_t ls = lane_st_[l];
+            S_TRY(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+            S_TRY(hipEventRecord(lane_fork_[l], ls));
+            for (int i : lane_inst_[l]) {
+                Stepper& st = *inst_[i];
+                S_TRY(hipStreamWaitEvent(st.stream(), lane_fork_[l], 0));
+                st.step_body(true);
+                S_TRY(hipEventRecord(join_[i], st.stream()));
+                S_TRY(hipStreamWaitEvent(ls, join_[i], 0));
+            }
+            hipGraph_t graph = nullptr;
+            S_TRY(hipStreamEndCapture(ls, &graph));
+            const hipError_t e = hipGraphInstantiate(&lane_exec_[l], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            S_TRY(e);
         }
-        hipGraph_t graph = nullptr;
-        S_TRY(hipStreamEndCapture(origin_, &graph));
-        const hipError_t e = hipGraphInstantiate(&exec_, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        S_TRY(e);
+        captured_ = true;
         // step_body counted one step per instance while capturing
         for (auto& p : inst_) p->uncount_step();
     }
@@ -1552,6 +1623,12 @@ psum_status psum_batch_reset_history(void* h, int64_t rows)
 {
     if (!h || rows < 0) { set_error("psum_batch_reset_history: bad argument"); return PSUM_ERR_ARG; }
     return guarded([&]() { static_cast<Batch*>(h)->hist_reset(rows); });
+}
+
+psum_status psum_batch_set_lanes(void* h, int lanes)
+{
+    if (!h) { set_error("null batch"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Batch*>(h)->set_lanes(lanes); });
 }
 
 psum_status psum_batch_run(void* h, int steps, double* t_ms)

@@ -464,6 +464,10 @@ class Batch:
     def reset_history(self, rows):
         check(lib().psum_batch_reset_history(self.h, rows), "psum_batch_reset_history")
 
+    def set_lanes(self, lanes):
+        """Replay the step as `lanes` per-group graphs on concurrent streams."""
+        check(lib().psum_batch_set_lanes(self.h, lanes), "psum_batch_set_lanes")
+
     def run(self, steps):
         """`steps` graph replays; returns the HIP-event milliseconds."""
         ms = C.c_double(0)

@@ -27,8 +27,11 @@ def _instance(n, ppc, k, amp, seed, dtype="f32"):
                 depth=depth, pre=2, post=2, max_cycles=40, tol=1e-8, dt=0.1, rho_bg=1.0)
 
 
-@pytest.mark.parametrize("dtype", ["f32", "f64"])
-def test_batch_matches_instances_stepped_alone(dtype):
+@pytest.mark.parametrize("dtype,lanes", [("f32", (1, 1)), ("f64", (1, 1)), ("f32", (3, 3)),
+                                         ("f64", (2, 2)), ("f32", (1, 3))])
+def test_batch_matches_instances_stepped_alone(dtype, lanes):
+    """lanes: the lane count for the first and second half of the steps (a
+    change between runs captures the lane graphs again)"""
     from psum_amd import engine as E
     specs = [_instance(32, 16, 1, 0.05, 1, dtype), _instance(64, 4, 3, 0.02, 2, dtype),
              _instance(48, 9, 2, 0.01, 3, dtype)]
@@ -38,7 +41,10 @@ def test_batch_matches_instances_stepped_alone(dtype):
     for i, (c, k, sh) in enumerate(sched):
         b.configure(i, c, k, sh)
     b.reset_history(steps)
-    b.run(steps)
+    b.set_lanes(lanes[0])
+    b.run(steps // 2)
+    b.set_lanes(lanes[1])
+    b.run(steps - steps // 2)
     for i, spec in enumerate(specs):
         a = dict(spec)
         g, species = a.pop("g"), a.pop("species")
