@@ -144,6 +144,96 @@ psum_status psum_hip_gather_push_bspline(const psum_grid* g, int dtype, int orde
                                          void* const* vel, int64_t n, const void* phi, double qm, double dt,
                                          double mass, double* diag, int64_t* counters, void* stream);
 
+/* ---- Monte Carlo collisions --------------------------------------------
+ * One call applies a sequence of collision processes (incident species
+ * against a background target each) to every valid particle, between the
+ * push and the next deposit.  Replaces the collision_processor /
+ * sequence_executor of particle_collision/foundation.hpp:17-107 run by
+ * execute_mcc_model (mcc_model.hpp:184-218), with the actions of the He
+ * CCP model (application/benchmark/CCP/custom_mccm.hpp:29-104) as channel
+ * kinds.  Per process and particle:
+ *   dens  = CIC interpolation of the process's node field (|coef| > 1e-10,
+ *           0 outside the grid) or the process's constant density
+ *   v2 = |v|^2, r = R(), sigma_k = cross section of channel k at v2
+ *   chosen = first k with r*sum <= sigma_0 + .. + sigma_k (sum > 0)
+ *   collide when chosen and R() < ((sum*dt)*dens)*|v|
+ * R() advances the particle's uint32 seed as the LCG
+ * x = x*1664525 + 1013904223 and returns x / (2^32-1)
+ * (random/rander.hpp:40-74).  The collision action draws from a copy of
+ * the stream; the seed keeps the state after the process's own draws, as
+ * in the reference, whose process wrapper writes its state back last.
+ * Velocities always have three components (vel[0..2]); positions have
+ * g->dim.  Arithmetic is FP64 for both dtypes.
+ *
+ * Cross sections (the form of CCP/HeCCS.h): piecewise polynomials of degree
+ * 4 in x = ln(v2 * arg_scale), segment k while x < seg_hi[k] (the last
+ * segment otherwise), c4 x^4 + c3 x^3 + c2 x^2 + c1 x + c0 summed left to
+ * right with x^k as repeated products, exponentiated when exp_form; nseg 0
+ * is the constant coef[0][4].
+ *
+ * Ionization products (a secondary electron with half the energy left
+ * after the threshold, an ion with a Maxwellian target velocity, each with
+ * a seed R()*(2^31-1)) are appended to `electrons` and `ions` at their
+ * current counts in (particle, process) order -- deterministic, unlike the
+ * reference's atomic push_back.  `electrons` may be the incident species
+ * itself (products land after its n particles and do not collide in this
+ * call).  When the products would exceed a capacity, nothing is modified
+ * and PSUM_ERR_ARG is returned with *produced = the number needed. */
+#define PSUM_MCC_MAX_SEG 12
+#define PSUM_MCC_MAX_CHAN 8
+#define PSUM_MCC_MAX_PROC 4
+#define PSUM_MCC_ELASTIC 0        /* col_ele_to_atom_elastic: v = |v| u, u isotropic */
+#define PSUM_MCC_EXCITATION 1     /* |v|^2 -= 2 e thr / m_incident, new direction (if > 0) */
+#define PSUM_MCC_IONIZATION 2     /* E' = (E - thr e)/2 for the incident and a new electron, new ion */
+#define PSUM_MCC_ION_ISOTROPIC 3  /* isotropic in the centre-of-mass frame of a Maxwellian atom */
+#define PSUM_MCC_ION_BACKWARD 4   /* charge exchange: v = Maxwellian target velocity */
+
+typedef struct {
+    int32_t kind;
+    int32_t nseg;
+    int32_t exp_form;
+    int32_t pad_;
+    double arg_scale;
+    double threshold_ev;
+    double seg_hi[PSUM_MCC_MAX_SEG];
+    double coef[PSUM_MCC_MAX_SEG][5];
+} psum_mcc_channel;
+
+typedef struct {
+    int32_t nchan;
+    int32_t pad_;
+    double density;             /* when the process has no density field */
+    double unit_charge;         /* J per eV */
+    double incident_mass;       /* kg */
+    double target_mass;         /* kg, background atoms (ion products, Maxwellians) */
+    double target_temperature;  /* K */
+    double boltzmann;           /* J/K */
+    psum_mcc_channel chan[PSUM_MCC_MAX_CHAN];
+} psum_mcc_process;
+
+typedef struct {
+    void* pos[3];               /* dim arrays */
+    void* vel[3];               /* 3 arrays */
+    uint32_t* seed;
+    int64_t n;                  /* particles present */
+    int64_t cap;                /* capacity of every array */
+} psum_mcc_species;
+
+/* Bytes of device workspace psum_hip_mcc needs for n incident particles. */
+size_t psum_hip_mcc_workspace(int64_t n, int nproc);
+/* density: nproc device pointers to FP64 node fields of g (or NULL), or
+ * NULL for all constant.  electrons / ions may be NULL when no channel is
+ * an ionization.  *produced (host) = product pairs appended (the caller
+ * adds it to both counts).  outcome (device, may be NULL): n*nproc int32,
+ * the channel that collided per (particle, process), -1 none, -2 invalid
+ * particle.  Synchronizes the stream when an ionization channel exists
+ * (the product count is checked against the capacities first). */
+psum_status psum_hip_mcc(const psum_grid* g, int dtype, const psum_mcc_process* procs, int nproc,
+                         const double* const* density, const psum_mcc_species* incident,
+                         const psum_mcc_species* electrons, const psum_mcc_species* ions, double dt,
+                         int64_t* produced, int32_t* outcome, void* work, size_t work_bytes,
+                         void* stream);
+
 /* out[k] = sum over blocks of diag[4*b + k], k < 4, in a fixed order
  * (deterministic).  work: psum_hip_reduce_workspace() bytes of device
  * memory (may be NULL when nblocks <= 1024). */

@@ -46,6 +46,14 @@
  *                          interpolant, continuity across the base-node switch;
  *                          deposits conserve charge and are the transpose of the
  *                          gather; a uniform field is gathered exactly
+ *   orc_mcc_*              LCG known answers (Numerical Recipes' ranqd1
+ *                          sequence); unit spherical vectors with isotropic
+ *                          moments; Maxwellian second moments kT/m;
+ *                          collision fraction sigma*n*|v|*dt and channel
+ *                          shares sigma_k/sum; |v| kept by elastic, 2e*thr/m
+ *                          lost by excitation, energy conserved by
+ *                          ionization; seed = state after the process's own
+ *                          draws; planted mistakes fail these checks
  *   orc_pic_adjoint        central finite differences of orc_pic_loss in
  *                          directions where no particle changes cell
  *                          (key_hash equal at +h and -h); dot-product test
@@ -286,6 +294,58 @@ void orc_adjoint_plant(int which);
 /* Forward mode: returns dL = <grad L, (dpos, dvel)>; *loss = L. */
 double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, double*** dpos,
                        double*** dvel, double* loss);
+
+/* Monte Carlo collisions (particle_collision/foundation.hpp:17-107,
+ * mcc_model.hpp:184-218; actions of CCP/custom_mccm.hpp:29-104; random
+ * streams of random/rander.hpp:40-139 and rand_functions.hpp:15-28).
+ * Layout identical to psum_mcc_channel / psum_mcc_process in
+ * include/psum_hip.h. */
+#define ORC_MCC_MAX_SEG 12
+#define ORC_MCC_MAX_CHAN 8
+enum { ORC_MCC_ELASTIC = 0, ORC_MCC_EXCITATION = 1, ORC_MCC_IONIZATION = 2,
+       ORC_MCC_ION_ISOTROPIC = 3, ORC_MCC_ION_BACKWARD = 4 };
+typedef struct {
+    int32_t kind;
+    int32_t nseg;          /* 0: constant sigma = coef[0][4] */
+    int32_t exp_form;      /* sigma = exp(p(x)) instead of p(x) */
+    int32_t pad_;
+    double arg_scale;      /* x = ln(v2 * arg_scale) */
+    double threshold_ev;
+    double seg_hi[ORC_MCC_MAX_SEG];        /* segment k while x < seg_hi[k] */
+    double coef[ORC_MCC_MAX_SEG][5];       /* c4 x^4 + c3 x^3 + c2 x^2 + c1 x + c0 */
+} orc_mcc_channel;
+typedef struct {
+    int32_t nchan;
+    int32_t pad_;
+    double density;        /* used when the process has no density field */
+    double unit_charge;
+    double incident_mass;
+    double target_mass;
+    double target_temperature;
+    double boltzmann;
+    orc_mcc_channel chan[ORC_MCC_MAX_CHAN];
+} orc_mcc_process;
+
+double orc_lcg_next(uint32_t* x);
+void orc_rand_spherical(uint32_t* x, double* o);
+void orc_rand_maxwell(uint32_t* x, double temperature, double mass, double boltzmann, double* o);
+double orc_mcc_sigma(const orc_mcc_channel* c, double v2);
+/* Applies the processes in order to every valid particle (pos[d], d < dim;
+ * vel[0..2]; seed).  density[k] (node field of g, may be NULL as a whole or
+ * per entry) gives process k's target density, else procs[k].density.
+ * Ionization products go to e_* / i_* in (particle, process) order, at
+ * most cap of them; returns the number made.  outcome[p*nproc+k] (may be
+ * NULL): channel that collided, -1 none, -2 invalid particle. */
+int64_t orc_mcc_f64(const orc_grid* g, const orc_mcc_process* procs, int nproc,
+                    const double* const* density, double** pos, double** vel, uint32_t* seed, int64_t n,
+                    double dt, double** e_pos, double** e_vel, uint32_t* e_seed, double** i_pos,
+                    double** i_vel, uint32_t* i_seed, int64_t cap, int32_t* outcome);
+int64_t orc_mcc_f32(const orc_grid* g, const orc_mcc_process* procs, int nproc,
+                    const double* const* density, float** pos, float** vel, uint32_t* seed, int64_t n,
+                    double dt, float** e_pos, float** e_vel, uint32_t* e_seed, float** i_pos,
+                    float** i_vel, uint32_t* i_seed, int64_t cap, int32_t* outcome);
+/* planted mistakes for the pin tests (0 = none) */
+void orc_mcc_plant(int which);
 
 #ifdef __cplusplus
 }

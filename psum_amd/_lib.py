@@ -96,6 +96,52 @@ class FuncTable(C.Structure):
 
 
 # (name, restype, argtypes) of every exported entry point
+MCC_MAX_SEG = 12
+MCC_MAX_CHAN = 8
+MCC_MAX_PROC = 4
+MCC_ELASTIC, MCC_EXCITATION, MCC_IONIZATION, MCC_ION_ISOTROPIC, MCC_ION_BACKWARD = range(5)
+
+
+class PsumMccChannel(C.Structure):
+    """psum_mcc_channel (include/psum_hip.h)."""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("nseg", C.c_int32),
+        ("exp_form", C.c_int32),
+        ("pad_", C.c_int32),
+        ("arg_scale", C.c_double),
+        ("threshold_ev", C.c_double),
+        ("seg_hi", C.c_double * MCC_MAX_SEG),
+        ("coef", (C.c_double * 5) * MCC_MAX_SEG),
+    ]
+
+
+class PsumMccProcess(C.Structure):
+    """psum_mcc_process (include/psum_hip.h)."""
+    _fields_ = [
+        ("nchan", C.c_int32),
+        ("pad_", C.c_int32),
+        ("density", C.c_double),
+        ("unit_charge", C.c_double),
+        ("incident_mass", C.c_double),
+        ("target_mass", C.c_double),
+        ("target_temperature", C.c_double),
+        ("boltzmann", C.c_double),
+        ("chan", PsumMccChannel * MCC_MAX_CHAN),
+    ]
+
+
+class PsumMccSpecies(C.Structure):
+    """psum_mcc_species (include/psum_hip.h)."""
+    _fields_ = [
+        ("pos", C.c_void_p * 3),
+        ("vel", C.c_void_p * 3),
+        ("seed", C.c_void_p),
+        ("n", C.c_int64),
+        ("cap", C.c_int64),
+    ]
+
+
 P = C.c_void_p
 I64 = C.c_int64
 ST = C.c_int
@@ -116,6 +162,10 @@ SIGNATURES = [
     ("psum_hip_deposit_bspline", ST, [_GP, C.c_int, C.c_int, P, I64, C.c_double, C.c_double, P, P, P]),
     ("psum_hip_gather_push_bspline", ST, [_GP, C.c_int, C.c_int, P, P, I64, P, C.c_double, C.c_double,
                                           C.c_double, P, P, P]),
+    ("psum_hip_mcc_workspace", C.c_size_t, [I64, C.c_int]),
+    ("psum_hip_mcc", ST, [_GP, C.c_int, C.POINTER(PsumMccProcess), C.c_int, P, C.POINTER(PsumMccSpecies),
+                          C.POINTER(PsumMccSpecies), C.POINTER(PsumMccSpecies), C.c_double,
+                          C.POINTER(I64), P, P, C.c_size_t, P]),
     ("psum_hip_reduce_workspace", C.c_size_t, []),
     ("psum_hip_reduce_diag", ST, [P, I64, P, P, P]),
     ("psum_hip_field_energy_workspace", C.c_size_t, [_GP]),

@@ -77,6 +77,72 @@ def node_shape(g):
 _lib = None
 
 
+MCC_KINDS = {"elastic": 0, "excitation": 1, "ionization": 2, "ion_isotropic": 3, "ion_backward": 4}
+
+
+class MccChannel(C.Structure):
+    """Same layout as orc_mcc_channel / psum_mcc_channel."""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("nseg", C.c_int32),
+        ("exp_form", C.c_int32),
+        ("pad_", C.c_int32),
+        ("arg_scale", C.c_double),
+        ("threshold_ev", C.c_double),
+        ("seg_hi", C.c_double * 12),
+        ("coef", (C.c_double * 5) * 12),
+    ]
+
+
+class MccProcess(C.Structure):
+    """Same layout as orc_mcc_process / psum_mcc_process."""
+    _fields_ = [
+        ("nchan", C.c_int32),
+        ("pad_", C.c_int32),
+        ("density", C.c_double),
+        ("unit_charge", C.c_double),
+        ("incident_mass", C.c_double),
+        ("target_mass", C.c_double),
+        ("target_temperature", C.c_double),
+        ("boltzmann", C.c_double),
+        ("chan", MccChannel * 8),
+    ]
+
+
+def mcc_channel(kind, sigma=None, seg_hi=(), coef=None, exp_form=True, arg_scale=1.0, threshold_ev=0.0):
+    ch = MccChannel()
+    ch.kind = MCC_KINDS[kind] if isinstance(kind, str) else int(kind)
+    ch.arg_scale = arg_scale
+    ch.threshold_ev = threshold_ev
+    if sigma is not None:
+        ch.coef[0][4] = sigma
+        return ch
+    coef = np.asarray(coef, np.float64).reshape(-1, 5)
+    ch.nseg = len(coef)
+    ch.exp_form = 1 if exp_form else 0
+    for i, b in enumerate(seg_hi):
+        ch.seg_hi[i] = b
+    for i, row in enumerate(coef):
+        for j in range(5):
+            ch.coef[i][j] = row[j]
+    return ch
+
+
+def mcc_process(channels, density=0.0, incident_mass=9.10956e-31, target_mass=6.67e-27,
+                target_temperature=300.0, unit_charge=1.602e-19, boltzmann=1.38065e-23):
+    P = MccProcess()
+    P.nchan = len(channels)
+    for i, ch in enumerate(channels):
+        P.chan[i] = ch
+    P.density = density
+    P.unit_charge = unit_charge
+    P.incident_mass = incident_mass
+    P.target_mass = target_mass
+    P.target_temperature = target_temperature
+    P.boltzmann = boltzmann
+    return P
+
+
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
@@ -132,6 +198,17 @@ def lib():
                                              C.c_float, C.c_double, P]
         L.orc_gather_push_bs_f32.restype = i64
         L.orc_bspline_plant.argtypes = [C.c_int]
+        L.orc_lcg_next.argtypes = [P]
+        L.orc_lcg_next.restype = C.c_double
+        L.orc_rand_spherical.argtypes = [P, P]
+        L.orc_rand_maxwell.argtypes = [P, C.c_double, C.c_double, C.c_double, P]
+        L.orc_mcc_sigma.argtypes = [C.POINTER(MccChannel), C.c_double]
+        L.orc_mcc_sigma.restype = C.c_double
+        for nm in ("orc_mcc_f64", "orc_mcc_f32"):
+            getattr(L, nm).argtypes = [C.POINTER(Grid), C.POINTER(MccProcess), C.c_int, P, P, P, P, i64,
+                                       C.c_double, P, P, P, P, P, P, i64, P]
+            getattr(L, nm).restype = i64
+        L.orc_mcc_plant.argtypes = [C.c_int]
         L.orc_pic_set_order.argtypes = [C.c_int]
         L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int]
@@ -569,3 +646,73 @@ def pic_run_multi(g, species, rho_bg, eps, dt, steps, depth, pre=2, post=1, tol=
     if cyc < 0:
         raise RuntimeError("orc_pic_run_multi failed")
     return hist, phi, cyc
+
+
+def lcg_draws(seed, k):
+    """k successive R() values from seed; returns (values, final state)"""
+    x = C.c_uint32(seed)
+    out = [lib().orc_lcg_next(C.byref(x)) for _ in range(k)]
+    return np.array(out), x.value
+
+
+def rand_spherical(seed):
+    x = C.c_uint32(seed)
+    o = np.zeros(3)
+    lib().orc_rand_spherical(C.byref(x), _ptr(o))
+    return o, x.value
+
+
+def rand_maxwell(seed, temperature, mass, boltzmann=1.38065e-23):
+    x = C.c_uint32(seed)
+    o = np.zeros(3)
+    lib().orc_rand_maxwell(C.byref(x), temperature, mass, boltzmann, _ptr(o))
+    return o, x.value
+
+
+def mcc_sigma(ch, v2):
+    return lib().orc_mcc_sigma(C.byref(ch), v2)
+
+
+def mcc(g, procs, pos, vel, seed, dt, density=None, cap=None):
+    """Runs the processes on copies; returns dict(pos, vel, seed, made,
+    e_pos, e_vel, e_seed, i_pos, i_vel, i_seed, outcome[n, nproc])."""
+    t = pos[0].dtype
+    pos = [np.array(a, t) for a in pos]
+    vel = [np.array(a, t) for a in vel]
+    seed = np.array(seed, np.uint32)
+    n = seed.size
+    nproc = len(procs)
+    arr = (MccProcess * nproc)(*procs)
+    cap = n * nproc if cap is None else cap
+    D = g.dim
+    ep = [np.zeros(max(cap, 1), t) for _ in range(D)]
+    ev = [np.zeros(max(cap, 1), t) for _ in range(3)]
+    es = np.zeros(max(cap, 1), np.uint32)
+    ip = [np.zeros(max(cap, 1), t) for _ in range(D)]
+    iv = [np.zeros(max(cap, 1), t) for _ in range(3)]
+    is_ = np.zeros(max(cap, 1), np.uint32)
+    outc = np.zeros(max(n * nproc, 1), np.int32)
+    dens_arr = None
+    keep = []
+    if density is not None:
+        dens_arr = (C.c_void_p * nproc)()
+        for k, f in enumerate(density):
+            if f is not None:
+                f = np.ascontiguousarray(f, np.float64)
+                keep.append(f)
+                dens_arr[k] = f.ctypes.data
+    f = lib().orc_mcc_f32 if t == np.float32 else lib().orc_mcc_f64
+    made = f(C.byref(g), arr, nproc, C.cast(dens_arr, C.c_void_p) if dens_arr is not None else None,
+             C.cast(_ptr_array(pos), C.c_void_p), C.cast(_ptr_array(vel), C.c_void_p), _ptr(seed), n, dt,
+             C.cast(_ptr_array(ep), C.c_void_p), C.cast(_ptr_array(ev), C.c_void_p), _ptr(es),
+             C.cast(_ptr_array(ip), C.c_void_p), C.cast(_ptr_array(iv), C.c_void_p), _ptr(is_), cap,
+             _ptr(outc))
+    m = min(made, cap)
+    return dict(pos=pos, vel=vel, seed=seed, made=made,
+                e_pos=[a[:m] for a in ep], e_vel=[a[:m] for a in ev], e_seed=es[:m],
+                i_pos=[a[:m] for a in ip], i_vel=[a[:m] for a in iv], i_seed=is_[:m],
+                outcome=outc[:n * nproc].reshape(n, nproc))
+
+
+def mcc_plant(which):
+    lib().orc_mcc_plant(which)
