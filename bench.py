@@ -253,6 +253,74 @@ def run_batch(E, n_inst, steps, lanes=0, seed=11):
     return out
 
 
+def run_mcc(E, n, calls):
+    """Monte Carlo collisions of n FP64 electrons (1D3V, 2 eV-ish core with
+    a 50 eV tail) against He-like atoms at a constant density on a 1000-cell
+    wall grid: elastic + two excitations (one pass), then the same plus
+    ionization (counting pass, scan, host check, apply pass; the product
+    counts are reset before each call so every call appends into the same
+    slots).  Cross sections are synthetic exp-quadratic fits in ln(v2) of
+    the reference's HeCCS form.  Times `calls` calls of each with HIP events
+    on the launching stream."""
+    import torch
+    from psum_amd import mcc as M
+    me, mhe, x0 = 9.10956e-31, 6.67e-27, 28.0
+
+    def fit(a, b, c, x_on=None):
+        row = [0.0, 0.0, c, b - 2 * c * x0, a - b * x0 + c * x0 * x0]
+        if x_on is None:
+            return dict(seg_hi=[], coef=[row])
+        return dict(seg_hi=[x_on], coef=[[0, 0, 0, 0, -200.0], row])
+    ch = [M.channel("elastic", **fit(-44.0, -0.4, -0.02)),
+          M.channel("excitation", **fit(-46.0, 0.8, -0.15, x0 + 2.16), threshold_ev=19.82),
+          M.channel("excitation", **fit(-46.5, 0.7, -0.12, x0 + 2.2), threshold_ev=20.61)]
+    ion = M.channel("ionization", **fit(-45.0, 0.9, -0.1, x0 + 2.37), threshold_ev=24.587)
+    kw = dict(incident_mass=me, target_mass=mhe, target_temperature=300.0)
+    no_ion = [M.Process(ch, 3e21, **kw)]
+    with_ion = [M.Process(ch + [ion], 3e21, **kw)]
+    g = E.wall_grid([1000], dx=0.067 / 1000)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    cap = n + n // 20
+    sp = M.Particles.empty(1, cap)
+    sp.pos[0][:n] = torch.rand(n, generator=gen, device="cuda", dtype=torch.float64) * 0.067
+    tail = (torch.rand(n, generator=gen, device="cuda") < 0.1).double()
+    for c in range(3):
+        sp.vel[c][:n] = torch.randn(n, generator=gen, device="cuda", dtype=torch.float64) * (6e5 + 2.4e6 * tail)
+    sp.seed[:n] = torch.randint(-2**31, 2**31 - 1, (n,), generator=gen, device="cuda", dtype=torch.int32)
+    sp.n = n
+    ions = M.Particles.empty(1, n // 20)
+    del tail
+    dt = 1e-11
+    out = {"workload": f"{n} FP64 electrons (1D3V) against He-like atoms at 3e21 m^-3, 1000-cell "
+                       f"wall grid, dt 1e-11, synthetic HeCCS-form cross sections", "particles": n}
+    for name, procs in (("no_ionization", no_ion), ("with_ionization", with_ion)):
+        made = 0
+        for _ in range(2):
+            sp.n, ions.n = n, 0
+            made, _ = M.collide(g, procs, sp, dt, electrons=sp, ions=ions, grow=False)
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(calls):
+            sp.n, ions.n = n, 0
+            made, _ = M.collide(g, procs, sp, dt, electrons=sp, ions=ions, grow=False)
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / calls
+        # algorithmic bytes: read x, v (3), seed; write seed (velocities of
+        # the few collided particles and the products are not counted)
+        algo = n * (8 + 24 + 4 + 4)
+        out[name] = {"ms_per_call": ms, "particles_per_s": n / (ms * 1e-3),
+                     "algorithmic_GBps": algo / (ms * 1e-3) / 1e9,
+                     "hbm_frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "products_per_call": made}
+        log(f"[bench] mcc {name}: {ms:.3f} ms/call for {n} electrons ({n / (ms * 1e-3):.3e}/s, "
+            f"{out[name]['algorithmic_GBps']:.0f} GB/s algorithmic), {made} product pairs")
+    del sp, ions
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_c3(E, cells, ppc, steps, depth, pre, post):
     """3D bounded box (phi = 0 on the faces, absorbing walls, compaction by
     the sort), electrons + ions (mass ratio 1836, Te/Ti = 10), FP32
@@ -316,6 +384,9 @@ def main():
     ap.add_argument("--batch-lanes", type=int, default=1,
                     help="concurrent graph lanes for the batch (0: pick from 1, 2, 4, 8)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--mcc-particles", type=int, default=100_000_000)
+    ap.add_argument("--mcc-calls", type=int, default=20)
+    ap.add_argument("--no-mcc", action="store_true")
     ap.add_argument("--c3", action="store_true", help="also run the 3D bounded box (256^3, 1.07e9 particles)")
     ap.add_argument("--c3-cells", type=int, default=256)
     ap.add_argument("--c3-ppc", type=int, default=32)
@@ -379,6 +450,9 @@ def main():
     batch = None
     if not args.no_batch:
         batch = run_batch(E, args.batch_instances, args.batch_steps, args.batch_lanes)
+    mcc = None
+    if not args.no_mcc:
+        mcc = run_mcc(E, args.mcc_particles, args.mcc_calls)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_steps)
@@ -418,6 +492,7 @@ def main():
         "adjoint": adjoint,
         "batch": batch,
         "c3": c3,
+        "mcc": mcc,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

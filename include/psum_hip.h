@@ -169,7 +169,7 @@ psum_status psum_hip_gather_push_bspline(const psum_grid* g, int dtype, int orde
  * 4 in x = ln(v2 * arg_scale), segment k while x < seg_hi[k] (the last
  * segment otherwise), c4 x^4 + c3 x^3 + c2 x^2 + c1 x + c0 summed left to
  * right with x^k as repeated products, exponentiated when exp_form; nseg 0
- * is the constant coef[0][4].
+ * is the constant coef[0][4]; arg_scale > 0.
  *
  * Ionization products (a secondary electron with half the energy left
  * after the threshold, an ion with a Maxwellian target velocity, each with
@@ -233,6 +233,53 @@ psum_status psum_hip_mcc(const psum_grid* g, int dtype, const psum_mcc_process* 
                          const psum_mcc_species* electrons, const psum_mcc_species* ions, double dt,
                          int64_t* produced, int32_t* outcome, void* work, size_t work_bytes,
                          void* stream);
+
+/* ---- triangle-mesh particle boundary -----------------------------------
+ * triangle_mesh_trigger (particle_boundary/triangle_mesh_trigger.hpp:
+ * 30-437) and boundary_router + basic_material_set (boundary_router.hpp:
+ * 20-92, basic_material_set.hpp:15-119).  psum_mesh_create builds the
+ * partition on the host: tri = ntri x 9 doubles (p1, p2, p3), ids[ntri]
+ * (the router reads an id as the triangle's material), an I x J x K
+ * uniform partition of `box` (x0 y0 z0 x1 y1 z1; NULL = the triangles'
+ * bounding box widened by the reference's fractions, :378-392).  Normals
+ * are (p2-p1) x (p3-p1) normalised; zero-area triangles are dropped.  A
+ * triangle is listed in each cell of its clamped bounding-box range whose
+ * cell box it overlaps (geometry.hpp:292-350), in triangle order.
+ * psum_hip_mesh_detect traces segments p1 -> p2 (3 coordinate arrays of
+ * dtype each): the cells along the segment from its start (restrict mode,
+ * geometry.hpp:40-206; fast mode: start and end cells only), the first
+ * cell with a hit giving the smallest Moller-Trumbore parameter k in
+ * (0, 1] among its triangles (first on ties); k, normal (3 per segment)
+ * and id (-1: no hit) out (k / normal may be NULL).  Segments with a
+ * non-finite end are not traced.  A traversal cell outside the partition
+ * (rounding at the entry face) is empty here; the reference reads outside
+ * its arrays there. */
+#define PSUM_MAT_DIFFUSE 0          /* cosine-law direction, |v| kept */
+#define PSUM_MAT_SPECULAR 1         /* v -= 2 (v.n) n */
+#define PSUM_MAT_MAXWELL_REFLECT 2  /* half-Maxwellian re-emission at (temperature, mass) */
+#define PSUM_MAT_ABSORB 3           /* position x = NaN (invalid) */
+void* psum_mesh_create(const double* tri, const int32_t* ids, int64_t ntri, int I, int J, int K,
+                       const double* box);
+/* info[0] triangles kept, [1] partition entries, [2] cells, [3] largest
+ * id; box[6] (either may be NULL) */
+psum_status psum_mesh_info(void* m, int64_t* info, double* box);
+/* host copies of the partition: cell_start[cells+1], cell_tri[entries],
+ * tri12[12*ntri] (vertices then unit normal; zeros for dropped triangles) */
+psum_status psum_mesh_partition(void* m, uint32_t* cell_start, uint32_t* cell_tri, double* tri12);
+/* triangle_mesh_trigger::set_fast / set_restrict (the router's default) */
+psum_status psum_mesh_set_fast(void* m, int fast);
+psum_status psum_hip_mesh_detect(void* m, int dtype, const void* const* p1, const void* const* p2, int64_t n,
+                                 double* k, double* normal, int32_t* id, void* stream);
+/* boundary_router::deal for every valid particle: segment old -> pos; on a
+ * hit at h = old + k (pos - old) the material of the triangle's id acts on
+ * pos, vel and the particle's seed (RandV_Cosine / RandV_halfMaxw draws,
+ * random/rand_functions.hpp:31-53).  Triangle ids must be 0..3.
+ * hit_id (device, may be NULL) per particle; *hits (device, may be NULL)
+ * is incremented by the number of hits. */
+psum_status psum_hip_mesh_boundary(void* m, int dtype, const void* const* old, void* const* pos,
+                                   void* const* vel, uint32_t* seed, int64_t n, double temperature,
+                                   double mass, int32_t* hit_id, unsigned long long* hits, void* stream);
+void psum_mesh_destroy(void* m);
 
 /* out[k] = sum over blocks of diag[4*b + k], k < 4, in a fixed order
  * (deterministic).  work: psum_hip_reduce_workspace() bytes of device

@@ -54,6 +54,20 @@
  *                          lost by excitation, energy conserved by
  *                          ionization; seed = state after the process's own
  *                          draws; planted mistakes fail these checks
+ *   orc_seg_tri etc.       the reference's own checks: a reported lambda
+ *                          puts the point inside the triangle by
+ *                          barycentric coordinates
+ *                          (test/particle_boundary/test_line_seg_tri_
+ *                          intersect.cpp); a direct 3x3 solve decides hits;
+ *                          traversed cells contain every sampled point of
+ *                          the segment (test_trace_to_idxs.cpp) and equal
+ *                          a dense sampling; box/triangle overlap is true
+ *                          whenever a sampled triangle point lies in the
+ *                          box (test_box_tri_overlap.cpp); detect equals a
+ *                          brute-force nearest hit over all triangles;
+ *                          specular reflection of a plane, diffuse
+ *                          directions in the wall's half space with a
+ *                          cosine law; planted mistakes fail these checks
  *   orc_pic_adjoint        central finite differences of orc_pic_loss in
  *                          directions where no particle changes cell
  *                          (key_hash equal at +h and -h); dot-product test
@@ -346,6 +360,47 @@ int64_t orc_mcc_f32(const orc_grid* g, const orc_mcc_process* procs, int nproc,
                     float** i_vel, uint32_t* i_seed, int64_t cap, int32_t* outcome);
 /* planted mistakes for the pin tests (0 = none) */
 void orc_mcc_plant(int which);
+
+/* Triangle-mesh particle boundary (particle_boundary/geometry.hpp,
+ * triangle_mesh_trigger.hpp, boundary_router.hpp, basic_material_set.hpp).
+ * Triangles are 9 doubles (p1, p2, p3); tri12 adds the unit normal. */
+enum { ORC_MAT_DIFFUSE = 0, ORC_MAT_SPECULAR = 1, ORC_MAT_MAXWELL = 2, ORC_MAT_ABSORB = 3 };
+typedef struct {
+    int I, J, K;
+    double p[3], d[3];
+    int ih, jh, kh;
+    double t;
+    int kx, ky, kz, bx, by, bz;
+    int last;
+    int finished;
+} orc_trace;
+typedef struct {
+    int I, J, K;
+    double box[6];            /* x0 y0 z0 x1 y1 z1 */
+    double rd[3];             /* cells per unit length */
+    const double* tri12;
+    const int32_t* ids;
+    const uint32_t* cell_start;
+    const uint32_t* cell_tri;
+} orc_mesh;
+int orc_seg_tri(const double* tri, const double* p1, const double* p2, double* lambda);
+int orc_trace_begin(int I, int J, int K, const double* a, const double* b, orc_trace* s);
+int orc_trace_next(orc_trace* s);
+int orc_box_tri_overlap(const double* tri, const double* box);
+void orc_mesh_auto_box(const double* tri, int64_t ntri, double* box);
+/* returns the number of (cell, triangle) entries; fills tri12[12*ntri],
+ * kept[ntri], cell_start[I*J*K+1] and up to cap entries of cell_tri */
+int64_t orc_mesh_build(const double* tri, const int32_t* ids, int64_t ntri, int I, int J, int K, const double* box,
+                       double* tri12, int32_t* kept, uint32_t* cell_start, uint32_t* cell_tri, int64_t cap);
+int orc_mesh_detect(const orc_mesh* m, const double* p1, const double* p2, int fast, double* k, double* nrm,
+                    int32_t* id);
+void orc_material_action(int mat, double* pos, double* vel, uint32_t* seed, const double* hit, const double* n,
+                         double temperature, double mass);
+int64_t orc_mesh_boundary_f64(const orc_mesh* m, int fast, const double* const* old, double** pos, double** vel,
+                              uint32_t* seed, int64_t n, double temperature, double mass, int32_t* hit_id);
+int64_t orc_mesh_boundary_f32(const orc_mesh* m, int fast, const float* const* old, float** pos, float** vel,
+                              uint32_t* seed, int64_t n, double temperature, double mass, int32_t* hit_id);
+void orc_mesh_plant(int which);
 
 #ifdef __cplusplus
 }

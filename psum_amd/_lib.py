@@ -166,6 +166,13 @@ SIGNATURES = [
     ("psum_hip_mcc", ST, [_GP, C.c_int, C.POINTER(PsumMccProcess), C.c_int, P, C.POINTER(PsumMccSpecies),
                           C.POINTER(PsumMccSpecies), C.POINTER(PsumMccSpecies), C.c_double,
                           C.POINTER(I64), P, P, C.c_size_t, P]),
+    ("psum_mesh_create", P, [P, P, I64, C.c_int, C.c_int, C.c_int, P]),
+    ("psum_mesh_info", ST, [P, P, P]),
+    ("psum_mesh_partition", ST, [P, P, P, P]),
+    ("psum_mesh_set_fast", ST, [P, C.c_int]),
+    ("psum_hip_mesh_detect", ST, [P, C.c_int, P, P, I64, P, P, P, P]),
+    ("psum_hip_mesh_boundary", ST, [P, C.c_int, P, P, P, P, I64, C.c_double, C.c_double, P, P, P]),
+    ("psum_mesh_destroy", None, [P]),
     ("psum_hip_reduce_workspace", C.c_size_t, []),
     ("psum_hip_reduce_diag", ST, [P, I64, P, P, P]),
     ("psum_hip_field_energy_workspace", C.c_size_t, [_GP]),

@@ -69,10 +69,15 @@ __device__ __forceinline__ void rand_maxwell(Lcg& R, double temperature, double 
 
 __device__ __forceinline__ double sqn(const double* v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
 
-__device__ double cross_section(const psum_mcc_channel& c, double v2)
+// x = ln(v2 * arg_scale) is shared by consecutive channels of one scale
+// (the same value the reference computes inside each cross-section call)
+__device__ double cross_section(const psum_mcc_channel& c, double v2, double& scale, double& x)
 {
     if (c.nseg == 0) return c.coef[0][4];
-    const double x = log(v2 * c.arg_scale);
+    if (c.arg_scale != scale) {
+        scale = c.arg_scale;
+        x = log(v2 * scale);
+    }
     int k = 0;
     while (k < c.nseg - 1 && !(x < c.seg_hi[k])) ++k;
     const double* a = c.coef[k];
@@ -154,11 +159,11 @@ __device__ int run_process(const psum_mcc_process& P, double dens, double dt, do
     const double vm = sqrt(v2);
     const double r = R();
     double area[PSUM_MCC_MAX_CHAN];
-    double sum = 0;
+    double sum = 0, scale = 0, x = 0;      // arg_scale 0 is never a valid scale
 #pragma unroll
     for (int k = 0; k < PSUM_MCC_MAX_CHAN; ++k)
         if (k < P.nchan) {
-            area[k] = cross_section(P.chan[k], v2);
+            area[k] = cross_section(P.chan[k], v2, scale, x);
             sum += area[k];
         }
     int idx = -1;
@@ -378,6 +383,7 @@ psum_status psum_hip_mcc(const psum_grid* g, int dtype, const psum_mcc_process* 
             PSUM_ARG_CHECK(ch.kind >= PSUM_MCC_ELASTIC && ch.kind <= PSUM_MCC_ION_BACKWARD,
                            "psum_hip_mcc: unknown channel kind");
             PSUM_ARG_CHECK(ch.nseg >= 0 && ch.nseg <= PSUM_MCC_MAX_SEG, "psum_hip_mcc: nseg must be 0..12");
+            PSUM_ARG_CHECK(ch.nseg == 0 || ch.arg_scale > 0, "psum_hip_mcc: arg_scale must be positive");
         }
     }
     const int64_t n = incident->n;

@@ -143,6 +143,20 @@ def mcc_process(channels, density=0.0, incident_mass=9.10956e-31, target_mass=6.
     return P
 
 
+class Trace(C.Structure):
+    """orc_trace"""
+    _fields_ = [("I", C.c_int), ("J", C.c_int), ("K", C.c_int), ("p", C.c_double * 3), ("d", C.c_double * 3),
+                ("ih", C.c_int), ("jh", C.c_int), ("kh", C.c_int), ("t", C.c_double),
+                ("kx", C.c_int), ("ky", C.c_int), ("kz", C.c_int), ("bx", C.c_int), ("by", C.c_int),
+                ("bz", C.c_int), ("last", C.c_int), ("finished", C.c_int)]
+
+
+class Mesh(C.Structure):
+    """orc_mesh"""
+    _fields_ = [("I", C.c_int), ("J", C.c_int), ("K", C.c_int), ("box", C.c_double * 6), ("rd", C.c_double * 3),
+                ("tri12", C.c_void_p), ("ids", C.c_void_p), ("cell_start", C.c_void_p), ("cell_tri", C.c_void_p)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
@@ -209,6 +223,24 @@ def lib():
                                        C.c_double, P, P, P, P, P, P, i64, P]
             getattr(L, nm).restype = i64
         L.orc_mcc_plant.argtypes = [C.c_int]
+        L.orc_seg_tri.argtypes = [P, P, P, P]
+        L.orc_seg_tri.restype = C.c_int
+        L.orc_trace_begin.argtypes = [C.c_int, C.c_int, C.c_int, P, P, C.POINTER(Trace)]
+        L.orc_trace_begin.restype = C.c_int
+        L.orc_trace_next.argtypes = [C.POINTER(Trace)]
+        L.orc_trace_next.restype = C.c_int
+        L.orc_box_tri_overlap.argtypes = [P, P]
+        L.orc_box_tri_overlap.restype = C.c_int
+        L.orc_mesh_auto_box.argtypes = [P, i64, P]
+        L.orc_mesh_build.argtypes = [P, P, i64, C.c_int, C.c_int, C.c_int, P, P, P, P, P, i64]
+        L.orc_mesh_build.restype = i64
+        L.orc_mesh_detect.argtypes = [C.POINTER(Mesh), P, P, C.c_int, P, P, P]
+        L.orc_mesh_detect.restype = C.c_int
+        L.orc_material_action.argtypes = [C.c_int, P, P, P, P, P, C.c_double, C.c_double]
+        for nm in ("orc_mesh_boundary_f64", "orc_mesh_boundary_f32"):
+            getattr(L, nm).argtypes = [C.POINTER(Mesh), C.c_int, P, P, P, P, i64, C.c_double, C.c_double, P]
+            getattr(L, nm).restype = i64
+        L.orc_mesh_plant.argtypes = [C.c_int]
         L.orc_pic_set_order.argtypes = [C.c_int]
         L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int]
@@ -716,3 +748,107 @@ def mcc(g, procs, pos, vel, seed, dt, density=None, cap=None):
 
 def mcc_plant(which):
     lib().orc_mcc_plant(which)
+
+
+MAT = {"diffuse": 0, "specular": 1, "maxwell_reflect": 2, "absorb": 3}
+
+
+def seg_tri(tri, p1, p2):
+    """(hit, lambda)"""
+    t = np.ascontiguousarray(tri, np.float64).ravel()
+    a = np.ascontiguousarray(p1, np.float64)
+    b = np.ascontiguousarray(p2, np.float64)
+    lam = np.zeros(1)
+    h = lib().orc_seg_tri(_ptr(t), _ptr(a), _ptr(b), _ptr(lam))
+    return bool(h), lam[0]
+
+
+def trace_cells(I, J, K, a, b):
+    """cell indices visited by the traversal of a->b (grid coordinates)"""
+    s = Trace()
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    out = []
+    idx = lib().orc_trace_begin(I, J, K, _ptr(a), _ptr(b), C.byref(s))
+    while not s.finished:
+        out.append(idx)
+        idx = lib().orc_trace_next(C.byref(s))
+    return out
+
+
+def box_tri_overlap(tri, box):
+    t = np.ascontiguousarray(tri, np.float64).ravel()
+    bx = np.ascontiguousarray(box, np.float64)
+    return bool(lib().orc_box_tri_overlap(_ptr(t), _ptr(bx)))
+
+
+class OracleMesh:
+    """triangle_mesh_trigger built by the oracle; keeps its arrays alive"""
+
+    def __init__(self, tri, ids, I, J, K, box=None):
+        self.tri = np.ascontiguousarray(tri, np.float64).reshape(-1, 9)
+        self.ids = np.ascontiguousarray(ids, np.int32)
+        n = len(self.tri)
+        if box is None:
+            box = np.zeros(6)
+            lib().orc_mesh_auto_box(_ptr(self.tri), n, _ptr(box))
+        self.box = np.ascontiguousarray(box, np.float64)
+        self.tri12 = np.zeros((n, 12))
+        self.kept = np.zeros(n, np.int32)
+        ncell = I * J * K
+        self.cell_start = np.zeros(ncell + 1, np.uint32)
+        total = lib().orc_mesh_build(_ptr(self.tri), _ptr(self.ids), n, I, J, K, _ptr(self.box), _ptr(self.tri12),
+                                     _ptr(self.kept), _ptr(self.cell_start), None, 0)
+        self.cell_tri = np.zeros(max(total, 1), np.uint32)
+        lib().orc_mesh_build(_ptr(self.tri), _ptr(self.ids), n, I, J, K, _ptr(self.box), _ptr(self.tri12),
+                             _ptr(self.kept), _ptr(self.cell_start), _ptr(self.cell_tri), total)
+        self.cell_tri = self.cell_tri[:total]
+        m = Mesh()
+        m.I, m.J, m.K = I, J, K
+        for c in range(6):
+            m.box[c] = self.box[c]
+        for c in range(3):
+            m.rd[c] = (I, J, K)[c] / (self.box[3 + c] - self.box[c])
+        m.tri12 = self.tri12.ctypes.data
+        m.ids = self.ids.ctypes.data
+        m.cell_start = self.cell_start.ctypes.data
+        m.cell_tri = self.cell_tri.ctypes.data if total else None
+        self.c = m
+
+    def detect(self, p1, p2, fast=False):
+        """(hit, k, normal, id)"""
+        a = np.ascontiguousarray(p1, np.float64)
+        b = np.ascontiguousarray(p2, np.float64)
+        k = np.zeros(1)
+        nrm = np.zeros(3)
+        idv = np.zeros(1, np.int32)
+        h = lib().orc_mesh_detect(C.byref(self.c), _ptr(a), _ptr(b), int(fast), _ptr(k), _ptr(nrm), _ptr(idv))
+        return bool(h), k[0], nrm, int(idv[0])
+
+    def boundary(self, old, pos, vel, seed, temperature=273.15, mass=9.11e-31, fast=False):
+        """router.deal on copies; returns dict(pos, vel, seed, hits, hit_id)"""
+        t = pos[0].dtype
+        old = [np.ascontiguousarray(a, t) for a in old]
+        pos = [np.array(a, t) for a in pos]
+        vel = [np.array(a, t) for a in vel]
+        seed = np.array(seed, np.uint32)
+        n = seed.size
+        hid = np.zeros(max(n, 1), np.int32)
+        f = lib().orc_mesh_boundary_f32 if t == np.float32 else lib().orc_mesh_boundary_f64
+        hits = f(C.byref(self.c), int(fast), C.cast(_ptr_array(old), C.c_void_p), C.cast(_ptr_array(pos), C.c_void_p),
+                 C.cast(_ptr_array(vel), C.c_void_p), _ptr(seed), n, temperature, mass, _ptr(hid))
+        return dict(pos=pos, vel=vel, seed=seed, hits=hits, hit_id=hid[:n])
+
+
+def material_action(mat, pos, vel, seed, hit, nrm, temperature=273.15, mass=9.11e-31):
+    p = np.array(pos, np.float64)
+    v = np.array(vel, np.float64)
+    s = C.c_uint32(seed)
+    h = np.ascontiguousarray(hit, np.float64)
+    nn = np.ascontiguousarray(nrm, np.float64)
+    lib().orc_material_action(mat, _ptr(p), _ptr(v), C.byref(s), _ptr(h), _ptr(nn), temperature, mass)
+    return p, v, s.value
+
+
+def mesh_plant(which):
+    lib().orc_mesh_plant(which)
