@@ -321,6 +321,55 @@ def run_mcc(E, n, calls):
     return out
 
 
+def run_mesh(E, n, steps):
+    """Triangle-mesh boundary: n FP64 particles (3D) in free flight through
+    a scene of the reference's router demonstration (an absorbing box, a
+    specular 64-segment nozzle, a diffuse plane; 100^3 partition).  Each step moves x += v dt with torch and calls the router;
+    only the router calls are timed (HIP events on the launching stream)."""
+    import torch
+    from psum_amd import mesh as Mh
+    noz = Mh.revolve_zr_curve([(-4, 1.0), (-3.5, 1.0), (-3, 0.9), (-2.5, 0.8), (-2, 0.6), (-1.5, 0.5), (-1, 0.6),
+                               (-0.5, 0.8), (0, 1.0), (0.5, 1.2), (1, 1.4), (1.5, 1.6), (2, 1.8), (2.5, 1.9),
+                               (3, 2.0), (3.5, 2.0), (4, 2.0)], 64)
+    scene = [(Mh.axis_aligned_cube(-5, -5, -5, 5, 5, 10), "absorb"), (noz, "specular"),
+             (Mh.axis_aligned_plane(-5, -5, -4.0, 5, 5, -4.0), "diffuse")]
+    router = Mh.BoundaryRouter().set(scene, 100, 100, 100)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(9)
+    r = torch.sqrt(torch.rand(n, generator=gen, device="cuda", dtype=torch.float64)) * 0.7
+    th = 2 * np.pi * torch.rand(n, generator=gen, device="cuda", dtype=torch.float64)
+    pos = [r * torch.cos(th), r * torch.sin(th),
+           -4 + 0.1 * torch.rand(n, generator=gen, device="cuda", dtype=torch.float64)]
+    vel = [torch.randn(n, generator=gen, device="cuda", dtype=torch.float64) for _ in range(3)]
+    seed = torch.randint(-2**31, 2**31 - 1, (n,), generator=gen, device="cuda", dtype=torch.int32)
+    del r, th
+    dt = 0.005
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    old = [torch.empty_like(p) for p in pos]
+    hits = 0
+    for k in range(steps):
+        for c in range(3):
+            old[c].copy_(pos[c])
+            pos[c].add_(vel[c] * dt)
+        ev[k][0].record()
+        h, _ = router.deal(old, pos, vel, seed)
+        ev[k][1].record()
+        hits += h
+    torch.cuda.synchronize()
+    ms = sum(a.elapsed_time(b) for a, b in ev[1:]) / (steps - 1)
+    alive = int(torch.isfinite(pos[0]).sum().item())
+    out = {"workload": f"{n} FP64 particles, router scene (box/nozzle/plane, {len(router.trigger.tri)} triangles, "
+                       f"100^3 cells), dt 0.005, {steps} steps", "particles": n,
+           "ms_per_call": ms, "segments_per_s": n / (ms * 1e-3), "hits": hits, "alive_at_end": alive,
+           "algorithmic_GBps": n * 48 / (ms * 1e-3) / 1e9}
+    log(f"[bench] mesh: {ms:.3f} ms per router call for {n} particles ({n / (ms * 1e-3):.3e} segments/s), "
+        f"{hits} hits over {steps} steps, {alive} alive")
+    router.trigger.close()
+    del pos, vel, old, seed
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_c3(E, cells, ppc, steps, depth, pre, post):
     """3D bounded box (phi = 0 on the faces, absorbing walls, compaction by
     the sort), electrons + ions (mass ratio 1836, Te/Ti = 10), FP32
@@ -387,6 +436,9 @@ def main():
     ap.add_argument("--mcc-particles", type=int, default=100_000_000)
     ap.add_argument("--mcc-calls", type=int, default=20)
     ap.add_argument("--no-mcc", action="store_true")
+    ap.add_argument("--mesh-particles", type=int, default=20_000_000)
+    ap.add_argument("--mesh-steps", type=int, default=20)
+    ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--c3", action="store_true", help="also run the 3D bounded box (256^3, 1.07e9 particles)")
     ap.add_argument("--c3-cells", type=int, default=256)
     ap.add_argument("--c3-ppc", type=int, default=32)
@@ -453,6 +505,9 @@ def main():
     mcc = None
     if not args.no_mcc:
         mcc = run_mcc(E, args.mcc_particles, args.mcc_calls)
+    mesh = None
+    if not args.no_mesh:
+        mesh = run_mesh(E, args.mesh_particles, args.mesh_steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_steps)
@@ -493,6 +548,7 @@ def main():
         "batch": batch,
         "c3": c3,
         "mcc": mcc,
+        "mesh": mesh,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
