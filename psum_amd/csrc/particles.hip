@@ -433,13 +433,16 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
     const int64_t p0 = (int64_t)blockIdx.x * kPushTile + (int64_t)threadIdx.x * kPPT;
     const bool full = p0 + kPPT <= n;
 
-    // load the thread's 4 particles (component-major registers)
+    // load the thread's kPPT particles (component-major registers)
     T X[D][kPPT], V[D][kPPT];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         if (full) {
-            load4(xs[d] + p0, X[d]);
-            if (PUSH) load4(vs[d] + p0, V[d]);
+#pragma unroll
+            for (int q = 0; q < kPPT; q += 4) {
+                load4(xs[d] + p0 + q, X[d] + q);
+                if (PUSH) load4(vs[d] + p0 + q, V[d] + q);
+            }
         } else {
 #pragma unroll
             for (int j = 0; j < kPPT; ++j) {
@@ -494,8 +497,11 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             if (full) {
-                store4(xs[d] + p0, X[d]);
-                store4(vs[d] + p0, V[d]);
+#pragma unroll
+                for (int q = 0; q < kPPT; q += 4) {
+                    store4(xs[d] + p0 + q, X[d] + q);
+                    store4(vs[d] + p0 + q, V[d] + q);
+                }
             } else {
 #pragma unroll
                 for (int j = 0; j < kPPT; ++j)
