@@ -454,7 +454,9 @@ psum_status psum_poisson_assemble_ex(const psum_grid* g, const psum_poisson_spec
  * hipGraph and replayed.  Inside the graph each solver runs a fixed V-cycle
  * count; sorts run between replays at per-instance intervals, on 2^shift-cell
  * bins.  psum_batch_tune is the autotuner: per instance, `warmup`
- * tolerance-controlled steps calibrate the cycle count (most needed + 1),
+ * tolerance-controlled steps calibrate the cycle count (most needed after
+ * the first, cold-started step, + 1: replayed solves start from the
+ * previous step's potential),
  * then each bin shift 0..2 gets a timed sort and `probe` timed steps, and the
  * (shift, interval) minimising the modelled sort + push cost is chosen.
  * psum_batch_configure sets the same three values by hand.

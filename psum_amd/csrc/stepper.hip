@@ -956,8 +956,12 @@ public:
     {
         if (warmup < 1 || probe < 2) throw std::invalid_argument("tune: warmup >= 1, probe >= 2");
         static const int kCand[] = {5, 10, 20, 50, 100, 200};
+        // the replayed step warm-starts every solve from the previous
+        // potential: the first warm-up step (a solve from a cold start) does
+        // not enter the calibration
+        if (warmup > 1) run_steps(1, nullptr, true, false);
         reset_cycles_seen();
-        run_steps(warmup, nullptr, true, false);
+        run_steps(warmup > 1 ? warmup - 1 : 1, nullptr, true, false);
         const int cycles = max_cycles_seen() + 1;
         int best_shift = 0, best_k = 0;
         double best_cost = 1e300, best_sort = 0.0, best_push = 0.0;
