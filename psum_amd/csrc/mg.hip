@@ -288,6 +288,260 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
     xf[i] = s * (T)1 + xf[i] * (T)1;
 }
 
+// ---- 2D V-cycle tail in one workgroup ------------------------------------
+// The coarse end of the hierarchy (levels lt..Lc, at most kMaxTail) is
+// latency-bound: a handful of nodes per level, three launches per level.
+// k_vtail runs all of it in one 1024-thread workgroup.  It first stages in
+// LDS every tail level's class ids, class tables and transfer tables, the
+// iterate and right-hand side of level lt, and then keeps every level's two
+// iterates and right-hand side there: pre-sweeps, residual, restriction down
+// to the coarsest level, its dense solve (the inverse is read from global
+// memory), then prolongation and post-sweeps back up to level lt, whose
+// iterate it writes back.  Each phase is the per-node arithmetic of
+// k_smooth, k_residual, k_restrict, k_coarse_solve and k_prolong_add in the
+// same order, so the result is bit-identical to the launch-per-phase path.
+constexpr int kMaxTail = 6;
+constexpr int kTailThreads = 1024;
+// coarsest nodes: one compute unit reads the dense inverse (n^2 doubles)
+// from memory at a few tens of GB/s, so it must stay small
+constexpr int kTailCoarse = 100;
+// one compute unit runs the tail: 64 lanes of VALU per clock, so a level of
+// 64^2 nodes already costs what its own multi-block launches cost; the tail
+// takes levels of at most 33^2 nodes
+constexpr int kTailLevelNodes = 1089;
+constexpr int kTailRows = (kTailCoarse + kTailThreads / 64 - 1) / (kTailThreads / 64);   // rows per wave
+
+template <typename T>
+struct TailLevel {
+    int n0, n1, wrap0, wrap1, N, compact, ncls;
+    // byte offsets into the workgroup's LDS
+    int ox, oy, ob, ocls, orx, oa, odinv;
+    const uint16_t* cls;
+    const T* rx;          // relax rows, ncls x 9
+    const T* a;           // scaled operator rows, ncls x 9
+    const T* dinv;        // ncls
+};
+
+template <typename T>
+struct TailTransfer {
+    int wrap0, wrap1;
+    // byte offsets: restriction (per coarse index) and prolongation (per fine index) tables
+    int orf0, orf1, orn0, orn1, orw0, orw1, opc0, opc1, opw0, opw1;
+    const int *rf0, *rf1, *rn0, *rn1, *pc0, *pc1;
+    const T *rw0, *rw1, *pw0, *pw1;
+};
+
+template <typename T>
+struct TailArgs {
+    int nl;                         // tail levels; the last is the coarsest
+    int pre, post;
+    T two3;
+    TailLevel<T> lev[kMaxTail];
+    TailTransfer<T> tr[kMaxTail - 1];
+    const T* x_in;                  // level lt iterate after the restriction
+    const T* b_in;                  // level lt right-hand side
+    T* x_out;
+    const double* cinv;
+};
+
+template <typename U>
+__device__ __forceinline__ void tail_copy(unsigned char* lds, int off, const U* __restrict__ g, int n)
+{
+    U* d = reinterpret_cast<U*>(lds + off);
+    for (int i = threadIdx.x; i < n; i += kTailThreads) d[i] = g[i];
+}
+
+// sum_k coef[k] * x[nb_k] over the 3x3 offsets (row offset slowest), clamped
+// or wrapped neighbours; COMPACT: the 5 axis offsets (stencil_apply<T,2,.>)
+template <typename T, bool COMPACT>
+__device__ __forceinline__ T tail_stencil(const T* coef, const T* x, int c0, int c1, int n0, int n1, int w0,
+                                          int w1)
+{
+    T s = (T)0;
+#pragma unroll
+    for (int a = -1; a <= 1; ++a)
+#pragma unroll
+        for (int b = -1; b <= 1; ++b) {
+            if (COMPACT && a != 0 && b != 0) continue;
+            const int k = (a + 1) * 3 + (b + 1);
+            const int q0 = nb_index(c0, a, n0, w0);
+            const int q1 = nb_index(c1, b, n1, w1);
+            s = s + coef[k] * x[q0 * n1 + q1];
+        }
+    return s;
+}
+
+// out = stencil(rows, x) * m + b * f for every node of level t
+template <typename T>
+__device__ __forceinline__ void tail_apply(unsigned char* lds, const TailLevel<T>& t, int orows, const T* x,
+                                           T* out, T m, T f)
+{
+    const uint16_t* cls = reinterpret_cast<const uint16_t*>(lds + t.ocls);
+    const T* rows = reinterpret_cast<const T*>(lds + orows);
+    const T* b = reinterpret_cast<const T*>(lds + t.ob);
+    for (int i = threadIdx.x; i < t.N; i += kTailThreads) {
+        const int c0 = i / t.n1, c1 = i - c0 * t.n1;
+        const T* coef = rows + (int)cls[i] * 9;
+        const T v = t.compact ? tail_stencil<T, true>(coef, x, c0, c1, t.n0, t.n1, t.wrap0, t.wrap1)
+                              : tail_stencil<T, false>(coef, x, c0, c1, t.n0, t.n1, t.wrap0, t.wrap1);
+        out[i] = v * m + b[i] * f;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_vtail(TailArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int nl = a.nl, Lc = a.nl - 1;
+    // stage tables, level lt's iterate and right-hand side
+    for (int l = 0; l < nl; ++l) {
+        const TailLevel<T>& t = a.lev[l];
+        tail_copy<uint16_t>(lds, t.ocls, t.cls, t.N);
+        tail_copy<T>(lds, t.orx, t.rx, t.ncls * 9);
+        tail_copy<T>(lds, t.oa, t.a, t.ncls * 9);
+        tail_copy<T>(lds, t.odinv, t.dinv, t.ncls);
+        if (l < Lc) {
+            const TailTransfer<T>& r = a.tr[l];
+            const TailLevel<T>& c = a.lev[l + 1];
+            tail_copy<int>(lds, r.orf0, r.rf0, c.n0);
+            tail_copy<int>(lds, r.orf1, r.rf1, c.n1);
+            tail_copy<int>(lds, r.orn0, r.rn0, c.n0);
+            tail_copy<int>(lds, r.orn1, r.rn1, c.n1);
+            tail_copy<T>(lds, r.orw0, r.rw0, 3 * c.n0);
+            tail_copy<T>(lds, r.orw1, r.rw1, 3 * c.n1);
+            tail_copy<int>(lds, r.opc0, r.pc0, t.n0);
+            tail_copy<int>(lds, r.opc1, r.pc1, t.n1);
+            tail_copy<T>(lds, r.opw0, r.pw0, 2 * t.n0);
+            tail_copy<T>(lds, r.opw1, r.pw1, 2 * t.n1);
+        }
+    }
+    tail_copy<T>(lds, a.lev[0].ox, a.x_in, a.lev[0].N);
+    tail_copy<T>(lds, a.lev[0].ob, a.b_in, a.lev[0].N);
+    __syncthreads();
+    // bit l: which of level l's two LDS iterates holds its current iterate
+    unsigned cur = 0;
+    auto X = [&](int l) {
+        return reinterpret_cast<T*>(lds + (((cur >> l) & 1u) ? a.lev[l].oy : a.lev[l].ox));
+    };
+    auto Y = [&](int l) {
+        return reinterpret_cast<T*>(lds + (((cur >> l) & 1u) ? a.lev[l].ox : a.lev[l].oy));
+    };
+    for (int l = 0; l < Lc; ++l) {
+        const TailLevel<T>& t = a.lev[l];
+        // coarse levels received their first sweep from the restriction
+        const int sweeps = a.pre > 0 ? a.pre - 1 : 0;
+        for (int s = 0; s < sweeps; ++s) {
+            tail_apply<T>(lds, t, t.orx, X(l), Y(l), (T)1, a.two3 * (T)1);
+            __syncthreads();
+            cur ^= 1u << l;
+        }
+        // r = (A x)*(-1) + b*1 into the free iterate
+        T* r = Y(l);
+        tail_apply<T>(lds, t, t.oa, X(l), r, (T)-1, (T)1);
+        __syncthreads();
+        // b_c = (R r) o D_c^-1 ; x_c = b_c * (2/3) (first sweep) or 0
+        const TailLevel<T>& cl = a.lev[l + 1];
+        const TailTransfer<T>& tr = a.tr[l];
+        const int* rf0 = reinterpret_cast<const int*>(lds + tr.orf0);
+        const int* rf1 = reinterpret_cast<const int*>(lds + tr.orf1);
+        const int* rn0 = reinterpret_cast<const int*>(lds + tr.orn0);
+        const int* rn1 = reinterpret_cast<const int*>(lds + tr.orn1);
+        const T* rw0 = reinterpret_cast<const T*>(lds + tr.orw0);
+        const T* rw1 = reinterpret_cast<const T*>(lds + tr.orw1);
+        const uint16_t* ccls = reinterpret_cast<const uint16_t*>(lds + cl.ocls);
+        const T* cdinv = reinterpret_cast<const T*>(lds + cl.odinv);
+        const T fs = (l + 1 < Lc && a.pre > 0) ? a.two3 : (T)0;
+        T* bc = reinterpret_cast<T*>(lds + cl.ob);
+        T* xc = reinterpret_cast<T*>(lds + cl.ox);
+        for (int i = threadIdx.x; i < cl.N; i += kTailThreads) {
+            const int c0 = i / cl.n1, c1 = i - c0 * cl.n1;
+            T sum = (T)0;
+            const int f0 = rf0[c0], m0 = rn0[c0];
+            const int f1 = rf1[c1], m1 = rn1[c1];
+            for (int p = 0; p < m0; ++p) {
+                const T w0 = rw0[3 * c0 + p];
+                const int i0 = wrap_index(f0 + p, t.n0, tr.wrap0);
+                for (int q = 0; q < m1; ++q) {
+                    const T w = w0 * rw1[3 * c1 + q];
+                    const int i1 = wrap_index(f1 + q, t.n1, tr.wrap1);
+                    sum = sum + w * r[i0 * t.n1 + i1];
+                }
+            }
+            const T bv = sum * cdinv[ccls[i]];
+            bc[i] = bv;
+            xc[i] = fs != (T)0 ? bv * (fs * (T)1) : (T)0;
+        }
+        cur &= ~(1u << (l + 1));
+        __syncthreads();
+    }
+    {
+        // x = Ainv b on the coarsest level: one wave per row, lane-strided
+        // FP64 partials and a butterfly (k_coarse_solve)
+        const int n = a.lev[Lc].N;
+        const T* b = reinterpret_cast<const T*>(lds + a.lev[Lc].ob);
+        T* x = reinterpret_cast<T*>(lds + a.lev[Lc].ox);
+        // rows wave, wave + 16, ... (at most kTailRows per wave) are summed
+        // together so that all their loads are in flight at once; each row
+        // still adds its terms in ascending j
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        double sacc[kTailRows];
+#pragma unroll
+        for (int k = 0; k < kTailRows; ++k) sacc[k] = 0.0;
+        for (int j = lane; j < n; j += 64) {
+            const double bj = (double)b[j];
+#pragma unroll
+            for (int k = 0; k < kTailRows; ++k) {
+                const int row = wave + k * (kTailThreads / 64);
+                if (row < n) sacc[k] += a.cinv[(int64_t)row * n + j] * bj;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kTailRows; ++k) {
+            const int row = wave + k * (kTailThreads / 64);
+            const double v = wave_sum(sacc[k]);
+            if (row < n && lane == 0) x[row] = (T)v;
+        }
+        __syncthreads();
+    }
+    for (int l = Lc - 1; l >= 0; --l) {
+        const TailLevel<T>& t = a.lev[l];
+        const TailLevel<T>& cl = a.lev[l + 1];
+        const TailTransfer<T>& tr = a.tr[l];
+        const int* pc0 = reinterpret_cast<const int*>(lds + tr.opc0);
+        const int* pc1 = reinterpret_cast<const int*>(lds + tr.opc1);
+        const T* pw0 = reinterpret_cast<const T*>(lds + tr.opw0);
+        const T* pw1 = reinterpret_cast<const T*>(lds + tr.opw1);
+        const T* xc = X(l + 1);
+        T* xf = X(l);
+        // x_f = (P x_c)*1 + x_f*(1*1)
+        for (int i = threadIdx.x; i < t.N; i += kTailThreads) {
+            const int c0 = i / t.n1, c1 = i - c0 * t.n1;
+            T sum = (T)0;
+            const int p0 = pc0[c0], p1 = pc1[c1];
+            const T* w0 = pw0 + 2 * c0;
+            const T* w1 = pw1 + 2 * c1;
+            for (int p = 0; p < 2; ++p) {
+                if (w0[p] == (T)0) continue;
+                const int i0 = wrap_index(p0 + p, cl.n0, tr.wrap0);
+                for (int q = 0; q < 2; ++q) {
+                    if (w1[q] == (T)0) continue;
+                    const int i1 = wrap_index(p1 + q, cl.n1, tr.wrap1);
+                    sum = sum + (w0[p] * w1[q]) * xc[i0 * cl.n1 + i1];
+                }
+            }
+            xf[i] = sum * (T)1 + xf[i] * (T)1;
+        }
+        __syncthreads();
+        for (int s = 0; s < a.post; ++s) {
+            tail_apply<T>(lds, t, t.orx, X(l), Y(l), (T)1, a.two3 * (T)1);
+            __syncthreads();
+            cur ^= 1u << l;
+        }
+    }
+    const T* x0 = X(0);
+    for (int i = threadIdx.x; i < a.lev[0].N; i += kTailThreads) a.x_out[i] = x0[i];
+}
+
 // ---- fused 2D kernels: several Jacobi sweeps of one tile in LDS ----------
 // A block owns a tile of kTileRows x (kTileX - 2h) output nodes and loads it
 // with a halo of h = `sweeps` nodes (kTileX = 64 columns: one wave per row,
@@ -1259,6 +1513,7 @@ public:
             } else if (key == "tol") iss >> tol_;
             else if (key == "periodic_mode") iss >> periodic_mode_;
             else if (key == "fuse") iss >> fuse_;
+            else if (key == "tail") { iss >> tail_; tail32_ = tail64_ = -2; }
             else if (key == "graph") iss >> use_graph_;
             else if (key == "num_conv_mask" || key == "num_conv_mask_relax" ||
                      key == "num_conv_mask_restrict" || key == "num_conv_mask_interpolate" ||
@@ -1330,6 +1585,7 @@ public:
         tr_host_ = std::move(T);
         rr_fit_.clear();
         upload();
+        tail32_ = tail64_ = -2;
     }
 
     void set_source_replace(unsigned long long n, const unsigned long long* idx, const double* v)
@@ -1460,6 +1716,9 @@ private:
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     int fuse_ = 1;   // 2D: fused multi-sweep tile smoothing and residual+restriction
+    int tail_ = 1;   // 2D: the coarse end of the V-cycle in one workgroup (k_vtail)
+    int tail32_ = -2, tail64_ = -2;    // first tail level per precision (-1 none, -2 not yet chosen)
+    size_t tail_bytes32_ = 0, tail_bytes64_ = 0;
     std::vector<int> rr_fit_;
     double tol_ = 0.0;
     int predicted_ = 1;                // cycles the previous solve needed
@@ -1734,6 +1993,129 @@ private:
             hipLaunchKernelGGL((k_prolong_add<T, 3>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xc, x);
     }
 
+    // First level of the one-workgroup tail for precision T, or -1: a 2D
+    // fused hierarchy whose levels lt..Lc (two or more, at most kMaxTail,
+    // lt >= 1, at most kTailLevelNodes nodes each) keep their iterates and
+    // right-hand sides in LDS and whose coarsest level has at most kTailCoarse
+    // nodes (its dense solve reads the inverse from one compute unit).
+    template <typename T>
+    int tail_start()
+    {
+        int& c = std::is_same<T, double>::value ? tail64_ : tail32_;
+        size_t& bytes = std::is_same<T, double>::value ? tail_bytes64_ : tail_bytes32_;
+        if (c != -2) return c;
+        c = -1;
+        const int nl = (int)host_.size(), Lc = nl - 1;
+        if (!tail_ || !fuse_ || nl < 3 || host_[Lc].N > kTailCoarse) return c;
+        for (const auto& h : host_)
+            if (h.D != 2) return c;
+        int dev = 0, lim = 0;
+        MG_HIP(hipGetDevice(&dev));
+        MG_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        for (int lt = 1; lt <= Lc - 1; ++lt) {
+            if (Lc - lt + 1 > kMaxTail || host_[lt].N > kTailLevelNodes) continue;
+            TailArgs<T> probe{};
+            const size_t b = tail_layout<T>(lt, probe);
+            if (b > (size_t)lim) continue;
+            if (b > 65536 &&
+                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vtail<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            c = lt;
+            bytes = b;
+            break;
+        }
+        return c;
+    }
+
+    // Fills the tail arguments for levels lt..Lc; returns the LDS bytes
+    template <typename T>
+    size_t tail_layout(int lt, TailArgs<T>& a) const
+    {
+        const int Lc = (int)host_.size() - 1;
+        size_t off = 0;
+        auto take = [&](size_t nbytes) {
+            const size_t o = off;
+            off = (off + nbytes + 15) & ~(size_t)15;
+            return (int)o;
+        };
+        a.nl = Lc - lt + 1;
+        for (int k = 0; k < a.nl; ++k) {
+            const int l = lt + k;
+            const HostLevel& h = host_[l];
+            TailLevel<T>& t = a.lev[k];
+            t.n0 = h.n[0];
+            t.n1 = h.n[1];
+            t.wrap0 = h.wrap[0];
+            t.wrap1 = h.wrap[1];
+            t.N = (int)h.N;
+            t.compact = h.compact ? 1 : 0;
+            t.ncls = (int)h.dinv.size();
+            t.ox = take((size_t)h.N * sizeof(T));
+            t.oy = l < Lc ? take((size_t)h.N * sizeof(T)) : t.ox;
+            t.ob = take((size_t)h.N * sizeof(T));
+            t.ocls = take((size_t)h.N * sizeof(uint16_t));
+            t.orx = take((size_t)t.ncls * 9 * sizeof(T));
+            t.oa = take((size_t)t.ncls * 9 * sizeof(T));
+            t.odinv = take((size_t)t.ncls * sizeof(T));
+            if (!dev_.empty()) {
+                const DevTable<T> rx = table<T>(l, true), A = table<T>(l, false);
+                t.cls = dev_[l].cls;
+                t.rx = rx.coef;
+                t.a = A.coef;
+                t.dinv = A.dinv;
+            }
+        }
+        for (int k = 0; k + 1 < a.nl; ++k) {
+            const int l = lt + k;
+            const HostLevel &f = host_[l], &c = host_[l + 1];
+            TailTransfer<T>& r = a.tr[k];
+            r.wrap0 = tr_host_[l][0].wrap;
+            r.wrap1 = tr_host_[l][1].wrap;
+            r.orf0 = take((size_t)c.n[0] * sizeof(int));
+            r.orf1 = take((size_t)c.n[1] * sizeof(int));
+            r.orn0 = take((size_t)c.n[0] * sizeof(int));
+            r.orn1 = take((size_t)c.n[1] * sizeof(int));
+            r.orw0 = take((size_t)3 * c.n[0] * sizeof(T));
+            r.orw1 = take((size_t)3 * c.n[1] * sizeof(T));
+            r.opc0 = take((size_t)f.n[0] * sizeof(int));
+            r.opc1 = take((size_t)f.n[1] * sizeof(int));
+            r.opw0 = take((size_t)2 * f.n[0] * sizeof(T));
+            r.opw1 = take((size_t)2 * f.n[1] * sizeof(T));
+            if (!dtr_.empty()) {
+                const DevTransfer<T> d = transfer<T>(l);
+                r.rf0 = d.rf[0];
+                r.rf1 = d.rf[1];
+                r.rn0 = d.rn[0];
+                r.rn1 = d.rn[1];
+                r.rw0 = d.rw[0];
+                r.rw1 = d.rw[1];
+                r.pc0 = d.pc[0];
+                r.pc1 = d.pc[1];
+                r.pw0 = d.pw[0];
+                r.pw1 = d.pw[1];
+            }
+        }
+        return off;
+    }
+
+    template <typename T>
+    void launch_tail(int lt, T* x, const T* b)
+    {
+        TailArgs<T> a{};
+        const size_t bytes = tail_layout<T>(lt, a);
+        a.pre = pre_;
+        a.post = post_;
+        a.two3 = (T)(2.0 / 3);
+        a.x_in = x;
+        a.b_in = b;
+        a.x_out = x;
+        a.cinv = cinv_;
+        hipLaunchKernelGGL((k_vtail<T>), dim3(1), dim3(kTailThreads), bytes, stream_, a);
+    }
+
     // One or more V-cycles on level-0 arrays (b0 scaled, x0 warm start).
     // xs[l] / ts[l] are the ping-pong pair of level l, bs[l] its rhs.
     // first_swept: xs[0] already holds the first level-0 sweep from zero.
@@ -1747,8 +2129,10 @@ private:
         const int Lc = nl - 1;
         const T two3 = (T)(2.0 / 3);
         bool accd = false;
+        const int lt = tail_start<T>();          // > 0: levels lt..Lc run in k_vtail
+        const int ld = lt > 0 ? lt : Lc;
         for (int cyc = 0; cyc < cycles; ++cyc) {
-            for (int l = 0; l < Lc; ++l) {
+            for (int l = 0; l < ld; ++l) {
                 // coarse levels received their first sweep from the restriction
                 const bool swept = (l > 0 || (cyc == 0 && first_swept)) && pre_ > 0;
                 smooth_n<T>(l, xs[l], ts[l], bs[l], swept ? pre_ - 1 : pre_);
@@ -1774,12 +2158,14 @@ private:
                 else
                     hipLaunchKernelGGL((k_restrict<T, 3>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
             }
-            {
+            if (lt > 0) {
+                launch_tail<T>(lt, xs[lt], bs[lt]);
+            } else {
                 const int n = (int)host_[Lc].N;
                 const unsigned nb = (unsigned)(((int64_t)n * 64 + kMgThreads - 1) / kMgThreads);
                 hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nb), dim3(kMgThreads), 0, stream_, cinv_, n, bs[Lc], xs[Lc]);
             }
-            for (int l = Lc - 1; l >= 0; --l) {
+            for (int l = ld - 1; l >= 0; --l) {
                 if (post_ > 0) {
                     double* a = (l == 0 && cyc == cycles - 1) ? acc : nullptr;
                     accd = smooth_n<T>(l, xs[l], ts[l], bs[l], post_, xs[l + 1], a) || accd;

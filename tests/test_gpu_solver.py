@@ -264,3 +264,34 @@ def test_fused_2d_kernels_are_bit_identical(backend, case, prepost, stop):
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1][0] == out[1][1][0]
     assert np.array_equal(np.float64(out[0][1][1]), np.float64(out[1][1][1]), equal_nan=True)
+
+
+@pytest.mark.parametrize("backend", ["hip_multigrid_gpu", "hip_multigrid32f_gpu"])
+@pytest.mark.parametrize("case", ["periodic256_d6", "periodic128_d4", "walls_129x97_d5"])
+@pytest.mark.parametrize("prepost", [(3, 3), (2, 1), (0, 2)])
+def test_vcycle_tail_is_bit_identical(backend, case, prepost):
+    """The one-workgroup tail (levels from the first that fits in LDS down to
+    the coarsest, in one launch) performs the per-level kernels' operations
+    in the same order: equal bits with the tail switched off."""
+    E = _E()
+    if case.startswith("periodic"):
+        n = int(case[8:case.index("_")])
+        og, nn, r, c, v, b = _periodic_problem(n, seed=5)
+        I, J = n, n
+    else:
+        I, J = 129, 97
+        og, nn, r, c, v, b = _wall_problem_2d(I, J, seed=6)
+    depth = int(case[case.rindex("d") + 1:])
+    pre, post = prepost
+    out = []
+    for tail in (0, 1):
+        s = E.SolverBackend(backend)
+        s.set_options(f"I {I} J {J} depth {depth} pre_relax {pre} post_relax {post} "
+                      f"max_cycles 30 tol 1e-9 tail {tail}")
+        s.set_matrix(nn, nn, r, c, v)
+        x = torch.zeros(nn, dtype=torch.float64, device="cuda")
+        s.solve(_dev(b), x)
+        s.solve(_dev(b * 1.01), x)
+        out.append((x.cpu().numpy(), s.stats()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1][0] == out[1][1][0]
