@@ -1716,7 +1716,9 @@ private:
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     int fuse_ = 1;   // 2D: fused multi-sweep tile smoothing and residual+restriction
-    int tail_ = 1;   // 2D: the coarse end of the V-cycle in one workgroup (k_vtail)
+    // 2D: the coarse end of the V-cycle in one workgroup (k_vtail); off unless
+    // "tail 1" is set
+    int tail_ = 0;
     int tail32_ = -2, tail64_ = -2;    // first tail level per precision (-1 none, -2 not yet chosen)
     size_t tail_bytes32_ = 0, tail_bytes64_ = 0;
     std::vector<int> rr_fit_;

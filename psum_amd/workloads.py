@@ -162,7 +162,7 @@ def ensemble(n_inst=64, seed=11, dtype=np.float32, sizes=(64, 128, 256, 512), pp
             vx = torch.randn(N, generator=gen, device=device, dtype=tdt)
             vy = torch.randn(N, generator=gen, device=device, dtype=tdt)
         sp = dict(q=-1.0, m=1.0, w=1.0 / ppc, pos=[x, y], vel=[vx, vy])
-        depth = max(2, int(np.log2(n)) - 3)      # coarsest level 8^2
+        depth = max(2, int(np.log2(n)) - 4)
         out.append(dict(g=periodic_grid([n, n]), species=[sp],
                         dtype="f32" if dtype == np.float32 else "f64",
                         mixed_mg=dtype == np.float32, depth=depth, pre=2, post=2, max_cycles=40,
