@@ -314,6 +314,10 @@ __global__ __launch_bounds__(kThreads) void k_compact_scatter(PtrPack pk, int nc
 constexpr int kPPT = 4;
 constexpr int kPushTile = kThreads * kPPT;
 constexpr int kTileNodes = 2048;
+// 3D blocks get a larger tile (32 KB): a block's box of drifted particles
+// grows with the cube of the drift
+template <int D>
+constexpr int tile_nodes() { return D == 3 ? 2 * kTileNodes : kTileNodes; }
 
 template <typename T>
 __device__ __forceinline__ void load4(const T* __restrict__ p, T* v)
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
     double mass, double w_dep, double scale, unsigned long long* __restrict__ acc,
     double* __restrict__ diag, unsigned long long* __restrict__ counters)
 {
-    __shared__ unsigned long long tile[DEPOSIT ? kTileNodes : 1];
+    __shared__ unsigned long long tile[DEPOSIT ? tile_nodes<D>() : 1];
     __shared__ int bmin[3], bmax[3], bref[3];
     __shared__ double wdiag[kThreads / 64][4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -583,7 +587,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
             e[d] = bmax[d] - bmin[d] + 2;   // support adds one node
             tsize *= any ? (int64_t)e[d] : 0;
         }
-        const bool use_lds = any && tsize <= kTileNodes;
+        const bool use_lds = any && tsize <= tile_nodes<D>();
         if (use_lds)
             for (int i = threadIdx.x; i < (int)tsize; i += kThreads) tile[i] = 0ull;
         __syncthreads();
