@@ -11,6 +11,7 @@
 #include "device_field.hpp"
 #include "device_vector.hpp"
 #include "grid.hpp"
+#include "particle_boundary.hpp"
 #include "particle_group.hpp"
 #include "poisson_solver.hpp"
 #include "serialization.hpp"
@@ -38,7 +39,14 @@ using psum::field::node_field2D;
 using psum::field::node_field3D;
 using psum::field::simple_grid;
 using psum::field::var_loc;
+using psum::particle_boundary::basic_material_set;
+using psum::particle_boundary::boundary_router;
 using psum::particle_boundary::box_absorber;
+using psum::particle_boundary::triangle_mesh_trigger;
+using psum::particle_boundary::triangle_mesh_with_id;
+namespace geometry = psum::particle_boundary::geometry;
+namespace mesh_generator = psum::particle_boundary::mesh_generator;
+namespace stl_loader = psum::particle_boundary::stl_loader;
 using psum::particle_boundary::periodic_wrap;
 using psum::particle_container::device_vector;
 using psum::particle_container::particle_group;

@@ -45,6 +45,10 @@ struct velocity {
 struct weight {
     static constexpr const char* tag_name = "weight";
 };
+// the particle's own random stream state, a uint32_t (property.hpp:82)
+struct random_seed {
+    static constexpr const char* tag_name = "random_seed";
+};
 }  // namespace property
 
 template <typename Tag, typename Type>

@@ -280,6 +280,19 @@ psum_status psum_hip_mesh_boundary(void* m, int dtype, const void* const* old, v
                                    void* const* vel, uint32_t* seed, int64_t n, double temperature,
                                    double mass, int32_t* hit_id, unsigned long long* hits, void* stream);
 void psum_mesh_destroy(void* m);
+/* Device view of a partition for kernels compiled elsewhere (the header-only
+ * C++ API, include/psum/particle_boundary.hpp): valid until
+ * psum_mesh_destroy. */
+typedef struct {
+    int32_t I, J, K, fast;
+    double box[6];
+    double rd[3];                 /* cells per unit length */
+    const uint32_t* cell_start;   /* I*J*K + 1 */
+    const uint32_t* cell_tri;
+    const double* tri12;          /* 12 per triangle: vertices, unit normal */
+    const int32_t* ids;
+} psum_mesh_view;
+psum_status psum_mesh_get_view(void* m, psum_mesh_view* out);
 
 /* out[k] = sum over blocks of diag[4*b + k], k < 4, in a fixed order
  * (deterministic).  work: psum_hip_reduce_workspace() bytes of device

@@ -142,6 +142,19 @@ class PsumMccSpecies(C.Structure):
     ]
 
 
+class PsumMeshView(C.Structure):
+    """psum_mesh_view (include/psum_hip.h): device view of a partition."""
+    _fields_ = [
+        ("I", C.c_int32), ("J", C.c_int32), ("K", C.c_int32), ("fast", C.c_int32),
+        ("box", C.c_double * 6),
+        ("rd", C.c_double * 3),
+        ("cell_start", C.c_void_p),
+        ("cell_tri", C.c_void_p),
+        ("tri12", C.c_void_p),
+        ("ids", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 I64 = C.c_int64
 ST = C.c_int
@@ -172,6 +185,7 @@ SIGNATURES = [
     ("psum_mesh_set_fast", ST, [P, C.c_int]),
     ("psum_hip_mesh_detect", ST, [P, C.c_int, P, P, I64, P, P, P, P]),
     ("psum_hip_mesh_boundary", ST, [P, C.c_int, P, P, P, P, I64, C.c_double, C.c_double, P, P, P]),
+    ("psum_mesh_get_view", ST, [P, C.POINTER(PsumMeshView)]),
     ("psum_mesh_destroy", None, [P]),
     ("psum_hip_reduce_workspace", C.c_size_t, []),
     ("psum_hip_reduce_diag", ST, [P, I64, P, P, P]),

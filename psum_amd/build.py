@@ -45,7 +45,7 @@ def _compile(src):
     return obj
 
 
-CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "api_step.cpp")]
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "api_step.cpp"), os.path.join(ROOT, "tests", "cpp", "mesh_router.cpp")]
 
 
 def build_cpp_tests(verbose=True):

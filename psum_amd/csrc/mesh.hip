@@ -24,24 +24,13 @@
 #include <vector>
 
 #include "common.hpp"
-#include "mesh_geom.hpp"
+#include "psum_mesh_geom.hpp"
 
 namespace psum_hip {
 namespace {
 
 constexpr int kMeshThreads = 256;
 inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
-
-struct DevMesh {
-    int I, J, K;
-    int fast;
-    double box[6];
-    double rd[3];
-    const uint32_t* cell_start;
-    const uint32_t* cell_tri;
-    const double* tri12;
-    const int32_t* ids;
-};
 
 struct Mesh {
     int I = 0, J = 0, K = 0;
@@ -65,20 +54,20 @@ struct Mesh {
         if (ids) (void)hipFree(ids);
     }
 
-    DevMesh dev() const
+    psum_mesh_view view() const
     {
-        DevMesh d{};
-        d.I = I;
-        d.J = J;
-        d.K = K;
-        d.fast = fast;
-        for (int c = 0; c < 6; ++c) d.box[c] = box[c];
-        for (int c = 0; c < 3; ++c) d.rd[c] = rd[c];
-        d.cell_start = cell_start;
-        d.cell_tri = cell_tri;
-        d.tri12 = tri12;
-        d.ids = ids;
-        return d;
+        psum_mesh_view v{};
+        v.I = I;
+        v.J = J;
+        v.K = K;
+        v.fast = fast;
+        for (int c = 0; c < 6; ++c) v.box[c] = box[c];
+        for (int c = 0; c < 3; ++c) v.rd[c] = rd[c];
+        v.cell_start = cell_start;
+        v.cell_tri = cell_tri;
+        v.tri12 = tri12;
+        v.ids = ids;
+        return v;
     }
 };
 
@@ -155,156 +144,9 @@ void build(Mesh& m, const double* tri, const int32_t* ids, int64_t ntri)
     for (auto& l : lists) m.h_tri.insert(m.h_tri.end(), l.begin(), l.end());
 }
 
-// ---------------------------------------------------------------- device
-
-__device__ bool detect_cell(const DevMesh& m, int cell, const double* p1, const double* p2, double& k,
-                            double* nrm, int32_t& id)
-{
-    if (cell < 0 || cell >= m.I * m.J * m.K) return false;
-    const uint32_t b = m.cell_start[cell], e = m.cell_start[cell + 1];
-    double best = 1.0;
-    int64_t bi = -1;
-    for (uint32_t j = b; j < e; ++j) {
-        const uint32_t ti = m.cell_tri[j];
-        double t9[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) t9[c] = m.tri12[12 * (int64_t)ti + c];
-        double lam;
-        if (mesh::seg_tri(t9, p1, p2, lam))
-            if (bi < 0 || lam < best) {
-                best = lam;
-                bi = ti;
-            }
-    }
-    if (bi < 0) return false;
-    k = best;
-    for (int c = 0; c < 3; ++c) nrm[c] = m.tri12[12 * bi + 9 + c];
-    id = m.ids[bi];
-    return true;
-}
-
-// linear cell of grid coordinates from the floors (trace_to_idxs_fast);
-// -1 when a floor is not a representable int
-__device__ int64_t fast_cell(const DevMesh& m, const double* g)
-{
-    double f[3];
-    for (int c = 0; c < 3; ++c) {
-        f[c] = floor(g[c]);
-        if (!(f[c] >= -2147483648.0 && f[c] <= 2147483647.0)) return -1;
-    }
-    return ((int64_t)f[0] * m.J + (int64_t)f[1]) * m.K + (int64_t)f[2];
-}
-
-// triangle_mesh_trigger_acc::detect (:54-101)
-__device__ bool detect(const DevMesh& m, const double* p1, const double* p2, double& k, double* nrm, int32_t& id)
-{
-    double g1[3], g2[3];
-    for (int c = 0; c < 3; ++c) {
-        g1[c] = (p1[c] - m.box[c]) * m.rd[c];
-        g2[c] = (p2[c] - m.box[c]) * m.rd[c];
-    }
-    if (m.fast) {
-        const bool in1 = g1[0] >= 0 && g1[0] < m.I && g1[1] >= 0 && g1[1] < m.J && g1[2] >= 0 && g1[2] < m.K;
-        const bool in2 = g2[0] >= 0 && g2[0] < m.I && g2[1] >= 0 && g2[1] < m.J && g2[2] >= 0 && g2[2] < m.K;
-        const int64_t c1 = fast_cell(m, g1), c2 = fast_cell(m, g2);
-        if (in1 && detect_cell(m, (int)c1, p1, p2, k, nrm, id)) return true;
-        if (in2 && c1 != c2) return detect_cell(m, (int)c2, p1, p2, k, nrm, id);
-        return false;
-    }
-    mesh::Trace s;
-    const int first = s.begin(m.I, m.J, m.K, g1, g2);
-    if (first < 0) return false;
-    for (int idx = first; !s.finished; idx = s.next())
-        if (detect_cell(m, idx, p1, p2, k, nrm, id)) return true;
-    return false;
-}
-
-struct Lcg {
-    uint32_t x;
-    __device__ __forceinline__ double operator()()
-    {
-        x = x * 1664525u + 1013904223u;
-        return (double)x / 4294967295.0;
-    }
-};
-
-__device__ __forceinline__ double norm3(const double* v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
-__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-
-// unit vector across `axis` (the biAxis of RandV_Cosine / RandV_halfMaxw,
-// rand_functions.hpp:31-53): axis x (isotropic unit vector), normalised
-__device__ void bi_axis(Lcg& R, const double* axis, double* bi)
-{
-    const double r1 = 2 * R() - 1;
-    const double r2 = 2 * 3.1415926535897932 * R();
-    const double s = sqrt(1 - r1 * r1);
-    const double r[3] = {r1, s * sin(r2), s * cos(r2)};
-    bi[0] = axis[1] * r[2] - axis[2] * r[1];
-    bi[1] = axis[2] * r[0] - axis[0] * r[2];
-    bi[2] = axis[0] * r[1] - axis[1] * r[0];
-    const double inv = 1 / norm3(bi);
-    for (int c = 0; c < 3; ++c) bi[c] = bi[c] * inv;
-}
-
-// basic_material_set::acc_type::action (basic_material_set.hpp:31-105)
-__device__ void material_action(int mat, double* pos, double* vel, uint32_t& seed, const double* hit,
-                                const double* n, double temperature, double mass)
-{
-    double over[3];
-    for (int c = 0; c < 3; ++c) over[c] = pos[c] - hit[c];
-    switch (mat) {
-    case PSUM_MAT_DIFFUSE: {
-        Lcg R{seed};
-        double bi[3], dir[3];
-        bi_axis(R, n, bi);
-        const double r1 = sqrt(R());
-        const double r2 = sqrt(1 - r1 * r1);
-        for (int c = 0; c < 3; ++c) dir[c] = n[c] * r1 + bi[c] * r2;
-        const double on = norm3(over), vn = norm3(vel);
-        for (int c = 0; c < 3; ++c) {
-            pos[c] = hit[c] + dir[c] * on * 0.001;
-            vel[c] = dir[c] * vn;
-        }
-        seed = R.x;
-        break;
-    }
-    case PSUM_MAT_SPECULAR: {
-        const double od = 2.0 * dot3(over, n);
-        const double vd = 2.0 * dot3(vel, n);
-        for (int c = 0; c < 3; ++c) {
-            pos[c] = hit[c] + (over[c] - od * n[c]) * 0.001;
-            vel[c] = vel[c] - vd * n[c];
-        }
-        break;
-    }
-    case PSUM_MAT_MAXWELL_REFLECT: {
-        Lcg R{seed};
-        const double ot = norm3(over) / norm3(vel);
-        double u[3], bi[3];
-        const double nn = norm3(n);
-        for (int c = 0; c < 3; ++c) u[c] = n[c] / nn;
-        bi_axis(R, u, bi);
-        const double a = sqrt(-log(R()));
-        const double b = sqrt(-log(R()));
-        const double s = sqrt(2.0 * 1.3805e-23 * temperature / mass);
-        for (int c = 0; c < 3; ++c) {
-            const double nv = (bi[c] * a + u[c] * b) * s;
-            pos[c] = hit[c] + nv * ot;
-            vel[c] = nv;
-        }
-        seed = R.x;
-        break;
-    }
-    case PSUM_MAT_ABSORB:
-        pos[0] = __builtin_nan("");
-        break;
-    default:
-        break;
-    }
-}
 
 struct SegArgs {
-    DevMesh m;
+    mesh::View m;
     const void* p1[3];
     const void* p2[3];
     int64_t n;
@@ -327,7 +169,7 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_detect(SegArgs a)
     }
     double k = 0, nrm[3] = {0, 0, 0};
     int32_t id = -1;
-    const bool hit = finite && detect(a.m, p1, p2, k, nrm, id);
+    const bool hit = finite && mesh::detect(a.m, p1, p2, k, nrm, id);
     if (a.k) a.k[i] = hit ? k : 0.0;
     if (a.nrm)
         for (int c = 0; c < 3; ++c) a.nrm[3 * i + c] = hit ? nrm[c] : 0.0;
@@ -335,7 +177,7 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_detect(SegArgs a)
 }
 
 struct RouteArgs {
-    DevMesh m;
+    mesh::View m;
     const void* old[3];
     void* pos[3];
     void* vel[3];
@@ -365,7 +207,7 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_boundary(RouteArgs a)
                 finite = finite && isfinite(p1[c]);
             }
             double k, nrm[3];
-            if (finite && detect(a.m, p1, b, k, nrm, id)) {
+            if (finite && mesh::detect(a.m, p1, b, k, nrm, id)) {
                 hit = 1;
                 double v[3], h[3];
                 for (int c = 0; c < 3; ++c) {
@@ -373,7 +215,7 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_boundary(RouteArgs a)
                     h[c] = p1[c] + k * (b[c] - p1[c]);
                 }
                 uint32_t s = a.seed[i];
-                material_action(id, b, v, s, h, nrm, a.temperature, a.mass);
+                mesh::material_action(id, b, v, s, h, nrm, a.temperature, a.mass);
                 for (int c = 0; c < 3; ++c) {
                     static_cast<T*>(a.pos[c])[i] = (T)b[c];
                     static_cast<T*>(a.vel[c])[i] = (T)v[c];
@@ -476,7 +318,7 @@ psum_status psum_hip_mesh_detect(void* h, int dtype, const void* const* p1, cons
     if (n == 0) return PSUM_OK;
     PSUM_ARG_CHECK(p1 && p2 && id, "psum_hip_mesh_detect: null pointer");
     SegArgs a{};
-    a.m = static_cast<const Mesh*>(h)->dev();
+    a.m = mesh::view_of(static_cast<const Mesh*>(h)->view());
     for (int c = 0; c < 3; ++c) {
         PSUM_ARG_CHECK(p1[c] && p2[c], "psum_hip_mesh_detect: null coordinate array");
         a.p1[c] = p1[c];
@@ -507,7 +349,7 @@ psum_status psum_hip_mesh_boundary(void* h, int dtype, const void* const* old, v
     if (n == 0) return PSUM_OK;
     PSUM_ARG_CHECK(old && pos && vel && seed, "psum_hip_mesh_boundary: null pointer");
     RouteArgs a{};
-    a.m = m->dev();
+    a.m = mesh::view_of(m->view());
     for (int c = 0; c < 3; ++c) {
         PSUM_ARG_CHECK(old[c] && pos[c] && vel[c], "psum_hip_mesh_boundary: null coordinate array");
         a.old[c] = old[c];
@@ -526,6 +368,13 @@ psum_status psum_hip_mesh_boundary(void* h, int dtype, const void* const* old, v
     else
         hipLaunchKernelGGL(k_mesh_boundary<float>, dim3(nb), dim3(kMeshThreads), 0, S(stream), a);
     PSUM_HIP_TRY(hipGetLastError());
+    return PSUM_OK;
+}
+
+psum_status psum_mesh_get_view(void* h, psum_mesh_view* out)
+{
+    PSUM_ARG_CHECK(h && out, "psum_mesh_get_view: null pointer");
+    *out = static_cast<const Mesh*>(h)->view();
     return PSUM_OK;
 }
 

@@ -74,8 +74,9 @@ def test_grid_struct_layout_matches_oracle():
 def test_cpp_api_program_is_built():
     """build() compiles the C++ API test program against the headers in
     include/psum/ and the library (it runs only on a GPU)."""
-    exe = os.path.join(ROOT, "tests", "cpp", "api_step")
-    assert os.path.exists(exe) and os.access(exe, os.X_OK)
+    for name in ("api_step", "mesh_router"):
+        exe = os.path.join(ROOT, "tests", "cpp", name)
+        assert os.path.exists(exe) and os.access(exe, os.X_OK), name
 
 
 @pytest.mark.parametrize("case", ["periodic2d", "walls2d", "walls3d"])
