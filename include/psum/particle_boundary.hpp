@@ -322,6 +322,12 @@ public:
     // cell the step crosses (restrict); the C ABI handle follows
     void set_fast() { mode_(false); }
     void set_restrict() { mode_(true); }
+    // the distance-field skip of the full traversal (psum_mesh_set_udf)
+    void set_acceleration_UDF(bool enabled)
+    {
+        if (!h_) throw std::runtime_error("triangle_mesh_trigger::set_acceleration_UDF called before set");
+        if (psum_mesh_set_udf(h_, enabled ? 1 : 0) != PSUM_OK) throw std::runtime_error(psum_hip_last_error());
+    }
 
     acc_type get_access(handler&) const
     {

@@ -291,8 +291,16 @@ typedef struct {
     const uint32_t* cell_tri;
     const double* tri12;          /* 12 per triangle: vertices, unit normal */
     const int32_t* ids;
+    const float* udf;             /* I*J*K distance field, NULL = no skip */
 } psum_mesh_view;
 psum_status psum_mesh_get_view(void* m, psum_mesh_view* out);
+/* triangle_mesh_trigger::set_acceleration_UDF (triangle_mesh_trigger.hpp:
+ * 383-394, geometry.hpp:343-460): enabled, a step that starts in a cell
+ * whose distance field (cell units, from the occupied cells; computed on
+ * the host at the first enable) exceeds one cell and is shorter than it is
+ * not traced, and a longer one starts its traversal that far along.  Only
+ * the full traversal (not fast mode) uses it. */
+psum_status psum_mesh_set_udf(void* m, int enabled);
 
 /* out[k] = sum over blocks of diag[4*b + k], k < 4, in a fixed order
  * (deterministic).  work: psum_hip_reduce_workspace() bytes of device

@@ -185,6 +185,7 @@ struct View {
     const uint32_t* cell_tri;
     const double* tri12;
     const int32_t* ids;
+    const float* udf;        // distance field of the full traversal, or null
 };
 
 inline View view_of(const psum_mesh_view& v)
@@ -200,6 +201,7 @@ inline View view_of(const psum_mesh_view& v)
     m.cell_tri = v.cell_tri;
     m.tri12 = v.tri12;
     m.ids = v.ids;
+    m.udf = v.udf;
     return m;
 }
 
@@ -260,6 +262,29 @@ __device__ inline bool detect(const View& m, const double* p1, const double* p2,
         if (in1 && detect_cell(m, (int)c1, p1, p2, k, nrm, id)) return true;
         if (in2 && c1 != c2) return detect_cell(m, (int)c2, p1, p2, k, nrm, id);
         return false;
+    }
+    if (m.udf) {
+        // distance-field skip (:66-92): from a start cell whose field
+        // exceeds one cell, a step shorter than the field cannot reach a
+        // triangle, and a longer one starts its traversal that far along
+        double f[3];
+        const int dims[3] = {m.I, m.J, m.K};
+        bool in = true;
+        for (int c = 0; c < 3; ++c) {
+            f[c] = floor(g1[c]);
+            in = in && f[c] >= 0 && f[c] < dims[c];
+        }
+        if (in) {
+            const float u = m.udf[((int64_t)f[0] * m.J + (int64_t)f[1]) * m.K + (int64_t)f[2]];
+            if (u > 1.0f) {
+                const double d[3] = {g2[0] - g1[0], g2[1] - g1[1], g2[2] - g1[2]};
+                const double sq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const float u2 = u * u;
+                if (sq < (double)u2) return false;
+                const double len = sqrt(sq);
+                for (int c = 0; c < 3; ++c) g1[c] = g1[c] + (double)u * d[c] / len;
+            }
+        }
     }
     Trace s;
     const int first = s.begin(m.I, m.J, m.K, g1, g2);

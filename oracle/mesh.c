@@ -16,7 +16,9 @@
  *                             automatic box :358-392
  *   detect                    triangle_mesh_trigger.hpp:54-155 (first
  *                             traversed cell with a hit; smallest lambda
- *                             in that cell, first on ties)
+ *                             in that cell, first on ties); with a
+ *                             distance field, the skip of :66-92
+ *   distance field            geometry.hpp:343-460 (compute_udf)
  *   router + materials        boundary_router.hpp:32-58,
  *                             basic_material_set.hpp:17-107 with
  *                             RandV_Cosine / RandV_halfMaxw
@@ -277,7 +279,7 @@ static int64_t fast_cell(const orc_mesh* m, const double* g)
 int orc_mesh_detect(const orc_mesh* m, const double* p1, const double* p2, int fast, double* k, double* nrm,
                     int32_t* id)
 {
-    const double g1[3] = {(p1[0] - m->box[0]) * m->rd[0], (p1[1] - m->box[1]) * m->rd[1], (p1[2] - m->box[2]) * m->rd[2]};
+    double g1[3] = {(p1[0] - m->box[0]) * m->rd[0], (p1[1] - m->box[1]) * m->rd[1], (p1[2] - m->box[2]) * m->rd[2]};
     const double g2[3] = {(p2[0] - m->box[0]) * m->rd[0], (p2[1] - m->box[1]) * m->rd[1], (p2[2] - m->box[2]) * m->rd[2]};
     if (fast) {
         /* trace_to_idxs_fast: both cell indices come from the floors whether
@@ -289,12 +291,99 @@ int orc_mesh_detect(const orc_mesh* m, const double* p1, const double* p2, int f
         if (in2 && c1 != c2) return detect_cell(m, c2, p1, p2, k, nrm, id);
         return 0;
     }
+    if (m->udf) {
+        /* distance-field skip (:66-92): from a start cell whose field
+         * exceeds one cell, a step shorter than the field cannot reach a
+         * triangle, and a longer one starts its traversal that far along */
+        double f[3];
+        int in = 1;
+        const int dims[3] = {m->I, m->J, m->K};
+        for (int c = 0; c < 3; ++c) {
+            f[c] = floor(g1[c]);
+            in = in && f[c] >= 0 && f[c] < dims[c];
+        }
+        if (in) {
+            const float u = m->udf[((int64_t)f[0] * m->J + (int64_t)f[1]) * m->K + (int64_t)f[2]];
+            if (u > 1.0f) {
+                const double d[3] = {g2[0] - g1[0], g2[1] - g1[1], g2[2] - g1[2]};
+                const double sq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const float u2 = u * u;
+                if (sq < (double)u2) return 0;
+                const double len = sqrt(sq);
+                for (int c = 0; c < 3; ++c) g1[c] = g1[c] + (double)u * d[c] / len;
+            }
+        }
+    }
     orc_trace s;
     const int first = orc_trace_begin(m->I, m->J, m->K, g1, g2, &s);
     if (first < 0) return 0;
     for (int idx = first; !s.finished; idx = orc_trace_next(&s))
         if (detect_cell(m, idx, p1, p2, k, nrm, id)) return 1;
     return 0;
+}
+
+void orc_compute_udf(int I, int J, int K, const uint8_t* occupied, float* udf, int udf_k, int max_loop)
+{
+    const int64_t n = (int64_t)I * J * K;
+    int any = 0;
+    for (int64_t c = 0; c < n && !any; ++c) any = occupied[c] != 0;
+    if (!any) {
+        const int lo = I < J ? (I < K ? I : K) : (J < K ? J : K);
+        for (int64_t c = 0; c < n; ++c) udf[c] = (float)(lo - 1);
+        return;
+    }
+    const float cap = (float)(udf_k - 1);
+    for (int i = 0; i < I; ++i)
+        for (int j = 0; j < J; ++j)
+            for (int k = 0; k < K; ++k) {
+                const int64_t c = ((int64_t)i * J + j) * K + k;
+                if (occupied[c]) { udf[c] = 0.0f; continue; }
+                float best = 1e30f;
+                for (int di = -udf_k; di <= udf_k; ++di)
+                    for (int dj = -udf_k; dj <= udf_k; ++dj)
+                        for (int dk = -udf_k; dk <= udf_k; ++dk) {
+                            const int a = i + di, b = j + dj, e = k + dk;
+                            if (a < 0 || a >= I || b < 0 || b >= J || e < 0 || e >= K) continue;
+                            if (!occupied[((int64_t)a * J + b) * K + e]) continue;
+                            /* cells apart by d share no point closer than d - 1 per axis */
+                            const int li = abs(di) - 1 > 0 ? abs(di) - 1 : 0;
+                            const int lj = abs(dj) - 1 > 0 ? abs(dj) - 1 : 0;
+                            const int lk = abs(dk) - 1 > 0 ? abs(dk) - 1 : 0;
+                            const float d = sqrtf((float)(li * li + lj * lj + lk * lk));
+                            if (d < best) best = d;
+                        }
+                float v = best < 1e29f ? best : cap;
+                udf[c] = v < cap ? v : cap;
+            }
+    float* nu = (float*)malloc((size_t)n * sizeof(float));
+    for (int loop = 0;; ++loop) {
+        int updated = 0;
+        for (int i = 0; i < I; ++i)
+            for (int j = 0; j < J; ++j)
+                for (int k = 0; k < K; ++k) {
+                    const int64_t c = ((int64_t)i * J + j) * K + k;
+                    const float cur = udf[c];
+                    nu[c] = cur;
+                    if (!(cur > 0.0f)) continue;
+                    int raise = 1, nb = 0;
+                    for (int di = -1; di <= 1; ++di)
+                        for (int dj = -1; dj <= 1; ++dj)
+                            for (int dk = -1; dk <= 1; ++dk) {
+                                if (!di && !dj && !dk) continue;
+                                const int a = i + di, b = j + dj, e = k + dk;
+                                if (a < 0 || a >= I || b < 0 || b >= J || e < 0 || e >= K) continue;
+                                ++nb;
+                                if (udf[((int64_t)a * J + b) * K + e] < cur) raise = 0;
+                            }
+                    if (raise && nb > 0) {
+                        nu[c] = cur + 1.0f;
+                        updated = 1;
+                    }
+                }
+        memcpy(udf, nu, (size_t)n * sizeof(float));
+        if (!updated || loop + 1 > max_loop) break;
+    }
+    free(nu);
 }
 
 /* planted mistakes: 1 = specular reflection keeps the normal velocity

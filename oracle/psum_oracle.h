@@ -382,6 +382,7 @@ typedef struct {
     const int32_t* ids;
     const uint32_t* cell_start;
     const uint32_t* cell_tri;
+    const float* udf;         /* per-cell skip distance, NULL = no skip */
 } orc_mesh;
 int orc_seg_tri(const double* tri, const double* p1, const double* p2, double* lambda);
 int orc_trace_begin(int I, int J, int K, const double* a, const double* b, orc_trace* s);
@@ -394,6 +395,11 @@ int64_t orc_mesh_build(const double* tri, const int32_t* ids, int64_t ntri, int 
                        double* tri12, int32_t* kept, uint32_t* cell_start, uint32_t* cell_tri, int64_t cap);
 int orc_mesh_detect(const orc_mesh* m, const double* p1, const double* p2, int fast, double* k, double* nrm,
                     int32_t* id);
+/* Unsigned distance field of the occupied cells (occupied[c] != 0), in cell
+ * units: a neighbourhood lower bound within +-udf_k cells, then raised by
+ * one per pass where no neighbour is lower, for at most max_loop + 1 passes
+ * (geometry.hpp:343-460; the reference's defaults are 6 and 200). */
+void orc_compute_udf(int I, int J, int K, const uint8_t* occupied, float* udf, int udf_k, int max_loop);
 void orc_material_action(int mat, double* pos, double* vel, uint32_t* seed, const double* hit, const double* n,
                          double temperature, double mass);
 int64_t orc_mesh_boundary_f64(const orc_mesh* m, int fast, const double* const* old, double** pos, double** vel,

@@ -152,6 +152,7 @@ class PsumMeshView(C.Structure):
         ("cell_tri", C.c_void_p),
         ("tri12", C.c_void_p),
         ("ids", C.c_void_p),
+        ("udf", C.c_void_p),
     ]
 
 
@@ -186,6 +187,7 @@ SIGNATURES = [
     ("psum_hip_mesh_detect", ST, [P, C.c_int, P, P, I64, P, P, P, P]),
     ("psum_hip_mesh_boundary", ST, [P, C.c_int, P, P, P, P, I64, C.c_double, C.c_double, P, P, P]),
     ("psum_mesh_get_view", ST, [P, C.POINTER(PsumMeshView)]),
+    ("psum_mesh_set_udf", ST, [P, C.c_int]),
     ("psum_mesh_destroy", None, [P]),
     ("psum_hip_reduce_workspace", C.c_size_t, []),
     ("psum_hip_reduce_diag", ST, [P, I64, P, P, P]),

@@ -45,9 +45,13 @@ struct Mesh {
     uint32_t* cell_tri = nullptr;
     double* tri12 = nullptr;
     int32_t* ids = nullptr;
+    std::vector<float> h_udf;
+    float* udf = nullptr;
+    int udf_on = 0;
 
     ~Mesh()
     {
+        if (udf) (void)hipFree(udf);
         if (cell_start) (void)hipFree(cell_start);
         if (cell_tri) (void)hipFree(cell_tri);
         if (tri12) (void)hipFree(tri12);
@@ -67,6 +71,7 @@ struct Mesh {
         v.cell_tri = cell_tri;
         v.tri12 = tri12;
         v.ids = ids;
+        v.udf = udf_on ? udf : nullptr;
         return v;
     }
 };
@@ -144,6 +149,70 @@ void build(Mesh& m, const double* tri, const int32_t* ids, int64_t ntri)
     for (auto& l : lists) m.h_tri.insert(m.h_tri.end(), l.begin(), l.end());
 }
 
+
+// compute_udf (geometry.hpp:343-460): per empty cell the smallest
+// cell-distance lower bound to an occupied cell within +-udf_k cells (cells d
+// apart share no point closer than max(|d| - 1, 0) per axis), capped at
+// udf_k - 1; then passes that raise a cell by one where no neighbour is
+// lower, until nothing changes or max_loop + 1 passes
+void compute_udf(int I, int J, int K, const std::vector<uint32_t>& start, std::vector<float>& udf, int udf_k = 6,
+                 int max_loop = 200)
+{
+    const int64_t n = (int64_t)I * J * K;
+    udf.assign((size_t)n, 0.0f);
+    std::vector<uint8_t> occ((size_t)n);
+    bool any = false;
+    for (int64_t c = 0; c < n; ++c) any |= (occ[c] = start[c + 1] > start[c]) != 0;
+    if (!any) {
+        std::fill(udf.begin(), udf.end(), (float)(std::min(I, std::min(J, K)) - 1));
+        return;
+    }
+    const float cap = (float)(udf_k - 1);
+    auto at = [&](int i, int j, int k) { return ((int64_t)i * J + j) * K + k; };
+    for (int i = 0; i < I; ++i)
+        for (int j = 0; j < J; ++j)
+            for (int k = 0; k < K; ++k) {
+                const int64_t c = at(i, j, k);
+                if (occ[c]) continue;
+                float best = 1e30f;
+                for (int a = std::max(0, i - udf_k); a <= std::min(I - 1, i + udf_k); ++a)
+                    for (int b = std::max(0, j - udf_k); b <= std::min(J - 1, j + udf_k); ++b)
+                        for (int e = std::max(0, k - udf_k); e <= std::min(K - 1, k + udf_k); ++e) {
+                            if (!occ[at(a, b, e)]) continue;
+                            const int li = std::max(std::abs(a - i) - 1, 0), lj = std::max(std::abs(b - j) - 1, 0),
+                                      lk = std::max(std::abs(e - k) - 1, 0);
+                            best = std::min(best, std::sqrt((float)(li * li + lj * lj + lk * lk)));
+                        }
+                udf[c] = std::min(best < 1e29f ? best : cap, cap);
+            }
+    std::vector<float> nu((size_t)n);
+    for (int loop = 0;; ++loop) {
+        bool updated = false;
+        for (int i = 0; i < I; ++i)
+            for (int j = 0; j < J; ++j)
+                for (int k = 0; k < K; ++k) {
+                    const int64_t c = at(i, j, k);
+                    const float cur = udf[c];
+                    nu[c] = cur;
+                    if (!(cur > 0.0f)) continue;
+                    bool raise = true;
+                    int nb = 0;
+                    for (int a = std::max(0, i - 1); a <= std::min(I - 1, i + 1); ++a)
+                        for (int b = std::max(0, j - 1); b <= std::min(J - 1, j + 1); ++b)
+                            for (int e = std::max(0, k - 1); e <= std::min(K - 1, k + 1); ++e) {
+                                if (a == i && b == j && e == k) continue;
+                                ++nb;
+                                if (udf[at(a, b, e)] < cur) raise = false;
+                            }
+                    if (raise && nb > 0) {
+                        nu[c] = cur + 1.0f;
+                        updated = true;
+                    }
+                }
+        udf.swap(nu);
+        if (!updated || loop + 1 > max_loop) break;
+    }
+}
 
 struct SegArgs {
     mesh::View m;
@@ -375,6 +444,24 @@ psum_status psum_mesh_get_view(void* h, psum_mesh_view* out)
 {
     PSUM_ARG_CHECK(h && out, "psum_mesh_get_view: null pointer");
     *out = static_cast<const Mesh*>(h)->view();
+    return PSUM_OK;
+}
+
+psum_status psum_mesh_set_udf(void* h, int enabled)
+{
+    PSUM_ARG_CHECK(h, "psum_mesh_set_udf: null mesh");
+    Mesh* m = static_cast<Mesh*>(h);
+    if (enabled && !m->udf) {
+        try {
+            compute_udf(m->I, m->J, m->K, m->h_start, m->h_udf);
+        } catch (const std::exception& e) {
+            set_error(std::string("psum_mesh_set_udf: ") + e.what());
+            return PSUM_ERR_NOMEM;
+        }
+        PSUM_HIP_TRY(hipMalloc(&m->udf, m->h_udf.size() * sizeof(float)));
+        PSUM_HIP_TRY(hipMemcpy(m->udf, m->h_udf.data(), m->h_udf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    m->udf_on = enabled ? 1 : 0;
     return PSUM_OK;
 }
 

@@ -173,6 +173,11 @@ class TriangleMesh:
     def set_fast(self, fast=True):
         check(lib().psum_mesh_set_fast(self.h, int(bool(fast))), "psum_mesh_set_fast")
 
+    def set_udf(self, enabled=True):
+        """set_acceleration_UDF: the full traversal skips empty space by the
+        cells' distance field (computed on the host at the first enable)"""
+        check(lib().psum_mesh_set_udf(self.h, int(bool(enabled))), "psum_mesh_set_udf")
+
     def partition(self):
         """(cell_start, cell_tri, tri12) host copies"""
         cs = np.zeros(self.cells + 1, np.uint32)

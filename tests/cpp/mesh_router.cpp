@@ -2,7 +2,7 @@
 // mesh_generator, triangle_mesh_trigger, basic_material_set,
 // boundary_router), compared on the GPU with the C ABI kernels on the same
 // particles:
-//   detect    trigger accessor in a kernel (fast and restrict)  vs  psum_hip_mesh_detect
+//   detect    trigger accessor in a kernel (fast, restrict, UDF)  vs  psum_hip_mesh_detect
 //   router    drift + boundary_router::deal in for_each          vs  psum_hip_mesh_boundary
 // Both sides run the same device code, so every comparison is bit-exact.
 // Prints one line per check and "ALL OK" at the end; exit status 0 on success.
@@ -127,9 +127,10 @@ int main(int argc, char** argv)
         psum::hip_check(hipMalloc(&nb, 3 * n * 8), "m");
         psum::hip_check(hipMalloc(&ia, n * 4), "m");
         psum::hip_check(hipMalloc(&ib, n * 4), "m");
-        for (int mode = 0; mode < 2; ++mode) {
+        for (int mode = 0; mode < 3; ++mode) {
             if (mode == 0) trig.set_fast();
             else trig.set_restrict();
+            if (mode == 2) trig.set_acceleration_UDF(true);
             psum::handler h{st};
             auto acc = trig.get_access(h);
             hipLaunchKernelGGL(k_detect<decltype(acc)>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, acc, d1,
@@ -149,8 +150,9 @@ int main(int argc, char** argv)
             for (size_t i = 0; i < n; ++i) hits += hia[i] >= 0;
             const bool same = hia == hib && std::memcmp(hka.data(), hkb.data(), n * 8) == 0 &&
                               std::memcmp(hna.data(), hnb.data(), 3 * n * 8) == 0;
-            report(mode == 0 ? "detect (fast): accessor == C ABI" : "detect (restrict): accessor == C ABI",
-                   same && hits > 2000, (double)hits);
+            const char* what[3] = {"detect (fast): accessor == C ABI", "detect (restrict): accessor == C ABI",
+                                   "detect (restrict, UDF): accessor == C ABI"};
+            report(what[mode], same && hits > 2000, (double)hits);
         }
         for (void* p : {(void*)d1, (void*)d2, (void*)ka, (void*)kb, (void*)na, (void*)nb, (void*)ia, (void*)ib})
             (void)hipFree(p);

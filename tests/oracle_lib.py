@@ -154,7 +154,8 @@ class Trace(C.Structure):
 class Mesh(C.Structure):
     """orc_mesh"""
     _fields_ = [("I", C.c_int), ("J", C.c_int), ("K", C.c_int), ("box", C.c_double * 6), ("rd", C.c_double * 3),
-                ("tri12", C.c_void_p), ("ids", C.c_void_p), ("cell_start", C.c_void_p), ("cell_tri", C.c_void_p)]
+                ("tri12", C.c_void_p), ("ids", C.c_void_p), ("cell_start", C.c_void_p), ("cell_tri", C.c_void_p),
+                ("udf", C.c_void_p)]
 
 
 def build():
@@ -241,6 +242,7 @@ def lib():
             getattr(L, nm).argtypes = [C.POINTER(Mesh), C.c_int, P, P, P, P, i64, C.c_double, C.c_double, P]
             getattr(L, nm).restype = i64
         L.orc_mesh_plant.argtypes = [C.c_int]
+        L.orc_compute_udf.argtypes = [C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.c_int]
         L.orc_pic_set_order.argtypes = [C.c_int]
         L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int]
@@ -782,6 +784,15 @@ def box_tri_overlap(tri, box):
     return bool(lib().orc_box_tri_overlap(_ptr(t), _ptr(bx)))
 
 
+def compute_udf(occupied, udf_k=6, max_loop=200):
+    """orc_compute_udf of an (I, J, K) occupancy array"""
+    occ = np.ascontiguousarray(occupied, np.uint8)
+    I, J, K = occ.shape
+    out = np.zeros(occ.size, np.float32)
+    lib().orc_compute_udf(I, J, K, _ptr(occ), _ptr(out), udf_k, max_loop)
+    return out
+
+
 class OracleMesh:
     """triangle_mesh_trigger built by the oracle; keeps its arrays alive"""
 
@@ -813,7 +824,20 @@ class OracleMesh:
         m.ids = self.ids.ctypes.data
         m.cell_start = self.cell_start.ctypes.data
         m.cell_tri = self.cell_tri.ctypes.data if total else None
+        m.udf = None
         self.c = m
+        self.shape = (I, J, K)
+
+    def set_udf(self, enabled=True):
+        """triangle_mesh_trigger::set_acceleration_UDF: the distance field of
+        the occupied cells (reference defaults) skips empty space in the
+        full traversal"""
+        if enabled:
+            occ = (np.diff(self.cell_start.astype(np.int64)) > 0).astype(np.uint8)
+            self.udf = compute_udf(occ.reshape(self.shape))
+            self.c.udf = self.udf.ctypes.data
+        else:
+            self.c.udf = None
 
     def detect(self, p1, p2, fast=False):
         """(hit, k, normal, id)"""

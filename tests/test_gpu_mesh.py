@@ -2,7 +2,8 @@
 against the oracle.
 
 Bar: the partition (cell lists, normals) is identical; detect is
-bit-identical (k, normal, id) in both traversal modes -- the geometry is
+bit-identical (k, normal, id) in both traversal modes, and with the
+distance-field skip of the full traversal -- the geometry is
 FP64 adds, multiplies and divides, uncontracted on both sides; the router
 is bit-identical for specular and absorbing walls and for every seed, and
 within 1e-12 (relative to the speed / to the position scale) where the
@@ -56,23 +57,27 @@ def test_partition_matches_oracle(shape):
     tm.close()
 
 
-@pytest.mark.parametrize("fast", [False, True])
+@pytest.mark.parametrize("mode", ["restrict", "fast", "udf"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_detect_bit_exact(fast, dtype):
+def test_detect_bit_exact(mode, dtype):
     from psum_amd import mesh as Mh
     meshes = _scene()
     tri, ids = _flatten(meshes)
     shape = (16, 16, 20)
+    fast = mode == "fast"
     tm = Mh.TriangleMesh(meshes, *shape)
     tm.set_fast(fast)
     om = O.OracleMesh(tri, ids, *shape)
+    if mode == "udf":
+        tm.set_udf(True)
+        om.set_udf(True)
     rng = np.random.default_rng(11)
     n = 20000
     p1 = rng.uniform([-4.9, -4.9, -4.9], [4.9, 4.9, 9.9], (n, 3)).astype(dtype)
     p2 = (p1 + rng.normal(0, 1.5, (n, 3))).astype(dtype)
     p2[::997] = p1[::997]                                  # zero-length segments
     p1[5::1013, 1] = np.nan                                # non-finite ends are not traced
-    print(f"mesh detect fast={fast} dtype={np.dtype(dtype).name} n={n}", flush=True)
+    print(f"mesh detect mode={mode} dtype={np.dtype(dtype).name} n={n}", flush=True)
     k, nrm, idt = tm.detect(_cuda(p1.T), _cuda(p2.T))
     k, nrm, idt = k.cpu().numpy(), nrm.cpu().numpy(), idt.cpu().numpy()
     hits = 0
@@ -116,8 +121,9 @@ def test_detect_stl_mesh():
     tm.close()
 
 
+@pytest.mark.parametrize("udf", [False, True])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_router_matches_oracle(dtype):
+def test_router_matches_oracle(dtype, udf):
     """flight through the scene: every step moves x += v dt and lets the
     router act on GPU and oracle copies; compared after each step"""
     from psum_amd import mesh as Mh
@@ -126,6 +132,10 @@ def test_router_matches_oracle(dtype):
     shape = (20, 20, 30)
     router = Mh.BoundaryRouter(temperature=300.0, mass=6.67e-27).set(meshes, *shape)
     om = O.OracleMesh(tri, ids, *shape)
+    if udf:
+        router.trigger.set_udf(True)
+        om.set_udf(True)
+    print(f"mesh router dtype={np.dtype(dtype).name} udf={udf}", flush=True)
     rng = np.random.default_rng(13)
     n = 6000
     pos = [rng.uniform(-0.6, 0.6, n), rng.uniform(-0.6, 0.6, n), rng.uniform(-3.9, 3.0, n)]

@@ -341,3 +341,97 @@ def test_router_moves_particles_consistently():
     for c, (lo, hi) in enumerate([(-5, 5), (-5, 5), (-5, 10)]):
         assert np.all(pos[c][alive] > lo) and np.all(pos[c][alive] < hi)
     np.testing.assert_allclose(np.sqrt(sum(v ** 2 for v in vel))[alive], speed0[alive], rtol=1e-12)
+
+
+# ---------------------------------------------------------------- distance field
+
+def _lower_bound(occ, i, j, k):
+    """exact lower bound on the distance (cell units) from cell (i, j, k) to
+    any occupied cell: per axis, cells d apart share no point closer than
+    max(|d| - 1, 0) (test_udf.cpp: compute_precise_lower_bound)"""
+    o = np.argwhere(occ)
+    if len(o) == 0:
+        return np.inf
+    d = np.maximum(np.abs(o - np.array([i, j, k])) - 1, 0).astype(np.float64)
+    return np.sqrt((d ** 2).sum(axis=1)).min()
+
+
+def test_udf_reference_cases():
+    """test_udf.cpp's basic cases: an empty grid gets min(I, J, K) - 1
+    everywhere, a full grid zeros, an occupied cell zero"""
+    assert np.all(O.compute_udf(np.zeros((10, 10, 10), np.uint8)) == 9.0)
+    assert np.all(O.compute_udf(np.zeros((7, 12, 9), np.uint8)) == 6.0)
+    assert np.all(O.compute_udf(np.ones((10, 10, 10), np.uint8)) == 0.0)
+    occ = np.zeros((10, 10, 10), np.uint8)
+    occ[5, 5, 5] = 1
+    u = O.compute_udf(occ).reshape(occ.shape)
+    assert u[5, 5, 5] == 0.0 and np.all(u[4:7, 4:7, 4:7] == 0.0) and u[0, 0, 0] > 0
+
+
+def test_udf_line_closed_form():
+    """one occupied end cell of a 1-cell-wide row: the field is the exact
+    distance max(i - 1, 0) once the passes have raised the capped cells
+    (the neighbourhood bound alone stops at udf_k - 1 = 5)"""
+    for shape, axis in (((14, 1, 1), 0), ((1, 14, 1), 1), ((1, 1, 14), 2)):
+        occ = np.zeros(shape, np.uint8)
+        occ[(0, 0, 0)] = 1
+        u = O.compute_udf(occ)
+        assert np.array_equal(u, np.maximum(np.arange(14) - 1, 0).astype(np.float32)), axis
+        # a single pass leaves the cap one higher only at the far cells
+        u1 = O.compute_udf(occ, max_loop=0)
+        assert np.array_equal(u1, np.minimum(np.maximum(np.arange(14) - 1, 0), [5] * 7 + [6] * 7).astype(np.float32))
+
+
+def test_udf_random_conservative():
+    """test_udf.cpp's random case: 100 cells of a 20^3 grid occupied by the
+    reference generator from seed 12345, then 1000 sampled cells; the field
+    never exceeds the exact lower bound there (and the neighbourhood bound
+    equals it wherever it is below the cap)"""
+    I = J = K = 20
+    r, _ = O.lcg_draws(12345, 3 * 100 + 3 * 1000)
+    occ = np.zeros((I, J, K), np.uint8)
+    for n in range(100):
+        occ[int(r[3 * n] * I), int(r[3 * n + 1] * J), int(r[3 * n + 2] * K)] = 1
+    u = O.compute_udf(occ).reshape(occ.shape)
+    u0 = O.compute_udf(occ, max_loop=0).reshape(occ.shape)
+    checked = 0
+    for n in range(1000):
+        i, j, k = (int(r[300 + 3 * n + c] * d) for c, d in enumerate((I, J, K)))
+        if occ[i, j, k]:
+            continue
+        lb = _lower_bound(occ, i, j, k)
+        assert u[i, j, k] - lb <= 1e-6, (i, j, k, u[i, j, k], lb)
+        assert u[i, j, k] >= min(lb, 5.0) - 1e-6
+        if lb < 5.0 and u0[i, j, k] == u[i, j, k]:
+            assert abs(u0[i, j, k] - np.float32(lb)) <= 1e-6
+        checked += 1
+    assert checked > 900
+
+
+def test_udf_skip_keeps_the_hits():
+    """detect with the field's skip against detect without it: on a scene
+    whose occupied cells are far apart the skipped cells are empty, so the
+    same hits are found (the skip can change a result only where the field
+    overestimates, next to occupied cells)"""
+    cube = np.array([[-5, -5, -5, 5, -5, -5, 5, 5, -5], [-5, -5, -5, 5, 5, -5, -5, 5, -5],
+                     [-5, -5, 5, 5, -5, 5, 5, 5, 5], [-5, -5, 5, 5, 5, 5, -5, 5, 5]], np.float64)
+    small = np.array([[-0.5, -0.5, 0.0, 0.5, -0.5, 0.0, 0.0, 0.5, 0.0]])
+    tri = np.concatenate([cube, small])
+    ids = np.array([0, 0, 0, 0, 1], np.int32)
+    plain = O.OracleMesh(tri, ids, 30, 30, 30)
+    skip = O.OracleMesh(tri, ids, 30, 30, 30)
+    skip.set_udf(True)
+    assert skip.udf.max() > 1.0
+    rng = _rng(31)
+    hits = skipped_same = 0
+    for _ in range(3000):
+        p1 = rng.uniform(-4.5, 4.5, 3)
+        p2 = p1 + rng.normal(0, 3.0, 3)
+        a = plain.detect(p1, p2)
+        b = skip.detect(p1, p2)
+        assert a[0] == b[0]
+        if a[0]:
+            hits += 1
+            assert a[3] == b[3] and np.array_equal(a[2], b[2])
+            skipped_same += a[1] == b[1]
+    assert hits > 300 and skipped_same == hits
