@@ -25,7 +25,7 @@ struct device_field_acc {
     using Value = Scalar;
     Scalar* content = nullptr;
     psum_hip::DevGrid grid{};
-    __device__ Scalar& operator()(size_t i) const { return content[i]; }
+    __host__ __device__ Scalar& operator()(size_t i) const { return content[i]; }
 };
 
 template <int D, var_loc Loc, typename Scalar, int QuantitySize = 1>
@@ -100,7 +100,7 @@ template <typename Scalar, int QS = 1> using cell_field3D = device_field<3, var_
 namespace detail {
 // CIC support of pos on a node field: indices and per-dimension factors
 template <int D, typename Pos>
-__device__ bool support(const psum_hip::DevGrid& g, const Pos& pos, int* cw, double (*W)[2], double (*G)[2])
+__host__ __device__ bool support(const psum_hip::DevGrid& g, const Pos& pos, int* cw, double (*W)[2], double (*G)[2])
 {
     double x[D], s[D];
     for (int d = 0; d < D; ++d) x[d] = (double)pos[d];
@@ -117,7 +117,7 @@ __device__ bool support(const psum_hip::DevGrid& g, const Pos& pos, int* cw, dou
 
 // phi(x): sum_i phi[idx_i] c_i, skipping |c_i| <= 1e-10 (field_interp.hpp:77-91)
 template <typename Pos, int D, typename Scalar>
-__device__ Scalar interp(const Pos& pos, const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
+__host__ __device__ Scalar interp(const Pos& pos, const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
 {
     int cw[D];
     double W[D][2], G[D][2];
@@ -132,7 +132,7 @@ __device__ Scalar interp(const Pos& pos, const device_field_acc<D, var_loc::node
 
 // grad phi(x) of the multilinear interpolant (field_interp.hpp:93-113)
 template <typename Pos, int D, typename Scalar>
-__device__ std::array<Scalar, D> interp_diff(const Pos& pos,
+__host__ __device__ std::array<Scalar, D> interp_diff(const Pos& pos,
                                              const device_field_acc<D, var_loc::nodeCentered, Scalar>& acc)
 {
     int cw[D];

@@ -11,6 +11,7 @@
 #include "device_field.hpp"
 #include "device_vector.hpp"
 #include "grid.hpp"
+#include "host_field.hpp"
 #include "particle_boundary.hpp"
 #include "particle_group.hpp"
 #include "poisson_solver.hpp"
@@ -30,6 +31,13 @@ using psum::field::cell_field2D;
 using psum::field::cell_field3D;
 using psum::field::device_field;
 using psum::field::grid1D;
+using psum::field::host_cell_field1D;
+using psum::field::host_cell_field2D;
+using psum::field::host_cell_field3D;
+using psum::field::host_field;
+using psum::field::host_node_field1D;
+using psum::field::host_node_field2D;
+using psum::field::host_node_field3D;
 using psum::field::grid2D;
 using psum::field::grid3D;
 using psum::field::interp;

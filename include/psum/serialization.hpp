@@ -7,6 +7,8 @@
 //   std::vector<vec<D>> name               (n x D x 1)        container_sl.hpp:96-134
 //   simple_grid       name.lower/.upper (double D x 1), name.cells (int32 D x 1)
 //                                                             simple_grid.hpp:429-443
+//   host_field        name.location (int32), name.grid.*, name.content
+//                                                             host_field.hpp:310-337
 //   device_field      name.location (int32), name.grid.*, name.content.device,
 //                     name.content.content                   device_field.hpp:327-352
 //   device_vector     name.device, name.content              device_vector.hpp:294-341
@@ -19,6 +21,7 @@
 #include <vector>
 
 #include "device_field.hpp"
+#include "host_field.hpp"
 #include "device_vector.hpp"
 #include "grid.hpp"
 #include "mas.hpp"
@@ -189,6 +192,29 @@ void load(mas_file& f, const std::string& name, field::device_field<D, Loc, Scal
     load(f, name + ".content.content", c);
     if (c.size() != fld.size()) throw std::runtime_error("MAS: block '" + name + "' content does not match its grid");
     fld.copy(c);
+}
+
+// host_field: name.location, name.grid.*, name.content (host_field.hpp:310-337)
+template <int D, field::var_loc Loc, typename Scalar>
+void save(mas_file& f, const std::string& name, const field::host_field<D, Loc, Scalar>& fld)
+{
+    save(f, name + ".location", (int32_t)Loc);
+    save(f, name + ".grid", fld.getGrid());
+    save(f, name + ".content", fld.getContent());
+}
+// the field takes the stored grid
+template <int D, field::var_loc Loc, typename Scalar>
+void load(mas_file& f, const std::string& name, field::host_field<D, Loc, Scalar>& fld)
+{
+    int32_t loc = -1;
+    load(f, name + ".location", loc);
+    if (loc != (int32_t)Loc) throw std::runtime_error("MAS: block '" + name + "' has another location");
+    field::simple_grid<D> g;
+    load(f, name + ".grid", g);
+    std::vector<Scalar> c;
+    load(f, name + ".content", c);
+    if (c.size() != g.contentSize(Loc)) throw std::runtime_error("MAS: block '" + name + "' content does not match its grid");
+    fld.reset(g, std::move(c));
 }
 
 template <typename T>

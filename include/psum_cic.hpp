@@ -56,14 +56,14 @@ inline bool grid_ok(const psum_grid* g)
     return n < (int64_t)0xFFFFFFFF;
 }
 
-// ---------------------------------------------------------------- device side
+// ------------------------------------------------- device side (and host where marked)
 
 // Base node of x and the fractional offset, FP64 (simple_grid.hpp:89-95,
 // 201-206, 327-328).  cw is the base node after one periodic wrap over the
 // w nodes of each dimension (simple_grid.hpp:338-343).  The reference result
 // is defined only while that single wrap stays in range: cN in [-w, 2w-2].
 template <int D>
-__device__ __forceinline__ bool base_cell(const DevGrid& g, const double* x, int* cw, double* s)
+__host__ __device__ __forceinline__ bool base_cell(const DevGrid& g, const double* x, int* cw, double* s)
 {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -108,7 +108,7 @@ __device__ __forceinline__ long long round_to_ll(double t)
 // Linear node index of support point i: bit j of i offsets dimension D-1-j;
 // cw + 1 == w wraps to node 0.
 template <int D>
-__device__ __forceinline__ uint32_t support_index(const DevGrid& g, const int* cw, int i)
+__host__ __device__ __forceinline__ uint32_t support_index(const DevGrid& g, const int* cw, int i)
 {
     uint32_t lin = 0;
 #pragma unroll
@@ -123,7 +123,7 @@ __device__ __forceinline__ uint32_t support_index(const DevGrid& g, const int* c
 // CIC weight of support point i: product of {1-s, s} factors taken from
 // dimension D-1 down to 0 (simple_interpolation.hpp:36-54).
 template <typename T, int D>
-__device__ __forceinline__ T cic_coef(const T (*W)[2], int i)
+__host__ __device__ __forceinline__ T cic_coef(const T (*W)[2], int i)
 {
     T c = W[D - 1][i & 1];
 #pragma unroll
@@ -134,7 +134,7 @@ __device__ __forceinline__ T cic_coef(const T (*W)[2], int i)
 // Gradient weight of support point i along dimension dd (interp_tensor_diff,
 // simple_interpolation.hpp:61-90): factor G for dimension dd, W otherwise.
 template <typename T, int D>
-__device__ __forceinline__ T cic_grad_coef(const T (*W)[2], const T (*G)[2], int i, int dd)
+__host__ __device__ __forceinline__ T cic_grad_coef(const T (*W)[2], const T (*G)[2], int i, int dd)
 {
     T c = (D - 1 == dd) ? G[D - 1][i & 1] : W[D - 1][i & 1];
 #pragma unroll

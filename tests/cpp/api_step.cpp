@@ -4,7 +4,12 @@
 //   deposit   add_back in for_each (FP64 atomics)  vs  psum_hip_deposit_cic (fixed point)
 //   push      interp_diff + kick + drift + wrap     vs  psum_hip_gather_push (bit-exact)
 //   compress  invalidated particles dropped, survivors in order
+//   shuffle   a permutation of the records
+//   host_field interp / interp_diff equal the device accessors' (bitwise),
+//             add_back, for_each with positions, MAS round trip
 // Prints one line per check and "ALL OK" at the end; exit status 0 on success.
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +25,20 @@ using Particle = tagged_struct<tag_bind<position, vec<2>>, tag_bind<velocity, ve
 using Group = particle_group<Particle, pos_x_nan_is_invalid>;
 
 static int fails = 0;
+
+template <typename Acc>
+__global__ void k_sample(Acc acc, const double* xy, size_t m, double* out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    psum::vec<2> p;
+    p[0] = xy[2 * i];
+    p[1] = xy[2 * i + 1];
+    out[3 * i] = interp(p, acc);
+    const auto g = interp_diff(p, acc);
+    out[3 * i + 1] = g[0];
+    out[3 * i + 2] = g[1];
+}
 static void report(const char* what, bool ok, double v)
 {
     std::printf("%-40s %s (%.3e)\n", what, ok ? "ok" : "FAIL", v);
@@ -423,6 +442,100 @@ int main()
                 ++diff;
         report("interp_diff_B_spline<3> push vs kernel (bitwise)", diff == 0, (double)diff);
         for (void* q : {(void*)bx, (void*)by, (void*)bvx, (void*)bvy, (void*)bacc}) psum::hip_check(hipFree(q), "f");
+    }
+
+    // ---- shuffle: the same records in another order
+    {
+        auto before = group.get_content().to_host();
+        group.shuffle(12345);
+        auto after_s = group.get_content().to_host();
+        size_t same_place = 0;
+        for (size_t i = 0; i < std::min(before.size(), after_s.size()); ++i)
+            same_place += std::memcmp(&before[i], &after_s[i], sizeof(Particle)) == 0;
+        auto key = [](const Particle& p) {
+            return std::array<double, 4>{get<position>(p).x(), get<position>(p).y(), get<velocity>(p).x(),
+                                         get<velocity>(p).y()};
+        };
+        auto less = [&](const Particle& a, const Particle& b) { return key(a) < key(b); };
+        std::sort(before.begin(), before.end(), less);
+        std::sort(after_s.begin(), after_s.end(), less);
+        bool perm = before.size() == after_s.size();
+        for (size_t i = 0; perm && i < before.size(); ++i) perm = key(before[i]) == key(after_s[i]);
+        report("shuffle permutes the records", perm && same_place < before.size() / 100, (double)same_place);
+    }
+
+    // ---- host_field: host interp / interp_diff == device accessors (bitwise)
+    {
+        host_node_field2D<double> hphi(phi);
+        const size_t m = 4096;
+        std::vector<double> xy(2 * m);
+        std::uniform_real_distribution<double> UL(-2.0, L + 2.0);
+        for (auto& v : xy) v = UL(rng);
+        double *dxy, *dout;
+        psum::hip_check(hipMalloc(&dxy, 2 * m * 8), "m");
+        psum::hip_check(hipMalloc(&dout, 3 * m * 8), "m");
+        psum::hip_check(hipMemcpy(dxy, xy.data(), 2 * m * 8, hipMemcpyHostToDevice), "m");
+        hipLaunchKernelGGL(k_sample<decltype(phi.acc())>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                           phi.acc(), dxy, m, dout);
+        psum::hip_check(hipGetLastError(), "k_sample");
+        std::vector<double> dev(3 * m);
+        psum::hip_check(hipMemcpy(dev.data(), dout, 3 * m * 8, hipMemcpyDeviceToHost), "m");
+        size_t diff = 0;
+        for (size_t i = 0; i < m; ++i) {
+            const std::array<double, 2> p{xy[2 * i], xy[2 * i + 1]};
+            const double v = interp(p, hphi);
+            const auto g = interp_diff(p, hphi);
+            diff += std::memcmp(&v, &dev[3 * i], 8) != 0 || std::memcmp(&g[0], &dev[3 * i + 1], 8) != 0 ||
+                    std::memcmp(&g[1], &dev[3 * i + 2], 8) != 0;
+        }
+        report("host_field interp/interp_diff == device (bitwise)", diff == 0, (double)diff);
+        psum::hip_check(hipFree(dxy), "f");
+        psum::hip_check(hipFree(dout), "f");
+
+        // add_back on the host vs the device (FP64 atomics: order only)
+        node_field2D<double> rd(st, grid);
+        host_node_field2D<double> rh(grid);
+        group.for_each([&](psum::handler& h) {
+            auto acc = rd.get_access(h);
+            return [=](Particle& p) { add_back(get<position>(p), 0.25, acc); };
+        });
+        for (const auto& p : group.get_content().to_host())
+            if (std::isfinite(get<position>(p).x())) add_back(get<position>(p), 0.25, rh);
+        const auto rdh = rd.to_host();
+        double e = 0, mx = 0;
+        for (size_t i = 0; i < rdh.size(); ++i) {
+            e = std::max(e, std::fabs(rdh[i] - rh(i)));
+            mx = std::max(mx, std::fabs(rh(i)));
+        }
+        report("host_field add_back vs device add_back", e <= 1e-12 * mx && mx > 0, e / mx);
+
+        // for_each with positions sets a linear field; interp reproduces it
+        host_node_field2D<double> lin(grid);
+        lin.for_each([](size_t, double& v, const std::array<double, 2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; });
+        double lerr = 0;
+        std::uniform_real_distribution<double> UI(1.0, L - 2.0);
+        for (int k = 0; k < 1000; ++k) {
+            const std::array<double, 2> p{UI(rng), UI(rng)};
+            lerr = std::max(lerr, std::fabs(interp(p, lin) - (2.0 * p[0] - 3.0 * p[1] + 1.0)));
+        }
+        report("host_field for_each(i, v, x) + interp is linear", lerr < 1e-10, lerr);
+
+        // MAS round trip
+        const std::string path = "/tmp/psum_api_step_hf.mas";
+        {
+            mas_file f(path, mas_file::replaceMode);
+            psum::serialization::save(f, "phi", hphi);
+        }
+        host_node_field2D<double> back;
+        {
+            mas_file f(path);
+            psum::serialization::load(f, "phi", back);
+        }
+        const bool same = back.has_grid() && back.getContent() == hphi.getContent() &&
+                          back.getGrid().numCells<0>() == grid.numCells<0>() &&
+                          back.getGrid().numCells<1>() == grid.numCells<1>();
+        report("host_field MAS save/load", same, (double)back.getContent().size());
+        std::remove(path.c_str());
     }
 
     std::printf(fails ? "FAILED %d\n" : "ALL OK\n", fails);
