@@ -8,6 +8,7 @@
 
 #include <array>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "../psum_bspline.hpp"
@@ -27,6 +28,37 @@ struct device_field_acc {
     psum_hip::DevGrid grid{};
     __host__ __device__ Scalar& operator()(size_t i) const { return content[i]; }
 };
+
+namespace detail {
+// position of element i: the node, or the cell centre
+template <int D, var_loc Loc>
+__host__ __device__ std::array<double, D> element_position(const simple_grid<D>& g, size_t i)
+{
+    if constexpr (Loc == var_loc::nodeCentered) {
+        return g.nodePosition(g.i2n(i));
+    } else {
+        return g.cellCenter(i);
+    }
+}
+
+template <int D, var_loc Loc, typename Scalar, typename F>
+__global__ void k_field_for_each(Scalar* v, size_t n, simple_grid<D> g, F f)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (std::is_invocable_v<F&, size_t, Scalar&, const std::array<double, D>&>)
+        f(i, v[i], element_position<D, Loc>(g, i));
+    else
+        f(i, v[i]);
+}
+
+template <typename Scalar>
+__global__ void k_field_fill(Scalar* v, size_t n, Scalar x)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = x;
+}
+}  // namespace detail
 
 template <int D, var_loc Loc, typename Scalar, int QuantitySize = 1>
 class device_field {
@@ -69,6 +101,36 @@ public:
     {
         hip_check(hipMemsetAsync(data_, 0, size() * sizeof(Scalar), stream_), "setZero");
         hip_check(hipStreamSynchronize(stream_), "setZero");
+    }
+    bool has_grid() const { return true; }
+    void setConstant(const Scalar& x)
+    {
+        const size_t n = size();
+        hipLaunchKernelGGL(detail::k_field_fill<Scalar>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream_,
+                           data_, n, x);
+        hip_check(hipGetLastError(), "setConstant");
+        hip_check(hipStreamSynchronize(stream_), "setConstant");
+    }
+    // content of b (same size)
+    device_field& operator<<(const device_field& b)
+    {
+        if (b.size() != size()) throw std::invalid_argument("device_field: operator<< of another size");
+        hip_check(hipMemcpyAsync(data_, b.data_, size() * sizeof(Scalar), hipMemcpyDeviceToDevice, stream_), "<<");
+        hip_check(hipStreamSynchronize(stream_), "<<");
+        return *this;
+    }
+    // func(handler&) returns f(i, value) or f(i, value, position of element
+    // i), applied to every element on the device (device_field.hpp:138-163)
+    template <typename FuncType>
+    void for_each(FuncType&& func)
+    {
+        const size_t n = size();
+        handler h{stream_};
+        auto f = func(h);
+        hipLaunchKernelGGL((detail::k_field_for_each<D, Loc, Scalar, decltype(f)>), dim3((unsigned)((n + 255) / 256)),
+                           dim3(256), 0, stream_, data_, n, grid_, f);
+        hip_check(hipGetLastError(), "for_each");
+        hip_check(hipStreamSynchronize(stream_), "for_each");
     }
     void copy(const std::vector<Scalar>& host)
     {

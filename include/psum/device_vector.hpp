@@ -42,6 +42,13 @@ struct device_vector_acc {
     }
 };
 
+template <typename T, typename F>
+__global__ void k_vector_for_each(T* data, size_t n, F f)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f(data[i]);
+}
+
 template <typename T>
 class device_vector {
 public:
@@ -114,6 +121,52 @@ public:
     }
 
     void clear() { set_size(0); }
+
+    // the accessor without a capacity check on push_back's index: the
+    // caller guarantees the room (device_vector.hpp:207-209)
+    device_vector_acc<T> get_access_without_overflow_check(handler& h) const { return get_access(h); }
+
+    // throws when a push_back found the vector full (its element was
+    // dropped) since the size was last set (device_vector.hpp:158)
+    void if_overflow() const
+    {
+        unsigned long long n = 0;
+        if (size_dev_) {
+            hip_check(hipMemcpyAsync(&n, size_dev_, sizeof n, hipMemcpyDeviceToHost, stream_), "if_overflow");
+            hip_check(hipStreamSynchronize(stream_), "if_overflow");
+        }
+        if (n > capacity_) throw std::overflow_error("device_vector: push_back beyond the capacity");
+    }
+
+    // the host vector's elements, size and all
+    void copy(const std::vector<T>& host)
+    {
+        reserve(host.size());
+        if (!host.empty())
+            hip_check(hipMemcpyAsync(data_, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream_),
+                      "device_vector copy");
+        set_size(host.size());
+    }
+
+    // func(handler&) returns the per-element function f(T&), applied to
+    // every element (device_vector.hpp:229-266)
+    template <typename FuncType>
+    void for_each(FuncType&& func)
+    {
+        const size_t n = size();
+        if (n == 0) return;
+        handler h{stream_};
+        auto f = func(h);
+        hipLaunchKernelGGL((k_vector_for_each<T, decltype(f)>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           stream_, data_, n, f);
+        hip_check(hipGetLastError(), "for_each");
+        hip_check(hipStreamSynchronize(stream_), "for_each");
+    }
+    template <typename FuncType>
+    void for_each(FuncType&& func) const
+    {
+        const_cast<device_vector*>(this)->for_each(std::forward<FuncType>(func));
+    }
 
     std::vector<T> to_host() const
     {

@@ -118,6 +118,17 @@ public:
         for (int d = 0; d < Dim; ++d) p[d] = lb_[d] + n[d] * delta_[d];
         return p;
     }
+    // centre of the cell of linear index i (last dimension fastest)
+    __host__ __device__ std::array<double, Dim> cellCenter(size_t i) const
+    {
+        std::array<double, Dim> p{};
+        for (int d = Dim - 1; d >= 0; --d) {
+            const size_t c = i % (size_t)n_[d];
+            i /= (size_t)n_[d];
+            p[d] = lb_[d] + ((double)c + 0.5) * delta_[d];
+        }
+        return p;
+    }
 
     // The C ABI's grid.  periodic[d] = 1 wraps particles into
     // [plo[d], plo[d] + plen[d]) (defaults: the reference periodic node

@@ -90,22 +90,7 @@ public:
     const Scalar& operator()(size_t i) const { return content_[i]; }
 
     // node position, or cell centre
-    Position position(size_t i) const
-    {
-        const Grid& g = grid();
-        if constexpr (Loc == var_loc::nodeCentered) {
-            return g.nodePosition(g.i2n(i));
-        } else {
-            Position p{};
-            const psum_grid pg = g.to_psum();
-            for (int d = D - 1; d >= 0; --d) {
-                const size_t c = i % (size_t)pg.cells[d];
-                i /= (size_t)pg.cells[d];
-                p[d] = pg.lb[d] + ((double)c + 0.5) * pg.delta[d];
-            }
-            return p;
-        }
-    }
+    Position position(size_t i) const { return detail::element_position<D, Loc>(grid(), i); }
 
     // accessor over the host content (interp / interp_diff take it)
     acc_type acc() const

@@ -27,6 +27,12 @@ using Group = particle_group<Particle, pos_x_nan_is_invalid>;
 static int fails = 0;
 
 template <typename Acc>
+__global__ void k_push(Acc acc)
+{
+    acc.push_back((int)threadIdx.x);
+}
+
+template <typename Acc>
 __global__ void k_sample(Acc acc, const double* xy, size_t m, double* out)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -536,6 +542,66 @@ int main()
                           back.getGrid().numCells<1>() == grid.numCells<1>();
         report("host_field MAS save/load", same, (double)back.getContent().size());
         std::remove(path.c_str());
+    }
+
+    // ---- device_field for_each / setConstant / <<, device_vector, insert
+    {
+        node_field2D<double> dn(st, grid);
+        dn.for_each([](psum::handler&) {
+            return [](size_t, double& v, const std::array<double, 2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; };
+        });
+        host_node_field2D<double> hn(grid);
+        hn.for_each([](size_t, double& v, const std::array<double, 2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; });
+        cell_field2D<double> dc(st, grid);
+        dc.for_each([](psum::handler&) {
+            return [](size_t i, double& v, const std::array<double, 2>& x) { v = x[0] * 1000.0 + x[1] + 1e-9 * i; };
+        });
+        host_cell_field2D<double> hc(grid);
+        hc.for_each([](size_t i, double& v, const std::array<double, 2>& x) { v = x[0] * 1000.0 + x[1] + 1e-9 * i; });
+        report("device_field for_each(i, v, x) == host_field (node, cell)",
+               dn.to_host() == hn.getContent() && dc.to_host() == hc.getContent(), (double)dc.size());
+        node_field2D<double> dk(st, grid);
+        dk.setConstant(2.5);
+        dn << dk;
+        const auto dkh = dn.to_host();
+        report("device_field setConstant and <<",
+               std::all_of(dkh.begin(), dkh.end(), [](double v) { return v == 2.5; }), dkh[0]);
+
+        psum::particle_container::device_vector<double> dv(st, 16);
+        dv.copy({1, 2, 3, 4, 5});
+        dv.for_each([](psum::handler&) { return [](double& v) { v = v * v; }; });
+        const auto dvh = dv.to_host();
+        bool vok = dvh == std::vector<double>{1, 4, 9, 16, 25};
+        bool threw = false;
+        try {
+            dv.if_overflow();
+        } catch (const std::overflow_error&) {
+            threw = true;
+        }
+        vok = vok && !threw;
+        psum::particle_container::device_vector<int> small(st, 4);
+        {
+            auto acc = small.get_access();
+            hipLaunchKernelGGL(k_push<decltype(acc)>, dim3(1), dim3(64), 0, st, acc);
+            psum::hip_check(hipStreamSynchronize(st), "push");
+        }
+        try {
+            small.if_overflow();
+        } catch (const std::overflow_error&) {
+            threw = true;
+        }
+        report("device_vector copy / for_each / if_overflow", vok && threw && small.size() == 4, (double)dvh.back());
+
+        std::vector<Particle> recs(100);
+        for (size_t i = 0; i < recs.size(); ++i) {
+            get<position>(recs[i]).x() = i % 10 == 3 ? std::nan("") : 1.0 + i * 0.1;
+            get<position>(recs[i]).y() = 2.0;
+        }
+        psum::particle_container::device_vector<Particle> drecs(st, recs);
+        Group g3(st, 64);
+        g3.insert(drecs);
+        report("particle_group insert(device_vector) lists invalid records",
+               g3.size() == 90 && g3.get_invalid_indexes().size() == 10, (double)g3.size());
     }
 
     std::printf(fails ? "FAILED %d\n" : "ALL OK\n", fails);
