@@ -8,6 +8,9 @@
 
 #include <array>
 #include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -138,6 +141,20 @@ public:
         hip_check(hipMemcpyAsync(data_, host.data(), size() * sizeof(Scalar), hipMemcpyHostToDevice, stream_), "copy");
         hip_check(hipStreamSynchronize(stream_), "copy");
     }
+    // Tecplot ASCII file of the field (device_field.hpp:203-226): variables
+    // x[,y[,z]],varName and the grid's zone; scientific, 4 digits
+    void plot(const std::string& filename, std::string varName = "_not_defined_",
+              double time = std::nan("")) const
+    {
+        std::ofstream fp(filename, std::ios::out);
+        if (!fp) throw std::runtime_error(filename + " cannot be opened");
+        fp << std::scientific << std::setprecision(4);
+        if (varName == "_not_defined_") varName = "v1";
+        fp << "variables=" << (D == 1 ? "x," : (D == 2 ? "x,y," : "x,y,z,")) << varName;
+        const auto h = to_host();
+        getGrid().plot(fp, h.data(), time, Loc);
+    }
+
     std::vector<Scalar> to_host() const
     {
         std::vector<Scalar> h(size());

@@ -9,6 +9,8 @@
 
 #include <array>
 #include <cmath>
+#include <fstream>
+#include <iomanip>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -91,6 +93,20 @@ public:
 
     // node position, or cell centre
     Position position(size_t i) const { return detail::element_position<D, Loc>(grid(), i); }
+
+    // Tecplot ASCII file of the field (device_field.hpp:203-226): variables
+    // x[,y[,z]],varName and the grid's zone; scientific, 4 digits
+    void plot(const std::string& filename, std::string varName = "_not_defined_",
+              double time = std::nan("")) const
+    {
+        std::ofstream fp(filename, std::ios::out);
+        if (!fp) throw std::runtime_error(filename + " cannot be opened");
+        fp << std::scientific << std::setprecision(4);
+        if (varName == "_not_defined_") varName = "v1";
+        fp << "variables=" << (D == 1 ? "x," : (D == 2 ? "x,y," : "x,y,z,")) << varName;
+        const auto h = getContent();
+        getGrid().plot(fp, h.data(), time, Loc);
+    }
 
     // accessor over the host content (interp / interp_diff take it)
     acc_type acc() const

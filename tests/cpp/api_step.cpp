@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <iterator>
 #include <random>
 #include <vector>
 
@@ -602,6 +604,39 @@ int main()
         g3.insert(drecs);
         report("particle_group insert(device_vector) lists invalid records",
                g3.size() == 90 && g3.get_invalid_indexes().size() == 10, (double)g3.size());
+    }
+
+    // ---- plot: the Tecplot text of a node and a cell field, character for character
+    {
+        grid2D pg2({0.0, 0.0}, {2.0, 1.0}, {2, 1});
+        host_node_field2D<double> hf(pg2);
+        hf.for_each([](size_t i, double& v) { v = 1.5 * i; });
+        cell_field2D<double> cf(st, pg2);
+        cf.copy({0.25, 0.75});
+        const std::string pn = "/tmp/psum_api_step_n.plt", pc = "/tmp/psum_api_step_c.plt";
+        hf.plot(pn, "phi");
+        cf.plot(pc, "rho", 2.0);
+        auto slurp = [](const std::string& f) {
+            std::ifstream in(f);
+            return std::string(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+        };
+        const std::string en =
+            "variables=x,y,phi\nzone i=2,j=3,F=point\n"
+            "\n0.0000e+00\t0.0000e+00\t0.0000e+00\n0.0000e+00\t1.0000e+00\t1.5000e+00"
+            "\n1.0000e+00\t0.0000e+00\t3.0000e+00\n1.0000e+00\t1.0000e+00\t4.5000e+00"
+            "\n2.0000e+00\t0.0000e+00\t6.0000e+00\n2.0000e+00\t1.0000e+00\t7.5000e+00";
+        const std::string ec =
+            "variables=x,y,rho\nzone i=2,j=3,F=block, VARLOCATION=([3]=CELLCENTERED)\n"
+            "AUXDATA TimeUnits=\"Seconds\" SolutionTime=2.0000e+00"
+            "\n0.0000e+00\n0.0000e+00\n1.0000e+00\n1.0000e+00\n2.0000e+00\n2.0000e+00"
+            "\n0.0000e+00\n1.0000e+00\n0.0000e+00\n1.0000e+00\n0.0000e+00\n1.0000e+00"
+            "\n2.5000e-01\n7.5000e-01";
+        const std::string gn = slurp(pn), gc = slurp(pc);
+        if (gn != en) std::printf("  node plot:\n%s\n", gn.c_str());
+        if (gc != ec) std::printf("  cell plot:\n%s\n", gc.c_str());
+        report("plot: Tecplot text of node and cell fields", gn == en && gc == ec, (double)(gn.size() + gc.size()));
+        std::remove(pn.c_str());
+        std::remove(pc.c_str());
     }
 
     std::printf(fails ? "FAILED %d\n" : "ALL OK\n", fails);
