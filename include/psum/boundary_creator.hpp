@@ -24,6 +24,7 @@
 
 namespace psum {
 namespace field {
+enum class boundary_direction_1d { L, R, Count };
 enum class boundary_direction_2d { N, S, E, W, Count };
 enum class boundary_direction_3d { Xpos, Xneg, Ypos, Yneg, Zpos, Zneg, Count };
 }  // namespace field
@@ -102,6 +103,8 @@ private:
     std::vector<fixed_boundary<Dim>> b_;
 };
 
+using fixed_boundary_1d = fixed_boundary<1>;
+using fixed_boundary_1d_list = fixed_boundary_list<1>;
 using fixed_boundary_2d = fixed_boundary<2>;
 using fixed_boundary_3d = fixed_boundary<3>;
 using fixed_boundary_2d_list = fixed_boundary_list<2>;
@@ -114,7 +117,9 @@ template <int Dim>
 class mixed_boundary {
 public:
     using Pos = std::array<double, Dim>;
-    using direction_type = std::conditional_t<Dim == 2, field::boundary_direction_2d, field::boundary_direction_3d>;
+    using direction_type =
+        std::conditional_t<Dim == 1, field::boundary_direction_1d,
+                           std::conditional_t<Dim == 2, field::boundary_direction_2d, field::boundary_direction_3d>>;
     using Coeffs = std::pair<double, double>;
     struct mixed_boundary_node {
         size_t element;
@@ -229,6 +234,14 @@ public:
             if (b.get_direction() == d && b.related_elements().count(idx)) return true;
         return false;
     }
+    // (value_coef, grad_coef) of node idx on side d (mixed_boundary.hpp:
+    // 219-229)
+    typename mixed_boundary<Dim>::Coeffs evaluate_coeffs_at(size_t idx, direction_type d) const
+    {
+        for (const auto& b : b_)
+            if (b.get_direction() == d && b.related_elements().count(idx)) return b.evaluate_coeffs_at(idx, d);
+        throw std::runtime_error("mixed_boundary_list: no condition at this node and side");
+    }
     bool element_exists(size_t idx) const
     {
         for (const auto& b : b_)
@@ -249,6 +262,8 @@ private:
     std::vector<mixed_boundary<Dim>> b_;
 };
 
+using mixed_boundary_1d = mixed_boundary<1>;
+using mixed_boundary_1d_list = mixed_boundary_list<1>;
 using mixed_boundary_2d = mixed_boundary<2>;
 using mixed_boundary_3d = mixed_boundary<3>;
 using mixed_boundary_2d_list = mixed_boundary_list<2>;
@@ -272,6 +287,23 @@ fixed_boundary<Dim> select(const field::simple_grid<Dim>& g, Pred&& keep)
     return b;
 }
 }  // namespace detail
+
+// The end node of a 1D grid (L: node 0, R: the last), or the nodes within
+// [x0, x1] widened by tolerance * span (Dirichlet.hpp:24-56)
+inline fixed_boundary_1d Dirichlet_point(const field::grid1D& g, field::boundary_direction_1d dir)
+{
+    const int last = g.numCells<0>();
+    return detail::select<1>(g, [&](const std::array<int, 1>& n, const std::array<double, 1>&) {
+        return n[0] == (dir == field::boundary_direction_1d::L ? 0 : last);
+    });
+}
+inline fixed_boundary_1d Dirichlet_range(const field::grid1D& g, double x0, double x1, double geo_tolerance = 1e-10)
+{
+    const double tol = geo_tolerance * g.span<0>();
+    return detail::select<1>(g, [&](const std::array<int, 1>&, const std::array<double, 1>& p) {
+        return p[0] >= x0 - tol && p[0] <= x1 + tol;
+    });
+}
 
 // The nodes of one edge of a 2D grid (S: first index of y, N: last, W:
 // first of x, E: last), optionally only where the coordinate along the edge
@@ -406,6 +438,37 @@ inline mixed_boundary_3d Neumann_plane(const field::grid3D& g, field::boundary_d
                                        std::pair<double, double> min_max = {NAN, NAN})
 {
     return detail::mixed_side<3>(g, dir, min_max, [](const std::array<double, 3>&) { return std::make_pair(0.0, 1.0); });
+}
+
+// mixed condition at an end node of a 1D grid (Neumann.hpp:24-44,
+// Robin.hpp:24-54)
+namespace detail {
+inline mixed_boundary_1d mixed_point(const field::grid1D& g, field::boundary_direction_1d dir,
+                                     const std::function<std::pair<double, double>(const std::array<double, 1>&)>& c)
+{
+    const size_t idx = dir == field::boundary_direction_1d::L ? 0 : (size_t)g.numCells<0>();
+    std::map<size_t, std::array<double, 1>> e;
+    e[idx] = g.nodePosition(g.i2n(idx));
+    mixed_boundary_1d b;
+    b.set_related_elements(e);
+    b.set_coeff_func(c);
+    b.set_direction(dir);
+    return b;
+}
+}  // namespace detail
+inline mixed_boundary_1d Neumann_point(const field::grid1D& g, field::boundary_direction_1d dir)
+{
+    return detail::mixed_point(g, dir, [](const std::array<double, 1>&) { return std::make_pair(0.0, 1.0); });
+}
+inline mixed_boundary_1d Robin_point(const field::grid1D& g, field::boundary_direction_1d dir,
+                                     const std::function<std::pair<double, double>(double)>& coef)
+{
+    return detail::mixed_point(g, dir, [coef](const std::array<double, 1>& p) { return coef(p[0]); });
+}
+inline mixed_boundary_1d Robin_point(const field::grid1D& g, field::boundary_direction_1d dir,
+                                     std::pair<double, double> coef)
+{
+    return detail::mixed_point(g, dir, [coef](const std::array<double, 1>&) { return coef; });
 }
 
 // Robin sides with (value_coef, grad_coef) constant or a function of position

@@ -15,6 +15,7 @@
 #include "particle_boundary.hpp"
 #include "particle_group.hpp"
 #include "poisson_solver.hpp"
+#include "poisson_solver_1d.hpp"
 #include "serialization.hpp"
 #include "tag.hpp"
 
@@ -62,14 +63,19 @@ using psum::particle_container::pos_x_nan_is_invalid;
 using psum::tag::get;
 using psum::tag::tag_bind;
 using psum::tag::tagged_struct;
+using psum::field::boundary_direction_1d;
 using psum::field::boundary_direction_2d;
 using psum::field::boundary_direction_3d;
+using psum::field_solver::fixed_boundary_1d_list;
 using psum::field_solver::fixed_boundary_2d_list;
 using psum::field_solver::fixed_boundary_3d_list;
+using psum::field_solver::mixed_boundary_1d;
+using psum::field_solver::mixed_boundary_1d_list;
 using psum::field_solver::mixed_boundary_2d;
 using psum::field_solver::mixed_boundary_2d_list;
 using psum::field_solver::mixed_boundary_3d;
 using psum::field_solver::mixed_boundary_3d_list;
+using psum::field_solver::Poisson_solver_1d;
 using psum::field_solver::Poisson_solver_2d;
 using psum::field_solver::Poisson_solver_3d;
 namespace boundary_creator = psum::field_solver::boundary_creator;

@@ -45,7 +45,8 @@ def _compile(src):
     return obj
 
 
-CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "api_step.cpp"), os.path.join(ROOT, "tests", "cpp", "mesh_router.cpp")]
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "api_step.cpp"), os.path.join(ROOT, "tests", "cpp", "mesh_router.cpp"),
+             os.path.join(ROOT, "tests", "cpp", "solver1d.cpp")]
 
 
 def build_cpp_tests(verbose=True):
