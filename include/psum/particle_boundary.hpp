@@ -219,6 +219,11 @@ inline std::vector<triangle_data> load_stl_file_binary(const std::string& filena
     if (!f.read(head, 84)) throw std::runtime_error("stl_loader: " + filename + ": too short for a binary STL");
     uint32_t n;
     std::memcpy(&n, head + 80, 4);
+    f.seekg(0, std::ios::end);
+    const std::streamoff bytes = f.tellg();
+    if (bytes < 84 + 50 * (std::streamoff)n)
+        throw std::runtime_error("stl_loader: " + filename + ": truncated (" + std::to_string(n) + " triangles declared)");
+    f.seekg(84, std::ios::beg);
     std::vector<triangle_data> out(n);
     char rec[50];
     for (uint32_t i = 0; i < n; ++i) {
@@ -325,6 +330,7 @@ public:
     // the distance-field skip of the full traversal (psum_mesh_set_udf)
     void set_acceleration_UDF(bool enabled)
     {
+        if (!h_ && !enabled) return;
         if (!h_) throw std::runtime_error("triangle_mesh_trigger::set_acceleration_UDF called before set");
         if (psum_mesh_set_udf(h_, enabled ? 1 : 0) != PSUM_OK) throw std::runtime_error(psum_hip_last_error());
     }

@@ -18,9 +18,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -155,6 +157,19 @@ void build(Mesh& m, const double* tri, const int32_t* ids, int64_t ntri)
 // apart share no point closer than max(|d| - 1, 0) per axis), capped at
 // udf_k - 1; then passes that raise a cell by one where no neighbour is
 // lower, until nothing changes or max_loop + 1 passes
+// f(i) for i in [0, n) on up to 16 host threads (each i independent)
+template <typename F>
+void host_parallel_for(int n, F&& f)
+{
+    const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int i = t; i < n; i += nt) f(i);
+        });
+    for (auto& th : pool) th.join();
+}
+
 void compute_udf(int I, int J, int K, const std::vector<uint32_t>& start, std::vector<float>& udf, int udf_k = 6,
                  int max_loop = 200)
 {
@@ -169,7 +184,7 @@ void compute_udf(int I, int J, int K, const std::vector<uint32_t>& start, std::v
     }
     const float cap = (float)(udf_k - 1);
     auto at = [&](int i, int j, int k) { return ((int64_t)i * J + j) * K + k; };
-    for (int i = 0; i < I; ++i)
+    host_parallel_for(I, [&](int i) {
         for (int j = 0; j < J; ++j)
             for (int k = 0; k < K; ++k) {
                 const int64_t c = at(i, j, k);
@@ -185,10 +200,11 @@ void compute_udf(int I, int J, int K, const std::vector<uint32_t>& start, std::v
                         }
                 udf[c] = std::min(best < 1e29f ? best : cap, cap);
             }
+    });
     std::vector<float> nu((size_t)n);
     for (int loop = 0;; ++loop) {
-        bool updated = false;
-        for (int i = 0; i < I; ++i)
+        std::atomic<bool> updated{false};
+        host_parallel_for(I, [&](int i) {
             for (int j = 0; j < J; ++j)
                 for (int k = 0; k < K; ++k) {
                     const int64_t c = at(i, j, k);
@@ -209,6 +225,7 @@ void compute_udf(int I, int J, int K, const std::vector<uint32_t>& start, std::v
                         updated = true;
                     }
                 }
+        });
         udf.swap(nu);
         if (!updated || loop + 1 > max_loop) break;
     }
