@@ -18,6 +18,7 @@
 #include "../psum_cic.hpp"
 #include "device_vector.hpp"
 #include "grid.hpp"
+#include "tag.hpp"
 
 namespace psum {
 namespace field {
@@ -33,15 +34,15 @@ struct device_field_acc {
 };
 
 namespace detail {
-// position of element i: the node, or the cell centre
+// position of element i (the node, or the cell centre) as a vec<D>, which
+// has the x() / y() / z() of the reference's position rows
 template <int D, var_loc Loc>
-__host__ __device__ std::array<double, D> element_position(const simple_grid<D>& g, size_t i)
+__host__ __device__ vec<D> element_position(const simple_grid<D>& g, size_t i)
 {
-    if constexpr (Loc == var_loc::nodeCentered) {
-        return g.nodePosition(g.i2n(i));
-    } else {
-        return g.cellCenter(i);
-    }
+    const std::array<double, D> a = Loc == var_loc::nodeCentered ? g.nodePosition(g.i2n(i)) : g.cellCenter(i);
+    vec<D> p;
+    for (int d = 0; d < D; ++d) p[d] = a[d];
+    return p;
 }
 
 template <int D, var_loc Loc, typename Scalar, typename F>
@@ -49,7 +50,7 @@ __global__ void k_field_for_each(Scalar* v, size_t n, simple_grid<D> g, F f)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if constexpr (std::is_invocable_v<F&, size_t, Scalar&, const std::array<double, D>&>)
+    if constexpr (std::is_invocable_v<F&, size_t, Scalar&, const vec<D>&>)
         f(i, v[i], element_position<D, Loc>(g, i));
     else
         f(i, v[i]);

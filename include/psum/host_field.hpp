@@ -33,7 +33,7 @@ public:
     static constexpr var_loc Location = Loc;
     using Value = Scalar;
     using Grid = simple_grid<D>;
-    using Position = std::array<double, D>;
+    using Position = vec<D>;   // x(), y(), z() and [d]
     using acc_type = device_field_acc<D, Loc, Scalar>;
 
     host_field() = default;

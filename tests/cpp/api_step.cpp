@@ -519,7 +519,7 @@ int main()
 
         // for_each with positions sets a linear field; interp reproduces it
         host_node_field2D<double> lin(grid);
-        lin.for_each([](size_t, double& v, const std::array<double, 2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; });
+        lin.for_each([](size_t, double& v, const vec<2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; });
         double lerr = 0;
         std::uniform_real_distribution<double> UI(1.0, L - 2.0);
         for (int k = 0; k < 1000; ++k) {
@@ -550,16 +550,16 @@ int main()
     {
         node_field2D<double> dn(st, grid);
         dn.for_each([](psum::handler&) {
-            return [](size_t, double& v, const std::array<double, 2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; };
+            return [](size_t, double& v, const vec<2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; };
         });
         host_node_field2D<double> hn(grid);
-        hn.for_each([](size_t, double& v, const std::array<double, 2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; });
+        hn.for_each([](size_t, double& v, const vec<2>& x) { v = 2.0 * x[0] - 3.0 * x[1] + 1.0; });
         cell_field2D<double> dc(st, grid);
         dc.for_each([](psum::handler&) {
-            return [](size_t i, double& v, const std::array<double, 2>& x) { v = x[0] * 1000.0 + x[1] + 1e-9 * i; };
+            return [](size_t i, double& v, const vec<2>& x) { v = x[0] * 1000.0 + x[1] + 1e-9 * i; };
         });
         host_cell_field2D<double> hc(grid);
-        hc.for_each([](size_t i, double& v, const std::array<double, 2>& x) { v = x[0] * 1000.0 + x[1] + 1e-9 * i; });
+        hc.for_each([](size_t i, double& v, const vec<2>& x) { v = x[0] * 1000.0 + x[1] + 1e-9 * i; });
         report("device_field for_each(i, v, x) == host_field (node, cell)",
                dn.to_host() == hn.getContent() && dc.to_host() == hc.getContent(), (double)dc.size());
         node_field2D<double> dk(st, grid);

@@ -29,10 +29,10 @@ int main()
         auto run = [&](Poisson_solver_1d& s, const grid1D& g, const std::function<double(double)>& rho_of,
                        const std::function<double(double)>& exact) {
             host_node_field1D<double> phi(g), rho(g);
-            rho.for_each([&](size_t, double& v, const std::array<double, 1>& x) { v = rho_of(x[0]); });
+            rho.for_each([&](size_t, double& v, const vec<1>& x) { v = rho_of(x[0]); });
             s.solve(phi.data(), rho.data());
             double e = 0;
-            phi.for_each([&](size_t, double& v, const std::array<double, 1>& x) { e = std::max(e, std::fabs(v - exact(x[0]))); });
+            phi.for_each([&](size_t, double& v, const vec<1>& x) { e = std::max(e, std::fabs(v - exact(x[0]))); });
             return e;
         };
         grid1D g({0.0}, {1.0}, {100});
@@ -55,12 +55,14 @@ int main()
             Poisson_solver_1d sk;
             sk.init(gk, fixed_boundary_1d_list{bc::Dirichlet_point(gk, D1::L) = 0, bc::Dirichlet_point(gk, D1::R) = 0});
             host_node_field1D<double> phi(gk), rho(gk);
-            rho.for_each([&](size_t, double& v, const std::array<double, 1>& x) {
-                v = -e0 * (-M_PI * M_PI * std::sin(M_PI * x[0]));
+            // the reference test's own lambda shape: (i, value, position) with pos.x()
+            rho.for_each([&](size_t, double& v, auto pos) {
+                const double x = pos.x();
+                v = -e0 * (-M_PI * M_PI * std::sin(M_PI * x));
             });
             sk.solve(phi.data(), rho.data());
             double s = 0;
-            phi.for_each([&](size_t, double& v, const std::array<double, 1>& x) {
+            phi.for_each([&](size_t, double& v, const vec<1>& x) {
                 s += (v - std::sin(M_PI * x[0])) * (v - std::sin(M_PI * x[0]));
             });
             l2.push_back(std::sqrt(s / phi.size()));
@@ -72,13 +74,13 @@ int main()
         sp.init(gp, fixed_boundary_1d_list{bc::Dirichlet_range(gp, 0.5, 0.5) = 0.0}, 1.0);
         host_node_field1D<double> pp(gp), rp(gp);
         const double per = 65.0 / 64.0;   // the cyclic row wraps the I + 1 nodes
-        rp.for_each([&](size_t, double& v, const std::array<double, 1>& x) { v = std::sin(2 * M_PI * x[0] / per); });
+        rp.for_each([&](size_t, double& v, const vec<1>& x) { v = std::sin(2 * M_PI * x[0] / per); });
         sp.solve(pp.data(), rp.data());
         // residual of the cyclic rows (the discrete equation holds)
         double res = 0;
         std::vector<double> rr(gp.contentSize(var_loc::nodeCentered));
         for (const auto& [i, j, a] : sp.coeff_triplets()) rr[i] += a * pp(j);
-        rp.for_each([&](size_t i, double& v, const std::array<double, 1>& x) {
+        rp.for_each([&](size_t i, double& v, const vec<1>& x) {
             if (std::fabs(x[0] - 0.5) > 1e-9) res = std::max(res, std::fabs(rr[i] - std::sin(2 * M_PI * x[0] / per)));
         });
         std::printf("  1D: linear %.2e, Neumann %.2e, Robin %.2e, order %.3f, cyclic residual %.2e\n", e1, e2, e3, order,
