@@ -179,3 +179,28 @@ def test_router_matches_oracle(dtype, udf):
         kinds += np.bincount(ref["hit_id"][ref["hit_id"] >= 0], minlength=4)
     assert total > 500 and np.all(kinds > 0), kinds
     router.trigger.close()
+
+
+def test_device_view_describes_the_partition():
+    """psum_mesh_get_view: dims, box, cells per length, the traversal mode
+    and the device arrays are the partition's; the distance field appears
+    in the view only while it is enabled"""
+    import ctypes as C
+    from psum_amd import mesh as Mh
+    from psum_amd._lib import PsumMeshView, lib
+    meshes = _scene()
+    shape = (12, 10, 14)
+    tm = Mh.TriangleMesh(meshes, *shape)
+    v = PsumMeshView()
+    assert lib().psum_mesh_get_view(tm.h, C.byref(v)) == 0
+    assert (v.I, v.J, v.K) == shape and v.fast == 0 and not v.udf
+    assert np.array_equal(np.array(v.box[:]), tm.box)
+    assert np.allclose(np.array(v.rd[:]), np.array(shape) / (tm.box[3:] - tm.box[:3]), rtol=0, atol=0)
+    assert v.cell_start and v.cell_tri and v.tri12 and v.ids
+    tm.set_fast(True)
+    assert lib().psum_mesh_get_view(tm.h, C.byref(v)) == 0 and v.fast == 1
+    tm.set_udf(True)
+    assert lib().psum_mesh_get_view(tm.h, C.byref(v)) == 0 and v.udf
+    tm.set_udf(False)
+    assert lib().psum_mesh_get_view(tm.h, C.byref(v)) == 0 and not v.udf
+    tm.close()
