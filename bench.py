@@ -1,15 +1,24 @@
 """Benchmark: particle-steps/s of the electrostatic PIC step, MG Poisson solve
 included, mixed precision vs FP64, on one MI355X.
 
-Workload (default): 2D two-stream instability, 1024 x 1024 periodic nodes,
-100 electrons per cell (104,857,600 particles), beams +-5 v_th, dt = 0.05,
-FP32 particle state with FP32 V-cycles inside FP64 residual refinement to
-||r||/||b|| <= 1e-8.  The same seeded inputs (cast to FP64) run through the
-plain FP64 path for the mixed/FP64 ratio.
+Headline workload (`value`): the 3D bounded box, the largest configuration
+one GPU holds: 256^3 cells with grounded (phi = 0) walls, 32 electrons + 32
+ions per cell (1,073,741,824 particles, mass ratio 1836, Te/Ti = 10),
+absorbing particle walls with compaction, 7-point multigrid, FP32 particle
+state with FP32 V-cycles inside FP64 residual refinement to
+||r||/||b|| <= 1e-8, dt 0.1.  The autotuner picks the sort interval and bin;
+every timed window starts on a sort step, so it holds whole sort periods
+(`sorts_in_window`).  The same inputs (cast to FP64) run through the plain
+FP64 path for the mixed/FP64 ratio.
 
-One step = deposit (fused into the previous push) + neutralisation + MG
-solve + field energy + gather/push/boundaries + diagnostics, all on the
-device; `value` = N_p * K / (wall time of K steps) for the mixed path.
+Beside it: the 2D two-stream case (1024^2 periodic nodes, 100 e-/cell,
+mixed and FP64), the plain FP64 2D Langmuir case (128^2, 64 e-/cell), the
+differentiable two-stream case (adjoint / forward), the batched ensemble,
+Monte Carlo collisions and the triangle-mesh boundary.
+
+One step = deposit (fused into the previous push) + MG solve + field energy
++ gather/push/boundaries + diagnostics, all on the device;
+`value` = N_p * K / (wall time of K steps) for the mixed path.
 
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this
 engine runs on one GPU.
@@ -33,38 +42,69 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every, pre=2, post=1,
-               graph=True, extrapolate=False):
+def _align_to_sort(st, sort_every):
+    """Untimed steps until the next step sorts (step index a multiple of
+    sort_every), so that a timed window starts on a sort step; returns how
+    many were run."""
+    if sort_every <= 0:
+        return 0
+    extra = (-st.stats()["steps"]) % sort_every
+    if extra:
+        st.run(extra, history=False)
+    return extra
+
+
+def timed_run(st, n, steps, sort_every):
+    """`steps` timed steps of stepper st (n particles) from a sort step;
+    wall clock around the loop, HIP events around every push launch."""
     import torch
-    sp = wl["species"]
-    t0 = time.time()
-    st = E.Stepper(wl["grid"], sp, dtype=dtype, mixed_mg=mixed, depth=depth, pre=pre, post=post,
-                   max_cycles=40, tol=tol, dt=wl["dt"], rho_bg=wl["rho_bg"],
-                   sort_every=sort_every, mg_graph=graph, extrapolate=extrapolate)
-    log(f"[bench] {dtype}{'+mixed MG' if mixed else ''}: setup {time.time() - t0:.1f}s")
-    if warmup:
-        st.run(warmup, history=False)
+    extra = _align_to_sort(st, sort_every)
     torch.cuda.synchronize()
-    cyc0 = st.stats()["cycles"]
+    s0 = st.stats()
     st.set_timing(True)
     t0 = time.perf_counter()
     hist = st.run(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     push_ms, launches, pushed = st.push_timing()
-    stats = st.stats()
+    s1 = st.stats()
+    sorts = 0 if sort_every <= 0 else (steps + sort_every - 1) // sort_every
+    return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, particle_steps_per_s=n * steps / wall,
+                push_ms_avg=push_ms / max(1, launches), push_launches=launches,
+                particles_per_launch=pushed / max(1, launches),
+                cycles_per_step=(s1["cycles"] - s0["cycles"]) / steps,
+                last_rel_res=s1["last_rel_res"], tile_fallback_blocks=s1["tile_fallback_blocks"],
+                absorbed=s1["absorbed"], sort_every=sort_every, sorts_in_window=sorts,
+                align_steps=extra, hist=hist)
+
+
+def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every, pre=2, post=1,
+               graph=True, extrapolate=False, label=""):
+    sp = wl["species"]
+    t0 = time.time()
+    st = E.Stepper(wl["grid"], sp, dtype=dtype, mixed_mg=mixed, depth=depth, pre=pre, post=post,
+                   max_cycles=40, tol=tol, dt=wl["dt"], rho_bg=wl["rho_bg"],
+                   sort_every=sort_every, mg_graph=graph, extrapolate=extrapolate,
+                   fixed=wl.get("fixed"))
+    log(f"[bench] {label} {dtype}{'+mixed MG' if mixed else ''}: setup {time.time() - t0:.1f}s")
+    if warmup:
+        st.run(warmup, history=False)
     n = sum(len(s["pos"][0]) for s in sp)
-    res = dict(wall_s=wall, ms_per_step=1e3 * wall / steps, particle_steps_per_s=n * steps / wall,
-               push_ms_avg=push_ms / max(1, launches), push_launches=launches,
-               particles_per_launch=pushed / max(1, launches),
-               cycles_per_step=(stats["cycles"] - cyc0) / steps,
-               last_rel_res=stats["last_rel_res"], tile_fallback_blocks=stats["tile_fallback_blocks"],
-               hist=hist)
+    res = timed_run(st, n, steps, sort_every)
     st.close()
-    log(f"[bench] {dtype}: {res['ms_per_step']:.3f} ms/step, "
+    log(f"[bench] {label} {dtype}: {res['ms_per_step']:.3f} ms/step, "
         f"{res['particle_steps_per_s']:.3e} particle-steps/s, push {res['push_ms_avg']:.3f} ms, "
-        f"{res['cycles_per_step']:.2f} V-cycles/step, tile fallbacks {res['tile_fallback_blocks']}")
+        f"{res['cycles_per_step']:.2f} V-cycles/step, {res['sorts_in_window']} sorts in window, "
+        f"tile fallbacks {res['tile_fallback_blocks']}")
     return res
+
+
+def fe_rel_err(h_mixed, h_fp64):
+    """max_k |W_k(mixed) - W_k(FP64)| / |W_k(FP64)| over the steps both
+    histories hold (W = field energy), with the step count."""
+    k = min(len(h_mixed), len(h_fp64))
+    err = np.abs(h_mixed[:k, 0] - h_fp64[:k, 0]) / np.abs(h_fp64[:k, 0])
+    return {"max": float(err.max()), "steps": int(k)}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "push_deposit_two_stream_1024_hbm_traffic.md")
@@ -370,69 +410,102 @@ def run_mesh(E, n, steps):
     return out
 
 
-def run_c3(E, cells, ppc, steps, depth, pre, post):
-    """3D bounded box (phi = 0 on the faces, absorbing walls, compaction by
-    the sort), electrons + ions (mass ratio 1836, Te/Ti = 10), FP32
-    particles with FP32 V-cycles in FP64 refinement to 1e-8, 7-point MG."""
-    import torch
+def c3_workload(cells, ppc):
     from psum_amd.workloads import bounded_3d
+    return bounded_3d(cells=cells, ppc=ppc, device="cuda")
+
+
+def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_probe=20):
+    """3D bounded box (phi = 0 on the faces, absorbing walls, compaction by
+    the sort), electrons + ions (mass ratio 1836, Te/Ti = 10), 7-point MG.
+    dtype "f32": FP32 particles with FP32 V-cycles in FP64 refinement to
+    1e-8; "f64": the plain FP64 path on the same inputs.  schedule = (sort
+    every, sort shift) or None to run the autotuner first."""
+    import torch
     t0 = time.time()
-    wl = bounded_3d(cells=cells, ppc=ppc, device="cuda")
-    n = sum(len(s["pos"][0]) for s in wl["species"])
-    st = E.Stepper(wl["grid"], wl["species"], dtype="f32", mixed_mg=True, depth=depth, pre=pre,
-                   post=post, max_cycles=40, tol=1e-8, dt=wl["dt"], rho_bg=0.0, sort_every=20,
-                   fixed=wl["fixed"])
-    for s in wl["species"]:
-        s["pos"] = s["vel"] = None
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    species = [dict(s, pos=[a.to(tdt) for a in s["pos"]], vel=[a.to(tdt) for a in s["vel"]])
+               for s in wl["species"]]
+    n = sum(len(s["pos"][0]) for s in species)
+    st = E.Stepper(wl["grid"], species, dtype=dtype, mixed_mg=(dtype == "f32"), depth=depth,
+                   pre=pre, post=post, max_cycles=40, tol=1e-8, dt=wl["dt"], rho_bg=0.0,
+                   sort_every=20, fixed=wl["fixed"])
+    del species
     torch.cuda.empty_cache()
-    log(f"[bench] c3: {n} particles, set up in {time.time() - t0:.1f}s")
-    # the autotuner picks the sort interval and bin size; the solve stays
-    # tolerance-controlled
-    t0 = time.time()
-    tun = st.tune(warmup=3, probe=20)
-    st.set_schedule(0, tun["sort_every"], tun["sort_shift"])
-    log(f"[bench] c3: tuned in {time.time() - t0:.1f}s: sort every {tun['sort_every']} steps on "
-        f"{2 ** tun['sort_shift']}^3-cell bins")
-    st.run(2, history=False)
-    torch.cuda.synchronize()
-    st.set_timing(True)
-    t0 = time.perf_counter()
-    hist = st.run(steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    push_ms, launches, pushed = st.push_timing()
-    stats = st.stats()
-    out = {"workload": f"3D bounded box {cells}^3 cells, {ppc} e- + {ppc} ions per cell ({n} "
-                       f"particles), grounded walls, absorbing boundary + compaction, FP32 + FP64 "
-                       f"MG refinement to 1e-8, {steps} steps",
-           "particles": n, "steps": steps, "ms_per_step": 1e3 * wall / steps,
-           "particle_steps_per_s": n * steps / wall, "push_ms_avg": push_ms / max(1, launches),
-           "absorbed": stats["absorbed"], "field_energy_last": float(hist[-1, 0]),
-           "sort_every": tun["sort_every"], "sort_shift": tun["sort_shift"],
-           "tile_fallback_blocks": stats["tile_fallback_blocks"]}
-    log(f"[bench] c3: {out['ms_per_step']:.2f} ms/step, {out['particle_steps_per_s']:.3e} "
-        f"particle-steps/s, absorbed {stats['absorbed']}")
+    log(f"[bench] c3 {dtype}: {n} particles, set up in {time.time() - t0:.1f}s")
+    tun = None
+    if schedule is None:
+        t0 = time.time()
+        tun = st.tune(warmup=3, probe=tune_probe)
+        schedule = (tun["sort_every"], tun["sort_shift"])
+        log(f"[bench] c3: tuned in {time.time() - t0:.1f}s: sort every {schedule[0]} steps on "
+            f"{2 ** schedule[1]}^3-cell bins")
+    st.set_schedule(0, schedule[0], schedule[1])
+    if warmup:
+        st.run(warmup, history=False)
+    res = timed_run(st, n, steps, schedule[0])
+    res["particles"] = n
+    res["schedule"] = {"sort_every": schedule[0], "sort_shift": schedule[1]}
+    res["field_energy_last"] = float(res["hist"][-1, 0])
     st.close()
-    return out
+    torch.cuda.empty_cache()
+    log(f"[bench] c3 {dtype}: {res['ms_per_step']:.2f} ms/step, {res['particle_steps_per_s']:.3e} "
+        f"particle-steps/s, push {res['push_ms_avg']:.3f} ms, {res['cycles_per_step']:.2f} "
+        f"V-cycles/step, {res['sorts_in_window']} sorts in window, absorbed {res['absorbed']}, "
+        f"tile fallbacks {res['tile_fallback_blocks']}")
+    return res, schedule
+
+
+def cpu_baseline_c3(cells, ppc, steps):
+    """The CPU oracle (FP64 restatement of the reference step, OpenMP over the
+    host cores) on a bounded sample of the 3D bounded-box workload: the same
+    particles per cell, species, temperatures, walls and solver on a smaller
+    box, `steps` steps, reported per particle-step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from psum_amd.workloads import bounded_3d
+    wl = bounded_3d(cells=cells, ppc=ppc, dtype=np.float64)
+    og = O.wall_grid([cells] * 3)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    species = [dict(q=s["q"], m=s["m"], w=s["w"], pos=list(s["pos"]), vel=list(s["vel"]))
+               for s in wl["species"]]
+    n = sum(len(s["pos"][0]) for s in species)
+    t0 = time.perf_counter()
+    O.pic_run_multi(og, species, 0.0, 1.0, wl["dt"], steps, 5, pre=3, post=3, tol=1e-8,
+                    max_cycles=40, fixed=wl["fixed"])
+    t = max(1e-9, time.perf_counter() - t0)
+    log(f"[bench] cpu oracle: {cells}^3 box, {n} particles, {steps} steps in {t:.1f}s "
+        f"(MG set-up inside), {cores} threads")
+    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of the 3D bounded-box workload on a {cells}^3-cell box "
+                      f"({ppc} e- + {ppc} ions per cell, {n} particles; the headline box is 256^3), "
+                      f"FP64 oracle restatement of the reference step, OpenMP, MG set-up included"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline (3D box) run")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--grid", type=int, default=1024)
-    ap.add_argument("--ppc", type=int, default=100)
-    ap.add_argument("--depth", type=int, default=5)
     ap.add_argument("--pre", type=int, default=3)
     ap.add_argument("--post", type=int, default=3)
+    ap.add_argument("--c3-cells", type=int, default=256)
+    ap.add_argument("--c3-ppc", type=int, default=32)
+    ap.add_argument("--c3-depth", type=int, default=6)
+    ap.add_argument("--c3-no-fp64", action="store_true")
+    ap.add_argument("--grid", type=int, default=1024, help="2D two-stream grid")
+    ap.add_argument("--ppc", type=int, default=100)
+    ap.add_argument("--depth", type=int, default=5)
+    ap.add_argument("--c2-steps", type=int, default=100)
+    ap.add_argument("--sort-every", type=int, default=25, help="2D two-stream sort interval")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--c1-steps", type=int, default=100)
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--extrapolate", action="store_true")
-    ap.add_argument("--sort-every", type=int, default=25)
-    ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-cells", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-grid", type=int, default=256)
     ap.add_argument("--adjoint-ppc", type=int, default=64)
@@ -448,14 +521,10 @@ def main():
     ap.add_argument("--mesh-particles", type=int, default=20_000_000)
     ap.add_argument("--mesh-steps", type=int, default=20)
     ap.add_argument("--no-mesh", action="store_true")
-    ap.add_argument("--c3", action="store_true", help="also run the 3D bounded box (256^3, 1.07e9 particles)")
-    ap.add_argument("--c3-cells", type=int, default=256)
-    ap.add_argument("--c3-ppc", type=int, default=32)
-    ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--calib-copy", action="store_true",
-                    help="after the runs, copy one buffer of the push kernel's algorithmic size "
-                         "(FP32 x, y, vx, vy of every particle) so that HBM counters can be "
-                         "scaled by a plain copy of known size")
+                    help="after the headline run, copy one buffer of the 3D push kernel's "
+                         "algorithmic size so that HBM counters can be scaled by a plain copy "
+                         "of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported",
@@ -464,50 +533,83 @@ def main():
 
     import torch
     from psum_amd import engine as E
-    from psum_amd.workloads import two_stream_2d
+    from psum_amd.workloads import langmuir_2d, two_stream_2d
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+
+    # ---- headline: 3D bounded box, mixed precision, and the FP64 path on the same inputs
     t0 = time.time()
-    wl = two_stream_2d(n=args.grid, ppc=args.ppc, dtype=np.float32)
-    n = len(wl["species"][0]["pos"][0])
-    log(f"[bench] workload {wl['name']}: {n} particles, generated in {time.time() - t0:.1f}s")
-
-    mixed = run_engine(E, wl, "f32", True, args.steps, args.warmup, 1e-8, args.depth,
-                       args.sort_every, args.pre, args.post, not args.no_graph, args.extrapolate)
-    fp64 = None
-    if not args.no_fp64:
-        wl64 = dict(wl)
-        wl64["species"] = [dict(s, pos=[a.astype(np.float64) for a in s["pos"]],
-                                vel=[a.astype(np.float64) for a in s["vel"]]) for s in wl["species"]]
-        fp64 = run_engine(E, wl64, "f64", False, args.fp64_steps or args.steps, args.warmup, 1e-8,
-                          args.depth, args.sort_every, args.pre, args.post, not args.no_graph,
-                          args.extrapolate)
-        del wl64
-
-    # dominant kernel: fused gather + push + deposit; algorithmic bytes per
-    # launch = N_p * 2 * (2D) * 4 B (read + write FP32 position and velocity)
-    dim = wl["grid"].dim
-    bytes_per_launch = mixed["particles_per_launch"] * 2 * (2 * dim) * 4
-    achieved = bytes_per_launch / (mixed["push_ms_avg"] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": profiled_traffic("k_push_deposit<float,2,true,true>", args.grid, args.ppc),
-                "kernel": "k_push_deposit<float,2,true,true>",
-                "bytes_per_launch": bytes_per_launch, "avg_launch_ms": mixed["push_ms_avg"]}
+    wl3 = c3_workload(args.c3_cells, args.c3_ppc)
+    log(f"[bench] c3 workload {wl3['name']}: generated in {time.time() - t0:.1f}s")
+    c3, sched = run_c3(E, wl3, "f32", args.steps, args.warmup, args.c3_depth, args.pre, args.post)
     if args.calib_copy:
-        src = torch.ones(int(mixed["particles_per_launch"]) * 2 * dim, dtype=torch.float32,
-                         device="cuda")
+        nb = int(c3["particles_per_launch"]) * 2 * 3
+        src = torch.ones(nb, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f"[bench] calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} written")
         del src, dst
+    c3_64 = None
+    if not args.c3_no_fp64:
+        c3_64, _ = run_c3(E, wl3, "f64", args.steps, args.warmup, args.c3_depth, args.pre,
+                          args.post, schedule=sched)
+    del wl3
+    torch.cuda.empty_cache()
+
+    # dominant kernel: the fused 3D gather + push + deposit; algorithmic bytes
+    # per launch = particles per launch * 2 * (2D) * 4 B (read + write FP32
+    # position and velocity once)
+    kern = "k_push_deposit<float,3,true,true>"
+    bytes_per_launch = c3["particles_per_launch"] * 2 * (2 * 3) * 4
+    achieved = bytes_per_launch / (c3["push_ms_avg"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": profiled_traffic(kern, args.c3_cells, args.c3_ppc),
+                "kernel": kern, "bytes_per_launch": bytes_per_launch,
+                "avg_launch_ms": c3["push_ms_avg"]}
+
+    # ---- 2D two-stream (mixed and FP64) and the plain FP64 Langmuir case
+    c2 = None
+    if not args.no_c2:
+        t0 = time.time()
+        wl = two_stream_2d(n=args.grid, ppc=args.ppc, dtype=np.float32)
+        log(f"[bench] c2 workload {wl['name']}: generated in {time.time() - t0:.1f}s")
+        m2 = run_engine(E, wl, "f32", True, args.c2_steps, args.warmup, 1e-8, args.depth,
+                        args.sort_every, args.pre, args.post, not args.no_graph, args.extrapolate,
+                        label="c2")
+        wl64 = dict(wl)
+        wl64["species"] = [dict(s, pos=[a.astype(np.float64) for a in s["pos"]],
+                                vel=[a.astype(np.float64) for a in s["vel"]]) for s in wl["species"]]
+        f2 = run_engine(E, wl64, "f64", False, args.c2_steps, args.warmup, 1e-8, args.depth,
+                        args.sort_every, args.pre, args.post, not args.no_graph, args.extrapolate,
+                        label="c2")
+        del wl, wl64
+        c2 = {"workload": f"2D two-stream, {args.grid}x{args.grid} periodic nodes, {args.ppc} e-/cell, "
+                          f"dt 0.05; mixed: FP32 state, FP32 V-cycles in FP64 refinement to 1e-8",
+              "particles": args.grid * args.grid * args.ppc, "steps": args.c2_steps,
+              "mixed": _leg(m2), "fp64": _leg(f2),
+              "mixed_over_fp64": m2["particle_steps_per_s"] / f2["particle_steps_per_s"],
+              "field_energy_rel_err_mixed_vs_fp64": fe_rel_err(m2["hist"], f2["hist"]),
+              "push_GBps": m2["particles_per_launch"] * 32 / (m2["push_ms_avg"] * 1e-3) / 1e9}
+    c1 = None
+    if not args.no_c1:
+        wl = langmuir_2d(n=128, ppc=64, seed=1)
+        f1 = run_engine(E, wl, "f64", False, args.c1_steps, args.warmup, 1e-10, 4, 0, 2, 1,
+                        not args.no_graph, label="c1")
+        wl32 = dict(wl)
+        wl32["species"] = [dict(s, pos=[a.astype(np.float32) for a in s["pos"]],
+                                vel=[a.astype(np.float32) for a in s["vel"]]) for s in wl["species"]]
+        m1 = run_engine(E, wl32, "f32", True, args.c1_steps, args.warmup, 1e-10, 4, 0, 2, 1,
+                        not args.no_graph, label="c1")
+        c1 = {"workload": "2D Langmuir, 128x128 periodic nodes, 64 e-/cell (1048576 particles), "
+                          "density 1 + 0.01 cos(2 pi x/L), dt 0.1, MG to 1e-10",
+              "steps": args.c1_steps, "fp64": _leg(f1), "mixed": _leg(m1),
+              "mixed_over_fp64": m1["particle_steps_per_s"] / f1["particle_steps_per_s"],
+              "field_energy_rel_err_mixed_vs_fp64": fe_rel_err(m1["hist"], f1["hist"])}
     adjoint = None
     if not args.no_adjoint:
         adjoint = run_adjoint(E, args.adjoint_grid, args.adjoint_ppc, args.adjoint_steps, 20,
                               args.depth, args.pre, args.post)
-    c3 = None
-    if args.c3:
-        c3 = run_c3(E, args.c3_cells, args.c3_ppc, args.c3_steps, 6, args.pre, args.post)
     batch = None
     if not args.no_batch:
         batch = run_batch(E, args.batch_instances, args.batch_steps, args.batch_lanes)
@@ -519,49 +621,54 @@ def main():
         mesh = run_mesh(E, args.mesh_particles, args.mesh_steps)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(wl, args.cpu_steps)
+        cpu = cpu_baseline_c3(args.cpu_cells, args.c3_ppc, args.cpu_steps)
 
-    fe_err = None
-    if fp64 is not None:
-        h32, h64 = mixed["hist"], fp64["hist"]
-        k = min(len(h32), len(h64), 100)
-        fe_err = float(np.max(np.abs(h32[:k, 0] - h64[:k, 0]) / np.abs(h64[:k, 0])))
     out = {
         "metric": METRIC,
-        "value": mixed["particle_steps_per_s"],
+        "value": c3["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mixed["ms_per_step"],
+        "ms_per_step": c3["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"2D two-stream, {args.grid}x{args.grid} periodic nodes, "
-                               f"{args.ppc} e-/cell ({n} particles), dt 0.05, FP32 state, "
-                               f"FP32 V-cycles in FP64 refinement to 1e-8",
-                   "grid": [args.grid, args.grid], "particles": n, "mg_depth": args.depth,
-                   "mg_pre_post": [args.pre, args.post], "mg_graph": not args.no_graph,
-                   "extrapolate": args.extrapolate,
-                   "sort_every": args.sort_every, "parallelism": "single GPU"},
-        "fp64": None if fp64 is None else {
-            "value": fp64["particle_steps_per_s"], "ms_per_step": fp64["ms_per_step"],
-            "cycles_per_step": fp64["cycles_per_step"], "push_ms_avg": fp64["push_ms_avg"]},
-        "mixed_over_fp64": None if fp64 is None else
-            mixed["particle_steps_per_s"] / fp64["particle_steps_per_s"],
-        "field_energy_rel_err_first100": fe_err,
-        "mixed_cycles_per_step": mixed["cycles_per_step"],
+        "config": {"workload": f"3D bounded box, {args.c3_cells}^3 cells, {args.c3_ppc} e- + "
+                               f"{args.c3_ppc} ions per cell ({c3['particles']} particles, mass "
+                               f"ratio 1836, Te/Ti 10), grounded walls, absorbing boundary + "
+                               f"compaction, 7-point MG, FP32 state, FP32 V-cycles in FP64 "
+                               f"refinement to 1e-8, dt 0.1",
+                   "cells": [args.c3_cells] * 3, "particles": c3["particles"],
+                   "mg_depth": args.c3_depth, "mg_pre_post": [args.pre, args.post],
+                   "sort_every": sched[0], "sort_bin_cells": 2 ** sched[1],
+                   "sorts_in_window": c3["sorts_in_window"], "parallelism": "single GPU"},
+        "fp64": None if c3_64 is None else _leg(c3_64),
+        "mixed_over_fp64": None if c3_64 is None else
+            c3["particle_steps_per_s"] / c3_64["particle_steps_per_s"],
+        "field_energy_rel_err_mixed_vs_fp64": None if c3_64 is None else
+            fe_rel_err(c3["hist"], c3_64["hist"]),
+        "headline": _leg(c3),
+        "c2": c2,
+        "c1": c1,
         "adjoint": adjoint,
         "batch": batch,
-        "c3": c3,
         "mcc": mcc,
         "mesh": mesh,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def _leg(r):
+    """The JSON fields of one timed run."""
+    keys = ("ms_per_step", "particle_steps_per_s", "push_ms_avg", "push_launches",
+            "particles_per_launch", "cycles_per_step", "last_rel_res", "tile_fallback_blocks",
+            "absorbed", "sort_every", "sorts_in_window", "align_steps")
+    return {k: r[k] for k in keys if k in r}
 
 
 if __name__ == "__main__":

@@ -360,8 +360,17 @@ psum_status psum_stepper_state(void* h, int s, void** pos, void** vel, int64_t* 
 psum_status psum_stepper_run(void* h, int steps, double* hist);
 /* copy phi (FP64, host) */
 psum_status psum_stepper_phi(void* h, double* phi_host);
+/* copy rho of the current positions (FP64, host, may be NULL): the charge
+ * density the next step's solve starts from, acc * 2^-S + rho_bg before
+ * neutralisation and fixed-node values, as add_back leaves it
+ * (device_field.hpp:295-310); *scale (may be NULL) = 2^S of the int64
+ * fixed-point accumulator */
+psum_status psum_stepper_rho(void* h, double* rho_host, double* scale);
 /* total V-cycles so far, last relative residual */
 psum_status psum_stepper_stats(void* h, int64_t* cycles, double* last_rel_res, int64_t* counters);
+/* steps taken since creation (a sort runs before step k when
+ * k % sort_every == 0); restore() sets it from the checkpoint */
+psum_status psum_stepper_step_count(void* h, int64_t* steps);
 psum_status psum_stepper_sync(void* h);
 /* HIP-event timing of the fused gather/push/deposit launches on the engine's
  * stream (reset by set_timing): total ms, launches, particles pushed. */
@@ -401,6 +410,14 @@ psum_status psum_stepper_sort(void* h);
  * psum_hip_gather_push_bspline / psum_hip_deposit_bspline.  The adjoint
  * entry points (record / reverse / adjoint) accept order 1 only. */
 psum_status psum_stepper_set_interp_order(void* h, int order);
+
+/* Multigrid kernel-shape options that may change after creation, as
+ * whitespace "key value" pairs: "fuse 0|1" (2D fused tile smoothing and
+ * residual + restriction, default 1) and "tail 0|1" (the coarse end of each
+ * 2D V-cycle in one workgroup, default 0).  Results are bit-identical
+ * either way.  A batch instance's change makes the batch capture its step
+ * graph again at its next run.  No reference counterpart (tuning knobs). */
+psum_status psum_stepper_mg_tuning(void* h, const char* options);
 
 /* Checkpoint files in the reference's MAS format (src/serialization/mas.hpp:
  * 49-453; particle_group / device_field save(), particle_group.hpp:272-275,

@@ -277,6 +277,13 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
                           const uint8_t* fixed, const double* fixed_val, double* hist,
                           double* phi_out);
 
+/* Test hooks of the step (see step.c): planted mistakes (1 = charge sign of
+ * the deposit, 2 = sign of rho_bg, 3 = push with the previous step's phi,
+ * 4 = no periodic neutralisation; 0 = none), and a probe recording the
+ * charge-weighted Fourier sums of the positions every step sees. */
+void orc_pic_plant(int which);
+void orc_pic_set_probe(int nk, const double* kvec, double* out);
+
 /* ---- derivatives (no reference counterpart: the reference has no adjoint) */
 #define ORC_MAX_SPECIES 4
 

@@ -18,6 +18,51 @@
 static int g_order = 1;
 void orc_pic_set_order(int order) { g_order = order; }
 
+/* planted mistakes for the whole-step pin tests (0 = none): 1 = the deposit
+ * takes the charge with the wrong sign, 2 = the background charge density
+ * enters with the wrong sign, 3 = the push gathers the potential of the
+ * previous step, 4 = the periodic neutralisation (mean subtraction) is
+ * dropped. */
+static int g_step_plant = 0;
+void orc_pic_plant(int which) { g_step_plant = which; }
+
+/* Test probe: when set, every step records the charge-weighted Fourier sums
+ * sum_s q_s w_s sum_p (cos(k.x_p), sin(k.x_p)) of the positions the step's
+ * solve sees, for nk wave vectors kvec[j*D..j*D+D-1], into
+ * out[(step*nk + j)*2 + {0,1}]. */
+static int g_probe_nk = 0;
+static const double* g_probe_k = NULL;
+static double* g_probe_out = NULL;
+void orc_pic_set_probe(int nk, const double* kvec, double* out)
+{
+    g_probe_nk = (kvec && out) ? nk : 0;
+    g_probe_k = kvec;
+    g_probe_out = out;
+}
+
+static void probe_record(int D, int nspecies, double*** pos, const int64_t* n, const double* q,
+                         const double* w, int st)
+{
+    for (int j = 0; j < g_probe_nk; ++j) {
+        const double* k = g_probe_k + (size_t)j * D;
+        double c = 0.0, s = 0.0;
+        for (int sp = 0; sp < nspecies; ++sp) {
+            double cs = 0.0, ss = 0.0;
+            for (int64_t p = 0; p < n[sp]; ++p) {
+                double ph = 0.0;
+                for (int d = 0; d < D; ++d) ph += k[d] * pos[sp][d][p];
+                if (!isfinite(ph)) continue;
+                cs += cos(ph);
+                ss += sin(ph);
+            }
+            c += q[sp] * w[sp] * cs;
+            s += q[sp] * w[sp] * ss;
+        }
+        g_probe_out[((size_t)st * g_probe_nk + j) * 2 + 0] = c;
+        g_probe_out[((size_t)st * g_probe_nk + j) * 2 + 1] = s;
+    }
+}
+
 static int fx_shift(double w)
 {
     int e = 0;
@@ -54,7 +99,7 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
     for (int d = 0; d < D; ++d) dV *= g->delta[d];
     double wdep[8], wmax = 0.0;
     for (int s = 0; s < nspecies; ++s) {
-        wdep[s] = q[s] * w[s] / dV;
+        wdep[s] = (g_step_plant == 1 ? -q[s] : q[s]) * w[s] / dV;
         if (fabs(wdep[s]) > wmax) wmax = fabs(wdep[s]);
     }
     const double scale = ldexp(1.0, fx_shift(wmax > 0 ? wmax : 1.0));
@@ -62,21 +107,24 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
     int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * (size_t)nn);
     double* b = (double*)malloc(sizeof(double) * (size_t)nn);
     double* phi = (double*)calloc((size_t)nn, sizeof(double));
+    double* phi_prev = g_step_plant == 3 ? (double*)calloc((size_t)nn, sizeof(double)) : NULL;
+    const double bg = g_step_plant == 2 ? -rho_bg : rho_bg;
     int64_t total_cycles = 0;
     for (int st = 0; st < steps; ++st) {
+        if (g_probe_nk) probe_record(D, nspecies, pos, n, q, w, st);
         if (dep_mode == 0) {
             memset(acc, 0, sizeof(int64_t) * (size_t)nn);
             for (int s = 0; s < nspecies; ++s)
                 if (g_order > 1) orc_deposit_bs_f64(g, g_order, (const double* const*)pos[s], n[s], wdep[s], scale, acc);
                 else orc_deposit_fx_f64(g, (const double* const*)pos[s], n[s], wdep[s], scale, acc);
-            for (int64_t i = 0; i < nn; ++i) b[i] = (double)acc[i] * (1.0 / scale) + rho_bg;
+            for (int64_t i = 0; i < nn; ++i) b[i] = (double)acc[i] * (1.0 / scale) + bg;
         } else {
             memset(b, 0, sizeof(double) * (size_t)nn);
             for (int s = 0; s < nspecies; ++s)
                 orc_deposit_ref_f64(g, (const double* const*)pos[s], n[s], wdep[s], b);
-            for (int64_t i = 0; i < nn; ++i) b[i] = b[i] + rho_bg;
+            for (int64_t i = 0; i < nn; ++i) b[i] = b[i] + bg;
         }
-        if (all_periodic) {
+        if (all_periodic && g_step_plant != 4) {
             double sum = 0.0;
             for (int64_t i = 0; i < nn; ++i) sum += b[i];
             const double mean = sum / (double)nn;
@@ -94,12 +142,14 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
         }
         const double fe = orc_field_energy(g, phi, eps);
         double tot[4] = {0, 0, 0, 0};
+        const double* phi_push = phi_prev ? phi_prev : phi;
         for (int s = 0; s < nspecies; ++s) {
             double diag[4];
-            if (g_order > 1) orc_gather_push_bs_f64(g, g_order, pos[s], vel[s], n[s], phi, q[s] / m[s], dt, m[s] * w[s], diag);
-            else orc_gather_push_f64(g, pos[s], vel[s], n[s], phi, q[s] / m[s], dt, m[s] * w[s], diag);
+            if (g_order > 1) orc_gather_push_bs_f64(g, g_order, pos[s], vel[s], n[s], phi_push, q[s] / m[s], dt, m[s] * w[s], diag);
+            else orc_gather_push_f64(g, pos[s], vel[s], n[s], phi_push, q[s] / m[s], dt, m[s] * w[s], diag);
             for (int k = 0; k < 4; ++k) tot[k] += diag[k];
         }
+        if (phi_prev) memcpy(phi_prev, phi, sizeof(double) * (size_t)nn);
         if (hist) {
             hist[5 * st + 0] = fe;
             for (int k = 0; k < 4; ++k) hist[5 * st + 1 + k] = tot[k];
@@ -107,7 +157,7 @@ int64_t orc_pic_run_multi(const orc_grid* g, int nspecies, double*** pos, double
     }
     if (phi_out) memcpy(phi_out, phi, sizeof(double) * (size_t)nn);
     orc_mg_free(mg);
-    free(acc); free(b); free(phi); free(fx);
+    free(acc); free(b); free(phi); free(fx); free(phi_prev);
     return total_cycles;
 }
 

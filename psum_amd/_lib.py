@@ -199,7 +199,9 @@ SIGNATURES = [
     ("psum_stepper_state", ST, [P, C.c_int, P, P, C.POINTER(I64)]),
     ("psum_stepper_run", ST, [P, C.c_int, P]),
     ("psum_stepper_phi", ST, [P, P]),
+    ("psum_stepper_rho", ST, [P, P, C.POINTER(C.c_double)]),
     ("psum_stepper_stats", ST, [P, C.POINTER(I64), C.POINTER(C.c_double), P]),
+    ("psum_stepper_step_count", ST, [P, C.POINTER(I64)]),
     ("psum_stepper_sync", ST, [P]),
     ("psum_stepper_set_timing", ST, [P, C.c_int]),
     ("psum_stepper_push_timing", ST, [P, C.POINTER(C.c_double), C.POINTER(I64), C.POINTER(I64)]),
@@ -215,6 +217,7 @@ SIGNATURES = [
     ("psum_stepper_save", ST, [P, C.c_char_p]),
     ("psum_stepper_restore", ST, [P, C.c_char_p]),
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
+    ("psum_stepper_mg_tuning", ST, [P, C.c_char_p]),
     ("psum_stepper_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
     ("psum_poisson_assemble", ST, [_GP, C.c_double, P, I64, P, P, P, C.POINTER(I64)]),
     ("psum_poisson_assemble_ex", ST, [_GP, C.POINTER(PsumPoissonSpec), I64, P, P, P, C.POINTER(I64),

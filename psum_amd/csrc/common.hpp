@@ -4,7 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <string>
+#include <system_error>
+#include <thread>
+#include <vector>
 
 #include "psum_hip.h"
 
@@ -36,6 +40,62 @@ uint32_t bin_count(const psum_grid& g, int shift);
             return PSUM_ERR_ARG;                                                        \
         }                                                                               \
     } while (0)
+
+// Runs body(t) for t = 0..nt-1 on host threads.  A thread that cannot be
+// created (std::system_error, e.g. under a thread limit) has its share run
+// on the calling thread instead; the first exception a body throws is
+// rethrown after every started thread has been joined.
+template <typename F>
+void host_threads(unsigned nt, F&& body)
+{
+    std::vector<std::thread> pool;
+    std::vector<std::exception_ptr> err(nt);
+    std::vector<unsigned> inline_ids;
+    pool.reserve(nt);
+    inline_ids.reserve(nt);
+    for (unsigned t = 0; t < nt; ++t) {
+        auto run = [t, &body, &err]() {
+            try {
+                body(t);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        };
+        try {
+            pool.emplace_back(run);
+        } catch (const std::system_error&) {
+            inline_ids.push_back(t);
+        }
+    }
+    for (unsigned t : inline_ids) {
+        try {
+            body(t);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    }
+    for (auto& th : pool) th.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
+// Ends a stream capture that is still open when the scope is left (an
+// exception between Begin and EndCapture) and drops the partial graph, so
+// the stream is usable again.  release() after a successful EndCapture.
+struct CaptureGuard {
+    hipStream_t s;
+    bool open = true;
+    explicit CaptureGuard(hipStream_t st) : s(st) {}
+    void release() { open = false; }
+    ~CaptureGuard()
+    {
+        if (!open) return;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+    }
+};
 
 }  // namespace psum_hip
 

@@ -162,12 +162,9 @@ template <typename F>
 void host_parallel_for(int n, F&& f)
 {
     const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t)
-        pool.emplace_back([&, t] {
-            for (int i = t; i < n; i += nt) f(i);
-        });
-    for (auto& th : pool) th.join();
+    psum_hip::host_threads((unsigned)nt, [&](unsigned t) {
+        for (int i = (int)t; i < n; i += nt) f(i);
+    });
 }
 
 void compute_udf(int I, int J, int K, const std::vector<uint32_t>& start, std::vector<float>& udf, int udf_k = 6,

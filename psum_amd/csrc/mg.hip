@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -62,16 +63,11 @@ void parallel_for(int64_t n, F&& f)
         for (int64_t i = 0; i < n; ++i) f(i);
         return;
     }
-    std::vector<std::thread> th;
     const int64_t chunk = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
+    host_threads(nt, [&](unsigned t) {
         const int64_t b = (int64_t)t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        th.emplace_back([b, e, &f]() {
-            for (int64_t i = b; i < e; ++i) f(i);
-        });
-    }
-    for (auto& t : th) t.join();
+        for (int64_t i = b; i < e; ++i) f(i);
+    });
 }
 
 // ------------------------------------------------------------ device data
@@ -1487,6 +1483,33 @@ struct DevTransferBuf {
 
 namespace psum_hip {
 
+// k_vtail<T>'s dynamic LDS limit on the current device: raised once per
+// device and precision to the device's per-workgroup maximum (the
+// attribute is process-wide, so every solver shares the one largest
+// value), and returned.  Called when a hierarchy is built, outside any
+// stream capture.
+template <typename T>
+int vtail_lds_limit()
+{
+    static std::mutex mu;
+    static std::map<int, int> limit;
+    int dev = 0;
+    MG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = limit.find(dev);
+    if (it != limit.end()) return it->second;
+    int lim = 0;
+    MG_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (lim > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vtail<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
+        (void)hipGetLastError();
+        lim = 65536;
+    }
+    limit[dev] = lim;
+    return lim;
+}
+
 class HipMultigrid {
 public:
     explicit HipMultigrid(bool mixed) : mixed_(mixed) {}
@@ -1512,8 +1535,8 @@ public:
                 corr_ = v;
             } else if (key == "tol") iss >> tol_;
             else if (key == "periodic_mode") iss >> periodic_mode_;
-            else if (key == "fuse") iss >> fuse_;
-            else if (key == "tail") { iss >> tail_; tail32_ = tail64_ = -2; }
+            else if (key == "fuse") { iss >> fuse_; resolve_tail(); }
+            else if (key == "tail") { iss >> tail_; resolve_tail(); }
             else if (key == "graph") iss >> use_graph_;
             else if (key == "num_conv_mask" || key == "num_conv_mask_relax" ||
                      key == "num_conv_mask_restrict" || key == "num_conv_mask_interpolate" ||
@@ -1523,6 +1546,24 @@ public:
             }
         }
         if (!k_provided) K_ = 1;
+    }
+
+    // Kernel-shape options that may change after set_matrix: "fuse 0|1"
+    // (2D fused tile kernels) and "tail 0|1" (the one-workgroup coarse end).
+    // Other keys are rejected.
+    void set_tuning(const char* options)
+    {
+        std::istringstream iss(options ? options : "");
+        std::string key;
+        while (iss >> key) {
+            int v = 0;
+            if (!(iss >> v)) throw std::invalid_argument("hip_multigrid: tuning option " + key + " needs a value");
+            if (key == "tail") tail_ = v;
+            else if (key == "fuse") fuse_ = v;
+            else throw std::invalid_argument("hip_multigrid: unknown tuning option " + key);
+        }
+        rr_fit_.clear();
+        resolve_tail();
     }
 
     void set_stream(hipStream_t s) { stream_ = s; }
@@ -1585,7 +1626,7 @@ public:
         tr_host_ = std::move(T);
         rr_fit_.clear();
         upload();
-        tail32_ = tail64_ = -2;
+        resolve_tail();
     }
 
     void set_source_replace(unsigned long long n, const unsigned long long* idx, const double* v)
@@ -2000,36 +2041,53 @@ private:
     // lt >= 1, at most kTailLevelNodes nodes each) keep their iterates and
     // right-hand sides in LDS and whose coarsest level has at most kTailCoarse
     // nodes (its dense solve reads the inverse from one compute unit).
+    // First level of the one-workgroup tail for precision T, or -1: a 2D
+    // fused hierarchy whose levels lt..Lc (two or more, at most kMaxTail,
+    // lt >= 1, at most kTailLevelNodes nodes each) keep their iterates and
+    // right-hand sides in LDS and whose coarsest level has at most kTailCoarse
+    // nodes (its dense solve reads the inverse from one compute unit).
+    // Chosen by resolve_tail() when the hierarchy is built or the option
+    // changes, never while a stream is being captured.
     template <typename T>
-    int tail_start()
+    int tail_start() const
+    {
+        const int c = std::is_same<T, double>::value ? tail64_ : tail32_;
+        if (c == -2) throw std::logic_error("hip_multigrid: V-cycle tail used before it was resolved");
+        return c;
+    }
+
+    template <typename T>
+    void resolve_tail_for()
     {
         int& c = std::is_same<T, double>::value ? tail64_ : tail32_;
         size_t& bytes = std::is_same<T, double>::value ? tail_bytes64_ : tail_bytes32_;
-        if (c != -2) return c;
         c = -1;
+        bytes = 0;
         const int nl = (int)host_.size(), Lc = nl - 1;
-        if (!tail_ || !fuse_ || nl < 3 || host_[Lc].N > kTailCoarse) return c;
+        if (!tail_ || !fuse_ || nl < 3 || host_[Lc].N > kTailCoarse) return;
         for (const auto& h : host_)
-            if (h.D != 2) return c;
-        int dev = 0, lim = 0;
-        MG_HIP(hipGetDevice(&dev));
-        MG_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            if (h.D != 2) return;
+        const int lim = vtail_lds_limit<T>();
         for (int lt = 1; lt <= Lc - 1; ++lt) {
             if (Lc - lt + 1 > kMaxTail || host_[lt].N > kTailLevelNodes) continue;
             TailArgs<T> probe{};
             const size_t b = tail_layout<T>(lt, probe);
             if (b > (size_t)lim) continue;
-            if (b > 65536 &&
-                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vtail<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess) {
-                (void)hipGetLastError();
-                continue;
-            }
             c = lt;
             bytes = b;
             break;
         }
-        return c;
+    }
+
+    // Both precisions' tails for the current hierarchy and options.
+    void resolve_tail()
+    {
+        if (host_.empty()) {
+            tail32_ = tail64_ = -2;
+            return;
+        }
+        resolve_tail_for<float>();
+        resolve_tail_for<double>();
     }
 
     // Fills the tail arguments for levels lt..Lc; returns the LDS bytes
@@ -2115,7 +2173,11 @@ private:
         a.b_in = b;
         a.x_out = x;
         a.cinv = cinv_;
+        const size_t chosen = std::is_same<T, double>::value ? tail_bytes64_ : tail_bytes32_;
+        if (bytes != chosen || bytes > (size_t)vtail_lds_limit<T>())
+            throw std::logic_error("hip_multigrid: V-cycle tail LDS size differs from the one resolved");
         hipLaunchKernelGGL((k_vtail<T>), dim3(1), dim3(kTailThreads), bytes, stream_, a);
+        MG_HIP(hipGetLastError());
     }
 
     // One or more V-cycles on level-0 arrays (b0 scaled, x0 warm start).
@@ -2268,8 +2330,10 @@ private:
             gc.next = (gc.next + 1) % 4;
             drop_graph(*c);
             MG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            CaptureGuard guard(stream_);
             body();
             hipGraph_t graph = nullptr;
+            guard.release();
             MG_HIP(hipStreamEndCapture(stream_, &graph));
             MG_HIP(hipGraphInstantiate(&c->exec, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
@@ -2421,6 +2485,7 @@ namespace psum_hip {
 HipMultigrid* mg_create(bool mixed) { return new HipMultigrid(mixed); }
 void mg_destroy(HipMultigrid* h) { delete h; }
 void mg_set_options(HipMultigrid* h, const char* o) { h->set_options(o); }
+void mg_set_tuning(HipMultigrid* h, const char* o) { h->set_tuning(o); }
 void mg_set_matrix(HipMultigrid* h, unsigned long long n, unsigned long long nnz,
                    const unsigned long long* r, const unsigned long long* c, const double* v)
 {

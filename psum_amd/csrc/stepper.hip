@@ -29,6 +29,7 @@ class HipMultigrid;
 HipMultigrid* mg_create(bool mixed);
 void mg_destroy(HipMultigrid* h);
 void mg_set_options(HipMultigrid* h, const char* o);
+void mg_set_tuning(HipMultigrid* h, const char* o);
 void mg_set_matrix(HipMultigrid* h, unsigned long long n, unsigned long long nnz,
                    const unsigned long long* r, const unsigned long long* c, const double* v);
 void mg_set_replace(HipMultigrid* h, unsigned long long n, const unsigned long long* i, const double* v);
@@ -892,6 +893,13 @@ public:
         order_ = o;
         acc_valid_ = false;
     }
+    void mg_tuning(const char* o)
+    {
+        mg_set_tuning(mg_, o);
+        ++tuning_epoch_;
+    }
+    // changes with every mg_tuning call (a batch re-captures its graphs)
+    int64_t tuning_epoch() const { return tuning_epoch_; }
     void require_cic(const char* what) const
     {
         if (order_ != 1) throw std::invalid_argument(std::string(what) + ": the adjoint covers CIC (order 1) only");
@@ -1034,6 +1042,20 @@ public:
         S_TRY(hipStreamSynchronize(st_));
     }
 
+    // rho of the current positions (the deposit the next step solves with,
+    // before neutralisation and fixed-node values): acc * 2^-S + rho_bg.
+    // Also the deposit scale 2^S.
+    void rho(double* out, double* scale)
+    {
+        if (!acc_valid_) deposit_all();
+        if (out) {
+            CHK(psum_hip_fixed_to_double(acc_, nn_, 1.0 / scale_, cfg_.rho_bg, b_, st_));
+            S_TRY(hipMemcpyAsync(out, b_, nn_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+        }
+        if (scale) *scale = scale_;
+        S_TRY(hipStreamSynchronize(st_));
+    }
+
     void stats(int64_t* cycles, double* rel, int64_t* counters)
     {
         if (cycles) *cycles = cycles_;
@@ -1091,6 +1113,7 @@ private:
     bool all_periodic_ = true;
     hipStream_t st_ = nullptr;
     HipMultigrid* mg_ = nullptr;
+    int64_t tuning_epoch_ = 0;
     Species sp_[4];
     int64_t max_cap_ = 0;
     double scale_ = 1.0;
@@ -1272,7 +1295,13 @@ public:
             if (p->fixed_cycles() < 1)
                 throw std::invalid_argument("psum_batch_run: every instance needs a fixed cycle count "
                                             "(psum_batch_configure or psum_batch_tune)");
-        if (!captured_) capture();
+        int64_t epoch = 0;
+        for (auto& p : inst_) epoch += p->tuning_epoch();
+        if (captured_ && epoch != captured_epoch_) drop();
+        if (!captured_) {
+            capture();
+            captured_epoch_ = epoch;
+        }
         // instances reloaded since the capture need their deposit redone
         for (int i = 0; i < size(); ++i)
             if (!inst_[i]->acc_valid()) {
@@ -1344,6 +1373,7 @@ private:
     std::vector<std::vector<int>> lane_inst_;
     std::vector<hipGraphExec_t> lane_exec_;
     bool captured_ = false;
+    int64_t captured_epoch_ = 0;     // sum of the instances' tuning epochs at capture
 
     void drop()
     {
@@ -1402,6 +1432,8 @@ private:
         for (size_t l = 0; l < lane_st_.size(); ++l) {
             hipStream_t ls = lane_st_[l];
             S_TRY(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+            // an exception below ends the capture and drops the partial graph
+            CaptureGuard guard(ls);
             S_TRY(hipEventRecord(lane_fork_[l], ls));
             for (int i : lane_inst_[l]) {
                 Stepper& st = *inst_[i];
@@ -1411,6 +1443,7 @@ private:
                 S_TRY(hipStreamWaitEvent(ls, join_[i], 0));
             }
             hipGraph_t graph = nullptr;
+            guard.release();
             S_TRY(hipStreamEndCapture(ls, &graph));
             const hipError_t e = hipGraphInstantiate(&lane_exec_[l], graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
@@ -1480,6 +1513,18 @@ psum_status psum_stepper_stats(void* h, int64_t* cycles, double* last_rel_res, i
 {
     if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
     return guarded([&]() { static_cast<Stepper*>(h)->stats(cycles, last_rel_res, counters); });
+}
+
+psum_status psum_stepper_rho(void* h, double* rho_host, double* scale)
+{
+    if (!h) { set_error("psum_stepper_rho: null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->rho(rho_host, scale); });
+}
+
+psum_status psum_stepper_step_count(void* h, int64_t* steps)
+{
+    if (!h || !steps) { set_error("psum_stepper_step_count: null argument"); return PSUM_ERR_ARG; }
+    return guarded([&]() { *steps = static_cast<Stepper*>(h)->steps_done(); });
 }
 
 psum_status psum_stepper_sync(void* h)
@@ -1552,6 +1597,12 @@ psum_status psum_stepper_set_interp_order(void* h, int order)
 {
     if (!h) { set_error("psum_stepper_set_interp_order: null stepper"); return PSUM_ERR_ARG; }
     return guarded([&]() { static_cast<Stepper*>(h)->set_interp_order(order); });
+}
+
+psum_status psum_stepper_mg_tuning(void* h, const char* options)
+{
+    if (!h || !options) { set_error("psum_stepper_mg_tuning: null argument"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->mg_tuning(options); });
 }
 
 psum_status psum_stepper_save(void* h, const char* path)

@@ -543,6 +543,11 @@ class Stepper:
         check(lib().psum_stepper_tune(self.h, warmup, probe, C.byref(t)), "psum_stepper_tune")
         return {f: getattr(t, f) for f, _ in PsumBatchTuning._fields_ if f != "pad_"}
 
+    def mg_tuning(self, options):
+        """Multigrid kernel-shape options ("fuse 0|1", "tail 0|1"); results
+        are bit-identical either way (psum_stepper_mg_tuning)."""
+        check(lib().psum_stepper_mg_tuning(self.h, options.encode()), "mg_tuning")
+
     def set_schedule(self, fixed_cycles=0, sort_every=0, sort_shift=0):
         """fixed_cycles > 0: that many V-cycles per solve, no tolerance check;
         sort every `sort_every` steps on 2^sort_shift-cell bins."""
@@ -551,7 +556,17 @@ class Stepper:
 
     def load(self, s, pos, vel):
         """Replace species s with pos / vel: numpy arrays, or device tensors of
-        the stepper's dtype (torch CUDA tensors, copied device to device)."""
+        the stepper's dtype (torch CUDA tensors, copied device to device).
+        The copy runs on the stepper's own stream, so torch's current stream
+        is synchronised first: work queued there that writes the tensors
+        (a displacement, a random draw) is complete before the copy reads
+        them.  The call returns after the copy."""
+        tensors = [a for a in list(pos) + list(vel) if hasattr(a, "data_ptr")]
+        if tensors:
+            import torch
+            for dev in {t.device for t in tensors if t.is_cuda}:
+                torch.cuda.current_stream(dev).synchronize()
+
         def addr(a):
             if hasattr(a, "data_ptr"):
                 if a.dtype != _torch_dtype(self.np_dtype) or not a.is_contiguous():
@@ -621,6 +636,14 @@ class Stepper:
         check(lib().psum_stepper_phi(self.h, out.ctypes.data_as(C.c_void_p)), "psum_stepper_phi")
         return out
 
+    def rho(self):
+        """(rho of the current positions as a node-shaped FP64 array, the
+        deposit's fixed-point scale 2^S)"""
+        out = np.zeros(num_nodes(self.g), np.float64)
+        sc = C.c_double(0)
+        check(lib().psum_stepper_rho(self.h, out.ctypes.data_as(C.c_void_p), C.byref(sc)), "rho")
+        return out.reshape(node_shape(self.g)), sc.value
+
     def count(self, s=0):
         """current particle count of species s"""
         pos = (C.c_void_p * 3)()
@@ -650,8 +673,10 @@ class Stepper:
         rel = C.c_double(0)
         cnt = (C.c_int64 * 4)()
         check(lib().psum_stepper_stats(self.h, C.byref(cyc), C.byref(rel), cnt), "stats")
+        steps = C.c_int64(0)
+        check(lib().psum_stepper_step_count(self.h, C.byref(steps)), "step_count")
         return {"cycles": cyc.value, "last_rel_res": rel.value, "oob": cnt[0],
-                "tile_fallback_blocks": cnt[1], "absorbed": cnt[2]}
+                "tile_fallback_blocks": cnt[1], "absorbed": cnt[2], "steps": steps.value}
 
     def set_timing(self, on=True):
         check(lib().psum_stepper_set_timing(self.h, int(on)), "set_timing")

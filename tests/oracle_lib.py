@@ -244,6 +244,8 @@ def lib():
         L.orc_mesh_plant.argtypes = [C.c_int]
         L.orc_compute_udf.argtypes = [C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.c_int]
         L.orc_pic_set_order.argtypes = [C.c_int]
+        L.orc_pic_plant.argtypes = [C.c_int]
+        L.orc_pic_set_probe.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
         L.orc_mg_create.argtypes = [i64, i64, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int]
         L.orc_mg_is_singular.argtypes = [P, P]
@@ -576,6 +578,35 @@ def gather_push_bs(g, order, pos, vel, phi, qm, dt, mass=1.0):
 
 def pic_set_order(order):
     lib().orc_pic_set_order(order)
+
+
+def pic_plant(which):
+    """Plant a mistake in the whole step (1 = deposit charge sign, 2 = rho_bg
+    sign, 3 = push with the previous step's phi, 4 = no periodic
+    neutralisation; 0 = none)."""
+    lib().orc_pic_plant(which)
+
+
+class pic_probe:
+    """Context manager: while active, every step of orc_pic_run_* records
+    sum_s q_s w_s sum_p exp(i k.x_p) for each wave vector in kvecs (shape
+    [nk, D]) over the positions the step's solve sees.  After a run,
+    .modes(steps) returns a complex array [steps, nk]."""
+
+    def __init__(self, kvecs, max_steps):
+        self.k = np.ascontiguousarray(np.atleast_2d(kvecs), np.float64)
+        self.out = np.zeros((max_steps, self.k.shape[0], 2), np.float64)
+
+    def __enter__(self):
+        lib().orc_pic_set_probe(self.k.shape[0], self.k.ctypes.data, self.out.ctypes.data)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_pic_set_probe(0, None, None)
+
+    def modes(self, steps):
+        o = self.out[:steps]
+        return o[..., 0] + 1j * o[..., 1]
 
 
 def bspline_plant(which):

@@ -91,3 +91,39 @@ def test_autotuned_batch_holds_the_tolerance():
         rel = np.abs(h[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])
         assert rel.max() < 1e-4, (i, rel.max())
     b.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_batch_with_vcycle_tail_matches_instances_stepped_alone(dtype):
+    """The one-workgroup V-cycle tail switched on in every instance (its
+    LDS size and kernel attribute are fixed when the hierarchy is built,
+    outside the batch's stream capture): the batch still reproduces each
+    instance stepped alone, and both reproduce the instances without the
+    tail bit for bit."""
+    from psum_amd import engine as E
+    # 32^2 at depth 2 ends at 8^2 and 64^2 at depth 3 ends at 8^2, so both
+    # run their coarse levels in the tail; 48^2 ends at 12^2 (no tail)
+    specs = [_instance(32, 16, 1, 0.05, 11, dtype), _instance(64, 4, 3, 0.02, 12, dtype),
+             _instance(48, 9, 2, 0.01, 13, dtype)]
+    sched = [(6, 5, 1), (7, 0, 0), (5, 3, 2)]
+    steps = 8
+    b = E.Batch(specs)
+    for i, (c, k, sh) in enumerate(sched):
+        b.configure(i, c, k, sh)
+        b.instances[i].mg_tuning("tail 1")
+    b.reset_history(steps)
+    b.run(steps)
+    for i, spec in enumerate(specs):
+        a = dict(spec)
+        g, species = a.pop("g"), a.pop("species")
+        hs = []
+        for tail in (1, 0):
+            st = E.Stepper(g, species, **a)
+            st.mg_tuning(f"tail {tail}")
+            st.set_schedule(*sched[i])
+            hs.append(st.run(steps))
+            st.close()
+        h = b.history(i)
+        assert np.array_equal(h, hs[0]), (i, np.abs(h - hs[0]).max())
+        assert np.array_equal(hs[0], hs[1]), (i, np.abs(hs[0] - hs[1]).max())
+    b.close()

@@ -1,0 +1,252 @@
+"""Every benchmark configuration at its stated size on the GPU:
+
+- plain FP64 2D Langmuir, 128^2 periodic, 64 e-/cell, 100 steps: histories
+  against the oracle's restated reference loop;
+- 2D two-stream, 1024^2, 100 e-/cell: 10 FP64 steps against the oracle,
+  and the mixed-precision field energy against FP64 over 100 steps;
+- 3D bounded box, 256^3 cells, 32 + 32 per cell (1.07e9 particles): charge
+  conservation of the deposit, particle bookkeeping and finite histories
+  every step; and the same workload on 64^3 cells against the oracle;
+- batched ensemble, 64 instances: each instance bit for bit against the
+  instance stepped alone, and one instance against the oracle;
+- differentiable two-stream, 256^2, 64 e-/cell, 200 steps: the FP64 adjoint's
+  directional derivatives against the oracle's FP64 tangent-linear run.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _oracle_grid(eg):
+    g = O.Grid()
+    for f, _ in O.Grid._fields_:
+        v = getattr(eg, f)
+        if hasattr(v, "__len__"):
+            getattr(g, f)[:] = list(v)
+        else:
+            setattr(g, f, v)
+    return g
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_c1_langmuir_full_size_fp64_history_matches_oracle():
+    from psum_amd import engine as E
+    from psum_amd.workloads import langmuir_2d
+    wl = langmuir_2d(n=128, ppc=64, seed=1)
+    sp = wl["species"][0]
+    steps = 100
+    _say("c1: oracle 100 steps, 1048576 particles")
+    rp = [a.copy() for a in sp["pos"]]
+    rv = [a.copy() for a in sp["vel"]]
+    h_ref, phi_ref, _ = O.pic_run(_oracle_grid(wl["grid"]), rp, rv, sp["q"], sp["m"], sp["w"],
+                                  wl["rho_bg"], 1.0, wl["dt"], steps, depth=4, tol=1e-10,
+                                  max_cycles=40)
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f64", depth=4, tol=1e-10, max_cycles=40,
+                   dt=wl["dt"], rho_bg=wl["rho_bg"])
+    h = st.run(steps)
+    assert np.max(np.abs(h[:, 0] - h_ref[:, 0]) / np.abs(h_ref[:, 0])) < 1e-8
+    assert np.max(np.abs(h[:, 1] - h_ref[:, 1]) / np.abs(h_ref[:, 1])) < 1e-10
+    scale_p = np.sqrt(np.abs(h_ref[:, 1]).max()) * 10
+    assert np.max(np.abs(h[:, 2:4] - h_ref[:, 2:4])) < 1e-9 * scale_p
+    gp, gv = st.state(0)
+    for a, b in zip(rp + rv, gp + gv):
+        assert np.max(np.abs(a - b)) < 1e-8
+    st.close()
+
+
+def test_c2_two_stream_full_size_fp64_matches_oracle():
+    from psum_amd import engine as E
+    from psum_amd.workloads import two_stream_2d
+    wl = two_stream_2d(n=1024, ppc=100, dtype=np.float64)
+    sp = wl["species"][0]
+    steps = 10
+    _say("c2: oracle 10 FP64 steps, 104857600 particles")
+    rp = [a.copy() for a in sp["pos"]]
+    rv = [a.copy() for a in sp["vel"]]
+    h_ref, phi_ref, _ = O.pic_run(_oracle_grid(wl["grid"]), rp, rv, sp["q"], sp["m"], sp["w"],
+                                  wl["rho_bg"], 1.0, wl["dt"], steps, depth=5, pre=3, post=3,
+                                  tol=1e-10, max_cycles=40)
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f64", depth=5, pre=3, post=3, tol=1e-10,
+                   max_cycles=40, dt=wl["dt"], rho_bg=wl["rho_bg"], sort_every=5)
+    h = st.run(steps)
+    assert np.max(np.abs(h[:, 0] - h_ref[:, 0]) / np.abs(h_ref[:, 0])) < 1e-8
+    assert np.max(np.abs(h[:, 1] - h_ref[:, 1]) / np.abs(h_ref[:, 1])) < 1e-10
+    phi = st.phi()
+    assert np.max(np.abs(phi - phi_ref)) < 1e-8 * np.abs(phi_ref).max()
+    st.close()
+
+
+def test_c2_two_stream_full_size_mixed_field_energy_within_1e5_of_fp64():
+    """The mixed-precision step (FP32 state, FP32 V-cycles in FP64
+    refinement to 1e-8) against the FP64 step on the same inputs: field
+    energy within 1e-5 relative over the first 100 steps."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import two_stream_2d
+    wl = two_stream_2d(n=1024, ppc=100, dtype=np.float32)
+    steps = 100
+    kw = dict(depth=5, pre=3, post=3, tol=1e-8, max_cycles=40, dt=wl["dt"], rho_bg=wl["rho_bg"],
+              sort_every=25)
+    _say("c2: mixed 100 steps")
+    h32 = E.Stepper(wl["grid"], wl["species"], dtype="f32", mixed_mg=True, **kw).run(steps)
+    wl64 = [dict(s, pos=[a.astype(np.float64) for a in s["pos"]],
+                 vel=[a.astype(np.float64) for a in s["vel"]]) for s in wl["species"]]
+    _say("c2: FP64 100 steps")
+    h64 = E.Stepper(wl["grid"], wl64, dtype="f64", **kw).run(steps)
+    rel = np.abs(h32[:, 0] - h64[:, 0]) / np.abs(h64[:, 0])
+    assert rel.max() < 1e-5, rel.max()
+    relk = np.abs(h32[:, 1] - h64[:, 1]) / np.abs(h64[:, 1])
+    assert relk.max() < 1e-6, relk.max()
+
+
+def test_c3_bounded_box_full_size_conservation():
+    """256^3 cells, 32 e- + 32 ions per cell: after every sort (which drops
+    absorbed particles), absorbed + alive = initial, and the deposited charge
+    sum(rho) dV equals sum_s q_s w_s n_s of the live particles up to the
+    fixed-point rounding of each term; histories stay finite."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import bounded_3d
+    _say("c3: generating 1073741824 particles on the device")
+    wl = bounded_3d(cells=256, ppc=32, device="cuda")
+    n0 = [len(s["pos"][0]) for s in wl["species"]]
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f32", mixed_mg=True, depth=6, pre=3, post=3,
+                   tol=1e-8, max_cycles=40, dt=wl["dt"], rho_bg=0.0, fixed=wl["fixed"],
+                   sort_every=0)
+    qw = [s["q"] * s["w"] for s in wl["species"]]
+    for s in wl["species"]:
+        s["pos"] = s["vel"] = None
+    torch.cuda.empty_cache()
+    for rnd in range(3):
+        _say(f"c3: 3 steps + sort, round {rnd}")
+        h = st.run(3)
+        assert np.all(np.isfinite(h))
+        st.sort()
+        n = [st.count(s) for s in range(2)]
+        absorbed = st.stats()["absorbed"]
+        assert sum(n) + absorbed == sum(n0)
+        rho, scale = st.rho()
+        total = float(np.sum(rho, dtype=np.float64))      # dV = 1
+        want = sum(q * k for q, k in zip(qw, n))
+        # each of the 8 terms per particle is rounded to a multiple of 1/scale
+        # (terms with |coef| <= 1e-10 are skipped: the second, tiny allowance)
+        bound = 8 * sum(n) * 0.5 / scale + 1e-12 * sum(abs(q) * k for q, k in zip(qw, n))
+        assert abs(total - want) <= bound, (total, want, bound)
+    assert st.stats()["absorbed"] > 0
+    st.close()
+
+
+def test_c3_bounded_box_64_cubed_matches_oracle():
+    """The 3D workload (32 + 32 per cell, mass ratio 1836, Te/Ti 10,
+    grounded walls, absorbing boundary) on 64^3 cells, FP64, against the
+    oracle's restated loop."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import bounded_3d
+    wl = bounded_3d(cells=64, ppc=32, seed=13, dtype=np.float64)
+    steps = 5
+    ref_sp = [dict(s, pos=[a.copy() for a in s["pos"]], vel=[a.copy() for a in s["vel"]])
+              for s in wl["species"]]
+    _say("c3/64: oracle 5 steps, 16777216 particles")
+    h_ref, phi_ref, _ = O.pic_run_multi(_oracle_grid(wl["grid"]), ref_sp, 0.0, 1.0, wl["dt"], steps,
+                                        depth=4, tol=1e-10, max_cycles=40, fixed=wl["fixed"])
+    st = E.Stepper(wl["grid"], wl["species"], dtype="f64", depth=4, tol=1e-10, max_cycles=40,
+                   dt=wl["dt"], rho_bg=0.0, fixed=wl["fixed"])
+    h = st.run(steps)
+    assert np.max(np.abs(h[:, 0] - h_ref[:, 0]) / np.abs(h_ref[:, 0])) < 1e-8
+    assert np.max(np.abs(h[:, 1] - h_ref[:, 1]) / np.abs(h_ref[:, 1])) < 1e-10
+    for s in range(2):
+        gp, gv = st.state(s)
+        for a, b in zip(ref_sp[s]["pos"] + ref_sp[s]["vel"], gp + gv):
+            assert np.array_equal(np.isnan(a), np.isnan(b))
+            ok = ~np.isnan(a)
+            assert np.max(np.abs(a[ok] - b[ok])) < 1e-9
+    st.close()
+
+
+def test_c4_full_ensemble_matches_instances_and_oracle():
+    """The 64-instance ensemble (grids 64^2..512^2, 16-128 e-/cell, FP32 with
+    FP64 MG refinement): 5 batched steps equal, bit for bit, each instance
+    stepped alone with the same schedule; instance 0 agrees with the oracle's
+    FP64 run of its inputs."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import ensemble
+    specs = ensemble(64, seed=11, device="cuda")
+    meta = [s.pop("meta") for s in specs]
+    steps = 5
+    _say(f"c4: batch of 64 instances, {sum(m['particles'] for m in meta)} particles")
+    b = E.Batch(specs)
+    sched = []
+    for i, m in enumerate(meta):
+        sc = (6, 2 + i % 3, i % 3)
+        sched.append(sc)
+        b.configure(i, *sc)
+    b.reset_history(steps)
+    b.run(steps)
+    for i, spec in enumerate(specs):
+        a = dict(spec)
+        g, species = a.pop("g"), a.pop("species")
+        st = E.Stepper(g, species, **a)
+        st.set_schedule(*sched[i])
+        h_ref = st.run(steps)
+        assert np.array_equal(b.history(i), h_ref), i
+        if i < 4:
+            p_ref, v_ref = st.state(0)
+            p, v = b.instances[i].state(0)
+            for x, y in zip(p + v, p_ref + v_ref):
+                assert np.array_equal(x, y), i
+        st.close()
+    # instance 0 against the oracle (FP64 inputs cast from the FP32 ones)
+    s0 = specs[0]["species"][0]
+    sp64 = dict(q=s0["q"], m=s0["m"], w=s0["w"],
+                pos=[t.double().cpu().numpy() for t in s0["pos"]],
+                vel=[t.double().cpu().numpy() for t in s0["vel"]])
+    _say(f"c4: instance 0 ({meta[0]['n']}^2, {meta[0]['ppc']} e-/cell) vs the oracle")
+    h_ref, _, _ = O.pic_run(_oracle_grid(specs[0]["g"]), sp64["pos"], sp64["vel"], sp64["q"],
+                            sp64["m"], sp64["w"], 1.0, 1.0, 0.1, steps, depth=specs[0]["depth"],
+                            pre=2, post=2, tol=1e-10, max_cycles=40)
+    h = b.history(0)
+    assert np.max(np.abs(h[:, 0] - h_ref[:, 0]) / np.abs(h_ref[:, 0])) < 1e-4
+    assert np.max(np.abs(h[:, 1] - h_ref[:, 1]) / np.abs(h_ref[:, 1])) < 1e-6
+    b.close()
+
+
+def test_c5_full_size_fp64_adjoint_matches_tangent_linear():
+    """256^2, 64 e-/cell, 200 steps, loss = final field energy: the FP64 GPU
+    adjoint, contracted with three Fourier-mode displacements and one random
+    velocity direction, against the oracle's FP64 tangent-linear run in those
+    directions (1e-6 relative)."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import mode_basis, two_stream_modes
+    wl = two_stream_modes(n=256, ppc=64, nm=32, amp=1e-3, dtype=np.float64)
+    sp = wl["species"][0]
+    L = 256.0
+    B = mode_basis(wl["xi"], 32, L)
+    x = np.mod(wl["xi"] + wl["amps"] @ B, L)
+    sp0 = dict(q=sp["q"], m=sp["m"], w=sp["w"], pos=[x, sp["pos"][1].astype(np.float64)],
+               vel=[a.astype(np.float64) for a in sp["vel"]])
+    steps = 200
+    st = E.Stepper(wl["grid"], [sp0], dtype="f64", depth=5, pre=3, post=3, tol=1e-12,
+                   max_cycles=80, dt=wl["dt"], rho_bg=wl["rho_bg"])
+    _say("c5: FP64 GPU adjoint, 200 steps, 4194304 particles")
+    Lg, gx, gv, _, _ = st.adjoint(steps, ckpt=20)
+    st.close()
+    pic = O.Pic(_oracle_grid(wl["grid"]), [sp0], rho_bg=wl["rho_bg"], dt=wl["dt"], steps=steps,
+                depth=5, pre=3, post=3, tol=1e-12, max_cycles=80)
+    rng = np.random.default_rng(17)
+    dirs = []
+    for k in (1, 5, 24):
+        dirs.append(([[B[k - 1], np.zeros_like(x)]], [[np.zeros_like(x), np.zeros_like(x)]]))
+    dv = [rng.standard_normal(len(x)), rng.standard_normal(len(x))]
+    dirs.append(([[np.zeros_like(x), np.zeros_like(x)]], [dv]))
+    for j, (dpos, dvel) in enumerate(dirs):
+        _say(f"c5: oracle tangent-linear run {j}")
+        dL, L_ref = pic.tangent(dpos, dvel)
+        assert abs(Lg - L_ref) <= 1e-9 * abs(L_ref)
+        adj = sum(float(np.dot(gx[0][d], dpos[0][d]) + np.dot(gv[0][d], dvel[0][d]))
+                  for d in range(2))
+        assert abs(adj - dL) <= 1e-6 * abs(dL), (j, adj, dL)

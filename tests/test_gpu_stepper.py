@@ -60,7 +60,7 @@ def test_mixed_vs_fp64_field_energy():
                      max_cycles=40, dt=wl["dt"], rho_bg=wl["rho_bg"])
     h32 = st32.run(steps)
     rel = np.abs(h32[:, 0] - h64[:, 0]) / np.abs(h64[:, 0])
-    assert rel.max() < 1e-4, rel.max()
+    assert rel.max() < 1e-5, rel.max()
     relk = np.abs(h32[:, 1] - h64[:, 1]) / np.abs(h64[:, 1])
     assert relk.max() < 1e-5
 
