@@ -34,6 +34,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 #include "psum_oracle.h"
 
 int orc_base(const orc_grid* g, double x0, double x1, double x2, int* c, double* s);
@@ -416,27 +417,19 @@ double orc_pic_adjoint(const orc_pic_problem* P, double*** pos, double*** vel, d
     return L;
 }
 
-double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, double*** dpos,
-                       double*** dvel, double* loss)
+/* drho = sum_p w_dep grad c_k(x_p) . dx_p over every species; particles are
+ * split over OpenMP threads with private grids summed in thread order */
+static void tangent_deposit(ctx_t* c, double*** x, double*** dx, double* drho)
 {
-    ctx_t c;
-    if (!ctx_init(&c, P)) return NAN;
-    const int D = c.D, N = P->steps;
-    const int64_t nn = c.nn;
-    double*** x = alloc_state(P, pos);
-    double*** v = alloc_state(P, vel);
-    double*** dx = alloc_state(P, dpos);
-    double*** dv = alloc_state(P, dvel);
-    double* b = (double*)malloc(sizeof(double) * (size_t)nn);
-    double* phi = (double*)calloc((size_t)nn, sizeof(double));
-    double* drho = (double*)malloc(sizeof(double) * (size_t)nn);
-    double* dphi = (double*)malloc(sizeof(double) * (size_t)nn);
-    double dL = 0.0;
-    for (int st = 0; st <= N; ++st) {
-        solve_field(&c, x, b, phi);
-        /* drho = sum_p w_dep grad c_k(x_p) . dx_p */
-        for (int64_t i = 0; i < nn; ++i) drho[i] = 0.0;
-        for (int s = 0; s < P->nspecies; ++s)
+    const orc_pic_problem* P = c->P;
+    const int D = c->D;
+    const int64_t nn = c->nn;
+    for (int64_t i = 0; i < nn; ++i) drho[i] = 0.0;
+#pragma omp parallel
+    {
+        double* priv = (double*)calloc((size_t)nn, sizeof(double));
+        for (int s = 0; s < P->nspecies; ++s) {
+#pragma omp for schedule(static)
             for (int64_t p = 0; p < P->n[s]; ++p) {
                 double xp[3] = {0, 0, 0};
                 for (int d = 0; d < D; ++d) xp[d] = x[s][d][p];
@@ -447,19 +440,61 @@ double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, d
                 for (int i = 0; i < (1 << D); ++i) {
                     double t = 0.0;
                     for (int e = 0; e < D; ++e) t += grad_c(D, W, G, i, e) * dx[s][e][p];
-                    drho[idx[i]] += c.wdep[s] * t;
+                    priv[idx[i]] += c->wdep[s] * t;
                 }
             }
-        solve_linear(&c, drho, dphi);
+        }
+#pragma omp for ordered schedule(static, 1)
+        for (int t = 0; t < omp_get_num_threads(); ++t) {
+#pragma omp ordered
+            for (int64_t i = 0; i < nn; ++i) drho[i] += priv[i];
+        }
+        free(priv);
+    }
+}
+
+double orc_pic_tangent_multi(const orc_pic_problem* P, double*** pos, double*** vel, int ndir,
+                             double**** dpos, double**** dvel, double* dL, double* loss)
+{
+    ctx_t c;
+    if (ndir < 1 || !ctx_init(&c, P)) return NAN;
+    const int D = c.D, N = P->steps;
+    const int64_t nn = c.nn;
+    double*** x = alloc_state(P, pos);
+    double*** v = alloc_state(P, vel);
+    double**** dx = (double****)malloc(sizeof(double***) * (size_t)ndir);
+    double**** dv = (double****)malloc(sizeof(double***) * (size_t)ndir);
+    for (int j = 0; j < ndir; ++j) {
+        dx[j] = alloc_state(P, dpos[j]);
+        dv[j] = alloc_state(P, dvel[j]);
+        dL[j] = 0.0;
+    }
+    double* b = (double*)malloc(sizeof(double) * (size_t)nn);
+    double* phi = (double*)calloc((size_t)nn, sizeof(double));
+    double* drho = (double*)malloc(sizeof(double) * (size_t)nn);
+    double* dphi = (double*)malloc(sizeof(double) * (size_t)nn * (size_t)ndir);
+    double* egrad = (double*)malloc(sizeof(double) * (size_t)nn);
+    for (int st = 0; st <= N; ++st) {
+        solve_field(&c, x, b, phi);
+        for (int j = 0; j < ndir; ++j) {
+            tangent_deposit(&c, x, dx[j], drho);
+            solve_linear(&c, drho, dphi + (size_t)j * nn);
+        }
         if (st == N) {
             if (loss) *loss = orc_field_energy(P->g, phi, P->eps);
-            energy_grad(P->g, phi, P->eps, drho);
-            for (int64_t i = 0; i < nn; ++i) dL += drho[i] * dphi[i];
+            energy_grad(P->g, phi, P->eps, egrad);
+            for (int j = 0; j < ndir; ++j) {
+                const double* dp = dphi + (size_t)j * nn;
+                double acc = 0.0;
+                for (int64_t i = 0; i < nn; ++i) acc += egrad[i] * dp[i];
+                dL[j] = acc;
+            }
             break;
         }
         /* dv' = dv - qm dt (H dx + sum_k dphi_k grad c_k); dx' = dx + dt dv' */
         for (int s = 0; s < P->nspecies; ++s) {
             const double qmdt = P->q[s] / P->m[s] * P->dt;
+#pragma omp parallel for schedule(static)
             for (int64_t p = 0; p < P->n[s]; ++p) {
                 double xp[3] = {0, 0, 0};
                 for (int d = 0; d < D; ++d) xp[d] = x[s][d][p];
@@ -467,21 +502,29 @@ double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, d
                 int64_t idx[8];
                 double W[3][2], G[3][2];
                 if (!support(P->g, xp, idx, W, G)) continue;
-                double ddv[3] = {0, 0, 0};
-                for (int e = 0; e < D; ++e) {
-                    double t = 0.0;
-                    for (int i = 0; i < (1 << D); ++i) t += dphi[idx[i]] * grad_c(D, W, G, i, e);
+                double hes[3][3];
+                for (int e = 0; e < D; ++e)
                     for (int f = 0; f < D; ++f) {
+                        hes[e][f] = 0.0;
                         if (f == e) continue;
-                        double h = 0.0;
-                        for (int i = 0; i < (1 << D); ++i) h += phi[idx[i]] * hess_c(D, W, G, i, e, f);
-                        t += h * dx[s][f][p];
+                        for (int i = 0; i < (1 << D); ++i) hes[e][f] += phi[idx[i]] * hess_c(D, W, G, i, e, f);
                     }
-                    ddv[e] = -qmdt * t;
-                }
-                for (int e = 0; e < D; ++e) {
-                    dv[s][e][p] += ddv[e];
-                    dx[s][e][p] += P->dt * dv[s][e][p];
+                for (int j = 0; j < ndir; ++j) {
+                    const double* dp = dphi + (size_t)j * nn;
+                    double ddv[3] = {0, 0, 0};
+                    for (int e = 0; e < D; ++e) {
+                        double t = 0.0;
+                        for (int i = 0; i < (1 << D); ++i) t += dp[idx[i]] * grad_c(D, W, G, i, e);
+                        for (int f = 0; f < D; ++f) {
+                            if (f == e) continue;
+                            t += hes[e][f] * dx[j][s][f][p];
+                        }
+                        ddv[e] = -qmdt * t;
+                    }
+                    for (int e = 0; e < D; ++e) {
+                        dv[j][s][e][p] += ddv[e];
+                        dx[j][s][e][p] += P->dt * dv[j][s][e][p];
+                    }
                 }
             }
         }
@@ -489,10 +532,23 @@ double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, d
         for (int s = 0; s < P->nspecies; ++s)       /* absorbed particles drop out */
             for (int64_t p = 0; p < P->n[s]; ++p)
                 if (!isfinite(x[s][0][p]))
-                    for (int d = 0; d < D; ++d) { dx[s][d][p] = 0.0; dv[s][d][p] = 0.0; }
+                    for (int j = 0; j < ndir; ++j)
+                        for (int d = 0; d < D; ++d) { dx[j][s][d][p] = 0.0; dv[j][s][d][p] = 0.0; }
     }
-    free_state(P, x); free_state(P, v); free_state(P, dx); free_state(P, dv);
-    free(b); free(phi); free(drho); free(dphi);
+    free_state(P, x); free_state(P, v);
+    for (int j = 0; j < ndir; ++j) { free_state(P, dx[j]); free_state(P, dv[j]); }
+    free(dx); free(dv);
+    free(b); free(phi); free(drho); free(dphi); free(egrad);
     ctx_free(&c);
+    return dL[0];
+}
+
+double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, double*** dpos,
+                       double*** dvel, double* loss)
+{
+    double dL = 0.0;
+    double*** dp[1] = {dpos};
+    double*** dv[1] = {dvel};
+    orc_pic_tangent_multi(P, pos, vel, 1, dp, dv, &dL, loss);
     return dL;
 }

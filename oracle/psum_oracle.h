@@ -315,6 +315,10 @@ void orc_adjoint_plant(int which);
 /* Forward mode: returns dL = <grad L, (dpos, dvel)>; *loss = L. */
 double orc_pic_tangent(const orc_pic_problem* P, double*** pos, double*** vel, double*** dpos,
                        double*** dvel, double* loss);
+/* The same for ndir directions (dpos[j], dvel[j]) along one forward
+ * trajectory: dL[j]; returns dL[0]. */
+double orc_pic_tangent_multi(const orc_pic_problem* P, double*** pos, double*** vel, int ndir,
+                             double**** dpos, double**** dvel, double* dL, double* loss);
 
 /* Monte Carlo collisions (particle_collision/foundation.hpp:17-107,
  * mcc_model.hpp:184-218; actions of CCP/custom_mccm.hpp:29-104; random

@@ -275,6 +275,8 @@ def lib():
         L.orc_pic_adjoint.restype = C.c_double
         L.orc_pic_tangent.argtypes = [C.POINTER(PicProblem), P, P, P, P, P]
         L.orc_pic_tangent.restype = C.c_double
+        L.orc_pic_tangent_multi.argtypes = [C.POINTER(PicProblem), P, P, C.c_int, P, P, P, P]
+        L.orc_pic_tangent_multi.restype = C.c_double
         L.orc_adjoint_plant.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -360,6 +362,28 @@ class Pic:
         dv, k4 = self._arrs(dvel)
         L = C.c_double(0)
         dL = lib().orc_pic_tangent(C.byref(self.P), pp, vp, dp, dv, C.cast(C.pointer(L), C.c_void_p))
+        return dL, L.value
+
+    def tangent_multi(self, dirs):
+        """(dL[j], L) for the directions dirs[j] = (dpos[s][d], dvel[s][d]),
+        all along one forward trajectory"""
+        pp, k1 = self._arrs([sp["pos"] for sp in self.species])
+        vp, k2 = self._arrs([sp["vel"] for sp in self.species])
+        nd = len(dirs)
+        dpa = (C.c_void_p * nd)()
+        dva = (C.c_void_p * nd)()
+        keep = []
+        for j, (dpos, dvel) in enumerate(dirs):
+            dp, k3 = self._arrs(dpos)
+            dv, k4 = self._arrs(dvel)
+            keep += [dp, dv, k3, k4]
+            dpa[j] = dp.value
+            dva[j] = dv.value
+        dL = np.zeros(nd, np.float64)
+        L = C.c_double(0)
+        lib().orc_pic_tangent_multi(C.byref(self.P), pp, vp, nd, C.cast(dpa, C.c_void_p),
+                                    C.cast(dva, C.c_void_p), _ptr(dL),
+                                    C.cast(C.pointer(L), C.c_void_p))
         return dL, L.value
 
 

@@ -81,6 +81,21 @@ def test_tangent_dot_adjoint_periodic():
     assert abs(dL - dot) <= 1e-10 * abs(dL)
 
 
+def test_tangent_multi_equals_single_directions():
+    """orc_pic_tangent_multi runs several directions along one forward
+    trajectory: each equals its own orc_pic_tangent run."""
+    g, sp, _ = two_stream_small()
+    pic = O.Pic(g, sp, rho_bg=1.0, dt=0.1, steps=8)
+    rng = np.random.default_rng(11)
+    dirs = [([[rng.standard_normal(len(a)) for a in sp[0]["pos"]]],
+             [[rng.standard_normal(len(a)) for a in sp[0]["vel"]]]) for _ in range(3)]
+    dL, L = pic.tangent_multi(dirs)
+    for j, (dpos, dvel) in enumerate(dirs):
+        d1, L1 = pic.tangent(dpos, dvel)
+        assert L1 == L
+        assert abs(dL[j] - d1) <= 1e-12 * abs(d1)
+
+
 def test_finite_differences_periodic():
     g, sp, _ = two_stream_small()
     pic = O.Pic(g, sp, rho_bg=1.0, dt=0.1, steps=10)
