@@ -440,6 +440,10 @@ def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_pr
         schedule = (tun["sort_every"], tun["sort_shift"])
         log(f"[bench] c3: tuned in {time.time() - t0:.1f}s: sort every {schedule[0]} steps on "
             f"{2 ** schedule[1]}^3-cell bins")
+        # tuning advanced the state: start again from the initial particles,
+        # so that the mixed and FP64 runs time the same physical steps
+        for i, s in enumerate(wl["species"]):
+            st.load(i, [a.to(tdt) for a in s["pos"]], [a.to(tdt) for a in s["vel"]])
     st.set_schedule(0, schedule[0], schedule[1])
     if warmup:
         st.run(warmup, history=False)

@@ -350,7 +350,8 @@ typedef struct psum_step_config {
 
 void* psum_stepper_create(const psum_step_config* cfg);
 /* copy species s in (d < dim arrays of n particles, dtype of cfg; host or
- * device memory) */
+ * device memory).  Starts a new run: the step counter that sets the sort
+ * cadence (a sort before step k when k % sort_every == 0) restarts at 0. */
 psum_status psum_stepper_load(void* h, int s, const void* const* pos, const void* const* vel,
                               int64_t n);
 /* device pointers of species s state (for zero-copy wrapping) */

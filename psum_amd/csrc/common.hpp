@@ -117,6 +117,13 @@ __device__ __forceinline__ int wave_isum(int v)
     return v;
 }
 
+__device__ __forceinline__ long long wave_lsum(long long v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 __device__ __forceinline__ int wave_min(int v)
 {
 #pragma unroll

@@ -318,6 +318,10 @@ constexpr int kTileNodes = 2048;
 // grows with the cube of the drift
 template <int D>
 constexpr int tile_nodes() { return D == 3 ? 2 * kTileNodes : kTileNodes; }
+// per-axis extent of a clipped box: kClip^D <= tile_nodes<D>()
+template <int D>
+constexpr int kClip() { return D == 3 ? 16 : (D == 2 ? 45 : 2048); }
+static_assert(16 * 16 * 16 <= 2 * kTileNodes && 45 * 45 <= kTileNodes, "clipped box exceeds the tile");
 
 template <typename T>
 __device__ __forceinline__ void load4(const T* __restrict__ p, T* v)
@@ -426,11 +430,15 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
 {
     __shared__ unsigned long long tile[DEPOSIT ? tile_nodes<D>() : 1];
     __shared__ int bmin[3], bmax[3], bref[3];
+    __shared__ long long bsum[3];
+    __shared__ int bcnt;
     __shared__ double wdiag[kThreads / 64][4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (DEPOSIT && threadIdx.x < 3) {
         bmin[threadIdx.x] = INT_MAX;
         bmax[threadIdx.x] = INT_MIN;
+        bsum[threadIdx.x] = 0;
+        if (threadIdx.x == 0) bcnt = 0;
     }
     T* const xs[3] = {x0, x1, x2};
     T* const vs[3] = {v0, v1, v2};
@@ -587,7 +595,45 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
             e[d] = bmax[d] - bmin[d] + 2;   // support adds one node
             tsize *= any ? (int64_t)e[d] : 0;
         }
-        const bool use_lds = any && tsize <= tile_nodes<D>();
+        // A box beyond the tile is clipped to at most kClip<D> nodes per
+        // axis (kClip^D <= tile nodes) around the block's mean cell: runs
+        // whose 2^D support lies inside go to the tile, the others (the few
+        // particles far from the block's centre) to global integer atomics.
+        const bool clipped = any && tsize > tile_nodes<D>();
+        if (clipped) {
+            long long ls[3] = {0, 0, 0};
+            int lc = 0;
+#pragma unroll
+            for (int j = 0; j < kPPT; ++j)
+                if (ok[j]) {
+                    ++lc;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) ls[d] += uw[j][d];
+                }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const long long t = wave_lsum(ls[d]);
+                if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&bsum[d]), (unsigned long long)t);
+            }
+            const int tc = wave_isum(lc);
+            if (lane == 0) atomicAdd(&bcnt, tc);
+            __syncthreads();
+            tsize = 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int c = min(e[d], kClip<D>());
+                // mean cell (floor division of a possibly negative sum)
+                const long long sm = bsum[d];
+                const int mean = (int)(sm >= 0 ? sm / bcnt : -((-sm + bcnt - 1) / bcnt));
+                int l = mean - c / 2 + 1;
+                l = max(l, bmin[d]);
+                l = min(l, bmax[d] + 2 - c);
+                lo[d] = l;
+                e[d] = c;
+                tsize *= c;
+            }
+        }
+        const bool use_lds = any;
         if (use_lds)
             for (int i = threadIdx.x; i < (int)tsize; i += kThreads) tile[i] = 0ull;
         __syncthreads();
@@ -598,12 +644,16 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         int cur[D], curu[D];
         bool have = false;
         auto flush = [&]() {
+            // the run's support (cells curu .. curu + 1 per axis) inside the box
+            bool inside = use_lds;
+#pragma unroll
+            for (int d = 0; d < D; ++d) inside = inside && curu[d] >= lo[d] && curu[d] + 1 < lo[d] + e[d];
 #pragma unroll
             for (int q = 0; q < NS; ++q) {
                 const int i = (q + lane) & (NS - 1);
                 const long long t = run[i];
                 if (t == 0) continue;
-                if (use_lds) {
+                if (inside) {
                     int li = 0;
 #pragma unroll
                     for (int d = 0; d < D; ++d) li = li * e[d] + (curu[d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
@@ -665,9 +715,8 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 for (int d = 1; d < D; ++d) lin = lin * (uint32_t)g.w[d] + (uint32_t)nd[d];
                 atomicAdd(&acc[lin], v);
             }
-        } else if (any && counters && threadIdx.x == 0) {
-            atomicAdd(&counters[1], 1ull);
         }
+        if (clipped && counters && threadIdx.x == 0) atomicAdd(&counters[1], 1ull);
     }
 
     if (PUSH && diag) {
@@ -997,12 +1046,17 @@ size_t psum_hip_compact_workspace(int64_t n)
     return align256((size_t)(nb + 1) * 4) + scan_workspace(nb) + 256;
 }
 
-psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
-                             void* const* out_pos, void* const* out_vel, int64_t* count_host,
-                             void* work, size_t work_bytes, void* stream)
+}  // extern "C"
+
+// Body of psum_hip_compact / psum_hip_compact_noalloc: `work` holds at least
+// psum_hip_compact_workspace(n) bytes of device memory.  Every word of it
+// that is read is written first in the same call (block counts, then the
+// scan's partial sums), so it needs no clearing between calls.
+static psum_status compact_impl(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
+                                void* const* out_pos, void* const* out_vel, int64_t* count_host,
+                                void* work, void* stream)
 {
     PSUM_ARG_CHECK(dim >= 1 && dim <= 3, "psum_hip_compact: bad dim");
-    PSUM_ARG_CHECK(work_bytes >= psum_hip_compact_workspace(n), "psum_hip_compact: workspace too small");
     hipStream_t st = S(stream);
     if (n == 0) {
         if (count_host) *count_host = 0;
@@ -1043,6 +1097,17 @@ psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* 
     }
     return PSUM_OK;
 }
+
+extern "C" {
+
+psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
+                             void* const* out_pos, void* const* out_vel, int64_t* count_host,
+                             void* work, size_t work_bytes, void* stream)
+{
+    PSUM_ARG_CHECK(work_bytes >= psum_hip_compact_workspace(n), "psum_hip_compact: workspace too small");
+    return compact_impl(dtype, dim, pos, vel, n, out_pos, out_vel, count_host, work, stream);
+}
+
 
 psum_status psum_hip_deposit_cic(const psum_grid* g, int dtype, void* const* pos, int64_t n,
                                  double w, double scale, int64_t* acc, int64_t* counters,

@@ -330,6 +330,7 @@ public:
         }
         sp.n = n;
         acc_valid_ = false;
+        step_ = 0;          // a new run: the sort cadence restarts
         S_TRY(hipStreamSynchronize(st_));
     }
 
