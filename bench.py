@@ -107,56 +107,22 @@ def fe_rel_err(h_mixed, h_fp64):
     return {"max": float(err.max()), "steps": int(k)}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "push_deposit_two_stream_1024_hbm_traffic.md")
-
-
 def profiled_traffic(kernel, grid, ppc):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-calibrated), for the workload it was measured on;
-    None for other workloads or when the summary is absent."""
-    try:
-        for line in open(TRAFFIC_PROFILE):
-            f = line.split()
-            if len(f) == 5 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel \
-                    and int(f[2]) == grid and int(f[3]) == ppc:
-                return float(f[4])
-    except OSError:
-        pass
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries (copy-calibrated) under profiles/, for
+    the workload they were measured on (a line "traffic_bytes_per_launch
+    <kernel> <grid> <ppc> <bytes>"); None when no summary covers it."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*hbm_traffic*.md"))):
+        try:
+            for line in open(path):
+                f = line.split()
+                if len(f) == 5 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel \
+                        and int(f[2]) == grid and int(f[3]) == ppc:
+                    return float(f[4])
+        except OSError:
+            pass
     return None
-
-
-def cpu_baseline(wl, steps):
-    """The CPU oracle (FP64 restatement of the reference step, OpenMP) on the
-    first `steps` steps of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    sp = wl["species"][0]
-    g = wl["grid"]
-    og = O.Grid()
-    for f, _ in O.Grid._fields_:
-        v = getattr(g, f)
-        if hasattr(v, "__len__"):
-            getattr(og, f)[:] = list(v)
-        else:
-            setattr(og, f, v)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    pos = [a.astype(np.float64) for a in sp["pos"]]
-    vel = [a.astype(np.float64) for a in sp["vel"]]
-    t0 = time.perf_counter()
-    O.pic_run(og, [a.copy() for a in pos[:1]] + [a.copy() for a in pos[1:]], [a.copy() for a in vel],
-              sp["q"], sp["m"], sp["w"], wl["rho_bg"], 1.0, wl["dt"], 0, depth=5, tol=1e-8,
-              max_cycles=40)
-    setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.pic_run(og, pos, vel, sp["q"], sp["m"], sp["w"], wl["rho_bg"], 1.0, wl["dt"], steps, depth=5,
-              tol=1e-8, max_cycles=40)
-    total = time.perf_counter() - t0
-    t = max(1e-9, total - setup)
-    n = len(sp["pos"][0])
-    log(f"[bench] cpu oracle: {steps} steps in {t:.1f}s (+{setup:.1f}s setup), {cores} threads")
-    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"first {steps} steps of the same workload ({n} particles, FP64 oracle "
-                      f"restatement of the reference step, OpenMP)"}
 
 
 def run_adjoint(E, grid, ppc, steps, ckpt, depth, pre, post):
@@ -497,6 +463,8 @@ def main():
     ap.add_argument("--c3-ppc", type=int, default=32)
     ap.add_argument("--c3-depth", type=int, default=6)
     ap.add_argument("--c3-no-fp64", action="store_true")
+    ap.add_argument("--c3-schedule", default=None,
+                    help="'every,shift': sort schedule of the 3D box instead of the autotuner's")
     ap.add_argument("--grid", type=int, default=1024, help="2D two-stream grid")
     ap.add_argument("--ppc", type=int, default=100)
     ap.add_argument("--depth", type=int, default=5)
@@ -544,7 +512,9 @@ def main():
     t0 = time.time()
     wl3 = c3_workload(args.c3_cells, args.c3_ppc)
     log(f"[bench] c3 workload {wl3['name']}: generated in {time.time() - t0:.1f}s")
-    c3, sched = run_c3(E, wl3, "f32", args.steps, args.warmup, args.c3_depth, args.pre, args.post)
+    fixed_sched = None if args.c3_schedule is None else tuple(int(v) for v in args.c3_schedule.split(","))
+    c3, sched = run_c3(E, wl3, "f32", args.steps, args.warmup, args.c3_depth, args.pre, args.post,
+                       schedule=fixed_sched)
     if args.calib_copy:
         nb = int(c3["particles_per_launch"]) * 2 * 3
         src = torch.ones(nb, dtype=torch.float32, device="cuda")
