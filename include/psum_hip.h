@@ -90,6 +90,17 @@ psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* 
                              void* const* out_pos, void* const* out_vel, int64_t* count_host,
                              void* work, size_t work_bytes, void* stream);
 
+/* psum_hip_compact without a workspace argument and without allocating:
+ * its scratch (block counts and scan partial sums, 8.06 MiB) is a static
+ * device array of the library, one per GPU, held until the process ends.
+ * No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  n <= 2^31 (PSUM_ERR_ARG beyond).  Calls that overlap
+ * on one GPU share the scratch and race: order them on one stream. */
+psum_status psum_hip_compact_noalloc(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
+                                     void* const* out_pos, void* const* out_vel, int64_t* count_host,
+                                     void* stream);
+
 /* ---- field: CIC deposit and gather ------------------------------------
  * Deposit (add_back, src/field/device_field.hpp:295-310 ->
  * field_interp.hpp:63-75 -> atomic_add.hpp:34-48): for every valid

@@ -168,6 +168,7 @@ SIGNATURES = [
     ("psum_hip_permute", ST, [C.c_int, C.c_int, P, P, P, I64, P]),
     ("psum_hip_compact_workspace", C.c_size_t, [I64]),
     ("psum_hip_compact", ST, [C.c_int, C.c_int, P, P, I64, P, P, C.POINTER(I64), P, C.c_size_t, P]),
+    ("psum_hip_compact_noalloc", ST, [C.c_int, C.c_int, P, P, I64, P, P, C.POINTER(I64), P]),
     ("psum_hip_deposit_cic", ST, [_GP, C.c_int, P, I64, C.c_double, C.c_double, P, P, P]),
     ("psum_hip_fixed_to_double", ST, [P, I64, C.c_double, C.c_double, P, P]),
     ("psum_hip_push_blocks", I64, [I64]),

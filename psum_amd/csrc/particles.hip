@@ -18,6 +18,7 @@
 
 #include <climits>
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -1098,6 +1099,39 @@ static psum_status compact_impl(int dtype, int dim, void* const* pos, void* cons
     return PSUM_OK;
 }
 
+// ------------------------------------------------ allocation-free compaction
+// Device scratch of psum_hip_compact_noalloc: block counts and the scan's
+// partial sums for up to kNoAllocMaxParticles particles.  Each GPU the
+// library runs on holds its own copy until the process ends; every call on
+// a GPU shares it, so calls that overlap on one GPU race.
+constexpr int64_t kNoAllocMaxParticles = (int64_t)1 << 31;
+constexpr size_t kNoAllocScratchBytes = ((size_t)8 << 20) + ((size_t)64 << 10);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (the host-side name is only a stand-in).
+psum_status scratch_on_current_device(void** out)
+{
+    int dev = -1;
+    PSUM_HIP_TRY(hipGetDevice(&dev));
+    PSUM_ARG_CHECK(dev >= 0 && dev < kMaxDevices, "psum_hip_compact_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        PSUM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PSUM_ARG_CHECK(p != nullptr, "psum_hip_compact_noalloc: no device address for the scratch array");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return PSUM_OK;
+}
+}  // namespace
+
 extern "C" {
 
 psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
@@ -1108,6 +1142,19 @@ psum_status psum_hip_compact(int dtype, int dim, void* const* pos, void* const* 
     return compact_impl(dtype, dim, pos, vel, n, out_pos, out_vel, count_host, work, stream);
 }
 
+psum_status psum_hip_compact_noalloc(int dtype, int dim, void* const* pos, void* const* vel, int64_t n,
+                                     void* const* out_pos, void* const* out_vel, int64_t* count_host,
+                                     void* stream)
+{
+    PSUM_ARG_CHECK(n >= 0 && n <= kNoAllocMaxParticles,
+                   "psum_hip_compact_noalloc: n exceeds the 2^31-particle limit of its scratch");
+    PSUM_ARG_CHECK(psum_hip_compact_workspace(n) <= kNoAllocScratchBytes,
+                   "psum_hip_compact_noalloc: scratch too small for n");
+    void* work = nullptr;
+    const psum_status s = scratch_on_current_device(&work);
+    if (s != PSUM_OK) return s;
+    return compact_impl(dtype, dim, pos, vel, n, out_pos, out_vel, count_host, work, stream);
+}
 
 psum_status psum_hip_deposit_cic(const psum_grid* g, int dtype, void* const* pos, int64_t n,
                                  double w, double scale, int64_t* acc, int64_t* counters,
