@@ -30,6 +30,9 @@ typedef int psum_status;
 
 #define PSUM_F64 0
 #define PSUM_F32 1
+/* adjoint kernels only: FP32 positions with the adjoint state (lx, lv),
+ * phi / lrho and the arithmetic in FP64 */
+#define PSUM_F32_ADJ64 2
 
 #define PSUM_KEY_INVALID 0xFFFFFFFFu
 
@@ -548,6 +551,9 @@ void* psum_batch_instance(void* b, int inst);
 void psum_batch_destroy(void* b);
 
 /* ---- adjoint kernels (used by psum_stepper_adjoint) --------------------
+ * dtype PSUM_F64 / PSUM_F32: positions, lx, lv and phi / lrho of that type;
+ * PSUM_F32_ADJ64: FP32 positions, FP64 lx, lv, phi / lrho and arithmetic
+ * (the FP32 stepper's reverse sweep).
  * Reverse of gather + kick + drift for one species, in place on (lx, lv):
  *   lvt = lv + dt lx;  lx -= qm dt H(x)^T lvt;  lv = lvt;
  *   acc[idx_k] += llrint((-qm dt sum_e lvt_e d_e c_k(x)) * scale_dev[0])

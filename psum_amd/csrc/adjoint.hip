@@ -76,24 +76,27 @@ __device__ __forceinline__ double wave_max_d(double v)
 
 // Reverse of kick + drift + gather for one species (in place on lx, lv);
 // gradient-weight deposit of lvt into acc with the device-side scale.
-template <typename T, int D>
+// T: precision of the positions (the forward state); A: precision of the
+// adjoint state lx, lv, of phi and of the arithmetic (A = double with FP32
+// positions keeps the reverse sweep's accumulations in FP64).
+template <typename T, typename A, int D>
 __global__ __launch_bounds__(kA) void k_adj_push(DevGrid g, const T* __restrict__ x0,
                                                  const T* __restrict__ x1, const T* __restrict__ x2,
-                                                 T* __restrict__ lx0, T* __restrict__ lx1,
-                                                 T* __restrict__ lx2, T* __restrict__ lv0,
-                                                 T* __restrict__ lv1, T* __restrict__ lv2,
-                                                 int64_t n, const T* __restrict__ phi, T qm, T dt,
+                                                 A* __restrict__ lx0, A* __restrict__ lx1,
+                                                 A* __restrict__ lx2, A* __restrict__ lv0,
+                                                 A* __restrict__ lv1, A* __restrict__ lv2,
+                                                 int64_t n, const A* __restrict__ phi, A qm, A dt,
                                                  const double* __restrict__ scale_dev,
                                                  unsigned long long* __restrict__ acc)
 {
     const int64_t p = (int64_t)blockIdx.x * kA + threadIdx.x;
     if (p >= n) return;
     const T* xs[3] = {x0, x1, x2};
-    T* lxs[3] = {lx0, lx1, lx2};
-    T* lvs[3] = {lv0, lv1, lv2};
-    T x[D], lx[D], lvt[D];
+    A* lxs[3] = {lx0, lx1, lx2};
+    A* lvs[3] = {lv0, lv1, lv2};
+    A x[D], lx[D], lvt[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) x[d] = xs[d][p];
+    for (int d = 0; d < D; ++d) x[d] = (A)xs[d][p];
     if (!isfinite((double)x[0])) return;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -101,21 +104,21 @@ __global__ __launch_bounds__(kA) void k_adj_push(DevGrid g, const T* __restrict_
         lvt[d] = lvs[d][p] + dt * lx[d];
     }
     int cw[D];
-    T W[D][2], G[D][2];
-    if (cic_factors<T, D>(g, x, cw, W, G)) {
-        T ph[1 << D];
+    A W[D][2], G[D][2];
+    if (cic_factors<A, D>(g, x, cw, W, G)) {
+        A ph[1 << D];
 #pragma unroll
         for (int i = 0; i < (1 << D); ++i) ph[i] = phi[support_index<D>(g, cw, i)];
         // lx_f -= qm dt sum_e lvt_e H_ef
 #pragma unroll
         for (int f = 0; f < D; ++f) {
-            T acc_f = (T)0;
+            A acc_f = (A)0;
 #pragma unroll
             for (int e = 0; e < D; ++e) {
                 if (e == f) continue;
-                T h = (T)0;
+                A h = (A)0;
 #pragma unroll
-                for (int i = 0; i < (1 << D); ++i) h = h + ph[i] * cic_hess_coef<T, D>(W, G, i, e, f);
+                for (int i = 0; i < (1 << D); ++i) h = h + ph[i] * cic_hess_coef<A, D>(W, G, i, e, f);
                 acc_f = acc_f + lvt[e] * h;
             }
             lx[f] = lx[f] - (qm * dt) * acc_f;
@@ -125,9 +128,9 @@ __global__ __launch_bounds__(kA) void k_adj_push(DevGrid g, const T* __restrict_
         const double mqdt = -(double)qm * (double)dt;
 #pragma unroll
         for (int i = 0; i < (1 << D); ++i) {
-            T t = (T)0;
+            A t = (A)0;
 #pragma unroll
-            for (int e = 0; e < D; ++e) t = t + lvt[e] * cic_grad_coef<T, D>(W, G, i, e);
+            for (int e = 0; e < D; ++e) t = t + lvt[e] * cic_grad_coef<A, D>(W, G, i, e);
             const long long term = round_to_ll((mqdt * (double)t) * scale);
             if (term) atomicAdd(&acc[support_index<D>(g, cw, i)], (unsigned long long)term);
         }
@@ -142,42 +145,42 @@ __global__ __launch_bounds__(kA) void k_adj_push(DevGrid g, const T* __restrict_
 // Reverse of the deposit for one species: lx += w_dep grad(lrho)(x).  Also
 // writes per-block maxima of sum_e |lv_e + dt lx_e| (the bound the next
 // reverse push needs for its fixed-point scale).
-template <typename T, int D>
+template <typename T, typename A, int D>
 __global__ __launch_bounds__(kA) void k_adj_gather(DevGrid g, const T* __restrict__ x0,
                                                    const T* __restrict__ x1,
-                                                   const T* __restrict__ x2, T* __restrict__ lx0,
-                                                   T* __restrict__ lx1, T* __restrict__ lx2,
-                                                   const T* __restrict__ lv0,
-                                                   const T* __restrict__ lv1,
-                                                   const T* __restrict__ lv2, int64_t n,
-                                                   const T* __restrict__ lrho, T w_dep, T dt,
+                                                   const T* __restrict__ x2, A* __restrict__ lx0,
+                                                   A* __restrict__ lx1, A* __restrict__ lx2,
+                                                   const A* __restrict__ lv0,
+                                                   const A* __restrict__ lv1,
+                                                   const A* __restrict__ lv2, int64_t n,
+                                                   const A* __restrict__ lrho, A w_dep, A dt,
                                                    double* __restrict__ bmax)
 {
     __shared__ double wm[kA / 64];
     const int64_t p = (int64_t)blockIdx.x * kA + threadIdx.x;
     const T* xs[3] = {x0, x1, x2};
-    T* lxs[3] = {lx0, lx1, lx2};
-    const T* lvs[3] = {lv0, lv1, lv2};
+    A* lxs[3] = {lx0, lx1, lx2};
+    const A* lvs[3] = {lv0, lv1, lv2};
     double m = 0.0;
     if (p < n) {
-        T x[D];
+        A x[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = xs[d][p];
+        for (int d = 0; d < D; ++d) x[d] = (A)xs[d][p];
         if (isfinite((double)x[0])) {
             int cw[D];
-            T W[D][2], G[D][2];
-            T lx[D];
+            A W[D][2], G[D][2];
+            A lx[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) lx[d] = lxs[d][p];
-            if (cic_factors<T, D>(g, x, cw, W, G)) {
-                T lr[1 << D];
+            if (cic_factors<A, D>(g, x, cw, W, G)) {
+                A lr[1 << D];
 #pragma unroll
                 for (int i = 0; i < (1 << D); ++i) lr[i] = lrho[support_index<D>(g, cw, i)];
 #pragma unroll
                 for (int f = 0; f < D; ++f) {
-                    T s = (T)0;
+                    A s = (A)0;
 #pragma unroll
-                    for (int i = 0; i < (1 << D); ++i) s = s + lr[i] * cic_grad_coef<T, D>(W, G, i, f);
+                    for (int i = 0; i < (1 << D); ++i) s = s + lr[i] * cic_grad_coef<A, D>(W, G, i, f);
                     lx[f] = lx[f] + w_dep * s;
                 }
 #pragma unroll
@@ -331,31 +334,33 @@ __global__ __launch_bounds__(kA) void k_mode_reduce(const double* __restrict__ p
     out[k] = s;
 }
 
-template <typename T, int D>
+template <typename T, typename A, int D>
 psum_status launch_adj_push(const psum_grid* g, void* const* x, void* const* lx, void* const* lv,
                             int64_t n, const void* phi, double qm, double dt,
                             const double* scale_dev, int64_t* acc, hipStream_t st)
 {
     const DevGrid dg = to_dev(*g);
     auto P = [&](void* const* a, int d) { return d < D ? static_cast<T*>(a[d]) : nullptr; };
-    hipLaunchKernelGGL((k_adj_push<T, D>), dim3(nblk(n, kA)), dim3(kA), 0, st, dg, P(x, 0), P(x, 1),
-                       P(x, 2), P(lx, 0), P(lx, 1), P(lx, 2), P(lv, 0), P(lv, 1), P(lv, 2), n,
-                       static_cast<const T*>(phi), (T)qm, (T)dt, scale_dev,
+    auto Q = [&](void* const* a, int d) { return d < D ? static_cast<A*>(a[d]) : nullptr; };
+    hipLaunchKernelGGL((k_adj_push<T, A, D>), dim3(nblk(n, kA)), dim3(kA), 0, st, dg, P(x, 0), P(x, 1),
+                       P(x, 2), Q(lx, 0), Q(lx, 1), Q(lx, 2), Q(lv, 0), Q(lv, 1), Q(lv, 2), n,
+                       static_cast<const A*>(phi), (A)qm, (A)dt, scale_dev,
                        reinterpret_cast<unsigned long long*>(acc));
     PSUM_HIP_TRY(hipGetLastError());
     return PSUM_OK;
 }
 
-template <typename T, int D>
+template <typename T, typename A, int D>
 psum_status launch_adj_gather(const psum_grid* g, void* const* x, void* const* lx, void* const* lv,
                               int64_t n, const void* lrho, double w_dep, double dt, double* bmax,
                               hipStream_t st)
 {
     const DevGrid dg = to_dev(*g);
     auto P = [&](void* const* a, int d) { return d < D ? static_cast<T*>(a[d]) : nullptr; };
-    hipLaunchKernelGGL((k_adj_gather<T, D>), dim3(nblk(n, kA)), dim3(kA), 0, st, dg, P(x, 0),
-                       P(x, 1), P(x, 2), P(lx, 0), P(lx, 1), P(lx, 2), P(lv, 0), P(lv, 1), P(lv, 2),
-                       n, static_cast<const T*>(lrho), (T)w_dep, (T)dt, bmax);
+    auto Q = [&](void* const* a, int d) { return d < D ? static_cast<A*>(a[d]) : nullptr; };
+    hipLaunchKernelGGL((k_adj_gather<T, A, D>), dim3(nblk(n, kA)), dim3(kA), 0, st, dg, P(x, 0),
+                       P(x, 1), P(x, 2), Q(lx, 0), Q(lx, 1), Q(lx, 2), Q(lv, 0), Q(lv, 1), Q(lv, 2),
+                       n, static_cast<const A*>(lrho), (A)w_dep, (A)dt, bmax);
     PSUM_HIP_TRY(hipGetLastError());
     return PSUM_OK;
 }
@@ -372,11 +377,14 @@ psum_status psum_hip_adjoint_push(const psum_grid* g, int dtype, void* const* x,
     PSUM_ARG_CHECK(n == 0 || (x && lx && lv && phi && scale_dev && acc), "psum_hip_adjoint_push: null pointer");
     if (n == 0) return PSUM_OK;
     if (dtype == PSUM_F64)
-        return g->dim == 2 ? launch_adj_push<double, 2>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream))
-                           : launch_adj_push<double, 3>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream));
+        return g->dim == 2 ? launch_adj_push<double, double, 2>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream))
+                           : launch_adj_push<double, double, 3>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream));
     if (dtype == PSUM_F32)
-        return g->dim == 2 ? launch_adj_push<float, 2>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream))
-                           : launch_adj_push<float, 3>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream));
+        return g->dim == 2 ? launch_adj_push<float, float, 2>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream))
+                           : launch_adj_push<float, float, 3>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream));
+    if (dtype == PSUM_F32_ADJ64)
+        return g->dim == 2 ? launch_adj_push<float, double, 2>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream))
+                           : launch_adj_push<float, double, 3>(g, x, lx, lv, n, phi, qm, dt, scale_dev, acc, S(stream));
     set_error("psum_hip_adjoint_push: bad dtype");
     return PSUM_ERR_ARG;
 }
@@ -391,11 +399,14 @@ psum_status psum_hip_adjoint_gather(const psum_grid* g, int dtype, void* const* 
     PSUM_ARG_CHECK(n == 0 || (x && lx && lv && lrho && bmax), "psum_hip_adjoint_gather: null pointer");
     if (n == 0) return PSUM_OK;
     if (dtype == PSUM_F64)
-        return g->dim == 2 ? launch_adj_gather<double, 2>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream))
-                           : launch_adj_gather<double, 3>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream));
+        return g->dim == 2 ? launch_adj_gather<double, double, 2>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream))
+                           : launch_adj_gather<double, double, 3>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream));
     if (dtype == PSUM_F32)
-        return g->dim == 2 ? launch_adj_gather<float, 2>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream))
-                           : launch_adj_gather<float, 3>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream));
+        return g->dim == 2 ? launch_adj_gather<float, float, 2>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream))
+                           : launch_adj_gather<float, float, 3>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream));
+    if (dtype == PSUM_F32_ADJ64)
+        return g->dim == 2 ? launch_adj_gather<float, double, 2>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream))
+                           : launch_adj_gather<float, double, 3>(g, x, lx, lv, n, lrho, w_dep, dt, bmax, S(stream));
     set_error("psum_hip_adjoint_gather: bad dtype");
     return PSUM_ERR_ARG;
 }

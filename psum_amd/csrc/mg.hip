@@ -120,6 +120,27 @@ __device__ __forceinline__ int nb_index(int c, int o, int n, int wrap)
     return q;
 }
 
+// Workgroups are dealt to the eight XCDs round-robin (workgroup b runs on
+// XCD b % 8).  xcd_block() renumbers them so that each XCD works through one
+// contiguous eighth of the index range: on 3D levels the neighbour planes a
+// stencil reads (n1*n2 nodes apart) then stay in the same XCD's L2.  Blocks
+// of an incomplete last round keep their index, so the map is a bijection
+// for any grid size.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block()
+{
+    const unsigned g = gridDim.x, b = blockIdx.x;
+    const unsigned full = g / kXcds * kXcds;
+    if (b >= full) return b;
+    return (int64_t)(b % kXcds) * (full / kXcds) + b / kXcds;
+}
+
+template <int D>
+__device__ __forceinline__ int64_t level_block()
+{
+    return D == 3 ? xcd_block() : (int64_t)blockIdx.x;
+}
+
 // sum_k coef[k] * x[nb_k], k ascending over the 3^D offsets (o0 slowest),
 // COMPACT: only the 2D+1 axis offsets (all other coefficients are zero).
 template <typename T, int D, bool COMPACT>
@@ -165,7 +186,7 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth(DevLevel L, DevTable<T> r
                                                        const T* __restrict__ b,
                                                        T* __restrict__ xout, T two3)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
     if (i >= L.N) return;
     int c[3];
     node_coords<D>(L, i, c);
@@ -181,7 +202,7 @@ __global__ __launch_bounds__(kMgThreads) void k_residual(DevLevel L, DevTable<T>
                                                          const T* __restrict__ b,
                                                          T* __restrict__ r)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
     if (i >= L.N) return;
     int c[3];
     node_coords<D>(L, i, c);
@@ -200,7 +221,7 @@ __global__ __launch_bounds__(kMgThreads) void k_restrict(DevLevel F, DevLevel Cl
                                                          T* __restrict__ bc, T* __restrict__ xc,
                                                          T first_sweep)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
     if (i >= Cl.N) return;
     int c[3];
     node_coords<D>(Cl, i, c);
@@ -245,7 +266,7 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
                                                             const T* __restrict__ xc,
                                                             T* __restrict__ xf)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
     if (i >= F.N) return;
     int c[3];
     node_coords<D>(F, i, c);
@@ -846,7 +867,8 @@ __global__ __launch_bounds__(kMgThreads) void k_outer_residual(
     float* __restrict__ b0, double* __restrict__ part, float* __restrict__ e0)
 {
     __shared__ double ws[kMgThreads / 64][2];
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    const int64_t blk = level_block<D>();
+    const int64_t i = blk * kMgThreads + threadIdx.x;
     double rr = 0.0, bb = 0.0;
     if (i < L.N) {
         int c[3];
@@ -873,7 +895,7 @@ __global__ __launch_bounds__(kMgThreads) void k_outer_residual(
     if (threadIdx.x < 2) {
         double s = 0.0;
         for (int w = 0; w < kMgThreads / 64; ++w) s += ws[w][threadIdx.x];
-        part[2 * (int64_t)blockIdx.x + threadIdx.x] = s;
+        part[2 * blk + threadIdx.x] = s;
     }
 }
 

@@ -641,62 +641,99 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
 
         constexpr int NS = 1 << D;
         const double wsc = w_dep * scale;
-        long long run[NS];
-        int cur[D], curu[D];
-        bool have = false;
-        auto flush = [&]() {
-            // the run's support (cells curu .. curu + 1 per axis) inside the box
-            bool inside = use_lds;
+        if constexpr (D == 3) {
+            // 3D: particles sorted on cell bins rarely share a cell with their
+            // neighbour, so each particle adds its 8 terms directly, with the
+            // support offsets fixed at compile time (no run bookkeeping).
+            const int s1 = e[2], s0 = e[1] * e[2];
 #pragma unroll
-            for (int d = 0; d < D; ++d) inside = inside && curu[d] >= lo[d] && curu[d] + 1 < lo[d] + e[d];
-#pragma unroll
-            for (int q = 0; q < NS; ++q) {
-                const int i = (q + lane) & (NS - 1);
-                const long long t = run[i];
-                if (t == 0) continue;
-                if (inside) {
-                    int li = 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) li = li * e[d] + (curu[d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
-                    atomicAdd(&tile[li], (unsigned long long)t);
-                } else {
-                    atomicAdd(&acc[support_index<D>(g, cur, i)], (unsigned long long)t);
-                }
-            }
-        };
-#pragma unroll
-        for (int j = 0; j < kPPT; ++j) {
-            if (!ok[j]) continue;
-            bool same = have;
-#pragma unroll
-            for (int d = 0; d < D; ++d) same = same && cur[d] == cw[j][d];
-            if (!same) {
-                if (have) flush();
+            for (int j = 0; j < kPPT; ++j) {
+                if (!ok[j]) continue;
+                T W[D][2];
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    cur[d] = cw[j][d];
-                    curu[d] = uw[j][d];
+                    W[d][0] = (T)1 - sw[j][d];
+                    W[d][1] = sw[j][d];
+                }
+                long long t8[NS];
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                    const T cf = cic_coef<T, D>(W, i);
+                    t8[i] = coef_kept<T>(cf) ? round_to_ll((double)cf * wsc) : 0ll;
+                }
+                bool inside = use_lds;
+#pragma unroll
+                for (int d = 0; d < D; ++d) inside = inside && uw[j][d] >= lo[d] && uw[j][d] + 1 < lo[d] + e[d];
+                if (inside) {
+                    const int li0 = ((uw[j][0] - lo[0]) * e[1] + (uw[j][1] - lo[1])) * e[2] + (uw[j][2] - lo[2]);
+#pragma unroll
+                    for (int i = 0; i < NS; ++i)
+                        if (t8[i]) atomicAdd(&tile[li0 + ((i >> 2) & 1) * s0 + ((i >> 1) & 1) * s1 + (i & 1)],
+                                             (unsigned long long)t8[i]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NS; ++i)
+                        if (t8[i]) atomicAdd(&acc[support_index<D>(g, cw[j], i)], (unsigned long long)t8[i]);
+                }
+            }
+        } else {
+            long long run[NS];
+            int cur[D], curu[D];
+            bool have = false;
+            auto flush = [&]() {
+                // the run's support (cells curu .. curu + 1 per axis) inside the box
+                bool inside = use_lds;
+#pragma unroll
+                for (int d = 0; d < D; ++d) inside = inside && curu[d] >= lo[d] && curu[d] + 1 < lo[d] + e[d];
+#pragma unroll
+                for (int q = 0; q < NS; ++q) {
+                    const int i = (q + lane) & (NS - 1);
+                    const long long t = run[i];
+                    if (t == 0) continue;
+                    if (inside) {
+                        int li = 0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) li = li * e[d] + (curu[d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
+                        atomicAdd(&tile[li], (unsigned long long)t);
+                    } else {
+                        atomicAdd(&acc[support_index<D>(g, cur, i)], (unsigned long long)t);
+                    }
+                }
+            };
+#pragma unroll
+            for (int j = 0; j < kPPT; ++j) {
+                if (!ok[j]) continue;
+                bool same = have;
+#pragma unroll
+                for (int d = 0; d < D; ++d) same = same && cur[d] == cw[j][d];
+                if (!same) {
+                    if (have) flush();
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        cur[d] = cw[j][d];
+                        curu[d] = uw[j][d];
+                    }
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) run[q] = 0;
+                    have = true;
+                }
+                T W[D][2];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    W[d][0] = (T)1 - sw[j][d];
+                    W[d][1] = sw[j][d];
                 }
 #pragma unroll
-                for (int q = 0; q < NS; ++q) run[q] = 0;
-                have = true;
+                for (int i = 0; i < NS; ++i) {
+                    const T cf = cic_coef<T, D>(W, i);
+                    if (!coef_kept<T>(cf)) continue;
+                    // (w*cf)*2^S: scaling by a power of two is exact, so one
+                    // product with w*2^S rounds identically
+                    run[i] += round_to_ll((double)cf * wsc);
+                }
             }
-            T W[D][2];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                W[d][0] = (T)1 - sw[j][d];
-                W[d][1] = sw[j][d];
-            }
-#pragma unroll
-            for (int i = 0; i < NS; ++i) {
-                const T cf = cic_coef<T, D>(W, i);
-                if (!coef_kept<T>(cf)) continue;
-                // (w*cf)*2^S: scaling by a power of two is exact, so one
-                // product with w*2^S rounds identically
-                run[i] += round_to_ll((double)cf * wsc);
-            }
+            if (have) flush();
         }
-        if (have) flush();
         if (use_lds) {
             __syncthreads();
             for (int i = threadIdx.x; i < (int)tsize; i += kThreads) {

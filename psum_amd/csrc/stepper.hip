@@ -308,7 +308,7 @@ public:
             if (p) (void)hipFree(p);
         for (auto& e : ev_)
             if (e) (void)hipEventDestroy(e);
-        for (void* p : {(void*)lphi_, (void*)lrho_, (void*)adj_scale_, (void*)bmax_, lrho_t_, (void*)adj_acc_})
+        for (void* p : {(void*)lphi_, (void*)lrho_, (void*)adj_scale_, (void*)bmax_, (void*)adj_acc_})
             if (p) (void)hipFree(p);
         for (void* p : lx_) if (p) (void)hipFree(p);
         for (void* p : lv_) if (p) (void)hipFree(p);
@@ -702,13 +702,12 @@ public:
         adj_predicted_ = mg_predicted(mg_);
         mg_set_predicted(mg_, fwd_pred);
         adj_cycles_ += mg_last_cycles(mg_);
-        if (cfg_.dtype == PSUM_F32) CHK(psum_hip_f64_to_f32(lrho_, (float*)lrho_t_, nn_, st_));
     }
 
-    const void* lrho_view() const
-    {
-        return cfg_.dtype == PSUM_F32 ? (const void*)lrho_t_ : (const void*)lrho_;
-    }
+    // The reverse sweep keeps lx, lv, phi and lrho in FP64 for either particle
+    // precision: FP32 positions use the PSUM_F32_ADJ64 kernels.
+    int adj_dtype() const { return cfg_.dtype == PSUM_F32 ? PSUM_F32_ADJ64 : PSUM_F64; }
+    static constexpr size_t kAdjBytes = sizeof(double);
 
     // lx += w_dep grad(lrho)(x) for every species, and the fixed-point scale
     // of the next adjoint push from the bound sum_e |lv_e + dt lx_e|
@@ -721,8 +720,8 @@ public:
         for (int q = 0; q < cfg_.nspecies; ++q)
             maxfac = std::max(maxfac, std::fabs(sp_[q].sp.q / sp_[q].sp.m) * cfg_.dt * dr);
         for (int q = 0; q < cfg_.nspecies; ++q) {
-            CHK(psum_hip_adjoint_gather(&cfg_.grid, cfg_.dtype, xs + q * 3, &lx_[q * 3], &lv_[q * 3],
-                                        sp_[q].n, lrho_view(), sp_[q].wdep, cfg_.dt, bmax_, st_));
+            CHK(psum_hip_adjoint_gather(&cfg_.grid, adj_dtype(), xs + q * 3, &lx_[q * 3], &lv_[q * 3],
+                                        sp_[q].n, lrho_, sp_[q].wdep, cfg_.dt, bmax_, st_));
             CHK(psum_hip_adjoint_scale(bmax_, psum_hip_adjoint_blocks(sp_[q].n), maxfac, adj_scale_,
                                        q > 0 ? 1 : 0, st_));
         }
@@ -732,7 +731,7 @@ public:
     {
         hipLaunchKernelGGL(k_zero_i64, dim3(nblk(nn_)), dim3(kT), 0, st_, (long long*)adj_acc_, nn_);
         for (int q = 0; q < cfg_.nspecies; ++q)
-            CHK(psum_hip_adjoint_push(&cfg_.grid, cfg_.dtype, xs + q * 3, &lx_[q * 3], &lv_[q * 3],
+            CHK(psum_hip_adjoint_push(&cfg_.grid, adj_dtype(), xs + q * 3, &lx_[q * 3], &lv_[q * 3],
                                       sp_[q].n, phi_t, sp_[q].sp.q / sp_[q].sp.m, cfg_.dt,
                                       adj_scale_, adj_acc_, st_));
         CHK(psum_hip_fixed_to_double_dev(adj_acc_, nn_, adj_scale_, lphi_, st_));
@@ -787,8 +786,8 @@ public:
         CHK(psum_hip_field_energy_grad(&cfg_.grid, phi_, cfg_.eps * seed, lphi_, st_));
         for (int q = 0; q < cfg_.nspecies; ++q)
             for (int d = 0; d < D; ++d) {
-                zero_dev(lx_[q * 3 + d], sp_[q].n * esz_);
-                zero_dev(lv_[q * 3 + d], sp_[q].n * esz_);
+                zero_dev(lx_[q * 3 + d], sp_[q].n * (int64_t)kAdjBytes);
+                zero_dev(lv_[q * 3 + d], sp_[q].n * (int64_t)kAdjBytes);
             }
         zero_dev(lrho_, nn_ * (int64_t)sizeof(double));
         adjoint_solve();
@@ -809,8 +808,7 @@ public:
                         S_TRY(hipMemcpyAsync(seg_x_[k][q * 3 + d], sp_[q].pos[d], (size_t)sp_[q].n * esz_,
                                              hipMemcpyDeviceToDevice, st_));
                 run_steps(1, nullptr, false, false);   // phi^(s0+k), then the push
-                const void* ph = cfg_.dtype == PSUM_F32 ? (const void*)phi32_ : (const void*)phi_;
-                S_TRY(hipMemcpyAsync(seg_phi_[k], ph, (size_t)nn_ * esz_, hipMemcpyDeviceToDevice, st_));
+                S_TRY(hipMemcpyAsync(seg_phi_[k], phi_, (size_t)nn_ * kAdjBytes, hipMemcpyDeviceToDevice, st_));
             }
             for (int k = len - 1; k >= 0; --k) {
                 adjoint_push_all(seg_x_[k].data(), seg_phi_[k]);
@@ -818,12 +816,18 @@ public:
                 adjoint_gather_all(seg_x_[k].data());
             }
         }
+        // gradients in the stepper's dtype (FP32 steppers: one rounding of
+        // the FP64 adjoint state)
+        auto out = [&](void* dst, const void* src, int64_t n) {
+            if (cfg_.dtype == PSUM_F32)
+                CHK(psum_hip_f64_to_f32(static_cast<const double*>(src), static_cast<float*>(dst), n, st_));
+            else
+                S_TRY(hipMemcpyAsync(dst, src, (size_t)n * kAdjBytes, hipMemcpyDeviceToDevice, st_));
+        };
         for (int q = 0; q < cfg_.nspecies; ++q)
             for (int d = 0; d < D; ++d) {
-                if (gx && gx[q * 3 + d])
-                    S_TRY(hipMemcpyAsync(gx[q * 3 + d], lx_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
-                if (gv && gv[q * 3 + d])
-                    S_TRY(hipMemcpyAsync(gv[q * 3 + d], lv_[q * 3 + d], (size_t)sp_[q].n * esz_, hipMemcpyDeviceToDevice, st_));
+                if (gx && gx[q * 3 + d]) out(gx[q * 3 + d], lx_[q * 3 + d], sp_[q].n);
+                if (gv && gv[q * 3 + d]) out(gv[q * 3 + d], lv_[q * 3 + d], sp_[q].n);
             }
         S_TRY(hipEventRecord(aev_[1], st_));
         snap_restore(cks_[0]);
@@ -848,7 +852,6 @@ public:
         const int D = cfg_.grid.dim;
         S_TRY(hipMalloc(&lphi_, nn_ * sizeof(double)));
         S_TRY(hipMalloc(&lrho_, nn_ * sizeof(double)));
-        S_TRY(hipMalloc(&lrho_t_, nn_ * sizeof(double)));
         S_TRY(hipMalloc(&adj_acc_, nn_ * sizeof(int64_t)));
         S_TRY(hipMalloc(&adj_scale_, 4 * sizeof(double)));
         int64_t nbmax = 1;
@@ -857,7 +860,7 @@ public:
         S_TRY(hipMalloc(&bmax_, (size_t)nbmax * sizeof(double)));
         for (int q = 0; q < cfg_.nspecies; ++q)
             for (int d = 0; d < D; ++d) {
-                const size_t b = (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_;
+                const size_t b = (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * kAdjBytes;
                 S_TRY(hipMalloc(&lx_[q * 3 + d], b));
                 S_TRY(hipMalloc(&lv_[q * 3 + d], b));
             }
@@ -877,7 +880,7 @@ public:
             for (int q = 0; q < cfg_.nspecies; ++q)
                 for (int d = 0; d < D; ++d)
                     S_TRY(hipMalloc(&seg_x_[k][q * 3 + d], (size_t)std::max<int64_t>(1, sp_[q].sp.capacity) * esz_));
-            S_TRY(hipMalloc(&seg_phi_[k], (size_t)nn_ * esz_));
+            S_TRY(hipMalloc(&seg_phi_[k], (size_t)nn_ * kAdjBytes));
         }
     }
 
@@ -1072,7 +1075,6 @@ public:
 private:
     bool no_sort_ = false;
     double *lphi_ = nullptr, *lrho_ = nullptr, *adj_scale_ = nullptr, *bmax_ = nullptr;
-    void* lrho_t_ = nullptr;
     int64_t* adj_acc_ = nullptr;
     void* lx_[12] = {};
     void* lv_[12] = {};
