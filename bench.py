@@ -203,8 +203,10 @@ def run_batch(E, n_inst, steps, lanes=0, seed=11):
     1..8, amplitude 1e-3..5e-2), FP32 with FP64 MG refinement, autotuned
     (cycle count, sort bin, sort interval), one step of all instances
     replayed as per-lane hipGraphs on concurrent streams (lanes=0: the
-    lane count in {1, 2, 4, 8} with the shortest 20-step replay).  Also
-    times the same instances stepped one after another without the graph."""
+    lane count in {1, 2, 4, 8} with the shortest 20-step replay).  With one
+    lane the solver cycles of all instances run as batched launches (one per
+    phase and kernel variant).  Also times the same instances stepped one
+    after another without the graph."""
     from psum_amd.workloads import ensemble
     t0 = time.time()
     specs = ensemble(n_inst, seed, device="cuda")
@@ -231,6 +233,7 @@ def run_batch(E, n_inst, steps, lanes=0, seed=11):
     b.set_lanes(lanes)
     b.reset_history(steps + 2)
     b.run(2)                      # capture + first replay
+    solver_launches, solver_ops = b.solver_launches()
     ms = b.run(steps)
     # the same instances, same schedules, stepped one after another without the graph
     seq_steps = max(1, min(steps, 20))
@@ -246,6 +249,7 @@ def run_batch(E, n_inst, steps, lanes=0, seed=11):
         "instances": n_inst, "particles": total, "steps": steps, "lanes": lanes,
         "lane_sweep_ms_per_step": {str(k): v for k, v in sweep.items()},
         "ms_per_step": ms / steps, "particle_steps_per_s": total * steps / (ms * 1e-3),
+        "solver_launches_per_step": solver_launches, "solver_ops_batched": solver_ops,
         "sequential_ms_per_step": seq_ms, "graph_speedup": seq_ms / (ms / steps),
         "histories_finite": bool(hist_ok),
         "tuning": {

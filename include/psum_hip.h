@@ -544,6 +544,16 @@ psum_status psum_batch_reset_history(void* b, int64_t rows);
  * groups run concurrently.  Takes effect at the next psum_batch_run (the
  * graphs are captured again). */
 psum_status psum_batch_set_lanes(void* b, int lanes);
+/* Solver batching (default on): with one lane and every instance's solve a
+ * 2D fixed-cycle solve through the fused tile kernels, the captured step
+ * runs the solver cycles of all instances as batched launches, one per
+ * phase (cycle, level, kernel variant) over the blocks of every instance,
+ * instead of one launch per instance and phase.  Results are bit-identical
+ * either way.  No reference counterpart. */
+psum_status psum_batch_set_solver_batching(void* b, int on);
+/* batched solver launches per step, and the per-instance launches they
+ * replace (0, 0 when the step does not batch its solves) */
+psum_status psum_batch_solver_launches(void* b, int64_t* launches, int64_t* ops);
 /* `steps` replays; *t_ms = HIP-event time of the whole call */
 psum_status psum_batch_run(void* b, int steps, double* t_ms);
 psum_status psum_batch_history(void* b, int inst, double* out, int64_t max_rows, int64_t* rows);

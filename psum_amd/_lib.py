@@ -229,6 +229,8 @@ SIGNATURES = [
     ("psum_batch_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
     ("psum_batch_reset_history", ST, [P, I64]),
     ("psum_batch_set_lanes", ST, [P, C.c_int]),
+    ("psum_batch_set_solver_batching", ST, [P, C.c_int]),
+    ("psum_batch_solver_launches", ST, [P, C.POINTER(I64), C.POINTER(I64)]),
     ("psum_batch_run", ST, [P, C.c_int, C.POINTER(C.c_double)]),
     ("psum_batch_history", ST, [P, C.c_int, P, I64, C.POINTER(I64)]),
     ("psum_batch_instance", P, [P, C.c_int]),

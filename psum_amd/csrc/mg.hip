@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "mg_batch.hpp"
 #include "psum_field_solver.h"
 
 using namespace psum_hip;
@@ -101,9 +102,24 @@ __device__ __forceinline__ int wrap_index(int i, int n, int wrap)
     return i < 0 ? i + n : (i >= n ? i - n : i);
 }
 
+// node coordinates of linear index i (row-major, last axis fastest); 32-bit
+// unsigned division when the level has fewer than 2^32 nodes (64-bit
+// division is a long software sequence on the GPU)
 template <int D>
 __device__ __forceinline__ void node_coords(const DevLevel& L, int64_t i, int* c)
 {
+    if (L.N <= (int64_t)0xffffffffll) {
+        uint32_t u = (uint32_t)i;
+        if (D == 3) {
+            const uint32_t q = u / (uint32_t)L.n[2];
+            c[2] = (int)(u - q * (uint32_t)L.n[2]);
+            u = q;
+        }
+        const uint32_t q = u / (uint32_t)L.n[1];
+        c[1] = (int)(u - q * (uint32_t)L.n[1]);
+        c[0] = (int)q;
+        return;
+    }
     if (D == 3) {
         c[2] = (int)(i % L.n[2]);
         i /= L.n[2];
@@ -559,6 +575,24 @@ __global__ __launch_bounds__(kTailThreads) void k_vtail(TailArgs<T> a)
     for (int i = threadIdx.x; i < a.lev[0].N; i += kTailThreads) a.x_out[i] = x0[i];
 }
 
+// ---- batched launches ------------------------------------------------------
+// A batched launch runs one kernel body for several independent solver
+// instances: instance i owns blocks first[i] .. first[i+1]-1 of the grid and
+// reads its arguments from args[i].  The bodies below take the block index
+// as a parameter, so a single-instance launch (block = blockIdx.x) and a
+// batched one execute identical arithmetic.
+__device__ __forceinline__ int batch_instance(const unsigned* __restrict__ first, int n)
+{
+    int lo = 0, hi = n - 1;
+    const unsigned b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---- fused 2D kernels: several Jacobi sweeps of one tile in LDS ----------
 // A block owns a tile of kTileRows x (kTileX - 2h) output nodes and loads it
 // with a halo of h = `sweeps` nodes (kTileX = 64 columns: one wave per row,
@@ -595,22 +629,44 @@ __device__ __forceinline__ int tile_coord(int g, int n, int wrap)
     return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
 
+template <typename T>
+struct SmoothTileArgs {
+    DevLevel L;
+    DevTable<T> relax;
+    int ncls;
+    const T* xin;
+    const T* b;
+    T* xout;
+    T two3;
+    int h;
+    DevLevel Cl;
+    DevTransfer<T> tr;
+    const T* xc;
+    double* xacc;
+};
+
 template <typename T, bool COMPACT, bool PROLONG, bool ACC>
-__global__ __launch_bounds__(kTileThreads) void k_smooth_tile(DevLevel L, DevTable<T> relax, int ncls,
-                                                              const T* __restrict__ xin,
-                                                              const T* __restrict__ b,
-                                                              T* __restrict__ xout, T two3, int h,
-                                                              DevLevel Cl, DevTransfer<T> tr,
-                                                              const T* __restrict__ xc,
-                                                              double* __restrict__ xacc)
+__device__ __forceinline__ void smooth_tile_body(const SmoothTileArgs<T>& A, unsigned bid)
 {
+    const DevLevel& L = A.L;
+    const DevTable<T>& relax = A.relax;
+    const int ncls = A.ncls;
+    const T* __restrict__ xin = A.xin;
+    const T* __restrict__ b = A.b;
+    T* __restrict__ xout = A.xout;
+    const T two3 = A.two3;
+    const int h = A.h;
+    const DevLevel& Cl = A.Cl;
+    const DevTransfer<T>& tr = A.tr;
+    const T* __restrict__ xc = A.xc;
+    double* __restrict__ xacc = A.xacc;
     __shared__ T sx[2][kTileRmax * kTileX];
     __shared__ T ctab[kMaxClsLds * 9];
     const int tx = threadIdx.x & (kTileX - 1), ty = threadIdx.x / kTileX;
     const int n0 = L.n[0], n1 = L.n[1];
     const int ostep = kTileX - 2 * h;
     const int ntc = (n1 + ostep - 1) / ostep;
-    const int t0 = (int)(blockIdx.x / ntc) * kTileRows, t1 = (int)(blockIdx.x % ntc) * ostep;
+    const int t0 = (int)(bid / ntc) * kTileRows, t1 = (int)(bid % ntc) * ostep;
     const int R = kTileRows + 2 * h;
     const bool lds_tab = ncls <= kMaxClsLds;
     if (lds_tab)
@@ -706,6 +762,20 @@ __global__ __launch_bounds__(kTileThreads) void k_smooth_tile(DevLevel L, DevTab
     }
 }
 
+template <typename T, bool COMPACT, bool PROLONG, bool ACC>
+__global__ __launch_bounds__(kTileThreads) void k_smooth_tile(SmoothTileArgs<T> a)
+{
+    smooth_tile_body<T, COMPACT, PROLONG, ACC>(a, blockIdx.x);
+}
+
+template <typename T, bool COMPACT, bool PROLONG, bool ACC>
+__global__ __launch_bounds__(kTileThreads) void k_smooth_tile_many(const SmoothTileArgs<T>* __restrict__ a,
+                                                                   const unsigned* __restrict__ first, int n)
+{
+    const int i = batch_instance(first, n);
+    smooth_tile_body<T, COMPACT, PROLONG, ACC>(a[i], blockIdx.x - first[i]);
+}
+
 // Tiled residual + restriction (2D): a block restricts a kRRRows x kRRCols
 // coarse tile.  It loads x on the fine support of that tile plus a one-node
 // halo into LDS, evaluates the fine residuals there (k_residual's
@@ -718,17 +788,39 @@ constexpr int kRRCols = 30;
 constexpr int kRRFineRows = 34;
 constexpr int kRRFineCols = 62;
 
+template <typename T>
+struct ResidRestrictArgs {
+    DevLevel F, Cl;
+    DevTransfer<T> tr;
+    DevTable<T> Af;
+    int ncls;
+    DevTable<T> Ac;
+    const T* x;
+    const T* bf;
+    T* bc;
+    T* xc;
+    T first_sweep;
+};
+
 template <typename T, bool COMPACT>
-__global__ __launch_bounds__(kTileThreads) void k_resid_restrict_tile(
-    DevLevel F, DevLevel Cl, DevTransfer<T> tr, DevTable<T> Af, int ncls, DevTable<T> Ac,
-    const T* __restrict__ x, const T* __restrict__ bf, T* __restrict__ bc, T* __restrict__ xc,
-    T first_sweep)
+__device__ __forceinline__ void resid_restrict_tile_body(const ResidRestrictArgs<T>& A, unsigned bid)
 {
+    const DevLevel& F = A.F;
+    const DevLevel& Cl = A.Cl;
+    const DevTransfer<T>& tr = A.tr;
+    const DevTable<T>& Af = A.Af;
+    const int ncls = A.ncls;
+    const DevTable<T>& Ac = A.Ac;
+    const T* __restrict__ x = A.x;
+    const T* __restrict__ bf = A.bf;
+    T* __restrict__ bc = A.bc;
+    T* __restrict__ xc = A.xc;
+    const T first_sweep = A.first_sweep;
     __shared__ T sxx[(kRRFineRows + 2) * 64];
     __shared__ T sr[kRRFineRows * 64];
     __shared__ T ctab[kMaxClsLds * 9];
     const int ntc = (Cl.n[1] + kRRCols - 1) / kRRCols;
-    const int c0lo = (int)(blockIdx.x / ntc) * kRRRows, c1lo = (int)(blockIdx.x % ntc) * kRRCols;
+    const int c0lo = (int)(bid / ntc) * kRRRows, c1lo = (int)(bid % ntc) * kRRCols;
     const int c0hi = min(c0lo + kRRRows, Cl.n[0]), c1hi = min(c1lo + kRRCols, Cl.n[1]);
     const int f0lo = tr.rf[0][c0lo], f0hi = tr.rf[0][c0hi - 1] + tr.rn[0][c0hi - 1];
     const int f1lo = tr.rf[1][c1lo], f1hi = tr.rf[1][c1hi - 1] + tr.rn[1][c1hi - 1];
@@ -792,19 +884,37 @@ __global__ __launch_bounds__(kTileThreads) void k_resid_restrict_tile(
     xc[ci] = first_sweep != (T)0 ? bv * (first_sweep * (T)1) : (T)0;
 }
 
+template <typename T, bool COMPACT>
+__global__ __launch_bounds__(kTileThreads) void k_resid_restrict_tile(ResidRestrictArgs<T> a)
+{
+    resid_restrict_tile_body<T, COMPACT>(a, blockIdx.x);
+}
+
+template <typename T, bool COMPACT>
+__global__ __launch_bounds__(kTileThreads) void k_resid_restrict_tile_many(
+    const ResidRestrictArgs<T>* __restrict__ a, const unsigned* __restrict__ first, int n)
+{
+    const int i = batch_instance(first, n);
+    resid_restrict_tile_body<T, COMPACT>(a[i], blockIdx.x - first[i]);
+}
+
 // k_residual fused into k_restrict (2D): the restriction evaluates the fine
 // residual b - A^ x at each of its fine support nodes with k_residual's
 // arithmetic instead of reading a stored residual.
 template <typename T, bool COMPACT>
-__global__ __launch_bounds__(kMgThreads) void k_resid_restrict(DevLevel F, DevLevel Cl,
-                                                               DevTransfer<T> tr, DevTable<T> Af,
-                                                               DevTable<T> Ac,
-                                                               const T* __restrict__ x,
-                                                               const T* __restrict__ bf,
-                                                               T* __restrict__ bc, T* __restrict__ xc,
-                                                               T first_sweep)
+__device__ __forceinline__ void resid_restrict_body(const ResidRestrictArgs<T>& A, unsigned bid)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
+    const DevLevel& F = A.F;
+    const DevLevel& Cl = A.Cl;
+    const DevTransfer<T>& tr = A.tr;
+    const DevTable<T>& Af = A.Af;
+    const DevTable<T>& Ac = A.Ac;
+    const T* __restrict__ x = A.x;
+    const T* __restrict__ bf = A.bf;
+    T* __restrict__ bc = A.bc;
+    T* __restrict__ xc = A.xc;
+    const T first_sweep = A.first_sweep;
+    const int64_t i = (int64_t)bid * kMgThreads + threadIdx.x;
     if (i >= Cl.N) return;
     int c[3];
     node_coords<2>(Cl, i, c);
@@ -829,29 +939,84 @@ __global__ __launch_bounds__(kMgThreads) void k_resid_restrict(DevLevel F, DevLe
     xc[i] = first_sweep != (T)0 ? bv * (first_sweep * (T)1) : (T)0;
 }
 
+template <typename T, bool COMPACT>
+__global__ __launch_bounds__(kMgThreads) void k_resid_restrict(ResidRestrictArgs<T> a)
+{
+    resid_restrict_body<T, COMPACT>(a, blockIdx.x);
+}
+
+template <typename T, bool COMPACT>
+__global__ __launch_bounds__(kMgThreads) void k_resid_restrict_many(const ResidRestrictArgs<T>* __restrict__ a,
+                                                                    const unsigned* __restrict__ first, int n)
+{
+    const int i = batch_instance(first, n);
+    resid_restrict_body<T, COMPACT>(a[i], blockIdx.x - first[i]);
+}
+
 // b0 = b o D0^-1  (hadamard_to)
 template <typename T>
-__global__ __launch_bounds__(kMgThreads) void k_hadamard(DevLevel L, DevTable<T> A,
-                                                         const T* __restrict__ b,
-                                                         T* __restrict__ out)
+struct HadamardArgs {
+    DevLevel L;
+    DevTable<T> A;
+    const T* b;
+    T* out;
+};
+
+template <typename T>
+__device__ __forceinline__ void hadamard_body(const HadamardArgs<T>& a, unsigned bid)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
-    if (i < L.N) out[i] = b[i] * A.dinv[L.cls[i]];
+    const int64_t i = (int64_t)bid * kMgThreads + threadIdx.x;
+    if (i < a.L.N) a.out[i] = a.b[i] * a.A.dinv[a.L.cls[i]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_hadamard(HadamardArgs<T> a)
+{
+    hadamard_body<T>(a, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_hadamard_many(const HadamardArgs<T>* __restrict__ a,
+                                                              const unsigned* __restrict__ first, int n)
+{
+    const int i = batch_instance(first, n);
+    hadamard_body<T>(a[i], blockIdx.x - first[i]);
 }
 
 // x = Ainv b, one wave per row, fixed-order lane partials + butterfly.
 template <typename T>
-__global__ __launch_bounds__(kMgThreads) void k_coarse_solve(const double* __restrict__ inv, int n,
-                                                             const T* __restrict__ b,
-                                                             T* __restrict__ x)
+struct CoarseArgs {
+    const double* inv;
+    int n;
+    const T* b;
+    T* x;
+};
+
+template <typename T>
+__device__ __forceinline__ void coarse_solve_body(const CoarseArgs<T>& a, unsigned bid)
 {
-    const int row = (int)(((int64_t)blockIdx.x * kMgThreads + threadIdx.x) >> 6);
+    const int row = (int)(((int64_t)bid * kMgThreads + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
+    const int n = a.n;
     if (row >= n) return;
     double s = 0.0;
-    for (int j = lane; j < n; j += 64) s += inv[(int64_t)row * n + j] * (double)b[j];
+    for (int j = lane; j < n; j += 64) s += a.inv[(int64_t)row * n + j] * (double)a.b[j];
     s = wave_sum(s);
-    if (lane == 0) x[row] = (T)s;
+    if (lane == 0) a.x[row] = (T)s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_coarse_solve(CoarseArgs<T> a)
+{
+    coarse_solve_body<T>(a, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_coarse_solve_many(const CoarseArgs<T>* __restrict__ a,
+                                                                  const unsigned* __restrict__ first, int n)
+{
+    const int i = batch_instance(first, n);
+    coarse_solve_body<T>(a[i], blockIdx.x - first[i]);
 }
 
 // FP64 residual of the unscaled operator: r = b - A x, partial sums of r^2
@@ -860,14 +1025,33 @@ __global__ __launch_bounds__(kMgThreads) void k_coarse_solve(const double* __res
 // FP64 residual r = b - A x of the unscaled level-0 operator; partial sums
 // of r^2 (skipping row `pin` of a pinned-periodic system, which the gauge
 // shift satisfies exactly) and of bn^2 (the caller's right-hand side).
+struct OuterArgs {
+    DevLevel L;
+    DevTable<double> K;
+    const double* x;
+    const double* b;
+    const double* bn;
+    int64_t pin;
+    const float* dinv32;
+    float* b0;
+    double* part;
+    float* e0;
+};
+
 template <int D, bool COMPACT, bool OUTF32>
-__global__ __launch_bounds__(kMgThreads) void k_outer_residual(
-    DevLevel L, DevTable<double> K, const double* __restrict__ x, const double* __restrict__ b,
-    const double* __restrict__ bn, int64_t pin, const float* __restrict__ dinv32,
-    float* __restrict__ b0, double* __restrict__ part, float* __restrict__ e0)
+__device__ __forceinline__ void outer_residual_body(const OuterArgs& A, int64_t blk)
 {
+    const DevLevel& L = A.L;
+    const DevTable<double>& K = A.K;
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ b = A.b;
+    const double* __restrict__ bn = A.bn;
+    const int64_t pin = A.pin;
+    const float* __restrict__ dinv32 = A.dinv32;
+    float* __restrict__ b0 = A.b0;
+    double* __restrict__ part = A.part;
+    float* __restrict__ e0 = A.e0;
     __shared__ double ws[kMgThreads / 64][2];
-    const int64_t blk = level_block<D>();
     const int64_t i = blk * kMgThreads + threadIdx.x;
     double rr = 0.0, bb = 0.0;
     if (i < L.N) {
@@ -897,6 +1081,20 @@ __global__ __launch_bounds__(kMgThreads) void k_outer_residual(
         for (int w = 0; w < kMgThreads / 64; ++w) s += ws[w][threadIdx.x];
         part[2 * blk + threadIdx.x] = s;
     }
+}
+
+template <int D, bool COMPACT, bool OUTF32>
+__global__ __launch_bounds__(kMgThreads) void k_outer_residual(OuterArgs a)
+{
+    outer_residual_body<D, COMPACT, OUTF32>(a, level_block<D>());
+}
+
+template <int D, bool COMPACT, bool OUTF32>
+__global__ __launch_bounds__(kMgThreads) void k_outer_residual_many(const OuterArgs* __restrict__ a,
+                                                                    const unsigned* __restrict__ first, int n)
+{
+    const int i = batch_instance(first, n);
+    outer_residual_body<D, COMPACT, OUTF32>(a[i], (int64_t)(blockIdx.x - first[i]));
 }
 
 __global__ __launch_bounds__(kMgThreads) void k_reduce2(const double* __restrict__ part, int64_t nb,
@@ -973,18 +1171,41 @@ __global__ __launch_bounds__(kMgThreads) void k_gauge_apply(double* __restrict__
     if (i < n) x[i] = x[i] - shift[0];
 }
 
-__global__ __launch_bounds__(kMgThreads) void k_zero_f32(float* __restrict__ x, int64_t n)
+// Element-wise level-0 vector operations: x = 0 (FP32), x += e (FP64 +=
+// FP32) and x = y (FP64 copy).
+struct VecArgs {
+    void* x;
+    const void* y;
+    int64_t n;
+};
+
+template <int OP>
+__device__ __forceinline__ void vec_body(const VecArgs& a, unsigned bid)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
-    if (i < n) x[i] = 0.0f;
+    const int64_t i = (int64_t)bid * kMgThreads + threadIdx.x;
+    if (i >= a.n) return;
+    if (OP == 0) static_cast<float*>(a.x)[i] = 0.0f;
+    else if (OP == 1) static_cast<double*>(a.x)[i] += (double)static_cast<const float*>(a.y)[i];
+    else static_cast<double*>(a.x)[i] = static_cast<const double*>(a.y)[i];
 }
 
-__global__ __launch_bounds__(kMgThreads) void k_add_correction(double* __restrict__ x,
-                                                               const float* __restrict__ e,
-                                                               int64_t n)
+template <int OP>
+__global__ __launch_bounds__(kMgThreads) void k_vec(VecArgs a)
 {
-    const int64_t i = (int64_t)blockIdx.x * kMgThreads + threadIdx.x;
-    if (i < n) x[i] += (double)e[i];
+    vec_body<OP>(a, blockIdx.x);
+}
+
+template <int OP>
+__global__ __launch_bounds__(kMgThreads) void k_vec_many(const VecArgs* __restrict__ a,
+                                                        const unsigned* __restrict__ first, int n)
+{
+    const int i = batch_instance(first, n);
+    vec_body<OP>(a[i], blockIdx.x - first[i]);
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_zero_f32(float* __restrict__ x, int64_t n)
+{
+    vec_body<0>(VecArgs{x, nullptr, n}, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kMgThreads) void k_addback(double* __restrict__ b,
@@ -1532,6 +1753,16 @@ int vtail_lds_limit()
     return lim;
 }
 
+// kernel kinds of recorded ops (base + template variant bits)
+constexpr uint16_t kKindSmooth = 0;      // + (FP64 << 3 | compact << 2 | prolong << 1 | acc)
+constexpr uint16_t kKindRRTile = 16;     // + (FP64 << 1 | compact)
+constexpr uint16_t kKindRR = 20;         // + (FP64 << 1 | compact)
+constexpr uint16_t kKindCoarse = 24;     // + FP64
+constexpr uint16_t kKindOuter = 26;      // + (compact << 1 | emit FP32), 2D
+constexpr uint16_t kKindVec = 30;        // + op (0 zero, 1 add correction, 2 copy)
+constexpr uint16_t kKindHadamard = 33;   // FP64
+constexpr uint16_t kKindCount = 34;
+
 class HipMultigrid {
 public:
     explicit HipMultigrid(bool mixed) : mixed_(mixed) {}
@@ -1672,16 +1903,22 @@ public:
     // solve, used by the adjoint)
     void solve(double* b, double* x, bool sync = true, bool homogeneous = false)
     {
+        const double* bs = solve_begin(b, homogeneous);
+        solve_core(b, bs, x);
+        solve_end(b, x, sync);
+    }
+
+    // The three parts of solve(): begin (addback, replace, the consistent
+    // right-hand side of a pinned periodic system; returns the right-hand
+    // side the V-cycles see), core (the cycles) and end (gauge, sync).
+    const double* solve_begin(double* b, bool homogeneous = false)
+    {
         if (host_.empty()) throw std::runtime_error("hip_multigrid: solve before set_matrix");
         hipStream_t st = stream_;
         if (add_n_ && !homogeneous) hipLaunchKernelGGL(k_addback, dim3(nblk(add_n_)), dim3(kMgThreads), 0, st, b, add_idx_, add_val_, add_n_);
         if (rep_n_) hipLaunchKernelGGL(k_replace, dim3(nblk(rep_n_)), dim3(kMgThreads), 0, st, b, rep_idx_, rep_val_, rep_n_, homogeneous ? 1 : 0);
         MG_HIP(hipGetLastError());
-        last_cycles_ = 0;
-        last_rel_ = NAN;
-        const bool use_tol = tol_ > 0.0;
         const int64_t N = N0();
-        const double* bs = b;   // right-hand side the V-cycles see
         if (pin_ >= 0) {
             // consistent singular form: b' = b, b'_p = b_p - sum(b)
             MG_HIP(hipMemcpyAsync(bprime_, b, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1689,14 +1926,24 @@ public:
             hipLaunchKernelGGL(k_fsum, dim3(1), dim3(kMgThreads), 0, st, part_, (int64_t)nblk(N), shift_);
             hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(64), 0, st, bprime_, b, shift_, pin_);
             MG_HIP(hipGetLastError());
-            bs = bprime_;
+            return bprime_;
         }
+        return b;
+    }
+
+    void solve_core(const double* b, const double* bs, double* x)
+    {
+        last_cycles_ = 0;
+        last_rel_ = NAN;
+        rec_iter_ = 1;                 // iteration 0 is the residual before the first cycle
+        const bool use_tol = tol_ > 0.0;
         // With a tolerance, the first check comes after the number of cycles
         // the previous solve needed minus one (warm-started solves of a
         // time-stepping loop need similar counts); after that every cycle is
         // checked.  A check reads ||b - A x||^2 and ||b||^2 back to the host.
         const int first_check = std::max(1, std::min(max_cycles_, predicted_ - 1));
         if (!mixed_) {
+            rec_phase(0, kStageOuter, 0);
             scale_rhs_f64(bs);
             if (!use_tol) {
                 vcycles_f64(x, max_cycles_);
@@ -1717,6 +1964,7 @@ public:
             // refinement iteration: r = b - A x (FP64) -> FP32 V-cycles on A e = r -> x += e
             double rr, bb;
             const bool fm = fused_mixed();
+            rec_phase(0, kStageOuter, 0);
             outer_residual(bs, b, x, true, fm, false, false, &rr, &bb);
             for (int cycle = 0; cycle < max_cycles_;) {
                 if (use_tol && cycle >= first_check) {
@@ -1738,7 +1986,10 @@ public:
                 vcycles_f32(batch, fm, x);
                 cycle += batch;
                 last_cycles_ += batch;
-                if (cycle < max_cycles_) outer_residual(bs, b, x, true, fm, false, false, &rr, &bb);
+                if (cycle < max_cycles_) {
+                    rec_phase(rec_iter_ - 1, kStageOuter, 0);
+                    outer_residual(bs, b, x, true, fm, false, false, &rr, &bb);
+                }
             }
             if (use_tol) {
                 if (last_cycles_ == 0) outer_residual(bs, b, x, false, false, true, false, &rr, &bb);
@@ -1747,13 +1998,54 @@ public:
             }
         }
         if (use_tol) predicted_ = last_cycles_;
+    }
+
+    void solve_end(const double* b, double* x, bool sync)
+    {
+        hipStream_t st = stream_;
         if (pin_ >= 0) {
             hipLaunchKernelGGL(k_gauge_read, dim3(1), dim3(64), 0, st, x, b, pin_, shift_ + 1);
-            hipLaunchKernelGGL(k_gauge_apply, dim3(nblk(N)), dim3(kMgThreads), 0, st, x, N, shift_ + 1);
+            hipLaunchKernelGGL(k_gauge_apply, dim3(nblk(N0())), dim3(kMgThreads), 0, st, x, N0(), shift_ + 1);
             MG_HIP(hipGetLastError());
         }
         if (sync) MG_HIP(hipStreamSynchronize(st));
     }
+
+    // ---- batched fixed-cycle solves --------------------------------------
+    // An op of the core of a fixed-cycle solve, recorded instead of launched:
+    // the kernel body and variant (kind), its block count and its argument
+    // struct.  Ops of different solvers with equal keys are independent and
+    // run as one batched launch; keys grow along each solver's sequence.
+    using RecOp = MgRecOp;
+
+    // whether the core of a solve can be recorded: a fixed cycle count on a
+    // 2D hierarchy through the fused tile kernels, without the one-workgroup
+    // tail, with pre- and post-sweeps
+    bool recordable() const
+    {
+        if (host_.empty() || tol_ > 0.0 || host_[0].D != 2 || !fuse_ || pre_ < 1 || post_ < 1) return false;
+        for (const auto& h : host_)
+            if (h.D != 2) return false;
+        return tail_start<float>() < 0 && tail_start<double>() < 0;
+    }
+
+    // The ops of solve_core(b, bs, x), in order, without launching anything
+    // (solve_begin must have produced bs; solve_end follows the ops).
+    void record_core(const double* b, const double* bs, double* x, std::vector<RecOp>& ops)
+    {
+        if (!recordable()) throw std::logic_error("hip_multigrid: this solve cannot be batched");
+        ops.clear();
+        rec_ = &ops;
+        try {
+            solve_core(b, bs, x);
+        } catch (...) {
+            rec_ = nullptr;
+            throw;
+        }
+        rec_ = nullptr;
+    }
+
+    double* pin_rhs() const { return pin_ >= 0 ? bprime_ : nullptr; }
 
     // fixed V-cycle count, one per refinement iteration, no host checks and
     // no graphs of its own: the solve can be captured into a caller's graph
@@ -1776,6 +2068,84 @@ public:
     int levels() const { return (int)host_.size(); }
 
 private:
+    // ---- recording of batched solves (see record_core)
+    static constexpr uint32_t kStageInit = 0, kStageDown = 1, kStageCoarse = 2, kStageUp = 3,
+                              kStageCorr = 4, kStageOuter = 5;
+    std::vector<RecOp>* rec_ = nullptr;
+    uint32_t rec_iter_ = 0, rec_key_ = 0, rec_seq_ = 0;
+
+    // key of the ops that follow: iteration, stage, level (ascending for the
+    // downward pass, descending for the upward one), then emission order
+    void rec_phase(uint32_t it, uint32_t stage, uint32_t lvl)
+    {
+        rec_key_ = (it << 20) | (stage << 16) | (lvl << 8);
+        rec_seq_ = 0;
+    }
+
+    template <typename A>
+    void emit(uint16_t kind, unsigned nb, const A& a)
+    {
+        RecOp op;
+        op.key = rec_key_ | (rec_seq_ & 0xffu);
+        ++rec_seq_;
+        op.kind = kind;
+        op.nblocks = nb;
+        op.args.resize(sizeof(A));
+        std::memcpy(op.args.data(), &a, sizeof(A));
+        rec_->push_back(std::move(op));
+    }
+
+    template <typename T>
+    static constexpr int tbit() { return std::is_same<T, double>::value ? 1 : 0; }
+
+    template <typename T, bool C, bool P, bool A>
+    void run_smooth_tile(unsigned nb, const SmoothTileArgs<T>& a)
+    {
+        if (rec_) { emit((uint16_t)(kKindSmooth + ((tbit<T>() << 3) | (C << 2) | (P << 1) | (A ? 1 : 0))), nb, a); return; }
+        hipLaunchKernelGGL((k_smooth_tile<T, C, P, A>), dim3(nb), dim3(kTileThreads), 0, stream_, a);
+    }
+
+    template <typename T, bool C, bool TILE>
+    void run_rr(unsigned nb, const ResidRestrictArgs<T>& a)
+    {
+        if (rec_) { emit((uint16_t)((TILE ? kKindRRTile : kKindRR) + ((tbit<T>() << 1) | (C ? 1 : 0))), nb, a); return; }
+        if (TILE) hipLaunchKernelGGL((k_resid_restrict_tile<T, C>), dim3(nb), dim3(kTileThreads), 0, stream_, a);
+        else hipLaunchKernelGGL((k_resid_restrict<T, C>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
+    }
+
+    template <typename T>
+    void run_coarse(unsigned nb, const CoarseArgs<T>& a)
+    {
+        if (rec_) { emit((uint16_t)(kKindCoarse + tbit<T>()), nb, a); return; }
+        hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
+    }
+
+    template <int D, bool C, bool F32>
+    void run_outer(unsigned nb, const OuterArgs& a)
+    {
+        if (rec_) {
+            if (D != 2) throw std::logic_error("hip_multigrid: only 2D solves are batched");
+            emit((uint16_t)(kKindOuter + ((C ? 2 : 0) | (F32 ? 1 : 0))), nb, a);
+            return;
+        }
+        hipLaunchKernelGGL((k_outer_residual<D, C, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
+    }
+
+    template <int OP>
+    void run_vec(const VecArgs& a)
+    {
+        const unsigned nb = nblk(a.n);
+        if (rec_) { emit((uint16_t)(kKindVec + OP), nb, a); return; }
+        hipLaunchKernelGGL((k_vec<OP>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
+    }
+
+    void run_hadamard(const HadamardArgs<double>& a)
+    {
+        const unsigned nb = nblk(a.L.N);
+        if (rec_) { emit((uint16_t)kKindHadamard, nb, a); return; }
+        hipLaunchKernelGGL((k_hadamard<double>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
+    }
+
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     int fuse_ = 1;   // 2D: fused multi-sweep tile smoothing and residual+restriction
@@ -2009,7 +2379,8 @@ private:
             const int ostep = kTileX - 2 * k;
             const unsigned nb = (unsigned)(((L.n[0] + kTileRows - 1) / kTileRows) * ((L.n[1] + ostep - 1) / ostep));
             double* a = (k == count) ? acc : nullptr;
-#define TILE(CMP, PRO, ACC) hipLaunchKernelGGL((k_smooth_tile<T, CMP, PRO, ACC>), dim3(nb), dim3(kTileThreads), 0, stream_, L, table<T>(l, true), ncls, x, b, buf, two3, k, C, tr, xc, a)
+            const SmoothTileArgs<T> ta{L, table<T>(l, true), ncls, x, b, buf, two3, k, C, tr, xc, a};
+#define TILE(CMP, PRO, ACC) run_smooth_tile<T, CMP, PRO, ACC>(nb, ta)
             if (a) {
                 if (xc) { if (cmp) TILE(true, true, true); else TILE(false, true, true); }
                 else { if (cmp) TILE(true, false, true); else TILE(false, false, true); }
@@ -2218,7 +2589,9 @@ private:
         const int lt = tail_start<T>();          // > 0: levels lt..Lc run in k_vtail
         const int ld = lt > 0 ? lt : Lc;
         for (int cyc = 0; cyc < cycles; ++cyc) {
+            const uint32_t it = rec_iter_ + (uint32_t)cyc;
             for (int l = 0; l < ld; ++l) {
+                rec_phase(it, kStageDown, (uint32_t)l);
                 // coarse levels received their first sweep from the restriction
                 const bool swept = (l > 0 || (cyc == 0 && first_swept)) && pre_ > 0;
                 smooth_n<T>(l, xs[l], ts[l], bs[l], swept ? pre_ - 1 : pre_);
@@ -2228,14 +2601,14 @@ private:
                     if (rr_tile_fits(l)) {
                         const unsigned nbt = (unsigned)(((C.n[0] + kRRRows - 1) / kRRRows) * ((C.n[1] + kRRCols - 1) / kRRCols));
                         const int ncls = (int)host_[l].dinv.size();
-                        if (host_[l].compact)
-                            hipLaunchKernelGGL((k_resid_restrict_tile<T, true>), dim3(nbt), dim3(kTileThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), ncls, table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
-                        else
-                            hipLaunchKernelGGL((k_resid_restrict_tile<T, false>), dim3(nbt), dim3(kTileThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), ncls, table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
-                    } else if (host_[l].compact)
-                        hipLaunchKernelGGL((k_resid_restrict<T, true>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
-                    else
-                        hipLaunchKernelGGL((k_resid_restrict<T, false>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l, false), table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs);
+                        const ResidRestrictArgs<T> ra{F, C, transfer<T>(l), table<T>(l, false), ncls, table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs};
+                        if (host_[l].compact) run_rr<T, true, true>(nbt, ra);
+                        else run_rr<T, false, true>(nbt, ra);
+                    } else {
+                        const ResidRestrictArgs<T> ra{F, C, transfer<T>(l), table<T>(l, false), 0, table<T>(l + 1, false), xs[l], bs[l], bs[l + 1], xs[l + 1], fs};
+                        if (host_[l].compact) run_rr<T, true, false>(nblk(C.N), ra);
+                        else run_rr<T, false, false>(nblk(C.N), ra);
+                    }
                     continue;
                 }
                 residual<T>(l, xs[l], bs[l], ts[l]);
@@ -2244,14 +2617,16 @@ private:
                 else
                     hipLaunchKernelGGL((k_restrict<T, 3>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
             }
+            rec_phase(it, kStageCoarse, 0);
             if (lt > 0) {
                 launch_tail<T>(lt, xs[lt], bs[lt]);
             } else {
                 const int n = (int)host_[Lc].N;
                 const unsigned nb = (unsigned)(((int64_t)n * 64 + kMgThreads - 1) / kMgThreads);
-                hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nb), dim3(kMgThreads), 0, stream_, cinv_, n, bs[Lc], xs[Lc]);
+                run_coarse<T>(nb, CoarseArgs<T>{cinv_, n, bs[Lc], xs[Lc]});
             }
             for (int l = ld - 1; l >= 0; --l) {
+                rec_phase(it, kStageUp, 255u - (uint32_t)l);
                 if (post_ > 0) {
                     double* a = (l == 0 && cyc == cycles - 1) ? acc : nullptr;
                     accd = smooth_n<T>(l, xs[l], ts[l], bs[l], post_, xs[l + 1], a) || accd;
@@ -2260,6 +2635,7 @@ private:
                 }
             }
         }
+        rec_iter_ += (uint32_t)cycles;
         MG_HIP(hipGetLastError());
         return accd;
     }
@@ -2267,7 +2643,7 @@ private:
     // b0 = b o D0^-1 into dev_[0].b (once per solve: the V-cycles read it)
     void scale_rhs_f64(const double* b)
     {
-        hipLaunchKernelGGL(k_hadamard<double>, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, dl(0), table<double>(0, false), b, dev_[0].b);
+        run_hadamard(HadamardArgs<double>{dl(0), table<double>(0, false), b, dev_[0].b});
     }
 
     // FP64 V-cycles on x with the scaled right-hand side in dev_[0].b
@@ -2281,7 +2657,8 @@ private:
             bs[l] = dev_[l].b;
         }
         vcycles<double>(xs, ts, bs, cycles);
-        if (xs[0] != x) MG_HIP(hipMemcpyAsync(x, xs[0], (size_t)N0() * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        rec_phase(rec_iter_ - 1, kStageCorr, 0);
+        if (xs[0] != x) run_vec<2>(VecArgs{x, xs[0], N0()});
     }
 
     // The mixed iteration fuses work into its neighbours when the level-0
@@ -2303,11 +2680,13 @@ private:
             ts[l] = dev_[l].tf;
             bs[l] = dev_[l].bf;
         }
+        rec_phase(rec_iter_, kStageInit, 0);
         if (!first_swept)
-            hipLaunchKernelGGL(k_zero_f32, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, xs[0], N0());
+            run_vec<0>(VecArgs{xs[0], nullptr, N0()});
         const bool accd = vcycles<float>(xs, ts, bs, cycles, first_swept, x);
+        rec_phase(rec_iter_ - 1, kStageCorr, 0);
         if (x && !accd) {
-            hipLaunchKernelGGL(k_add_correction, dim3(nblk(N0())), dim3(kMgThreads), 0, stream_, x, xs[0], N0());
+            run_vec<1>(VecArgs{x, xs[0], N0()});
         } else if (!x && xs[0] != dev_[0].xf) {
             MG_HIP(hipMemcpyAsync(dev_[0].xf, xs[0], (size_t)N0() * sizeof(float), hipMemcpyDeviceToDevice, stream_));
         }
@@ -2400,7 +2779,8 @@ private:
         float* b0 = emit_f32 ? dev_[0].bf : nullptr;
         const float* d32 = dev_[0].dinv32;
         float* e0 = (emit_f32 && first_sweep) ? dev_[0].xf : nullptr;
-#define OUTER(DIM, CMP, F32) hipLaunchKernelGGL((k_outer_residual<DIM, CMP, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, L, K, x, b, bn, pin_, d32, b0, part_, e0)
+        const OuterArgs oa{L, K, x, b, bn, pin_, d32, b0, part_, e0};
+#define OUTER(DIM, CMP, F32) run_outer<DIM, CMP, F32>(nb, oa)
         if (host_[0].D == 2) {
             if (c) { if (emit_f32) OUTER(2, true, true); else OUTER(2, true, false); }
             else { if (emit_f32) OUTER(2, false, true); else OUTER(2, false, false); }
@@ -2424,6 +2804,159 @@ private:
         *bb = red_host_[1];
     }
 };
+
+// ---- batched launches of recorded solves -----------------------------------
+// Ops of all solvers are ordered by (key, kind, solver); each run of equal
+// (key, kind) becomes one launch of the kind's batched kernel over the
+// concatenated blocks of its solvers.  Every solver's ops keep their own
+// order (keys grow along each solver's sequence), and ops sharing a launch
+// belong to different, independent solvers.
+struct MgBatchGroup {
+    uint16_t kind;
+    unsigned blocks;
+    int n;
+    size_t args_off, first_off;
+};
+
+struct MgBatchPlan {
+    std::vector<MgBatchGroup> groups;
+    void* dev = nullptr;
+    size_t bytes = 0;
+    int64_t ops = 0;
+    ~MgBatchPlan() { release(); }
+    void release()
+    {
+        if (dev) (void)hipFree(dev);
+        dev = nullptr;
+        bytes = 0;
+        groups.clear();
+        ops = 0;
+    }
+};
+
+void mg_batch_build(const std::vector<const std::vector<MgRecOp>*>& per, MgBatchPlan& plan)
+{
+    plan.release();
+    struct E {
+        uint32_t key;
+        uint16_t kind;
+        int inst;
+        const MgRecOp* op;
+    };
+    std::vector<E> all;
+    for (int i = 0; i < (int)per.size(); ++i)
+        for (const auto& op : *per[i]) {
+            if (op.kind >= kKindCount) throw std::logic_error("hip_multigrid: unknown recorded kind");
+            all.push_back(E{op.key, op.kind, i, &op});
+        }
+    std::stable_sort(all.begin(), all.end(), [](const E& a, const E& b) {
+        if (a.key != b.key) return a.key < b.key;
+        if (a.kind != b.kind) return a.kind < b.kind;
+        return a.inst < b.inst;
+    });
+    std::vector<unsigned char> host;
+    auto align16 = [&]() { host.resize((host.size() + 15) & ~(size_t)15, 0); };
+    for (size_t a = 0; a < all.size();) {
+        size_t b = a;
+        while (b < all.size() && all[b].key == all[a].key && all[b].kind == all[a].kind) ++b;
+        MgBatchGroup g{};
+        g.kind = all[a].kind;
+        g.n = (int)(b - a);
+        const size_t asz = all[a].op->args.size();
+        align16();
+        g.args_off = host.size();
+        for (size_t k = a; k < b; ++k) {
+            if (all[k].op->args.size() != asz) throw std::logic_error("hip_multigrid: argument size mismatch");
+            host.insert(host.end(), all[k].op->args.begin(), all[k].op->args.end());
+        }
+        align16();
+        g.first_off = host.size();
+        uint64_t blocks = 0;
+        for (size_t k = a; k < b; ++k) {
+            const unsigned f = (unsigned)blocks;
+            const unsigned char* q = reinterpret_cast<const unsigned char*>(&f);
+            host.insert(host.end(), q, q + sizeof(unsigned));
+            blocks += all[k].op->nblocks;
+        }
+        if (blocks > 0x7fffffffull) throw std::runtime_error("hip_multigrid: batched launch too large");
+        g.blocks = (unsigned)blocks;
+        if (g.blocks > 0) plan.groups.push_back(g);
+        a = b;
+    }
+    plan.ops = (int64_t)all.size();
+    plan.bytes = std::max<size_t>(host.size(), 16);
+    MG_HIP(hipMalloc(&plan.dev, plan.bytes));
+    if (!host.empty()) MG_HIP(hipMemcpy(plan.dev, host.data(), host.size(), hipMemcpyHostToDevice));
+}
+
+void mg_batch_launch(const MgBatchPlan& plan, hipStream_t st)
+{
+    const unsigned char* base = static_cast<const unsigned char*>(plan.dev);
+    for (const auto& g : plan.groups) {
+        const void* a = base + g.args_off;
+        const unsigned* f = reinterpret_cast<const unsigned*>(base + g.first_off);
+        const dim3 grid(g.blocks);
+        const uint16_t k = g.kind;
+        if (k < kKindRRTile) {
+            const int v = k - kKindSmooth;
+#define SMT(T, C, P, A)                                                                             \
+    hipLaunchKernelGGL((k_smooth_tile_many<T, C, P, A>), grid, dim3(kTileThreads), 0, st,           \
+                       static_cast<const SmoothTileArgs<T>*>(a), f, g.n)
+#define SMT4(T, C)                                            \
+    switch (v & 3) {                                          \
+    case 0: SMT(T, C, false, false); break;                   \
+    case 1: SMT(T, C, false, true); break;                    \
+    case 2: SMT(T, C, true, false); break;                    \
+    default: SMT(T, C, true, true); break;                    \
+    }
+            if (v & 8) { if (v & 4) { SMT4(double, true) } else { SMT4(double, false) } }
+            else { if (v & 4) { SMT4(float, true) } else { SMT4(float, false) } }
+#undef SMT4
+#undef SMT
+        } else if (k < kKindCoarse) {
+            const bool tile = k < kKindRR;
+            const int v = k - (tile ? kKindRRTile : kKindRR);
+#define RRL(T, C)                                                                                   \
+    if (tile) hipLaunchKernelGGL((k_resid_restrict_tile_many<T, C>), grid, dim3(kTileThreads), 0, st, \
+                                 static_cast<const ResidRestrictArgs<T>*>(a), f, g.n);              \
+    else hipLaunchKernelGGL((k_resid_restrict_many<T, C>), grid, dim3(kMgThreads), 0, st,            \
+                            static_cast<const ResidRestrictArgs<T>*>(a), f, g.n)
+            switch (v) {
+            case 0: RRL(float, false); break;
+            case 1: RRL(float, true); break;
+            case 2: RRL(double, false); break;
+            default: RRL(double, true); break;
+            }
+#undef RRL
+        } else if (k < kKindOuter) {
+            if (k - kKindCoarse)
+                hipLaunchKernelGGL((k_coarse_solve_many<double>), grid, dim3(kMgThreads), 0, st,
+                                   static_cast<const CoarseArgs<double>*>(a), f, g.n);
+            else
+                hipLaunchKernelGGL((k_coarse_solve_many<float>), grid, dim3(kMgThreads), 0, st,
+                                   static_cast<const CoarseArgs<float>*>(a), f, g.n);
+        } else if (k < kKindVec) {
+            const OuterArgs* oa = static_cast<const OuterArgs*>(a);
+            switch (k - kKindOuter) {
+            case 0: hipLaunchKernelGGL((k_outer_residual_many<2, false, false>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
+            case 1: hipLaunchKernelGGL((k_outer_residual_many<2, false, true>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
+            case 2: hipLaunchKernelGGL((k_outer_residual_many<2, true, false>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
+            default: hipLaunchKernelGGL((k_outer_residual_many<2, true, true>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
+            }
+        } else if (k < kKindHadamard) {
+            const VecArgs* va = static_cast<const VecArgs*>(a);
+            switch (k - kKindVec) {
+            case 0: hipLaunchKernelGGL((k_vec_many<0>), grid, dim3(kMgThreads), 0, st, va, f, g.n); break;
+            case 1: hipLaunchKernelGGL((k_vec_many<1>), grid, dim3(kMgThreads), 0, st, va, f, g.n); break;
+            default: hipLaunchKernelGGL((k_vec_many<2>), grid, dim3(kMgThreads), 0, st, va, f, g.n); break;
+            }
+        } else {
+            hipLaunchKernelGGL((k_hadamard_many<double>), grid, dim3(kMgThreads), 0, st,
+                               static_cast<const HadamardArgs<double>*>(a), f, g.n);
+        }
+    }
+    MG_HIP(hipGetLastError());
+}
 
 }  // namespace psum_hip
 
@@ -2525,6 +3058,29 @@ void mg_set_fixed_cycles(HipMultigrid* h, int cycles) { h->set_fixed_cycles(cycl
 void mg_set_predicted(HipMultigrid* h, int p) { h->set_predicted(p); }
 int mg_last_cycles(HipMultigrid* h) { return h->last_cycles(); }
 double mg_last_rel(HipMultigrid* h) { return h->last_rel(); }
+const double* mg_solve_begin(HipMultigrid* h, double* b) { return h->solve_begin(b); }
+void mg_solve_core(HipMultigrid* h, const double* b, const double* bs, double* x) { h->solve_core(b, bs, x); }
+void mg_solve_end(HipMultigrid* h, const double* b, double* x) { h->solve_end(b, x, false); }
+bool mg_recordable(HipMultigrid* h) { return h->recordable(); }
+const double* mg_core_rhs(HipMultigrid* h, double* b)
+{
+    double* p = h->pin_rhs();
+    return p ? p : b;
+}
+void mg_record_core(HipMultigrid* h, const double* b, const double* bs, double* x,
+                    std::vector<MgRecOp>& ops)
+{
+    h->record_core(b, bs, x, ops);
+}
+MgBatchPlan* mg_batch_plan_create() { return new MgBatchPlan(); }
+void mg_batch_plan_destroy(MgBatchPlan* p) { delete p; }
+void mg_batch_plan_build(MgBatchPlan* p, const std::vector<const std::vector<MgRecOp>*>& per)
+{
+    mg_batch_build(per, *p);
+}
+void mg_batch_plan_launch(const MgBatchPlan* p, hipStream_t st) { mg_batch_launch(*p, st); }
+int64_t mg_batch_plan_launches(const MgBatchPlan* p) { return (int64_t)p->groups.size(); }
+int64_t mg_batch_plan_ops(const MgBatchPlan* p) { return p->ops; }
 
 }  // namespace psum_hip
 

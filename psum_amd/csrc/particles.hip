@@ -820,10 +820,20 @@ __global__ __launch_bounds__(kThreads) void k_field_energy(DevGrid g, const doub
     double acc = 0.0;
     if (i < nn) {
         int64_t idx[3] = {0, 0, 0};
-        int64_t rem = i;
-        for (int d = D - 1; d >= 0; --d) {
-            idx[d] = rem % g.w[d];
-            rem /= g.w[d];
+        if (nn <= (int64_t)0xffffffffll) {
+            // 32-bit unsigned division (a 64-bit one is a long software sequence)
+            uint32_t rem = (uint32_t)i;
+            for (int d = D - 1; d >= 0; --d) {
+                const uint32_t q = rem / (uint32_t)g.w[d];
+                idx[d] = rem - q * (uint32_t)g.w[d];
+                rem = q;
+            }
+        } else {
+            int64_t rem = i;
+            for (int d = D - 1; d >= 0; --d) {
+                idx[d] = rem % g.w[d];
+                rem /= g.w[d];
+            }
         }
 #pragma unroll
         for (int d = 0; d < D; ++d) {

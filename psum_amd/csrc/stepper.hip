@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "mg_batch.hpp"
 #include "psum/mas.hpp"
 
 namespace psum_hip {
@@ -377,9 +378,15 @@ public:
     // go to the device-side row counter.
     void step_body(bool diagnostics)
     {
-        const int D = cfg_.grid.dim;
+        solve_field();
+        step_after_solve(diagnostics);
+    }
+
+    // the rest of a step once phi is solved: field energy, gather + push +
+    // deposit of every species, diagnostics
+    void step_after_solve(bool diagnostics)
+    {
         {
-            solve_field();
             if (diagnostics) CHK(psum_hip_field_energy(&cfg_.grid, phi_, cfg_.eps, fe_, few_, few_bytes_, st_));
             const void* ph = cfg_.dtype == PSUM_F32 ? (const void*)phi32_ : (const void*)phi_;
             hipLaunchKernelGGL(k_zero_i64, dim3(nblk(nn_)), dim3(kT), 0, st_, (long long*)acc_, nn_);
@@ -414,7 +421,6 @@ public:
                                    (long long*)hist_count_, (long long)hist_cap_);
             S_TRY(hipGetLastError());
             ++step_;
-            (void)D;
         }
     }
 
@@ -442,6 +448,17 @@ public:
     // then phi by the multigrid (warm start); phi32 refreshed in FP32 mode
     void solve_field()
     {
+        const double* bs = solve_field_begin();
+        mg_solve_core(mg_, b_, bs, phi_);
+        solve_field_end();
+    }
+
+    // solve_field in three parts, so that a batch can run the solver cores of
+    // several instances as batched launches: begin (rho from the deposit,
+    // neutralisation, extrapolation, the solver's replace / pin; returns the
+    // right-hand side the cycles see) and end (gauge, bookkeeping, FP32 copy)
+    const double* solve_field_begin()
+    {
         if (!acc_valid_) deposit_all();
         CHK(psum_hip_fixed_to_double(acc_, nn_, 1.0 / scale_, cfg_.rho_bg, b_, st_));
         if (all_periodic_) {
@@ -451,7 +468,12 @@ public:
         }
         if (cfg_.extrapolate && have_prev_)
             hipLaunchKernelGGL(k_extrapolate, dim3(nblk(nn_)), dim3(kT), 0, st_, phi_, phi_prev_, nn_);
-        mg_solve(mg_, b_, phi_, false);
+        return mg_solve_begin(mg_, b_);
+    }
+
+    void solve_field_end()
+    {
+        mg_solve_end(mg_, b_, phi_);
         if (cfg_.extrapolate && !have_prev_) {
             S_TRY(hipMemcpyAsync(phi_prev_, phi_, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
             have_prev_ = true;
@@ -460,6 +482,13 @@ public:
         max_cyc_seen_ = std::max(max_cyc_seen_, mg_last_cycles(mg_));
         last_rel_ = mg_last_rel(mg_);
         if (cfg_.dtype == PSUM_F32) CHK(psum_hip_f64_to_f32(phi_, phi32_, nn_, st_));
+    }
+
+    // batch hooks: whether the solve's core can be recorded, and its ops
+    bool solve_recordable() const { return mg_recordable(mg_); }
+    void record_solve_core(std::vector<MgRecOp>& ops)
+    {
+        mg_record_core(mg_, b_, mg_core_rhs(mg_, b_), phi_, ops);
     }
 
     // Loss L = field energy of phi solved from the current positions.
@@ -1235,6 +1264,10 @@ public:
         S_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
         join_.resize(n);
         for (auto& e : join_) S_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        join2_.resize(n);
+        for (auto& e : join2_) S_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        S_TRY(hipEventCreateWithFlags(&fork2_, hipEventDisableTiming));
+        plan_ = mg_batch_plan_create();
         S_TRY(hipEventCreate(&t0_));
         S_TRY(hipEventCreate(&t1_));
     }
@@ -1243,11 +1276,29 @@ public:
     {
         release_lanes();
         for (auto& e : join_) if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {fork_, t0_, t1_}) if (e) (void)hipEventDestroy(e);
+        for (auto& e : join2_) if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {fork_, fork2_, t0_, t1_}) if (e) (void)hipEventDestroy(e);
+        if (plan_) mg_batch_plan_destroy(plan_);
         if (origin_) (void)hipStreamDestroy(origin_);
     }
 
     int size() const { return (int)inst_.size(); }
+    // batched solver cores on (default) or off: off, each instance's solver
+    // launches its own kernels inside the captured step
+    void set_solver_batching(bool on)
+    {
+        batch_mg_ = on;
+        drop();
+    }
+    // graph launches per step of the batched solver cores (0 when the step
+    // does not batch them), and the ops they replace
+    void solver_launches(int64_t* launches, int64_t* ops)
+    {
+        if (!captured_) capture();
+        const bool b = batched_solve();
+        if (launches) *launches = b ? mg_batch_plan_launches(plan_) : 0;
+        if (ops) *ops = b ? mg_batch_plan_ops(plan_) : 0;
+    }
     void set_lanes(int lanes)
     {
         if (lanes < 1 || lanes > 64) throw std::invalid_argument("psum_batch_set_lanes: lanes must be 1..64");
@@ -1377,6 +1428,13 @@ private:
     std::vector<hipGraphExec_t> lane_exec_;
     bool captured_ = false;
     int64_t captured_epoch_ = 0;     // sum of the instances' tuning epochs at capture
+    // batched solver cores (one lane): the recorded ops of each instance and
+    // their merged launches; batch_mg_ = 0 steps each solver on its own
+    bool batch_mg_ = true;
+    std::vector<std::vector<MgRecOp>> rec_ops_;
+    MgBatchPlan* plan_ = nullptr;
+    hipEvent_t fork2_ = nullptr;
+    std::vector<hipEvent_t> join2_;
 
     void drop()
     {
@@ -1426,11 +1484,71 @@ private:
         }
     }
 
+    // whether one step runs the instances' solver cores as batched launches:
+    // one lane and every instance's solve recordable
+    bool batched_solve() const
+    {
+        if (lanes_ != 1 || !batch_mg_) return false;
+        for (auto& p : inst_)
+            if (!p->solve_recordable()) return false;
+        return true;
+    }
+
+    // One step of every instance, captured on the single lane's stream: each
+    // instance's part before its solve on its own stream (joined back), the
+    // batched solver cores of all instances on the lane stream, then each
+    // instance's part after its solve.
+    void capture_batched()
+    {
+        rec_ops_.assign(inst_.size(), {});
+        std::vector<const std::vector<MgRecOp>*> per;
+        for (size_t i = 0; i < inst_.size(); ++i) {
+            inst_[i]->record_solve_core(rec_ops_[i]);
+            per.push_back(&rec_ops_[i]);
+        }
+        mg_batch_plan_build(plan_, per);
+        lane_exec_.assign(1, nullptr);
+        hipStream_t ls = lane_st_[0];
+        S_TRY(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+        // an exception below ends the capture and drops the partial graph
+        CaptureGuard guard(ls);
+        S_TRY(hipEventRecord(lane_fork_[0], ls));
+        for (size_t i = 0; i < inst_.size(); ++i) {
+            Stepper& st = *inst_[i];
+            S_TRY(hipStreamWaitEvent(st.stream(), lane_fork_[0], 0));
+            (void)st.solve_field_begin();
+            S_TRY(hipEventRecord(join_[i], st.stream()));
+            S_TRY(hipStreamWaitEvent(ls, join_[i], 0));
+        }
+        mg_batch_plan_launch(plan_, ls);
+        S_TRY(hipEventRecord(fork2_, ls));
+        for (size_t i = 0; i < inst_.size(); ++i) {
+            Stepper& st = *inst_[i];
+            S_TRY(hipStreamWaitEvent(st.stream(), fork2_, 0));
+            st.solve_field_end();
+            st.step_after_solve(true);
+            S_TRY(hipEventRecord(join2_[i], st.stream()));
+            S_TRY(hipStreamWaitEvent(ls, join2_[i], 0));
+        }
+        hipGraph_t graph = nullptr;
+        guard.release();
+        S_TRY(hipStreamEndCapture(ls, &graph));
+        const hipError_t e = hipGraphInstantiate(&lane_exec_[0], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        S_TRY(e);
+    }
+
     void capture()
     {
         if (lane_st_.empty()) plan_lanes();
         for (auto& p : inst_) p->prepare_capture();
         for (auto& p : inst_) S_TRY(hipStreamSynchronize(p->stream()));
+        if (batched_solve()) {
+            capture_batched();
+            captured_ = true;
+            for (auto& p : inst_) p->uncount_step();
+            return;
+        }
         lane_exec_.assign(lane_st_.size(), nullptr);
         for (size_t l = 0; l < lane_st_.size(); ++l) {
             hipStream_t ls = lane_st_[l];
@@ -1687,6 +1805,18 @@ psum_status psum_batch_set_lanes(void* h, int lanes)
 {
     if (!h) { set_error("null batch"); return PSUM_ERR_ARG; }
     return guarded([&]() { static_cast<Batch*>(h)->set_lanes(lanes); });
+}
+
+psum_status psum_batch_set_solver_batching(void* h, int on)
+{
+    if (!h) { set_error("psum_batch_set_solver_batching: null batch"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Batch*>(h)->set_solver_batching(on != 0); });
+}
+
+psum_status psum_batch_solver_launches(void* h, int64_t* launches, int64_t* ops)
+{
+    if (!h) { set_error("psum_batch_solver_launches: null batch"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Batch*>(h)->solver_launches(launches, ops); });
 }
 
 psum_status psum_batch_run(void* h, int steps, double* t_ms)

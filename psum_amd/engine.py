@@ -468,6 +468,20 @@ class Batch:
         """Replay the step as `lanes` per-group graphs on concurrent streams."""
         check(lib().psum_batch_set_lanes(self.h, lanes), "psum_batch_set_lanes")
 
+    def set_solver_batching(self, on=True):
+        """Run the solver cycles of all instances as batched launches (the
+        default; needs one lane and 2D fixed-cycle solves) or per instance."""
+        check(lib().psum_batch_set_solver_batching(self.h, int(bool(on))),
+              "psum_batch_set_solver_batching")
+
+    def solver_launches(self):
+        """(batched solver launches per step, per-instance launches they
+        replace); (0, 0) when the step does not batch its solves"""
+        a, b = C.c_int64(0), C.c_int64(0)
+        check(lib().psum_batch_solver_launches(self.h, C.byref(a), C.byref(b)),
+              "psum_batch_solver_launches")
+        return a.value, b.value
+
     def run(self, steps):
         """`steps` graph replays; returns the HIP-event milliseconds."""
         ms = C.c_double(0)

@@ -127,3 +127,47 @@ def test_batch_with_vcycle_tail_matches_instances_stepped_alone(dtype):
         assert np.array_equal(h, hs[0]), (i, np.abs(h - hs[0]).max())
         assert np.array_equal(hs[0], hs[1]), (i, np.abs(hs[0] - hs[1]).max())
     b.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_batched_solver_launches_match_per_instance_launches(dtype):
+    """With solver batching (the default with one lane), every phase of the
+    instances' V-cycles runs as one launch over all instances; the results
+    equal, bit for bit, the step with each solver launching its own kernels
+    and each instance stepped alone."""
+    from psum_amd import engine as E
+    specs = [_instance(32, 16, 1, 0.05, 21, dtype), _instance(64, 4, 3, 0.02, 22, dtype),
+             _instance(48, 9, 2, 0.01, 23, dtype), _instance(128, 2, 4, 0.03, 24, dtype)]
+    sched = [(6, 5, 1), (7, 0, 0), (5, 3, 2), (4, 2, 1)]
+    steps = 9
+    hists = []
+    for batching in (True, False):
+        b = E.Batch(specs)
+        for i, (c, k, sh) in enumerate(sched):
+            b.configure(i, c, k, sh)
+        b.set_lanes(1)
+        b.set_solver_batching(batching)
+        b.reset_history(steps)
+        b.run(steps)
+        launches, ops = b.solver_launches()
+        if batching:
+            # one launch per phase instead of one per instance and phase
+            assert 0 < launches < ops
+        else:
+            assert launches == 0
+        hists.append([b.history(i) for i in range(len(specs))])
+        if batching:
+            states = [b.instances[i].state(0) for i in range(len(specs))]
+        b.close()
+    for i, spec in enumerate(specs):
+        assert np.array_equal(hists[0][i], hists[1][i]), i
+        a = dict(spec)
+        g, species = a.pop("g"), a.pop("species")
+        st = E.Stepper(g, species, **a)
+        st.set_schedule(*sched[i])
+        h_ref = st.run(steps)
+        assert np.array_equal(hists[0][i], h_ref), i
+        p_ref, v_ref = st.state(0)
+        for x, y in zip(states[i][0] + states[i][1], p_ref + v_ref):
+            assert np.array_equal(x, y), i
+        st.close()
