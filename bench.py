@@ -385,7 +385,8 @@ def c3_workload(cells, ppc):
     return bounded_3d(cells=cells, ppc=ppc, device="cuda")
 
 
-def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_probe=20):
+def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_probe=20,
+           extrapolate=False):
     """3D bounded box (phi = 0 on the faces, absorbing walls, compaction by
     the sort), electrons + ions (mass ratio 1836, Te/Ti = 10), 7-point MG.
     dtype "f32": FP32 particles with FP32 V-cycles in FP64 refinement to
@@ -399,7 +400,7 @@ def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_pr
     n = sum(len(s["pos"][0]) for s in species)
     st = E.Stepper(wl["grid"], species, dtype=dtype, mixed_mg=(dtype == "f32"), depth=depth,
                    pre=pre, post=post, max_cycles=40, tol=1e-8, dt=wl["dt"], rho_bg=0.0,
-                   sort_every=20, fixed=wl["fixed"])
+                   sort_every=20, fixed=wl["fixed"], extrapolate=extrapolate)
     del species
     torch.cuda.empty_cache()
     log(f"[bench] c3 {dtype}: {n} particles, set up in {time.time() - t0:.1f}s")
@@ -467,6 +468,8 @@ def main():
     ap.add_argument("--c3-ppc", type=int, default=32)
     ap.add_argument("--c3-depth", type=int, default=6)
     ap.add_argument("--c3-no-fp64", action="store_true")
+    ap.add_argument("--c3-extrapolate", action="store_true",
+                    help="warm-start each 3D solve from 2 phi_n - phi_(n-1)")
     ap.add_argument("--c3-schedule", default=None,
                     help="'every,shift': sort schedule of the 3D box instead of the autotuner's")
     ap.add_argument("--grid", type=int, default=1024, help="2D two-stream grid")
@@ -518,7 +521,7 @@ def main():
     log(f"[bench] c3 workload {wl3['name']}: generated in {time.time() - t0:.1f}s")
     fixed_sched = None if args.c3_schedule is None else tuple(int(v) for v in args.c3_schedule.split(","))
     c3, sched = run_c3(E, wl3, "f32", args.steps, args.warmup, args.c3_depth, args.pre, args.post,
-                       schedule=fixed_sched)
+                       schedule=fixed_sched, extrapolate=args.c3_extrapolate)
     if args.calib_copy:
         nb = int(c3["particles_per_launch"]) * 2 * 3
         src = torch.ones(nb, dtype=torch.float32, device="cuda")
@@ -530,7 +533,7 @@ def main():
     c3_64 = None
     if not args.c3_no_fp64:
         c3_64, _ = run_c3(E, wl3, "f64", args.steps, args.warmup, args.c3_depth, args.pre,
-                          args.post, schedule=sched)
+                          args.post, schedule=sched, extrapolate=args.c3_extrapolate)
     del wl3
     torch.cuda.empty_cache()
 

@@ -195,12 +195,15 @@ __device__ __forceinline__ T stencil_apply(const DevLevel& L, const T* __restric
     return s;
 }
 
-// x_out = relax x_in + b * (2/3)   (multiply_add(relax, x, b, buf, 2/3, 1))
-template <typename T, int D, bool COMPACT>
+// x_out = relax x_in + b * (2/3)   (multiply_add(relax, x, b, buf, 2/3, 1));
+// ACC: also xacc += x_out in FP64 (the mixed refinement's x += e, fused into
+// the last level-0 sweep)
+template <typename T, int D, bool COMPACT, bool ACC = false>
 __global__ __launch_bounds__(kMgThreads) void k_smooth(DevLevel L, DevTable<T> relax,
                                                        const T* __restrict__ xin,
                                                        const T* __restrict__ b,
-                                                       T* __restrict__ xout, T two3)
+                                                       T* __restrict__ xout, T two3,
+                                                       double* __restrict__ xacc = nullptr)
 {
     const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
     if (i >= L.N) return;
@@ -208,7 +211,9 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth(DevLevel L, DevTable<T> r
     node_coords<D>(L, i, c);
     const T* coef = relax.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
     const T t = stencil_apply<T, D, COMPACT>(L, coef, xin, c);
-    xout[i] = t * (T)1 + b[i] * (two3 * (T)1);
+    const T v = t * (T)1 + b[i] * (two3 * (T)1);
+    xout[i] = v;
+    if (ACC) xacc[i] += (double)v;
 }
 
 // r = (A^ x)*(-1) + b*((-1)*(-1))   (multiply_add(A, x, b, buf, -1, -1))
@@ -1038,7 +1043,7 @@ struct OuterArgs {
     float* e0;
 };
 
-template <int D, bool COMPACT, bool OUTF32>
+template <int D, bool COMPACT, bool OUTF32, bool NORMS>
 __device__ __forceinline__ void outer_residual_body(const OuterArgs& A, int64_t blk)
 {
     const DevLevel& L = A.L;
@@ -1060,8 +1065,10 @@ __device__ __forceinline__ void outer_residual_body(const OuterArgs& A, int64_t 
         const double* coef = K.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
         const double ax = stencil_apply<double, D, COMPACT>(L, coef, x, c);
         const double r = b[i] - ax;
-        rr = i == pin ? 0.0 : r * r;
-        bb = bn[i] * bn[i];
+        if (NORMS) {
+            rr = i == pin ? 0.0 : r * r;
+            bb = bn[i] * bn[i];
+        }
         if (OUTF32) {
             const float bv = (float)r * dinv32[L.cls[i]];
             b0[i] = bv;
@@ -1069,6 +1076,7 @@ __device__ __forceinline__ void outer_residual_body(const OuterArgs& A, int64_t 
             if (e0) e0[i] = bv * (2.0f / 3.0f * 1.0f);
         }
     }
+    if (!NORMS) return;
     rr = wave_sum(rr);
     bb = wave_sum(bb);
     if ((threadIdx.x & 63) == 0) {
@@ -1083,18 +1091,18 @@ __device__ __forceinline__ void outer_residual_body(const OuterArgs& A, int64_t 
     }
 }
 
-template <int D, bool COMPACT, bool OUTF32>
+template <int D, bool COMPACT, bool OUTF32, bool NORMS>
 __global__ __launch_bounds__(kMgThreads) void k_outer_residual(OuterArgs a)
 {
-    outer_residual_body<D, COMPACT, OUTF32>(a, level_block<D>());
+    outer_residual_body<D, COMPACT, OUTF32, NORMS>(a, level_block<D>());
 }
 
-template <int D, bool COMPACT, bool OUTF32>
+template <int D, bool COMPACT, bool OUTF32, bool NORMS>
 __global__ __launch_bounds__(kMgThreads) void k_outer_residual_many(const OuterArgs* __restrict__ a,
                                                                     const unsigned* __restrict__ first, int n)
 {
     const int i = batch_instance(first, n);
-    outer_residual_body<D, COMPACT, OUTF32>(a[i], (int64_t)(blockIdx.x - first[i]));
+    outer_residual_body<D, COMPACT, OUTF32, NORMS>(a[i], (int64_t)(blockIdx.x - first[i]));
 }
 
 __global__ __launch_bounds__(kMgThreads) void k_reduce2(const double* __restrict__ part, int64_t nb,
@@ -1758,10 +1766,10 @@ constexpr uint16_t kKindSmooth = 0;      // + (FP64 << 3 | compact << 2 | prolon
 constexpr uint16_t kKindRRTile = 16;     // + (FP64 << 1 | compact)
 constexpr uint16_t kKindRR = 20;         // + (FP64 << 1 | compact)
 constexpr uint16_t kKindCoarse = 24;     // + FP64
-constexpr uint16_t kKindOuter = 26;      // + (compact << 1 | emit FP32), 2D
-constexpr uint16_t kKindVec = 30;        // + op (0 zero, 1 add correction, 2 copy)
-constexpr uint16_t kKindHadamard = 33;   // FP64
-constexpr uint16_t kKindCount = 34;
+constexpr uint16_t kKindOuter = 26;      // + (norms << 2 | compact << 1 | emit FP32), 2D
+constexpr uint16_t kKindVec = 34;        // + op (0 zero, 1 add correction, 2 copy)
+constexpr uint16_t kKindHadamard = 37;   // FP64
+constexpr uint16_t kKindCount = 38;
 
 class HipMultigrid {
 public:
@@ -2120,15 +2128,15 @@ private:
         hipLaunchKernelGGL((k_coarse_solve<T>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
     }
 
-    template <int D, bool C, bool F32>
+    template <int D, bool C, bool F32, bool NORMS>
     void run_outer(unsigned nb, const OuterArgs& a)
     {
         if (rec_) {
             if (D != 2) throw std::logic_error("hip_multigrid: only 2D solves are batched");
-            emit((uint16_t)(kKindOuter + ((C ? 2 : 0) | (F32 ? 1 : 0))), nb, a);
+            emit((uint16_t)(kKindOuter + ((NORMS ? 4 : 0) | (C ? 2 : 0) | (F32 ? 1 : 0))), nb, a);
             return;
         }
-        hipLaunchKernelGGL((k_outer_residual<D, C, F32>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
+        hipLaunchKernelGGL((k_outer_residual<D, C, F32, NORMS>), dim3(nb), dim3(kMgThreads), 0, stream_, a);
     }
 
     template <int OP>
@@ -2325,18 +2333,20 @@ private:
     }
 
     template <typename T>
-    void smooth(int l, T*& x, T*& buf, const T* b)
+    void smooth(int l, T*& x, T*& buf, const T* b, double* acc = nullptr)
     {
         const DevLevel L = dl(l);
         const T two3 = (T)(2.0 / 3);
         const unsigned nb = nblk(L.N);
+#define SMOOTH(DIM, CMP) \
+        if (acc) hipLaunchKernelGGL((k_smooth<T, DIM, CMP, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3, acc); \
+        else hipLaunchKernelGGL((k_smooth<T, DIM, CMP, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3, nullptr)
         if (host_[l].D == 2) {
-            if (host_[l].compact) hipLaunchKernelGGL((k_smooth<T, 2, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
-            else hipLaunchKernelGGL((k_smooth<T, 2, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
+            if (host_[l].compact) { SMOOTH(2, true); } else { SMOOTH(2, false); }
         } else {
-            if (host_[l].compact) hipLaunchKernelGGL((k_smooth<T, 3, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
-            else hipLaunchKernelGGL((k_smooth<T, 3, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3);
+            if (host_[l].compact) { SMOOTH(3, true); } else { SMOOTH(3, false); }
         }
+#undef SMOOTH
         std::swap(x, buf);
     }
 
@@ -2364,8 +2374,8 @@ private:
     {
         if (host_[l].D != 2 || !fuse_ || count == 0) {
             if (xc) prolong(l, xc, x);
-            for (int s = 0; s < count; ++s) smooth<T>(l, x, buf, b);
-            return false;
+            for (int s = 0; s < count; ++s) smooth<T>(l, x, buf, b, s == count - 1 ? acc : nullptr);
+            return acc != nullptr && count > 0;
         }
         const DevLevel L = dl(l);
         const T two3 = (T)(2.0 / 3);
@@ -2780,7 +2790,10 @@ private:
         const float* d32 = dev_[0].dinv32;
         float* e0 = (emit_f32 && first_sweep) ? dev_[0].xf : nullptr;
         const OuterArgs oa{L, K, x, b, bn, pin_, d32, b0, part_, e0};
-#define OUTER(DIM, CMP, F32) run_outer<DIM, CMP, F32>(nb, oa)
+        // the squared norms only when they are read (norms); otherwise the
+        // kernel skips bn and the reductions
+#define OUTER(DIM, CMP, F32) \
+        do { if (norms) run_outer<DIM, CMP, F32, true>(nb, oa); else run_outer<DIM, CMP, F32, false>(nb, oa); } while (0)
         if (host_[0].D == 2) {
             if (c) { if (emit_f32) OUTER(2, true, true); else OUTER(2, true, false); }
             else { if (emit_f32) OUTER(2, false, true); else OUTER(2, false, false); }
@@ -2937,12 +2950,18 @@ void mg_batch_launch(const MgBatchPlan& plan, hipStream_t st)
                                    static_cast<const CoarseArgs<float>*>(a), f, g.n);
         } else if (k < kKindVec) {
             const OuterArgs* oa = static_cast<const OuterArgs*>(a);
+#define OUT(C, F, N) hipLaunchKernelGGL((k_outer_residual_many<2, C, F, N>), grid, dim3(kMgThreads), 0, st, oa, f, g.n)
             switch (k - kKindOuter) {
-            case 0: hipLaunchKernelGGL((k_outer_residual_many<2, false, false>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
-            case 1: hipLaunchKernelGGL((k_outer_residual_many<2, false, true>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
-            case 2: hipLaunchKernelGGL((k_outer_residual_many<2, true, false>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
-            default: hipLaunchKernelGGL((k_outer_residual_many<2, true, true>), grid, dim3(kMgThreads), 0, st, oa, f, g.n); break;
+            case 0: OUT(false, false, false); break;
+            case 1: OUT(false, true, false); break;
+            case 2: OUT(true, false, false); break;
+            case 3: OUT(true, true, false); break;
+            case 4: OUT(false, false, true); break;
+            case 5: OUT(false, true, true); break;
+            case 6: OUT(true, false, true); break;
+            default: OUT(true, true, true); break;
             }
+#undef OUT
         } else if (k < kKindHadamard) {
             const VecArgs* va = static_cast<const VecArgs*>(a);
             switch (k - kKindVec) {
