@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -14,9 +15,27 @@
 namespace psum {
 
 // Stands where the reference's SYCL handler is: the argument of the
-// functions given to for_each, from which accessors are taken.
+// functions given to for_each, from which accessors are taken.  An accessor
+// that needs work once the kernel has run (a fixed-point deposit folded into
+// its field) registers it with after(); the launching for_each calls
+// finish() when the kernel has completed.
 struct handler {
     hipStream_t stream = nullptr;
+    std::vector<std::pair<const void*, std::function<void()>>> done;
+
+    // f runs once after the kernel, at most once per key
+    void after(const void* key, std::function<void()> f)
+    {
+        for (const auto& d : done)
+            if (d.first == key) return;
+        done.emplace_back(key, std::move(f));
+    }
+    void finish()
+    {
+        auto d = std::move(done);
+        done.clear();
+        for (auto& x : d) x.second();
+    }
 };
 
 inline void hip_check(hipError_t e, const char* what)
@@ -161,6 +180,7 @@ public:
                            stream_, data_, n, f);
         hip_check(hipGetLastError(), "for_each");
         hip_check(hipStreamSynchronize(stream_), "for_each");
+        h.finish();
     }
     template <typename FuncType>
     void for_each(FuncType&& func) const

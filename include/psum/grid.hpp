@@ -123,18 +123,23 @@ public:
     // 295): after the caller's "variables=" line, "zone i=..,j=..,F=point"
     // and one "x y value" line per node, or for cell-centred content
     // F=block with the node coordinates block by block, then the values;
-    // SolutionTime when time is not NaN.  1D has no zone line.
+    // SolutionTime when time is not NaN.  1D has no zone line.  vnum values
+    // per element (content[i * vnum + v]): one column each for nodes, one
+    // block each for cells.
     template <typename Scalar>
-    void plot(std::ostream& fp, const Scalar* content, double time, var_loc loc) const
+    void plot(std::ostream& fp, const Scalar* content, double time, var_loc loc, int vnum = 1) const
     {
         static const char ijk[3] = {'i', 'j', 'k'};
         if (Dim > 1) {
             fp << "\nzone ";
             for (int i = 0; i < Dim; ++i) fp << ijk[i] << "=" << (n_[Dim - 1 - i] + 1) << ",";
-            if (loc == var_loc::cellCentered)
-                fp << "F=block, VARLOCATION=([" << Dim + 1 << "]=CELLCENTERED)" << std::endl;
-            else
+            if (loc == var_loc::cellCentered) {
+                fp << "F=block, VARLOCATION=([" << Dim + 1;
+                if (vnum > 1) fp << "-" << Dim + vnum;
+                fp << "]=CELLCENTERED)" << std::endl;
+            } else {
                 fp << "F=point" << std::endl;
+            }
         }
         if (!std::isnan(time)) fp << "AUXDATA TimeUnits=\"Seconds\" SolutionTime=" << time;
         const size_t nodes = contentSize(var_loc::nodeCentered);
@@ -142,13 +147,14 @@ public:
             for (int d = 0; d < Dim; ++d)
                 for (size_t i = 0; i < nodes; ++i) fp << std::endl << lb_[d] + i2n(i)[d] * delta_[d];
             const size_t cells = contentSize(var_loc::cellCentered);
-            for (size_t i = 0; i < cells; ++i) fp << std::endl << content[i];
+            for (int v = 0; v < vnum; ++v)
+                for (size_t i = 0; i < cells; ++i) fp << std::endl << content[i * vnum + v];
         } else {
             for (size_t i = 0; i < nodes; ++i) {
                 const auto p = nodePosition(i2n(i));
                 fp << std::endl << p[0];
                 for (int d = 1; d < Dim; ++d) fp << "\t" << p[d];
-                fp << "\t" << content[i];
+                for (int v = 0; v < vnum; ++v) fp << "\t" << content[i * vnum + v];
             }
         }
     }

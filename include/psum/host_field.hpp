@@ -26,36 +26,38 @@ namespace field {
 
 template <int D, var_loc Loc, typename Scalar, int QuantitySize = 1>
 class host_field {
-    static_assert(QuantitySize == 1, "host_field: vector-valued fields are not part of this build");
+    static_assert(QuantitySize >= 1, "host_field: QuantitySize >= 1");
 
 public:
     static constexpr int Dimension = D;
     static constexpr var_loc Location = Loc;
-    using Value = Scalar;
+    static constexpr int QS = QuantitySize;
+    using ScalarType = Scalar;
+    using Value = field_value_t<Scalar, QuantitySize>;
     using Grid = simple_grid<D>;
     using Position = vec<D>;   // x(), y(), z() and [d]
-    using acc_type = device_field_acc<D, Loc, Scalar>;
+    using acc_type = device_field_acc<D, Loc, Scalar, QuantitySize>;
 
     host_field() = default;
-    explicit host_field(const Grid& g) : grid_(g), content_(g.contentSize(Loc), Scalar(0)) {}
-    host_field(std::vector<Scalar>&& content, const Grid& g) : grid_(g), content_(std::move(content))
+    explicit host_field(const Grid& g) : grid_(g), content_(g.contentSize(Loc), Value{}) {}
+    host_field(std::vector<Value>&& content, const Grid& g) : grid_(g), content_(std::move(content))
     {
         if (content_.size() != g.contentSize(Loc))
             throw std::runtime_error("host_field: content size is not the grid's");
     }
     // a host copy of a device field
-    template <int Q>
-    explicit host_field(const device_field<D, Loc, Scalar, Q>& f) : grid_(f.getGrid()), content_(f.to_host())
+    explicit host_field(const device_field<D, Loc, Scalar, QuantitySize>& f)
+        : grid_(f.getGrid()), content_(f.to_host())
     {
     }
 
     bool has_grid() const { return grid_.has_value(); }
     size_t size() const { return grid().contentSize(Loc); }
-    Scalar* data() { return (void)grid(), content_.data(); }
-    const Scalar* data() const { return (void)grid(), content_.data(); }
+    Value* data() { return (void)grid(), content_.data(); }
+    const Value* data() const { return (void)grid(), content_.data(); }
     const Grid& getGrid() const { return grid(); }
     var_loc getLocation() const { return Loc; }
-    std::vector<Scalar> getContent() const { return (void)grid(), content_; }
+    std::vector<Value> getContent() const { return (void)grid(), content_; }
 
     // func(i, value) or func(i, value, position of element i)
     template <typename F>
@@ -63,19 +65,19 @@ public:
     {
         const size_t n = size();
         for (size_t i = 0; i < n; ++i) {
-            if constexpr (std::is_invocable_v<F&, size_t, Scalar&, const Position&>)
+            if constexpr (std::is_invocable_v<F&, size_t, Value&, const Position&>)
                 func(i, content_[i], position(i));
             else
                 func(i, content_[i]);
         }
     }
 
-    void setConstant(const Scalar& v)
+    void setConstant(const Value& v)
     {
         (void)grid();
         for (auto& x : content_) x = v;
     }
-    void setZero() { setConstant(Scalar(0)); }
+    void setZero() { setConstant(Value{}); }
     // content of b (same size)
     host_field& operator<<(const host_field& b)
     {
@@ -83,13 +85,19 @@ public:
         content_ = b.content_;
         return *this;
     }
-    void copy(const std::vector<Scalar>& v)
+    void copy(const std::vector<Value>& v)
     {
         if (v.size() != size()) throw std::runtime_error("host_field::copy: size mismatch");
         content_ = v;
     }
-    Scalar& operator()(size_t i) { return content_[i]; }
-    const Scalar& operator()(size_t i) const { return content_[i]; }
+    Value& operator()(size_t i) { return content_[i]; }
+    const Value& operator()(size_t i) const { return content_[i]; }
+    // component col of element row (QuantitySize > 1)
+    Scalar& operator()(size_t row, size_t col)
+    {
+        static_assert(QuantitySize > 1, "operator()(row, col) needs QuantitySize > 1");
+        return content_[row][(int)col];
+    }
 
     // node position, or cell centre
     Position position(size_t i) const { return detail::element_position<D, Loc>(grid(), i); }
@@ -102,17 +110,20 @@ public:
         std::ofstream fp(filename, std::ios::out);
         if (!fp) throw std::runtime_error(filename + " cannot be opened");
         fp << std::scientific << std::setprecision(4);
-        if (varName == "_not_defined_") varName = "v1";
+        if (varName == "_not_defined_") {
+            varName = "v1";
+            for (int q = 1; q < QuantitySize; ++q) varName += ",v" + std::to_string(q + 1);
+        }
         fp << "variables=" << (D == 1 ? "x," : (D == 2 ? "x,y," : "x,y,z,")) << varName;
         const auto h = getContent();
-        getGrid().plot(fp, h.data(), time, Loc);
+        getGrid().plot(fp, reinterpret_cast<const Scalar*>(h.data()), time, Loc, QuantitySize);
     }
 
     // accessor over the host content (interp / interp_diff take it)
     acc_type acc() const
     {
         acc_type a;
-        a.content = const_cast<Scalar*>(data());
+        a.content = const_cast<Value*>(data());
         a.grid = psum_hip::to_dev(grid().to_psum());
         return a;
     }
@@ -125,11 +136,11 @@ private:
     }
 
     std::optional<Grid> grid_;
-    std::vector<Scalar> content_;
+    std::vector<Value> content_;
 
 public:
     // grid and content together (the MAS load)
-    void reset(const Grid& g, std::vector<Scalar>&& c)
+    void reset(const Grid& g, std::vector<Value>&& c)
     {
         if (c.size() != g.contentSize(Loc)) throw std::runtime_error("host_field: content size is not the grid's");
         grid_ = g;
@@ -144,21 +155,22 @@ template <typename Scalar, int QS = 1> using host_cell_field1D = host_field<1, v
 template <typename Scalar, int QS = 1> using host_cell_field2D = host_field<2, var_loc::cellCentered, Scalar, QS>;
 template <typename Scalar, int QS = 1> using host_cell_field3D = host_field<3, var_loc::cellCentered, Scalar, QS>;
 
-template <typename Pos, int D, typename Scalar>
-Scalar interp(const Pos& pos, const host_field<D, var_loc::nodeCentered, Scalar>& f)
+template <typename Pos, int D, typename Scalar, int QS>
+field_value_t<Scalar, QS> interp(const Pos& pos, const host_field<D, var_loc::nodeCentered, Scalar, QS>& f)
 {
     return interp(pos, f.acc());
 }
 
-template <typename Pos, int D, typename Scalar>
-std::array<Scalar, D> interp_diff(const Pos& pos, const host_field<D, var_loc::nodeCentered, Scalar>& f)
+template <typename Pos, int D, typename Scalar, int QS>
+std::array<field_value_t<Scalar, QS>, D> interp_diff(const Pos& pos,
+                                                     const host_field<D, var_loc::nodeCentered, Scalar, QS>& f)
 {
     return interp_diff(pos, f.acc());
 }
 
 // f[idx_i] += w c_i, skipping |c_i| <= 1e-10, in support order
-template <typename Pos, int D, typename Scalar>
-void add_back(const Pos& pos, const Scalar& w, host_field<D, var_loc::nodeCentered, Scalar>& f)
+template <typename Pos, int D, typename Scalar, int QS>
+void add_back(const Pos& pos, const field_value_t<Scalar, QS>& w, host_field<D, var_loc::nodeCentered, Scalar, QS>& f)
 {
     const auto a = f.acc();
     int cw[D];
@@ -166,8 +178,29 @@ void add_back(const Pos& pos, const Scalar& w, host_field<D, var_loc::nodeCenter
     if (!detail::support<D>(a.grid, pos, cw, W, G)) return;
     for (int i = 0; i < (1 << D); ++i) {
         const double c = psum_hip::cic_coef<double, D>(W, i);
-        if (std::fabs(c) > 1e-10) a.content[psum_hip::support_index<D>(a.grid, cw, i)] += w * c;
+        if (std::fabs(c) > 1e-10) {
+            auto& v = a.content[psum_hip::support_index<D>(a.grid, cw, i)];
+            for (int q = 0; q < QS; ++q) detail::comp<QS>(v, q) += detail::comp<QS>(w, q) * c;
+        }
     }
+}
+
+// the element nearest to pos (field_interp.hpp:38-45; host_field.hpp
+// interp_nearest); zero outside the grid
+template <typename Pos, int D, var_loc Loc, typename Scalar, int QS>
+field_value_t<Scalar, QS> interp_nearest(const Pos& pos, const host_field<D, Loc, Scalar, QS>& f)
+{
+    return interp_nearest(pos, f.acc());
+}
+
+// the element nearest to pos += w (field_interp.hpp:24-31); nothing outside
+template <typename Pos, int D, var_loc Loc, typename Scalar, int QS>
+void add_back_nearest(const Pos& pos, const field_value_t<Scalar, QS>& w, host_field<D, Loc, Scalar, QS>& f)
+{
+    const auto a = f.acc();
+    size_t k = 0;
+    if (!detail::nearest_element<D, Loc>(a.grid, pos, &k)) return;
+    for (int q = 0; q < QS; ++q) detail::comp<QS>(a.content[k], q) += detail::comp<QS>(w, q);
 }
 
 }  // namespace field

@@ -168,17 +168,50 @@ void load(mas_file& f, const std::string& name, field::simple_grid<D>& g)
     g = field::simple_grid<D>(lo, up, std::vector<int>(cells.begin(), cells.end()));
 }
 
-template <int D, field::var_loc Loc, typename Scalar>
-void save(mas_file& f, const std::string& name, const field::device_field<D, Loc, Scalar>& fld)
+namespace detail {
+// field content: scalars as an n x 1 block; rows of QS components as an
+// n x 1 x QS block, the shape of a vector of Eigen::RowVector
+// (container_sl.hpp:94-134)
+template <typename Scalar, int QS>
+void save_content(mas_file& f, const std::string& name, const std::vector<field::field_value_t<Scalar, QS>>& v)
+{
+    if constexpr (QS == 1) {
+        save(f, name, v);
+    } else {
+        std::vector<Scalar> flat(v.size() * QS);
+        for (size_t i = 0; i < v.size(); ++i)
+            for (int q = 0; q < QS; ++q) flat[i * QS + q] = v[i][q];
+        f.writeData<Scalar>(name, flat.data(), {v.size(), (size_t)1, (size_t)QS});
+    }
+}
+template <typename Scalar, int QS>
+void load_content(mas_file& f, const std::string& name, std::vector<field::field_value_t<Scalar, QS>>& v)
+{
+    if constexpr (QS == 1) {
+        load(f, name, v);
+    } else {
+        std::vector<size_t> shp;
+        const std::vector<Scalar> flat = f.read<Scalar>(name, &shp);
+        if (shp.size() != 3 || shp[1] != 1 || shp[2] != (size_t)QS)
+            throw std::runtime_error("MAS: block '" + name + "' is not a vector of " + std::to_string(QS) + "-rows");
+        v.assign(shp[0], {});
+        for (size_t i = 0; i < v.size(); ++i)
+            for (int q = 0; q < QS; ++q) v[i][q] = flat[i * QS + q];
+    }
+}
+}  // namespace detail
+
+template <int D, field::var_loc Loc, typename Scalar, int QS>
+void save(mas_file& f, const std::string& name, const field::device_field<D, Loc, Scalar, QS>& fld)
 {
     save(f, name + ".location", (int32_t)Loc);
     save(f, name + ".grid", fld.getGrid());
     save(f, name + ".content.device", detail::device_label(nullptr));
-    save(f, name + ".content.content", fld.to_host());
+    detail::save_content<Scalar, QS>(f, name + ".content.content", fld.to_host());
 }
 // the field keeps its grid: the stored grid must have the same cells
-template <int D, field::var_loc Loc, typename Scalar>
-void load(mas_file& f, const std::string& name, field::device_field<D, Loc, Scalar>& fld)
+template <int D, field::var_loc Loc, typename Scalar, int QS>
+void load(mas_file& f, const std::string& name, field::device_field<D, Loc, Scalar, QS>& fld)
 {
     int32_t loc = -1;
     load(f, name + ".location", loc);
@@ -188,31 +221,31 @@ void load(mas_file& f, const std::string& name, field::device_field<D, Loc, Scal
     for (int d = 0; d < D; ++d)
         if (g.to_psum().cells[d] != fld.getGrid().to_psum().cells[d])
             throw std::runtime_error("MAS: block '" + name + "' has another grid");
-    std::vector<Scalar> c;
-    load(f, name + ".content.content", c);
+    std::vector<field::field_value_t<Scalar, QS>> c;
+    detail::load_content<Scalar, QS>(f, name + ".content.content", c);
     if (c.size() != fld.size()) throw std::runtime_error("MAS: block '" + name + "' content does not match its grid");
     fld.copy(c);
 }
 
 // host_field: name.location, name.grid.*, name.content (host_field.hpp:310-337)
-template <int D, field::var_loc Loc, typename Scalar>
-void save(mas_file& f, const std::string& name, const field::host_field<D, Loc, Scalar>& fld)
+template <int D, field::var_loc Loc, typename Scalar, int QS>
+void save(mas_file& f, const std::string& name, const field::host_field<D, Loc, Scalar, QS>& fld)
 {
     save(f, name + ".location", (int32_t)Loc);
     save(f, name + ".grid", fld.getGrid());
-    save(f, name + ".content", fld.getContent());
+    detail::save_content<Scalar, QS>(f, name + ".content", fld.getContent());
 }
 // the field takes the stored grid
-template <int D, field::var_loc Loc, typename Scalar>
-void load(mas_file& f, const std::string& name, field::host_field<D, Loc, Scalar>& fld)
+template <int D, field::var_loc Loc, typename Scalar, int QS>
+void load(mas_file& f, const std::string& name, field::host_field<D, Loc, Scalar, QS>& fld)
 {
     int32_t loc = -1;
     load(f, name + ".location", loc);
     if (loc != (int32_t)Loc) throw std::runtime_error("MAS: block '" + name + "' has another location");
     field::simple_grid<D> g;
     load(f, name + ".grid", g);
-    std::vector<Scalar> c;
-    load(f, name + ".content", c);
+    std::vector<field::field_value_t<Scalar, QS>> c;
+    detail::load_content<Scalar, QS>(f, name + ".content", c);
     if (c.size() != g.contentSize(Loc)) throw std::runtime_error("MAS: block '" + name + "' content does not match its grid");
     fld.reset(g, std::move(c));
 }

@@ -24,6 +24,7 @@ struct vec {
     __host__ __device__ S& z() { static_assert(D > 2); return v[2]; }
     __host__ __device__ const S& z() const { static_assert(D > 2); return v[2]; }
     static constexpr int size() { return D; }
+    __host__ __device__ static vec Zero() { return vec{}; }
     __host__ __device__ vec& operator+=(const vec& o) { for (int i = 0; i < D; ++i) v[i] += o.v[i]; return *this; }
     __host__ __device__ vec& operator-=(const vec& o) { for (int i = 0; i < D; ++i) v[i] -= o.v[i]; return *this; }
     __host__ __device__ vec& operator*=(S a) { for (int i = 0; i < D; ++i) v[i] *= a; return *this; }

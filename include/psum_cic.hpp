@@ -105,6 +105,26 @@ __device__ __forceinline__ long long round_to_ll(double t)
     return (long long)(__double_as_longlong(u) - __double_as_longlong(m));
 }
 
+// acc += v on a global int64 accumulator.  With ovf non-null, an add whose
+// two's-complement partial sum wraps (old value and v of one sign, the sum
+// of the other) is counted in *ovf, read from the value the atomic returns.
+// Counting every global add this way leaves no wrap unseen; a sum of mixed
+// signs may be counted although its final value is back in range (the
+// integer result is then still exact: the adds are modulo 2^64).
+__device__ __forceinline__ bool add_wraps(unsigned long long old, unsigned long long v)
+{
+    const unsigned long long sum = old + v;
+    return (long long)((old ^ sum) & (v ^ sum)) < 0;
+}
+__device__ __forceinline__ void acc_add(unsigned long long* acc, unsigned long long v, unsigned long long* ovf)
+{
+    if (ovf) {
+        if (add_wraps(atomicAdd(acc, v), v)) atomicAdd(ovf, 1ull);
+    } else {
+        atomicAdd(acc, v);
+    }
+}
+
 // Linear node index of support point i: bit j of i offsets dimension D-1-j;
 // cw + 1 == w wraps to node 0.
 template <int D>

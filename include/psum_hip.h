@@ -109,11 +109,15 @@ psum_status psum_hip_compact_noalloc(int dtype, int dim, void* const* pos, void*
  * field_interp.hpp:63-75 -> atomic_add.hpp:34-48): for every valid
  * particle and every support node with |coef| > 1e-10,
  *     acc[node] += llrint((w * coef) * scale)          (int64, exact)
- * scale must be a power of two.  Integer accumulation makes the result
- * independent of particle order and thread schedule; no float atomics.
- * acc is accumulated into (zero it first).  counters (device, may be NULL):
- * [0] particles whose support was out of range, [1] blocks that fell back
- * from the LDS tile to global integer atomics. */
+ * scale must be a power of two with |w| * scale <= 2^50.  Integer
+ * accumulation makes the result independent of particle order and thread
+ * schedule; no float atomics.  acc is accumulated into (zero it first).
+ * counters (device, 4 entries, may be NULL): [0] particles whose support was
+ * out of range, [1] blocks that fell back from the LDS tile to global
+ * integer atomics, [3] adds to acc whose int64 partial sum wrapped (the
+ * accumulator's overflow flag: zero means every node's sum stayed in range,
+ * so acc is exact; a node overflows after about 2^63 / (|w| * scale) full
+ * terms).  Entries are added to, never cleared. */
 psum_status psum_hip_deposit_cic(const psum_grid* g, int dtype, void* const* pos, int64_t n,
                                  double w, double scale, int64_t* acc, int64_t* counters,
                                  void* stream);
@@ -132,8 +136,9 @@ psum_status psum_hip_fixed_to_double(const int64_t* acc, int64_t nn, double inv_
  * psum_hip_deposit_cic with (w_dep, scale) -- the next step's charge.
  * diag (device, may be NULL) receives per-block partial sums of
  * 1/2 m|v|^2 and m v_d over the velocities entering the step; reduce them
- * with psum_hip_reduce_diag.  counters (device, may be NULL): [0] support
- * out of range, [1] tile fallbacks, [2] particles absorbed this call. */
+ * with psum_hip_reduce_diag.  counters (device, 4 entries, may be NULL):
+ * [0] support out of range, [1] tile fallbacks, [2] particles absorbed this
+ * call, [3] int64 wraps of the deposit (as psum_hip_deposit_cic). */
 int64_t psum_hip_push_blocks(int64_t n);
 psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos,
                                  void* const* vel, int64_t n, const void* phi, double qm,
@@ -148,7 +153,8 @@ psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos
  * support), wrapped once (simple_grid.hpp:316-385).  Weights are FP64 for
  * both dtypes.  order 1 forwards to the CIC entry points.  2D and 3D.
  * Deposit: as psum_hip_deposit_cic with these weights (exact int64 sums,
- * scale a power of two; counters [0] out of range, [1] tile fallbacks).
+ * scale a power of two, |w| * scale <= 2^50; counters [0] out of range,
+ * [1] tile fallbacks, [3] int64 wraps).
  * Gather + push: E = -grad(phi) summed in FP64 and rounded to the dtype,
  * then the kick, drift and boundaries of psum_hip_gather_push (no fused
  * deposit); diag has psum_hip_push_blocks(n) blocks as there. */
@@ -381,7 +387,11 @@ psum_status psum_stepper_phi(void* h, double* phi_host);
  * (device_field.hpp:295-310); *scale (may be NULL) = 2^S of the int64
  * fixed-point accumulator */
 psum_status psum_stepper_rho(void* h, double* rho_host, double* scale);
-/* total V-cycles so far, last relative residual */
+/* total V-cycles so far, last relative residual; counters (host, 4
+ * entries, may be NULL): [0] support out of range, [1] tile fallbacks,
+ * [2] particles absorbed, [3] wrapped adds to the int64 charge accumulator
+ * since the last load (psum_stepper_run and psum_batch_run fail with an
+ * overflow message when it is non-zero) */
 psum_status psum_stepper_stats(void* h, int64_t* cycles, double* last_rel_res, int64_t* counters);
 /* steps taken since creation (a sort runs before step k when
  * k % sort_every == 0); restore() sets it from the checkpoint */

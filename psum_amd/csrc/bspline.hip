@@ -177,6 +177,8 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
     __shared__ int bmin[3], bmax[3], bref[3];
     const T* const xs[3] = {x0, x1, x2};
     const int lane = threadIdx.x & 63;
+    // wraps of the global int64 sums (an LDS entry: <= kBsTile terms below 2^50)
+    unsigned long long* const ovf = counters ? counters + 3 : nullptr;
     if (threadIdx.x < 3) {
         bmin[threadIdx.x] = INT_MAX;
         bmax[threadIdx.x] = INT_MIN;
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
                     const int nd = mod_node(base[j][d] + k[d] + shift, g.w[d]);
                     lin = d == 0 ? (uint32_t)nd : lin * (uint32_t)g.w[d] + (uint32_t)nd;
                 }
-                atomicAdd(&acc[lin], (unsigned long long)term);
+                acc_add(&acc[lin], (unsigned long long)term, ovf);
             }
         }
     }
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
             uint32_t lin = (uint32_t)nd[0];
 #pragma unroll
             for (int d = 1; d < D; ++d) lin = lin * (uint32_t)g.w[d] + (uint32_t)nd[d];
-            atomicAdd(&acc[lin], v);
+            acc_add(&acc[lin], v, ovf);
         }
     } else if (any && counters && threadIdx.x == 0) {
         atomicAdd(&counters[1], 1ull);
@@ -374,6 +376,7 @@ psum_status psum_hip_deposit_bspline(const psum_grid* g, int dtype, int order, v
     PSUM_ARG_CHECK(order >= 2 && order <= 4, "psum_hip_deposit_bspline: order must be 1..4");
     PSUM_ARG_CHECK(acc && (n == 0 || pos), "psum_hip_deposit_bspline: null pointer");
     PSUM_ARG_CHECK(scale_pow2(scale), "psum_hip_deposit_bspline: scale must be a power of two");
+    PSUM_ARG_CHECK(std::fabs(w) * scale <= 0x1p50, "psum_hip_deposit_bspline: |w| * scale must not exceed 2^50");
     if (n == 0) return PSUM_OK;
     const DevGrid dg = to_dev(*g);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);

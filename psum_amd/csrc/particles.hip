@@ -435,6 +435,9 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
     __shared__ int bcnt;
     __shared__ double wdiag[kThreads / 64][4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wraps of the global int64 sums; an LDS tile entry holds at most
+    // kPushTile terms below 2^50 (|w_dep| * scale <= 2^50) and cannot wrap
+    unsigned long long* const ovf = counters ? counters + 3 : nullptr;
     if (DEPOSIT && threadIdx.x < 3) {
         bmin[threadIdx.x] = INT_MAX;
         bmax[threadIdx.x] = INT_MIN;
@@ -670,6 +673,18 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                     for (int i = 0; i < NS; ++i)
                         if (t8[i]) atomicAdd(&tile[li0 + ((i >> 2) & 1) * s0 + ((i >> 1) & 1) * s1 + (i & 1)],
                                              (unsigned long long)t8[i]);
+                } else if (ovf) {
+                    // all eight adds are issued before their returned values
+                    // are checked (a zero term leaves old unchanged: no wrap)
+                    unsigned long long old8[NS];
+#pragma unroll
+                    for (int i = 0; i < NS; ++i)
+                        old8[i] = t8[i] ? atomicAdd(&acc[support_index<D>(g, cw[j], i)], (unsigned long long)t8[i])
+                                        : 0ull;
+                    bool wr = false;
+#pragma unroll
+                    for (int i = 0; i < NS; ++i) wr |= add_wraps(old8[i], (unsigned long long)t8[i]);
+                    if (wr) atomicAdd(ovf, 1ull);
                 } else {
 #pragma unroll
                     for (int i = 0; i < NS; ++i)
@@ -685,19 +700,31 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 bool inside = use_lds;
 #pragma unroll
                 for (int d = 0; d < D; ++d) inside = inside && curu[d] >= lo[d] && curu[d] + 1 < lo[d] + e[d];
+                if (inside) {
 #pragma unroll
-                for (int q = 0; q < NS; ++q) {
-                    const int i = (q + lane) & (NS - 1);
-                    const long long t = run[i];
-                    if (t == 0) continue;
-                    if (inside) {
+                    for (int q = 0; q < NS; ++q) {
+                        const int i = (q + lane) & (NS - 1);
+                        const long long t = run[i];
+                        if (t == 0) continue;
                         int li = 0;
 #pragma unroll
                         for (int d = 0; d < D; ++d) li = li * e[d] + (curu[d] + ((i >> (D - 1 - d)) & 1) - lo[d]);
                         atomicAdd(&tile[li], (unsigned long long)t);
-                    } else {
-                        atomicAdd(&acc[support_index<D>(g, cur, i)], (unsigned long long)t);
                     }
+                } else if (ovf) {
+                    unsigned long long old[NS];
+#pragma unroll
+                    for (int i = 0; i < NS; ++i)
+                        old[i] = run[i] ? atomicAdd(&acc[support_index<D>(g, cur, i)], (unsigned long long)run[i])
+                                        : 0ull;
+                    bool wr = false;
+#pragma unroll
+                    for (int i = 0; i < NS; ++i) wr |= add_wraps(old[i], (unsigned long long)run[i]);
+                    if (wr) atomicAdd(ovf, 1ull);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NS; ++i)
+                        if (run[i]) atomicAdd(&acc[support_index<D>(g, cur, i)], (unsigned long long)run[i]);
                 }
             };
 #pragma unroll
@@ -736,9 +763,8 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         }
         if (use_lds) {
             __syncthreads();
-            for (int i = threadIdx.x; i < (int)tsize; i += kThreads) {
-                const unsigned long long v = tile[i];
-                if (v == 0ull) continue;
+            // tile entry i -> its global node
+            auto node_of = [&](int i) {
                 int rem = i;
                 int nd[3];
 #pragma unroll
@@ -751,7 +777,32 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 uint32_t lin = (uint32_t)nd[0];
 #pragma unroll
                 for (int d = 1; d < D; ++d) lin = lin * (uint32_t)g.w[d] + (uint32_t)nd[d];
-                atomicAdd(&acc[lin], v);
+                return lin;
+            };
+            if (ovf) {
+                // kFlushBatch entries per thread in flight before the
+                // returned values are checked
+                constexpr int kFlushBatch = 4;
+                bool wr = false;
+                for (int i0 = threadIdx.x; i0 < (int)tsize; i0 += kFlushBatch * kThreads) {
+                    unsigned long long v[kFlushBatch], old[kFlushBatch];
+#pragma unroll
+                    for (int u = 0; u < kFlushBatch; ++u) {
+                        const int i = i0 + u * kThreads;
+                        v[u] = i < (int)tsize ? tile[i] : 0ull;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kFlushBatch; ++u)
+                        old[u] = v[u] ? atomicAdd(&acc[node_of(i0 + u * kThreads)], v[u]) : 0ull;
+#pragma unroll
+                    for (int u = 0; u < kFlushBatch; ++u) wr |= add_wraps(old[u], v[u]);
+                }
+                if (wr) atomicAdd(ovf, 1ull);
+            } else {
+                for (int i = threadIdx.x; i < (int)tsize; i += kThreads) {
+                    const unsigned long long v = tile[i];
+                    if (v) atomicAdd(&acc[node_of(i)], v);
+                }
             }
         }
         if (clipped && counters && threadIdx.x == 0) atomicAdd(&counters[1], 1ull);
@@ -1210,6 +1261,7 @@ psum_status psum_hip_deposit_cic(const psum_grid* g, int dtype, void* const* pos
     PSUM_ARG_CHECK(grid_ok(g), "psum_hip_deposit_cic: invalid grid");
     PSUM_ARG_CHECK(acc && (n == 0 || pos), "psum_hip_deposit_cic: null pointer");
     PSUM_ARG_CHECK(pow2(scale), "psum_hip_deposit_cic: scale must be a power of two");
+    PSUM_ARG_CHECK(std::fabs(w) * scale <= 0x1p50, "psum_hip_deposit_cic: |w| * scale must not exceed 2^50");
     if (dtype == PSUM_F64)
         return launch_push<double>(g, pos, nullptr, n, nullptr, 0, 0, 0, w, scale, acc, nullptr,
                                    counters, false, S(stream));
@@ -1241,6 +1293,8 @@ psum_status psum_hip_gather_push(const psum_grid* g, int dtype, void* const* pos
     PSUM_ARG_CHECK(grid_ok(g), "psum_hip_gather_push: invalid grid");
     PSUM_ARG_CHECK(n == 0 || (pos && vel && phi), "psum_hip_gather_push: null pointer");
     PSUM_ARG_CHECK(!acc || pow2(scale), "psum_hip_gather_push: scale must be a power of two");
+    PSUM_ARG_CHECK(!acc || std::fabs(w_dep) * scale <= 0x1p50,
+                   "psum_hip_gather_push: |w_dep| * scale must not exceed 2^50");
     if (dtype == PSUM_F64)
         return launch_push<double>(g, pos, vel, n, phi, qm, dt, mass, w_dep, scale, acc, diag,
                                    counters, true, S(stream));

@@ -332,7 +332,25 @@ public:
         sp.n = n;
         acc_valid_ = false;
         step_ = 0;          // a new run: the sort cadence restarts
+        S_TRY(hipMemsetAsync(counters_ + 3, 0, sizeof(int64_t), st_));   // the accumulator's overflow flag
         S_TRY(hipStreamSynchronize(st_));
+    }
+
+    // Throws when an add to the int64 charge accumulator wrapped since the
+    // particles were loaded (counters [3]; the kernels count every wrap).
+    void check_overflow(const int64_t* wraps_host)
+    {
+        if (*wraps_host > 0)
+            throw std::overflow_error("psum_stepper: the int64 charge accumulator overflowed (" +
+                                      std::to_string(*wraps_host) +
+                                      " wrapped adds): a node collected more charge than 2^63 / 2^S holds");
+    }
+    void check_overflow()
+    {
+        int64_t w = 0;
+        S_TRY(hipMemcpyAsync(&w, counters_ + 3, sizeof w, hipMemcpyDeviceToHost, st_));
+        S_TRY(hipStreamSynchronize(st_));
+        check_overflow(&w);
     }
 
     void state(int s, void** pos, void** vel, int64_t* n)
@@ -361,8 +379,11 @@ public:
         if (hist_host)
             S_TRY(hipMemcpyAsync(hist_host, hist_, (size_t)steps * 5 * sizeof(double), hipMemcpyDeviceToHost, st_));
         if (!sync) return;
+        int64_t wraps = 0;
+        S_TRY(hipMemcpyAsync(&wraps, counters_ + 3, sizeof wraps, hipMemcpyDeviceToHost, st_));
         S_TRY(hipStreamSynchronize(st_));
         if (timing_) flush_events();
+        check_overflow(&wraps);
     }
 
     void reserve_hist(int64_t rows)
@@ -1094,7 +1115,7 @@ public:
         if (cycles) *cycles = cycles_;
         if (rel) *rel = last_rel_;
         if (counters) {
-            S_TRY(hipMemcpyAsync(counters, counters_, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st_));
+            S_TRY(hipMemcpyAsync(counters, counters_, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st_));   // [3]: int64 wraps
             S_TRY(hipStreamSynchronize(st_));
         }
     }
@@ -1398,6 +1419,7 @@ public:
         float ms = 0;
         S_TRY(hipEventElapsedTime(&ms, t0_, t1_));
         if (t_ms) *t_ms = ms;
+        for (auto& p : inst_) p->check_overflow();
     }
 
     void hist_reset(int64_t rows)

@@ -690,7 +690,7 @@ class Stepper:
         steps = C.c_int64(0)
         check(lib().psum_stepper_step_count(self.h, C.byref(steps)), "step_count")
         return {"cycles": cyc.value, "last_rel_res": rel.value, "oob": cnt[0],
-                "tile_fallback_blocks": cnt[1], "absorbed": cnt[2], "steps": steps.value}
+                "tile_fallback_blocks": cnt[1], "absorbed": cnt[2], "acc_wraps": cnt[3], "steps": steps.value}
 
     def set_timing(self, on=True):
         check(lib().psum_stepper_set_timing(self.h, int(on)), "set_timing")

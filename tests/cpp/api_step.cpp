@@ -1,7 +1,12 @@
 // One step of the reference user loop written against the C++ particle /
 // field API (psum/psum.hpp), compared on the GPU with the fused C ABI
 // kernels on the same particles:
-//   deposit   add_back in for_each (FP64 atomics)  vs  psum_hip_deposit_cic (fixed point)
+//   deposit   add_back in for_each (FP64 atomics)  vs  psum_hip_deposit_cic (fixed point);
+//             add_back after use_fixed_point: bitwise the fused kernel's, order independent,
+//             overflow of the int64 sums raised
+//   vector-valued fields (QuantitySize 2, 3): interp / interp_diff / add_back /
+//             nearest per component == scalar fields, MAS block shape, plot
+//   insert    reuse of invalidated slots
 //   push      interp_diff + kick + drift + wrap     vs  psum_hip_gather_push (bit-exact)
 //   compress  invalidated particles dropped, survivors in order
 //   shuffle   a permutation of the records
@@ -117,6 +122,39 @@ int main()
         mx = std::max(mx, std::fabs(fx));
     }
     report("add_back vs fixed-point deposit", err <= 1e-12 * mx, err / mx);
+
+    // ---- deterministic add_back: after use_fixed_point the accessor's
+    // add_back adds int64 terms, folded in when for_each ends; the result is
+    // bitwise the fused kernel's, and again after the records are shuffled
+    {
+        node_field2D<double> rfx(st, grid);
+        const double sc = rfx.use_fixed_point(std::fabs(w));
+        auto deposit = [&]() {
+            rfx.setZero();
+            group.for_each([&](psum::handler& h) {
+                auto acc = rfx.get_access(h);
+                return [=](Particle& p) { add_back(get<position>(p), w, acc); };
+            });
+            return rfx.to_host();
+        };
+        const std::vector<double> r1 = deposit();
+        size_t bad = sc == scale ? 0 : 1;
+        for (size_t i = 0; i < nn; ++i) {
+            const double ref = (double)acc_h[i] * (1.0 / scale);
+            bad += std::memcmp(&ref, &r1[i], 8) != 0;
+        }
+        report("fixed-point add_back == fused deposit (bitwise)", bad == 0, (double)bad);
+        Group shuffled(st, N);
+        shuffled.insert(group.get_content().to_host());
+        shuffled.shuffle(12345);
+        rfx.setZero();
+        shuffled.for_each([&](psum::handler& h) {
+            auto acc = rfx.get_access(h);
+            return [=](Particle& p) { add_back(get<position>(p), w, acc); };
+        });
+        const std::vector<double> r2 = rfx.to_host();
+        report("fixed-point add_back is order independent", r1 == r2, (double)r2.size());
+    }
 
     // ---- push: interp_diff + kick + drift + wrap (TSI/main.cpp:59-75 in 2D)
     node_field2D<double> phi(st, grid);
@@ -602,8 +640,304 @@ int main()
         psum::particle_container::device_vector<Particle> drecs(st, recs);
         Group g3(st, 64);
         g3.insert(drecs);
-        report("particle_group insert(device_vector) lists invalid records",
-               g3.size() == 90 && g3.get_invalid_indexes().size() == 10, (double)g3.size());
+        report("particle_group insert(device_vector) drops invalid records",
+               g3.size() == 90 && g3.get_content().size() == 90 && g3.get_invalid_indexes().size() == 0,
+               (double)g3.size());
+    }
+
+    // ---- vector-valued fields (QuantitySize 3): each component behaves as
+    // a scalar field of its own, bit for bit
+    {
+        using V3 = vec<3>;
+        node_field2D<double, 3> f3(st, grid);
+        node_field2D<double> fa(st, grid), fb(st, grid), fc(st, grid);
+        auto comp_fn = [](int q, const vec<2>& x) {
+            return q == 0 ? 2.0 * x[0] - 3.0 * x[1] + 1.0 : (q == 1 ? std::sin(0.1 * x[0]) * x[1] : 0.5 - 0.01 * x[0] * x[0]);
+        };
+        f3.for_each([&](psum::handler&) {
+            return [=](size_t, V3& v, const vec<2>& x) {
+                for (int q = 0; q < 3; ++q) v[q] = comp_fn(q, x);
+            };
+        });
+        node_field2D<double>* sc3[3] = {&fa, &fb, &fc};
+        for (int q = 0; q < 3; ++q)
+            sc3[q]->for_each([&](psum::handler&) {
+                return [=](size_t, double& v, const vec<2>& x) { v = comp_fn(q, x); };
+            });
+        const auto h3 = f3.to_host();
+        const auto ha = fa.to_host(), hb = fb.to_host(), hc3 = fc.to_host();
+        size_t bad = 0;
+        for (size_t i = 0; i < nn; ++i) bad += h3[i][0] != ha[i] || h3[i][1] != hb[i] || h3[i][2] != hc3[i];
+        report("QS=3 for_each(i, v, x) == three scalar fields", bad == 0, (double)bad);
+
+        // interp / interp_diff per component, on the device and the host
+        const size_t m = 4096;
+        double *dv3 = nullptr, *ds3 = nullptr;
+        psum::hip_check(hipMalloc(&dv3, m * 9 * 8), "m");
+        psum::hip_check(hipMalloc(&ds3, m * 9 * 8), "m");
+        std::vector<double> xy(2 * m);
+        std::uniform_real_distribution<double> UL(-1.0, L + 1.0);
+        for (auto& v : xy) v = UL(rng);
+        double* dxy = nullptr;
+        psum::hip_check(hipMalloc(&dxy, 2 * m * 8), "m");
+        psum::hip_check(hipMemcpy(dxy, xy.data(), 2 * m * 8, hipMemcpyHostToDevice), "m");
+        {
+            auto a3 = f3.acc();
+            auto aa = fa.acc(), ab = fb.acc(), ac = fc.acc();
+            psum::particle_container::device_vector<int> idx(st, m);
+            std::vector<int> hidx(m);
+            for (size_t i = 0; i < m; ++i) hidx[i] = (int)i;
+            idx.copy(hidx);
+            idx.for_each([&](psum::handler&) {
+                return [=](int& i) {
+                    psum::vec<2> p;
+                    p[0] = dxy[2 * i];
+                    p[1] = dxy[2 * i + 1];
+                    const V3 v = interp(p, a3);
+                    const auto g = interp_diff(p, a3);
+                    const double s[3] = {interp(p, aa), interp(p, ab), interp(p, ac)};
+                    const auto ga = interp_diff(p, aa), gb = interp_diff(p, ab), gc = interp_diff(p, ac);
+                    for (int q = 0; q < 3; ++q) {
+                        dv3[9 * i + q] = v[q];
+                        dv3[9 * i + 3 + q] = g[0][q];
+                        dv3[9 * i + 6 + q] = g[1][q];
+                        ds3[9 * i + q] = s[q];
+                    }
+                    ds3[9 * i + 3] = ga[0];
+                    ds3[9 * i + 4] = gb[0];
+                    ds3[9 * i + 5] = gc[0];
+                    ds3[9 * i + 6] = ga[1];
+                    ds3[9 * i + 7] = gb[1];
+                    ds3[9 * i + 8] = gc[1];
+                };
+            });
+        }
+        std::vector<double> hv3(9 * m), hs3(9 * m);
+        psum::hip_check(hipMemcpy(hv3.data(), dv3, m * 9 * 8, hipMemcpyDeviceToHost), "m");
+        psum::hip_check(hipMemcpy(hs3.data(), ds3, m * 9 * 8, hipMemcpyDeviceToHost), "m");
+        report("QS=3 interp / interp_diff == per-component scalar (bitwise)",
+               std::memcmp(hv3.data(), hs3.data(), m * 9 * 8) == 0, (double)m);
+        host_node_field2D<double, 3> hf3(f3);
+        size_t hbad = 0;
+        for (size_t i = 0; i < m; ++i) {
+            const std::array<double, 2> p{xy[2 * i], xy[2 * i + 1]};
+            const V3 v = interp(p, hf3);
+            const auto g = interp_diff(p, hf3);
+            for (int q = 0; q < 3; ++q)
+                hbad += v[q] != hv3[9 * i + q] || g[0][q] != hv3[9 * i + 3 + q] || g[1][q] != hv3[9 * i + 6 + q];
+        }
+        report("QS=3 host_field interp / interp_diff == device (bitwise)", hbad == 0, (double)hbad);
+
+        // deterministic add_back of a row weight == three scalar deposits
+        node_field2D<double, 3> r3(st, grid);
+        node_field2D<double> ra(st, grid), rb(st, grid), rc(st, grid);
+        const double wq[3] = {0.25, -0.5, 0.125};
+        r3.use_fixed_point(0.5);
+        ra.use_fixed_point(0.5);
+        rb.use_fixed_point(0.5);
+        rc.use_fixed_point(0.5);
+        V3 w3;
+        for (int q = 0; q < 3; ++q) w3[q] = wq[q];
+        group.for_each([&](psum::handler& h) {
+            auto a3 = r3.get_access(h);
+            auto aa = ra.get_access(h), ab = rb.get_access(h), ac = rc.get_access(h);
+            return [=](Particle& p) {
+                add_back(get<position>(p), w3, a3);
+                add_back(get<position>(p), wq[0], aa);
+                add_back(get<position>(p), wq[1], ab);
+                add_back(get<position>(p), wq[2], ac);
+            };
+        });
+        const auto g3h = r3.to_host();
+        const auto gah = ra.to_host(), gbh = rb.to_host(), gch = rc.to_host();
+        size_t dbad = 0;
+        double tot = 0;
+        for (size_t i = 0; i < nn; ++i) {
+            dbad += g3h[i][0] != gah[i] || g3h[i][1] != gbh[i] || g3h[i][2] != gch[i];
+            tot += gah[i];
+        }
+        report("QS=3 fixed-point add_back == three scalar deposits", dbad == 0 && tot != 0.0, (double)dbad);
+
+        // nearest-element interp and add_back on node and cell fields
+        cell_field2D<double, 2> c2(st, grid);
+        std::vector<vec<2>> c2h(c2.size());
+        for (size_t i = 0; i < c2h.size(); ++i) {
+            c2h[i][0] = (double)i;
+            c2h[i][1] = -0.5 * (double)i;
+        }
+        c2.copy(c2h);
+        cell_field2D<double, 2> c2d(st, grid);
+        c2d.use_fixed_point(1.0);
+        double* dn2 = nullptr;
+        psum::hip_check(hipMalloc(&dn2, m * 4 * 8), "m");
+        {
+            auto ac = c2.acc();
+            auto an = f3.acc();
+            auto ad = c2d.acc();
+            psum::particle_container::device_vector<int> idx(st, m);
+            std::vector<int> hidx(m);
+            for (size_t i = 0; i < m; ++i) hidx[i] = (int)i;
+            idx.copy(hidx);
+            idx.for_each([&](psum::handler&) {
+                return [=](int& i) {
+                    psum::vec<2> p;
+                    p[0] = dxy[2 * i];
+                    p[1] = dxy[2 * i + 1];
+                    const vec<2> c = interp_nearest(p, ac);
+                    const V3 v = interp_nearest(p, an);
+                    dn2[4 * i] = c[0];
+                    dn2[4 * i + 1] = c[1];
+                    dn2[4 * i + 2] = v[2];
+                    vec<2> one;
+                    one[0] = 1.0;
+                    one[1] = 2.0;
+                    add_back_nearest(p, one, ad);
+                    dn2[4 * i + 3] = 0.0;
+                };
+            });
+            c2d.fold_fixed_point();
+        }
+        std::vector<double> hn2(4 * m);
+        psum::hip_check(hipMemcpy(hn2.data(), dn2, m * 4 * 8, hipMemcpyDeviceToHost), "m");
+        const auto cdh = c2d.to_host();
+        std::vector<double> cnt(c2.size(), 0.0);
+        size_t nbad = 0;
+        const int ncx = grid.numCells<0>(), ncy = grid.numCells<1>();
+        for (size_t i = 0; i < m; ++i) {
+            const double tx = (xy[2 * i] - 0.5), ty = (xy[2 * i + 1] - 0.5);
+            const double fx = std::floor(tx), fy = std::floor(ty);
+            const bool cin = fx >= 0 && fx < ncx && fy >= 0 && fy < ncy;
+            const double ci = cin ? fx * ncy + fy : -1;
+            if (cin) cnt[(size_t)ci] += 1.0;
+            nbad += hn2[4 * i] != (cin ? ci : 0.0) || hn2[4 * i + 1] != (cin ? -0.5 * ci : 0.0);
+            const double rx = std::round(tx), ry = std::round(ty);
+            const bool nin = rx >= 0 && rx <= ncx && ry >= 0 && ry <= ncy;
+            const double nv = nin ? h3[(size_t)(rx * (ncy + 1) + ry)][2] : 0.0;
+            nbad += hn2[4 * i + 2] != nv;
+        }
+        for (size_t i = 0; i < cnt.size(); ++i) nbad += cdh[i][0] != cnt[i] || cdh[i][1] != 2.0 * cnt[i];
+        report("interp_nearest / add_back_nearest (node, cell, QS=2,3)", nbad == 0, (double)nbad);
+        psum::hip_check(hipFree(dn2), "f");
+        psum::hip_check(hipFree(dxy), "f");
+        psum::hip_check(hipFree(dv3), "f");
+        psum::hip_check(hipFree(ds3), "f");
+
+        // MAS round trip of a QS=3 device field: an n x 1 x 3 block
+        const std::string path = "/tmp/psum_api_step_f3.mas";
+        {
+            mas_file f(path, mas_file::replaceMode);
+            psum::serialization::save(f, "B", f3);
+        }
+        node_field2D<double, 3> back(st, grid);
+        host_node_field2D<double, 3> hback;
+        std::vector<size_t> shp;
+        {
+            mas_file f(path);
+            psum::serialization::load(f, "B", back);
+            (void)f.read<double>("B.content.content", &shp);
+        }
+        const auto bh = back.to_host();
+        bool same = shp.size() == 3 && shp[0] == nn && shp[1] == 1 && shp[2] == 3;
+        for (size_t i = 0; same && i < nn; ++i) same = bh[i][0] == h3[i][0] && bh[i][1] == h3[i][1] && bh[i][2] == h3[i][2];
+        report("QS=3 device_field MAS save/load (n x 1 x 3)", same, (double)nn);
+        std::remove(path.c_str());
+
+        // Tecplot text of a two-component cell field
+        grid2D pg2({0.0, 0.0}, {2.0, 1.0}, {2, 1});
+        cell_field2D<double, 2> cf2(st, pg2);
+        std::vector<vec<2>> v2(2);
+        v2[0][0] = 0.25;
+        v2[0][1] = 1.0;
+        v2[1][0] = 0.75;
+        v2[1][1] = 2.0;
+        cf2.copy(v2);
+        const std::string pc = "/tmp/psum_api_step_c2.plt";
+        cf2.plot(pc);
+        std::ifstream in(pc);
+        const std::string got((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        const std::string exp =
+            "variables=x,y,v1,v2\nzone i=2,j=3,F=block, VARLOCATION=([3-4]=CELLCENTERED)\n"
+            "\n0.0000e+00\n0.0000e+00\n1.0000e+00\n1.0000e+00\n2.0000e+00\n2.0000e+00"
+            "\n0.0000e+00\n1.0000e+00\n0.0000e+00\n1.0000e+00\n0.0000e+00\n1.0000e+00"
+            "\n2.5000e-01\n7.5000e-01\n1.0000e+00\n2.0000e+00";
+        if (got != exp) std::printf("  QS=2 cell plot:\n%s\n", got.c_str());
+        report("plot: Tecplot text of a QS=2 cell field", got == exp, (double)got.size());
+        std::remove(pc.c_str());
+    }
+
+    // ---- fixed-point overflow: a term beyond 2^62, and int64 sums that wrap,
+    // raise std::overflow_error when the deposit is folded
+    {
+        grid2D g1({0.0, 0.0}, {4.0, 4.0}, {4, 4});
+        std::vector<Particle> at(1 << 14);
+        for (auto& p : at) {
+            get<position>(p).x() = 2.0;   // exactly on node (2, 2): one term, coefficient 1
+            get<position>(p).y() = 2.0;
+            get<velocity>(p).x() = 0.0;
+            get<velocity>(p).y() = 0.0;
+        }
+        Group pile(st, at.size());
+        pile.insert(at);
+        auto run = [&](node_field2D<double>& f, double wt) {
+            try {
+                pile.for_each([&](psum::handler& h) {
+                    auto a = f.get_access(h);
+                    return [=](Particle& p) { add_back(get<position>(p), wt, a); };
+                });
+            } catch (const std::overflow_error&) {
+                return true;
+            }
+            return false;
+        };
+        node_field2D<double> ok(st, g1), big(st, g1), wrap(st, g1);
+        ok.use_fixed_point(1.0);
+        big.use_fixed_point(1e-9);      // 2^S = 2^69: a unit term is 2^69
+        wrap.use_fixed_point(1.0);      // 2^S = 2^39: terms of 2^50, 2^14 of them sum to 2^64
+        const bool t_ok = run(ok, 1.0), t_big = run(big, 1.0), t_wrap = run(wrap, 2048.0);
+        const auto okh = ok.to_host();
+        const double expect_ok = (double)at.size();
+        report("fixed-point overflow: flagged when terms or sums leave int64",
+               !t_ok && t_big && t_wrap && okh[g1.n2i({2, 2})] == expect_ok, okh[g1.n2i({2, 2})]);
+    }
+
+    // ---- insert reuses the slots of invalidated records (particle_group.hpp:103-145)
+    {
+        std::vector<Particle> base(100);
+        for (size_t i = 0; i < base.size(); ++i) {
+            get<position>(base[i]).x() = 1.0 + (double)i;
+            get<position>(base[i]).y() = 0.0;
+        }
+        Group g4(st, 256);
+        g4.insert(base);
+        g4.for_each([&](psum::handler&) {
+            return [=](Particle& p) {
+                if (((long)get<position>(p).x() - 1) % 10 == 3) pos_x_nan_is_invalid<Particle>::make_invalid(p);
+            };
+        });
+        std::vector<Particle> fresh(14);
+        for (size_t i = 0; i < fresh.size(); ++i) {
+            get<position>(fresh[i]).x() = 1000.0 + (double)i;
+            get<position>(fresh[i]).y() = 1.0;
+        }
+        // 4 records: the highest 4 of the 10 slots (93, 83, 73, 63), size unchanged
+        g4.insert(std::vector<Particle>(fresh.begin(), fresh.begin() + 4));
+        auto c1 = g4.get_content().to_host();
+        bool ok = g4.size() == 94 && c1.size() == 100 && g4.get_invalid_indexes().size() == 6;
+        const size_t hi[4] = {93, 83, 73, 63};
+        for (int j = 0; ok && j < 4; ++j) ok = get<position>(c1[hi[j]]).x() == 1000.0 + j;
+        // 10 more: the 6 remaining slots in ascending order, then 4 appended
+        g4.insert(std::vector<Particle>(fresh.begin() + 4, fresh.end()));
+        auto c2 = g4.get_content().to_host();
+        ok = ok && g4.size() == 104 && c2.size() == 104 && g4.get_invalid_indexes().size() == 0;
+        const size_t lo[6] = {3, 13, 23, 33, 43, 53};
+        for (int j = 0; ok && j < 6; ++j) ok = get<position>(c2[lo[j]]).x() == 1004.0 + j;
+        for (int j = 0; ok && j < 4; ++j) ok = get<position>(c2[100 + j]).x() == 1010.0 + j;
+        // invalid incoming records are dropped before insertion
+        std::vector<Particle> mixed(fresh.begin(), fresh.begin() + 3);
+        pos_x_nan_is_invalid<Particle>::make_invalid(mixed[1]);
+        g4.insert(mixed);
+        ok = ok && g4.size() == 106 && g4.get_content().size() == 106;
+        report("particle_group insert reuses invalid slots", ok, (double)g4.size());
     }
 
     // ---- plot: the Tecplot text of a node and a cell field, character for character

@@ -171,6 +171,39 @@ def test_deposit_lds_fallback_counter():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_deposit_overflow_flag_counts_int64_wraps(dtype):
+    """counters[3] is the int64 accumulator's overflow flag.  Particles stacked
+    on one node each add a term of 2^50 (w = 2^10, scale = 2^40): 2^12 of them
+    sum to 2^62 (exact, flag clear), 2^14 pass 2^63 (flagged).  The CIC
+    deposit, the fused push + deposit and the quadratic B-spline deposit
+    (centre weight 9/16: 2^14 terms reach 1.125 * 2^63) all count it."""
+    E = _E()
+    eg = E.periodic_grid([16, 16])
+    x0 = [5.5, 7.5]                               # node (5, 7): coefficient 1
+    for n, wraps in ((1 << 12, False), (1 << 14, True)):
+        print(f"[overflow] dtype={np.dtype(dtype).name} n={n}", flush=True)
+        pos = [_dev(np.full(n, x, dtype)) for x in x0]
+        cnt = torch.zeros(4, dtype=torch.int64, device="cuda")
+        acc, _ = E.deposit(eg, pos, 1024.0, scale=2.0**40, counters=cnt)
+        c = _host(cnt)
+        assert (c[3] > 0) == wraps, c
+        if not wraps:
+            a = _host(acc)
+            assert a[5 * 16 + 7] == n * 2**50 and np.count_nonzero(a) == 1
+        cnt.zero_()
+        vel = [_dev(np.zeros(n, dtype)) for _ in range(2)]
+        phi = _dev(np.zeros(256, dtype))
+        E.gather_push(eg, pos, vel, phi, -1.0, 0.1, 1.0, w_dep=1024.0, scale=2.0**40, counters=cnt)
+        assert (_host(cnt)[3] > 0) == wraps
+        cnt.zero_()
+        E.deposit_bspline(eg, pos, 1024.0, 2, scale=2.0**40, counters=cnt)
+        assert (_host(cnt)[3] > 0) == wraps
+    # a term above 2^50 could wrap an LDS tile entry: refused
+    with pytest.raises(Exception, match="2\\^50"):
+        E.deposit(eg, pos, 2048.0, scale=2.0**40)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_gather_push_bit_exact(dtype):
     E = _E()
     rng = np.random.default_rng(33)

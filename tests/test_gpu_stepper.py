@@ -142,3 +142,25 @@ def test_sort_order_is_cell_then_velocity_signs():
     p1, v1 = st.state(0)
     for a, b in zip(p0 + v0, p1 + v1):
         np.testing.assert_array_equal(a[order], b)
+
+
+def test_charge_accumulator_overflow_fails_the_run():
+    """2^24 ions stacked on one node add 2^24 terms of 2^39 (the stepper's
+    scale keeps one term in [2^39, 2^40)), 2^63 in all, one past the int64
+    range (a negative charge would end exactly on -2^63, still in range):
+    the run reports the accumulator's overflow instead of stepping on a
+    wrapped charge density, and stats() shows the wrapped adds."""
+    from psum_amd import engine as E
+    n = 1 << 24
+    g = E.periodic_grid([8, 8])
+    sp = [dict(q=1.0, m=1.0, w=1.0 / 16, pos=[np.full(n, 2.5), np.full(n, 3.5)],
+               vel=[np.zeros(n), np.zeros(n)])]
+    st = E.Stepper(g, sp, dtype="f64", depth=2, max_cycles=4, tol=1e-6, dt=0.1, rho_bg=-n / 16 / 64)
+    with pytest.raises(Exception, match="overflow"):
+        st.run(1)
+    assert st.stats()["acc_wraps"] > 0
+    # a fresh load clears the flag; a harmless state runs
+    small = 1 << 12
+    st.load(0, [np.full(small, 2.5), np.full(small, 3.5)], [np.zeros(small), np.zeros(small)])
+    st.run(1)
+    assert st.stats()["acc_wraps"] == 0
