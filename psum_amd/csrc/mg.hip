@@ -198,15 +198,12 @@ __device__ __forceinline__ T stencil_apply(const DevLevel& L, const T* __restric
 // x_out = relax x_in + b * (2/3)   (multiply_add(relax, x, b, buf, 2/3, 1));
 // ACC: also xacc += x_out in FP64 (the mixed refinement's x += e, fused into
 // the last level-0 sweep)
-template <typename T, int D, bool COMPACT, bool ACC = false>
-__global__ __launch_bounds__(kMgThreads) void k_smooth(DevLevel L, DevTable<T> relax,
-                                                       const T* __restrict__ xin,
-                                                       const T* __restrict__ b,
-                                                       T* __restrict__ xout, T two3,
-                                                       double* __restrict__ xacc = nullptr)
+template <typename T, int D, bool COMPACT, bool ACC>
+__device__ __forceinline__ void smooth_node(const DevLevel& L, const DevTable<T>& relax,
+                                            const T* __restrict__ xin, const T* __restrict__ b,
+                                            T* __restrict__ xout, T two3, double* __restrict__ xacc,
+                                            int64_t i)
 {
-    const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
-    if (i >= L.N) return;
     int c[3];
     node_coords<D>(L, i, c);
     const T* coef = relax.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
@@ -216,7 +213,30 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth(DevLevel L, DevTable<T> r
     if (ACC) xacc[i] += (double)v;
 }
 
+template <typename T, int D, bool COMPACT, bool ACC = false>
+__global__ __launch_bounds__(kMgThreads) void k_smooth(DevLevel L, DevTable<T> relax,
+                                                       const T* __restrict__ xin,
+                                                       const T* __restrict__ b,
+                                                       T* __restrict__ xout, T two3,
+                                                       double* __restrict__ xacc = nullptr)
+{
+    const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
+    if (i >= L.N) return;
+    smooth_node<T, D, COMPACT, ACC>(L, relax, xin, b, xout, two3, xacc, i);
+}
+
 // r = (A^ x)*(-1) + b*((-1)*(-1))   (multiply_add(A, x, b, buf, -1, -1))
+template <typename T, int D, bool COMPACT>
+__device__ __forceinline__ void residual_node(const DevLevel& L, const DevTable<T>& A, const T* __restrict__ x,
+                                              const T* __restrict__ b, T* __restrict__ r, int64_t i)
+{
+    int c[3];
+    node_coords<D>(L, i, c);
+    const T* coef = A.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
+    const T t = stencil_apply<T, D, COMPACT>(L, coef, x, c);
+    r[i] = t * (T)-1 + b[i] * (T)1;
+}
+
 template <typename T, int D, bool COMPACT>
 __global__ __launch_bounds__(kMgThreads) void k_residual(DevLevel L, DevTable<T> A,
                                                          const T* __restrict__ x,
@@ -225,11 +245,161 @@ __global__ __launch_bounds__(kMgThreads) void k_residual(DevLevel L, DevTable<T>
 {
     const int64_t i = level_block<D>() * kMgThreads + threadIdx.x;
     if (i >= L.N) return;
-    int c[3];
-    node_coords<D>(L, i, c);
-    const T* coef = A.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
-    const T t = stencil_apply<T, D, COMPACT>(L, coef, x, c);
-    r[i] = t * (T)-1 + b[i] * (T)1;
+    residual_node<T, D, COMPACT>(L, A, x, b, r, i);
+}
+
+// ---- 3D 7-point levels in chunks of kLine consecutive nodes ---------------
+// On a compact (7-point) 3D level nearly every node lies inside the box and
+// carries the same stencil class.  A chunk of kLine nodes (linear indices
+// kLine*t .. kLine*t + kLine-1) all on one grid line, strictly inside the
+// box, all of that class, is updated by one thread with the class row in
+// registers and vector loads of the line and its four neighbour lines (no
+// coordinate division per node, no clamping, no per-node table reads).  The
+// per-node terms and their order are stencil_apply's, so the result is
+// bit-identical to k_smooth / k_residual.  The other nodes (box faces, line
+// ends, other classes, the tail) are listed when the level is built and
+// updated by the per-node kernels over that list.
+constexpr int kLine = 4;
+
+struct LineGeom {
+    int n0, n1, n2;
+    uint32_t nchunks;          // full chunks: N / kLine
+    const uint16_t* cls;
+    int cls_in;                // the class the chunk kernels handle
+};
+
+// the chunk of nodes i .. i+kLine-1 (i = kLine*t) is a chunk-kernel chunk
+// (the host lists the others with line_chunk_fast_host, the same test)
+__device__ __forceinline__ bool line_chunk_fast(const LineGeom& G, uint32_t i)
+{
+    const uint32_t q = i / (uint32_t)G.n2;
+    const uint32_t z = i - q * (uint32_t)G.n2;
+    const uint32_t x = q / (uint32_t)G.n1;
+    const uint32_t y = q - x * (uint32_t)G.n1;
+    if (z < 1u || z + kLine > (uint32_t)G.n2 - 1u || y < 1u || y + 2u > (uint32_t)G.n1 || x < 1u ||
+        x + 2u > (uint32_t)G.n0)
+        return false;
+    unsigned long long c4;
+    __builtin_memcpy(&c4, G.cls + i, sizeof c4);
+    const unsigned long long in = (unsigned long long)(uint16_t)G.cls_in;
+    return c4 == (in | in << 16 | in << 32 | in << 48);
+}
+
+template <typename T>
+__device__ __forceinline__ void ld_line(const T* __restrict__ p, T* v)
+{
+    struct alignas(sizeof(T)) Q { T a[kLine]; } q;
+    __builtin_memcpy(&q, p, sizeof q);
+#pragma unroll
+    for (int k = 0; k < kLine; ++k) v[k] = q.a[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void st_line(T* __restrict__ p, const T* v)
+{
+    struct alignas(sizeof(T)) Q { T a[kLine]; } q;
+#pragma unroll
+    for (int k = 0; k < kLine; ++k) q.a[k] = v[k];
+    __builtin_memcpy(p, &q, sizeof q);
+}
+
+// the 7-point sum of stencil_apply<T, 3, true> at the chunk's nodes: terms
+// in offset order x-1, y-1, z-1, centre, z+1, y+1, x+1
+template <typename T>
+__device__ __forceinline__ void line_apply(const LineGeom& G, const T* __restrict__ row,
+                                           const T* __restrict__ x, uint32_t i, T* s)
+{
+    const T cxm = row[4], cym = row[10], czm = row[12], cc = row[13], czp = row[14], cyp = row[16],
+            cxp = row[22];
+    const uint32_t sy = (uint32_t)G.n2, sx = (uint32_t)G.n1 * (uint32_t)G.n2;
+    T xm[kLine], ym[kLine], zz[kLine + 2], yp[kLine], xp[kLine];
+    ld_line(x + (i - sx), xm);
+    ld_line(x + (i - sy), ym);
+    zz[0] = x[i - 1];
+    ld_line(x + i, zz + 1);
+    zz[kLine + 1] = x[i + kLine];
+    ld_line(x + (i + sy), yp);
+    ld_line(x + (i + sx), xp);
+#pragma unroll
+    for (int k = 0; k < kLine; ++k) {
+        T t = (T)0;
+        t = t + cxm * xm[k];
+        t = t + cym * ym[k];
+        t = t + czm * zz[k];
+        t = t + cc * zz[k + 1];
+        t = t + czp * zz[k + 2];
+        t = t + cyp * yp[k];
+        t = t + cxp * xp[k];
+        s[k] = t;
+    }
+}
+
+template <typename T, bool ACC>
+__global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, const T* __restrict__ row,
+                                                            const T* __restrict__ xin,
+                                                            const T* __restrict__ b,
+                                                            T* __restrict__ xout, T two3,
+                                                            double* __restrict__ xacc)
+{
+    const int64_t t = level_block<3>() * kMgThreads + threadIdx.x;
+    if (t >= (int64_t)G.nchunks) return;
+    const uint32_t i = (uint32_t)t * kLine;
+    if (!line_chunk_fast(G, i)) return;
+    T s[kLine], bb[kLine], v[kLine];
+    line_apply(G, row, xin, i, s);
+    ld_line(b + i, bb);
+#pragma unroll
+    for (int k = 0; k < kLine; ++k) v[k] = s[k] * (T)1 + bb[k] * (two3 * (T)1);
+    st_line(xout + i, v);
+    if (ACC) {
+        double a[kLine];
+        ld_line(xacc + i, a);
+#pragma unroll
+        for (int k = 0; k < kLine; ++k) a[k] += (double)v[k];
+        st_line(xacc + i, a);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, const T* __restrict__ row,
+                                                              const T* __restrict__ x,
+                                                              const T* __restrict__ b, T* __restrict__ r)
+{
+    const int64_t t = level_block<3>() * kMgThreads + threadIdx.x;
+    if (t >= (int64_t)G.nchunks) return;
+    const uint32_t i = (uint32_t)t * kLine;
+    if (!line_chunk_fast(G, i)) return;
+    T s[kLine], bb[kLine], v[kLine];
+    line_apply(G, row, x, i, s);
+    ld_line(b + i, bb);
+#pragma unroll
+    for (int k = 0; k < kLine; ++k) v[k] = s[k] * (T)-1 + bb[k] * (T)1;
+    st_line(r + i, v);
+}
+
+// the listed nodes of a chunked level, one per thread
+template <typename T, bool ACC>
+__global__ __launch_bounds__(kMgThreads) void k_smooth_list(DevLevel L, DevTable<T> relax,
+                                                            const uint32_t* __restrict__ list, uint32_t n,
+                                                            const T* __restrict__ xin,
+                                                            const T* __restrict__ b,
+                                                            T* __restrict__ xout, T two3,
+                                                            double* __restrict__ xacc)
+{
+    const uint32_t t = blockIdx.x * kMgThreads + threadIdx.x;
+    if (t >= n) return;
+    smooth_node<T, 3, true, ACC>(L, relax, xin, b, xout, two3, xacc, (int64_t)list[t]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_residual_list(DevLevel L, DevTable<T> A,
+                                                              const uint32_t* __restrict__ list, uint32_t n,
+                                                              const T* __restrict__ x,
+                                                              const T* __restrict__ b, T* __restrict__ r)
+{
+    const uint32_t t = blockIdx.x * kMgThreads + threadIdx.x;
+    if (t >= n) return;
+    residual_node<T, 3, true>(L, A, x, b, r, (int64_t)list[t]);
 }
 
 // b_c = (R r) o D_c^-1 ; x_c = 0, or, when the coarse level is pre-smoothed,
@@ -1714,6 +1884,11 @@ std::vector<T> cast_vec(const std::vector<double>& v)
 
 struct DevLevelBuf {
     uint16_t* cls = nullptr;
+    // chunked 7-point 3D level: nodes the chunk kernels leave to the
+    // per-node kernels, and the class the chunk kernels handle
+    uint32_t* line_list = nullptr;
+    uint32_t line_n = 0;
+    int line_cls = -1;
     double *K = nullptr, *Ah = nullptr, *Rx = nullptr, *dinv = nullptr;
     float *Ah32 = nullptr, *Rx32 = nullptr, *dinv32 = nullptr;
     // vectors: FP64 x/b/t and FP32 x/b/t
@@ -1798,6 +1973,7 @@ public:
             else if (key == "periodic_mode") iss >> periodic_mode_;
             else if (key == "fuse") { iss >> fuse_; resolve_tail(); }
             else if (key == "tail") { iss >> tail_; resolve_tail(); }
+            else if (key == "line") { iss >> line_; }
             else if (key == "graph") iss >> use_graph_;
             else if (key == "num_conv_mask" || key == "num_conv_mask_relax" ||
                      key == "num_conv_mask_restrict" || key == "num_conv_mask_interpolate" ||
@@ -1810,7 +1986,8 @@ public:
     }
 
     // Kernel-shape options that may change after set_matrix: "fuse 0|1"
-    // (2D fused tile kernels) and "tail 0|1" (the one-workgroup coarse end).
+    // (2D fused tile kernels), "tail 0|1" (the one-workgroup coarse end) and
+    // "line 0|1" (chunked sweeps and residuals on 7-point 3D levels).
     // Other keys are rejected.
     void set_tuning(const char* options)
     {
@@ -1821,6 +1998,7 @@ public:
             if (!(iss >> v)) throw std::invalid_argument("hip_multigrid: tuning option " + key + " needs a value");
             if (key == "tail") tail_ = v;
             else if (key == "fuse") fuse_ = v;
+            else if (key == "line") line_ = v;
             else throw std::invalid_argument("hip_multigrid: unknown tuning option " + key);
         }
         rr_fit_.clear();
@@ -2157,6 +2335,7 @@ private:
     bool mixed_;
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     int fuse_ = 1;   // 2D: fused multi-sweep tile smoothing and residual+restriction
+    int line_ = 1;   // 3D 7-point levels: chunk kernels (k_smooth_line, k_residual_line)
     // 2D: the coarse end of the V-cycle in one workgroup (k_vtail); off unless
     // "tail 1" is set
     int tail_ = 0;
@@ -2212,7 +2391,7 @@ private:
         for (auto& c : cap64_.e) drop_graph(c);
         for (auto& c : cap32_.e) drop_graph(c);
         for (auto& d : dev_) {
-            for (void* p : {(void*)d.cls, (void*)d.K, (void*)d.Ah, (void*)d.Rx, (void*)d.dinv,
+            for (void* p : {(void*)d.cls, (void*)d.line_list, (void*)d.K, (void*)d.Ah, (void*)d.Rx, (void*)d.dinv,
                             (void*)d.Ah32, (void*)d.Rx32, (void*)d.dinv32, (void*)d.t, (void*)d.bf,
                             (void*)d.xf, (void*)d.tf, (void*)d.b, (void*)d.x})
                 if (p) (void)hipFree(p);
@@ -2246,6 +2425,7 @@ private:
             const HostLevel& h = host_[l];
             DevLevelBuf& d = dev_[l];
             d.cls = dev_copy(h.cls);
+            build_line_list(h, d);
             d.K = dev_copy(h.K);
             d.Ah = dev_copy(h.Ah);
             d.Rx = dev_copy(h.Rx);
@@ -2284,6 +2464,49 @@ private:
         MG_HIP(hipMalloc(&shift_, 4 * sizeof(double)));
         if (pin_ >= 0) MG_HIP(hipMalloc(&bprime_, (size_t)host_[0].N * sizeof(double)));
         MG_HIP(hipHostMalloc(&red_host_, 2 * sizeof(double), hipHostMallocDefault));
+    }
+
+    // A 7-point 3D level of at least 2^16 nodes (and fewer than 2^32) runs
+    // its smoothing sweeps and residuals in chunks of kLine nodes: the class
+    // that covers most nodes goes to the chunk kernels, and every node of a
+    // chunk they skip (by the same test as line_chunk_fast) is listed.
+    static bool line_chunk_fast_host(const HostLevel& h, uint32_t i, int cls_in)
+    {
+        const uint32_t n0 = (uint32_t)h.n[0], n1 = (uint32_t)h.n[1], n2 = (uint32_t)h.n[2];
+        const uint32_t q = i / n2, z = i - q * n2, x = q / n1, y = q - x * n1;
+        if (z < 1u || z + kLine > n2 - 1u || y < 1u || y + 2u > n1 || x < 1u || x + 2u > n0) return false;
+        for (int k = 0; k < kLine; ++k)
+            if ((int)h.cls[i + k] != cls_in) return false;
+        return true;
+    }
+    void build_line_list(const HostLevel& h, DevLevelBuf& d)
+    {
+        d.line_cls = -1;
+        d.line_n = 0;
+        if (h.D != 3 || !h.compact || h.N < (1 << 16) || h.N >= (int64_t)0xffffffffll) return;
+        std::vector<int64_t> count(h.dinv.size(), 0);
+        for (int64_t i = 0; i < h.N; ++i) ++count[h.cls[i]];
+        const int cls_in = (int)(std::max_element(count.begin(), count.end()) - count.begin());
+        std::vector<uint32_t> list;
+        const uint32_t nchunks = (uint32_t)(h.N / kLine);
+        for (uint32_t t = 0; t < nchunks; ++t)
+            if (!line_chunk_fast_host(h, t * kLine, cls_in))
+                for (int k = 0; k < kLine; ++k) list.push_back(t * kLine + k);
+        for (int64_t i = (int64_t)nchunks * kLine; i < h.N; ++i) list.push_back((uint32_t)i);
+        d.line_cls = cls_in;
+        d.line_n = (uint32_t)list.size();
+        if (!list.empty()) d.line_list = dev_copy(list);
+    }
+    LineGeom line_geom(int l) const
+    {
+        LineGeom g;
+        g.n0 = host_[l].n[0];
+        g.n1 = host_[l].n[1];
+        g.n2 = host_[l].n[2];
+        g.nchunks = (uint32_t)(host_[l].N / kLine);
+        g.cls = dev_[l].cls;
+        g.cls_in = dev_[l].line_cls;
+        return g;
     }
 
     DevLevel dl(int l) const
@@ -2341,7 +2564,20 @@ private:
 #define SMOOTH(DIM, CMP) \
         if (acc) hipLaunchKernelGGL((k_smooth<T, DIM, CMP, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3, acc); \
         else hipLaunchKernelGGL((k_smooth<T, DIM, CMP, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3, nullptr)
-        if (host_[l].D == 2) {
+        if (line_ && dev_[l].line_cls >= 0) {
+            const LineGeom G = line_geom(l);
+            const DevTable<T> tb = table<T>(l, true);
+            const T* row = tb.coef + (int64_t)G.cls_in * 27;
+            const unsigned nc = nblk(G.nchunks);
+            const uint32_t nl = dev_[l].line_n;
+            if (acc) {
+                hipLaunchKernelGGL((k_smooth_line<T, true>), dim3(nc), dim3(kMgThreads), 0, stream_, G, row, x, b, buf, two3, acc);
+                if (nl) hipLaunchKernelGGL((k_smooth_list<T, true>), dim3(nblk(nl)), dim3(kMgThreads), 0, stream_, L, tb, dev_[l].line_list, nl, x, b, buf, two3, acc);
+            } else {
+                hipLaunchKernelGGL((k_smooth_line<T, false>), dim3(nc), dim3(kMgThreads), 0, stream_, G, row, x, b, buf, two3, nullptr);
+                if (nl) hipLaunchKernelGGL((k_smooth_list<T, false>), dim3(nblk(nl)), dim3(kMgThreads), 0, stream_, L, tb, dev_[l].line_list, nl, x, b, buf, two3, nullptr);
+            }
+        } else if (host_[l].D == 2) {
             if (host_[l].compact) { SMOOTH(2, true); } else { SMOOTH(2, false); }
         } else {
             if (host_[l].compact) { SMOOTH(3, true); } else { SMOOTH(3, false); }
@@ -2355,6 +2591,15 @@ private:
     {
         const DevLevel L = dl(l);
         const unsigned nb = nblk(L.N);
+        if (line_ && dev_[l].line_cls >= 0) {
+            const LineGeom G = line_geom(l);
+            const DevTable<T> tb = table<T>(l, false);
+            const T* row = tb.coef + (int64_t)G.cls_in * 27;
+            hipLaunchKernelGGL((k_residual_line<T>), dim3(nblk(G.nchunks)), dim3(kMgThreads), 0, stream_, G, row, x, b, r);
+            const uint32_t nl = dev_[l].line_n;
+            if (nl) hipLaunchKernelGGL((k_residual_list<T>), dim3(nblk(nl)), dim3(kMgThreads), 0, stream_, L, tb, dev_[l].line_list, nl, x, b, r);
+            return;
+        }
         if (host_[l].D == 2) {
             if (host_[l].compact) hipLaunchKernelGGL((k_residual<T, 2, true>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, false), x, b, r);
             else hipLaunchKernelGGL((k_residual<T, 2, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, false), x, b, r);

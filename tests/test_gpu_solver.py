@@ -143,9 +143,11 @@ def test_mixed_backend_converges_and_matches_fp64():
     assert np.max(np.abs(xb.cpu().numpy() - x_ref)) <= 1e-5 * np.max(np.abs(x_ref))
 
 
-def test_vcycle_matches_oracle_3d_walls():
+@pytest.mark.parametrize("n", [33, 49])
+def test_vcycle_matches_oracle_3d_walls(n):
+    """n = 49 (117649 nodes) runs the fine level's sweeps and residuals in
+    the chunk kernels."""
     E = _E()
-    n = 33
     og, nn, r, c, v, b, fixed = _wall_problem_3d(n, seed=3)
     mg = O.MG(nn, r, c, v, n, n, n, depth=3, pre=2, post=1)
     s = E.SolverBackend("hip_multigrid_gpu")
@@ -295,3 +297,31 @@ def test_vcycle_tail_is_bit_identical(backend, case, prepost):
         out.append((x.cpu().numpy(), s.stats()))
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1][0] == out[1][1][0]
+
+
+@pytest.mark.parametrize("backend", ["hip_multigrid_gpu", "hip_multigrid32f_gpu"])
+@pytest.mark.parametrize("n", [49, 68])
+@pytest.mark.parametrize("prepost", [(3, 3), (2, 1)])
+def test_chunked_3d_kernels_are_bit_identical(backend, n, prepost):
+    """On 7-point 3D levels of 2^16 nodes or more the sweeps and residuals run
+    four nodes per thread (interior nodes of the dominant class) plus a
+    per-node pass over the listed rest; the terms and their order are the
+    per-node kernels', so the bits equal those with the chunks switched off.
+    n = 49 gives lines whose length is not a multiple of four, 68 one that is
+    (chunks then never straddle a line)."""
+    E = _E()
+    og, nn, r, c, v, b, fixed = _wall_problem_3d(n, seed=7)
+    pre, post = prepost
+    out = []
+    for line in (0, 1):
+        s = E.SolverBackend(backend)
+        s.set_options(f"I {n} J {n} K {n} depth 4 pre_relax {pre} post_relax {post} "
+                      f"max_cycles 30 tol 1e-9 line {line}")
+        s.set_matrix(nn, nn, r, c, v)
+        x = torch.zeros(nn, dtype=torch.float64, device="cuda")
+        s.solve(_dev(b), x)
+        s.solve(_dev(b * 1.01), x)
+        out.append((x.cpu().numpy(), s.stats()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1][0] == out[1][1][0]
+    assert np.array_equal(np.float64(out[0][1][1]), np.float64(out[1][1][1]), equal_nan=True)
