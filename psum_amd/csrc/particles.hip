@@ -348,7 +348,66 @@ __device__ __forceinline__ void store4(T* __restrict__ p, const T* v)
     }
 }
 
-template <typename T, int D, bool PUSH>
+// Linear indices of the 2^D support nodes of cell cw, as support_index for
+// i = 0 .. 2^D-1: the cell's index plus, per dimension whose bit is set, one
+// stride, or minus cw * stride where cw + 1 wraps to node 0.
+template <int D>
+__device__ __forceinline__ void support_indices(const DevGrid& g, const int* cw, uint32_t* idx)
+{
+    uint32_t stride[3];
+    stride[D - 1] = 1u;
+#pragma unroll
+    for (int d = D - 2; d >= 0; --d) stride[d] = stride[d + 1] * (uint32_t)g.w[d + 1];
+    uint32_t base = 0u, step[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        base += (uint32_t)cw[d] * stride[d];
+        step[d] = cw[d] + 1 == g.w[d] ? 0u - (uint32_t)cw[d] * stride[d] : stride[d];
+    }
+#pragma unroll
+    for (int i = 0; i < (1 << D); ++i) {
+        uint32_t o = base;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if ((i >> (D - 1 - d)) & 1) o += step[d];
+        idx[i] = o;
+    }
+}
+
+// grad phi when every delta_r is 1.  Gradient weight i along dd is then
+// cic_grad_coef's product with G[dd] = -1 (bit of dd clear) or +1 in the
+// place of the dd factor; multiplying by +-1 is exact and rounding is
+// symmetric in sign, so the weight is +-q, q the product of the other
+// dimensions' W factors in the same order, and ph * weight = +-(ph * q).
+// The terms, their skips (|q| > 1e-10) and their order are the general
+// form's: equal bits.
+template <typename T, int D>
+__device__ __forceinline__ void unit_grad(const T (*W)[2], const T* ph, T* grad)
+{
+#pragma unroll
+    for (int dd = 0; dd < D; ++dd) {
+        T r = (T)0;
+#pragma unroll
+        for (int i = 0; i < (1 << D); ++i) {
+            T q = (T)1;
+            bool first = true;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const int dim = D - 1 - j;
+                if (dim == dd) continue;
+                const T f = W[dim][(i >> j) & 1];
+                q = first ? f : q * f;
+                first = false;
+            }
+            if (!coef_kept<T>(q)) continue;
+            const T t = ph[i] * q;
+            r = ((i >> (D - 1 - dd)) & 1) ? r + t : r - t;
+        }
+        grad[dd] = r;
+    }
+}
+
+template <typename T, int D, bool PUSH, bool UNIT>
 __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__ phi, T qm, T dt,
                                          double mass, T* x, T* v, double& ke, double* pm)
 {
@@ -378,18 +437,24 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
             G[d][0] = (T)-1 * dr;
             G[d][1] = (T)1 * dr;
         }
+        uint32_t sidx[1 << D];
+        support_indices<D>(g, c, sidx);
         T ph[1 << D];
 #pragma unroll
-        for (int i = 0; i < (1 << D); ++i) ph[i] = phi[support_index<D>(g, c, i)];
+        for (int i = 0; i < (1 << D); ++i) ph[i] = phi[sidx[i]];
+        if constexpr (UNIT) {
+            unit_grad<T, D>(W, ph, grad);
+        } else {
 #pragma unroll
-        for (int dd = 0; dd < D; ++dd) {
-            T r = (T)0;
+            for (int dd = 0; dd < D; ++dd) {
+                T r = (T)0;
 #pragma unroll
-            for (int i = 0; i < (1 << D); ++i) {
-                const T gc = cic_grad_coef<T, D>(W, G, i, dd);
-                if (coef_kept<T>(gc)) r = r + ph[i] * gc;
+                for (int i = 0; i < (1 << D); ++i) {
+                    const T gc = cic_grad_coef<T, D>(W, G, i, dd);
+                    if (coef_kept<T>(gc)) r = r + ph[i] * gc;
+                }
+                grad[dd] = r;
             }
-            grad[dd] = r;
         }
     }
     // kick then drift (TSI/main.cpp:65-68)
@@ -422,7 +487,7 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
     return alive;
 }
 
-template <typename T, int D, bool PUSH, bool DEPOSIT>
+template <typename T, int D, bool PUSH, bool DEPOSIT, bool UNIT = false>
 __global__ __launch_bounds__(kThreads) void k_push_deposit(
     DevGrid g, T* __restrict__ x0, T* __restrict__ x1, T* __restrict__ x2, T* __restrict__ v0,
     T* __restrict__ v1, T* __restrict__ v2, int64_t n, const T* __restrict__ phi, T qm, T dt,
@@ -487,7 +552,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         if (!isfinite((double)x[0])) continue;       // invalid (or beyond n)
         bool alive = true;
         if (PUSH) {
-            alive = push_one<T, D, PUSH>(g, phi, qm, dt, mass, x, v, ke, pm);
+            alive = push_one<T, D, PUSH, UNIT>(g, phi, qm, dt, mass, x, v, ke, pm);
             if (!alive) ++absorbed;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -939,14 +1004,21 @@ psum_status launch_push(const psum_grid* g, void* const* pos, void* const* vel, 
     auto* a = reinterpret_cast<unsigned long long*>(acc);
     auto* c = reinterpret_cast<unsigned long long*>(counters);
     const T* ph = static_cast<const T*>(phi);
-#define PSUM_LAUNCH(DIM, PU, DE)                                                              \
-    hipLaunchKernelGGL((k_push_deposit<T, DIM, PU, DE>), dim3(nb), dim3(kThreads), 0, st, dg,  \
-                       p[0], p[1], p[2], v[0], v[1], v[2], n, ph, (T)qm, (T)dt, mass, w_dep,   \
-                       scale, a, diag, c)
+    // unit node spacing: the gather's gradient weights are +-products of W
+    bool unit = true;
+    for (int d = 0; d < g->dim; ++d) unit = unit && g->delta_r[d] == 1.0;
+#define PSUM_LAUNCH(DIM, PU, DE, UN)                                                          \
+    hipLaunchKernelGGL((k_push_deposit<T, DIM, PU, DE, UN>), dim3(nb), dim3(kThreads), 0, st, \
+                       dg, p[0], p[1], p[2], v[0], v[1], v[2], n, ph, (T)qm, (T)dt, mass,      \
+                       w_dep, scale, a, diag, c)
 #define PSUM_DISPATCH(DIM)                                                                    \
-    if (push && acc) PSUM_LAUNCH(DIM, true, true);                                            \
-    else if (push) PSUM_LAUNCH(DIM, true, false);                                             \
-    else PSUM_LAUNCH(DIM, false, true);
+    if (push && acc) {                                                                        \
+        if (unit) PSUM_LAUNCH(DIM, true, true, true);                                         \
+        else PSUM_LAUNCH(DIM, true, true, false);                                             \
+    } else if (push) {                                                                        \
+        if (unit) PSUM_LAUNCH(DIM, true, false, true);                                        \
+        else PSUM_LAUNCH(DIM, true, false, false);                                            \
+    } else PSUM_LAUNCH(DIM, false, true, false);
     switch (g->dim) {
     case 1: PSUM_DISPATCH(1); break;
     case 2: PSUM_DISPATCH(2); break;
