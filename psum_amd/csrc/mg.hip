@@ -334,14 +334,29 @@ __device__ __forceinline__ void line_apply(const LineGeom& G, const T* __restric
     }
 }
 
+// Blocks [0, nbc) of the launch take chunks; the blocks after them take the
+// listed nodes, one per thread, with the per-node kernels' code.
+struct LineList {
+    const uint32_t* list;
+    uint32_t n;
+    uint32_t nbc;              // blocks of chunks
+};
+
 template <typename T, bool ACC>
-__global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, const T* __restrict__ row,
+__global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, LineList LL, DevLevel L, DevTable<T> relax,
+                                                            const T* __restrict__ row,
                                                             const T* __restrict__ xin,
                                                             const T* __restrict__ b,
                                                             T* __restrict__ xout, T two3,
                                                             double* __restrict__ xacc)
 {
-    const int64_t t = level_block<3>() * kMgThreads + threadIdx.x;
+    const int64_t blk = level_block<3>();
+    if (blk >= (int64_t)LL.nbc) {
+        const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
+        if (t < (int64_t)LL.n) smooth_node<T, 3, true, ACC>(L, relax, xin, b, xout, two3, xacc, (int64_t)LL.list[t]);
+        return;
+    }
+    const int64_t t = blk * kMgThreads + threadIdx.x;
     if (t >= (int64_t)G.nchunks) return;
     const uint32_t i = (uint32_t)t * kLine;
     if (!line_chunk_fast(G, i)) return;
@@ -361,11 +376,18 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, const T*
 }
 
 template <typename T>
-__global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, const T* __restrict__ row,
+__global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, LineList LL, DevLevel L, DevTable<T> A,
+                                                              const T* __restrict__ row,
                                                               const T* __restrict__ x,
                                                               const T* __restrict__ b, T* __restrict__ r)
 {
-    const int64_t t = level_block<3>() * kMgThreads + threadIdx.x;
+    const int64_t blk = level_block<3>();
+    if (blk >= (int64_t)LL.nbc) {
+        const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
+        if (t < (int64_t)LL.n) residual_node<T, 3, true>(L, A, x, b, r, (int64_t)LL.list[t]);
+        return;
+    }
+    const int64_t t = blk * kMgThreads + threadIdx.x;
     if (t >= (int64_t)G.nchunks) return;
     const uint32_t i = (uint32_t)t * kLine;
     if (!line_chunk_fast(G, i)) return;
@@ -375,31 +397,6 @@ __global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, const 
 #pragma unroll
     for (int k = 0; k < kLine; ++k) v[k] = s[k] * (T)-1 + bb[k] * (T)1;
     st_line(r + i, v);
-}
-
-// the listed nodes of a chunked level, one per thread
-template <typename T, bool ACC>
-__global__ __launch_bounds__(kMgThreads) void k_smooth_list(DevLevel L, DevTable<T> relax,
-                                                            const uint32_t* __restrict__ list, uint32_t n,
-                                                            const T* __restrict__ xin,
-                                                            const T* __restrict__ b,
-                                                            T* __restrict__ xout, T two3,
-                                                            double* __restrict__ xacc)
-{
-    const uint32_t t = blockIdx.x * kMgThreads + threadIdx.x;
-    if (t >= n) return;
-    smooth_node<T, 3, true, ACC>(L, relax, xin, b, xout, two3, xacc, (int64_t)list[t]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kMgThreads) void k_residual_list(DevLevel L, DevTable<T> A,
-                                                              const uint32_t* __restrict__ list, uint32_t n,
-                                                              const T* __restrict__ x,
-                                                              const T* __restrict__ b, T* __restrict__ r)
-{
-    const uint32_t t = blockIdx.x * kMgThreads + threadIdx.x;
-    if (t >= n) return;
-    residual_node<T, 3, true>(L, A, x, b, r, (int64_t)list[t]);
 }
 
 // b_c = (R r) o D_c^-1 ; x_c = 0, or, when the coarse level is pre-smoothed,
@@ -490,6 +487,45 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
                     const int i2 = wrap_index(p2 + e, Cl.n[2], tr.wrap[2]);
                     s = s + (w01 * w2[e]) * xc[((int64_t)i0 * Cl.n[1] + i1) * Cl.n[2] + i2];
                 }
+            }
+        }
+    }
+    xf[i] = s * (T)1 + xf[i] * (T)1;
+}
+
+// ---- 3D prolongation over planes ------------------------------------------
+// k_prolong_add on 3D levels with a two-dimensional launch: blockIdx.y is
+// the fine node's slowest coordinate, so its transfer table entries are
+// uniform across the block, and one division splits the rest of the index.
+// Each node's terms and their order are k_prolong_add's, so the bits are the
+// same.
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_prolong_add_plane(DevLevel F, DevLevel Cl, DevTransfer<T> tr,
+                                                                  const T* __restrict__ xc, T* __restrict__ xf)
+{
+    const uint32_t j = blockIdx.x * kMgThreads + threadIdx.x;
+    const uint32_t plane = (uint32_t)F.n[1] * (uint32_t)F.n[2];
+    if (j >= plane) return;
+    const int c0 = blockIdx.y;
+    const int c1 = (int)(j / (uint32_t)F.n[2]);
+    const int c2 = (int)(j - (uint32_t)c1 * (uint32_t)F.n[2]);
+    const int64_t i = (int64_t)c0 * plane + j;
+    const int p0 = tr.pc[0][c0], p1 = tr.pc[1][c1], p2 = tr.pc[2][c2];
+    const T* w0 = tr.pw[0] + 2 * c0;
+    const T* w1 = tr.pw[1] + 2 * c1;
+    const T* w2 = tr.pw[2] + 2 * c2;
+    T s = (T)0;
+    for (int a = 0; a < 2; ++a) {
+        if (w0[a] == (T)0) continue;
+        const int i0 = wrap_index(p0 + a, Cl.n[0], tr.wrap[0]);
+        for (int b = 0; b < 2; ++b) {
+            if (w1[b] == (T)0) continue;
+            const T w01 = w0[a] * w1[b];
+            const int i1 = wrap_index(p1 + b, Cl.n[1], tr.wrap[1]);
+            for (int e = 0; e < 2; ++e) {
+                if (w2[e] == (T)0) continue;
+                const int i2 = wrap_index(p2 + e, Cl.n[2], tr.wrap[2]);
+                s = s + (w01 * w2[e]) * xc[((int64_t)i0 * Cl.n[1] + i1) * Cl.n[2] + i2];
             }
         }
     }
@@ -1273,6 +1309,50 @@ __global__ __launch_bounds__(kMgThreads) void k_outer_residual_many(const OuterA
 {
     const int i = batch_instance(first, n);
     outer_residual_body<D, COMPACT, OUTF32, NORMS>(a[i], (int64_t)(blockIdx.x - first[i]));
+}
+
+// The outer residual without norms on a chunked 7-point 3D level: the
+// FP32 right-hand side (and first sweep) of outer_residual_body at the
+// chunk kernels' nodes (line_apply in FP64 over the unscaled class row),
+// the listed nodes one per thread; same terms and order, same bits.
+__device__ __forceinline__ void outer_node_f32(const OuterArgs& A, int64_t i)
+{
+    const DevLevel& L = A.L;
+    int c[3];
+    node_coords<3>(L, i, c);
+    const double* coef = A.K.coef + (int64_t)L.cls[i] * 27;
+    const double ax = stencil_apply<double, 3, true>(L, coef, A.x, c);
+    const double r = A.b[i] - ax;
+    const float bv = (float)r * A.dinv32[L.cls[i]];
+    A.b0[i] = bv;
+    if (A.e0) A.e0[i] = bv * (2.0f / 3.0f * 1.0f);
+}
+
+__global__ __launch_bounds__(kMgThreads) void k_outer_line(LineGeom G, LineList LL, OuterArgs A,
+                                                           const double* __restrict__ row, float dinv_in)
+{
+    const int64_t blk = level_block<3>();
+    if (blk >= (int64_t)LL.nbc) {
+        const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
+        if (t < (int64_t)LL.n) outer_node_f32(A, (int64_t)LL.list[t]);
+        return;
+    }
+    const int64_t t = blk * kMgThreads + threadIdx.x;
+    if (t >= (int64_t)G.nchunks) return;
+    const uint32_t i = (uint32_t)t * kLine;
+    if (!line_chunk_fast(G, i)) return;
+    double ax[kLine], bb[kLine];
+    line_apply(G, row, A.x, i, ax);
+    ld_line(A.b + i, bb);
+    float bv[kLine], e[kLine];
+#pragma unroll
+    for (int k = 0; k < kLine; ++k) {
+        const double r = bb[k] - ax[k];
+        bv[k] = (float)r * dinv_in;
+        e[k] = bv[k] * (2.0f / 3.0f * 1.0f);
+    }
+    st_line(A.b0 + i, bv);
+    if (A.e0) st_line(A.e0 + i, e);
 }
 
 __global__ __launch_bounds__(kMgThreads) void k_reduce2(const double* __restrict__ part, int64_t nb,
@@ -2497,6 +2577,21 @@ private:
         d.line_n = (uint32_t)list.size();
         if (!list.empty()) d.line_list = dev_copy(list);
     }
+    // dinv of class k of level l as the FP32 table holds it
+    float host_dinv32(int l, int k) const { return (float)host_[l].dinv[k]; }
+
+    // 3D prolongation to level l over planes (k_prolong_add_plane): levels
+    // of 2^15 nodes or more
+    bool planar(int l) const { return line_ && host_[l].D == 3 && host_[l].N >= (1 << 15); }
+
+    LineList line_list(int l) const
+    {
+        LineList ll;
+        ll.list = dev_[l].line_list;
+        ll.n = dev_[l].line_n;
+        ll.nbc = nblk(host_[l].N / kLine);
+        return ll;
+    }
     LineGeom line_geom(int l) const
     {
         LineGeom g;
@@ -2566,17 +2661,12 @@ private:
         else hipLaunchKernelGGL((k_smooth<T, DIM, CMP, false>), dim3(nb), dim3(kMgThreads), 0, stream_, L, table<T>(l, true), x, b, buf, two3, nullptr)
         if (line_ && dev_[l].line_cls >= 0) {
             const LineGeom G = line_geom(l);
+            const LineList LL = line_list(l);
             const DevTable<T> tb = table<T>(l, true);
             const T* row = tb.coef + (int64_t)G.cls_in * 27;
-            const unsigned nc = nblk(G.nchunks);
-            const uint32_t nl = dev_[l].line_n;
-            if (acc) {
-                hipLaunchKernelGGL((k_smooth_line<T, true>), dim3(nc), dim3(kMgThreads), 0, stream_, G, row, x, b, buf, two3, acc);
-                if (nl) hipLaunchKernelGGL((k_smooth_list<T, true>), dim3(nblk(nl)), dim3(kMgThreads), 0, stream_, L, tb, dev_[l].line_list, nl, x, b, buf, two3, acc);
-            } else {
-                hipLaunchKernelGGL((k_smooth_line<T, false>), dim3(nc), dim3(kMgThreads), 0, stream_, G, row, x, b, buf, two3, nullptr);
-                if (nl) hipLaunchKernelGGL((k_smooth_list<T, false>), dim3(nblk(nl)), dim3(kMgThreads), 0, stream_, L, tb, dev_[l].line_list, nl, x, b, buf, two3, nullptr);
-            }
+            const unsigned nb = LL.nbc + nblk(LL.n);
+            if (acc) hipLaunchKernelGGL((k_smooth_line<T, true>), dim3(nb), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, buf, two3, acc);
+            else hipLaunchKernelGGL((k_smooth_line<T, false>), dim3(nb), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, buf, two3, nullptr);
         } else if (host_[l].D == 2) {
             if (host_[l].compact) { SMOOTH(2, true); } else { SMOOTH(2, false); }
         } else {
@@ -2593,11 +2683,10 @@ private:
         const unsigned nb = nblk(L.N);
         if (line_ && dev_[l].line_cls >= 0) {
             const LineGeom G = line_geom(l);
+            const LineList LL = line_list(l);
             const DevTable<T> tb = table<T>(l, false);
             const T* row = tb.coef + (int64_t)G.cls_in * 27;
-            hipLaunchKernelGGL((k_residual_line<T>), dim3(nblk(G.nchunks)), dim3(kMgThreads), 0, stream_, G, row, x, b, r);
-            const uint32_t nl = dev_[l].line_n;
-            if (nl) hipLaunchKernelGGL((k_residual_list<T>), dim3(nblk(nl)), dim3(kMgThreads), 0, stream_, L, tb, dev_[l].line_list, nl, x, b, r);
+            hipLaunchKernelGGL((k_residual_line<T>), dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
             return;
         }
         if (host_[l].D == 2) {
@@ -2680,15 +2769,12 @@ private:
         const DevLevel F = dl(l), C = dl(l + 1);
         if (host_[l].D == 2)
             hipLaunchKernelGGL((k_prolong_add<T, 2>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xc, x);
+        else if (planar(l))
+            hipLaunchKernelGGL((k_prolong_add_plane<T>), dim3(nblk((int64_t)F.n[1] * F.n[2]), F.n[0]), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xc, x);
         else
             hipLaunchKernelGGL((k_prolong_add<T, 3>), dim3(nblk(F.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), xc, x);
     }
 
-    // First level of the one-workgroup tail for precision T, or -1: a 2D
-    // fused hierarchy whose levels lt..Lc (two or more, at most kMaxTail,
-    // lt >= 1, at most kTailLevelNodes nodes each) keep their iterates and
-    // right-hand sides in LDS and whose coarsest level has at most kTailCoarse
-    // nodes (its dense solve reads the inverse from one compute unit).
     // First level of the one-workgroup tail for precision T, or -1: a 2D
     // fused hierarchy whose levels lt..Lc (two or more, at most kMaxTail,
     // lt >= 1, at most kTailLevelNodes nodes each) keep their iterates and
@@ -3035,6 +3121,14 @@ private:
         const float* d32 = dev_[0].dinv32;
         float* e0 = (emit_f32 && first_sweep) ? dev_[0].xf : nullptr;
         const OuterArgs oa{L, K, x, b, bn, pin_, d32, b0, part_, e0};
+        if (!norms && emit_f32 && !rec_ && line_ && dev_[0].line_cls >= 0) {
+            const LineGeom G = line_geom(0);
+            const LineList LL = line_list(0);
+            hipLaunchKernelGGL(k_outer_line, dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, oa,
+                               K.coef + (int64_t)G.cls_in * 27, host_dinv32(0, G.cls_in));
+            MG_HIP(hipGetLastError());
+            return;
+        }
         // the squared norms only when they are read (norms); otherwise the
         // kernel skips bn and the reductions
 #define OUTER(DIM, CMP, F32) \

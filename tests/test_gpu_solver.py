@@ -303,10 +303,12 @@ def test_vcycle_tail_is_bit_identical(backend, case, prepost):
 @pytest.mark.parametrize("n", [49, 68])
 @pytest.mark.parametrize("prepost", [(3, 3), (2, 1)])
 def test_chunked_3d_kernels_are_bit_identical(backend, n, prepost):
-    """On 7-point 3D levels of 2^16 nodes or more the sweeps and residuals run
-    four nodes per thread (interior nodes of the dominant class) plus a
-    per-node pass over the listed rest; the terms and their order are the
-    per-node kernels', so the bits equal those with the chunks switched off.
+    """On 7-point 3D levels of 2^16 nodes or more the sweeps, residuals and
+    the mixed solver's outer residual run four nodes per thread (interior
+    nodes of the dominant class) plus a per-node pass over the listed rest,
+    and prolongation to 3D levels of 2^15 nodes or more runs over planes; the
+    terms and their order are the per-node kernels', so the bits equal those
+    with "line 0".
     n = 49 gives lines whose length is not a multiple of four, 68 one that is
     (chunks then never straddle a line)."""
     E = _E()
