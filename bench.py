@@ -73,8 +73,9 @@ def timed_run(st, n, steps, sort_every):
                 push_ms_avg=push_ms / max(1, launches), push_launches=launches,
                 particles_per_launch=pushed / max(1, launches),
                 cycles_per_step=(s1["cycles"] - s0["cycles"]) / steps,
-                last_rel_res=s1["last_rel_res"], tile_fallback_blocks=s1["tile_fallback_blocks"],
-                absorbed=s1["absorbed"], sort_every=sort_every, sorts_in_window=sorts,
+                last_rel_res=s1["last_rel_res"],
+                tile_fallback_blocks_in_window=s1["tile_fallback_blocks"] - s0["tile_fallback_blocks"],
+                absorbed_in_window=s1["absorbed"] - s0["absorbed"], sort_every=sort_every, sorts_in_window=sorts,
                 align_steps=extra, hist=hist)
 
 
@@ -95,7 +96,7 @@ def run_engine(E, wl, dtype, mixed, steps, warmup, tol, depth, sort_every, pre=2
     log(f"[bench] {label} {dtype}: {res['ms_per_step']:.3f} ms/step, "
         f"{res['particle_steps_per_s']:.3e} particle-steps/s, push {res['push_ms_avg']:.3f} ms, "
         f"{res['cycles_per_step']:.2f} V-cycles/step, {res['sorts_in_window']} sorts in window, "
-        f"tile fallbacks {res['tile_fallback_blocks']}")
+        f"tile fallbacks {res['tile_fallback_blocks_in_window']}")
     return res
 
 
@@ -385,13 +386,19 @@ def c3_workload(cells, ppc):
     return bounded_3d(cells=cells, ppc=ppc, device="cuda")
 
 
+# (pre, post) smoothing sweeps the 3D box's autotuner chooses from
+SMOOTHING_CANDIDATES = ((3, 3), (4, 4), (5, 5), (6, 6))
+
+
 def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_probe=20,
-           extrapolate=False):
+           extrapolate=False, tune_smoothing=True):
     """3D bounded box (phi = 0 on the faces, absorbing walls, compaction by
     the sort), electrons + ions (mass ratio 1836, Te/Ti = 10), 7-point MG.
     dtype "f32": FP32 particles with FP32 V-cycles in FP64 refinement to
     1e-8; "f64": the plain FP64 path on the same inputs.  schedule = (sort
-    every, sort shift) or None to run the autotuner first."""
+    every, sort shift) or None to run the autotuner first; tune_smoothing:
+    the smoothing autotuner picks (pre, post) from SMOOTHING_CANDIDATES,
+    else (pre, post) as given."""
     import torch
     t0 = time.time()
     tdt = torch.float32 if dtype == "f32" else torch.float64
@@ -411,6 +418,14 @@ def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_pr
         schedule = (tun["sort_every"], tun["sort_shift"])
         log(f"[bench] c3: tuned in {time.time() - t0:.1f}s: sort every {schedule[0]} steps on "
             f"{2 ** schedule[1]}^3-cell bins")
+    smoothing = (pre, post)
+    smoothing_ms = None
+    if tune_smoothing:
+        t0 = time.time()
+        smoothing, smoothing_ms = st.tune_smoothing(SMOOTHING_CANDIDATES, steps=8)
+        log(f"[bench] c3 {dtype}: smoothing tuned in {time.time() - t0:.1f}s: pre/post {smoothing} ("
+            + ", ".join(f"{c}: {v:.2f} ms" for c, v in smoothing_ms.items()) + ")")
+    if tun is not None or tune_smoothing:
         # tuning advanced the state: start again from the initial particles,
         # so that the mixed and FP64 runs time the same physical steps
         for i, s in enumerate(wl["species"]):
@@ -421,13 +436,16 @@ def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_pr
     res = timed_run(st, n, steps, schedule[0])
     res["particles"] = n
     res["schedule"] = {"sort_every": schedule[0], "sort_shift": schedule[1]}
+    res["smoothing"] = list(smoothing)
+    if smoothing_ms is not None:
+        res["smoothing_probe_ms_per_step"] = {f"{c[0]},{c[1]}": v for c, v in smoothing_ms.items()}
     res["field_energy_last"] = float(res["hist"][-1, 0])
     st.close()
     torch.cuda.empty_cache()
     log(f"[bench] c3 {dtype}: {res['ms_per_step']:.2f} ms/step, {res['particle_steps_per_s']:.3e} "
         f"particle-steps/s, push {res['push_ms_avg']:.3f} ms, {res['cycles_per_step']:.2f} "
-        f"V-cycles/step, {res['sorts_in_window']} sorts in window, absorbed {res['absorbed']}, "
-        f"tile fallbacks {res['tile_fallback_blocks']}")
+        f"V-cycles/step, {res['sorts_in_window']} sorts in window, absorbed {res['absorbed_in_window']}, "
+        f"tile fallbacks {res['tile_fallback_blocks_in_window']} (in the window)")
     return res, schedule
 
 
@@ -468,6 +486,8 @@ def main():
     ap.add_argument("--c3-ppc", type=int, default=32)
     ap.add_argument("--c3-depth", type=int, default=6)
     ap.add_argument("--c3-no-fp64", action="store_true")
+    ap.add_argument("--c3-fixed-smoothing", action="store_true",
+                    help="3D box: --pre/--post sweeps instead of the smoothing autotuner")
     ap.add_argument("--c3-extrapolate", action="store_true",
                     help="warm-start each 3D solve from 2 phi_n - phi_(n-1)")
     ap.add_argument("--c3-schedule", default=None,
@@ -521,7 +541,8 @@ def main():
     log(f"[bench] c3 workload {wl3['name']}: generated in {time.time() - t0:.1f}s")
     fixed_sched = None if args.c3_schedule is None else tuple(int(v) for v in args.c3_schedule.split(","))
     c3, sched = run_c3(E, wl3, "f32", args.steps, args.warmup, args.c3_depth, args.pre, args.post,
-                       schedule=fixed_sched, extrapolate=args.c3_extrapolate)
+                       schedule=fixed_sched, extrapolate=args.c3_extrapolate,
+                       tune_smoothing=not args.c3_fixed_smoothing)
     if args.calib_copy:
         nb = int(c3["particles_per_launch"]) * 2 * 3
         src = torch.ones(nb, dtype=torch.float32, device="cuda")
@@ -533,7 +554,8 @@ def main():
     c3_64 = None
     if not args.c3_no_fp64:
         c3_64, _ = run_c3(E, wl3, "f64", args.steps, args.warmup, args.c3_depth, args.pre,
-                          args.post, schedule=sched, extrapolate=args.c3_extrapolate)
+                          args.post, schedule=sched, extrapolate=args.c3_extrapolate,
+                          tune_smoothing=not args.c3_fixed_smoothing)
     del wl3
     torch.cuda.empty_cache()
 
@@ -623,7 +645,7 @@ def main():
                                f"compaction, 7-point MG, FP32 state, FP32 V-cycles in FP64 "
                                f"refinement to 1e-8, dt 0.1",
                    "cells": [args.c3_cells] * 3, "particles": c3["particles"],
-                   "mg_depth": args.c3_depth, "mg_pre_post": [args.pre, args.post],
+                   "mg_depth": args.c3_depth, "mg_pre_post": c3["smoothing"],
                    "sort_every": sched[0], "sort_bin_cells": 2 ** sched[1],
                    "sorts_in_window": c3["sorts_in_window"], "parallelism": "single GPU"},
         "fp64": None if c3_64 is None else _leg(c3_64),
@@ -647,8 +669,8 @@ def main():
 def _leg(r):
     """The JSON fields of one timed run."""
     keys = ("ms_per_step", "particle_steps_per_s", "push_ms_avg", "push_launches",
-            "particles_per_launch", "cycles_per_step", "last_rel_res", "tile_fallback_blocks",
-            "absorbed", "sort_every", "sorts_in_window", "align_steps")
+            "particles_per_launch", "cycles_per_step", "last_rel_res", "tile_fallback_blocks_in_window",
+            "absorbed_in_window", "sort_every", "sorts_in_window", "align_steps")
     return {k: r[k] for k in keys if k in r}
 
 

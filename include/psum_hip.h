@@ -438,11 +438,25 @@ psum_status psum_stepper_set_interp_order(void* h, int order);
 
 /* Multigrid kernel-shape options that may change after creation, as
  * whitespace "key value" pairs: "fuse 0|1" (2D fused tile smoothing and
- * residual + restriction, default 1) and "tail 0|1" (the coarse end of each
- * 2D V-cycle in one workgroup, default 0).  Results are bit-identical
- * either way.  A batch instance's change makes the batch capture its step
- * graph again at its next run.  No reference counterpart (tuning knobs). */
+ * residual + restriction, default 1), "tail 0|1" (the coarse end of each
+ * 2D V-cycle in one workgroup, default 0) and "line 0|1" (3D 7-point
+ * levels in chunks of four nodes per thread, 3D prolongation over planes,
+ * default 1).  Results are bit-identical either way.  A batch instance's
+ * change makes the batch capture its step graph again at its next run.  No
+ * reference counterpart (tuning knobs). */
 psum_status psum_stepper_mg_tuning(void* h, const char* options);
+/* Smoothing sweeps per level before and after the coarse-grid correction
+ * (the backend's pre_relax / post_relax options, backend.hpp:84-92),
+ * changed after creation: 0 <= pre, post <= 64, pre + post >= 1.  Changes
+ * the iterates (and so the cycles a tolerance needs), not the fixed point. */
+psum_status psum_stepper_set_smoothing(void* h, int pre, int post);
+/* Autotuner for the smoothing sweeps: for each candidate (pre[c], post[c])
+ * a sort, one untimed step (residual checked after every cycle) and `steps`
+ * steps timed with HIP events (no sorts, tolerance-controlled solves); the
+ * fastest is kept and returned in *best_pre / *best_post, ms (may be NULL)
+ * gets n per-step times.  The state advances.  No reference counterpart. */
+psum_status psum_stepper_tune_smoothing(void* h, const int* pre, const int* post, int n, int steps,
+                                        int* best_pre, int* best_post, double* ms);
 
 /* Checkpoint files in the reference's MAS format (src/serialization/mas.hpp:
  * 49-453; particle_group / device_field save(), particle_group.hpp:272-275,

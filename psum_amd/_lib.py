@@ -219,6 +219,8 @@ SIGNATURES = [
     ("psum_stepper_restore", ST, [P, C.c_char_p]),
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
     ("psum_stepper_mg_tuning", ST, [P, C.c_char_p]),
+    ("psum_stepper_set_smoothing", ST, [P, C.c_int, C.c_int]),
+    ("psum_stepper_tune_smoothing", ST, [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), P]),
     ("psum_stepper_tune", ST, [P, C.c_int, C.c_int, C.POINTER(PsumBatchTuning)]),
     ("psum_poisson_assemble", ST, [_GP, C.c_double, P, I64, P, P, P, C.POINTER(I64)]),
     ("psum_poisson_assemble_ex", ST, [_GP, C.POINTER(PsumPoissonSpec), I64, P, P, P, C.POINTER(I64),

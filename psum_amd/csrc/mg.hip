@@ -2083,6 +2083,21 @@ public:
         }
         rr_fit_.clear();
         resolve_tail();
+        drop_graphs();   // captured cycles hold the previous kernel shapes
+    }
+
+    // Smoothing sweeps per level before and after the coarse correction
+    // (pre_relax / post_relax), changed after set-up; captured cycles are
+    // dropped.
+    void set_smoothing(int pre, int post)
+    {
+        if (pre < 0 || post < 0 || pre > 64 || post > 64 || pre + post < 1)
+            throw std::invalid_argument("hip_multigrid: smoothing sweeps need 0 <= pre, post <= 64, pre + post >= 1");
+        pre_ = pre;
+        post_ = post;
+        rr_fit_.clear();
+        resolve_tail();
+        drop_graphs();
     }
 
     void set_stream(hipStream_t s) { stream_ = s; }
@@ -3056,6 +3071,11 @@ private:
         if (c.exec) (void)hipGraphExecDestroy(c.exec);
         c = Captured{};
     }
+    void drop_graphs()
+    {
+        for (auto* gc : {&cap64_, &cap32_})
+            for (auto& e : gc->e) drop_graph(e);
+    }
 
     template <typename F>
     void replay(GraphCache& gc, const void* b, const void* x, int flag, F&& body)
@@ -3399,6 +3419,7 @@ HipMultigrid* mg_create(bool mixed) { return new HipMultigrid(mixed); }
 void mg_destroy(HipMultigrid* h) { delete h; }
 void mg_set_options(HipMultigrid* h, const char* o) { h->set_options(o); }
 void mg_set_tuning(HipMultigrid* h, const char* o) { h->set_tuning(o); }
+void mg_set_smoothing(HipMultigrid* h, int pre, int post) { h->set_smoothing(pre, post); }
 void mg_set_matrix(HipMultigrid* h, unsigned long long n, unsigned long long nnz,
                    const unsigned long long* r, const unsigned long long* c, const double* v)
 {

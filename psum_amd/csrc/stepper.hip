@@ -31,6 +31,7 @@ HipMultigrid* mg_create(bool mixed);
 void mg_destroy(HipMultigrid* h);
 void mg_set_options(HipMultigrid* h, const char* o);
 void mg_set_tuning(HipMultigrid* h, const char* o);
+void mg_set_smoothing(HipMultigrid* h, int pre, int post);
 void mg_set_matrix(HipMultigrid* h, unsigned long long n, unsigned long long nnz,
                    const unsigned long long* r, const unsigned long long* c, const double* v);
 void mg_set_replace(HipMultigrid* h, unsigned long long n, const unsigned long long* i, const double* v);
@@ -952,7 +953,51 @@ public:
         mg_set_tuning(mg_, o);
         ++tuning_epoch_;
     }
-    // changes with every mg_tuning call (a batch re-captures its graphs)
+    void set_smoothing(int pre, int post)
+    {
+        mg_set_smoothing(mg_, pre, post);
+        ++tuning_epoch_;
+    }
+    // For each candidate (pre[c], post[c]): a sort (so that every candidate
+    // sees the same particle order), one untimed step that checks the
+    // residual after every cycle (the timed solves then start from the cycle
+    // count this candidate needs), and `steps` steps timed with HIP events
+    // around the whole step (no sorts, tolerance-controlled solves).  Keeps
+    // the fastest.  The state advances.
+    void tune_smoothing(const int* pre, const int* post, int n, int steps, int* best_pre, int* best_post,
+                        double* ms_out)
+    {
+        if (n < 1 || steps < 1 || !pre || !post) throw std::invalid_argument("tune_smoothing: n >= 1, steps >= 1");
+        if (!aev_[0]) {
+            S_TRY(hipEventCreate(&aev_[0]));
+            S_TRY(hipEventCreate(&aev_[1]));
+        }
+        double best = 1e300;
+        int bc = 0;
+        for (int c = 0; c < n; ++c) {
+            if (cfg_.sort_every > 0 && !no_sort_) sort_all();
+            set_smoothing(pre[c], post[c]);
+            mg_set_predicted(mg_, 1);
+            step_body(false);
+            S_TRY(hipEventRecord(aev_[0], st_));
+            for (int s = 0; s < steps; ++s) step_body(false);
+            S_TRY(hipEventRecord(aev_[1], st_));
+            S_TRY(hipEventSynchronize(aev_[1]));
+            float ms = 0;
+            S_TRY(hipEventElapsedTime(&ms, aev_[0], aev_[1]));
+            const double per = (double)ms / steps;
+            if (ms_out) ms_out[c] = per;
+            if (per < best) {
+                best = per;
+                bc = c;
+            }
+        }
+        set_smoothing(pre[bc], post[bc]);
+        if (best_pre) *best_pre = pre[bc];
+        if (best_post) *best_post = post[bc];
+    }
+    // changes with every mg_tuning or set_smoothing call (a batch re-captures
+    // its graphs)
     int64_t tuning_epoch() const { return tuning_epoch_; }
     void require_cic(const char* what) const
     {
@@ -1740,6 +1785,19 @@ psum_status psum_stepper_set_interp_order(void* h, int order)
 {
     if (!h) { set_error("psum_stepper_set_interp_order: null stepper"); return PSUM_ERR_ARG; }
     return guarded([&]() { static_cast<Stepper*>(h)->set_interp_order(order); });
+}
+
+psum_status psum_stepper_set_smoothing(void* h, int pre, int post)
+{
+    if (!h) { set_error("psum_stepper_set_smoothing: null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->set_smoothing(pre, post); });
+}
+
+psum_status psum_stepper_tune_smoothing(void* h, const int* pre, const int* post, int n, int steps, int* best_pre,
+                                        int* best_post, double* ms)
+{
+    if (!h) { set_error("psum_stepper_tune_smoothing: null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->tune_smoothing(pre, post, n, steps, best_pre, best_post, ms); });
 }
 
 psum_status psum_stepper_mg_tuning(void* h, const char* options)

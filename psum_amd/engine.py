@@ -558,9 +558,26 @@ class Stepper:
         return {f: getattr(t, f) for f, _ in PsumBatchTuning._fields_ if f != "pad_"}
 
     def mg_tuning(self, options):
-        """Multigrid kernel-shape options ("fuse 0|1", "tail 0|1"); results
-        are bit-identical either way (psum_stepper_mg_tuning)."""
+        """Multigrid kernel-shape options ("fuse 0|1", "tail 0|1", "line 0|1");
+        results are bit-identical either way (psum_stepper_mg_tuning)."""
         check(lib().psum_stepper_mg_tuning(self.h, options.encode()), "mg_tuning")
+
+    def set_smoothing(self, pre, post):
+        """Smoothing sweeps per level before / after the coarse correction."""
+        check(lib().psum_stepper_set_smoothing(self.h, int(pre), int(post)), "set_smoothing")
+
+    def tune_smoothing(self, candidates=((3, 3), (4, 4), (5, 5), (6, 6)), steps=4):
+        """Times `steps` steps with each (pre, post) candidate and keeps the
+        fastest (psum_stepper_tune_smoothing); returns ((pre, post), {cand:
+        ms per step}).  The state advances."""
+        n = len(candidates)
+        pre = (C.c_int * n)(*[int(c[0]) for c in candidates])
+        post = (C.c_int * n)(*[int(c[1]) for c in candidates])
+        ms = (C.c_double * n)()
+        bp, bq = C.c_int(0), C.c_int(0)
+        check(lib().psum_stepper_tune_smoothing(self.h, pre, post, n, int(steps), C.byref(bp), C.byref(bq), ms),
+              "tune_smoothing")
+        return (bp.value, bq.value), {tuple(c): ms[i] for i, c in enumerate(candidates)}
 
     def set_schedule(self, fixed_cycles=0, sort_every=0, sort_shift=0):
         """fixed_cycles > 0: that many V-cycles per solve, no tolerance check;

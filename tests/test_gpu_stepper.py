@@ -164,3 +164,39 @@ def test_charge_accumulator_overflow_fails_the_run():
     st.load(0, [np.full(small, 2.5), np.full(small, 3.5)], [np.zeros(small), np.zeros(small)])
     st.run(1)
     assert st.stats()["acc_wraps"] == 0
+
+
+def test_set_smoothing_equals_creation_options_and_tuner_keeps_a_candidate():
+    """set_smoothing(pre, post) after creation steps exactly as a stepper
+    created with those sweeps (captured cycles are dropped); the smoothing
+    autotuner returns one of its candidates with a time for each, and the
+    stepper keeps stepping with it; bad counts are refused."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import bounded_3d
+    wl = bounded_3d(cells=16, ppc=4, seed=8, dtype=np.float32)
+    kw = dict(dtype="f32", mixed_mg=True, depth=3, tol=1e-8, max_cycles=40, dt=wl["dt"], rho_bg=0.0,
+              fixed=wl["fixed"])
+    a = E.Stepper(wl["grid"], wl["species"], pre=5, post=4, **kw)
+    ha = a.run(6)
+    b = E.Stepper(wl["grid"], wl["species"], pre=2, post=1, **kw)
+    b.set_smoothing(5, 4)
+    hb = b.run(6)
+    assert np.array_equal(ha, hb)
+    # mid-run: the solves replayed from captured cycles take the new sweeps
+    c = E.Stepper(wl["grid"], wl["species"], pre=1, post=1, **kw)
+    c.run(3)
+    c0 = c.stats()["cycles"]
+    c.run(3)
+    per_11 = c.stats()["cycles"] - c0
+    c.set_smoothing(6, 6)
+    c1 = c.stats()["cycles"]
+    c.run(3)
+    per_66 = c.stats()["cycles"] - c1
+    assert per_66 < per_11, (per_11, per_66)
+    best, ms = b.tune_smoothing(((2, 2), (3, 3), (4, 4)), steps=2)
+    assert best in ((2, 2), (3, 3), (4, 4)) and len(ms) == 3 and all(v > 0 for v in ms.values())
+    assert np.all(np.isfinite(b.run(3)))
+    with pytest.raises(Exception, match="smoothing"):
+        b.set_smoothing(0, 0)
+    with pytest.raises(Exception, match="smoothing"):
+        b.set_smoothing(-1, 2)
