@@ -248,17 +248,18 @@ __global__ __launch_bounds__(kMgThreads) void k_residual(DevLevel L, DevTable<T>
     residual_node<T, D, COMPACT>(L, A, x, b, r, i);
 }
 
-// ---- 3D 7-point levels in chunks of kLine consecutive nodes ---------------
-// On a compact (7-point) 3D level nearly every node lies inside the box and
-// carries the same stencil class.  A chunk of kLine nodes (linear indices
-// kLine*t .. kLine*t + kLine-1) all on one grid line, strictly inside the
-// box, all of that class, is updated by one thread with the class row in
-// registers and vector loads of the line and its four neighbour lines (no
-// coordinate division per node, no clamping, no per-node table reads).  The
-// per-node terms and their order are stencil_apply's, so the result is
-// bit-identical to k_smooth / k_residual.  The other nodes (box faces, line
-// ends, other classes, the tail) are listed when the level is built and
-// updated by the per-node kernels over that list.
+// ---- 3D levels in chunks of kLine consecutive nodes ------------------------
+// On a 3D level nearly every node lies inside the box and carries the same
+// stencil class.  A chunk of kLine nodes (linear indices kLine*t ..
+// kLine*t + kLine-1) all on one grid line, strictly inside the box, all of
+// that class, is updated by one thread with the class row in registers and
+// vector loads of the line and its neighbour lines (four for a 7-point
+// level, eight for a 27-point one): no coordinate division per node, no
+// clamping, no per-node table reads.  The per-node terms and their order
+// are stencil_apply's, so the result is bit-identical to k_smooth /
+// k_residual.  The other nodes (box faces, line ends, other classes, the
+// tail) are listed when the level is built and updated by the per-node
+// kernels over that list.
 constexpr int kLine = 4;
 
 struct LineGeom {
@@ -303,34 +304,56 @@ __device__ __forceinline__ void st_line(T* __restrict__ p, const T* v)
     __builtin_memcpy(p, &q, sizeof q);
 }
 
-// the 7-point sum of stencil_apply<T, 3, true> at the chunk's nodes: terms
-// in offset order x-1, y-1, z-1, centre, z+1, y+1, x+1
-template <typename T>
+// stencil_apply<T, 3, COMPACT> at the chunk's nodes: the 7-point sum in
+// offset order x-1, y-1, z-1, centre, z+1, y+1, x+1, or the 27-point sum
+// with offsets (a, b, e) in x, y, z ascending, z fastest
+template <typename T, bool COMPACT>
 __device__ __forceinline__ void line_apply(const LineGeom& G, const T* __restrict__ row,
                                            const T* __restrict__ x, uint32_t i, T* s)
 {
-    const T cxm = row[4], cym = row[10], czm = row[12], cc = row[13], czp = row[14], cyp = row[16],
-            cxp = row[22];
     const uint32_t sy = (uint32_t)G.n2, sx = (uint32_t)G.n1 * (uint32_t)G.n2;
-    T xm[kLine], ym[kLine], zz[kLine + 2], yp[kLine], xp[kLine];
-    ld_line(x + (i - sx), xm);
-    ld_line(x + (i - sy), ym);
-    zz[0] = x[i - 1];
-    ld_line(x + i, zz + 1);
-    zz[kLine + 1] = x[i + kLine];
-    ld_line(x + (i + sy), yp);
-    ld_line(x + (i + sx), xp);
+    if constexpr (COMPACT) {
+        const T cxm = row[4], cym = row[10], czm = row[12], cc = row[13], czp = row[14], cyp = row[16],
+                cxp = row[22];
+        T xm[kLine], ym[kLine], zz[kLine + 2], yp[kLine], xp[kLine];
+        ld_line(x + (i - sx), xm);
+        ld_line(x + (i - sy), ym);
+        zz[0] = x[i - 1];
+        ld_line(x + i, zz + 1);
+        zz[kLine + 1] = x[i + kLine];
+        ld_line(x + (i + sy), yp);
+        ld_line(x + (i + sx), xp);
 #pragma unroll
-    for (int k = 0; k < kLine; ++k) {
-        T t = (T)0;
-        t = t + cxm * xm[k];
-        t = t + cym * ym[k];
-        t = t + czm * zz[k];
-        t = t + cc * zz[k + 1];
-        t = t + czp * zz[k + 2];
-        t = t + cyp * yp[k];
-        t = t + cxp * xp[k];
-        s[k] = t;
+        for (int k = 0; k < kLine; ++k) {
+            T t = (T)0;
+            t = t + cxm * xm[k];
+            t = t + cym * ym[k];
+            t = t + czm * zz[k];
+            t = t + cc * zz[k + 1];
+            t = t + czp * zz[k + 2];
+            t = t + cyp * yp[k];
+            t = t + cxp * xp[k];
+            s[k] = t;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kLine; ++k) s[k] = (T)0;
+#pragma unroll
+        for (int a = -1; a <= 1; ++a)
+#pragma unroll
+            for (int b = -1; b <= 1; ++b) {
+                const T* ln = x + (i + (uint32_t)((int)sx * a + (int)sy * b));
+                T zz[kLine + 2];
+                zz[0] = ln[-1];
+                ld_line(ln, zz + 1);
+                zz[kLine + 1] = ln[kLine];
+#pragma unroll
+                for (int e = -1; e <= 1; ++e) {
+                    const T c = row[(a + 1) * 9 + (b + 1) * 3 + (e + 1)];
+#pragma unroll
+                    for (int k = 0; k < kLine; ++k) s[k] = s[k] + c * zz[k + 1 + e];
+                }
+            }
     }
 }
 
@@ -342,7 +365,7 @@ struct LineList {
     uint32_t nbc;              // blocks of chunks
 };
 
-template <typename T, bool ACC>
+template <typename T, bool COMPACT, bool ACC>
 __global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, LineList LL, DevLevel L, DevTable<T> relax,
                                                             const T* __restrict__ row,
                                                             const T* __restrict__ xin,
@@ -353,7 +376,7 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, LineList
     const int64_t blk = level_block<3>();
     if (blk >= (int64_t)LL.nbc) {
         const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
-        if (t < (int64_t)LL.n) smooth_node<T, 3, true, ACC>(L, relax, xin, b, xout, two3, xacc, (int64_t)LL.list[t]);
+        if (t < (int64_t)LL.n) smooth_node<T, 3, COMPACT, ACC>(L, relax, xin, b, xout, two3, xacc, (int64_t)LL.list[t]);
         return;
     }
     const int64_t t = blk * kMgThreads + threadIdx.x;
@@ -361,7 +384,7 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, LineList
     const uint32_t i = (uint32_t)t * kLine;
     if (!line_chunk_fast(G, i)) return;
     T s[kLine], bb[kLine], v[kLine];
-    line_apply(G, row, xin, i, s);
+    line_apply<T, COMPACT>(G, row, xin, i, s);
     ld_line(b + i, bb);
 #pragma unroll
     for (int k = 0; k < kLine; ++k) v[k] = s[k] * (T)1 + bb[k] * (two3 * (T)1);
@@ -375,7 +398,7 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, LineList
     }
 }
 
-template <typename T>
+template <typename T, bool COMPACT>
 __global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, LineList LL, DevLevel L, DevTable<T> A,
                                                               const T* __restrict__ row,
                                                               const T* __restrict__ x,
@@ -384,7 +407,7 @@ __global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, LineLi
     const int64_t blk = level_block<3>();
     if (blk >= (int64_t)LL.nbc) {
         const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
-        if (t < (int64_t)LL.n) residual_node<T, 3, true>(L, A, x, b, r, (int64_t)LL.list[t]);
+        if (t < (int64_t)LL.n) residual_node<T, 3, COMPACT>(L, A, x, b, r, (int64_t)LL.list[t]);
         return;
     }
     const int64_t t = blk * kMgThreads + threadIdx.x;
@@ -392,7 +415,7 @@ __global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, LineLi
     const uint32_t i = (uint32_t)t * kLine;
     if (!line_chunk_fast(G, i)) return;
     T s[kLine], bb[kLine], v[kLine];
-    line_apply(G, row, x, i, s);
+    line_apply<T, COMPACT>(G, row, x, i, s);
     ld_line(b + i, bb);
 #pragma unroll
     for (int k = 0; k < kLine; ++k) v[k] = s[k] * (T)-1 + bb[k] * (T)1;
@@ -1342,7 +1365,7 @@ __global__ __launch_bounds__(kMgThreads) void k_outer_line(LineGeom G, LineList 
     const uint32_t i = (uint32_t)t * kLine;
     if (!line_chunk_fast(G, i)) return;
     double ax[kLine], bb[kLine];
-    line_apply(G, row, A.x, i, ax);
+    line_apply<double, true>(G, row, A.x, i, ax);
     ld_line(A.b + i, bb);
     float bv[kLine], e[kLine];
 #pragma unroll
@@ -2561,10 +2584,10 @@ private:
         MG_HIP(hipHostMalloc(&red_host_, 2 * sizeof(double), hipHostMallocDefault));
     }
 
-    // A 7-point 3D level of at least 2^16 nodes (and fewer than 2^32) runs
-    // its smoothing sweeps and residuals in chunks of kLine nodes: the class
-    // that covers most nodes goes to the chunk kernels, and every node of a
-    // chunk they skip (by the same test as line_chunk_fast) is listed.
+    // A 3D level of at least 2^16 nodes (and fewer than 2^32) runs its
+    // smoothing sweeps and residuals in chunks of kLine nodes: the class that
+    // covers most nodes goes to the chunk kernels, and every node of a chunk
+    // they skip (by the same test as line_chunk_fast) is listed.
     static bool line_chunk_fast_host(const HostLevel& h, uint32_t i, int cls_in)
     {
         const uint32_t n0 = (uint32_t)h.n[0], n1 = (uint32_t)h.n[1], n2 = (uint32_t)h.n[2];
@@ -2578,7 +2601,7 @@ private:
     {
         d.line_cls = -1;
         d.line_n = 0;
-        if (h.D != 3 || !h.compact || h.N < (1 << 16) || h.N >= (int64_t)0xffffffffll) return;
+        if (h.D != 3 || h.N < (1 << 16) || h.N >= (int64_t)0xffffffffll) return;
         std::vector<int64_t> count(h.dinv.size(), 0);
         for (int64_t i = 0; i < h.N; ++i) ++count[h.cls[i]];
         const int cls_in = (int)(std::max_element(count.begin(), count.end()) - count.begin());
@@ -2680,8 +2703,10 @@ private:
             const DevTable<T> tb = table<T>(l, true);
             const T* row = tb.coef + (int64_t)G.cls_in * 27;
             const unsigned nb = LL.nbc + nblk(LL.n);
-            if (acc) hipLaunchKernelGGL((k_smooth_line<T, true>), dim3(nb), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, buf, two3, acc);
-            else hipLaunchKernelGGL((k_smooth_line<T, false>), dim3(nb), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, buf, two3, nullptr);
+#define SLINE(CMP, ACC, A) hipLaunchKernelGGL((k_smooth_line<T, CMP, ACC>), dim3(nb), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, buf, two3, A)
+            if (host_[l].compact) { if (acc) SLINE(true, true, acc); else SLINE(true, false, nullptr); }
+            else { if (acc) SLINE(false, true, acc); else SLINE(false, false, nullptr); }
+#undef SLINE
         } else if (host_[l].D == 2) {
             if (host_[l].compact) { SMOOTH(2, true); } else { SMOOTH(2, false); }
         } else {
@@ -2701,7 +2726,8 @@ private:
             const LineList LL = line_list(l);
             const DevTable<T> tb = table<T>(l, false);
             const T* row = tb.coef + (int64_t)G.cls_in * 27;
-            hipLaunchKernelGGL((k_residual_line<T>), dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
+            if (host_[l].compact) hipLaunchKernelGGL((k_residual_line<T, true>), dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
+            else hipLaunchKernelGGL((k_residual_line<T, false>), dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
             return;
         }
         if (host_[l].D == 2) {
@@ -3141,7 +3167,7 @@ private:
         const float* d32 = dev_[0].dinv32;
         float* e0 = (emit_f32 && first_sweep) ? dev_[0].xf : nullptr;
         const OuterArgs oa{L, K, x, b, bn, pin_, d32, b0, part_, e0};
-        if (!norms && emit_f32 && !rec_ && line_ && dev_[0].line_cls >= 0) {
+        if (!norms && emit_f32 && !rec_ && line_ && dev_[0].line_cls >= 0 && host_[0].compact) {
             const LineGeom G = line_geom(0);
             const LineList LL = line_list(0);
             hipLaunchKernelGGL(k_outer_line, dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, oa,

@@ -300,7 +300,7 @@ def test_vcycle_tail_is_bit_identical(backend, case, prepost):
 
 
 @pytest.mark.parametrize("backend", ["hip_multigrid_gpu", "hip_multigrid32f_gpu"])
-@pytest.mark.parametrize("n", [49, 68])
+@pytest.mark.parametrize("n", [49, 68, 97])
 @pytest.mark.parametrize("prepost", [(3, 3), (2, 1)])
 def test_chunked_3d_kernels_are_bit_identical(backend, n, prepost):
     """On 7-point 3D levels of 2^16 nodes or more the sweeps, residuals and
@@ -310,7 +310,8 @@ def test_chunked_3d_kernels_are_bit_identical(backend, n, prepost):
     terms and their order are the per-node kernels', so the bits equal those
     with "line 0".
     n = 49 gives lines whose length is not a multiple of four, 68 one that is
-    (chunks then never straddle a line)."""
+    (chunks then never straddle a line); at n = 97 the second level (49^3,
+    27-point Galerkin rows) is chunked too."""
     E = _E()
     og, nn, r, c, v, b, fixed = _wall_problem_3d(n, seed=7)
     pre, post = prepost
