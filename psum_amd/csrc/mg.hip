@@ -3047,7 +3047,10 @@ private:
     // smoothing runs through the 2D tile kernels: the outer residual also
     // writes the first sweep of e from zero, and the last post-sweep adds e
     // into x.
-    bool fused_mixed() const { return fuse_ && host_[0].D == 2 && pre_ > 0 && post_ > 0; }
+    // the mixed solve's first level-0 sweep from e = 0 taken from the FP64
+    // outer residual (2D with the fused tile kernels, 3D with the chunk
+    // kernels; "fuse 0" / "line 0" run it as a sweep of its own)
+    bool fused_mixed() const { return (host_[0].D == 2 ? fuse_ : line_) && pre_ > 0 && post_ > 0; }
 
     // FP32 V-cycles on dev_[0].bf (already scaled) for the correction e from
     // e = 0.  first_swept: dev_[0].xf already holds the first level-0 sweep.

@@ -307,7 +307,8 @@ def test_chunked_3d_kernels_are_bit_identical(backend, n, prepost):
     the mixed solver's outer residual run four nodes per thread (interior
     nodes of the dominant class) plus a per-node pass over the listed rest,
     and prolongation to 3D levels of 2^15 nodes or more runs over planes; the
-    terms and their order are the per-node kernels', so the bits equal those
+    mixed solver takes its first level-0 sweep from the outer residual.  The
+    terms and their order are the separate kernels', so the bits equal those
     with "line 0".
     n = 49 gives lines whose length is not a multiple of four, 68 one that is
     (chunks then never straddle a line); at n = 97 the second level (49^3,
