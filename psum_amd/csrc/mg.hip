@@ -516,12 +516,21 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add(DevLevel F, DevLevel
     xf[i] = s * (T)1 + xf[i] * (T)1;
 }
 
-// ---- 3D prolongation over planes ------------------------------------------
-// k_prolong_add on 3D levels with a two-dimensional launch: blockIdx.y is
-// the fine node's slowest coordinate, so its transfer table entries are
-// uniform across the block, and one division splits the rest of the index.
-// Each node's terms and their order are k_prolong_add's, so the bits are the
-// same.
+// ---- 3D transfers over planes ---------------------------------------------
+// k_prolong_add / k_restrict on 3D levels with a two-dimensional launch:
+// blockIdx.y is the output node's slowest coordinate, so its transfer table
+// entries are uniform across the block, and one division splits the rest of
+// the index.  Each node's terms and their order are the per-node kernels',
+// so the bits are the same.
+// index p + o on an axis of n nodes, wrapped on a periodic axis, else kept
+// in range (the caller selects the term away when p + o is outside)
+__device__ __forceinline__ uint32_t axis_index(int p, int n, int wrap)
+{
+    int q = wrap_index(p, n, wrap);
+    q = q < 0 ? 0 : (q >= n ? n - 1 : q);
+    return (uint32_t)q;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kMgThreads) void k_prolong_add_plane(DevLevel F, DevLevel Cl, DevTransfer<T> tr,
                                                                   const T* __restrict__ xc, T* __restrict__ xf)
@@ -534,25 +543,96 @@ __global__ __launch_bounds__(kMgThreads) void k_prolong_add_plane(DevLevel F, De
     const int c2 = (int)(j - (uint32_t)c1 * (uint32_t)F.n[2]);
     const int64_t i = (int64_t)c0 * plane + j;
     const int p0 = tr.pc[0][c0], p1 = tr.pc[1][c1], p2 = tr.pc[2][c2];
-    const T* w0 = tr.pw[0] + 2 * c0;
-    const T* w1 = tr.pw[1] + 2 * c1;
-    const T* w2 = tr.pw[2] + 2 * c2;
-    T s = (T)0;
+    T w0[2], w1[2], w2[2];
+#pragma unroll
     for (int a = 0; a < 2; ++a) {
-        if (w0[a] == (T)0) continue;
-        const int i0 = wrap_index(p0 + a, Cl.n[0], tr.wrap[0]);
+        w0[a] = tr.pw[0][2 * c0 + a];
+        w1[a] = tr.pw[1][2 * c1 + a];
+        w2[a] = tr.pw[2][2 * c2 + a];
+    }
+    // all eight coarse values are loaded before any is used (addresses kept
+    // in range); a term whose weight is zero is then skipped by a select, so
+    // the sum is k_prolong_add's, term for term
+    const uint32_t cs1 = (uint32_t)Cl.n[2], cs0 = (uint32_t)Cl.n[1] * (uint32_t)Cl.n[2];
+    uint32_t q0[2], q1[2], q2[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        q0[a] = axis_index(p0 + a, Cl.n[0], tr.wrap[0]) * cs0;
+        q1[a] = axis_index(p1 + a, Cl.n[1], tr.wrap[1]) * cs1;
+        q2[a] = axis_index(p2 + a, Cl.n[2], tr.wrap[2]);
+    }
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = xc[q0[k >> 2] + q1[(k >> 1) & 1] + q2[k & 1]];
+    const T xo = xf[i];
+    T s = (T)0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b) {
-            if (w1[b] == (T)0) continue;
             const T w01 = w0[a] * w1[b];
-            const int i1 = wrap_index(p1 + b, Cl.n[1], tr.wrap[1]);
+#pragma unroll
             for (int e = 0; e < 2; ++e) {
-                if (w2[e] == (T)0) continue;
-                const int i2 = wrap_index(p2 + e, Cl.n[2], tr.wrap[2]);
-                s = s + (w01 * w2[e]) * xc[((int64_t)i0 * Cl.n[1] + i1) * Cl.n[2] + i2];
+                const bool use = w0[a] != (T)0 && w1[b] != (T)0 && w2[e] != (T)0;
+                const T t = s + (w01 * w2[e]) * v[a * 4 + b * 2 + e];
+                s = use ? t : s;
             }
         }
+    xf[i] = s * (T)1 + xo * (T)1;
+}
+
+// k_restrict on 3D levels: the slowest coarse coordinate in blockIdx.y and
+// all (up to 27) fine residuals loaded before any is used; a term outside
+// the node's support (a >= m0, ...) is skipped by a select, so the sum is
+// k_restrict's, term for term.
+template <typename T>
+__global__ __launch_bounds__(kMgThreads) void k_restrict_plane(DevLevel F, DevLevel Cl, DevTransfer<T> tr,
+                                                               DevTable<T> Ac, const T* __restrict__ r,
+                                                               T* __restrict__ bc, T* __restrict__ xc,
+                                                               T first_sweep)
+{
+    const uint32_t j = blockIdx.x * kMgThreads + threadIdx.x;
+    const uint32_t plane = (uint32_t)Cl.n[1] * (uint32_t)Cl.n[2];
+    if (j >= plane) return;
+    const int c0 = blockIdx.y;
+    const int c1 = (int)(j / (uint32_t)Cl.n[2]);
+    const int c2 = (int)(j - (uint32_t)c1 * (uint32_t)Cl.n[2]);
+    const int64_t i = (int64_t)c0 * plane + j;
+    const int f0 = tr.rf[0][c0], m0 = tr.rn[0][c0];
+    const int f1 = tr.rf[1][c1], m1 = tr.rn[1][c1];
+    const int f2 = tr.rf[2][c2], m2 = tr.rn[2][c2];
+    T w0[3], w1[3], w2[3];
+    uint32_t g0[3], g1[3], g2[3];
+    const uint32_t fs1 = (uint32_t)F.n[2], fs0 = (uint32_t)F.n[1] * (uint32_t)F.n[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        w0[a] = tr.rw[0][3 * c0 + a];
+        w1[a] = tr.rw[1][3 * c1 + a];
+        w2[a] = tr.rw[2][3 * c2 + a];
+        g0[a] = axis_index(f0 + a, F.n[0], tr.wrap[0]) * fs0;
+        g1[a] = axis_index(f1 + a, F.n[1], tr.wrap[1]) * fs1;
+        g2[a] = axis_index(f2 + a, F.n[2], tr.wrap[2]);
     }
-    xf[i] = s * (T)1 + xf[i] * (T)1;
+    T v[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) v[k] = r[g0[k / 9] + g1[(k / 3) % 3] + g2[k % 3]];
+    const uint16_t ci = Cl.cls[i];
+    T s = (T)0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const T w01 = w0[a] * w1[b];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const bool use = a < m0 && b < m1 && e < m2;
+                const T t = s + (w01 * w2[e]) * v[a * 9 + b * 3 + e];
+                s = use ? t : s;
+            }
+        }
+    const T bv = s * Ac.dinv[ci];
+    bc[i] = bv;
+    xc[i] = first_sweep != (T)0 ? bv * (first_sweep * (T)1) : (T)0;
 }
 
 // ---- 2D V-cycle tail in one workgroup ------------------------------------
@@ -2618,9 +2698,13 @@ private:
     // dinv of class k of level l as the FP32 table holds it
     float host_dinv32(int l, int k) const { return (float)host_[l].dinv[k]; }
 
-    // 3D prolongation to level l over planes (k_prolong_add_plane): levels
-    // of 2^15 nodes or more
-    bool planar(int l) const { return line_ && host_[l].D == 3 && host_[l].N >= (1 << 15); }
+    // 3D transfers writing level l over planes (k_prolong_add_plane,
+    // k_restrict_plane): levels of 2^15 nodes or more
+    bool planar(int l) const
+    {
+        return line_ && host_[l].D == 3 && host_[l].N >= (1 << 15) && host_[l].N < (int64_t)0xffffffffll &&
+               (l == 0 || host_[l - 1].N < (int64_t)0xffffffffll);
+    }
 
     LineList line_list(int l) const
     {
@@ -2996,6 +3080,8 @@ private:
                 residual<T>(l, xs[l], bs[l], ts[l]);
                 if (host_[l].D == 2)
                     hipLaunchKernelGGL((k_restrict<T, 2>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
+                else if (planar(l + 1))
+                    hipLaunchKernelGGL((k_restrict_plane<T>), dim3(nblk((int64_t)C.n[1] * C.n[2]), C.n[0]), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
                 else
                     hipLaunchKernelGGL((k_restrict<T, 3>), dim3(nblk(C.N)), dim3(kMgThreads), 0, stream_, F, C, transfer<T>(l), table<T>(l + 1, false), ts[l], bs[l + 1], xs[l + 1], fs);
             }
