@@ -1350,6 +1350,10 @@ struct OuterArgs {
     float* b0;
     double* part;
     float* e0;
+    // 3D 7-point level 0: the dominant class and its row (K.coef +
+    // cls_in * 27), read as uniform values; -1: every node reads its row
+    int cls_in;
+    const double* row_in;
 };
 
 template <int D, bool COMPACT, bool OUTF32, bool NORMS>
@@ -1371,8 +1375,14 @@ __device__ __forceinline__ void outer_residual_body(const OuterArgs& A, int64_t 
     if (i < L.N) {
         int c[3];
         node_coords<D>(L, i, c);
-        const double* coef = K.coef + (int64_t)L.cls[i] * (D == 2 ? 9 : 27);
-        const double ax = stencil_apply<double, D, COMPACT>(L, coef, x, c);
+        const uint16_t ci = L.cls[i];
+        double ax;
+        if (D == 3 && COMPACT && (int)ci == A.cls_in) {
+            ax = stencil_apply<double, D, COMPACT>(L, A.row_in, x, c);
+        } else {
+            const double* coef = K.coef + (int64_t)ci * (D == 2 ? 9 : 27);
+            ax = stencil_apply<double, D, COMPACT>(L, coef, x, c);
+        }
         const double r = b[i] - ax;
         if (NORMS) {
             rr = i == pin ? 0.0 : r * r;
@@ -1463,9 +1473,22 @@ __global__ __launch_bounds__(kMgThreads) void k_reduce2(const double* __restrict
 {
     __shared__ double sh[2][kMgThreads];
     double s0 = 0.0, s1 = 0.0;
-    for (int64_t b = threadIdx.x; b < nb; b += kMgThreads) {
-        s0 += part[2 * b];
-        s1 += part[2 * b + 1];
+    // thread t sums pairs 8t .. 8t+7, then 8(t + 256) .., in order; the eight
+    // pairs of a round are loaded before they are added
+    constexpr int kU = 8;
+    for (int64_t b0 = (int64_t)threadIdx.x * kU; b0 < nb; b0 += (int64_t)kMgThreads * kU) {
+        double v[2 * kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const bool in = b0 + u < nb;
+            v[2 * u] = in ? part[2 * (b0 + u)] : 0.0;
+            v[2 * u + 1] = in ? part[2 * (b0 + u) + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            s0 += v[2 * u];
+            s1 += v[2 * u + 1];
+        }
     }
     sh[0][threadIdx.x] = s0;
     sh[1][threadIdx.x] = s1;
@@ -3255,7 +3278,9 @@ private:
         float* b0 = emit_f32 ? dev_[0].bf : nullptr;
         const float* d32 = dev_[0].dinv32;
         float* e0 = (emit_f32 && first_sweep) ? dev_[0].xf : nullptr;
-        const OuterArgs oa{L, K, x, b, bn, pin_, d32, b0, part_, e0};
+        const bool dom = line_ && host_[0].D == 3 && c && dev_[0].line_cls >= 0;
+        const OuterArgs oa{L, K, x, b, bn, pin_, d32, b0, part_, e0, dom ? dev_[0].line_cls : -1,
+                           dom ? K.coef + (int64_t)dev_[0].line_cls * 27 : nullptr};
         if (!norms && emit_f32 && !rec_ && line_ && dev_[0].line_cls >= 0 && host_[0].compact) {
             const LineGeom G = line_geom(0);
             const LineList LL = line_list(0);
