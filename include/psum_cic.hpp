@@ -62,18 +62,20 @@ inline bool grid_ok(const psum_grid* g)
 // 201-206, 327-328).  cw is the base node after one periodic wrap over the
 // w nodes of each dimension (simple_grid.hpp:338-343).  The reference result
 // is defined only while that single wrap stays in range: cN in [-w, 2w-2].
-template <int D>
+// UNIT: every delta and delta_r is exactly 1, so the products with them are
+// exact and are left out (same results).
+template <int D, bool UNIT = false>
 __host__ __device__ __forceinline__ bool base_cell(const DevGrid& g, const double* x, int* cw, double* s)
 {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const double t = (x[d] - g.lb[d]) * g.delta_r[d];
+        const double t = UNIT ? x[d] - g.lb[d] : (x[d] - g.lb[d]) * g.delta_r[d];
         const double f = floor(t);
         const int w = g.w[d];
         if (!(f >= (double)(-w) && f <= (double)(2 * w - 2))) return false;
         const int ci = (int)f;
-        const double base = g.lb[d] + (double)ci * g.delta[d];
-        s[d] = (x[d] - base) * g.delta_r[d];
+        const double base = UNIT ? g.lb[d] + (double)ci : g.lb[d] + (double)ci * g.delta[d];
+        s[d] = UNIT ? x[d] - base : (x[d] - base) * g.delta_r[d];
         cw[d] = ci < 0 ? ci + w : (ci >= w ? ci - w : ci);
     }
     return true;

@@ -426,7 +426,7 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
     int c[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
-    if (base_cell<D>(g, xd, c, s)) {
+    if (base_cell<D, UNIT>(g, xd, c, s)) {
         T W[D][2], G[D][2];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
             double xd[D], s[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
-            if (base_cell<D>(g, xd, cw[j], s)) {
+            if (base_cell<D, UNIT>(g, xd, cw[j], s)) {
                 ok[j] = true;
 #pragma unroll
                 for (int d = 0; d < D; ++d) sw[j][d] = (T)s[d];
@@ -735,9 +735,11 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 if (inside) {
                     const int li0 = ((uw[j][0] - lo[0]) * e[1] + (uw[j][1] - lo[1])) * e[2] + (uw[j][2] - lo[2]);
 #pragma unroll
+                    // a zero term (a coefficient below the cut) adds nothing:
+                    // no branch around it
                     for (int i = 0; i < NS; ++i)
-                        if (t8[i]) atomicAdd(&tile[li0 + ((i >> 2) & 1) * s0 + ((i >> 1) & 1) * s1 + (i & 1)],
-                                             (unsigned long long)t8[i]);
+                        atomicAdd(&tile[li0 + ((i >> 2) & 1) * s0 + ((i >> 1) & 1) * s1 + (i & 1)],
+                                  (unsigned long long)t8[i]);
                 } else if (ovf) {
                     // all eight adds are issued before their returned values
                     // are checked (a zero term leaves old unchanged: no wrap)
@@ -828,14 +830,42 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         }
         if (use_lds) {
             __syncthreads();
-            // tile entry i -> its global node
+            // tile entry i -> its global node.  Entry i holds box offsets k
+            // with i = (k0 * e1 + k1) * e2 + k2; the divisions by e[d] >= 2
+            // are high products with ceil(2^32 / e[d]), exact for
+            // i < 2^32 / e[d] (entries < 2^13).  A box that crosses no
+            // periodic seam maps to base + sum k[d] * stride[d].
+            uint32_t mdiv[3], gst[3];
+            gst[D - 1] = 1u;
+#pragma unroll
+            for (int d = D - 2; d >= 0; --d) gst[d] = gst[d + 1] * (uint32_t)g.w[d + 1];
+            bool seam = false;
+            uint32_t gbase = 0u;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                mdiv[d] = 0xFFFFFFFFu / (uint32_t)e[d] + 1u;
+                seam = seam || lo[d] < 0 || lo[d] + e[d] > g.w[d];
+                gbase += (uint32_t)lo[d] * gst[d];
+            }
             auto node_of = [&](int i) {
-                int rem = i;
+                uint32_t rem = (uint32_t)i, k[3];
+#pragma unroll
+                for (int d = D - 1; d >= 1; --d) {
+                    const uint32_t q = __umulhi(rem, mdiv[d]);
+                    k[d] = rem - q * (uint32_t)e[d];
+                    rem = q;
+                }
+                k[0] = rem;
+                if (!seam) {
+                    uint32_t lin = gbase;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) lin += k[d] * gst[d];
+                    return lin;
+                }
                 int nd[3];
 #pragma unroll
-                for (int d = D - 1; d >= 0; --d) {
-                    nd[d] = lo[d] + rem % e[d];
-                    rem /= e[d];
+                for (int d = 0; d < D; ++d) {
+                    nd[d] = lo[d] + (int)k[d];
                     if (nd[d] < 0) nd[d] += g.w[d];
                     else if (nd[d] >= g.w[d]) nd[d] -= g.w[d];
                 }
@@ -1005,8 +1035,9 @@ psum_status launch_push(const psum_grid* g, void* const* pos, void* const* vel, 
     auto* c = reinterpret_cast<unsigned long long*>(counters);
     const T* ph = static_cast<const T*>(phi);
     // unit node spacing: the gather's gradient weights are +-products of W
+    // and the cell index needs no products with the spacing
     bool unit = true;
-    for (int d = 0; d < g->dim; ++d) unit = unit && g->delta_r[d] == 1.0;
+    for (int d = 0; d < g->dim; ++d) unit = unit && g->delta_r[d] == 1.0 && g->delta[d] == 1.0;
 #define PSUM_LAUNCH(DIM, PU, DE, UN)                                                          \
     hipLaunchKernelGGL((k_push_deposit<T, DIM, PU, DE, UN>), dim3(nb), dim3(kThreads), 0, st, \
                        dg, p[0], p[1], p[2], v[0], v[1], v[2], n, ph, (T)qm, (T)dt, mass,      \
