@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBsThreads) void k_gather_push_bs(
         const double vals[4] = {ke, pm[0], pm[1], pm[2]};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const double s = wave_sum(vals[q]);
+            const double s = wave_sum_full(vals[q]);
             if (lane == 0) wd[wave][q] = s;
         }
         __syncthreads();
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBsThreads) void k_gather_push_bs(
         }
     }
     if (counters) {
-        const int ab = wave_isum((int)absorbed);
+        const int ab = wave_isum_full((int)absorbed);
         if (lane == 0 && ab) atomicAdd(&counters[2], (unsigned long long)ab);
     }
 }
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
         ok[j] = true;
     }
     if (counters) {
-        const int oo = wave_isum((int)oor);
+        const int oo = wave_isum_full((int)oor);
         if (lane == 0 && oo) atomicAdd(&counters[0], (unsigned long long)oo);
     }
     // box over base nodes unwrapped around thread 0's first particle on
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kBsThreads) void k_deposit_bs(DevGrid g, const T* _
             }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const int mn = wave_min(lmin[d]), mx = wave_max(lmax[d]);
+        const int mn = wave_min_full(lmin[d]), mx = wave_max_full(lmax[d]);
         if (lane == 0 && mn != INT_MAX) {
             atomicMin(&bmin[d], mn);
             atomicMax(&bmax[d], mx);

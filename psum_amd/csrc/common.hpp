@@ -138,6 +138,86 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
+// Reductions over a full wave (all 64 lanes active) without the LDS
+// permute: a butterfly inside each row of 16 lanes through DPP moves
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror;
+// every source lane is in range), after which each lane of a row holds the
+// row's total, then the four row totals read from lanes 0, 16, 32 and 48.
+// The result is the same in every lane.  Sums of doubles are taken in this
+// fixed order (deterministic, not the order of wave_sum).
+namespace dpp {
+constexpr int kXor1 = 0xB1;         // quad_perm [1,0,3,2]
+constexpr int kXor2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int kHalfMirror = 0x141;  // lane i of 8 reads lane 7 - i
+constexpr int kMirror = 0x140;      // lane i of 16 reads lane 15 - i
+
+template <int CTRL>
+__device__ __forceinline__ int mov(int v, int identity)
+{
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ long long mov64(long long v)
+{
+    const int lo = mov<CTRL>((int)(unsigned long long)v, 0);
+    const int hi = mov<CTRL>((int)((unsigned long long)v >> 32), 0);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ long long readlane64(long long v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)v, lane);
+    const int hi = __builtin_amdgcn_readlane((int)((unsigned long long)v >> 32), lane);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+}  // namespace dpp
+
+__device__ __forceinline__ double wave_sum_full(double v)
+{
+    v += __longlong_as_double(dpp::mov64<dpp::kXor1>(__double_as_longlong(v)));
+    v += __longlong_as_double(dpp::mov64<dpp::kXor2>(__double_as_longlong(v)));
+    v += __longlong_as_double(dpp::mov64<dpp::kHalfMirror>(__double_as_longlong(v)));
+    v += __longlong_as_double(dpp::mov64<dpp::kMirror>(__double_as_longlong(v)));
+    const long long b = __double_as_longlong(v);
+    const double r0 = __longlong_as_double(dpp::readlane64(b, 0));
+    const double r1 = __longlong_as_double(dpp::readlane64(b, 16));
+    const double r2 = __longlong_as_double(dpp::readlane64(b, 32));
+    const double r3 = __longlong_as_double(dpp::readlane64(b, 48));
+    return (r0 + r1) + (r2 + r3);
+}
+
+__device__ __forceinline__ long long wave_lsum_full(long long v)
+{
+    v += dpp::mov64<dpp::kXor1>(v);
+    v += dpp::mov64<dpp::kXor2>(v);
+    v += dpp::mov64<dpp::kHalfMirror>(v);
+    v += dpp::mov64<dpp::kMirror>(v);
+    return dpp::readlane64(v, 0) + dpp::readlane64(v, 16) + dpp::readlane64(v, 32) + dpp::readlane64(v, 48);
+}
+
+template <typename Op>
+__device__ __forceinline__ int wave_reduce_full(int v, int identity, Op op)
+{
+    v = op(v, dpp::mov<dpp::kXor1>(v, identity));
+    v = op(v, dpp::mov<dpp::kXor2>(v, identity));
+    v = op(v, dpp::mov<dpp::kHalfMirror>(v, identity));
+    v = op(v, dpp::mov<dpp::kMirror>(v, identity));
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+__device__ __forceinline__ int wave_isum_full(int v)
+{
+    return wave_reduce_full(v, 0, [](int a, int b) { return a + b; });
+}
+__device__ __forceinline__ int wave_min_full(int v)
+{
+    return wave_reduce_full(v, INT_MAX, [](int a, int b) { return min(a, b); });
+}
+__device__ __forceinline__ int wave_max_full(int v)
+{
+    return wave_reduce_full(v, INT_MIN, [](int a, int b) { return max(a, b); });
+}
+
 // Exclusive scan of n uint32 values (out may alias in).  Workspace bytes
 // from scan_workspace(n).
 size_t scan_workspace(int64_t n);

@@ -407,6 +407,10 @@ __device__ __forceinline__ void unit_grad(const T (*W)[2], const T* ph, T* grad)
     }
 }
 
+// two consecutive values at the alignment of one
+template <typename T>
+using pair_t = T __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+
 template <typename T, int D, bool PUSH, bool UNIT>
 __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__ phi, T qm, T dt,
                                          double mass, T* x, T* v, double& ke, double* pm)
@@ -440,8 +444,19 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
         uint32_t sidx[1 << D];
         support_indices<D>(g, c, sidx);
         T ph[1 << D];
+        if (c[D - 1] + 1 != g.w[D - 1]) {
+            // support nodes i and i + 1 differ in the fastest axis only and
+            // are neighbours in memory unless that axis wraps: paired loads
 #pragma unroll
-        for (int i = 0; i < (1 << D); ++i) ph[i] = phi[sidx[i]];
+            for (int i = 0; i < (1 << D); i += 2) {
+                const pair_t<T> q = *reinterpret_cast<const pair_t<T>*>(phi + sidx[i]);
+                ph[i] = q.x;
+                ph[i + 1] = q.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < (1 << D); ++i) ph[i] = phi[sidx[i]];
+        }
         if constexpr (UNIT) {
             unit_grad<T, D>(W, ph, grad);
         } else {
@@ -598,12 +613,12 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
         double vals[4] = {ke, pm[0], pm[1], pm[2]};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const double r = wave_sum(vals[q]);
+            const double r = wave_sum_full(vals[q]);
             if (lane == 0) wdiag[wave][q] = r;
         }
     }
     if (counters) {
-        const int ab = wave_isum((int)absorbed), oo = wave_isum((int)oor);
+        const int ab = wave_isum_full((int)absorbed), oo = wave_isum_full((int)oor);
         if (lane == 0) {
             if (ab) atomicAdd(&counters[2], (unsigned long long)ab);
             if (oo) atomicAdd(&counters[0], (unsigned long long)oo);
@@ -648,7 +663,7 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
             }
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int mn = wave_min(lmin[d]), mx = wave_max(lmax[d]);
+            const int mn = wave_min_full(lmin[d]), mx = wave_max_full(lmax[d]);
             if (lane == 0 && mn != INT_MAX) {
                 atomicMin(&bmin[d], mn);
                 atomicMax(&bmax[d], mx);
@@ -681,10 +696,10 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 }
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const long long t = wave_lsum(ls[d]);
+                const long long t = wave_lsum_full(ls[d]);
                 if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&bsum[d]), (unsigned long long)t);
             }
-            const int tc = wave_isum(lc);
+            const int tc = wave_isum_full(lc);
             if (lane == 0) atomicAdd(&bcnt, tc);
             __syncthreads();
             tsize = 1;
@@ -723,11 +738,15 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                     W[d][0] = (T)1 - sw[j][d];
                     W[d][1] = sw[j][d];
                 }
+                // a coefficient below the cut is replaced by 0, whose term
+                // rounds to 0 (one select on the coefficient, not two on
+                // the int64 term)
                 long long t8[NS];
 #pragma unroll
                 for (int i = 0; i < NS; ++i) {
-                    const T cf = cic_coef<T, D>(W, i);
-                    t8[i] = coef_kept<T>(cf) ? round_to_ll((double)cf * wsc) : 0ll;
+                    const T c0 = cic_coef<T, D>(W, i);
+                    const T cf = coef_kept<T>(c0) ? c0 : (T)0;
+                    t8[i] = round_to_ll((double)cf * wsc);
                 }
                 bool inside = use_lds;
 #pragma unroll
@@ -819,10 +838,11 @@ __global__ __launch_bounds__(kThreads) void k_push_deposit(
                 }
 #pragma unroll
                 for (int i = 0; i < NS; ++i) {
-                    const T cf = cic_coef<T, D>(W, i);
-                    if (!coef_kept<T>(cf)) continue;
+                    // a coefficient below the cut adds a zero term.
                     // (w*cf)*2^S: scaling by a power of two is exact, so one
                     // product with w*2^S rounds identically
+                    const T c0 = cic_coef<T, D>(W, i);
+                    const T cf = coef_kept<T>(c0) ? c0 : (T)0;
                     run[i] += round_to_ll((double)cf * wsc);
                 }
             }
