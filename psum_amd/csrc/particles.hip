@@ -502,8 +502,13 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
     return alive;
 }
 
+// 2D: at least 5 waves per SIMD (the 2D kernels fit it without spills and
+// were measured faster); 3D keeps the compiler's register budget
+template <int D>
+constexpr int push_min_waves() { return D == 2 ? 5 : 1; }
+
 template <typename T, int D, bool PUSH, bool DEPOSIT, bool UNIT = false>
-__global__ __launch_bounds__(kThreads) void k_push_deposit(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_min_waves<D>(), 8))) void k_push_deposit(
     DevGrid g, T* __restrict__ x0, T* __restrict__ x1, T* __restrict__ x2, T* __restrict__ v0,
     T* __restrict__ v1, T* __restrict__ v2, int64_t n, const T* __restrict__ phi, T qm, T dt,
     double mass, double w_dep, double scale, unsigned long long* __restrict__ acc,
