@@ -6,10 +6,14 @@ one GPU holds: 256^3 cells with grounded (phi = 0) walls, 32 electrons + 32
 ions per cell (1,073,741,824 particles, mass ratio 1836, Te/Ti = 10),
 absorbing particle walls with compaction, 7-point multigrid, FP32 particle
 state with FP32 V-cycles inside FP64 residual refinement to
-||r||/||b|| <= 1e-8, dt 0.1.  The autotuner picks the sort interval and bin;
-every timed window starts on a sort step, so it holds whole sort periods
-(`sorts_in_window`).  The same inputs (cast to FP64) run through the plain
-FP64 path for the mixed/FP64 ratio.
+||r||/||b|| <= 1e-8, dt 0.1.  The autotuner picks the sort bin and a sort
+interval per species (ions keep their order far longer than electrons).
+Every timed window starts on a step where every species sorts; the default
+window (200 steps) is a whole number of every species' sort periods, so each
+sort is counted at its share (`sorts_in_window`, per species).  A shorter
+--steps still times exactly that many steps and counts the sorts at its
+start whole, which can only overstate the cost per step.  The same inputs
+(cast to FP64) run through the plain FP64 path for the mixed/FP64 ratio.
 
 Beside it: the 2D two-stream case (1024^2 periodic nodes, 100 e-/cell,
 mixed and FP64), the plain FP64 2D Langmuir case (128^2, 64 e-/cell), the
@@ -25,6 +29,7 @@ engine runs on one GPU.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -42,23 +47,37 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def _align_to_sort(st, sort_every):
-    """Untimed steps until the next step sorts (step index a multiple of
-    sort_every), so that a timed window starts on a sort step; returns how
-    many were run."""
-    if sort_every <= 0:
+def _sort_period(intervals):
+    """Least common multiple of the positive sort intervals (0: none)."""
+    period = 0
+    for k in intervals:
+        if k > 0:
+            period = k if period == 0 else period * k // math.gcd(period, k)
+    return period
+
+
+def _align_to_sort(st, period):
+    """Untimed steps until the step index is a multiple of `period`, the step
+    where every species sorts, so that a timed window starts there; returns
+    how many were run."""
+    if period <= 0:
         return 0
-    extra = (-st.stats()["steps"]) % sort_every
+    extra = (-st.stats()["steps"]) % period
     if extra:
         st.run(extra, history=False)
     return extra
 
 
-def timed_run(st, n, steps, sort_every):
-    """`steps` timed steps of stepper st (n particles) from a sort step;
-    wall clock around the loop, HIP events around every push launch."""
+def timed_run(st, n, steps, sort_every, absorbed_at_load=0):
+    """`steps` timed steps of stepper st (n particles when loaded, the
+    absorbed counter then at absorbed_at_load) from a step where every
+    species sorts; sort_every: one interval or one per species.  Wall clock
+    around the loop, HIP events around every push launch.  Particle-steps
+    count the particles alive: K * (alive at the window's start + at its
+    end) / 2 (absorption is steady over a window)."""
     import torch
-    extra = _align_to_sort(st, sort_every)
+    every = list(sort_every) if isinstance(sort_every, (list, tuple)) else [sort_every]
+    extra = _align_to_sort(st, _sort_period(every))
     torch.cuda.synchronize()
     s0 = st.stats()
     st.set_timing(True)
@@ -68,8 +87,11 @@ def timed_run(st, n, steps, sort_every):
     wall = time.perf_counter() - t0
     push_ms, launches, pushed = st.push_timing()
     s1 = st.stats()
-    sorts = 0 if sort_every <= 0 else (steps + sort_every - 1) // sort_every
-    return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, particle_steps_per_s=n * steps / wall,
+    sorts = [0 if k <= 0 else (steps + k - 1) // k for k in every]
+    sorts = sorts[0] if len(sorts) == 1 else sorts
+    alive = n - 0.5 * ((s0["absorbed"] - absorbed_at_load) + (s1["absorbed"] - absorbed_at_load))
+    return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, particle_steps_per_s=alive * steps / wall,
+                particles_alive_avg=alive,
                 push_ms_avg=push_ms / max(1, launches), push_launches=launches,
                 particles_per_launch=pushed / max(1, launches),
                 cycles_per_step=(s1["cycles"] - s0["cycles"]) / steps,
@@ -415,9 +437,9 @@ def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_pr
     if schedule is None:
         t0 = time.time()
         tun = st.tune(warmup=3, probe=tune_probe)
-        schedule = (tun["sort_every"], tun["sort_shift"])
-        log(f"[bench] c3: tuned in {time.time() - t0:.1f}s: sort every {schedule[0]} steps on "
-            f"{2 ** schedule[1]}^3-cell bins")
+        schedule = (tun["sort_every"], tun["sort_shift"], tun["species_sort_every"])
+        log(f"[bench] c3: tuned in {time.time() - t0:.1f}s: sort every {schedule[2]} steps (per species) "
+            f"on {2 ** schedule[1]}^3-cell bins")
     smoothing = (pre, post)
     smoothing_ms = None
     if tune_smoothing:
@@ -425,17 +447,24 @@ def run_c3(E, wl, dtype, steps, warmup, depth, pre, post, schedule=None, tune_pr
         smoothing, smoothing_ms = st.tune_smoothing(SMOOTHING_CANDIDATES, steps=8)
         log(f"[bench] c3 {dtype}: smoothing tuned in {time.time() - t0:.1f}s: pre/post {smoothing} ("
             + ", ".join(f"{c}: {v:.2f} ms" for c, v in smoothing_ms.items()) + ")")
+    absorbed0 = 0
     if tun is not None or tune_smoothing:
         # tuning advanced the state: start again from the initial particles,
         # so that the mixed and FP64 runs time the same physical steps
         for i, s in enumerate(wl["species"]):
             st.load(i, [a.to(tdt) for a in s["pos"]], [a.to(tdt) for a in s["vel"]])
+        absorbed0 = st.stats()["absorbed"]
     st.set_schedule(0, schedule[0], schedule[1])
+    every = [schedule[0]] * len(wl["species"])
+    if len(schedule) > 2:
+        every = list(schedule[2])
+        for q, k in enumerate(every):
+            st.set_species_sort(q, k)
     if warmup:
         st.run(warmup, history=False)
-    res = timed_run(st, n, steps, schedule[0])
+    res = timed_run(st, n, steps, every, absorbed0)
     res["particles"] = n
-    res["schedule"] = {"sort_every": schedule[0], "sort_shift": schedule[1]}
+    res["schedule"] = {"sort_every": every, "sort_shift": schedule[1]}
     res["smoothing"] = list(smoothing)
     if smoothing_ms is not None:
         res["smoothing_probe_ms_per_step"] = {f"{c[0]},{c[1]}": v for c, v in smoothing_ms.items()}
@@ -478,7 +507,8 @@ def cpu_baseline_c3(cells, ppc, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline (3D box) run")
+    ap.add_argument("--steps", type=int, default=200,
+                    help="timed steps of the headline (3D box) run (200: whole sort periods of every species)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--pre", type=int, default=3)
     ap.add_argument("--post", type=int, default=3)
@@ -646,7 +676,7 @@ def main():
                                f"refinement to 1e-8, dt 0.1",
                    "cells": [args.c3_cells] * 3, "particles": c3["particles"],
                    "mg_depth": args.c3_depth, "mg_pre_post": c3["smoothing"],
-                   "sort_every": sched[0], "sort_bin_cells": 2 ** sched[1],
+                   "sort_every": c3["schedule"]["sort_every"], "sort_bin_cells": 2 ** sched[1],
                    "sorts_in_window": c3["sorts_in_window"], "parallelism": "single GPU"},
         "fp64": None if c3_64 is None else _leg(c3_64),
         "mixed_over_fp64": None if c3_64 is None else
@@ -669,7 +699,8 @@ def main():
 def _leg(r):
     """The JSON fields of one timed run."""
     keys = ("ms_per_step", "particle_steps_per_s", "push_ms_avg", "push_launches",
-            "particles_per_launch", "cycles_per_step", "last_rel_res", "tile_fallback_blocks_in_window",
+            "particles_per_launch", "particles_alive_avg", "cycles_per_step", "last_rel_res",
+            "tile_fallback_blocks_in_window",
             "absorbed_in_window", "sort_every", "sorts_in_window", "align_steps")
     return {k: r[k] for k in keys if k in r}
 

@@ -471,8 +471,18 @@ psum_status psum_stepper_restore(void* h, const char* path);
 /* Solver and sort schedule of a stepper: fixed_cycles > 0 switches its
  * multigrid to that many V-cycles per solve (one per refinement iteration,
  * no tolerance check); sorts every sort_every steps (0 = never) on
- * 2^sort_shift-cell bins.  No reference counterpart (tuning knobs). */
+ * 2^sort_shift-cell bins.  sort_every is the interval of the species
+ * without one of their own (psum_stepper_set_species_sort); those keep
+ * theirs.  No reference counterpart (tuning knobs). */
 psum_status psum_stepper_set_schedule(void* h, int fixed_cycles, int sort_every, int sort_shift);
+/* Species s sorts every `every` steps (every > 0; a sort before step k when
+ * k % every == 0) instead of on the stepper's sort_every, or on sort_every
+ * again (every = 0).  psum_stepper_tune sets every species' interval (slow
+ * species, such as ions, keep their order much longer than fast ones).  No
+ * reference counterpart (tuning knob). */
+psum_status psum_stepper_set_species_sort(void* h, int species, int every);
+/* The interval species s sorts on: its own, else sort_every (0: never). */
+psum_status psum_stepper_species_sort(void* h, int species, int* every);
 
 /* FD Poisson matrix A = -div(eps grad) of the reference frontends
  * (Poisson_solver_2d.hpp:182-410, Poisson_solver_3d.hpp:178-431, Cartesian,
@@ -550,9 +560,11 @@ typedef struct {
 } psum_batch_tuning;
 
 /* The autotuner of psum_batch_tune for one stepper: returns the calibrated
- * cycle count and the chosen sort interval / bin shift (the bin shift is
- * applied; apply the rest with psum_stepper_set_schedule).  The state
- * advances by the tuning steps. */
+ * cycle count and the chosen sort interval / bin shift.  Each species gets
+ * its own interval (timed sort and push per species); sort_every returns
+ * the shortest.  The bin shift and the per-species intervals are applied
+ * (psum_stepper_species_sort reads them); apply the cycle count with
+ * psum_stepper_set_schedule.  The state advances by the tuning steps. */
 psum_status psum_stepper_tune(void* h, int warmup, int probe, psum_batch_tuning* out);
 
 void* psum_batch_create(const psum_step_config* cfgs, int n);

@@ -218,6 +218,8 @@ SIGNATURES = [
     ("psum_stepper_save", ST, [P, C.c_char_p]),
     ("psum_stepper_restore", ST, [P, C.c_char_p]),
     ("psum_stepper_set_schedule", ST, [P, C.c_int, C.c_int, C.c_int]),
+    ("psum_stepper_set_species_sort", ST, [P, C.c_int, C.c_int]),
+    ("psum_stepper_species_sort", ST, [P, C.c_int, C.POINTER(C.c_int)]),
     ("psum_stepper_mg_tuning", ST, [P, C.c_char_p]),
     ("psum_stepper_set_smoothing", ST, [P, C.c_int, C.c_int]),
     ("psum_stepper_tune_smoothing", ST, [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), P]),

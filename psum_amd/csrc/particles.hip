@@ -407,6 +407,37 @@ __device__ __forceinline__ void unit_grad(const T (*W)[2], const T* ph, T* grad)
     }
 }
 
+// Cell index and offsets of x as base_cell<D, UNIT> takes them in FP64,
+// rounded to T.  UNIT (unit spacing, origin 0, upper bounds exact in float)
+// with T = float takes them in FP32 with the same bits: x - lb is x, floor
+// and the range test are exact, and x - ci is exact (Sterbenz for ci >= 1
+// and ci <= -2; ci = 0 gives x) except for ci = -1 and |x| < 2^-29, where
+// both round 1 - |x| to 1.0f.
+template <typename T, int D, bool UNIT>
+__device__ __forceinline__ bool cell_of(const DevGrid& g, const T* x, int* cw, T* s)
+{
+    if constexpr (UNIT && sizeof(T) == 4) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const float f = floorf(x[d]);
+            const int w = g.w[d];
+            if (!(f >= (float)(-w) && f <= (float)(2 * w - 2))) return false;
+            const int ci = (int)f;
+            s[d] = x[d] - (float)ci;
+            cw[d] = ci < 0 ? ci + w : (ci >= w ? ci - w : ci);
+        }
+        return true;
+    } else {
+        double xd[D], sd[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
+        if (!base_cell<D, UNIT>(g, xd, cw, sd)) return false;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s[d] = (T)sd[d];
+        return true;
+    }
+}
+
 // two consecutive values at the alignment of one
 template <typename T>
 using pair_t = T __attribute__((ext_vector_type(2), aligned(sizeof(T))));
@@ -426,15 +457,13 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
     T grad[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) grad[d] = (T)0;
-    double xd[D], s[D];
+    T s[D];
     int c[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
-    if (base_cell<D, UNIT>(g, xd, c, s)) {
+    if (cell_of<T, D, UNIT>(g, x, c, s)) {
         T W[D][2], G[D][2];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const T sd = (T)s[d];
+            const T sd = s[d];
             W[d][0] = (T)1 - sd;
             W[d][1] = sd;
             const T dr = (T)g.delta_r[d];
@@ -495,8 +524,13 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
 #pragma unroll
     for (int d = 0; d < D; ++d)
         if (!g.periodic[d]) {
-            const double xx = (double)x[d];
-            if (!(xx >= g.lb[d] && xx < g.ub[d])) alive = false;
+            if constexpr (UNIT && sizeof(T) == 4) {
+                // lb = 0 and ub exact in float: the same test in FP32
+                if (!(x[d] >= 0.0f && x[d] < (float)g.ub[d])) alive = false;
+            } else {
+                const double xx = (double)x[d];
+                if (!(xx >= g.lb[d] && xx < g.ub[d])) alive = false;
+            }
         }
     if (!alive) x[0] = (T)NAN;
     return alive;
@@ -582,13 +616,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_m
             moved = true;
         }
         if (DEPOSIT && alive) {
-            double xd[D], s[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
-            if (base_cell<D, UNIT>(g, xd, cw[j], s)) {
+            T s[D];
+            if (cell_of<T, D, UNIT>(g, x, cw[j], s)) {
                 ok[j] = true;
 #pragma unroll
-                for (int d = 0; d < D; ++d) sw[j][d] = (T)s[d];
+                for (int d = 0; d < D; ++d) sw[j][d] = s[d];
             } else {
                 ++oor;
             }
@@ -1059,10 +1091,13 @@ psum_status launch_push(const psum_grid* g, void* const* pos, void* const* vel, 
     auto* a = reinterpret_cast<unsigned long long*>(acc);
     auto* c = reinterpret_cast<unsigned long long*>(counters);
     const T* ph = static_cast<const T*>(phi);
-    // unit node spacing: the gather's gradient weights are +-products of W
-    // and the cell index needs no products with the spacing
+    // unit node spacing from the origin 0 (upper bounds exact in float): the
+    // gather's gradient weights are +-products of W, the cell index needs no
+    // products with the spacing, and FP32 positions take it in FP32
     bool unit = true;
-    for (int d = 0; d < g->dim; ++d) unit = unit && g->delta_r[d] == 1.0 && g->delta[d] == 1.0;
+    for (int d = 0; d < g->dim; ++d)
+        unit = unit && g->delta_r[d] == 1.0 && g->delta[d] == 1.0 && g->lb[d] == 0.0 &&
+               (double)(float)g->ub[d] == g->ub[d];
 #define PSUM_LAUNCH(DIM, PU, DE, UN)                                                          \
     hipLaunchKernelGGL((k_push_deposit<T, DIM, PU, DE, UN>), dim3(nb), dim3(kThreads), 0, st, \
                        dg, p[0], p[1], p[2], v[0], v[1], v[2], n, ph, (T)qm, (T)dt, mass,      \

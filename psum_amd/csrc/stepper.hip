@@ -374,7 +374,7 @@ public:
         reserve_hist(steps);
         hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(kT), 0, st_, (long long*)hist_count_, (int64_t)1);
         for (int s = 0; s < steps; ++s) {
-            if (cfg_.sort_every > 0 && !no_sort_ && step_ % cfg_.sort_every == 0) sort_all();
+            if (!no_sort_) sort_due_species();
             step_body(diagnostics);
         }
         if (hist_host)
@@ -425,6 +425,7 @@ public:
                 if (timing_) {
                     S_TRY(hipEventRecord(ev_[2 * nev_ + 1], st_));
                     push_particles_ += sp.n;
+                    ev_sp_[nev_] = q;
                     if (++nev_ == kEv) flush_events();
                 }
                 if (diagnostics) {
@@ -454,6 +455,7 @@ public:
             for (auto& e : ev_) S_TRY(hipEventCreate(&e));
         timing_ = on;
         push_ms_ = 0.0;
+        for (double& t : push_ms_sp_) t = 0.0;
         push_launches_ = 0;
         push_particles_ = 0;
         nev_ = 0;
@@ -941,6 +943,32 @@ public:
     void set_sort_shift(int b) { sort_shift_ = b; }
     int sort_shift() const { return sort_shift_; }
     void set_sort_every(int k) { cfg_.sort_every = k; }
+    // species q sorts every k steps (k > 0), or on the stepper's interval
+    // (k = 0)
+    void set_species_sort_every(int q, int k)
+    {
+        if (q < 0 || q >= cfg_.nspecies || k < 0)
+            throw std::invalid_argument("psum_stepper_set_species_sort: species in range, interval >= 0");
+        sp_every_[q] = k;
+    }
+    int species_sort_every(int q) const { return sp_every_[q] > 0 ? sp_every_[q] : cfg_.sort_every; }
+    bool sort_due(int q) const
+    {
+        const int k = species_sort_every(q);
+        return k > 0 && step_ % k == 0 && sp_[q].n > 0;
+    }
+    bool any_sort_due() const
+    {
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            if (sort_due(q)) return true;
+        return false;
+    }
+    // the species whose interval divides the step count sort before the step
+    void sort_due_species()
+    {
+        for (int q = 0; q < cfg_.nspecies; ++q)
+            if (sort_due(q)) sort_species(q);
+    }
     void invalidate_deposit() { acc_valid_ = false; }
     void set_interp_order(int o)
     {
@@ -1004,6 +1032,7 @@ public:
         if (order_ != 1) throw std::invalid_argument(std::string(what) + ": the adjoint covers CIC (order 1) only");
     }
     int sort_every() const { return cfg_.sort_every; }
+    int nspecies() const { return cfg_.nspecies; }
     void set_stable_buffers(bool on) { stable_buffers_ = on; }
     void set_fixed_cycles(int k)
     {
@@ -1017,11 +1046,14 @@ public:
     int64_t steps_done() const { return step_; }
     void count_step() { ++step_; }
     void uncount_step() { --step_; }
-    // total push milliseconds since set_timing(true), after a stream sync
-    void flush_timing(double* ms)
+    // total push milliseconds since set_timing(true), after a stream sync;
+    // ms_sp: the same per species
+    void flush_timing(double* ms, double* ms_sp = nullptr)
     {
         flush_events();
         if (ms) *ms = push_ms_;
+        if (ms_sp)
+            for (int q = 0; q < cfg_.nspecies; ++q) ms_sp[q] = push_ms_sp_[q];
     }
     bool acc_valid() const { return acc_valid_; }
     void prepare_capture()
@@ -1054,15 +1086,18 @@ public:
     }
     // Autotuner: `warmup` tolerance-controlled steps calibrate the fixed
     // V-cycle count (the most any solve needed, plus one); then for each sort
-    // bin shift 0..2 a timed sort and `probe` timed steps give t_sort and the
-    // push time t_push(j) j steps after a sort, and the interval K in
-    // {5, 10, 20, 50, 100, 200} minimising (t_sort + sum_{j<K} t_push(j)) / K
-    // is chosen (t_push beyond the probe window extrapolated linearly from its
-    // second half).  Returns the choice; the state has advanced.
+    // bin shift 0..2 a timed sort of each species and `probe` timed steps give
+    // per species t_sort and the push time t_push(j) j steps after its sort,
+    // and its interval K in {5, 10, 20, 50, 100, 200} minimising
+    // (t_sort + sum_{j<K} t_push(j)) / K (t_push beyond the probe window
+    // extrapolated linearly from its second half).  The shift with the
+    // smallest summed cost is kept with its per-species intervals; the
+    // returned sort_every is the shortest of them.  The state has advanced.
     void tune(int warmup, int probe, psum_batch_tuning* out)
     {
         if (warmup < 1 || probe < 2) throw std::invalid_argument("tune: warmup >= 1, probe >= 2");
         static const int kCand[] = {5, 10, 20, 50, 100, 200};
+        const int ns = cfg_.nspecies;
         // the replayed step warm-starts every solve from the previous
         // potential: the first warm-up step (a solve from a cold start) does
         // not enter the calibration
@@ -1070,47 +1105,65 @@ public:
         reset_cycles_seen();
         run_steps(warmup > 1 ? warmup - 1 : 1, nullptr, true, false);
         const int cycles = max_cycles_seen() + 1;
-        int best_shift = 0, best_k = 0;
+        int best_shift = 0, best_k[4] = {0, 0, 0, 0};
         double best_cost = 1e300, best_sort = 0.0, best_push = 0.0;
         for (int shift = 0; shift <= 2; ++shift) {
             set_sort_shift(shift);
-            const float t_sort = timed_sort();
-            std::vector<double> tp(probe);
+            double t_sort[4] = {0, 0, 0, 0};
+            for (int q = 0; q < ns; ++q) t_sort[q] = timed_sort(q);
+            std::vector<double> tp((size_t)probe * ns);
             for (int j = 0; j < probe; ++j) {
                 set_timing(true);
                 step_body(false);
                 S_TRY(hipStreamSynchronize(st_));
-                double ms = 0.0;
-                flush_timing(&ms);
-                tp[j] = ms;
+                double ms = 0.0, ms_sp[4] = {0, 0, 0, 0};
+                flush_timing(&ms, ms_sp);
+                for (int q = 0; q < ns; ++q) tp[(size_t)j * ns + q] = ms_sp[q];
             }
             set_timing(false);
-            const int h0 = probe / 2;
-            double sx = 0, sy = 0, sxx = 0, sxy = 0;
-            for (int j = h0; j < probe; ++j) {
-                sx += j; sy += tp[j]; sxx += (double)j * j; sxy += j * tp[j];
-            }
-            const double m = probe - h0;
-            const double den = m * sxx - sx * sx;
-            const double slope = den > 0 ? std::max(0.0, (m * sxy - sx * sy) / den) : 0.0;
-            const double icpt = (sy - slope * sx) / m;
-            for (int K : kCand) {
-                double sum = t_sort;
-                for (int j = 0; j < K; ++j) sum += j < probe ? tp[j] : icpt + slope * j;
-                const double cost = sum / K;
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_shift = shift;
-                    best_k = K;
-                    best_sort = t_sort;
-                    best_push = tp[0];
+            double cost = 0.0, sort_sum = 0.0, push0 = 0.0;
+            int ks[4] = {0, 0, 0, 0};
+            for (int q = 0; q < ns; ++q) {
+                const int h0 = probe / 2;
+                double sx = 0, sy = 0, sxx = 0, sxy = 0;
+                for (int j = h0; j < probe; ++j) {
+                    const double y = tp[(size_t)j * ns + q];
+                    sx += j; sy += y; sxx += (double)j * j; sxy += j * y;
                 }
+                const double m = probe - h0;
+                const double den = m * sxx - sx * sx;
+                const double slope = den > 0 ? std::max(0.0, (m * sxy - sx * sy) / den) : 0.0;
+                const double icpt = (sy - slope * sx) / m;
+                double cq = 1e300;
+                for (int K : kCand) {
+                    double sum = t_sort[q];
+                    for (int j = 0; j < K; ++j) sum += j < probe ? tp[(size_t)j * ns + q] : icpt + slope * j;
+                    if (sum / K < cq) {
+                        cq = sum / K;
+                        ks[q] = K;
+                    }
+                }
+                cost += cq;
+                sort_sum += t_sort[q];
+                push0 += tp[q];
+            }
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_shift = shift;
+                for (int q = 0; q < ns; ++q) best_k[q] = ks[q];
+                best_sort = sort_sum;
+                best_push = push0;
             }
         }
         set_sort_shift(best_shift);
+        int kmin = 0;
+        for (int q = 0; q < ns; ++q) {
+            set_species_sort_every(q, best_k[q]);
+            if (best_k[q] > 0 && (kmin == 0 || best_k[q] < kmin)) kmin = best_k[q];
+        }
         if (out) {
             out->fixed_cycles = cycles;
-            out->sort_every = best_k;
+            out->sort_every = kmin;
             out->sort_shift = best_shift;
             out->pad_ = 0;
             out->sort_ms = best_sort;
@@ -1119,15 +1172,16 @@ public:
         }
     }
 
-    // HIP-event time of one sort pass (all species), ms
-    float timed_sort()
+    // HIP-event time of one sort pass of species q (all species: q < 0), ms
+    float timed_sort(int q = -1)
     {
         if (!aev_[0]) {
             S_TRY(hipEventCreate(&aev_[0]));
             S_TRY(hipEventCreate(&aev_[1]));
         }
         S_TRY(hipEventRecord(aev_[0], st_));
-        sort_all();
+        if (q < 0) sort_all();
+        else sort_species(q);
         S_TRY(hipEventRecord(aev_[1], st_));
         S_TRY(hipEventSynchronize(aev_[1]));
         float ms = 0;
@@ -1187,6 +1241,9 @@ private:
     hipEvent_t aev_[2] = {nullptr, nullptr};
     static constexpr int kEv = 64;
     hipEvent_t ev_[2 * kEv] = {};
+    int ev_sp_[kEv] = {};          // species of each event pair
+    double push_ms_sp_[4] = {};    // push milliseconds per species since set_timing(true)
+    int sp_every_[4] = {};         // per-species sort interval; 0: cfg_.sort_every
     int nev_ = 0;
     bool timing_ = false;
     double push_ms_ = 0.0;
@@ -1200,6 +1257,7 @@ private:
             float ms = 0.f;
             S_TRY(hipEventElapsedTime(&ms, ev_[2 * i], ev_[2 * i + 1]));
             push_ms_ += ms;
+            push_ms_sp_[ev_sp_[i]] += ms;
             ++push_launches_;
         }
         nev_ = 0;
@@ -1263,6 +1321,11 @@ private:
 public:
     void sort_all()
     {
+        for (int q = 0; q < cfg_.nspecies; ++q) sort_species(q);
+    }
+
+    void sort_species(int q)
+    {
         const int D = cfg_.grid.dim;
         sort_buffers();
         // keys: the 2^shift-cell bin refined by the velocity signs, so that
@@ -1271,9 +1334,9 @@ public:
         const uint32_t nbins = sort_shift_ ? bin_count(cfg_.grid, sort_shift_) : (uint32_t)nn_;
         const bool vel_keys = ((uint64_t)nbins << D) < (uint64_t)PSUM_KEY_INVALID;
         const uint32_t nkeys = vel_keys ? nbins << D : nbins;
-        for (int q = 0; q < cfg_.nspecies; ++q) {
+        {
             Species& sp = sp_[q];
-            if (sp.n == 0) continue;
+            if (sp.n == 0) return;
             CHK(bin_keys(&cfg_.grid, cfg_.dtype, sp.pos, sp.n, keys_, sort_shift_, st_, vel_keys ? sp.vel : nullptr));
             CHK(psum_hip_sort(keys_, sp.n, nkeys, skeys_, perm_, sort_work_, sort_bytes_, st_));
             void* src[6];
@@ -1439,14 +1502,13 @@ public:
                 bool forked = false;
                 for (int i : lane_inst_[l]) {
                     Stepper& p = *inst_[i];
-                    const int k = p.sort_every();
-                    if (k > 0 && p.steps_done() % k == 0) {
+                    if (p.any_sort_due()) {
                         if (!forked) {
                             S_TRY(hipEventRecord(lane_fork_[l], lane_st_[l]));
                             forked = true;
                         }
                         S_TRY(hipStreamWaitEvent(p.stream(), lane_fork_[l], 0));
-                        p.sort_all();
+                        p.sort_due_species();
                         S_TRY(hipEventRecord(join_[i], p.stream()));
                         S_TRY(hipStreamWaitEvent(lane_st_[l], join_[i], 0));
                     }
@@ -1838,6 +1900,23 @@ psum_status psum_stepper_set_schedule(void* h, int fixed_cycles, int sort_every,
         if (fixed_cycles > 0) s->set_fixed_cycles(fixed_cycles);
         s->set_sort_every(sort_every);
         s->set_sort_shift(sort_shift);
+    });
+}
+
+psum_status psum_stepper_set_species_sort(void* h, int species, int every)
+{
+    if (!h) { set_error("null stepper"); return PSUM_ERR_ARG; }
+    return guarded([&]() { static_cast<Stepper*>(h)->set_species_sort_every(species, every); });
+}
+
+psum_status psum_stepper_species_sort(void* h, int species, int* every)
+{
+    if (!h || !every) { set_error("psum_stepper_species_sort: null argument"); return PSUM_ERR_ARG; }
+    return guarded([&]() {
+        Stepper* s = static_cast<Stepper*>(h);
+        if (species < 0 || species >= s->nspecies())
+            throw std::invalid_argument("psum_stepper_species_sort: species out of range");
+        *every = s->species_sort_every(species);
     });
 }
 

@@ -555,7 +555,9 @@ class Stepper:
         from ._lib import PsumBatchTuning
         t = PsumBatchTuning()
         check(lib().psum_stepper_tune(self.h, warmup, probe, C.byref(t)), "psum_stepper_tune")
-        return {f: getattr(t, f) for f, _ in PsumBatchTuning._fields_ if f != "pad_"}
+        out = {f: getattr(t, f) for f, _ in PsumBatchTuning._fields_ if f != "pad_"}
+        out["species_sort_every"] = [self.species_sort(q) for q in range(len(self.n))]
+        return out
 
     def mg_tuning(self, options):
         """Multigrid kernel-shape options ("fuse 0|1", "tail 0|1", "line 0|1");
@@ -581,9 +583,21 @@ class Stepper:
 
     def set_schedule(self, fixed_cycles=0, sort_every=0, sort_shift=0):
         """fixed_cycles > 0: that many V-cycles per solve, no tolerance check;
-        sort every `sort_every` steps on 2^sort_shift-cell bins."""
+        sort every `sort_every` steps on 2^sort_shift-cell bins (the species
+        without an interval of their own, see set_species_sort)."""
         check(lib().psum_stepper_set_schedule(self.h, fixed_cycles, sort_every, sort_shift),
               "set_schedule")
+
+    def set_species_sort(self, s, every):
+        """Species s sorts every `every` steps (> 0), or on the stepper's
+        sort_every again (0).  tune() sets every species' interval."""
+        check(lib().psum_stepper_set_species_sort(self.h, int(s), int(every)), "set_species_sort")
+
+    def species_sort(self, s):
+        """The interval species s sorts on (0: never)."""
+        v = C.c_int()
+        check(lib().psum_stepper_species_sort(self.h, int(s), C.byref(v)), "species_sort")
+        return v.value
 
     def load(self, s, pos, vel):
         """Replace species s with pos / vel: numpy arrays, or device tensors of

@@ -123,6 +123,49 @@ def test_bounded_3d_sort_compacts_absorbed():
     assert absorbed > 0 and alive < n0
 
 
+def test_per_species_sort_intervals():
+    """Ions sorted every 6 steps, electrons every 2: the deposit is an exact
+    integer sum, so phi and the field energy are bit-identical to sorting
+    both species every 2 steps; each species holds the same particles (as a
+    set) with the same absorbed count.  set_schedule keeps a species' own
+    interval; the tuner gives every species one of its candidates."""
+    from psum_amd import engine as E
+    from psum_amd.workloads import bounded_3d
+    wl = bounded_3d(cells=12, ppc=4, seed=9, dtype=np.float32)
+    kw = dict(dtype="f32", mixed_mg=True, depth=3, tol=1e-8, max_cycles=40, dt=wl["dt"], rho_bg=0.0,
+              fixed=wl["fixed"], sort_every=2)
+    a = E.Stepper(wl["grid"], wl["species"], **kw)
+    b = E.Stepper(wl["grid"], wl["species"], **kw)
+    b.set_species_sort(1, 6)
+    assert (b.species_sort(0), b.species_sort(1)) == (2, 6)
+    b.set_schedule(0, 3, 0)
+    assert (b.species_sort(0), b.species_sort(1)) == (3, 6)
+    b.set_schedule(0, 2, 0)
+    ha, hb = a.run(12), b.run(12)
+    assert np.array_equal(ha[:, 0], hb[:, 0])
+    assert np.max(np.abs(ha[:, 1] - hb[:, 1]) / np.abs(ha[:, 1])) < 1e-12
+    assert np.array_equal(a.phi(), b.phi())
+    assert a.stats()["absorbed"] == b.stats()["absorbed"]
+
+    def alive_sorted(st, s):
+        p, v = st.state(s)
+        m = ~np.isnan(p[0])
+        rows = np.stack([x[m] for x in p + v], axis=1)
+        return rows[np.lexsort(rows.T[::-1])]
+
+    for s in range(2):
+        assert np.array_equal(alive_sorted(a, s), alive_sorted(b, s))
+    with pytest.raises(E.PsumError):
+        b.set_species_sort(2, 4)
+    tun = a.tune(warmup=3, probe=4)
+    every = tun["species_sort_every"]
+    assert len(every) == 2 and all(k in (5, 10, 20, 50, 100, 200) for k in every)
+    assert tun["sort_every"] == min(every)
+    assert [a.species_sort(q) for q in range(2)] == every
+    a.close()
+    b.close()
+
+
 def test_sort_order_is_cell_then_velocity_signs():
     """psum_stepper_sort: stable order by cell key * 2^D + velocity-sign bits
     (dimension 0 the most significant), restated with the oracle's keys and
