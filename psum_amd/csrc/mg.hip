@@ -357,13 +357,43 @@ __device__ __forceinline__ void line_apply(const LineGeom& G, const T* __restric
     }
 }
 
-// Blocks [0, nbc) of the launch take chunks; the blocks after them take the
-// listed nodes, one per thread, with the per-node kernels' code.
+// A launch takes nbc blocks of chunks and nblk(n) blocks of listed nodes
+// (one per thread, with the per-node kernels' code).  Workgroup b runs on
+// XCD x = b % kXcds as that XCD's j = b / kXcds-th: j < lb takes listed block
+// x*lb + j, later ones chunk block x*cb + (j - lb) (lb, cb: the listed and
+// chunk blocks per XCD, rounded up; indexes past the end exit).  Every XCD
+// thus gets an eighth of the listed nodes, whose per-node path is slower,
+// started first, and a contiguous eighth of the chunks.
 struct LineList {
     const uint32_t* list;
     uint32_t n;
     uint32_t nbc;              // blocks of chunks
 };
+
+__host__ __device__ __forceinline__ uint32_t line_listed_blocks(const LineList& LL)
+{
+    return (LL.n + kMgThreads - 1) / kMgThreads;
+}
+__host__ __device__ __forceinline__ uint32_t line_grid(const LineList& LL)
+{
+    const uint32_t lb = (line_listed_blocks(LL) + kXcds - 1) / kXcds, cb = (LL.nbc + kXcds - 1) / kXcds;
+    return kXcds * (lb + cb);
+}
+// this workgroup's block: listed (true) or chunk, and its index (-1: none)
+__device__ __forceinline__ int64_t line_block(const LineList& LL, bool* listed)
+{
+    const uint32_t nbl = line_listed_blocks(LL);
+    const uint32_t lb = (nbl + kXcds - 1) / kXcds, cb = (LL.nbc + kXcds - 1) / kXcds;
+    const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    if (j < lb) {
+        *listed = true;
+        const uint32_t k = x * lb + j;
+        return k < nbl ? (int64_t)k : -1;
+    }
+    *listed = false;
+    const uint32_t k = x * cb + (j - lb);
+    return k < LL.nbc ? (int64_t)k : -1;
+}
 
 template <typename T, bool COMPACT, bool ACC>
 __global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, LineList LL, DevLevel L, DevTable<T> relax,
@@ -373,9 +403,11 @@ __global__ __launch_bounds__(kMgThreads) void k_smooth_line(LineGeom G, LineList
                                                             T* __restrict__ xout, T two3,
                                                             double* __restrict__ xacc)
 {
-    const int64_t blk = level_block<3>();
-    if (blk >= (int64_t)LL.nbc) {
-        const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
+    bool listed;
+    const int64_t blk = line_block(LL, &listed);
+    if (blk < 0) return;
+    if (listed) {
+        const int64_t t = blk * kMgThreads + threadIdx.x;
         if (t < (int64_t)LL.n) smooth_node<T, 3, COMPACT, ACC>(L, relax, xin, b, xout, two3, xacc, (int64_t)LL.list[t]);
         return;
     }
@@ -404,9 +436,11 @@ __global__ __launch_bounds__(kMgThreads) void k_residual_line(LineGeom G, LineLi
                                                               const T* __restrict__ x,
                                                               const T* __restrict__ b, T* __restrict__ r)
 {
-    const int64_t blk = level_block<3>();
-    if (blk >= (int64_t)LL.nbc) {
-        const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
+    bool listed;
+    const int64_t blk = line_block(LL, &listed);
+    if (blk < 0) return;
+    if (listed) {
+        const int64_t t = blk * kMgThreads + threadIdx.x;
         if (t < (int64_t)LL.n) residual_node<T, 3, COMPACT>(L, A, x, b, r, (int64_t)LL.list[t]);
         return;
     }
@@ -1444,9 +1478,11 @@ __device__ __forceinline__ void outer_node_f32(const OuterArgs& A, int64_t i)
 __global__ __launch_bounds__(kMgThreads) void k_outer_line(LineGeom G, LineList LL, OuterArgs A,
                                                            const double* __restrict__ row, float dinv_in)
 {
-    const int64_t blk = level_block<3>();
-    if (blk >= (int64_t)LL.nbc) {
-        const int64_t t = (blk - LL.nbc) * kMgThreads + threadIdx.x;
+    bool listed;
+    const int64_t blk = line_block(LL, &listed);
+    if (blk < 0) return;
+    if (listed) {
+        const int64_t t = blk * kMgThreads + threadIdx.x;
         if (t < (int64_t)LL.n) outer_node_f32(A, (int64_t)LL.list[t]);
         return;
     }
@@ -2809,7 +2845,7 @@ private:
             const LineList LL = line_list(l);
             const DevTable<T> tb = table<T>(l, true);
             const T* row = tb.coef + (int64_t)G.cls_in * 27;
-            const unsigned nb = LL.nbc + nblk(LL.n);
+            const unsigned nb = line_grid(LL);
 #define SLINE(CMP, ACC, A) hipLaunchKernelGGL((k_smooth_line<T, CMP, ACC>), dim3(nb), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, buf, two3, A)
             if (host_[l].compact) { if (acc) SLINE(true, true, acc); else SLINE(true, false, nullptr); }
             else { if (acc) SLINE(false, true, acc); else SLINE(false, false, nullptr); }
@@ -2833,8 +2869,8 @@ private:
             const LineList LL = line_list(l);
             const DevTable<T> tb = table<T>(l, false);
             const T* row = tb.coef + (int64_t)G.cls_in * 27;
-            if (host_[l].compact) hipLaunchKernelGGL((k_residual_line<T, true>), dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
-            else hipLaunchKernelGGL((k_residual_line<T, false>), dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
+            if (host_[l].compact) hipLaunchKernelGGL((k_residual_line<T, true>), dim3(line_grid(LL)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
+            else hipLaunchKernelGGL((k_residual_line<T, false>), dim3(line_grid(LL)), dim3(kMgThreads), 0, stream_, G, LL, L, tb, row, x, b, r);
             return;
         }
         if (host_[l].D == 2) {
@@ -3284,7 +3320,7 @@ private:
         if (!norms && emit_f32 && !rec_ && line_ && dev_[0].line_cls >= 0 && host_[0].compact) {
             const LineGeom G = line_geom(0);
             const LineList LL = line_list(0);
-            hipLaunchKernelGGL(k_outer_line, dim3(LL.nbc + nblk(LL.n)), dim3(kMgThreads), 0, stream_, G, LL, oa,
+            hipLaunchKernelGGL(k_outer_line, dim3(line_grid(LL)), dim3(kMgThreads), 0, stream_, G, LL, oa,
                                K.coef + (int64_t)G.cls_in * 27, host_dinv32(0, G.cls_in));
             MG_HIP(hipGetLastError());
             return;
