@@ -439,7 +439,8 @@ psum_status psum_stepper_set_interp_order(void* h, int order);
 /* Multigrid kernel-shape options that may change after creation, as
  * whitespace "key value" pairs: "fuse 0|1" (2D fused tile smoothing and
  * residual + restriction, default 1), "tail 0|1" (the coarse end of each
- * 2D V-cycle in one workgroup, default 0) and "line 0|1" (3D 7-point
+ * V-cycle, levels of at most 33^2 / 9^3 nodes down to a coarsest level of
+ * at most 10^2 / 5^3, in one workgroup; default 0) and "line 0|1" (3D 7-point
  * levels in chunks of four nodes per thread, 3D prolongation over planes,
  * default 1).  Results are bit-identical either way.  A batch instance's
  * change makes the batch capture its step graph again at its next run.  No

@@ -669,7 +669,7 @@ __global__ __launch_bounds__(kMgThreads) void k_restrict_plane(DevLevel F, DevLe
     xc[i] = first_sweep != (T)0 ? bv * (first_sweep * (T)1) : (T)0;
 }
 
-// ---- 2D V-cycle tail in one workgroup ------------------------------------
+// ---- V-cycle tail in one workgroup -----------------------------------------
 // The coarse end of the hierarchy (levels lt..Lc, at most kMaxTail) is
 // latency-bound: a handful of nodes per level, three launches per level.
 // k_vtail runs all of it in one 1024-thread workgroup.  It first stages in
@@ -679,37 +679,45 @@ __global__ __launch_bounds__(kMgThreads) void k_restrict_plane(DevLevel F, DevLe
 // to the coarsest level, its dense solve (the inverse is read from global
 // memory), then prolongation and post-sweeps back up to level lt, whose
 // iterate it writes back.  Each phase is the per-node arithmetic of
-// k_smooth, k_residual, k_restrict, k_coarse_solve and k_prolong_add in the
-// same order, so the result is bit-identical to the launch-per-phase path.
+// k_smooth, k_residual, k_restrict, k_coarse_solve and k_prolong_add (2D or
+// 3D) in the same order, so the result is bit-identical to the
+// launch-per-phase path.
 constexpr int kMaxTail = 6;
 constexpr int kTailThreads = 1024;
 // coarsest nodes: one compute unit reads the dense inverse (n^2 doubles)
-// from memory at a few tens of GB/s, so it must stay small
-constexpr int kTailCoarse = 100;
-// one compute unit runs the tail: 64 lanes of VALU per clock, so a level of
-// 64^2 nodes already costs what its own multi-block launches cost; the tail
-// takes levels of at most 33^2 nodes
-constexpr int kTailLevelNodes = 1089;
-constexpr int kTailRows = (kTailCoarse + kTailThreads / 64 - 1) / (kTailThreads / 64);   // rows per wave
+// from memory at a few tens of GB/s, so it must stay small (2D: 10^2, 3D:
+// 5^3 nodes)
+template <int D>
+constexpr int tail_coarse() { return D == 2 ? 100 : 125; }
+// one compute unit runs the tail: 64 lanes of VALU per clock, so a 2D level
+// of 64^2 nodes already costs what its own multi-block launches cost; the
+// tail takes 2D levels of at most 33^2 nodes and 3D levels of at most 9^3
+// (a 17^3 level's 27-point sweep takes ~20 us on one compute unit against
+// ~4.4 us as a launch of its own; levels 9^3 and 5^3 of the 3D box take
+// 52 us per V-cycle in the tail, 54 us in their own launches)
+template <int D>
+constexpr int tail_level_nodes() { return D == 2 ? 1089 : 729; }
+constexpr int kTailRows = (125 + kTailThreads / 64 - 1) / (kTailThreads / 64);   // rows per wave
 
 template <typename T>
 struct TailLevel {
-    int n0, n1, wrap0, wrap1, N, compact, ncls;
+    int n[3], wrap[3], N, compact, ncls;
     // byte offsets into the workgroup's LDS
     int ox, oy, ob, ocls, orx, oa, odinv;
     const uint16_t* cls;
-    const T* rx;          // relax rows, ncls x 9
-    const T* a;           // scaled operator rows, ncls x 9
+    const T* rx;          // relax rows, ncls x 3^D
+    const T* a;           // scaled operator rows, ncls x 3^D
     const T* dinv;        // ncls
 };
 
 template <typename T>
 struct TailTransfer {
-    int wrap0, wrap1;
-    // byte offsets: restriction (per coarse index) and prolongation (per fine index) tables
-    int orf0, orf1, orn0, orn1, orw0, orw1, opc0, opc1, opw0, opw1;
-    const int *rf0, *rf1, *rn0, *rn1, *pc0, *pc1;
-    const T *rw0, *rw1, *pw0, *pw1;
+    int wrap[3];
+    // byte offsets per axis: restriction (per coarse index) and prolongation
+    // (per fine index) tables
+    int orf[3], orn[3], orw[3], opc[3], opw[3];
+    const int *rf[3], *rn[3], *pc[3];
+    const T *rw[3], *pw[3];
 };
 
 template <typename T>
@@ -732,68 +740,98 @@ __device__ __forceinline__ void tail_copy(unsigned char* lds, int off, const U* 
     for (int i = threadIdx.x; i < n; i += kTailThreads) d[i] = g[i];
 }
 
-// sum_k coef[k] * x[nb_k] over the 3x3 offsets (row offset slowest), clamped
-// or wrapped neighbours; COMPACT: the 5 axis offsets (stencil_apply<T,2,.>)
-template <typename T, bool COMPACT>
-__device__ __forceinline__ T tail_stencil(const T* coef, const T* x, int c0, int c1, int n0, int n1, int w0,
-                                          int w1)
+// stencil_apply<T, D, COMPACT> on a tail level: sum_k coef[k] * x[nb_k] over
+// the 3^D offsets (first axis slowest), clamped or wrapped neighbours;
+// COMPACT: the 2D+1 axis offsets
+template <typename T, int D, bool COMPACT>
+__device__ __forceinline__ T tail_stencil(const T* coef, const T* x, const int* c, const TailLevel<T>& t)
 {
     T s = (T)0;
+    if constexpr (D == 2) {
 #pragma unroll
-    for (int a = -1; a <= 1; ++a)
+        for (int a = -1; a <= 1; ++a)
 #pragma unroll
-        for (int b = -1; b <= 1; ++b) {
-            if (COMPACT && a != 0 && b != 0) continue;
-            const int k = (a + 1) * 3 + (b + 1);
-            const int q0 = nb_index(c0, a, n0, w0);
-            const int q1 = nb_index(c1, b, n1, w1);
-            s = s + coef[k] * x[q0 * n1 + q1];
-        }
+            for (int b = -1; b <= 1; ++b) {
+                if (COMPACT && a != 0 && b != 0) continue;
+                const int k = (a + 1) * 3 + (b + 1);
+                const int q0 = nb_index(c[0], a, t.n[0], t.wrap[0]);
+                const int q1 = nb_index(c[1], b, t.n[1], t.wrap[1]);
+                s = s + coef[k] * x[q0 * t.n[1] + q1];
+            }
+    } else {
+#pragma unroll
+        for (int a = -1; a <= 1; ++a)
+#pragma unroll
+            for (int b = -1; b <= 1; ++b)
+#pragma unroll
+                for (int e = -1; e <= 1; ++e) {
+                    if (COMPACT && ((a != 0) + (b != 0) + (e != 0)) > 1) continue;
+                    const int k = (a + 1) * 9 + (b + 1) * 3 + (e + 1);
+                    const int q0 = nb_index(c[0], a, t.n[0], t.wrap[0]);
+                    const int q1 = nb_index(c[1], b, t.n[1], t.wrap[1]);
+                    const int q2 = nb_index(c[2], e, t.n[2], t.wrap[2]);
+                    s = s + coef[k] * x[(q0 * t.n[1] + q1) * t.n[2] + q2];
+                }
+    }
     return s;
 }
 
+template <int D, typename T>
+__device__ __forceinline__ void tail_coords(const TailLevel<T>& t, int i, int* c)
+{
+    if constexpr (D == 3) {
+        const int q = i / t.n[2];
+        c[2] = i - q * t.n[2];
+        c[1] = q % t.n[1];
+        c[0] = q / t.n[1];
+    } else {
+        c[0] = i / t.n[1];
+        c[1] = i - c[0] * t.n[1];
+    }
+}
+
 // out = stencil(rows, x) * m + b * f for every node of level t
-template <typename T>
+template <typename T, int D>
 __device__ __forceinline__ void tail_apply(unsigned char* lds, const TailLevel<T>& t, int orows, const T* x,
                                            T* out, T m, T f)
 {
+    constexpr int K = D == 2 ? 9 : 27;
     const uint16_t* cls = reinterpret_cast<const uint16_t*>(lds + t.ocls);
     const T* rows = reinterpret_cast<const T*>(lds + orows);
     const T* b = reinterpret_cast<const T*>(lds + t.ob);
     for (int i = threadIdx.x; i < t.N; i += kTailThreads) {
-        const int c0 = i / t.n1, c1 = i - c0 * t.n1;
-        const T* coef = rows + (int)cls[i] * 9;
-        const T v = t.compact ? tail_stencil<T, true>(coef, x, c0, c1, t.n0, t.n1, t.wrap0, t.wrap1)
-                              : tail_stencil<T, false>(coef, x, c0, c1, t.n0, t.n1, t.wrap0, t.wrap1);
+        int c[3];
+        tail_coords<D>(t, i, c);
+        const T* coef = rows + (int)cls[i] * K;
+        const T v = t.compact ? tail_stencil<T, D, true>(coef, x, c, t) : tail_stencil<T, D, false>(coef, x, c, t);
         out[i] = v * m + b[i] * f;
     }
 }
 
-template <typename T>
+template <typename T, int D>
 __global__ __launch_bounds__(kTailThreads) void k_vtail(TailArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
+    constexpr int K = D == 2 ? 9 : 27;
     const int nl = a.nl, Lc = a.nl - 1;
     // stage tables, level lt's iterate and right-hand side
     for (int l = 0; l < nl; ++l) {
         const TailLevel<T>& t = a.lev[l];
         tail_copy<uint16_t>(lds, t.ocls, t.cls, t.N);
-        tail_copy<T>(lds, t.orx, t.rx, t.ncls * 9);
-        tail_copy<T>(lds, t.oa, t.a, t.ncls * 9);
+        tail_copy<T>(lds, t.orx, t.rx, t.ncls * K);
+        tail_copy<T>(lds, t.oa, t.a, t.ncls * K);
         tail_copy<T>(lds, t.odinv, t.dinv, t.ncls);
         if (l < Lc) {
             const TailTransfer<T>& r = a.tr[l];
             const TailLevel<T>& c = a.lev[l + 1];
-            tail_copy<int>(lds, r.orf0, r.rf0, c.n0);
-            tail_copy<int>(lds, r.orf1, r.rf1, c.n1);
-            tail_copy<int>(lds, r.orn0, r.rn0, c.n0);
-            tail_copy<int>(lds, r.orn1, r.rn1, c.n1);
-            tail_copy<T>(lds, r.orw0, r.rw0, 3 * c.n0);
-            tail_copy<T>(lds, r.orw1, r.rw1, 3 * c.n1);
-            tail_copy<int>(lds, r.opc0, r.pc0, t.n0);
-            tail_copy<int>(lds, r.opc1, r.pc1, t.n1);
-            tail_copy<T>(lds, r.opw0, r.pw0, 2 * t.n0);
-            tail_copy<T>(lds, r.opw1, r.pw1, 2 * t.n1);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                tail_copy<int>(lds, r.orf[d], r.rf[d], c.n[d]);
+                tail_copy<int>(lds, r.orn[d], r.rn[d], c.n[d]);
+                tail_copy<T>(lds, r.orw[d], r.rw[d], 3 * c.n[d]);
+                tail_copy<int>(lds, r.opc[d], r.pc[d], t.n[d]);
+                tail_copy<T>(lds, r.opw[d], r.pw[d], 2 * t.n[d]);
+            }
         }
     }
     tail_copy<T>(lds, a.lev[0].ox, a.x_in, a.lev[0].N);
@@ -812,40 +850,60 @@ __global__ __launch_bounds__(kTailThreads) void k_vtail(TailArgs<T> a)
         // coarse levels received their first sweep from the restriction
         const int sweeps = a.pre > 0 ? a.pre - 1 : 0;
         for (int s = 0; s < sweeps; ++s) {
-            tail_apply<T>(lds, t, t.orx, X(l), Y(l), (T)1, a.two3 * (T)1);
+            tail_apply<T, D>(lds, t, t.orx, X(l), Y(l), (T)1, a.two3 * (T)1);
             __syncthreads();
             cur ^= 1u << l;
         }
         // r = (A x)*(-1) + b*1 into the free iterate
         T* r = Y(l);
-        tail_apply<T>(lds, t, t.oa, X(l), r, (T)-1, (T)1);
+        tail_apply<T, D>(lds, t, t.oa, X(l), r, (T)-1, (T)1);
         __syncthreads();
         // b_c = (R r) o D_c^-1 ; x_c = b_c * (2/3) (first sweep) or 0
         const TailLevel<T>& cl = a.lev[l + 1];
         const TailTransfer<T>& tr = a.tr[l];
-        const int* rf0 = reinterpret_cast<const int*>(lds + tr.orf0);
-        const int* rf1 = reinterpret_cast<const int*>(lds + tr.orf1);
-        const int* rn0 = reinterpret_cast<const int*>(lds + tr.orn0);
-        const int* rn1 = reinterpret_cast<const int*>(lds + tr.orn1);
-        const T* rw0 = reinterpret_cast<const T*>(lds + tr.orw0);
-        const T* rw1 = reinterpret_cast<const T*>(lds + tr.orw1);
+        const int *rf[3], *rn[3];
+        const T* rw[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            rf[d] = reinterpret_cast<const int*>(lds + tr.orf[d]);
+            rn[d] = reinterpret_cast<const int*>(lds + tr.orn[d]);
+            rw[d] = reinterpret_cast<const T*>(lds + tr.orw[d]);
+        }
         const uint16_t* ccls = reinterpret_cast<const uint16_t*>(lds + cl.ocls);
         const T* cdinv = reinterpret_cast<const T*>(lds + cl.odinv);
         const T fs = (l + 1 < Lc && a.pre > 0) ? a.two3 : (T)0;
         T* bc = reinterpret_cast<T*>(lds + cl.ob);
         T* xc = reinterpret_cast<T*>(lds + cl.ox);
         for (int i = threadIdx.x; i < cl.N; i += kTailThreads) {
-            const int c0 = i / cl.n1, c1 = i - c0 * cl.n1;
+            int c[3];
+            tail_coords<D>(cl, i, c);
             T sum = (T)0;
-            const int f0 = rf0[c0], m0 = rn0[c0];
-            const int f1 = rf1[c1], m1 = rn1[c1];
-            for (int p = 0; p < m0; ++p) {
-                const T w0 = rw0[3 * c0 + p];
-                const int i0 = wrap_index(f0 + p, t.n0, tr.wrap0);
-                for (int q = 0; q < m1; ++q) {
-                    const T w = w0 * rw1[3 * c1 + q];
-                    const int i1 = wrap_index(f1 + q, t.n1, tr.wrap1);
-                    sum = sum + w * r[i0 * t.n1 + i1];
+            const int f0 = rf[0][c[0]], m0 = rn[0][c[0]];
+            const int f1 = rf[1][c[1]], m1 = rn[1][c[1]];
+            if constexpr (D == 2) {
+                for (int p = 0; p < m0; ++p) {
+                    const T w0 = rw[0][3 * c[0] + p];
+                    const int i0 = wrap_index(f0 + p, t.n[0], tr.wrap[0]);
+                    for (int q = 0; q < m1; ++q) {
+                        const T w = w0 * rw[1][3 * c[1] + q];
+                        const int i1 = wrap_index(f1 + q, t.n[1], tr.wrap[1]);
+                        sum = sum + w * r[i0 * t.n[1] + i1];
+                    }
+                }
+            } else {
+                const int f2 = rf[2][c[2]], m2 = rn[2][c[2]];
+                for (int p = 0; p < m0; ++p) {
+                    const T w0 = rw[0][3 * c[0] + p];
+                    const int i0 = wrap_index(f0 + p, t.n[0], tr.wrap[0]);
+                    for (int q = 0; q < m1; ++q) {
+                        const T w01 = w0 * rw[1][3 * c[1] + q];
+                        const int i1 = wrap_index(f1 + q, t.n[1], tr.wrap[1]);
+                        const int row = (i0 * t.n[1] + i1) * t.n[2];
+                        for (int e = 0; e < m2; ++e) {
+                            const T w = w01 * rw[2][3 * c[2] + e];
+                            sum = sum + w * r[row + wrap_index(f2 + e, t.n[2], tr.wrap[2])];
+                        }
+                    }
                 }
             }
             const T bv = sum * cdinv[ccls[i]];
@@ -888,33 +946,56 @@ __global__ __launch_bounds__(kTailThreads) void k_vtail(TailArgs<T> a)
         const TailLevel<T>& t = a.lev[l];
         const TailLevel<T>& cl = a.lev[l + 1];
         const TailTransfer<T>& tr = a.tr[l];
-        const int* pc0 = reinterpret_cast<const int*>(lds + tr.opc0);
-        const int* pc1 = reinterpret_cast<const int*>(lds + tr.opc1);
-        const T* pw0 = reinterpret_cast<const T*>(lds + tr.opw0);
-        const T* pw1 = reinterpret_cast<const T*>(lds + tr.opw1);
+        const int* pc[3];
+        const T* pw[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            pc[d] = reinterpret_cast<const int*>(lds + tr.opc[d]);
+            pw[d] = reinterpret_cast<const T*>(lds + tr.opw[d]);
+        }
         const T* xc = X(l + 1);
         T* xf = X(l);
         // x_f = (P x_c)*1 + x_f*(1*1)
         for (int i = threadIdx.x; i < t.N; i += kTailThreads) {
-            const int c0 = i / t.n1, c1 = i - c0 * t.n1;
+            int c[3];
+            tail_coords<D>(t, i, c);
             T sum = (T)0;
-            const int p0 = pc0[c0], p1 = pc1[c1];
-            const T* w0 = pw0 + 2 * c0;
-            const T* w1 = pw1 + 2 * c1;
-            for (int p = 0; p < 2; ++p) {
-                if (w0[p] == (T)0) continue;
-                const int i0 = wrap_index(p0 + p, cl.n0, tr.wrap0);
-                for (int q = 0; q < 2; ++q) {
-                    if (w1[q] == (T)0) continue;
-                    const int i1 = wrap_index(p1 + q, cl.n1, tr.wrap1);
-                    sum = sum + (w0[p] * w1[q]) * xc[i0 * cl.n1 + i1];
+            const int p0 = pc[0][c[0]], p1 = pc[1][c[1]];
+            const T* w0 = pw[0] + 2 * c[0];
+            const T* w1 = pw[1] + 2 * c[1];
+            if constexpr (D == 2) {
+                for (int p = 0; p < 2; ++p) {
+                    if (w0[p] == (T)0) continue;
+                    const int i0 = wrap_index(p0 + p, cl.n[0], tr.wrap[0]);
+                    for (int q = 0; q < 2; ++q) {
+                        if (w1[q] == (T)0) continue;
+                        const int i1 = wrap_index(p1 + q, cl.n[1], tr.wrap[1]);
+                        sum = sum + (w0[p] * w1[q]) * xc[i0 * cl.n[1] + i1];
+                    }
+                }
+            } else {
+                const int p2 = pc[2][c[2]];
+                const T* w2 = pw[2] + 2 * c[2];
+                for (int p = 0; p < 2; ++p) {
+                    if (w0[p] == (T)0) continue;
+                    const int i0 = wrap_index(p0 + p, cl.n[0], tr.wrap[0]);
+                    for (int q = 0; q < 2; ++q) {
+                        if (w1[q] == (T)0) continue;
+                        const T w01 = w0[p] * w1[q];
+                        const int i1 = wrap_index(p1 + q, cl.n[1], tr.wrap[1]);
+                        for (int e = 0; e < 2; ++e) {
+                            if (w2[e] == (T)0) continue;
+                            const int i2 = wrap_index(p2 + e, cl.n[2], tr.wrap[2]);
+                            sum = sum + (w01 * w2[e]) * xc[(i0 * cl.n[1] + i1) * cl.n[2] + i2];
+                        }
+                    }
                 }
             }
             xf[i] = sum * (T)1 + xf[i] * (T)1;
         }
         __syncthreads();
         for (int s = 0; s < a.post; ++s) {
-            tail_apply<T>(lds, t, t.orx, X(l), Y(l), (T)1, a.two3 * (T)1);
+            tail_apply<T, D>(lds, t, t.orx, X(l), Y(l), (T)1, a.two3 * (T)1);
             __syncthreads();
             cur ^= 1u << l;
         }
@@ -2151,12 +2232,12 @@ struct DevTransferBuf {
 
 namespace psum_hip {
 
-// k_vtail<T>'s dynamic LDS limit on the current device: raised once per
-// device and precision to the device's per-workgroup maximum (the
+// k_vtail<T, D>'s dynamic LDS limit on the current device: raised once per
+// device, precision and dimension to the device's per-workgroup maximum (the
 // attribute is process-wide, so every solver shares the one largest
 // value), and returned.  Called when a hierarchy is built, outside any
 // stream capture.
-template <typename T>
+template <typename T, int D>
 int vtail_lds_limit()
 {
     static std::mutex mu;
@@ -2169,7 +2250,7 @@ int vtail_lds_limit()
     int lim = 0;
     MG_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     if (lim > 65536 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vtail<T>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vtail<T, D>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
         (void)hipGetLastError();
         lim = 65536;
@@ -2593,8 +2674,9 @@ private:
     int I_ = 0, J_ = 0, K_ = 0, depth_ = 3, pre_ = 2, post_ = 1, max_cycles_ = 10, corr_ = 4;
     int fuse_ = 1;   // 2D: fused multi-sweep tile smoothing and residual+restriction
     int line_ = 1;   // 3D 7-point levels: chunk kernels (k_smooth_line, k_residual_line)
-    // 2D: the coarse end of the V-cycle in one workgroup (k_vtail); off unless
-    // "tail 1" is set
+    // the coarse end of the V-cycle in one workgroup (k_vtail); off unless
+    // "tail 1" is set (measured: the levels' own launches are about as fast,
+    // in 2D and in 3D)
     int tail_ = 0;
     int tail32_ = -2, tail64_ = -2;    // first tail level per precision (-1 none, -2 not yet chosen)
     size_t tail_bytes32_ = 0, tail_bytes64_ = 0;
@@ -2960,10 +3042,11 @@ private:
     }
 
     // First level of the one-workgroup tail for precision T, or -1: a 2D
-    // fused hierarchy whose levels lt..Lc (two or more, at most kMaxTail,
-    // lt >= 1, at most kTailLevelNodes nodes each) keep their iterates and
-    // right-hand sides in LDS and whose coarsest level has at most kTailCoarse
-    // nodes (its dense solve reads the inverse from one compute unit).
+    // fused or a 3D hierarchy whose levels lt..Lc (two or more, at most
+    // kMaxTail, lt >= 1, at most tail_level_nodes<D> nodes each) keep their
+    // iterates and right-hand sides in LDS and whose coarsest level has at
+    // most tail_coarse<D> nodes (its dense solve reads the inverse from one
+    // compute unit).
     // Chosen by resolve_tail() when the hierarchy is built or the option
     // changes, never while a stream is being captured.
     template <typename T>
@@ -2982,12 +3065,18 @@ private:
         c = -1;
         bytes = 0;
         const int nl = (int)host_.size(), Lc = nl - 1;
-        if (!tail_ || !fuse_ || nl < 3 || host_[Lc].N > kTailCoarse) return;
+        if (nl < 3) return;
+        const int D = host_[0].D;
+        if (D != 2 && D != 3) return;
         for (const auto& h : host_)
-            if (h.D != 2) return;
-        const int lim = vtail_lds_limit<T>();
+            if (h.D != D) return;
+        if (tail_ != 1 || (D == 2 && !fuse_)) return;
+        const int64_t coarse = D == 2 ? tail_coarse<2>() : tail_coarse<3>();
+        const int64_t maxn = D == 2 ? tail_level_nodes<2>() : tail_level_nodes<3>();
+        if (host_[Lc].N > coarse) return;
+        const int lim = D == 2 ? vtail_lds_limit<T, 2>() : vtail_lds_limit<T, 3>();
         for (int lt = 1; lt <= Lc - 1; ++lt) {
-            if (Lc - lt + 1 > kMaxTail || host_[lt].N > kTailLevelNodes) continue;
+            if (Lc - lt + 1 > kMaxTail || host_[lt].N > maxn) continue;
             TailArgs<T> probe{};
             const size_t b = tail_layout<T>(lt, probe);
             if (b > (size_t)lim) continue;
@@ -3013,6 +3102,8 @@ private:
     size_t tail_layout(int lt, TailArgs<T>& a) const
     {
         const int Lc = (int)host_.size() - 1;
+        const int D = host_[lt].D;
+        const int K = D == 2 ? 9 : 27;
         size_t off = 0;
         auto take = [&](size_t nbytes) {
             const size_t o = off;
@@ -3024,10 +3115,10 @@ private:
             const int l = lt + k;
             const HostLevel& h = host_[l];
             TailLevel<T>& t = a.lev[k];
-            t.n0 = h.n[0];
-            t.n1 = h.n[1];
-            t.wrap0 = h.wrap[0];
-            t.wrap1 = h.wrap[1];
+            for (int d = 0; d < 3; ++d) {
+                t.n[d] = d < D ? (int)h.n[d] : 1;
+                t.wrap[d] = d < D ? h.wrap[d] : 0;
+            }
             t.N = (int)h.N;
             t.compact = h.compact ? 1 : 0;
             t.ncls = (int)h.dinv.size();
@@ -3035,8 +3126,8 @@ private:
             t.oy = l < Lc ? take((size_t)h.N * sizeof(T)) : t.ox;
             t.ob = take((size_t)h.N * sizeof(T));
             t.ocls = take((size_t)h.N * sizeof(uint16_t));
-            t.orx = take((size_t)t.ncls * 9 * sizeof(T));
-            t.oa = take((size_t)t.ncls * 9 * sizeof(T));
+            t.orx = take((size_t)t.ncls * K * sizeof(T));
+            t.oa = take((size_t)t.ncls * K * sizeof(T));
             t.odinv = take((size_t)t.ncls * sizeof(T));
             if (!dev_.empty()) {
                 const DevTable<T> rx = table<T>(l, true), A = table<T>(l, false);
@@ -3050,30 +3141,21 @@ private:
             const int l = lt + k;
             const HostLevel &f = host_[l], &c = host_[l + 1];
             TailTransfer<T>& r = a.tr[k];
-            r.wrap0 = tr_host_[l][0].wrap;
-            r.wrap1 = tr_host_[l][1].wrap;
-            r.orf0 = take((size_t)c.n[0] * sizeof(int));
-            r.orf1 = take((size_t)c.n[1] * sizeof(int));
-            r.orn0 = take((size_t)c.n[0] * sizeof(int));
-            r.orn1 = take((size_t)c.n[1] * sizeof(int));
-            r.orw0 = take((size_t)3 * c.n[0] * sizeof(T));
-            r.orw1 = take((size_t)3 * c.n[1] * sizeof(T));
-            r.opc0 = take((size_t)f.n[0] * sizeof(int));
-            r.opc1 = take((size_t)f.n[1] * sizeof(int));
-            r.opw0 = take((size_t)2 * f.n[0] * sizeof(T));
-            r.opw1 = take((size_t)2 * f.n[1] * sizeof(T));
-            if (!dtr_.empty()) {
-                const DevTransfer<T> d = transfer<T>(l);
-                r.rf0 = d.rf[0];
-                r.rf1 = d.rf[1];
-                r.rn0 = d.rn[0];
-                r.rn1 = d.rn[1];
-                r.rw0 = d.rw[0];
-                r.rw1 = d.rw[1];
-                r.pc0 = d.pc[0];
-                r.pc1 = d.pc[1];
-                r.pw0 = d.pw[0];
-                r.pw1 = d.pw[1];
+            const DevTransfer<T> dt = dtr_.empty() ? DevTransfer<T>{} : transfer<T>(l);
+            for (int d = 0; d < D; ++d) {
+                r.wrap[d] = tr_host_[l][d].wrap;
+                r.orf[d] = take((size_t)c.n[d] * sizeof(int));
+                r.orn[d] = take((size_t)c.n[d] * sizeof(int));
+                r.orw[d] = take((size_t)3 * c.n[d] * sizeof(T));
+                r.opc[d] = take((size_t)f.n[d] * sizeof(int));
+                r.opw[d] = take((size_t)2 * f.n[d] * sizeof(T));
+                if (!dtr_.empty()) {
+                    r.rf[d] = dt.rf[d];
+                    r.rn[d] = dt.rn[d];
+                    r.rw[d] = dt.rw[d];
+                    r.pc[d] = dt.pc[d];
+                    r.pw[d] = dt.pw[d];
+                }
             }
         }
         return off;
@@ -3091,10 +3173,13 @@ private:
         a.b_in = b;
         a.x_out = x;
         a.cinv = cinv_;
+        const int D = host_[lt].D;
+        const int lim = D == 2 ? vtail_lds_limit<T, 2>() : vtail_lds_limit<T, 3>();
         const size_t chosen = std::is_same<T, double>::value ? tail_bytes64_ : tail_bytes32_;
-        if (bytes != chosen || bytes > (size_t)vtail_lds_limit<T>())
+        if (bytes != chosen || bytes > (size_t)lim)
             throw std::logic_error("hip_multigrid: V-cycle tail LDS size differs from the one resolved");
-        hipLaunchKernelGGL((k_vtail<T>), dim3(1), dim3(kTailThreads), bytes, stream_, a);
+        if (D == 2) hipLaunchKernelGGL((k_vtail<T, 2>), dim3(1), dim3(kTailThreads), bytes, stream_, a);
+        else hipLaunchKernelGGL((k_vtail<T, 3>), dim3(1), dim3(kTailThreads), bytes, stream_, a);
         MG_HIP(hipGetLastError());
     }
 

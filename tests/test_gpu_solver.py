@@ -300,6 +300,34 @@ def test_vcycle_tail_is_bit_identical(backend, case, prepost):
 
 
 @pytest.mark.parametrize("backend", ["hip_multigrid_gpu", "hip_multigrid32f_gpu"])
+@pytest.mark.parametrize("case", [(33, 4), (37, 4), (65, 5), (17, 3)])
+@pytest.mark.parametrize("prepost", [(3, 3), (2, 1), (0, 2)])
+def test_vcycle_tail_3d_is_bit_identical(backend, case, prepost):
+    """3D walls: the one-workgroup tail ("tail 1") runs levels of at most 9^3 nodes down
+    to a coarsest level of at most 5^3 (33^3 depth 4 and 65^3 depth 5: 9, 5;
+    37^3 depth 4: 10^3 is too large, so no tail) with the 27-point rows, 3D
+    restriction, dense solve and prolongation of the per-level kernels:
+    equal bits and cycle counts with the tail switched off."""
+    E = _E()
+    n, depth = case
+    og, nn, r, c, v, b, fixed = _wall_problem_3d(n, seed=11)
+    pre, post = prepost
+    out = []
+    for tail in (0, 1):
+        s = E.SolverBackend(backend)
+        s.set_options(f"I {n} J {n} K {n} depth {depth} pre_relax {pre} post_relax {post} "
+                      f"max_cycles 30 tol 1e-9 tail {tail}")
+        s.set_matrix(nn, nn, r, c, v)
+        x = torch.zeros(nn, dtype=torch.float64, device="cuda")
+        s.solve(_dev(b), x)
+        s.solve(_dev(b * 1.01), x)
+        out.append((x.cpu().numpy(), s.stats()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1][0] == out[1][1][0]
+    assert np.array_equal(np.float64(out[0][1][1]), np.float64(out[1][1][1]), equal_nan=True)
+
+
+@pytest.mark.parametrize("backend", ["hip_multigrid_gpu", "hip_multigrid32f_gpu"])
 @pytest.mark.parametrize("n", [49, 68, 97])
 @pytest.mark.parametrize("prepost", [(3, 3), (2, 1)])
 def test_chunked_3d_kernels_are_bit_identical(backend, n, prepost):
