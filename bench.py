@@ -592,7 +592,7 @@ def main():
     # dominant kernel: the fused 3D gather + push + deposit; algorithmic bytes
     # per launch = particles per launch * 2 * (2D) * 4 B (read + write FP32
     # position and velocity once)
-    kern = "k_push_deposit<float,3,true,true,true>"
+    kern = "k_push_deposit<float,3,true,true,2>"
     bytes_per_launch = c3["particles_per_launch"] * 2 * (2 * 3) * 4
     achieved = bytes_per_launch / (c3["push_ms_avg"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -18,6 +18,7 @@ struct DevGrid {
     double lb[3], ub[3], delta[3], delta_r[3];
     int periodic[3];
     double plo[3], phi[3], plen[3];   // periodic particle domain [plo, phi), phi = plo + plen
+    int origin0;         // every lb is 0 and every ub exact in float
 };
 
 inline DevGrid to_dev(const psum_grid& g)
@@ -35,6 +36,9 @@ inline DevGrid to_dev(const psum_grid& g)
         d.phi[i] = g.plo[i] + g.plen[i];
         d.plen[i] = g.plen[i];
     }
+    d.origin0 = 1;
+    for (int i = 0; i < g.dim; ++i)
+        if (!(g.lb[i] == 0.0 && (double)(float)g.ub[i] == g.ub[i])) d.origin0 = 0;
     return d;
 }
 

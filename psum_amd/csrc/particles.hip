@@ -407,16 +407,20 @@ __device__ __forceinline__ void unit_grad(const T (*W)[2], const T* ph, T* grad)
     }
 }
 
+// Grid kinds the push is compiled for: any spacing (0), unit spacing (1),
+// and unit spacing from the origin 0 with upper bounds exact in float and
+// FP32 positions (2, DevGrid::origin0).
+constexpr int kGridAny = 0, kGridUnit = 1, kGridUnit0 = 2;
+
 // Cell index and offsets of x as base_cell<D, UNIT> takes them in FP64,
-// rounded to T.  UNIT (unit spacing, origin 0, upper bounds exact in float)
-// with T = float takes them in FP32 with the same bits: x - lb is x, floor
-// and the range test are exact, and x - ci is exact (Sterbenz for ci >= 1
-// and ci <= -2; ci = 0 gives x) except for ci = -1 and |x| < 2^-29, where
-// both round 1 - |x| to 1.0f.
-template <typename T, int D, bool UNIT>
+// rounded to T.  kGridUnit0 takes them in FP32 with the same bits: x - lb
+// is x, floor and the range test are exact, and x - ci is exact (Sterbenz
+// for ci >= 1 and ci <= -2; ci = 0 gives x) except for ci = -1 and
+// |x| < 2^-29, where both round 1 - |x| to 1.0f.
+template <typename T, int D, int UNIT>
 __device__ __forceinline__ bool cell_of(const DevGrid& g, const T* x, int* cw, T* s)
 {
-    if constexpr (UNIT && sizeof(T) == 4) {
+    if constexpr (UNIT == kGridUnit0 && sizeof(T) == 4) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const float f = floorf(x[d]);
@@ -427,22 +431,21 @@ __device__ __forceinline__ bool cell_of(const DevGrid& g, const T* x, int* cw, T
             cw[d] = ci < 0 ? ci + w : (ci >= w ? ci - w : ci);
         }
         return true;
-    } else {
-        double xd[D], sd[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
-        if (!base_cell<D, UNIT>(g, xd, cw, sd)) return false;
-#pragma unroll
-        for (int d = 0; d < D; ++d) s[d] = (T)sd[d];
-        return true;
     }
+    double xd[D], sd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xd[d] = (double)x[d];
+    if (!base_cell<D, (UNIT > kGridAny)>(g, xd, cw, sd)) return false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) s[d] = (T)sd[d];
+    return true;
 }
 
 // two consecutive values at the alignment of one
 template <typename T>
 using pair_t = T __attribute__((ext_vector_type(2), aligned(sizeof(T))));
 
-template <typename T, int D, bool PUSH, bool UNIT>
+template <typename T, int D, bool PUSH, int UNIT>
 __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__ phi, T qm, T dt,
                                          double mass, T* x, T* v, double& ke, double* pm)
 {
@@ -486,7 +489,7 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
 #pragma unroll
             for (int i = 0; i < (1 << D); ++i) ph[i] = phi[sidx[i]];
         }
-        if constexpr (UNIT) {
+        if constexpr (UNIT > kGridAny) {
             unit_grad<T, D>(W, ph, grad);
         } else {
 #pragma unroll
@@ -524,7 +527,7 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
 #pragma unroll
     for (int d = 0; d < D; ++d)
         if (!g.periodic[d]) {
-            if constexpr (UNIT && sizeof(T) == 4) {
+            if constexpr (UNIT == kGridUnit0 && sizeof(T) == 4) {
                 // lb = 0 and ub exact in float: the same test in FP32
                 if (!(x[d] >= 0.0f && x[d] < (float)g.ub[d])) alive = false;
             } else {
@@ -541,7 +544,7 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
 template <int D>
 constexpr int push_min_waves() { return D == 2 ? 5 : 1; }
 
-template <typename T, int D, bool PUSH, bool DEPOSIT, bool UNIT = false>
+template <typename T, int D, bool PUSH, bool DEPOSIT, int UNIT = kGridAny>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_min_waves<D>(), 8))) void k_push_deposit(
     DevGrid g, T* __restrict__ x0, T* __restrict__ x1, T* __restrict__ x2, T* __restrict__ v0,
     T* __restrict__ v1, T* __restrict__ v2, int64_t n, const T* __restrict__ phi, T qm, T dt,
@@ -1091,25 +1094,26 @@ psum_status launch_push(const psum_grid* g, void* const* pos, void* const* vel, 
     auto* a = reinterpret_cast<unsigned long long*>(acc);
     auto* c = reinterpret_cast<unsigned long long*>(counters);
     const T* ph = static_cast<const T*>(phi);
-    // unit node spacing from the origin 0 (upper bounds exact in float): the
-    // gather's gradient weights are +-products of W, the cell index needs no
-    // products with the spacing, and FP32 positions take it in FP32
+    // unit node spacing: the gather's gradient weights are +-products of W
+    // and the cell index needs no products with the spacing; from the origin
+    // 0, FP32 positions take it in FP32
     bool unit = true;
-    for (int d = 0; d < g->dim; ++d)
-        unit = unit && g->delta_r[d] == 1.0 && g->delta[d] == 1.0 && g->lb[d] == 0.0 &&
-               (double)(float)g->ub[d] == g->ub[d];
+    for (int d = 0; d < g->dim; ++d) unit = unit && g->delta_r[d] == 1.0 && g->delta[d] == 1.0;
+    const bool unit0 = unit && dg.origin0 && sizeof(T) == 4;
 #define PSUM_LAUNCH(DIM, PU, DE, UN)                                                          \
     hipLaunchKernelGGL((k_push_deposit<T, DIM, PU, DE, UN>), dim3(nb), dim3(kThreads), 0, st, \
                        dg, p[0], p[1], p[2], v[0], v[1], v[2], n, ph, (T)qm, (T)dt, mass,      \
                        w_dep, scale, a, diag, c)
 #define PSUM_DISPATCH(DIM)                                                                    \
     if (push && acc) {                                                                        \
-        if (unit) PSUM_LAUNCH(DIM, true, true, true);                                         \
-        else PSUM_LAUNCH(DIM, true, true, false);                                             \
+        if (unit0) PSUM_LAUNCH(DIM, true, true, kGridUnit0);                                  \
+        else if (unit) PSUM_LAUNCH(DIM, true, true, kGridUnit);                               \
+        else PSUM_LAUNCH(DIM, true, true, kGridAny);                                          \
     } else if (push) {                                                                        \
-        if (unit) PSUM_LAUNCH(DIM, true, false, true);                                        \
-        else PSUM_LAUNCH(DIM, true, false, false);                                            \
-    } else PSUM_LAUNCH(DIM, false, true, false);
+        if (unit0) PSUM_LAUNCH(DIM, true, false, kGridUnit0);                                 \
+        else if (unit) PSUM_LAUNCH(DIM, true, false, kGridUnit);                              \
+        else PSUM_LAUNCH(DIM, true, false, kGridAny);                                         \
+    } else PSUM_LAUNCH(DIM, false, true, kGridAny);
     switch (g->dim) {
     case 1: PSUM_DISPATCH(1); break;
     case 2: PSUM_DISPATCH(2); break;
