@@ -421,16 +421,21 @@ template <typename T, int D, int UNIT>
 __device__ __forceinline__ bool cell_of(const DevGrid& g, const T* x, int* cw, T* s)
 {
     if constexpr (UNIT == kGridUnit0 && sizeof(T) == 4) {
+        // every axis in turn, one validity test at the end (an axis out of
+        // range takes cell 0; the caller uses no output of an invalid x)
+        bool ok = true;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const float f = floorf(x[d]);
             const int w = g.w[d];
-            if (!(f >= (float)(-w) && f <= (float)(2 * w - 2))) return false;
-            const int ci = (int)f;
-            s[d] = x[d] - (float)ci;
+            const bool in = f >= (float)(-w) && f <= (float)(2 * w - 2);
+            ok = ok && in;
+            const float fc = in ? f : 0.0f;
+            const int ci = (int)fc;
+            s[d] = x[d] - fc;
             cw[d] = ci < 0 ? ci + w : (ci >= w ? ci - w : ci);
         }
-        return true;
+        return ok;
     }
     double xd[D], sd[D];
 #pragma unroll
@@ -476,9 +481,10 @@ __device__ __forceinline__ bool push_one(const DevGrid& g, const T* __restrict__
         uint32_t sidx[1 << D];
         support_indices<D>(g, c, sidx);
         T ph[1 << D];
-        if (c[D - 1] + 1 != g.w[D - 1]) {
+        if (__all(c[D - 1] + 1 != g.w[D - 1])) {
             // support nodes i and i + 1 differ in the fastest axis only and
             // are neighbours in memory unless that axis wraps: paired loads
+            // (a wave with a wrapping particle loads them one by one)
 #pragma unroll
             for (int i = 0; i < (1 << D); i += 2) {
                 const pair_t<T> q = *reinterpret_cast<const pair_t<T>*>(phi + sidx[i]);
