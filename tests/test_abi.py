@@ -60,6 +60,43 @@ def test_argument_errors_do_not_touch_the_gpu():
     acc = (C.c_int64 * 25)()
     assert L.psum_hip_deposit_cic(C.byref(g2), 0, None, 0, 1.0, 3.0, acc, None, None) == 1
     assert b"power of two" in L.psum_hip_last_error()
+    # per-species sort intervals: a null stepper is refused before any GPU work
+    every = C.c_int(-7)
+    assert L.psum_stepper_set_species_sort(None, 0, 5) == 1
+    assert L.psum_stepper_species_sort(None, 0, C.byref(every)) == 1
+    assert every.value == -7
+
+
+def test_bench_window_covers_whole_sort_periods():
+    """bench.py times windows that start on a step where every species sorts:
+    the period is the least common multiple of the per-species intervals
+    (zero intervals never sort), and the untimed steps before the window
+    bring the step counter to a multiple of it."""
+    import importlib.util, os
+    spec = importlib.util.spec_from_file_location(
+        "bench_mod", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench._sort_period([20, 200]) == 200
+    assert bench._sort_period([20, 50]) == 100
+    assert bench._sort_period([0, 20]) == 20
+    assert bench._sort_period([0, 0]) == 0
+
+    class FakeStepper:
+        def __init__(self, steps):
+            self.steps, self.ran = steps, []
+
+        def stats(self):
+            return {"steps": self.steps}
+
+        def run(self, n, history=True):
+            self.ran.append(n)
+            self.steps += n
+
+    st = FakeStepper(41)
+    assert bench._align_to_sort(st, 200) == 159 and st.steps == 200
+    assert bench._align_to_sort(st, 200) == 0 and st.ran == [159]
+    assert bench._align_to_sort(FakeStepper(7), 0) == 0
 
 
 def test_grid_struct_layout_matches_oracle():
