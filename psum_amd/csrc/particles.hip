@@ -181,7 +181,18 @@ __global__ __launch_bounds__(kThreads) void k_sort_hist(const uint32_t* __restri
 #pragma unroll
     for (int j = 0; j < kSortWaveItems; ++j) {
         const int64_t p = base + j * 64 + lane;
-        if (p < n) atomicAdd(&hist[sort_digit(keys[p], nkeys, shift)], 1u);
+        if (base + j * 64 >= n) break;   // no active key in this row (uniform)
+        const bool act = p < n;
+        const uint32_t d = act ? sort_digit(keys[p], nkeys, shift) : 0u;
+        // a row whose keys share one digit (nearly sorted keys) counts them
+        // with one add; the counts are the same
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint32_t nact = (uint32_t)__popcll(__ballot(act));   // with every lane of the wave
+        if (__all(!act || d == d0)) {
+            if (lane == 0) atomicAdd(&hist[d0], nact);
+        } else if (act) {
+            atomicAdd(&hist[d], 1u);
+        }
     }
     __syncthreads();
     ghist[(int64_t)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
@@ -236,12 +247,62 @@ __global__ __launch_bounds__(kThreads) void k_sort_scatter(
     }
 }
 
+// 16-byte vector load / store of four consecutive values (two for double)
+template <typename T>
+__device__ __forceinline__ void load4(const T* __restrict__ p, T* v)
+{
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        const double2 b = *reinterpret_cast<const double2*>(p + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T* __restrict__ p, const T* v)
+{
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+    }
+}
+
 // -------------------------------------------------------------- permute
 
 struct PtrPack {
     const void* src[8];
     void* dst[8];
 };
+
+// four outputs per thread: perm read and the outputs written with 16-byte
+// vector accesses (perm and every output 16-byte aligned, n a multiple of 4
+// for all but the last thread, which goes one by one)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_permute4(PtrPack pk, int ncomp,
+                                                       const uint32_t* __restrict__ perm, int64_t n)
+{
+    const int64_t i0 = ((int64_t)blockIdx.x * kThreads + threadIdx.x) * 4;
+    if (i0 >= n) return;
+    if (i0 + 4 <= n) {
+        const uint4 s = *reinterpret_cast<const uint4*>(perm + i0);
+        for (int c = 0; c < ncomp; ++c) {
+            const T* src = static_cast<const T*>(pk.src[c]);
+            T v[4] = {src[s.x], src[s.y], src[s.z], src[s.w]};
+            store4(static_cast<T*>(pk.dst[c]) + i0, v);
+        }
+    } else {
+        for (int64_t i = i0; i < n; ++i) {
+            const uint32_t s = perm[i];
+            for (int c = 0; c < ncomp; ++c)
+                static_cast<T*>(pk.dst[c])[i] = static_cast<const T*>(pk.src[c])[s];
+        }
+    }
+}
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_permute(PtrPack pk, int ncomp,
@@ -323,30 +384,6 @@ constexpr int tile_nodes() { return D == 3 ? 2 * kTileNodes : kTileNodes; }
 template <int D>
 constexpr int kClip() { return D == 3 ? 16 : (D == 2 ? 45 : 2048); }
 static_assert(16 * 16 * 16 <= 2 * kTileNodes && 45 * 45 <= kTileNodes, "clipped box exceeds the tile");
-
-template <typename T>
-__device__ __forceinline__ void load4(const T* __restrict__ p, T* v)
-{
-    if constexpr (sizeof(T) == 4) {
-        const float4 q = *reinterpret_cast<const float4*>(p);
-        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-    } else {
-        const double2 a = *reinterpret_cast<const double2*>(p);
-        const double2 b = *reinterpret_cast<const double2*>(p + 2);
-        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void store4(T* __restrict__ p, const T* v)
-{
-    if constexpr (sizeof(T) == 4) {
-        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-    } else {
-        *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
-        *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
-    }
-}
 
 // Linear indices of the 2^D support nodes of cell cw, as support_index for
 // i = 0 .. 2^D-1: the cell's index plus, per dimension whose bit is set, one
@@ -1299,11 +1336,16 @@ psum_status psum_hip_permute(int dtype, int ncomp, void* const* src, void* const
         pk.src[c] = src[c];
         pk.dst[c] = dst[c];
     }
-    const unsigned nb = grid_blocks(n, kThreads);
-    if (dtype == PSUM_F64)
-        hipLaunchKernelGGL(k_permute<double>, dim3(nb), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
-    else if (dtype == PSUM_F32)
-        hipLaunchKernelGGL(k_permute<float>, dim3(nb), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
+    bool a16 = ((uintptr_t)perm & 15) == 0;
+    for (int c = 0; c < ncomp; ++c) a16 = a16 && ((uintptr_t)dst[c] & 15) == 0;
+    const unsigned nb = grid_blocks(n, kThreads), nb4 = grid_blocks((n + 3) / 4, kThreads);
+    if (dtype == PSUM_F64) {
+        if (a16) hipLaunchKernelGGL(k_permute4<double>, dim3(nb4), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
+        else hipLaunchKernelGGL(k_permute<double>, dim3(nb), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
+    } else if (dtype == PSUM_F32) {
+        if (a16) hipLaunchKernelGGL(k_permute4<float>, dim3(nb4), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
+        else hipLaunchKernelGGL(k_permute<float>, dim3(nb), dim3(kThreads), 0, S(stream), pk, ncomp, perm, n);
+    }
     else {
         set_error("psum_hip_permute: bad dtype");
         return PSUM_ERR_ARG;

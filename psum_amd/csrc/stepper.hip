@@ -162,15 +162,33 @@ __global__ void k_zero_f64(double* __restrict__ a, int n)
     if ((int)threadIdx.x < n) a[threadIdx.x] = 0.0;
 }
 
-__global__ void k_count_valid(const uint32_t* __restrict__ sorted_keys, int64_t n,
-                              unsigned long long* __restrict__ out)
+// First index whose key is PSUM_KEY_INVALID in sorted_keys[0, n) (n when
+// none; keys are sorted, invalid last), found by one wave: the answer a lies
+// in [lo, hi]; each round probes lo, lo + step, ... (step = ceil((hi - lo) /
+// 64), one probe per lane, below hi) and keeps the gap before the first
+// invalid probe, or the part past the last probe.
+__global__ __launch_bounds__(64) void k_count_valid(const uint32_t* __restrict__ sorted_keys, int64_t n,
+                                                    unsigned long long* __restrict__ out)
 {
-    // first index whose key is PSUM_KEY_INVALID (keys are sorted, invalid last)
-    const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-    if (i >= n) return;
-    const bool inv = sorted_keys[i] == PSUM_KEY_INVALID;
-    const bool prev_ok = i == 0 || sorted_keys[i - 1] != PSUM_KEY_INVALID;
-    if (inv && prev_ok) out[0] = (unsigned long long)i;
+    const int lane = threadIdx.x;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t step = (hi - lo + 63) / 64;
+        const int64_t p = lo + (int64_t)lane * step;
+        const bool probe = p < hi;
+        const bool inv = probe && sorted_keys[p] == PSUM_KEY_INVALID;
+        const unsigned long long m = __ballot(inv);
+        if (m) {
+            const int f = __ffsll((long long)m) - 1;
+            const int64_t pf = lo + (int64_t)f * step;
+            lo = f == 0 ? lo : pf - step + 1;
+            hi = pf;
+        } else {
+            const int last = 63 - __clzll((long long)__ballot(probe));
+            lo = lo + (int64_t)last * step + 1;
+        }
+    }
+    if (lane == 0) out[0] = (unsigned long long)lo;
 }
 
 int fx_shift(double w)
@@ -1360,7 +1378,7 @@ public:
             // drop particles whose key is invalid (absorbed): they sort last
             unsigned long long nv = (unsigned long long)sp.n;
             S_TRY(hipMemcpyAsync(nvalid_, &nv, sizeof nv, hipMemcpyHostToDevice, st_));
-            hipLaunchKernelGGL(k_count_valid, dim3(nblk(sp.n)), dim3(kT), 0, st_, skeys_, sp.n, nvalid_);
+            hipLaunchKernelGGL(k_count_valid, dim3(1), dim3(64), 0, st_, skeys_, sp.n, nvalid_);
             S_TRY(hipMemcpyAsync(&nv, nvalid_, sizeof nv, hipMemcpyDeviceToHost, st_));
             S_TRY(hipStreamSynchronize(st_));
             sp.n = (int64_t)nv;
