@@ -550,8 +550,9 @@ class Stepper:
 
     def tune(self, warmup=20, probe=10):
         """Autotune (psum_stepper_tune): dict with the calibrated fixed cycle
-        count and the chosen sort interval and bin shift (the shift is
-        applied)."""
+        count, the chosen bin shift and sort intervals ("sort_every": the
+        shortest; "species_sort_every": one per species).  The shift and the
+        per-species intervals are applied."""
         from ._lib import PsumBatchTuning
         t = PsumBatchTuning()
         check(lib().psum_stepper_tune(self.h, warmup, probe, C.byref(t)), "psum_stepper_tune")
