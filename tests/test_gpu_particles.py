@@ -116,6 +116,37 @@ def test_sort_small_key_space_and_permute():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 4097, 9001])
+def test_permute_vector_and_scalar_paths(dtype, n):
+    """psum_hip_permute takes four outputs per thread when perm and every
+    output are 16-byte aligned (tail of n % 4 one by one) and one per thread
+    otherwise (an output offset by one element): both equal a[perm]."""
+    import ctypes as C
+    E = _E()
+    from psum_amd import _lib
+    rng = np.random.default_rng(n + 11)
+    perm = rng.permutation(n).astype(np.uint32)
+    src = [rng.standard_normal(n).astype(dtype) for _ in range(3)]
+    want = [a[perm] for a in src]
+    got = E.permute([_dev(a) for a in src], _dev(perm.view(np.int32)))
+    for w, g in zip(want, got):
+        assert np.array_equal(w, _host(g))
+    # unaligned outputs: views one element into larger buffers
+    ds = [_dev(a) for a in src]
+    big = [torch.empty(n + 1, dtype=d.dtype, device="cuda") for d in ds]
+    dst = [b[1:] for b in big]
+    dp = _dev(perm.view(np.int32))
+    code = _lib.PSUM_F64 if dtype == np.float64 else _lib.PSUM_F32
+    arr = C.c_void_p * 3
+    rc = _lib.lib().psum_hip_permute(code, 3, arr(*[d.data_ptr() for d in ds]), arr(*[d.data_ptr() for d in dst]),
+                                     C.c_void_p(dp.data_ptr()), n, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    for w, d in zip(want, dst):
+        assert np.array_equal(w, _host(d))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("n", [1, 1023, 1024, 1025, 50_000])
 def test_compact_bit_exact(dtype, n):
     E = _E()
