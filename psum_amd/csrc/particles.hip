@@ -366,12 +366,12 @@ __global__ __launch_bounds__(kThreads) void k_compact_scatter(PtrPack pk, int nc
 // loads and stores.  With DEPOSIT the block deposits the (new) positions of
 // its particles into an LDS int64 tile spanning the bounding box of their
 // support nodes, then flushes the non-zero entries to the global int64
-// accumulator; a block whose box exceeds the tile uses global integer
-// atomics directly.  Consecutive particles of cell-sorted storage share a
-// cell, so each thread first sums the terms of equal-cell runs in registers
-// (exact integer adds) and issues one atomic per run and support node, in a
-// lane-rotated node order to spread same-address conflicts.  Integer sums
-// make every path exact and order independent.
+// accumulator; particles whose support leaves a box clipped to the tile use
+// global integer atomics directly.  In 2D and 3D each particle adds its
+// terms on its own; in 1D each thread first sums the terms of equal-cell
+// runs in registers (exact integer adds) and issues one atomic per run and
+// support node, in a lane-rotated node order to spread same-address
+// conflicts.  Integer sums make every path exact and order independent.
 
 constexpr int kPPT = 4;
 constexpr int kPushTile = kThreads * kPPT;
@@ -807,11 +807,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_m
 
         constexpr int NS = 1 << D;
         const double wsc = w_dep * scale;
-        if constexpr (D == 3) {
-            // 3D: particles sorted on cell bins rarely share a cell with their
-            // neighbour, so each particle adds its 8 terms directly, with the
-            // support offsets fixed at compile time (no run bookkeeping).
-            const int s1 = e[2], s0 = e[1] * e[2];
+        if constexpr (D >= 2) {
+            // 2D and 3D: each particle adds its 2^D terms directly, with the
+            // support offsets fixed at compile time.  In 3D particles sorted
+            // on cell bins rarely share a cell with their neighbour; in 2D
+            // they often do (100 per cell), but the run bookkeeping below
+            // cost more instructions than its fewer LDS atomics saved.
+            int st[3];
+            st[D - 1] = 1;
+#pragma unroll
+            for (int d = D - 2; d >= 0; --d) st[d] = st[d + 1] * e[d + 1];
 #pragma unroll
             for (int j = 0; j < kPPT; ++j) {
                 if (!ok[j]) continue;
@@ -835,13 +840,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_m
 #pragma unroll
                 for (int d = 0; d < D; ++d) inside = inside && uw[j][d] >= lo[d] && uw[j][d] + 1 < lo[d] + e[d];
                 if (inside) {
-                    const int li0 = ((uw[j][0] - lo[0]) * e[1] + (uw[j][1] - lo[1])) * e[2] + (uw[j][2] - lo[2]);
+                    int li0 = 0;
 #pragma unroll
+                    for (int d = 0; d < D; ++d) li0 += (uw[j][d] - lo[d]) * st[d];
                     // a zero term (a coefficient below the cut) adds nothing:
                     // no branch around it
-                    for (int i = 0; i < NS; ++i)
-                        atomicAdd(&tile[li0 + ((i >> 2) & 1) * s0 + ((i >> 1) & 1) * s1 + (i & 1)],
-                                  (unsigned long long)t8[i]);
+#pragma unroll
+                    for (int i = 0; i < NS; ++i) {
+                        int o = 0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) o += ((i >> (D - 1 - d)) & 1) * st[d];
+                        atomicAdd(&tile[li0 + o], (unsigned long long)t8[i]);
+                    }
                 } else if (ovf) {
                     // all eight adds are issued before their returned values
                     // are checked (a zero term leaves old unchanged: no wrap)
