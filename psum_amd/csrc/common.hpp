@@ -171,6 +171,17 @@ __device__ __forceinline__ long long readlane64(long long v, int lane)
 }
 }  // namespace dpp
 
+// the butterfly of wave_sum_full within each row of 16 lanes: every lane
+// holds its row's total (all 64 lanes active)
+__device__ __forceinline__ double row_sum_full(double v)
+{
+    v += __longlong_as_double(dpp::mov64<dpp::kXor1>(__double_as_longlong(v)));
+    v += __longlong_as_double(dpp::mov64<dpp::kXor2>(__double_as_longlong(v)));
+    v += __longlong_as_double(dpp::mov64<dpp::kHalfMirror>(__double_as_longlong(v)));
+    v += __longlong_as_double(dpp::mov64<dpp::kMirror>(__double_as_longlong(v)));
+    return v;
+}
+
 __device__ __forceinline__ double wave_sum_full(double v)
 {
     v += __longlong_as_double(dpp::mov64<dpp::kXor1>(__double_as_longlong(v)));

@@ -598,7 +598,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_m
     __shared__ int bmin[3], bmax[3], bref[3];
     __shared__ long long bsum[3];
     __shared__ int bcnt;
-    __shared__ double wdiag[kThreads / 64][4];
+    __shared__ double wdiag[kThreads / 16][4];   // per 16-lane row
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // wraps of the global int64 sums; an LDS tile entry holds at most
     // kPushTile terms below 2^50 (|w_dep| * scale <= 2^50) and cannot wrap
@@ -696,12 +696,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_m
         double vals[4] = {ke, pm[0], pm[1], pm[2]};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const double r = wave_sum_full(vals[q]);
-            if (lane == 0) wdiag[wave][q] = r;
+            // row totals (16 lanes each) go to LDS; thread q sums them in
+            // row order below
+            const double r = row_sum_full(vals[q]);
+            if ((lane & 15) == 0) wdiag[wave * 4 + (lane >> 4)][q] = r;
         }
     }
     if (counters) {
-        const int ab = wave_isum_full((int)absorbed), oo = wave_isum_full((int)oor);
+        // both counts (at most kPPT per thread, 256 per wave) in one sum
+        const int both = wave_isum_full((int)((absorbed << 16) | oor));
+        const int ab = both >> 16, oo = both & 0xFFFF;
         if (lane == 0) {
             if (ab) atomicAdd(&counters[2], (unsigned long long)ab);
             if (oo) atomicAdd(&counters[0], (unsigned long long)oo);
@@ -1020,7 +1024,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(push_m
         __syncthreads();
         if (threadIdx.x < 4) {
             double s = 0.0;
-            for (int w = 0; w < kThreads / 64; ++w) s += wdiag[w][threadIdx.x];
+            for (int w = 0; w < kThreads / 16; ++w) s += wdiag[w][threadIdx.x];
             diag[(int64_t)blockIdx.x * 4 + threadIdx.x] = s;
         }
     }
